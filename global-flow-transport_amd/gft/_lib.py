@@ -1,0 +1,100 @@
+"""Loader of libgft.so (the HIP kernels behind a C ABI, include/gft.h).
+
+The library is loaded after torch so that both share torch's HIP runtime
+(both link libamdhip64.so.7). There is no fallback: a missing or unloadable
+library raises, so a GPU run can never silently compute elsewhere.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime the library binds to)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgft.so")
+_LIB = None
+
+GFT_OK = 0
+GFT_ERR_INVALID_ARGUMENT = 1
+GFT_ERR_UNIMPLEMENTED = 2
+GFT_ERR_RUNTIME = 3
+GFT_ERR_WORKSPACE = 4
+
+
+class GftError(RuntimeError):
+    """HIP runtime or workspace failure reported by libgft."""
+
+
+class InvalidArgumentError(GftError, ValueError):
+    """Counterpart of TF's InvalidArgument raised by the reference ops."""
+
+
+class UnimplementedError(GftError, NotImplementedError):
+    """Counterpart of TF's Unimplemented raised by the reference ops."""
+
+
+_i32 = ctypes.c_int32
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "gft_last_error": (ctypes.c_char_p, []),
+    "gft_abi_version": (ctypes.c_int, []),
+    "gft_raymarch_grid_transform_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _i32,
+                                                       _i32, _i32, _i32, _i32, _vp]),
+    "gft_raymarch_grid_transform_bwd_workspace": (_sz, [_i64p, _i32]),
+    "gft_raymarch_grid_transform_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _i32p,
+                                                       _i32, _i32, _i32, _i32, _i32, _i32, _vp, _sz, _vp]),
+}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgft.so is not built (%s); run __graft_entry__.build() or "
+                               "python global-flow-transport_amd/build.py" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                raise RuntimeError("libgft.so lacks symbol %s" % name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGNATURES.keys())
+
+
+def check(rc):
+    if rc == GFT_OK:
+        return
+    msg = lib().gft_last_error().decode("utf-8", "replace")
+    if rc == GFT_ERR_INVALID_ARGUMENT:
+        raise InvalidArgumentError(msg)
+    if rc == GFT_ERR_UNIMPLEMENTED:
+        raise UnimplementedError(msg)
+    raise GftError("libgft error %d: %s" % (rc, msg))
+
+
+def i64_array(values):
+    vals = [int(v) for v in values]
+    return (ctypes.c_int64 * len(vals))(*vals)
+
+
+def i32_array(values):
+    vals = [int(v) for v in values]
+    return (ctypes.c_int32 * len(vals))(*vals)
+
+
+def stream_of(tensor):
+    return ctypes.c_void_p(torch.cuda.current_stream(tensor.device).cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)

@@ -1,0 +1,91 @@
+/*
+ * gft: C ABI of the MI355X (gfx950) rendering/transport kernels.
+ *
+ * Every entry point takes device pointers for grids and images, host pointers
+ * for matrices, frustum parameters and shapes (the reference passes these as
+ * HostMemory op inputs), a hipStream_t passed as void*, and returns a
+ * gft_status. On failure gft_last_error() returns a thread-local message.
+ * Nothing calls exit(); there is no global device state; calls are
+ * stream-ordered and re-entrant. Scratch memory is supplied by the caller
+ * through (workspace, workspace_bytes) sized by the matching *_workspace()
+ * query.
+ *
+ * Layouts follow the reference: grids are NDHWC float32 with C in {1,2,4};
+ * matrices are 4x4 float32 in column-major storage (the reference's
+ * numpy ``.transpose()`` of row-major matrices, renderer.py:108-112);
+ * frustum parameters are [near, far, left, right, top, bottom]
+ * (camera.py:61-62).
+ */
+#ifndef GFT_H
+#define GFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GFT_OK = 0,
+  GFT_ERR_INVALID_ARGUMENT = 1, /* TF InvalidArgument in the reference */
+  GFT_ERR_UNIMPLEMENTED = 2,    /* TF Unimplemented */
+  GFT_ERR_RUNTIME = 3,          /* HIP runtime failure (reference: exit(10)) */
+  GFT_ERR_WORKSPACE = 4         /* workspace smaller than the query returned */
+} gft_status;
+
+/* op attribute enums, same spelling as the reference's string attrs */
+enum { GFT_NEAREST = 0, GFT_LINEAR = 1, GFT_MIN = 2, GFT_MAX = 3 };             /* interpolation */
+enum { GFT_BORDER = 0, GFT_CLAMP = 1, GFT_WRAP = 2 };                          /* boundary */
+enum { GFT_BEER_LAMBERT = 0, GFT_ALPHA = 1, GFT_ALPHA_ADDITIVE = 2, GFT_ADDITIVE = 3 }; /* blending_mode */
+enum { GFT_MIP_NONE = 0, GFT_MIP_NEAREST = 1, GFT_MIP_LINEAR = 2 };             /* mipmapping */
+enum {
+  GFT_TRANSFORM_LINDEPTH = 0,
+  GFT_TRANSFORM_LINDEPTH_REVERSE = 1,
+  GFT_TRANSFORM = 2,
+  GFT_TRANSFORM_REVERSE = 3
+}; /* coordinate_mode */
+enum { GFT_GRAD_REFERENCE = 0, GFT_GRAD_EXACT = 1 }; /* gradient flavour */
+
+const char* gft_last_error(void);
+int gft_abi_version(void);
+
+/*
+ * RaymarchGridTransform forward.
+ * Replaces RaymarchGridTransformOp::Compute (phitest/render/cuda/src/
+ * raymarch_grid.cc:192-297) and RaymarchGridKernel<GPUDevice,T,C>
+ * (raymarch_grid.hpp:12-20, raymarch_grid.cu.cc:222-312).
+ *   input        [N,D,H,W,C] device
+ *   M            [num_M,4,4] host, num_M == N
+ *   V, P         [num_cameras,4,4] host; frustum [num_cameras,6] host
+ *   output_shape DHW (D = samples per ray)
+ *   output       keep_dims: [N,V',1,H,W,C]; else [1,V',H,W,C] (N must be 1)
+ *                with V' = num_cameras if separate_camera_batch else 1.
+ */
+int gft_raymarch_grid_transform_fwd(void* stream, const float* input, const int64_t input_shape[5], const float* M,
+                                    int32_t num_M, const float* V, const float* P, const float* frustum,
+                                    int32_t num_cameras, const int32_t output_shape[3], int32_t interpolation,
+                                    int32_t boundary, int32_t blending_mode, int32_t keep_dims,
+                                    int32_t separate_camera_batch, float* output);
+
+/*
+ * RaymarchGridTransformGrad.
+ * Replaces RaymarchGridTransformGradOp::Compute (raymarch_grid.cc:347-481) and
+ * RaymarchGridGradKernel (raymarch_grid.hpp:45-53, raymarch_grid.cu.cc:451-594).
+ * interpolation must be LINEAR (the reference's grad op attr, :66).
+ * grad_mode REFERENCE reproduces the per-camera weight normalisation;
+ * EXACT is the true adjoint. input_grad is fully overwritten.
+ */
+size_t gft_raymarch_grid_transform_bwd_workspace(const int64_t input_shape[5], int32_t grad_mode);
+int gft_raymarch_grid_transform_bwd(void* stream, const float* input, const int64_t input_shape[5],
+                                    const float* output, const float* output_grad, const float* M, int32_t num_M,
+                                    const float* V, const float* P, const float* frustum, int32_t num_cameras,
+                                    const int32_t output_shape[3], int32_t interpolation, int32_t boundary,
+                                    int32_t blending_mode, int32_t keep_dims, int32_t separate_camera_batch,
+                                    int32_t grad_mode, void* workspace, size_t workspace_bytes, float* input_grad);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GFT_H */

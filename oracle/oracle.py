@@ -1,0 +1,348 @@
+"""Python face of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker or as the timed CPU baseline; the product path
+never does. Wraps liboracle.so (oracle.cpp, built by oracle/Makefile) with
+numpy arrays and restates, in numpy, the Python-level compositions of the
+reference:
+
+* MacCormack advection with the MC_SMOOTH revert clamp
+  (phitest/render/data_structures.py:870-905) and its adjoint, both the
+  reference's (TF autodiff through count-normalised SampleGridLut gradients,
+  cuda/ops_loader.py:35-51) and the exact one;
+* the density render of Renderer.render_density with an ambient light
+  (renderer.py:603-651, lighting.py:23-26): light = c*i*d, data scaled by the
+  camera depth step (renderer.py:420-437), fused raymarch, image = rgb;
+* the scalar L2 image loss and the chain rule back to density and velocity.
+
+``prec`` is 1 for FP64 (the oracle proper) and 0 for FP32 (the reference's
+arithmetic).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FILTER = {"NEAREST": 0, "LINEAR": 1, "MIN": 2, "MAX": 3}
+BOUNDARY = {"BORDER": 0, "CLAMP": 1, "WRAP": 2}
+BLEND = {"BEER_LAMBERT": 0, "ALPHA": 1, "ALPHA_ADDITIVE": 2, "ADDITIVE": 3}
+MIP = {"NONE": 0, "NEAREST": 1, "LINEAR": 2}
+COORD = {"TRANSFORM_LINDEPTH": 0, "TRANSFORM_LINDEPTH_REVERSE": 1, "TRANSFORM": 2, "TRANSFORM_REVERSE": 3}
+GRAD = {"REFERENCE": 0, "EXACT": 1}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle library missing: run `make -C oracle`")
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _p(a):
+    # the numpy ctypes object keeps the array alive for the duration of the call
+    return a.ctypes
+
+
+def _mats(M, V, P, F):
+    M = _f(M).reshape(-1, 16)
+    V = _f(V).reshape(-1, 16)
+    P = _f(P).reshape(-1, 16)
+    F = _f(F).reshape(-1, 6)
+    return M, V, P, F
+
+
+def _check(rc, name):
+    if rc != 0:
+        raise RuntimeError("oracle %s failed (%d)" % (name, rc))
+
+
+# ---------------------------------------------------------------- raymarch
+def raymarch_fwd(data, M, V, P, F, out_dhw, interpolation="LINEAR", boundary="BORDER", blending_mode="BEER_LAMBERT",
+                 separate_camera_batch=True, prec=1):
+    """raymarch_grid_transform (keep_dims=False): returns [N', V', H, W, C]."""
+    data = _d(data)
+    N, C = data.shape[0], data.shape[4]
+    M, V, P, F = _mats(M, V, P, F)
+    nCam = V.shape[0]
+    out_dhw = _i64(out_dhw)
+    nv = nCam if separate_camera_batch else 1
+    out = np.zeros((N, nv, int(out_dhw[1]), int(out_dhw[2]), C), dtype=np.float64)
+    _check(lib().orc_raymarch_fwd(ctypes.c_int(prec), _p(data), _p(_i64(data.shape)), _p(M), _p(V), _p(P), _p(F),
+                                  ctypes.c_int(nCam), _p(out_dhw), ctypes.c_int(FILTER[interpolation]),
+                                  ctypes.c_int(BOUNDARY[boundary]), ctypes.c_int(BLEND[blending_mode]),
+                                  ctypes.c_int(int(separate_camera_batch)), _p(out)), "raymarch_fwd")
+    return out
+
+
+def raymarch_bwd(data, output, output_grad, M, V, P, F, out_dhw, boundary="BORDER", blending_mode="BEER_LAMBERT",
+                 separate_camera_batch=True, grad_mode="EXACT", prec=1):
+    data = _d(data)
+    M, V, P, F = _mats(M, V, P, F)
+    grad = np.zeros_like(data)
+    _check(lib().orc_raymarch_bwd(ctypes.c_int(prec), ctypes.c_int(GRAD[grad_mode]), _p(data), _p(_i64(data.shape)),
+                                  _p(_d(output)), _p(_d(output_grad)), _p(M), _p(V), _p(P), _p(F),
+                                  ctypes.c_int(V.shape[0]), _p(_i64(out_dhw)), ctypes.c_int(BOUNDARY[boundary]),
+                                  ctypes.c_int(BLEND[blending_mode]), ctypes.c_int(int(separate_camera_batch)),
+                                  _p(grad)), "raymarch_bwd")
+    return grad
+
+
+# ------------------------------------------------------------ reduce blend
+def reduce_blend_fwd(data, blending_mode="BEER_LAMBERT", keep_dims=False, prec=1):
+    data = _d(data)
+    N, D, H, W, C = data.shape
+    out = np.zeros(data.shape if keep_dims else (N, H, W, C), dtype=np.float64)
+    _check(lib().orc_reduce_blend_fwd(ctypes.c_int(prec), _p(data), _p(_i64(data.shape)),
+                                      ctypes.c_int(BLEND[blending_mode]), ctypes.c_int(int(keep_dims)), _p(out)),
+           "reduce_blend_fwd")
+    return out
+
+
+def reduce_blend_bwd(data, output, output_grad, blending_mode="BEER_LAMBERT", keep_dims=False, grad_mode="REFERENCE",
+                     prec=1):
+    data = _d(data)
+    grad = np.zeros_like(data)
+    _check(lib().orc_reduce_blend_bwd(ctypes.c_int(prec), ctypes.c_int(GRAD[grad_mode]), _p(data),
+                                      _p(_i64(data.shape)), _p(_d(output)), _p(_d(output_grad)),
+                                      ctypes.c_int(BLEND[blending_mode]), ctypes.c_int(int(keep_dims)), _p(grad)),
+           "reduce_blend_bwd")
+    return grad
+
+
+# -------------------------------------------------------- sample transform
+def sample_transform_fwd(data, M, V, P, F, out_dhw, interpolation="LINEAR", boundary="BORDER", mipmapping="NONE",
+                         num_mipmaps=0, mip_bias=0.0, coordinate_mode="TRANSFORM_LINDEPTH",
+                         separate_camera_batch=True, prec=1):
+    data = _d(data)
+    N, C = data.shape[0], data.shape[4]
+    M, V, P, F = _mats(M, V, P, F)
+    nCam = V.shape[0]
+    nv = nCam if separate_camera_batch else 1
+    out_dhw = _i64(out_dhw)
+    out = np.zeros((N, nv) + tuple(int(s) for s in out_dhw) + (C,), dtype=np.float64)
+    _check(lib().orc_sample_transform_fwd(
+        ctypes.c_int(prec), _p(data), _p(_i64(data.shape)), _p(M), _p(V), _p(P), _p(F), ctypes.c_int(nCam),
+        _p(out_dhw), ctypes.c_int(FILTER[interpolation]), ctypes.c_int(BOUNDARY[boundary]),
+        ctypes.c_int(MIP[mipmapping]), ctypes.c_int(num_mipmaps), ctypes.c_double(mip_bias),
+        ctypes.c_int(COORD[coordinate_mode]), ctypes.c_int(int(separate_camera_batch)), _p(out)),
+        "sample_transform_fwd")
+    return out
+
+
+def sample_transform_bwd(data, output_grad, M, V, P, F, interpolation="LINEAR", boundary="BORDER",
+                         mipmapping="NONE", num_mipmaps=0, mip_bias=0.0, coordinate_mode="TRANSFORM_LINDEPTH",
+                         separate_camera_batch=True, grad_mode="REFERENCE", prec=1):
+    data = _d(data)
+    og = _d(output_grad)
+    M, V, P, F = _mats(M, V, P, F)
+    grad = np.zeros_like(data)
+    out_dhw = _i64(og.shape[2:5])
+    _check(lib().orc_sample_transform_bwd(
+        ctypes.c_int(prec), ctypes.c_int(GRAD[grad_mode]), _p(data), _p(_i64(data.shape)), _p(og), _p(M), _p(V),
+        _p(P), _p(F), ctypes.c_int(V.shape[0]), _p(out_dhw), ctypes.c_int(FILTER[interpolation]),
+        ctypes.c_int(BOUNDARY[boundary]), ctypes.c_int(MIP[mipmapping]), ctypes.c_int(num_mipmaps),
+        ctypes.c_double(mip_bias), ctypes.c_int(COORD[coordinate_mode]), ctypes.c_int(int(separate_camera_batch)),
+        _p(grad)), "sample_transform_bwd")
+    return grad
+
+
+# --------------------------------------------------------------------- LuT
+def sample_lut_fwd(data, lut, interpolation="LINEAR", boundary="BORDER", mipmapping="NONE", num_mipmaps=0,
+                   mip_bias=0.0, cell_center_offset=0.0, separate_camera_batch=True, relative_coords=True,
+                   normalized_coords=False, prec=1):
+    data = _d(data)
+    lut = _d(lut)
+    N, C = data.shape[0], data.shape[4]
+    nv = lut.shape[0] if separate_camera_batch else 1
+    out = np.zeros((N, nv) + lut.shape[1:4] + (C,), dtype=np.float64)
+    _check(lib().orc_sample_lut_fwd(
+        ctypes.c_int(prec), _p(data), _p(_i64(data.shape)), _p(lut), _p(_i64(lut.shape)),
+        ctypes.c_int(FILTER[interpolation]), ctypes.c_int(BOUNDARY[boundary]), ctypes.c_int(MIP[mipmapping]),
+        ctypes.c_int(num_mipmaps), ctypes.c_double(mip_bias), ctypes.c_double(cell_center_offset),
+        ctypes.c_int(int(separate_camera_batch)), ctypes.c_int(int(relative_coords)),
+        ctypes.c_int(int(normalized_coords)), _p(out)), "sample_lut_fwd")
+    return out
+
+
+def sample_lut_bwd(data, lut, output_grad, interpolation="LINEAR", boundary="BORDER", cell_center_offset=0.0,
+                   separate_camera_batch=True, relative_coords=True, normalized_coords=False, grad_mode="REFERENCE",
+                   prec=1):
+    data = _d(data)
+    lut = _d(lut)
+    gin = np.zeros_like(data)
+    glut = np.zeros_like(lut)
+    _check(lib().orc_sample_lut_bwd(
+        ctypes.c_int(prec), ctypes.c_int(GRAD[grad_mode]), _p(data), _p(_i64(data.shape)), _p(lut),
+        _p(_i64(lut.shape)), _p(_d(output_grad)), ctypes.c_int(FILTER[interpolation]),
+        ctypes.c_int(BOUNDARY[boundary]), ctypes.c_double(cell_center_offset),
+        ctypes.c_int(int(separate_camera_batch)), ctypes.c_int(int(relative_coords)),
+        ctypes.c_int(int(normalized_coords)), _p(gin), _p(glut)), "sample_lut_bwd")
+    return gin, glut
+
+
+def lod_transform(in_dhw, MV, P, F, out_dhw, inverse_transform=False, prec=1):
+    MV = _f(MV).reshape(-1, 16)
+    P = _f(P).reshape(-1, 16)
+    F = _f(F).reshape(-1, 6)
+    out = np.zeros((MV.shape[0],) + tuple(int(s) for s in out_dhw) + (4,), dtype=np.float64)
+    _check(lib().orc_lod_transform(ctypes.c_int(prec), _p(_i64(in_dhw)), _p(MV), _p(P), _p(F),
+                                   ctypes.c_int(MV.shape[0]), _p(_i64(out_dhw)), ctypes.c_int(int(inverse_transform)),
+                                   _p(out)), "lod_transform")
+    return out
+
+
+def build_mips(data_dhwc, levels, prec=1):
+    data = _d(data_dhwc)
+    D, H, W, C = data.shape
+    dims = []
+    total = 0
+    for m in range(1, levels + 1):
+        s = (D >> m, H >> m, W >> m)
+        dims.append(s)
+        total += s[0] * s[1] * s[2] * C
+    buf = np.zeros(total, dtype=np.float64)
+    _check(lib().orc_build_mips(ctypes.c_int(prec), _p(data), _p(_i64(data.shape)), ctypes.c_int(levels), _p(buf)),
+           "build_mips")
+    out, o = [], 0
+    for s in dims:
+        n = s[0] * s[1] * s[2] * C
+        out.append(buf[o:o + n].reshape(s + (C,)))
+        o += n
+    return out
+
+
+# ------------------------------------------------------- advection (A10)
+def _lut_from_velocity(vel_c, dt=1.0):
+    """centered_lut_grid (data_structures.py:864-868) padded with a zero LoD
+    channel (renderer.py:448-454): [1,D,H,W,4]."""
+    v = _d(vel_c)
+    lut = np.concatenate([-v * dt, np.zeros(v.shape[:-1] + (1,))], axis=-1)
+    return lut
+
+
+def advect_fwd(d, vel_c, order=2, clamp="MC_SMOOTH", dt=1.0, boundary="CLAMP", prec=1, return_parts=False):
+    """VelocityGrid.warp (data_structures.py:870-905) on a [1,D,H,W,C] grid."""
+    lut = _lut_from_velocity(vel_c, dt)
+    kw = dict(boundary=boundary, separate_camera_batch=False, relative_coords=True, prec=prec)
+    f = sample_lut_fwd(d, lut, **kw)[:, 0]
+    parts = {"f": f}
+    out = f
+    if order == 2:
+        nlut = lut.copy()
+        nlut[..., :3] *= -1
+        b = sample_lut_fwd(f, nlut, **kw)[:, 0]
+        corr = f + 0.5 * (_d(d) - b)
+        parts.update(b=b, corr=corr)
+        if clamp in ("MC", "MC_SMOOTH"):
+            mn = sample_lut_fwd(d, lut, interpolation="MIN", **kw)[:, 0]
+            mx = sample_lut_fwd(d, lut, interpolation="MAX", **kw)[:, 0]
+            mask = np.logical_or(corr < mn, corr > mx)
+            parts.update(mn=mn, mx=mx, mask=mask)
+            if clamp == "MC":
+                raise NotImplementedError("MC clamp raises in the reference (data_structures.py:890-893)")
+            corr = np.where(mask, f, corr)
+        out = corr
+    if clamp == "NEGATIVE":
+        out = np.maximum(out, 0)
+    return (out, parts) if return_parts else out
+
+
+def advect_bwd(d, vel_c, out_grad, order=2, clamp="MC_SMOOTH", dt=1.0, boundary="CLAMP", grad_mode="EXACT",
+               prec=1):
+    """Gradients of advect_fwd w.r.t. the density and the centred velocity.
+
+    REFERENCE follows TF autodiff through the composition with the
+    reference's SampleGridLut gradient op; EXACT uses the exact LuT adjoint.
+    MIN/MAX samples and the clamp mask carry no gradient (tf.less/greater)."""
+    lut = _lut_from_velocity(vel_c, dt)
+    kw = dict(boundary=boundary, separate_camera_batch=False, relative_coords=True, grad_mode=grad_mode, prec=prec)
+    out, parts = advect_fwd(d, vel_c, order, clamp, dt, boundary, prec, return_parts=True)
+    g = _d(out_grad)
+    if clamp == "NEGATIVE":
+        g = g * (out > 0)
+    gd = np.zeros_like(_d(d))
+    glut = np.zeros_like(lut)
+    gf = np.zeros_like(g)
+    if order == 2:
+        if "mask" in parts:
+            gf += np.where(parts["mask"], g, 0.0)
+            gc = np.where(parts["mask"], 0.0, g)
+        else:
+            gc = g
+        gf += gc
+        gd += 0.5 * gc
+        gb = -0.5 * gc
+        nlut = lut.copy()
+        nlut[..., :3] *= -1
+        gin, gnlut = sample_lut_bwd(parts["f"], nlut, gb[:, None], **kw)
+        gf += gin
+        glut[..., :3] -= gnlut[..., :3]
+    else:
+        gf = g
+    gin, gl = sample_lut_bwd(d, lut, gf[:, None], **kw)
+    gd += gin
+    glut[..., :3] += gl[..., :3]
+    gvel = -glut[..., :3] * dt
+    return gd, gvel
+
+
+# -------------------------------------------------- density render (A11)
+def render_density_fwd(d, M, V, P, F, out_dhw, depth_step, light_color=(1.0, 1.0, 1.0), light_intensity=1.0,
+                       boundary="BORDER", prec=1):
+    """Ambient-light density render: grid = [c*i*d, d] * depth_step, fused
+    BEER_LAMBERT raymarch. Returns [1,V,H,W,4] (rgb, optical depth)."""
+    d = _d(d)
+    light = d * (np.asarray(light_color, dtype=np.float64) * light_intensity)
+    grid = np.concatenate([light, d], axis=-1) * depth_step
+    return raymarch_fwd(grid, M, V, P, F, out_dhw, "LINEAR", boundary, "BEER_LAMBERT", True, prec)
+
+
+def render_density_bwd(d, out, out_grad, M, V, P, F, out_dhw, depth_step, light_color=(1.0, 1.0, 1.0),
+                       light_intensity=1.0, boundary="BORDER", grad_mode="EXACT", prec=1):
+    d = _d(d)
+    cvec = np.asarray(light_color, dtype=np.float64) * light_intensity
+    grid = np.concatenate([d * cvec, d], axis=-1) * depth_step
+    g4 = raymarch_bwd(grid, out, out_grad, M, V, P, F, out_dhw, boundary, "BEER_LAMBERT", True, grad_mode, prec)
+    return depth_step * (g4[..., :3] @ cvec + g4[..., 3])[..., None]
+
+
+def l2_loss_and_grad(img, target):
+    """loss = mean((rgb - target)^2); returns (loss, d loss / d img) with the
+    optical-depth channel receiving zero gradient."""
+    rgb = img[..., :3]
+    diff = rgb - target
+    loss = float(np.mean(diff * diff))
+    g = np.zeros_like(img)
+    g[..., :3] = 2.0 * diff / diff.size
+    return loss, g
+
+
+def step_fwd_bwd(d0, vel_c, target, M, V, P, F, out_dhw, depth_step, order=2, clamp="MC_SMOOTH", prec=1,
+                 grad_mode="EXACT", boundary_render="BORDER"):
+    """One advect -> render -> L2 loss step and its gradients w.r.t. d0 and
+    the centred velocity. Returns dict(loss, img, d1, gd, gv)."""
+    d1 = advect_fwd(d0, vel_c, order, clamp, 1.0, "CLAMP", prec)
+    img = render_density_fwd(d1, M, V, P, F, out_dhw, depth_step, boundary=boundary_render, prec=prec)
+    loss, gimg = l2_loss_and_grad(img, target)
+    gd1 = render_density_bwd(d1, img, gimg, M, V, P, F, out_dhw, depth_step, boundary=boundary_render,
+                             grad_mode=grad_mode, prec=prec)
+    gd, gv = advect_bwd(d0, vel_c, gd1, order, clamp, 1.0, "CLAMP", grad_mode, prec)
+    return dict(loss=loss, img=img, d1=d1, gd=gd, gv=gv, gd1=gd1)
