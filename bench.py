@@ -1,0 +1,164 @@
+#!/usr/bin/env python
+"""Benchmark of the fused transport/render step on one MI355X.
+
+One step = MacCormack advection (MC_SMOOTH) of the density by a curl-noise
+velocity, ambient-light emission-absorption render of the advected density
+into every view, L2 loss against target images, and the exact backward to the
+density and the velocity. Workload c2: grid 128x224x128 (x,y,z), 48 Gaussian
+blobs (seed 0), curl-noise velocity max 1.5 cells/step (seed 1), 5 views of
+256x456 px on a -60..60 deg arc, 182 samples per ray; targets are renders of a
+second blob density (seed 2).
+
+Prints one JSON line: voxel-samples/s (one 8-corner trilinear evaluation = 1
+voxel-sample; N_vs = 6V + 2 sum W*H*S per step), the HBM roofline of the
+dominant kernel from HIP events in the timed region, and a CPU baseline (the
+FP32 oracle, one full step on the host cores).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "global-flow-transport_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "voxel-samples/s fwd+bwd (advect+resample+raymarch), HBM GB/s vs plain FP32"
+
+
+def build_inputs(name):
+    import numpy as np
+
+    from gft import synthetic as syn
+    from gft.camera import camera_params_batch, model_matrices
+    w = syn.WORKLOADS[name]
+    grid, image = w["grid"], w["image"]
+    d0 = syn.gaussian_blobs(grid, w["blobs"], seed=0)
+    vel = syn.curl_noise_velocity(grid, seed=1, max_speed=w["max_speed"])
+    dt = syn.gaussian_blobs(grid, w["blobs"], seed=2)
+    cams, S = syn.arc_cameras(grid, image, w["views"])
+    M = model_matrices([syn.grid_transform(grid)])
+    V, P, F = camera_params_batch(cams)
+    return dict(grid=grid, image=image, S=S, views=w["views"], d0=d0, vel=vel, d_target=dt, M=M, V=V, P=P, F=F,
+                step=float(cams[0].depth_step), out_dhw=(S,) + tuple(image))
+
+
+def cpu_baseline(inp, target_np):
+    """The FP32 oracle (C++/OpenMP restatement of the reference ops) timed on
+    one full fwd+bwd step of the same workload."""
+    from oracle import oracle as orc
+    threads = int(orc.lib().orc_num_threads())
+    t0 = time.perf_counter()
+    orc.step_fwd_bwd(inp["d0"], inp["vel"], target_np, inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"],
+                     inp["step"], prec=0)
+    return time.perf_counter() - t0, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from gft.pipeline import TransportRenderStep, render_density_forward, _Cameras
+
+    inp = build_inputs(args.workload)
+    dev = torch.device("cuda:0")
+    d0 = torch.from_numpy(inp["d0"]).to(dev)
+    vel = torch.from_numpy(inp["vel"]).to(dev)
+    step = TransportRenderStep(inp["grid"], inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"])
+    # targets: render of the second density through the same kernels
+    cams = _Cameras(inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"])
+    tgt_img = render_density_forward(torch.from_numpy(inp["d_target"]).to(dev), cams, inp["step"])
+    target = tgt_img[..., :3].contiguous()
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        step.run(d0, vel, target)
+    torch.cuda.synchronize()
+
+    timers = {}
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        step.run(d0, vel, target, timers)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = start.elapsed_time(stop)
+    ms_per_step = ms_total / args.steps
+
+    phase_ms = {}
+    for name, evs in timers.items():
+        pairs = list(zip(evs[0::2], evs[1::2]))
+        phase_ms[name] = sum(a.elapsed_time(b) for a, b in pairs) / len(pairs)
+    alg = step.algorithmic_bytes()
+    dominant = max(phase_ms, key=phase_ms.get)
+    achieved = alg[dominant] / (phase_ms[dominant] * 1e-3) / 1e9
+    nvs = step.voxel_samples()
+    value = nvs / (ms_per_step * 1e-3)
+    b_alg = sum(alg.values())
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian blobs, curl-noise velocity, arc cameras)",
+        "config": {
+            "workload": "%s: fused fwd+bwd advect(MacCormack+MC_SMOOTH)->raymarch->L2, grid %dx%dx%d (x,y,z), "
+                        "%d views %dx%d px, %d samples/ray" % (args.workload, inp["grid"][2], inp["grid"][1],
+                                                               inp["grid"][0], inp["views"], inp["image"][1],
+                                                               inp["image"][0], inp["S"]),
+            "voxel_samples_per_step": nvs,
+            "grad_mode": "EXACT",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "algorithmic_bytes": alg[dominant],
+            "avg_ms": phase_ms[dominant],
+        },
+        "phases_ms": phase_ms,
+        "step_algorithmic_bytes": b_alg,
+        "step_hbm_alg_GBps": b_alg / (ms_per_step * 1e-3) / 1e9,
+        "loss": float(step.loss.item()),
+        "wall_s_timed": wall,
+    }
+    if not args.no_cpu_baseline:
+        tgt_np = target.cpu().numpy().astype(np.float64)
+        secs, threads = cpu_baseline(inp, tgt_np)
+        result["cpu_baseline"] = {"value": nvs / secs, "unit": "voxel-samples/s", "cores": threads, "kind": "port",
+                                  "sample": "one full fwd+bwd step of the same workload with the FP32 oracle "
+                                            "(%.1f s)" % secs}
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,271 @@
+// Fused semi-Lagrangian / MacCormack advection of a scalar grid with the
+// MC_SMOOTH revert clamp, and its exact adjoint w.r.t. the scalar and the
+// centred velocity.
+//
+// Restates VelocityGrid.warp (phitest/render/data_structures.py:870-905) over
+// SampleGridLut in relative, CLAMP, cell-centre-offset-0 mode
+// (resample_grid.cu.cc:465-484, sampling.hpp:158-491):
+//   p(x)   = x - v(x)*dt                       (lut = -v*dt, relative)
+//   f(x)   = trilinear(d, p(x))                (forward warp)
+//   b(x)   = trilinear(f, x + v(x)*dt)         (back warp with -lut)
+//   c(x)   = f(x) + 0.5*(d(x) - b(x))
+//   mn,mx  = min/max of the 8 corners of d around p(x)
+//   out(x) = (c < mn || c > mx) ? f(x) : c     (MC_SMOOTH)
+// The composition is two launches: the first writes f, the second computes
+// b, the corner extrema and the selection, and stores the clamp mask for the
+// backward. The backward is two scatter launches (float atomics):
+//   A: gb = mask ? 0 : -g/2 is scattered into gf at x+v*dt; gd(x) = mask?0:g/2;
+//      gv(x) = gb * dt * grad f(x+v*dt)
+//   B: gf = g + scattered gb; gf is scattered into gd at x-v*dt;
+//      gv(x) -= gf * dt * grad d(x-v*dt)
+// Trilinear derivatives use the floor/floor+1 corners (right derivative at
+// integer positions) with CLAMP indices, so the derivative vanishes where the
+// clamped read is constant.
+#include "gft_device.hpp"
+
+namespace gft {
+
+
+enum { CLAMP_NONE = 0, CLAMP_MC_SMOOTH = 1, CLAMP_NEGATIVE = 2 };
+
+__device__ __forceinline__ float3 load_vel(const float* __restrict__ vel, size_t i) {
+  return make_float3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+}
+
+// Clamped trilinear read of a scalar grid plus its 8 corner values.
+struct Tri {
+  float v[2][2][2];  // [z][y][x], corners floor / floor+1 (clamped)
+  float wx, wy, wz;
+  int ix[2], iy[2], iz[2];
+};
+
+__device__ __forceinline__ Tri tri_clamp(const float* __restrict__ buf, Dims3 d, float px, float py, float pz) {
+  Tri t;
+  float fxf = floorf(px), fyf = floorf(py), fzf = floorf(pz);
+  t.wx = px - fxf; t.wy = py - fyf; t.wz = pz - fzf;
+  int fx = (int)fxf, fy = (int)fyf, fz = (int)fzf;
+  t.ix[0] = clampi(fx, 0, d.x - 1); t.ix[1] = clampi(fx + 1, 0, d.x - 1);
+  t.iy[0] = clampi(fy, 0, d.y - 1); t.iy[1] = clampi(fy + 1, 0, d.y - 1);
+  t.iz[0] = clampi(fz, 0, d.z - 1); t.iz[1] = clampi(fz + 1, 0, d.z - 1);
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) t.v[k][j][i] = buf[flat3(t.ix[i], t.iy[j], t.iz[k], d)];
+  return t;
+}
+
+__device__ __forceinline__ float tri_value(const Tri& t) {
+  float v00 = lerpf(t.v[0][0][0], t.v[0][0][1], t.wx);
+  float v01 = lerpf(t.v[0][1][0], t.v[0][1][1], t.wx);
+  float v0 = lerpf(v00, v01, t.wy);
+  float v10 = lerpf(t.v[1][0][0], t.v[1][0][1], t.wx);
+  float v11 = lerpf(t.v[1][1][0], t.v[1][1][1], t.wx);
+  float v1 = lerpf(v10, v11, t.wy);
+  return lerpf(v0, v1, t.wz);
+}
+
+__device__ __forceinline__ float3 tri_grad(const Tri& t) {
+  const float (*v)[2][2] = t.v;
+  float gx = lerpf(lerpf(v[0][0][1] - v[0][0][0], v[0][1][1] - v[0][1][0], t.wy),
+                   lerpf(v[1][0][1] - v[1][0][0], v[1][1][1] - v[1][1][0], t.wy), t.wz);
+  float gy = lerpf(lerpf(v[0][1][0] - v[0][0][0], v[0][1][1] - v[0][0][1], t.wx),
+                   lerpf(v[1][1][0] - v[1][0][0], v[1][1][1] - v[1][0][1], t.wx), t.wz);
+  float gz = lerpf(lerpf(v[1][0][0] - v[0][0][0], v[1][0][1] - v[0][0][1], t.wx),
+                   lerpf(v[1][1][0] - v[0][1][0], v[1][1][1] - v[0][1][1], t.wx), t.wy);
+  return make_float3(gx, gy, gz);
+}
+
+__device__ __forceinline__ void tri_scatter(float* __restrict__ g, Dims3 d, const Tri& t, float val) {
+  const float wx[2] = {1.f - t.wx, t.wx}, wy[2] = {1.f - t.wy, t.wy}, wz[2] = {1.f - t.wz, t.wz};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        float w = wx[i] * wy[j] * wz[k];
+        if (w != 0.f) atomic_addf(g + flat3(t.ix[i], t.iy[j], t.iz[k], d), val * w);
+      }
+}
+
+__device__ __forceinline__ void voxel_of(size_t i, Dims3 d, int& x, int& y, int& z) {
+  x = (int)(i % d.x);
+  size_t r = i / d.x;
+  y = (int)(r % d.y);
+  z = (int)(r / d.y);
+}
+
+// pass 1: f = trilinear(d, x - v dt)
+__global__ __launch_bounds__(256) void k_advect_sl(const float* __restrict__ dens, const float* __restrict__ vel,
+                                                   float* __restrict__ f, AdvectDims a, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  voxel_of(i, a.d, x, y, z);
+  float3 v = load_vel(vel, i);
+  Tri t = tri_clamp(dens, a.d, x - v.x * a.dt, y - v.y * a.dt, z - v.z * a.dt);
+  f[i] = tri_value(t);
+}
+
+// order 1 output: out = f, or max(f, 0) with NEGATIVE (mask marks clamped cells;
+// tf.maximum passes the gradient at equality)
+__global__ __launch_bounds__(256) void k_advect_o1_out(const float* __restrict__ f, float* __restrict__ out,
+                                                       uint8_t* __restrict__ mask, AdvectDims a, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float r = f[i];
+  uint8_t m = 0;
+  if (a.clamp == CLAMP_NEGATIVE) {
+    m = r < 0.f;
+    r = fmaxf(r, 0.f);
+  }
+  out[i] = r;
+  mask[i] = m;
+}
+
+// pass 2: MacCormack correction with the MC_SMOOTH revert clamp
+__global__ __launch_bounds__(256) void k_advect_mc(const float* __restrict__ dens, const float* __restrict__ vel,
+                                                   const float* __restrict__ f, float* __restrict__ out,
+                                                   uint8_t* __restrict__ mask, AdvectDims a, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  voxel_of(i, a.d, x, y, z);
+  float3 v = load_vel(vel, i);
+  float fx = f[i];
+  Tri tb = tri_clamp(f, a.d, x + v.x * a.dt, y + v.y * a.dt, z + v.z * a.dt);
+  float b = tri_value(tb);
+  float c = fx + 0.5f * (dens[i] - b);
+  uint8_t m = 0;
+  if (a.clamp == CLAMP_MC_SMOOTH) {
+    // corner extrema with the reference's ceil corners (ceil == floor at
+    // integer positions), CLAMP indices (sampling.hpp:372-491)
+    float px = x - v.x * a.dt, py = y - v.y * a.dt, pz = z - v.z * a.dt;
+    int fxi = (int)floorf(px), fyi = (int)floorf(py), fzi = (int)floorf(pz);
+    int cxi = (int)ceilf(px), cyi = (int)ceilf(py), czi = (int)ceilf(pz);
+    int ix[2] = {clampi(fxi, 0, a.d.x - 1), clampi(cxi, 0, a.d.x - 1)};
+    int iy[2] = {clampi(fyi, 0, a.d.y - 1), clampi(cyi, 0, a.d.y - 1)};
+    int iz[2] = {clampi(fzi, 0, a.d.z - 1), clampi(czi, 0, a.d.z - 1)};
+    float mn = kFloatMax, mx = -kFloatMax;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int ii = 0; ii < 2; ++ii) {
+          float s = dens[flat3(ix[ii], iy[j], iz[k], a.d)];
+          mn = fminf(mn, s);
+          mx = fmaxf(mx, s);
+        }
+    m = (c < mn) | (c > mx);
+    if (m) c = fx;
+  }
+  if (a.clamp == CLAMP_NEGATIVE) {
+    m = c < 0.f;  // max(c, 0): zero gradient where clamped
+    c = fmaxf(c, 0.f);
+  }
+  out[i] = c;
+  mask[i] = m;
+}
+
+// backward A (MacCormack only): scatter of the back-warp gradient into gf,
+// direct density term, velocity term of the back warp
+__global__ __launch_bounds__(256) void k_advect_bwd_a(const float* __restrict__ vel, const float* __restrict__ f,
+                                                      const uint8_t* __restrict__ mask,
+                                                      const float* __restrict__ gout, float* __restrict__ gf_acc,
+                                                      float* __restrict__ gd, float* __restrict__ gv, AdvectDims a,
+                                                      size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  voxel_of(i, a.d, x, y, z);
+  float3 v = load_vel(vel, i);
+  float g = gout[i];
+  uint8_t m = mask[i];
+  // gradient of the corrected value c: zero where MC_SMOOTH reverted to f or
+  // NEGATIVE clamped to 0
+  float gc = m ? 0.f : g;
+  float gb = -0.5f * gc;
+  gd[i] = 0.5f * gc;
+  Tri tb = tri_clamp(f, a.d, x + v.x * a.dt, y + v.y * a.dt, z + v.z * a.dt);
+  float3 df = tri_grad(tb);
+  gv[3 * i] = gb * a.dt * df.x;
+  gv[3 * i + 1] = gb * a.dt * df.y;
+  gv[3 * i + 2] = gb * a.dt * df.z;
+  if (gb != 0.f) tri_scatter(gf_acc, a.d, tb, gb);
+}
+
+// backward B: total gf scattered into gd at x - v dt, velocity term of the
+// forward warp. FIRST = true when pass A did not run (order 1): gv is
+// initialised here and gd must have been cleared.
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_advect_bwd_b(const float* __restrict__ dens, const float* __restrict__ vel,
+                                                      const float* __restrict__ gout,
+                                                      const float* __restrict__ gf_acc,
+                                                      const uint8_t* __restrict__ mask, float* __restrict__ gd,
+                                                      float* __restrict__ gv, AdvectDims a, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  voxel_of(i, a.d, x, y, z);
+  float3 v = load_vel(vel, i);
+  float g = gout[i];
+  float gf;
+  if (FIRST) {
+    // order 1: out = f (or max(f,0) with NEGATIVE)
+    gf = (a.clamp == CLAMP_NEGATIVE && mask[i]) ? 0.f : g;
+  } else {
+    // f(x) enters out directly in both clamp branches, plus the back-warp scatter
+    uint8_t m = mask[i];
+    float gc = m ? 0.f : g;
+    gf = (a.clamp == CLAMP_MC_SMOOTH ? g : gc) + gf_acc[i];
+  }
+  Tri tf = tri_clamp(dens, a.d, x - v.x * a.dt, y - v.y * a.dt, z - v.z * a.dt);
+  float3 dd = tri_grad(tf);
+  float3 gvv = make_float3(-gf * a.dt * dd.x, -gf * a.dt * dd.y, -gf * a.dt * dd.z);
+  if (FIRST) {
+    gv[3 * i] = gvv.x;
+    gv[3 * i + 1] = gvv.y;
+    gv[3 * i + 2] = gvv.z;
+  } else {
+    gv[3 * i] += gvv.x;
+    gv[3 * i + 1] += gvv.y;
+    gv[3 * i + 2] += gvv.z;
+  }
+  if (gf != 0.f) tri_scatter(gd, a.d, tf, gf);
+}
+
+static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel, float* f, float* out,
+                             uint8_t* mask, const AdvectDims& a) {
+  size_t n = size_t(a.d.x) * a.d.y * a.d.z;
+  hipLaunchKernelGGL(k_advect_sl, grid1(n), dim3(256), 0, s, dens, vel, f, a, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (a.order == 2) {
+    hipLaunchKernelGGL(k_advect_mc, grid1(n), dim3(256), 0, s, dens, vel, f, out, mask, a, n);
+  } else {
+    hipLaunchKernelGGL(k_advect_o1_out, grid1(n), dim3(256), 0, s, f, out, mask, a, n);
+  }
+  return hipGetLastError();
+}
+
+hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
+                             const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a) {
+  size_t n = size_t(a.d.x) * a.d.y * a.d.z;
+  if (a.order == 2) {
+    hipLaunchKernelGGL(k_advect_bwd_a, grid1(n), dim3(256), 0, s, vel, f, mask, gout, gf_acc, gd, gv, a, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_advect_bwd_b<false>, grid1(n), dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a, n);
+  } else {
+    // gd must be zero on entry (the caller clears it)
+    hipLaunchKernelGGL(k_advect_bwd_b<true>, grid1(n), dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a, n);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gft

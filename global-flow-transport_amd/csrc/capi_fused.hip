@@ -1,0 +1,204 @@
+// C ABI of the fused transport/render path: MacCormack advection with the
+// MC_SMOOTH clamp (VelocityGrid.warp, data_structures.py:870-905) and the
+// ambient-light density render with the L2 image loss
+// (Renderer.render_density, renderer.py:603-651; loss_dens_target,
+// optimization.py:396-412).
+#include "gft_host.hpp"
+
+namespace gft {
+
+
+hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel, float* f, float* out, uint8_t* mask,
+                             const AdvectDims& a);
+hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
+                             const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a);
+hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
+                                     const RayLaunch& L, const RenderParams& rp);
+hipError_t render_density_bwd_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
+                                     const float* target, float* grad, float* loss, const RayDims& rd,
+                                     const RayLaunch& L, const RenderParams& rp);
+hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v);
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static int check_scalar_grid(const int64_t* shape) {
+  for (int i = 0; i < 5; ++i)
+    if (shape[i] < 1) return fail(GFT_ERR_INVALID_ARGUMENT, "Empty input (NDHWC): dimension %d is %lld", i, (long long)shape[i]);
+  if (shape[0] != 1) return fail(GFT_ERR_INVALID_ARGUMENT, "fused transport/render expects a data batch of 1");
+  if (shape[4] != 1) return fail(GFT_ERR_UNIMPLEMENTED, "fused transport/render expects a 1-channel density grid");
+  return GFT_OK;
+}
+
+static int check_advect_mode(int32_t order, int32_t clamp, float dt) {
+  if (order != 1 && order != 2) return fail(GFT_ERR_INVALID_ARGUMENT, "Unsupported warp order '%d'", order);
+  if (clamp < 0 || clamp > 2) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid warp clamp %d", clamp);
+  if (!std::isfinite(dt)) return fail(GFT_ERR_INVALID_ARGUMENT, "dt must be finite");
+  return GFT_OK;
+}
+
+static int ray_views(const std::vector<ViewPair>& pairs, size_t first, size_t count, const float* M, const float* V,
+                     const float* P, RayLaunch& L) {
+  L.num_views = (int)count;
+  for (size_t i = 0; i < count; ++i) {
+    const ViewPair& vp = pairs[first + i];
+    Mat4d mv = mat_mul(mat_from_colmajor(V + 16 * vp.cam), mat_from_colmajor(M + 16 * vp.batch)), mvinv, pinv;
+    if (!mat_inverse(mv, mvinv)) return fail(GFT_ERR_INVALID_ARGUMENT, "model-view matrix is singular");
+    if (!mat_inverse(mat_from_colmajor(P + 16 * vp.cam), pinv))
+      return fail(GFT_ERR_INVALID_ARGUMENT, "projection matrix is singular");
+    mat_to_float(pinv, L.view[i].pinv);
+    mat_to_float(mvinv, L.view[i].mvinv);
+    L.view[i].in_batch = vp.batch;
+    L.view[i].out_slot = vp.slot;
+  }
+  return GFT_OK;
+}
+
+}  // namespace gft
+
+using namespace gft;
+
+extern "C" {
+
+size_t gft_advect_saved_bytes(const int64_t shape[5]) {
+  size_t vox = (size_t)shape[1] * shape[2] * shape[3];
+  return align256(vox * sizeof(float)) + align256(vox);
+}
+
+int gft_advect_fwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
+                   int32_t order, int32_t clamp, float* output, void* saved, size_t saved_bytes) {
+  int rc;
+  if ((rc = check_scalar_grid(shape)) || (rc = check_advect_mode(order, clamp, dt))) return rc;
+  if (!input || !velocity || !output) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  size_t need = gft_advect_saved_bytes(shape);
+  if (!saved || saved_bytes < need)
+    return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes, need);
+  size_t vox = (size_t)shape[1] * shape[2] * shape[3];
+  AdvectDims a{Dims3{(int)shape[3], (int)shape[2], (int)shape[1]}, dt, order, clamp};
+  float* f = (float*)saved;
+  uint8_t* mask = (uint8_t*)saved + align256(vox * sizeof(float));
+  return check_hip(advect_fwd_launch((hipStream_t)stream, input, velocity, f, output, mask, a), "advect forward");
+}
+
+size_t gft_advect_bwd_workspace(const int64_t shape[5]) {
+  return align256((size_t)shape[1] * shape[2] * shape[3] * sizeof(float));
+}
+
+int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
+                   int32_t order, int32_t clamp, const void* saved, const float* output_grad, float* input_grad,
+                   float* velocity_grad, void* workspace, size_t workspace_bytes) {
+  int rc;
+  if ((rc = check_scalar_grid(shape)) || (rc = check_advect_mode(order, clamp, dt))) return rc;
+  if (!input || !velocity || !saved || !output_grad || !input_grad || !velocity_grad)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  size_t need = gft_advect_bwd_workspace(shape);
+  if (order == 2 && (!workspace || workspace_bytes < need))
+    return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes, need);
+  hipStream_t s = (hipStream_t)stream;
+  size_t vox = (size_t)shape[1] * shape[2] * shape[3];
+  AdvectDims a{Dims3{(int)shape[3], (int)shape[2], (int)shape[1]}, dt, order, clamp};
+  const float* f = (const float*)saved;
+  const uint8_t* mask = (const uint8_t*)saved + align256(vox * sizeof(float));
+  float* gf_acc = (float*)workspace;
+  if (order == 2) {
+    if ((rc = check_hip(fill_launch(s, gf_acc, vox, 0.f), "clear gf"))) return rc;
+  } else {
+    if ((rc = check_hip(fill_launch(s, input_grad, vox, 0.f), "clear input_grad"))) return rc;
+  }
+  return check_hip(advect_bwd_launch(s, input, velocity, f, mask, output_grad, gf_acc, input_grad, velocity_grad, a),
+                   "advect backward");
+}
+
+int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
+                           const float* V, const float* P, const float* frustum, int32_t num_cameras,
+                           const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
+                           float* images) {
+  (void)frustum;
+  int rc;
+  if ((rc = check_scalar_grid(shape))) return rc;
+  if (num_M != 1) return fail(GFT_ERR_INVALID_ARGUMENT, "model matrix batch size mismatch");
+  if (num_cameras < 1) return fail(GFT_ERR_INVALID_ARGUMENT, "camera batch size mismatch");
+  for (int i = 0; i < 3; ++i)
+    if (output_shape[i] < 1)
+      return fail(GFT_ERR_INVALID_ARGUMENT, "All shape dimensions must be positive, dimension %d is %d", i, output_shape[i]);
+  if (boundary < 0 || boundary > 2) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid boundary mode");
+  if (!density || !images || !M || !V || !P || !light) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  RayDims rd;
+  rd.in = Dims3{(int)shape[3], (int)shape[2], (int)shape[1]};
+  rd.C = 1;
+  rd.out_d = output_shape[0];
+  rd.out_h = output_shape[1];
+  rd.out_w = output_shape[2];
+  rd.in_batch_stride = (size_t)shape[1] * shape[2] * shape[3];
+  rd.out_view_stride = (size_t)rd.out_h * rd.out_w * 4;
+  RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
+  auto pairs = view_pairs(1, num_cameras, true);
+  for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
+    size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
+    RayLaunch L;
+    if ((rc = ray_views(pairs, first, cnt, M, V, P, L))) return rc;
+    if ((rc = check_hip(render_density_fwd_launch(boundary, (hipStream_t)stream, density, images, rd, L, rp),
+                        "render forward")))
+      return rc;
+  }
+  return GFT_OK;
+}
+
+static int render_bwd_common(void* stream, const float* density, const int64_t shape[5], const float* M,
+                             int32_t num_M, const float* V, const float* P, int32_t num_cameras,
+                             const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
+                             const float* images, const float* second, bool l2, float loss_scale, float* density_grad,
+                             float* loss) {
+  int rc;
+  if ((rc = check_scalar_grid(shape))) return rc;
+  if (num_M != 1) return fail(GFT_ERR_INVALID_ARGUMENT, "model matrix batch size mismatch");
+  if (num_cameras < 1) return fail(GFT_ERR_INVALID_ARGUMENT, "camera batch size mismatch");
+  for (int i = 0; i < 3; ++i)
+    if (output_shape[i] < 1)
+      return fail(GFT_ERR_INVALID_ARGUMENT, "All shape dimensions must be positive, dimension %d is %d", i, output_shape[i]);
+  if (boundary < 0 || boundary > 2) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid boundary mode");
+  if (!density || !images || !second || !density_grad || (l2 && !loss) || !M || !V || !P || !light)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  hipStream_t s = (hipStream_t)stream;
+  RayDims rd;
+  rd.in = Dims3{(int)shape[3], (int)shape[2], (int)shape[1]};
+  rd.C = 1;
+  rd.out_d = output_shape[0];
+  rd.out_h = output_shape[1];
+  rd.out_w = output_shape[2];
+  rd.in_batch_stride = (size_t)shape[1] * shape[2] * shape[3];
+  rd.out_view_stride = (size_t)rd.out_h * rd.out_w * 4;
+  RenderParams rp{{light[0], light[1], light[2]}, depth_step, loss_scale};
+  if ((rc = check_hip(fill_launch(s, density_grad, rd.in_batch_stride, 0.f), "clear density_grad"))) return rc;
+  if (l2 && (rc = check_hip(fill_launch(s, loss, 1, 0.f), "clear loss"))) return rc;
+  auto pairs = view_pairs(1, num_cameras, true);
+  for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
+    size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
+    RayLaunch L;
+    if ((rc = ray_views(pairs, first, cnt, M, V, P, L))) return rc;
+    if ((rc = check_hip(render_density_bwd_launch(boundary, l2, s, density, images, second, density_grad, loss, rd, L, rp),
+                        "render backward")))
+      return rc;
+  }
+  return GFT_OK;
+}
+
+int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t shape[5], const float* M,
+                              int32_t num_M, const float* V, const float* P, const float* frustum,
+                              int32_t num_cameras, const int32_t output_shape[3], const float light[3],
+                              float depth_step, int32_t boundary, const float* images, const float* targets,
+                              float loss_scale, float* density_grad, float* loss) {
+  (void)frustum;
+  return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
+                           boundary, images, targets, true, loss_scale, density_grad, loss);
+}
+
+int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
+                           const float* V, const float* P, const float* frustum, int32_t num_cameras,
+                           const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
+                           const float* images, const float* images_grad, float* density_grad) {
+  (void)frustum;
+  return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
+                           boundary, images, images_grad, false, 0.f, density_grad, nullptr);
+}
+
+}  // extern "C"

@@ -311,6 +311,21 @@ struct RayLaunch {
   int num_views;
 };
 
+// Fused advection launch parameters.
+struct AdvectDims {
+  Dims3 d;
+  float dt;
+  int order;   // 1 = semi-Lagrangian, 2 = MacCormack
+  int clamp;   // 0 none, 1 MC_SMOOTH, 2 NEGATIVE
+};
+
+// Fused density render parameters.
+struct RenderParams {
+  float light[3];  // light colour * intensity
+  float step;      // camera depth step (renderer.py:420-437)
+  float loss_scale;
+};
+
 // Shape of one raymarch launch.
 struct RayDims {
   Dims3 in;       // W, H, D of the sampled grid

@@ -37,6 +37,8 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
+_f32 = ctypes.c_float
+_f32p = ctypes.POINTER(ctypes.c_float)
 
 # name -> (restype, argtypes)
 _SIGNATURES = {
@@ -47,6 +49,16 @@ _SIGNATURES = {
     "gft_raymarch_grid_transform_bwd_workspace": (_sz, [_i64p, _i32]),
     "gft_raymarch_grid_transform_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _i32p,
                                                        _i32, _i32, _i32, _i32, _i32, _i32, _vp, _sz, _vp]),
+    "gft_advect_saved_bytes": (_sz, [_i64p]),
+    "gft_advect_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _f32, _i32, _i32, _vp, _vp, _sz]),
+    "gft_advect_bwd_workspace": (_sz, [_i64p]),
+    "gft_advect_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _f32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "gft_render_density_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
+                                              _i32, _vp]),
+    "gft_render_density_bwd_l2": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
+                                                 _i32, _vp, _vp, _f32, _vp, _vp]),
+    "gft_render_density_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
+                                              _i32, _vp, _vp, _vp]),
 }
 
 

@@ -1,0 +1,247 @@
+"""Fused transport/render path: advect -> render -> L2 loss, forward and
+exact backward, on the HIP kernels of libgft.
+
+``advect`` and ``render_density`` are differentiable building blocks
+(torch.autograd.Function). ``TransportRenderStep`` runs one optimisation
+step's fused forward and backward on preallocated buffers, issuing only HIP
+kernel launches on torch's current stream, so it can be captured in a
+hipGraph (torch.cuda.CUDAGraph).
+
+Semantics: the density warp of State.density_advected / VelocityGrid.warp
+(phitest/render/data_structures.py:263-266,870-905) with a centred velocity
+LUT (lut = -v*dt, data_structures.py:864-868), followed by
+Renderer.render_density with an ambient light (renderer.py:603-651) and the
+density target loss (optimization.py:396-412) with an L2 loss function.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .ops import BOUNDARY, _cams, _enum, _np_ptr, _shape3
+
+CLAMP = {"NONE": 0, "MC_SMOOTH": 1, "NEGATIVE": 2}
+
+
+def _scalar_grid(t, name):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32:
+        raise _lib.InvalidArgumentError("%s must be a float32 CUDA tensor" % name)
+    if t.dim() != 5 or t.shape[0] != 1 or t.shape[4] != 1:
+        raise _lib.InvalidArgumentError("%s must be [1,D,H,W,1], got %s" % (name, list(t.shape)))
+    return t.contiguous()
+
+
+def _velocity(v, dens):
+    if not isinstance(v, torch.Tensor) or not v.is_cuda or v.dtype != torch.float32:
+        raise _lib.InvalidArgumentError("velocity must be a float32 CUDA tensor")
+    if tuple(v.shape) != tuple(dens.shape[:4]) + (3,):
+        raise _lib.InvalidArgumentError("Shape mismatch: velocity %s for grid %s" % (list(v.shape), list(dens.shape)))
+    return v.contiguous()
+
+
+# -------------------------------------------------------------- advection
+def advect_forward(d, vel, dt=1.0, order=2, clamp="MC_SMOOTH", out=None, saved=None):
+    L = _lib.lib()
+    shape = _lib.i64_array(d.shape)
+    nsaved = L.gft_advect_saved_bytes(shape)
+    if saved is None:
+        saved = torch.empty((nsaved,), dtype=torch.uint8, device=d.device)
+    if out is None:
+        out = torch.empty_like(d)
+    _lib.check(L.gft_advect_fwd(_lib.stream_of(d), _lib.ptr(d), shape, _lib.ptr(vel), float(dt), int(order),
+                                CLAMP[clamp], _lib.ptr(out), _lib.ptr(saved), nsaved))
+    return out, saved
+
+
+def advect_backward(d, vel, saved, grad_out, dt=1.0, order=2, clamp="MC_SMOOTH", gd=None, gv=None, ws=None):
+    L = _lib.lib()
+    shape = _lib.i64_array(d.shape)
+    nws = L.gft_advect_bwd_workspace(shape)
+    if ws is None:
+        ws = torch.empty((nws,), dtype=torch.uint8, device=d.device)
+    gd = torch.empty_like(d) if gd is None else gd
+    gv = torch.empty_like(vel) if gv is None else gv
+    _lib.check(L.gft_advect_bwd(_lib.stream_of(d), _lib.ptr(d), shape, _lib.ptr(vel), float(dt), int(order),
+                                CLAMP[clamp], _lib.ptr(saved), _lib.ptr(grad_out.contiguous()), _lib.ptr(gd),
+                                _lib.ptr(gv), _lib.ptr(ws), nws))
+    return gd, gv
+
+
+class _AdvectFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, d, vel, dt, order, clamp):
+        out, saved = advect_forward(d, vel, dt, order, clamp)
+        ctx.save_for_backward(d, vel, saved)
+        ctx.cfg = (dt, order, clamp)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        d, vel, saved = ctx.saved_tensors
+        dt, order, clamp = ctx.cfg
+        gd, gv = advect_backward(d, vel, saved, g, dt, order, clamp)
+        return gd, gv, None, None, None
+
+
+def advect(d, velocity, dt=1.0, order=2, clamp="MC_SMOOTH"):
+    """Warp a scalar grid [1,D,H,W,1] by a centred velocity [1,D,H,W,3]
+    (VelocityGrid.warp with a CLAMP LuT sampler); exact gradients for both."""
+    d = _scalar_grid(d, "density")
+    velocity = _velocity(velocity, d)
+    if order not in (1, 2):
+        raise _lib.InvalidArgumentError("Unsupported warp order '%s'" % order)
+    clamp = str(clamp).upper()
+    if clamp == "MC":
+        raise NotImplementedError("MIM and MAX warp sampling have wrong gradients.")
+    if clamp not in CLAMP:
+        raise _lib.InvalidArgumentError("invalid warp clamp '%s'" % clamp)
+    return _AdvectFn.apply(d, velocity, float(dt), int(order), clamp)
+
+
+# ----------------------------------------------------------------- render
+class _Cameras(object):
+    """Host camera arrays kept alive for the C calls."""
+
+    def __init__(self, M, V, P, F, out_dhw):
+        self.M, self.nM, self.V, self.P, self.F, self.nCam = _cams(M, V, P, F)
+        self.out_dhw = _shape3(out_dhw)
+        self.c_out = _lib.i32_array(self.out_dhw)
+
+
+def render_density_forward(d, cams, depth_step, light=(1.0, 1.0, 1.0), boundary="BORDER", out=None):
+    L = _lib.lib()
+    if out is None:
+        out = torch.empty((1, cams.nCam, cams.out_dhw[1], cams.out_dhw[2], 4), device=d.device, dtype=torch.float32)
+    lt = (ctypes.c_float * 3)(*[float(x) for x in light])
+    _lib.check(L.gft_render_density_fwd(_lib.stream_of(d), _lib.ptr(d), _lib.i64_array(d.shape), _np_ptr(cams.M),
+                                        cams.nM, _np_ptr(cams.V), _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam,
+                                        cams.c_out, lt, float(depth_step), _enum(BOUNDARY, boundary, "boundary"),
+                                        _lib.ptr(out)))
+    return out
+
+
+class _RenderFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, d, cams, depth_step, light, boundary):
+        img = render_density_forward(d, cams, depth_step, light, boundary)
+        ctx.save_for_backward(d, img)
+        ctx.cfg = (cams, depth_step, light, boundary)
+        return img
+
+    @staticmethod
+    def backward(ctx, g):
+        d, img = ctx.saved_tensors
+        cams, depth_step, light, boundary = ctx.cfg
+        gd = torch.empty_like(d)
+        lt = (ctypes.c_float * 3)(*[float(x) for x in light])
+        _lib.check(_lib.lib().gft_render_density_bwd(
+            _lib.stream_of(d), _lib.ptr(d), _lib.i64_array(d.shape), _np_ptr(cams.M), cams.nM, _np_ptr(cams.V),
+            _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt, float(depth_step),
+            _enum(BOUNDARY, boundary, "boundary"), _lib.ptr(img), _lib.ptr(g.contiguous()), _lib.ptr(gd)))
+        return gd, None, None, None, None
+
+
+def render_density(d, matrix_m, matrix_v, matrix_p, frustum_params, output_shape, depth_step,
+                   light=(1.0, 1.0, 1.0), boundary="BORDER"):
+    """Images [1,V,H,W,4] = (light * E, optical depth) of a density [1,D,H,W,1]
+    under an ambient light; differentiable w.r.t. the density (exact)."""
+    d = _scalar_grid(d, "density")
+    cams = _Cameras(matrix_m, matrix_v, matrix_p, frustum_params, output_shape)
+    return _RenderFn.apply(d, cams, float(depth_step), tuple(float(x) for x in light), boundary)
+
+
+# ------------------------------------------------------------- fused step
+class TransportRenderStep(object):
+    """advect (MacCormack + MC_SMOOTH) -> ambient render -> L2 loss, forward
+    and exact backward, on fixed buffers. ``run(d0, vel, target)`` leaves the
+    loss in ``self.loss`` and the gradients in ``self.grad_density`` /
+    ``self.grad_velocity``."""
+
+    def __init__(self, grid_dhw, matrix_m, matrix_v, matrix_p, frustum_params, output_shape, depth_step,
+                 light=(1.0, 1.0, 1.0), order=2, clamp="MC_SMOOTH", dt=1.0, boundary="BORDER", device="cuda"):
+        D, H, W = grid_dhw
+        self.shape = (1, D, H, W, 1)
+        self.cams = _Cameras(matrix_m, matrix_v, matrix_p, frustum_params, output_shape)
+        self.depth_step = float(depth_step)
+        self.light = (ctypes.c_float * 3)(*[float(x) for x in light])
+        self.order, self.clamp, self.dt = int(order), CLAMP[str(clamp).upper()], float(dt)
+        self.boundary = _enum(BOUNDARY, boundary, "boundary")
+        L = _lib.lib()
+        self._L = L
+        self._shape = _lib.i64_array(self.shape)
+        dev = torch.device(device)
+        nV = self.cams.nCam
+        h, w = self.cams.out_dhw[1], self.cams.out_dhw[2]
+        self.n_pixels = nV * h * w
+        self.loss_scale = 1.0 / (self.n_pixels * 3)
+        self.d1 = torch.empty(self.shape, device=dev)
+        self.images = torch.empty((1, nV, h, w, 4), device=dev)
+        self.loss = torch.zeros((1,), device=dev)
+        self.grad_d1 = torch.empty(self.shape, device=dev)
+        self.grad_density = torch.empty(self.shape, device=dev)
+        self.grad_velocity = torch.empty((1, D, H, W, 3), device=dev)
+        self._nsaved = L.gft_advect_saved_bytes(self._shape)
+        self._nws = L.gft_advect_bwd_workspace(self._shape)
+        self.saved = torch.empty((self._nsaved,), dtype=torch.uint8, device=dev)
+        self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
+
+    PHASES = ("advect_fwd", "render_fwd", "render_bwd", "advect_bwd")
+
+    def _mark(self, timers, name, begin):
+        if timers is not None:
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record()
+            timers.setdefault(name, []).append(ev) if begin else timers[name].append(ev)
+
+    def forward(self, d0, vel, timers=None):
+        L, c, s = self._L, self.cams, _lib.stream_of(d0)
+        self._mark(timers, "advect_fwd", True)
+        _lib.check(L.gft_advect_fwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,
+                                    _lib.ptr(self.d1), _lib.ptr(self.saved), self._nsaved))
+        self._mark(timers, "advect_fwd", False)
+        self._mark(timers, "render_fwd", True)
+        _lib.check(L.gft_render_density_fwd(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
+                                            _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light, self.depth_step,
+                                            self.boundary, _lib.ptr(self.images)))
+        self._mark(timers, "render_fwd", False)
+        return self.images
+
+    def backward(self, d0, vel, target, timers=None):
+        L, c, s = self._L, self.cams, _lib.stream_of(d0)
+        self._mark(timers, "render_bwd", True)
+        _lib.check(L.gft_render_density_bwd_l2(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
+                                               _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
+                                               self.depth_step, self.boundary, _lib.ptr(self.images),
+                                               _lib.ptr(target), self.loss_scale, _lib.ptr(self.grad_d1),
+                                               _lib.ptr(self.loss)))
+        self._mark(timers, "render_bwd", False)
+        self._mark(timers, "advect_bwd", True)
+        _lib.check(L.gft_advect_bwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,
+                                    _lib.ptr(self.saved), _lib.ptr(self.grad_d1), _lib.ptr(self.grad_density),
+                                    _lib.ptr(self.grad_velocity), _lib.ptr(self.ws), self._nws))
+        self._mark(timers, "advect_bwd", False)
+
+    def run(self, d0, vel, target, timers=None):
+        """Fused forward + backward. ``timers`` (a dict) collects HIP event
+        pairs around each phase on the current stream."""
+        self.forward(d0, vel, timers)
+        self.backward(d0, vel, target, timers)
+        return self.loss, self.grad_density, self.grad_velocity
+
+    def algorithmic_bytes(self):
+        """Compulsory HBM bytes per phase at FP32 storage: each persistent
+        tensor read once and written once (intermediates excluded)."""
+        V = int(np.prod(self.shape))
+        N = self.n_pixels
+        return {"advect_fwd": 20 * V, "render_fwd": 4 * V + 16 * N, "render_bwd": 8 * V + 28 * N,
+                "advect_bwd": 36 * V}
+
+    def voxel_samples(self):
+        """Trilinear evaluations of one fwd+bwd step: advection fwd a*V
+        (a = 4 for MacCormack+MC_SMOOTH: forward, back, min, max; 1 for
+        semi-Lagrangian), bwd b*V (b = 2 or 1), raymarch sum W*H*S each way."""
+        V = int(np.prod(self.shape))
+        ray = self.n_pixels * self.cams.out_dhw[0]
+        a, b = (4, 2) if self.order == 2 else (1, 1)
+        return a * V + ray + b * V + ray

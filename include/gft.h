@@ -84,6 +84,54 @@ int gft_raymarch_grid_transform_bwd(void* stream, const float* input, const int6
                                     int32_t blending_mode, int32_t keep_dims, int32_t separate_camera_batch,
                                     int32_t grad_mode, void* workspace, size_t workspace_bytes, float* input_grad);
 
+/*
+ * Fused advection of a scalar grid [1,D,H,W,1] by a centred velocity
+ * [1,D,H,W,3] (cells per unit time): semi-Lagrangian (order 1) or MacCormack
+ * (order 2) with clamp 0 = NONE, 1 = MC_SMOOTH, 2 = NEGATIVE, CLAMP boundary,
+ * cell-centre offset 0. Replaces the SampleGridLut composition of
+ * VelocityGrid.warp (phitest/render/data_structures.py:870-905) over
+ * SampleGridLuTKernelLauncher (resample_grid.cu.cc:498-660).
+ * The forward writes the intermediates the backward needs into `saved`
+ * (gft_advect_saved_bytes). The backward is the exact adjoint w.r.t. the
+ * scalar and the velocity; input_grad and velocity_grad are overwritten.
+ */
+size_t gft_advect_saved_bytes(const int64_t shape[5]);
+int gft_advect_fwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
+                   int32_t order, int32_t clamp, float* output, void* saved, size_t saved_bytes);
+size_t gft_advect_bwd_workspace(const int64_t shape[5]);
+int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
+                   int32_t order, int32_t clamp, const void* saved, const float* output_grad, float* input_grad,
+                   float* velocity_grad, void* workspace, size_t workspace_bytes);
+
+/*
+ * Fused density render: BEER_LAMBERT raymarch of a density [1,D,H,W,1] lit
+ * by an ambient light (rgb emission = light * density * depth_step, optical
+ * depth = density * depth_step), images [1,V,H,W,4] = (r, g, b, optical
+ * depth). Replaces Renderer.render_density with an ambient Light
+ * (renderer.py:603-651, lighting.py:23-26) over raymarch_grid_transform
+ * (raymarch_grid.cu.cc:107-143) without the materialised 4-channel grid.
+ */
+int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
+                           const float* V, const float* P, const float* frustum, int32_t num_cameras,
+                           const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
+                           float* images);
+/*
+ * Backward of gft_render_density_fwd with the L2 image loss folded in:
+ * loss = loss_scale * sum (rgb - target)^2 (targets [1,V,H,W,3]); writes the
+ * scalar loss (device float) and the exact density gradient (overwritten).
+ * Replaces loss_dens_target (optimization.py:396-412) + the raymarch gradient.
+ */
+int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t shape[5], const float* M,
+                              int32_t num_M, const float* V, const float* P, const float* frustum,
+                              int32_t num_cameras, const int32_t output_shape[3], const float light[3],
+                              float depth_step, int32_t boundary, const float* images, const float* targets,
+                              float loss_scale, float* density_grad, float* loss);
+/* Backward of gft_render_density_fwd from an image gradient [1,V,H,W,4]. */
+int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
+                           const float* V, const float* P, const float* frustum, int32_t num_cameras,
+                           const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
+                           const float* images, const float* images_grad, float* density_grad);
+
 #ifdef __cplusplus
 }
 #endif

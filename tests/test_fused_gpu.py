@@ -1,0 +1,125 @@
+"""Parity of the fused HIP transport/render path with the oracle.
+
+Advection (MacCormack + MC_SMOOTH, semi-Lagrangian, NEGATIVE) forward and
+exact adjoint, ambient density render forward/backward, the L2 loss, and the
+whole advect -> render -> loss step. Tolerances (normalised max error vs the
+FP64 oracle): images 1e-5, advected grids 1e-5 outside clamp-switch voxels,
+gradients 1e-3.
+"""
+import numpy as np
+import pytest
+import torch
+
+from gft import synthetic as syn
+from helpers import norm_err, scene
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(grid, seed=0, speed=1.5, blobs=6):
+    d = syn.gaussian_blobs(grid, blobs, seed=seed)
+    v = syn.curl_noise_velocity(grid, seed=seed + 1, max_speed=speed, periods=(8, 4, 2))
+    return d, v
+
+
+@pytest.mark.parametrize("order,clamp", [(2, "MC_SMOOTH"), (2, "NONE"), (1, "NONE"), (1, "NEGATIVE"),
+                                         (2, "NEGATIVE")])
+def test_advect_forward(cuda, order, clamp):
+    from gft.pipeline import advect
+    grid = (14, 22, 18)
+    d, v = _fields(grid, seed=2)
+    if clamp == "NEGATIVE":
+        d = d - 0.3
+    got = advect(torch.from_numpy(d).cuda(), torch.from_numpy(v).cuda(), 1.0, order, clamp).cpu().numpy()
+    ref, parts = orc.advect_fwd(d, v, order, clamp, prec=0, return_parts=True)
+    diff = np.abs(got - ref) > 1e-5 * np.abs(ref).max()
+    # a clamp decision may flip between FP32 implementations on exact ties
+    assert diff.mean() < 2e-3
+    ref64 = orc.advect_fwd(d, v, order, clamp, prec=1)
+    assert norm_err(got[~diff], ref64[~diff]) < 1e-5
+
+
+@pytest.mark.parametrize("order,clamp", [(2, "MC_SMOOTH"), (1, "NONE"), (2, "NONE")])
+def test_advect_backward(cuda, order, clamp):
+    from gft.pipeline import advect
+    grid = (14, 22, 18)
+    d, v = _fields(grid, seed=4)
+    rng = np.random.Generator(np.random.PCG64(9))
+    g = rng.uniform(-1, 1, size=d.shape).astype(np.float32)
+    td = torch.from_numpy(d).cuda().requires_grad_(True)
+    tv = torch.from_numpy(v).cuda().requires_grad_(True)
+    out = advect(td, tv, 1.0, order, clamp)
+    out.backward(torch.from_numpy(g).cuda())
+    gd_ref, gv_ref = orc.advect_bwd(d, v, g, order, clamp, grad_mode="EXACT", prec=1)
+    assert norm_err(td.grad.cpu().numpy(), gd_ref) < 1e-3
+    assert norm_err(tv.grad.cpu().numpy(), gv_ref) < 1e-3
+
+
+def test_render_density_forward_backward(cuda):
+    from gft.pipeline import render_density
+    sc = scene(grid=(14, 22, 18), image=(20, 13), views=4, seed=6)
+    d = sc["data"]
+    td = torch.from_numpy(d).cuda().requires_grad_(True)
+    light = (0.9, 1.0, 1.1)
+    img = render_density(td, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], light)
+    ref = orc.render_density_fwd(d, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], light)
+    assert norm_err(img.detach().cpu().numpy(), ref) < 1e-5
+    rng = np.random.Generator(np.random.PCG64(3))
+    g = rng.uniform(-1, 1, size=ref.shape).astype(np.float32)
+    img.backward(torch.from_numpy(g).cuda())
+    gref = orc.render_density_bwd(d, img.detach().cpu().numpy(), g, sc["M"], sc["V"], sc["P"], sc["F"],
+                                  sc["out_dhw"], sc["step"], light)
+    assert norm_err(td.grad.cpu().numpy(), gref) < 1e-4
+
+
+@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP"])
+def test_fused_step_matches_oracle(cuda, boundary):
+    from gft.pipeline import TransportRenderStep
+    grid = (16, 26, 18)
+    sc = scene(grid=grid, image=(24, 14), views=3, seed=8)
+    d, v = _fields(grid, seed=10)
+    tgt_d = syn.gaussian_blobs(grid, 6, seed=2)
+    tgt = orc.render_density_fwd(tgt_d, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"],
+                                 boundary=boundary)[..., :3]
+    step = TransportRenderStep(grid, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], boundary=boundary)
+    td, tv = torch.from_numpy(d).cuda(), torch.from_numpy(v).cuda()
+    loss, gd, gv = step.run(td, tv, torch.from_numpy(tgt.astype(np.float32)).cuda())
+    torch.cuda.synchronize()
+    r = orc.step_fwd_bwd(d, v, tgt, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"],
+                         boundary_render=boundary)
+    assert abs(loss.item() - r["loss"]) <= 1e-5 * abs(r["loss"])
+    assert norm_err(step.images.cpu().numpy(), r["img"]) < 1e-5
+    assert norm_err(gd.cpu().numpy(), r["gd"]) < 1e-3
+    assert norm_err(gv.cpu().numpy(), r["gv"]) < 1e-3
+
+
+def test_fused_step_repeatable_and_graph_capturable(cuda):
+    from gft.pipeline import TransportRenderStep
+    grid = (16, 26, 18)
+    sc = scene(grid=grid, image=(24, 14), views=3, seed=8)
+    d, v = _fields(grid, seed=12)
+    tgt = torch.zeros((1, 3) + sc["out_dhw"][1:] + (3,), device="cuda")
+    step = TransportRenderStep(grid, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"])
+    td, tv = torch.from_numpy(d).cuda(), torch.from_numpy(v).cuda()
+    step.run(td, tv, tgt)
+    torch.cuda.synchronize()
+    eager = (step.loss.clone(), step.grad_density.clone(), step.grad_velocity.clone(), step.images.clone())
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step.run(td, tv, tgt)
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step.run(td, tv, tgt)
+    for _ in range(3):
+        step.grad_density.fill_(123.0)
+        step.images.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(step.images, eager[3])
+        assert abs(step.loss.item() - eager[0].item()) <= 1e-6 * abs(eager[0].item())
+        assert norm_err(step.grad_density.cpu().numpy(), eager[1].cpu().numpy()) < 1e-5
+        assert norm_err(step.grad_velocity.cpu().numpy(), eager[2].cpu().numpy()) < 1e-5
