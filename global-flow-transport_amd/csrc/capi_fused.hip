@@ -14,9 +14,14 @@ hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel,
                              const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a);
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
                                      const RayLaunch& L, const RenderParams& rp);
-hipError_t render_density_bwd_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
-                                     const float* target, float* grad, float* loss, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp);
+hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* dens, const float* img,
+                                      const float* second, float* gs, float4* rays, float* loss, const RayDims& rd,
+                                      const RayLaunch& L, const RenderParams& rp);
+hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
+                                        const RayDims& rd, const ProjLaunch& PL);
+hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
+                                         const float* second, float* grad, float* loss, const RayDims& rd,
+                                         const RayLaunch& L, const RenderParams& rp);
 hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v);
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -37,19 +42,51 @@ static int check_advect_mode(int32_t order, int32_t clamp, float dt) {
 }
 
 static int ray_views(const std::vector<ViewPair>& pairs, size_t first, size_t count, const float* M, const float* V,
-                     const float* P, RayLaunch& L) {
+                     const float* P, RayLaunch& L, ProjLaunch* PL = nullptr) {
   L.num_views = (int)count;
+  if (PL) PL->num_views = (int)count;
   for (size_t i = 0; i < count; ++i) {
     const ViewPair& vp = pairs[first + i];
     Mat4d mv = mat_mul(mat_from_colmajor(V + 16 * vp.cam), mat_from_colmajor(M + 16 * vp.batch)), mvinv, pinv;
+    Mat4d p = mat_from_colmajor(P + 16 * vp.cam);
     if (!mat_inverse(mv, mvinv)) return fail(GFT_ERR_INVALID_ARGUMENT, "model-view matrix is singular");
-    if (!mat_inverse(mat_from_colmajor(P + 16 * vp.cam), pinv))
-      return fail(GFT_ERR_INVALID_ARGUMENT, "projection matrix is singular");
+    if (!mat_inverse(p, pinv)) return fail(GFT_ERR_INVALID_ARGUMENT, "projection matrix is singular");
     mat_to_float(pinv, L.view[i].pinv);
     mat_to_float(mvinv, L.view[i].mvinv);
     L.view[i].in_batch = vp.batch;
     L.view[i].out_slot = vp.slot;
+    if (PL) {
+      mat_to_float(mat_mul(p, mv), PL->view[i].pmv);
+      PL->view[i].out_slot = vp.slot;
+    }
   }
+  return GFT_OK;
+}
+
+static RayDims render_dims(const int64_t* shape, const int32_t* output_shape) {
+  RayDims rd;
+  rd.in = Dims3{(int)shape[3], (int)shape[2], (int)shape[1]};
+  rd.C = 1;
+  rd.out_d = output_shape[0];
+  rd.out_h = output_shape[1];
+  rd.out_w = output_shape[2];
+  rd.in_batch_stride = (size_t)shape[1] * shape[2] * shape[3];
+  rd.out_view_stride = (size_t)rd.out_h * rd.out_w * 4;
+  return rd;
+}
+
+static int check_render_args(const int64_t* shape, int32_t num_M, int32_t num_cameras, const int32_t* output_shape,
+                             int32_t boundary) {
+  int rc;
+  if ((rc = check_scalar_grid(shape))) return rc;
+  if (num_M != 1) return fail(GFT_ERR_INVALID_ARGUMENT, "model matrix batch size mismatch");
+  if (num_cameras < 1) return fail(GFT_ERR_INVALID_ARGUMENT, "camera batch size mismatch");
+  for (int i = 0; i < 3; ++i)
+    if (output_shape[i] < 1)
+      return fail(GFT_ERR_INVALID_ARGUMENT, "All shape dimensions must be positive, dimension %d is %d", i, output_shape[i]);
+  if (boundary < 0 || boundary > 2) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid boundary mode");
+  if ((int64_t)shape[1] * shape[2] * shape[3] >= ((int64_t)1 << 31))
+    return fail(GFT_ERR_INVALID_ARGUMENT, "grid too large (2^31 voxels)");
   return GFT_OK;
 }
 
@@ -114,23 +151,10 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
                            float* images) {
   (void)frustum;
   int rc;
-  if ((rc = check_scalar_grid(shape))) return rc;
-  if (num_M != 1) return fail(GFT_ERR_INVALID_ARGUMENT, "model matrix batch size mismatch");
-  if (num_cameras < 1) return fail(GFT_ERR_INVALID_ARGUMENT, "camera batch size mismatch");
-  for (int i = 0; i < 3; ++i)
-    if (output_shape[i] < 1)
-      return fail(GFT_ERR_INVALID_ARGUMENT, "All shape dimensions must be positive, dimension %d is %d", i, output_shape[i]);
-  if (boundary < 0 || boundary > 2) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid boundary mode");
+  if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
   if (!density || !images || !M || !V || !P || !light) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
-  RayDims rd;
-  rd.in = Dims3{(int)shape[3], (int)shape[2], (int)shape[1]};
-  rd.C = 1;
-  rd.out_d = output_shape[0];
-  rd.out_h = output_shape[1];
-  rd.out_w = output_shape[2];
-  rd.in_batch_stride = (size_t)shape[1] * shape[2] * shape[3];
-  rd.out_view_stride = (size_t)rd.out_h * rd.out_w * 4;
-  RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
+  const RayDims rd = render_dims(shape, output_shape);
+  const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
   auto pairs = view_pairs(1, num_cameras, true);
   for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
     size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
@@ -143,39 +167,58 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
   return GFT_OK;
 }
 
+size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
+                                        int32_t boundary) {
+  (void)shape;
+  if (boundary != GFT_BORDER) return 0;
+  const size_t plane = (size_t)output_shape[1] * output_shape[2];
+  return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
+         align256(plane * num_cameras * 2 * sizeof(float4));
+}
+
 static int render_bwd_common(void* stream, const float* density, const int64_t shape[5], const float* M,
                              int32_t num_M, const float* V, const float* P, int32_t num_cameras,
                              const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                             const float* images, const float* second, bool l2, float loss_scale, float* density_grad,
-                             float* loss) {
+                             const float* images, const float* second, bool l2, float loss_scale, void* workspace,
+                             size_t workspace_bytes, float* density_grad, float* loss) {
   int rc;
-  if ((rc = check_scalar_grid(shape))) return rc;
-  if (num_M != 1) return fail(GFT_ERR_INVALID_ARGUMENT, "model matrix batch size mismatch");
-  if (num_cameras < 1) return fail(GFT_ERR_INVALID_ARGUMENT, "camera batch size mismatch");
-  for (int i = 0; i < 3; ++i)
-    if (output_shape[i] < 1)
-      return fail(GFT_ERR_INVALID_ARGUMENT, "All shape dimensions must be positive, dimension %d is %d", i, output_shape[i]);
-  if (boundary < 0 || boundary > 2) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid boundary mode");
+  if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
   if (!density || !images || !second || !density_grad || (l2 && !loss) || !M || !V || !P || !light)
     return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  const size_t need = gft_render_density_bwd_workspace(shape, num_cameras, output_shape, boundary);
+  if (need && (!workspace || workspace_bytes < need))
+    return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
+                need);
   hipStream_t s = (hipStream_t)stream;
-  RayDims rd;
-  rd.in = Dims3{(int)shape[3], (int)shape[2], (int)shape[1]};
-  rd.C = 1;
-  rd.out_d = output_shape[0];
-  rd.out_h = output_shape[1];
-  rd.out_w = output_shape[2];
-  rd.in_batch_stride = (size_t)shape[1] * shape[2] * shape[3];
-  rd.out_view_stride = (size_t)rd.out_h * rd.out_w * 4;
-  RenderParams rp{{light[0], light[1], light[2]}, depth_step, loss_scale};
-  if ((rc = check_hip(fill_launch(s, density_grad, rd.in_batch_stride, 0.f), "clear density_grad"))) return rc;
+  const RayDims rd = render_dims(shape, output_shape);
+  const RenderParams rp{{light[0], light[1], light[2]}, depth_step, loss_scale};
   if (l2 && (rc = check_hip(fill_launch(s, loss, 1, 0.f), "clear loss"))) return rc;
   auto pairs = view_pairs(1, num_cameras, true);
+  if (boundary == GFT_BORDER) {
+    const size_t plane = (size_t)output_shape[1] * output_shape[2];
+    float* gs = (float*)workspace;
+    float4* rays = (float4*)((char*)workspace + align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)));
+    for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
+      size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
+      RayLaunch L;
+      ProjLaunch PL;
+      if ((rc = ray_views(pairs, first, cnt, M, V, P, L, &PL))) return rc;
+      PL.accumulate = first > 0;
+      if ((rc = check_hip(render_density_walk_launch(l2, s, density, images, second, gs, rays, loss, rd, L, rp),
+                          "render backward walk")))
+        return rc;
+      if ((rc = check_hip(render_density_gather_launch(s, gs, rays, density_grad, rd, PL), "render backward gather")))
+        return rc;
+    }
+    return GFT_OK;
+  }
+  if ((rc = check_hip(fill_launch(s, density_grad, rd.in_batch_stride, 0.f), "clear density_grad"))) return rc;
   for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
     size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
     RayLaunch L;
     if ((rc = ray_views(pairs, first, cnt, M, V, P, L))) return rc;
-    if ((rc = check_hip(render_density_bwd_launch(boundary, l2, s, density, images, second, density_grad, loss, rd, L, rp),
+    if ((rc = check_hip(render_density_scatter_launch(boundary, l2, s, density, images, second, density_grad, loss, rd,
+                                                      L, rp),
                         "render backward")))
       return rc;
   }
@@ -186,19 +229,22 @@ int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t 
                               int32_t num_M, const float* V, const float* P, const float* frustum,
                               int32_t num_cameras, const int32_t output_shape[3], const float light[3],
                               float depth_step, int32_t boundary, const float* images, const float* targets,
-                              float loss_scale, float* density_grad, float* loss) {
+                              float loss_scale, void* workspace, size_t workspace_bytes, float* density_grad,
+                              float* loss) {
   (void)frustum;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
-                           boundary, images, targets, true, loss_scale, density_grad, loss);
+                           boundary, images, targets, true, loss_scale, workspace, workspace_bytes, density_grad, loss);
 }
 
 int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                           const float* images, const float* images_grad, float* density_grad) {
+                           const float* images, const float* images_grad, void* workspace, size_t workspace_bytes,
+                           float* density_grad) {
   (void)frustum;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
-                           boundary, images, images_grad, false, 0.f, density_grad, nullptr);
+                           boundary, images, images_grad, false, 0.f, workspace, workspace_bytes, density_grad,
+                           nullptr);
 }
 
 }  // extern "C"

@@ -311,6 +311,18 @@ struct RayLaunch {
   int num_views;
 };
 
+// Forward projection of one view for the voxel-parallel gather: object space
+// -> clip space (P * V * M) and the output slot of the view.
+struct ProjView {
+  float pmv[16];
+  int out_slot;
+};
+struct ProjLaunch {
+  ProjView view[kMaxViewsPerLaunch];
+  int num_views;
+  int accumulate;  // 0: write the sum, 1: add it to the existing gradient
+};
+
 // Fused advection launch parameters.
 struct AdvectDims {
   Dims3 d;

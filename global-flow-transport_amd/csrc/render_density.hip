@@ -10,15 +10,25 @@
 // so rgb = (c*i) * E with one scalar accumulator E = sum exp(-D_i) sigma_i.
 // Output pixels are float4 (r, g, b, optical depth), like the reference.
 //
-// Backward: either from an image gradient, or with the L2 loss
-// loss = scale * sum (rgb - target)^2 over all views and pixels folded in,
-// in which case each lane computes g_rgb from its own pixel. The lane walks the ray from far to near
-// recovering D_{i-1} = max(D_i - sigma_i, 0) (no per-sample storage), forms
-// the density gradient of the sample
+// Backward, from an image gradient or with the L2 loss
+// loss = scale * sum (rgb - target)^2 folded in (each lane then forms g_rgb
+// from its own pixel): the ray is walked from far to near recovering
+// D_{i-1} = max(D_i - sigma_i, 0) (no forward state is stored), and the
+// density gradient of each sample is
 //   g_d = s * (exp(-D_i) * G + g_D),  G = sum_c (c*i)_c g_rgb_c,
-//   g_D <- g_D - exp(-D_i) * sigma_i * G
-// and scatters it to the 8 trilinear corners with float atomics (exact
-// adjoint, BORDER samples outside the grid carry no gradient).
+//   g_D <- g_D - exp(-D_i) * sigma_i * G.
+// Spreading g_d to the 8 trilinear corners is done in one of three ways:
+//   BORDER : atomic-free transpose. The walk stores g_d per sample (and the
+//            ray of every pixel); a voxel-parallel gather then enumerates, for
+//            each view, exactly the samples inside the voxel's trilinear
+//            support (box -> projected pixel range -> ray/box slab) and sums
+//            g_d * w in a fixed order: deterministic, one write per voxel.
+//   CLAMP  : samples outside the grid land on boundary voxels, so the support
+//            is unbounded; a tiled walk accumulates the scatter in an LDS
+//            brick per 16x16-pixel tile and K-sample segment and flushes it
+//            with coalesced global atomics.
+//   WRAP   : per-sample global atomics.
+// Samples of a BORDER ray outside the grid read 0 and are skipped (exact).
 #include "gft_device.hpp"
 
 namespace gft {
@@ -26,7 +36,70 @@ namespace gft {
 constexpr int kRTileX = 16;
 constexpr int kRTileY = 16;
 
+// ----------------------------------------------------------------- helpers
+__device__ __forceinline__ void pixel_ray(const RayView& v, const RayDims& rd, int x, int y, float3& nearp,
+                                          float3& step) {
+  const float dinvx = 1.f / (float)rd.out_w, dinvy = 1.f / (float)rd.out_h, dinvz = 1.f / (float)rd.out_d;
+  const float zmax = (float)max(rd.out_d - 1, 1);
+  const float izm = 1.f / zmax;
+  nearp = idx_to_os(v, x + 0.5f, y + 0.5f, 0.5f, dinvx, dinvy, dinvz);
+  float3 f = idx_to_os(v, x + 0.5f, y + 0.5f, zmax + 0.5f, dinvx, dinvy, dinvz);
+  step = make_float3((f.x - nearp.x) * izm, (f.y - nearp.y) * izm, (f.z - nearp.z) * izm);
+}
 
+// Sample range of a ray that can hold samples inside the grid (positions carry
+// the +0.5 cell-centre offset, so inside means 0 <= p < dims); one sample of
+// slack on each end, the per-sample test stays exact. lo > hi: no sample.
+__device__ __forceinline__ void clip_ray(float3 n, float3 st, Dims3 d, int S, int& lo, int& hi) {
+  float tlo = 0.f, thi = (float)(S - 1);
+  const float nn[3] = {n.x, n.y, n.z}, ss[3] = {st.x, st.y, st.z};
+  const float dd[3] = {(float)d.x, (float)d.y, (float)d.z};
+  bool miss = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (ss[a] != 0.f) {
+      float t0 = (0.f - nn[a]) / ss[a], t1 = (dd[a] - nn[a]) / ss[a];
+      tlo = fmaxf(tlo, fminf(t0, t1));
+      thi = fminf(thi, fmaxf(t0, t1));
+    } else if (!(0.f <= nn[a] && nn[a] < dd[a])) {
+      miss = true;
+    }
+  }
+  lo = max(0, (int)floorf(tlo) - 1);
+  hi = min(S - 1, (int)ceilf(thi) + 1);
+  if (miss || thi < tlo - 2.f) {
+    lo = 1;
+    hi = 0;
+  }
+}
+
+// BORDER trilinear read of a scalar grid at p (cell-centre offset removed).
+// Returns false when the sample is outside [-0.5, dims-0.5) (it reads 0).
+// Interior footprints use 32-bit offsets and no per-corner checks.
+__device__ __forceinline__ bool read_border(const float* __restrict__ buf, Dims3 d, float px, float py, float pz,
+                                            float& val) {
+  const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+  if (!((0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) & (qz < (float)d.z)))
+    return false;
+  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  if (((unsigned)ix < (unsigned)(d.x - 1)) & ((unsigned)iy < (unsigned)(d.y - 1)) &
+      ((unsigned)iz < (unsigned)(d.z - 1))) {
+    const float wx = px - fx, wy = py - fy, wz = pz - fz;
+    const unsigned sx = (unsigned)d.x, sxy = (unsigned)d.x * (unsigned)d.y;
+    const unsigned b = ((unsigned)iz * (unsigned)d.y + (unsigned)iy) * sx + (unsigned)ix;
+    const float c000 = buf[b], c001 = buf[b + 1], c010 = buf[b + sx], c011 = buf[b + sx + 1];
+    const float c100 = buf[b + sxy], c101 = buf[b + sxy + 1], c110 = buf[b + sxy + sx], c111 = buf[b + sxy + sx + 1];
+    const float v0 = lerpf(lerpf(c000, c001, wx), lerpf(c010, c011, wx), wy);
+    const float v1 = lerpf(lerpf(c100, c101, wx), lerpf(c110, c111, wx), wy);
+    val = lerpf(v0, v1, wz);
+  } else {
+    val = read_linear<1, B_BORDER>(buf, d, px, py, pz).v[0];
+  }
+  return true;
+}
+
+// ----------------------------------------------------------------- forward
 template <int BM>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
                                                             RayDims rd, RayLaunch L, RenderParams rp) {
@@ -34,90 +107,308 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
   if (x >= rd.out_w || y >= rd.out_h) return;
-  float dinvx = 1.f / (float)rd.out_w, dinvy = 1.f / (float)rd.out_h, dinvz = 1.f / (float)rd.out_d;
-  float zmax = (float)max(rd.out_d - 1, 1);
-  float3 nearp = idx_to_os(v, x + 0.5f, y + 0.5f, 0.5f, dinvx, dinvy, dinvz);
-  float3 farp = idx_to_os(v, x + 0.5f, y + 0.5f, zmax + 0.5f, dinvx, dinvy, dinvz);
-  float iz = 1.f / zmax;
-  float3 step = make_float3((farp.x - nearp.x) * iz, (farp.y - nearp.y) * iz, (farp.z - nearp.z) * iz);
+  float3 nearp, step;
+  pixel_ray(v, rd, x, y, nearp, step);
   const float* buf = dens + rd.in_batch_stride * v.in_batch;
   float D = 0.f, E = 0.f;
-  for (int i = 0; i < rd.out_d; ++i) {
-    float fi = (float)i;
-    float px = fmaf(fi, step.x, nearp.x) - 0.5f, py = fmaf(fi, step.y, nearp.y) - 0.5f,
-          pz = fmaf(fi, step.z, nearp.z) - 0.5f;
-    float sigma = rp.step * read_linear<1, BM>(buf, rd.in, px, py, pz).v[0];
-    D = fmaxf(D + sigma, 0.f);
-    E = fmaf(expf(-D), sigma, E);
+  if (BM == B_BORDER) {
+    int lo, hi;
+    clip_ray(nearp, step, rd.in, rd.out_d, lo, hi);
+    for (int i = lo; i <= hi; ++i) {
+      const float fi = (float)i;
+      float val;
+      if (!read_border(buf, rd.in, fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
+                       fmaf(fi, step.z, nearp.z) - 0.5f, val))
+        continue;
+      const float sigma = rp.step * val;
+      D = fmaxf(D + sigma, 0.f);
+      E = fmaf(__expf(-D), sigma, E);
+    }
+  } else {
+    for (int i = 0; i < rd.out_d; ++i) {
+      const float fi = (float)i;
+      const float sigma = rp.step * read_linear<1, BM>(buf, rd.in, fmaf(fi, step.x, nearp.x) - 0.5f,
+                                                       fmaf(fi, step.y, nearp.y) - 0.5f,
+                                                       fmaf(fi, step.z, nearp.z) - 0.5f).v[0];
+      D = fmaxf(D + sigma, 0.f);
+      E = fmaf(__expf(-D), sigma, E);
+    }
   }
-  float4 o = make_float4(rp.light[0] * E, rp.light[1] * E, rp.light[2] * E, D);
+  const float4 o = make_float4(rp.light[0] * E, rp.light[1] * E, rp.light[2] * E, D);
   reinterpret_cast<float4*>(img + rd.out_view_stride * v.out_slot)[size_t(y) * rd.out_w + x] = o;
 }
 
-// L2 = true: `target` holds rgb targets and the loss is accumulated;
-// L2 = false: `target` holds the image gradient (float4 per pixel).
-template <int BM, bool L2>
-__global__ __launch_bounds__(256) void k_render_density_bwd(const float* __restrict__ dens,
-                                                            const float* __restrict__ img,
-                                                            const float* __restrict__ target,
-                                                            float* __restrict__ grad, float* __restrict__ loss,
-                                                            RayDims rd, RayLaunch L, RenderParams rp) {
+// Per-pixel loss gradient: G (light-weighted rgb gradient), g_D (optical depth
+// gradient) and the squared error for the loss.
+template <bool L2>
+__device__ __forceinline__ void pixel_grad(const float* __restrict__ img, const float* __restrict__ second,
+                                           const RayDims& rd, const RayView& v, size_t pix, const RenderParams& rp,
+                                           float& G, float& gD, float& D, float& sq) {
+  const float4 o = reinterpret_cast<const float4*>(img + rd.out_view_stride * v.out_slot)[pix];
+  D = o.w;
+  if (L2) {
+    const float* t = second + (size_t)v.out_slot * rd.out_w * rd.out_h * 3 + pix * 3;
+    const float e0 = o.x - t[0], e1 = o.y - t[1], e2 = o.z - t[2];
+    sq = e0 * e0 + e1 * e1 + e2 * e2;
+    G = 2.f * rp.loss_scale * (rp.light[0] * e0 + rp.light[1] * e1 + rp.light[2] * e2);
+    gD = 0.f;  // no loss term on the optical depth channel
+  } else {
+    const float4 go = reinterpret_cast<const float4*>(second + rd.out_view_stride * v.out_slot)[pix];
+    G = rp.light[0] * go.x + rp.light[1] * go.y + rp.light[2] * go.z;
+    gD = go.w;
+    sq = 0.f;
+  }
+}
+
+__device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, float scale) {
   __shared__ float red[256 / 64];
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.f;
+    for (int w = 0; w < 256 / 64; ++w) s += red[w];
+    atomic_addf(loss, s * scale);
+  }
+}
+
+// -------------------------------------------------- BORDER: walk + gather
+// Walk: stores g_d of every inside sample at gs[((slot*S + i)*H + y)*W + x]
+// and the ray of every pixel at rays[2*((slot*H + y)*W + x) + {0,1}] =
+// (near, lo), (step, hi); lo > hi marks a pixel with no stored sample.
+template <bool L2>
+__global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __restrict__ dens,
+                                                                 const float* __restrict__ img,
+                                                                 const float* __restrict__ second,
+                                                                 float* __restrict__ gs, float4* __restrict__ rays,
+                                                                 float* __restrict__ loss, RayDims rd, RayLaunch L,
+                                                                 RenderParams rp) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
-  const bool active = x < rd.out_w && y < rd.out_h;
   float sq = 0.f;
-  if (active) {
+  if (x < rd.out_w && y < rd.out_h) {
     const size_t pix = size_t(y) * rd.out_w + x;
-    float4 o = reinterpret_cast<const float4*>(img + rd.out_view_stride * v.out_slot)[pix];
-    float G, gD;
-    if (L2) {
-      const float* t = target + (size_t)v.out_slot * rd.out_w * rd.out_h * 3 + pix * 3;
-      float e0 = o.x - t[0], e1 = o.y - t[1], e2 = o.z - t[2];
-      sq = e0 * e0 + e1 * e1 + e2 * e2;
-      G = 2.f * rp.loss_scale * (rp.light[0] * e0 + rp.light[1] * e1 + rp.light[2] * e2);
-      gD = 0.f;  // no loss term on the optical depth channel
-    } else {
-      float4 go = reinterpret_cast<const float4*>(target + rd.out_view_stride * v.out_slot)[pix];
-      G = rp.light[0] * go.x + rp.light[1] * go.y + rp.light[2] * go.z;
-      gD = go.w;
+    float G, gD, D;
+    pixel_grad<L2>(img, second, rd, v, pix, rp, G, gD, D, sq);
+    float3 nearp, step;
+    pixel_ray(v, rd, x, y, nearp, step);
+    int lo, hi;
+    clip_ray(nearp, step, rd.in, rd.out_d, lo, hi);
+    if (G == 0.f && gD == 0.f) {
+      lo = 1;
+      hi = 0;
     }
-    float dinvx = 1.f / (float)rd.out_w, dinvy = 1.f / (float)rd.out_h, dinvz = 1.f / (float)rd.out_d;
-    float zmax = (float)max(rd.out_d - 1, 1);
-    float3 nearp = idx_to_os(v, x + 0.5f, y + 0.5f, 0.5f, dinvx, dinvy, dinvz);
-    float3 farp = idx_to_os(v, x + 0.5f, y + 0.5f, zmax + 0.5f, dinvx, dinvy, dinvz);
-    float iz = 1.f / zmax;
-    float3 step = make_float3((farp.x - nearp.x) * iz, (farp.y - nearp.y) * iz, (farp.z - nearp.z) * iz);
+    const size_t plane = size_t(rd.out_w) * rd.out_h;
+    float4* r = rays + 2 * ((size_t)v.out_slot * plane + pix);
+    r[0] = make_float4(nearp.x, nearp.y, nearp.z, (float)lo);
+    r[1] = make_float4(step.x, step.y, step.z, (float)hi);
     const float* buf = dens + rd.in_batch_stride * v.in_batch;
-    float* gout = grad + rd.in_batch_stride * v.in_batch;
-    float D = o.w;
-    if (G != 0.f || gD != 0.f) {
-      for (int i = rd.out_d - 1; i >= 0; --i) {
-        float fi = (float)i;
-        float px = fmaf(fi, step.x, nearp.x) - 0.5f, py = fmaf(fi, step.y, nearp.y) - 0.5f,
-              pz = fmaf(fi, step.z, nearp.z) - 0.5f;
-        Taps tp = make_taps<BM>(px, py, pz, rd.in);
-        // a BORDER sample outside the grid read 0: D is unchanged and the
-        // sample does not depend on the density
-        if (BM == B_BORDER && !tp.inside) continue;
+    float* g = gs + ((size_t)v.out_slot * rd.out_d + (size_t)max(hi, 0)) * plane + pix;
+    for (int i = hi; i >= lo; --i, g -= plane) {
+      const float fi = (float)i;
+      float val;
+      if (!read_border(buf, rd.in, fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
+                       fmaf(fi, step.z, nearp.z) - 0.5f, val))
+        continue;
+      const float sigma = rp.step * val;
+      const float e = __expf(-D);
+      D = fmaxf(D - sigma, 0.f);
+      const float esg = e * sigma * G;
+      *g = rp.step * (e * G + (gD - esg));
+      gD -= esg;
+    }
+  }
+  if (L2) block_loss(sq, loss, rp.loss_scale);
+}
+
+// Gather: one lane per voxel c, all views of the launch summed in order.
+// A sample q (cell-centre offset removed) reaches c with weight
+// prod_a max(0, 1 - |q_a - c_a|); its support box (c-1, c+1) is projected to
+// find the candidate pixels, and each candidate ray is cut by the box.
+__global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* __restrict__ gs,
+                                                                   const float4* __restrict__ rays,
+                                                                   float* __restrict__ grad, RayDims rd,
+                                                                   ProjLaunch PL) {
+  const int cx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int cy = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int cz = blockIdx.z;
+  if (cx >= rd.in.x || cy >= rd.in.y) return;
+  const Dims3 d = rd.in;
+  const size_t plane = size_t(rd.out_w) * rd.out_h;
+  const float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz;
+  float acc = 0.f;
+  for (int vi = 0; vi < PL.num_views; ++vi) {
+    const ProjView& pv = PL.view[vi];
+    // pixel range of the projected support box (object-space corners c+0.5+-1)
+    float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
+    bool behind = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float ox = fcx + ((k & 1) ? 1.5f : -0.5f), oy = fcy + ((k & 2) ? 1.5f : -0.5f),
+                  oz = fcz + ((k & 4) ? 1.5f : -0.5f);
+      const float4 c = mat_vec(pv.pmv, ox, oy, oz, 1.f);
+      behind |= !(c.w > 0.f);
+      const float iw = 1.f / c.w;
+      const float px = (c.x * iw * 0.5f + 0.5f) * (float)rd.out_w, py = (c.y * iw * 0.5f + 0.5f) * (float)rd.out_h;
+      mnx = fminf(mnx, px);
+      mxx = fmaxf(mxx, px);
+      mny = fminf(mny, py);
+      mxy = fmaxf(mxy, py);
+    }
+    int xs = 0, xe = rd.out_w - 1, ys = 0, ye = rd.out_h - 1;
+    if (!behind) {
+      // the ray of pixel x passes at x + 0.5; small slack for rounding
+      xs = max(xs, (int)ceilf(mnx - 0.5f - 1e-3f));
+      xe = min(xe, (int)floorf(mxx - 0.5f + 1e-3f));
+      ys = max(ys, (int)ceilf(mny - 0.5f - 1e-3f));
+      ye = min(ye, (int)floorf(mxy - 0.5f + 1e-3f));
+    }
+    const float* gview = gs + (size_t)pv.out_slot * rd.out_d * plane;
+    const float4* rview = rays + 2 * (size_t)pv.out_slot * plane;
+    for (int py = ys; py <= ye; ++py) {
+      for (int px = xs; px <= xe; ++px) {
+        const size_t pix = size_t(py) * rd.out_w + px;
+        const float4 r0 = rview[2 * pix], r1 = rview[2 * pix + 1];
+        const int lo = (int)r0.w, hi = (int)r1.w;
+        // cut the ray by the support box in position coordinates (c-0.5, c+1.5)
+        const float nn[3] = {r0.x, r0.y, r0.z}, ss[3] = {r1.x, r1.y, r1.z}, cc[3] = {fcx, fcy, fcz};
+        float tlo = (float)lo, thi = (float)hi;
+        bool miss = lo > hi;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (ss[a] != 0.f) {
+            const float t0 = (cc[a] - 0.5f - nn[a]) / ss[a], t1 = (cc[a] + 1.5f - nn[a]) / ss[a];
+            tlo = fmaxf(tlo, fminf(t0, t1));
+            thi = fminf(thi, fmaxf(t0, t1));
+          } else {
+            miss |= !((cc[a] - 0.5f < nn[a]) & (nn[a] < cc[a] + 1.5f));
+          }
+        }
+        if (miss) continue;
+        const int i0 = max(lo, (int)floorf(tlo)), i1 = min(hi, (int)ceilf(thi));
+        for (int i = i0; i <= i1; ++i) {
+          const float fi = (float)i;
+          const float qx = fmaf(fi, r1.x, r0.x) - 0.5f, qy = fmaf(fi, r1.y, r0.y) - 0.5f,
+                      qz = fmaf(fi, r1.z, r0.z) - 0.5f;
+          // the walk stored only samples inside [-0.5, dims-0.5)
+          const float ux = qx + 0.5f, uy = qy + 0.5f, uz = qz + 0.5f;
+          if (!((0.f <= ux) & (ux < (float)d.x) & (0.f <= uy) & (uy < (float)d.y) & (0.f <= uz) &
+                (uz < (float)d.z)))
+            continue;
+          const float wx = 1.f - fabsf(qx - fcx), wy = 1.f - fabsf(qy - fcy), wz = 1.f - fabsf(qz - fcz);
+          if ((wx <= 0.f) | (wy <= 0.f) | (wz <= 0.f)) continue;
+          acc = fmaf(gview[(size_t)i * plane + pix], wx * wy * wz, acc);
+        }
+      }
+    }
+  }
+  float* g = grad + flat3(cx, cy, cz, d);
+  *g = PL.accumulate ? *g + acc : acc;
+}
+
+// ----------------------------------------- CLAMP: tiled LDS pre-reduction
+// A workgroup owns a 16x16 pixel tile and walks its rays in lock-step
+// segments of K samples from far to near. The samples of a segment lie in the
+// convex hull of the tile's four corner rays at the segment's first and last
+// sample (the index->object map is projective), so their trilinear corners fit
+// in an axis-aligned brick; the scatter is accumulated there with LDS atomics
+// and flushed once per segment with coalesced global atomics (zero entries
+// skipped). A brick over the LDS capacity, or a corner outside the brick
+// through rounding, goes straight to a global atomic.
+constexpr int kSegK = 8;
+constexpr int kCap = 7168;
+
+struct Brick {
+  int lo[3], n[3];
+  int size;
+};
+
+__device__ __forceinline__ Brick segment_brick(const float3* cn, const float3* cs, int ia, int ib, Dims3 d) {
+  float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const float fi = (float)(e ? ib : ia);
+      const float p[3] = {fmaf(fi, cs[c].x, cn[c].x) - 0.5f, fmaf(fi, cs[c].y, cn[c].y) - 0.5f,
+                          fmaf(fi, cs[c].z, cn[c].z) - 0.5f};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = fminf(mn[a], p[a]);
+        mx[a] = fmaxf(mx[a], p[a]);
+      }
+    }
+  }
+  const int dim[3] = {d.x, d.y, d.z};
+  Brick b;
+  b.size = 1;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    // every corner index is clamp(floor(p)) or clamp(floor(p)+1)
+    const int lo = clampi((int)floorf(mn[a] - 1e-3f), 0, dim[a] - 1);
+    const int hi = clampi((int)floorf(mx[a] + 1e-3f) + 1, 0, dim[a] - 1);
+    b.lo[a] = lo;
+    b.n[a] = hi - lo + 1;
+    b.size *= b.n[a];
+  }
+  return b;
+}
+
+template <bool L2>
+__global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float* __restrict__ dens,
+                                                                  const float* __restrict__ img,
+                                                                  const float* __restrict__ second,
+                                                                  float* __restrict__ grad, float* __restrict__ loss,
+                                                                  RayDims rd, RayLaunch L, RenderParams rp) {
+  __shared__ float lg[kCap];
+  const RayView& v = L.view[blockIdx.z];
+  const int x0 = blockIdx.x * kRTileX, y0 = blockIdx.y * kRTileY;
+  const int x = x0 + (threadIdx.x % kRTileX), y = y0 + (threadIdx.x / kRTileX);
+  const bool active = x < rd.out_w && y < rd.out_h;
+  const float* buf = dens + rd.in_batch_stride * v.in_batch;
+  float* gout = grad + rd.in_batch_stride * v.in_batch;
+  float3 cn[4], cs[4];
+  {
+    const int x1 = min(x0 + kRTileX - 1, rd.out_w - 1), y1 = min(y0 + kRTileY - 1, rd.out_h - 1);
+    const int px[4] = {x0, x1, x0, x1}, py[4] = {y0, y0, y1, y1};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pixel_ray(v, rd, px[c], py[c], cn[c], cs[c]);
+  }
+  float sq = 0.f, G = 0.f, gD = 0.f, D = 0.f;
+  float3 nearp = make_float3(0.f, 0.f, 0.f), step = nearp;
+  if (active) {
+    pixel_grad<L2>(img, second, rd, v, size_t(y) * rd.out_w + x, rp, G, gD, D, sq);
+    pixel_ray(v, rd, x, y, nearp, step);
+  }
+  const bool walk = active && (G != 0.f || gD != 0.f);
+  for (int i = threadIdx.x; i < kCap; i += 256) lg[i] = 0.f;
+  __syncthreads();
+  const int S = rd.out_d;
+  for (int sgi = (S + kSegK - 1) / kSegK - 1; sgi >= 0; --sgi) {
+    const int ia = sgi * kSegK, ib = min(ia + kSegK, S) - 1;
+    const Brick br = segment_brick(cn, cs, ia, ib, rd.in);
+    const bool use_lds = br.size <= kCap;
+    if (walk) {
+      for (int i = ib; i >= ia; --i) {
+        const float fi = (float)i;
+        const Taps tp = make_taps<B_CLAMP>(fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
+                                           fmaf(fi, step.z, nearp.z) - 0.5f, rd.in);
         float c[2][2][2];
 #pragma unroll
         for (int k = 0; k < 2; ++k)
 #pragma unroll
           for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int ii = 0; ii < 2; ++ii) {
-              float val = buf[flat3(tp.ix[ii], tp.iy[j], tp.iz[k], rd.in)];
-              c[k][j][ii] = (BM == B_BORDER && !(tp.vx[ii] & tp.vy[j] & tp.vz[k])) ? 0.f : val;
-            }
-        float v00 = lerpf(c[0][0][0], c[0][0][1], tp.wx), v01 = lerpf(c[0][1][0], c[0][1][1], tp.wx);
-        float v10 = lerpf(c[1][0][0], c[1][0][1], tp.wx), v11 = lerpf(c[1][1][0], c[1][1][1], tp.wx);
-        float sigma = rp.step * lerpf(lerpf(v00, v01, tp.wy), lerpf(v10, v11, tp.wy), tp.wz);
-        float e = expf(-D);
+            for (int ii = 0; ii < 2; ++ii) c[k][j][ii] = buf[flat3(tp.ix[ii], tp.iy[j], tp.iz[k], rd.in)];
+        const float v0 = lerpf(lerpf(c[0][0][0], c[0][0][1], tp.wx), lerpf(c[0][1][0], c[0][1][1], tp.wx), tp.wy);
+        const float v1 = lerpf(lerpf(c[1][0][0], c[1][0][1], tp.wx), lerpf(c[1][1][0], c[1][1][1], tp.wx), tp.wy);
+        const float sigma = rp.step * lerpf(v0, v1, tp.wz);
+        const float e = __expf(-D);
         D = fmaxf(D - sigma, 0.f);
-        float gd_sample = rp.step * (e * G + (gD - e * sigma * G));
-        gD = gD - e * sigma * G;
+        const float esg = e * sigma * G;
+        const float gds = rp.step * (e * G + (gD - esg));
+        gD -= esg;
         const float wx[2] = {1.f - tp.wx, tp.wx}, wy[2] = {1.f - tp.wy, tp.wy}, wz[2] = {1.f - tp.wz, tp.wz};
 #pragma unroll
         for (int k = 0; k < 2; ++k)
@@ -125,28 +416,107 @@ __global__ __launch_bounds__(256) void k_render_density_bwd(const float* __restr
           for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int ii = 0; ii < 2; ++ii) {
-              if (BM == B_BORDER && !(tp.vx[ii] & tp.vy[j] & tp.vz[k])) continue;
-              float w = wx[ii] * wy[j] * wz[k];
-              if (w != 0.f) atomic_addf(gout + flat3(tp.ix[ii], tp.iy[j], tp.iz[k], rd.in), gd_sample * w);
+              const float w = wx[ii] * wy[j] * wz[k];
+              if (w == 0.f) continue;
+              const int bx = tp.ix[ii] - br.lo[0], by = tp.iy[j] - br.lo[1], bz = tp.iz[k] - br.lo[2];
+              const bool in = use_lds & ((unsigned)bx < (unsigned)br.n[0]) & ((unsigned)by < (unsigned)br.n[1]) &
+                              ((unsigned)bz < (unsigned)br.n[2]);
+              if (in) atomicAdd(&lg[(bz * br.n[1] + by) * br.n[0] + bx], gds * w);
+              else atomic_addf(gout + flat3(tp.ix[ii], tp.iy[j], tp.iz[k], rd.in), gds * w);
+            }
+      }
+    }
+    if (use_lds) {
+      __syncthreads();
+      const int nx = br.n[0], ny = br.n[1];
+      const int q = 256 / nx, r = 256 - q * nx;
+      int bx = threadIdx.x % nx, t = threadIdx.x / nx;
+      int by = t % ny, bz = t / ny;
+      for (int idx = threadIdx.x; idx < br.size; idx += 256) {
+        const float val = lg[idx];
+        if (val != 0.f) {
+          atomic_addf(gout + flat3(br.lo[0] + bx, br.lo[1] + by, br.lo[2] + bz, rd.in), val);
+          lg[idx] = 0.f;
+        }
+        bx += r;
+        by += q;
+        if (bx >= nx) {
+          bx -= nx;
+          ++by;
+        }
+        while (by >= ny) {
+          by -= ny;
+          ++bz;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (L2) block_loss(sq, loss, rp.loss_scale);
+}
+
+// -------------------------------------------------- WRAP: global atomics
+template <bool L2>
+__global__ __launch_bounds__(256) void k_render_density_bwd_wrap(const float* __restrict__ dens,
+                                                                 const float* __restrict__ img,
+                                                                 const float* __restrict__ second,
+                                                                 float* __restrict__ grad, float* __restrict__ loss,
+                                                                 RayDims rd, RayLaunch L, RenderParams rp) {
+  const RayView& v = L.view[blockIdx.z];
+  const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
+  const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
+  float sq = 0.f;
+  if (x < rd.out_w && y < rd.out_h) {
+    float G, gD, D;
+    pixel_grad<L2>(img, second, rd, v, size_t(y) * rd.out_w + x, rp, G, gD, D, sq);
+    float3 nearp, step;
+    pixel_ray(v, rd, x, y, nearp, step);
+    const float* buf = dens + rd.in_batch_stride * v.in_batch;
+    float* gout = grad + rd.in_batch_stride * v.in_batch;
+    if (G != 0.f || gD != 0.f) {
+      for (int i = rd.out_d - 1; i >= 0; --i) {
+        const float fi = (float)i;
+        const Taps tp = make_taps<B_WRAP>(fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
+                                          fmaf(fi, step.z, nearp.z) - 0.5f, rd.in);
+        float c[2][2][2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) c[k][j][ii] = buf[flat3(tp.ix[ii], tp.iy[j], tp.iz[k], rd.in)];
+        const float v0 = lerpf(lerpf(c[0][0][0], c[0][0][1], tp.wx), lerpf(c[0][1][0], c[0][1][1], tp.wx), tp.wy);
+        const float v1 = lerpf(lerpf(c[1][0][0], c[1][0][1], tp.wx), lerpf(c[1][1][0], c[1][1][1], tp.wx), tp.wy);
+        const float sigma = rp.step * lerpf(v0, v1, tp.wz);
+        const float e = __expf(-D);
+        D = fmaxf(D - sigma, 0.f);
+        const float esg = e * sigma * G;
+        const float gds = rp.step * (e * G + (gD - esg));
+        gD -= esg;
+        const float wx[2] = {1.f - tp.wx, tp.wx}, wy[2] = {1.f - tp.wy, tp.wy}, wz[2] = {1.f - tp.wz, tp.wz};
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) {
+              const float w = wx[ii] * wy[j] * wz[k];
+              if (w != 0.f) atomic_addf(gout + flat3(tp.ix[ii], tp.iy[j], tp.iz[k], rd.in), gds * w);
             }
       }
     }
   }
-  if (!L2) return;
-  // block reduction of the squared error, one atomic per block
-  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float s = 0.f;
-    for (int w = 0; w < 256 / 64; ++w) s += red[w];
-    atomic_addf(loss, s * rp.loss_scale);
-  }
+  if (L2) block_loss(sq, loss, rp.loss_scale);
+}
+
+// ---------------------------------------------------------------- launches
+static dim3 tile_grid(const RayDims& rd, int views) {
+  return dim3((rd.out_w + kRTileX - 1) / kRTileX, (rd.out_h + kRTileY - 1) / kRTileY, views);
 }
 
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
                                      const RayLaunch& L, const RenderParams& rp) {
-  dim3 grid((rd.out_w + kRTileX - 1) / kRTileX, (rd.out_h + kRTileY - 1) / kRTileY, L.num_views);
+  const dim3 grid = tile_grid(rd, L.num_views);
   switch (bm) {
     case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp); break;
     case B_CLAMP: hipLaunchKernelGGL(k_render_density_fwd<B_CLAMP>, grid, dim3(256), 0, s, dens, img, rd, L, rp); break;
@@ -155,22 +525,35 @@ hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, f
   return hipGetLastError();
 }
 
-template <bool L2>
-static void launch_bwd(int bm, dim3 grid, hipStream_t s, const float* dens, const float* img, const float* target,
-                       float* grad, float* loss, const RayDims& rd, const RayLaunch& L, const RenderParams& rp) {
-  switch (bm) {
-    case B_BORDER: hipLaunchKernelGGL((k_render_density_bwd<B_BORDER, L2>), grid, dim3(256), 0, s, dens, img, target, grad, loss, rd, L, rp); break;
-    case B_CLAMP: hipLaunchKernelGGL((k_render_density_bwd<B_CLAMP, L2>), grid, dim3(256), 0, s, dens, img, target, grad, loss, rd, L, rp); break;
-    default: hipLaunchKernelGGL((k_render_density_bwd<B_WRAP, L2>), grid, dim3(256), 0, s, dens, img, target, grad, loss, rd, L, rp); break;
-  }
+// BORDER walk (writes gs, rays) for one chunk of views
+hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* dens, const float* img,
+                                      const float* second, float* gs, float4* rays, float* loss, const RayDims& rd,
+                                      const RayLaunch& L, const RenderParams& rp) {
+  const dim3 grid = tile_grid(rd, L.num_views);
+  if (l2) hipLaunchKernelGGL(k_render_density_bwd_walk<true>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp);
+  else hipLaunchKernelGGL(k_render_density_bwd_walk<false>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp);
+  return hipGetLastError();
 }
 
-hipError_t render_density_bwd_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
-                                     const float* target, float* grad, float* loss, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp) {
-  dim3 grid((rd.out_w + kRTileX - 1) / kRTileX, (rd.out_h + kRTileY - 1) / kRTileY, L.num_views);
-  if (l2) launch_bwd<true>(bm, grid, s, dens, img, target, grad, loss, rd, L, rp);
-  else launch_bwd<false>(bm, grid, s, dens, img, target, grad, loss, rd, L, rp);
+hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
+                                        const RayDims& rd, const ProjLaunch& PL) {
+  const dim3 grid((rd.in.x + 63) / 64, (rd.in.y + 3) / 4, rd.in.z);
+  hipLaunchKernelGGL(k_render_density_bwd_gather, grid, dim3(256), 0, s, gs, rays, grad, rd, PL);
+  return hipGetLastError();
+}
+
+// CLAMP / WRAP scatter backward (grad must be zero on entry)
+hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
+                                         const float* second, float* grad, float* loss, const RayDims& rd,
+                                         const RayLaunch& L, const RenderParams& rp) {
+  const dim3 grid = tile_grid(rd, L.num_views);
+  if (bm == B_CLAMP) {
+    if (l2) hipLaunchKernelGGL(k_render_density_bwd_clamp<true>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
+    else hipLaunchKernelGGL(k_render_density_bwd_clamp<false>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
+  } else {
+    if (l2) hipLaunchKernelGGL(k_render_density_bwd_wrap<true>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
+    else hipLaunchKernelGGL(k_render_density_bwd_wrap<false>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
+  }
   return hipGetLastError();
 }
 

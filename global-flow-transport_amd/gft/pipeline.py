@@ -135,10 +135,15 @@ class _RenderFn(torch.autograd.Function):
         cams, depth_step, light, boundary = ctx.cfg
         gd = torch.empty_like(d)
         lt = (ctypes.c_float * 3)(*[float(x) for x in light])
-        _lib.check(_lib.lib().gft_render_density_bwd(
-            _lib.stream_of(d), _lib.ptr(d), _lib.i64_array(d.shape), _np_ptr(cams.M), cams.nM, _np_ptr(cams.V),
-            _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt, float(depth_step),
-            _enum(BOUNDARY, boundary, "boundary"), _lib.ptr(img), _lib.ptr(g.contiguous()), _lib.ptr(gd)))
+        L = _lib.lib()
+        shape = _lib.i64_array(d.shape)
+        bm = _enum(BOUNDARY, boundary, "boundary")
+        nws = L.gft_render_density_bwd_workspace(shape, cams.nCam, cams.c_out, bm)
+        ws = torch.empty((max(int(nws), 1),), dtype=torch.uint8, device=d.device)
+        _lib.check(L.gft_render_density_bwd(
+            _lib.stream_of(d), _lib.ptr(d), shape, _np_ptr(cams.M), cams.nM, _np_ptr(cams.V),
+            _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt, float(depth_step), bm, _lib.ptr(img),
+            _lib.ptr(g.contiguous()), _lib.ptr(ws), nws, _lib.ptr(gd)))
         return gd, None, None, None, None
 
 
@@ -185,6 +190,8 @@ class TransportRenderStep(object):
         self._nws = L.gft_advect_bwd_workspace(self._shape)
         self.saved = torch.empty((self._nsaved,), dtype=torch.uint8, device=dev)
         self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
+        self._nrws = L.gft_render_density_bwd_workspace(self._shape, self.cams.nCam, self.cams.c_out, self.boundary)
+        self.render_ws = torch.empty((max(self._nrws, 1),), dtype=torch.uint8, device=dev)
 
     PHASES = ("advect_fwd", "render_fwd", "render_bwd", "advect_bwd")
 
@@ -213,8 +220,8 @@ class TransportRenderStep(object):
         _lib.check(L.gft_render_density_bwd_l2(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                                _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
                                                self.depth_step, self.boundary, _lib.ptr(self.images),
-                                               _lib.ptr(target), self.loss_scale, _lib.ptr(self.grad_d1),
-                                               _lib.ptr(self.loss)))
+                                               _lib.ptr(target), self.loss_scale, _lib.ptr(self.render_ws),
+                                               self._nrws, _lib.ptr(self.grad_d1), _lib.ptr(self.loss)))
         self._mark(timers, "render_bwd", False)
         self._mark(timers, "advect_bwd", True)
         _lib.check(L.gft_advect_bwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,

@@ -119,18 +119,25 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
  * Backward of gft_render_density_fwd with the L2 image loss folded in:
  * loss = loss_scale * sum (rgb - target)^2 (targets [1,V,H,W,3]); writes the
  * scalar loss (device float) and the exact density gradient (overwritten).
- * Replaces loss_dens_target (optimization.py:396-412) + the raymarch gradient.
+ * Replaces loss_dens_target (optimization.py:396-412) + the raymarch gradient
+ * (raymarch_grid.cu.cc:343-372). BORDER needs the workspace returned by
+ * gft_render_density_bwd_workspace (per-sample gradients of every view and
+ * the per-pixel rays) and is deterministic; CLAMP/WRAP need none.
  */
+size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
+                                        int32_t boundary);
 int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t shape[5], const float* M,
                               int32_t num_M, const float* V, const float* P, const float* frustum,
                               int32_t num_cameras, const int32_t output_shape[3], const float light[3],
                               float depth_step, int32_t boundary, const float* images, const float* targets,
-                              float loss_scale, float* density_grad, float* loss);
+                              float loss_scale, void* workspace, size_t workspace_bytes, float* density_grad,
+                              float* loss);
 /* Backward of gft_render_density_fwd from an image gradient [1,V,H,W,4]. */
 int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                           const float* images, const float* images_grad, float* density_grad);
+                           const float* images, const float* images_grad, void* workspace, size_t workspace_bytes,
+                           float* density_grad);
 
 #ifdef __cplusplus
 }

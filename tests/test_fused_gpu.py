@@ -73,7 +73,7 @@ def test_render_density_forward_backward(cuda):
     assert norm_err(td.grad.cpu().numpy(), gref) < 1e-4
 
 
-@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP"])
+@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP", "WRAP"])
 def test_fused_step_matches_oracle(cuda, boundary):
     from gft.pipeline import TransportRenderStep
     grid = (16, 26, 18)
@@ -123,3 +123,19 @@ def test_fused_step_repeatable_and_graph_capturable(cuda):
         assert abs(step.loss.item() - eager[0].item()) <= 1e-6 * abs(eager[0].item())
         assert norm_err(step.grad_density.cpu().numpy(), eager[1].cpu().numpy()) < 1e-5
         assert norm_err(step.grad_velocity.cpu().numpy(), eager[2].cpu().numpy()) < 1e-5
+
+
+def test_border_backward_is_deterministic(cuda):
+    from gft.pipeline import TransportRenderStep
+    grid = (16, 26, 18)
+    sc = scene(grid=grid, image=(24, 14), views=3, seed=8)
+    d, v = _fields(grid, seed=14)
+    tgt = torch.zeros((1, 3) + sc["out_dhw"][1:] + (3,), device="cuda")
+    step = TransportRenderStep(grid, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], boundary="BORDER")
+    td, tv = torch.from_numpy(d).cuda(), torch.from_numpy(v).cuda()
+    step.forward(td, tv)
+    step.backward(td, tv, tgt)
+    first = step.grad_d1.clone()
+    for _ in range(3):
+        step.backward(td, tv, tgt)
+        assert torch.equal(step.grad_d1, first)
