@@ -99,6 +99,63 @@ __device__ __forceinline__ bool read_border(const float* __restrict__ buf, Dims3
   return true;
 }
 
+// Branch-free BORDER read of U consecutive samples i0, i0+dir, ... of one ray
+// (dir = +1 or -1): all 8U corner loads are issued before any value is used,
+// so a lane keeps 8U loads in flight. Samples outside [-0.5, dims-0.5) or past
+// `last` (in walk direction) read 0 and report inside = false. Corner indices
+// are clamped into the grid for the address and masked for BORDER validity.
+template <int U>
+__device__ __forceinline__ void read_border_batch(const float* __restrict__ buf, Dims3 d, float3 n, float3 st,
+                                                  int i0, int dir, int last, float val[U], bool inside[U]) {
+  const unsigned sx = (unsigned)d.x, sxy = (unsigned)d.x * (unsigned)d.y;
+  unsigned a[U][8];
+  float w[U][3];
+  unsigned m[U];  // bit k: corner k valid
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int i = i0 + dir * u;
+    const float fi = (float)i;
+    const float px = fmaf(fi, st.x, n.x) - 0.5f, py = fmaf(fi, st.y, n.y) - 0.5f, pz = fmaf(fi, st.z, n.z) - 0.5f;
+    const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+    const bool in_range = dir > 0 ? i <= last : i >= last;
+    inside[u] = in_range & (0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) &
+                (qz < (float)d.z);
+    const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    w[u][0] = px - fx;
+    w[u][1] = py - fy;
+    w[u][2] = pz - fz;
+    const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    const bool vx0 = ix >= 0, vx1 = ix + 1 < d.x, vy0 = iy >= 0, vy1 = iy + 1 < d.y, vz0 = iz >= 0,
+               vz1 = iz + 1 < d.z;
+    const unsigned x0 = (unsigned)clampi(ix, 0, d.x - 1), x1 = (unsigned)clampi(ix + 1, 0, d.x - 1);
+    const unsigned y0 = (unsigned)clampi(iy, 0, d.y - 1), y1 = (unsigned)clampi(iy + 1, 0, d.y - 1);
+    const unsigned z0 = (unsigned)clampi(iz, 0, d.z - 1), z1 = (unsigned)clampi(iz + 1, 0, d.z - 1);
+    const unsigned r00 = z0 * sxy + y0 * sx, r01 = z0 * sxy + y1 * sx, r10 = z1 * sxy + y0 * sx,
+                   r11 = z1 * sxy + y1 * sx;
+    a[u][0] = r00 + x0; a[u][1] = r00 + x1; a[u][2] = r01 + x0; a[u][3] = r01 + x1;
+    a[u][4] = r10 + x0; a[u][5] = r10 + x1; a[u][6] = r11 + x0; a[u][7] = r11 + x1;
+    m[u] = (unsigned)(vx0 & vy0 & vz0) | ((unsigned)(vx1 & vy0 & vz0) << 1) | ((unsigned)(vx0 & vy1 & vz0) << 2) |
+           ((unsigned)(vx1 & vy1 & vz0) << 3) | ((unsigned)(vx0 & vy0 & vz1) << 4) |
+           ((unsigned)(vx1 & vy0 & vz1) << 5) | ((unsigned)(vx0 & vy1 & vz1) << 6) |
+           ((unsigned)(vx1 & vy1 & vz1) << 7);
+  }
+  float c[U][8];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[u][k] = buf[a[u][k]];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[u][k] = ((m[u] >> k) & 1u) ? c[u][k] : 0.f;
+    const float v0 = lerpf(lerpf(c[u][0], c[u][1], w[u][0]), lerpf(c[u][2], c[u][3], w[u][0]), w[u][1]);
+    const float v1 = lerpf(lerpf(c[u][4], c[u][5], w[u][0]), lerpf(c[u][6], c[u][7], w[u][0]), w[u][1]);
+    val[u] = inside[u] ? lerpf(v0, v1, w[u][2]) : 0.f;
+  }
+}
+
+constexpr int kBatch = 4;
+
 // ----------------------------------------------------------------- forward
 template <int BM>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
@@ -114,15 +171,17 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
   if (BM == B_BORDER) {
     int lo, hi;
     clip_ray(nearp, step, rd.in, rd.out_d, lo, hi);
-    for (int i = lo; i <= hi; ++i) {
-      const float fi = (float)i;
-      float val;
-      if (!read_border(buf, rd.in, fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
-                       fmaf(fi, step.z, nearp.z) - 0.5f, val))
-        continue;
-      const float sigma = rp.step * val;
-      D = fmaxf(D + sigma, 0.f);
-      E = fmaf(__expf(-D), sigma, E);
+    for (int i = lo; i <= hi; i += kBatch) {
+      float val[kBatch];
+      bool in[kBatch];
+      read_border_batch<kBatch>(buf, rd.in, nearp, step, i, 1, hi, val, in);
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        // samples outside the grid read 0 and leave D and E unchanged
+        const float sigma = rp.step * val[u];
+        D = in[u] ? fmaxf(D + sigma, 0.f) : D;
+        E = fmaf(__expf(-D), sigma, E);
+      }
     }
   } else {
     for (int i = 0; i < rd.out_d; ++i) {
@@ -204,19 +263,21 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
     r[0] = make_float4(nearp.x, nearp.y, nearp.z, (float)lo);
     r[1] = make_float4(step.x, step.y, step.z, (float)hi);
     const float* buf = dens + rd.in_batch_stride * v.in_batch;
-    float* g = gs + ((size_t)v.out_slot * rd.out_d + (size_t)max(hi, 0)) * plane + pix;
-    for (int i = hi; i >= lo; --i, g -= plane) {
-      const float fi = (float)i;
-      float val;
-      if (!read_border(buf, rd.in, fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
-                       fmaf(fi, step.z, nearp.z) - 0.5f, val))
-        continue;
-      const float sigma = rp.step * val;
-      const float e = __expf(-D);
-      D = fmaxf(D - sigma, 0.f);
-      const float esg = e * sigma * G;
-      *g = rp.step * (e * G + (gD - esg));
-      gD -= esg;
+    float* gview = gs + (size_t)v.out_slot * rd.out_d * plane + pix;
+    for (int i = hi; i >= lo; i -= kBatch) {
+      float val[kBatch];
+      bool in[kBatch];
+      read_border_batch<kBatch>(buf, rd.in, nearp, step, i, -1, lo, val, in);
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (!in[u]) continue;  // the sample read 0: nothing stored, D and g_D unchanged
+        const float sigma = rp.step * val[u];
+        const float e = __expf(-D);
+        D = fmaxf(D - sigma, 0.f);
+        const float esg = e * sigma * G;
+        gview[(size_t)(i - u) * plane] = rp.step * (e * G + (gD - esg));
+        gD -= esg;
+      }
     }
   }
   if (L2) block_loss(sq, loss, rp.loss_scale);
@@ -240,17 +301,21 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
   float acc = 0.f;
   for (int vi = 0; vi < PL.num_views; ++vi) {
     const ProjView& pv = PL.view[vi];
-    // pixel range of the projected support box (object-space corners c+0.5+-1)
+    const float* A = pv.pmv;
+    // clip coordinates of the support box corners c+0.5 +- 1 from the centre
+    // and the three column vectors (the map is linear before the divide)
+    const float4 cc = mat_vec(A, fcx + 0.5f, fcy + 0.5f, fcz + 0.5f, 1.f);
     float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
     bool behind = false;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float ox = fcx + ((k & 1) ? 1.5f : -0.5f), oy = fcy + ((k & 2) ? 1.5f : -0.5f),
-                  oz = fcz + ((k & 4) ? 1.5f : -0.5f);
-      const float4 c = mat_vec(pv.pmv, ox, oy, oz, 1.f);
-      behind |= !(c.w > 0.f);
-      const float iw = 1.f / c.w;
-      const float px = (c.x * iw * 0.5f + 0.5f) * (float)rd.out_w, py = (c.y * iw * 0.5f + 0.5f) * (float)rd.out_h;
+      const float sx = (k & 1) ? 1.f : -1.f, sy = (k & 2) ? 1.f : -1.f, sz = (k & 4) ? 1.f : -1.f;
+      const float cxk = fmaf(sz, A[8], fmaf(sy, A[4], fmaf(sx, A[0], cc.x)));
+      const float cyk = fmaf(sz, A[9], fmaf(sy, A[5], fmaf(sx, A[1], cc.y)));
+      const float cwk = fmaf(sz, A[11], fmaf(sy, A[7], fmaf(sx, A[3], cc.w)));
+      behind |= !(cwk > 0.f);
+      const float iw = __builtin_amdgcn_rcpf(cwk);
+      const float px = fmaf(cxk * iw, 0.5f, 0.5f) * (float)rd.out_w, py = fmaf(cyk * iw, 0.5f, 0.5f) * (float)rd.out_h;
       mnx = fminf(mnx, px);
       mxx = fmaxf(mxx, px);
       mny = fminf(mny, py);
@@ -258,11 +323,12 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
     }
     int xs = 0, xe = rd.out_w - 1, ys = 0, ye = rd.out_h - 1;
     if (!behind) {
-      // the ray of pixel x passes at x + 0.5; small slack for rounding
-      xs = max(xs, (int)ceilf(mnx - 0.5f - 1e-3f));
-      xe = min(xe, (int)floorf(mxx - 0.5f + 1e-3f));
-      ys = max(ys, (int)ceilf(mny - 0.5f - 1e-3f));
-      ye = min(ye, (int)floorf(mxy - 0.5f + 1e-3f));
+      // the ray of pixel x passes at x + 0.5; slack covers the approximate
+      // reciprocal and rounding
+      xs = max(xs, (int)ceilf(mnx - 0.5f - 1e-2f));
+      xe = min(xe, (int)floorf(mxx - 0.5f + 1e-2f));
+      ys = max(ys, (int)ceilf(mny - 0.5f - 1e-2f));
+      ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
     }
     const float* gview = gs + (size_t)pv.out_slot * rd.out_d * plane;
     const float4* rview = rays + 2 * (size_t)pv.out_slot * plane;
@@ -271,33 +337,28 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
         const size_t pix = size_t(py) * rd.out_w + px;
         const float4 r0 = rview[2 * pix], r1 = rview[2 * pix + 1];
         const int lo = (int)r0.w, hi = (int)r1.w;
-        // cut the ray by the support box in position coordinates (c-0.5, c+1.5)
-        const float nn[3] = {r0.x, r0.y, r0.z}, ss[3] = {r1.x, r1.y, r1.z}, cc[3] = {fcx, fcy, fcz};
-        float tlo = (float)lo, thi = (float)hi;
-        bool miss = lo > hi;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          if (ss[a] != 0.f) {
-            const float t0 = (cc[a] - 0.5f - nn[a]) / ss[a], t1 = (cc[a] + 1.5f - nn[a]) / ss[a];
-            tlo = fmaxf(tlo, fminf(t0, t1));
-            thi = fminf(thi, fmaxf(t0, t1));
-          } else {
-            miss |= !((cc[a] - 0.5f < nn[a]) & (nn[a] < cc[a] + 1.5f));
-          }
-        }
-        if (miss) continue;
-        const int i0 = max(lo, (int)floorf(tlo)), i1 = min(hi, (int)ceilf(thi));
+        if (lo > hi) continue;
+        // sample parameter range where the ray is inside the support box
+        // (c-0.5, c+1.5); an infinite reciprocal of a zero step component
+        // yields the unbounded or empty interval by itself
+        const float ax = (fcx - 0.5f - r0.x) * __builtin_amdgcn_rcpf(r1.x), bx = fmaf(2.f, __builtin_amdgcn_rcpf(r1.x), ax);
+        const float ay = (fcy - 0.5f - r0.y) * __builtin_amdgcn_rcpf(r1.y), by = fmaf(2.f, __builtin_amdgcn_rcpf(r1.y), ay);
+        const float az = (fcz - 0.5f - r0.z) * __builtin_amdgcn_rcpf(r1.z), bz = fmaf(2.f, __builtin_amdgcn_rcpf(r1.z), az);
+        const float tlo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+        const float thi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+        if (!(tlo <= thi + 1.f)) continue;
+        const int i0 = max(lo, (int)floorf(fmaxf(tlo, -1.f)) - 1), i1 = min(hi, (int)ceilf(fminf(thi, 1e9f)) + 1);
         for (int i = i0; i <= i1; ++i) {
           const float fi = (float)i;
           const float qx = fmaf(fi, r1.x, r0.x) - 0.5f, qy = fmaf(fi, r1.y, r0.y) - 0.5f,
                       qz = fmaf(fi, r1.z, r0.z) - 0.5f;
+          const float wx = 1.f - fabsf(qx - fcx), wy = 1.f - fabsf(qy - fcy), wz = 1.f - fabsf(qz - fcz);
+          if ((wx <= 0.f) | (wy <= 0.f) | (wz <= 0.f)) continue;
           // the walk stored only samples inside [-0.5, dims-0.5)
           const float ux = qx + 0.5f, uy = qy + 0.5f, uz = qz + 0.5f;
           if (!((0.f <= ux) & (ux < (float)d.x) & (0.f <= uy) & (uy < (float)d.y) & (0.f <= uz) &
                 (uz < (float)d.z)))
             continue;
-          const float wx = 1.f - fabsf(qx - fcx), wy = 1.f - fabsf(qy - fcy), wz = 1.f - fabsf(qz - fcz);
-          if ((wx <= 0.f) | (wy <= 0.f) | (wz <= 0.f)) continue;
           acc = fmaf(gview[(size_t)i * plane + pix], wx * wy * wz, acc);
         }
       }
