@@ -348,6 +348,59 @@ struct RayDims {
 };
 
 
+// ------------------------------------------------ resampling (v1 sampler)
+// Mip pyramid of a batch of grids. Level 0 is the grid itself; levels 1..L
+// live in an atlas, one block of levels per batch (MipAtlas3D,
+// resample_grid.cu.cc:104-236, without its pinned-host pointer table).
+constexpr int kMaxMips = 15;
+struct MipTable {
+  int levels;                    // levels above the base (0 = no mips)
+  int C;
+  Dims3 dims[kMaxMips + 1];      // W, H, D per level
+  size_t offset[kMaxMips + 1];   // float offset of level m >= 1 in a batch's atlas block
+  size_t atlas_stride;           // floats per batch in the atlas
+  size_t base_stride;            // floats per batch of the base grid
+};
+
+// Sampler settings of the resample/LuT ops (SamplerSettings, sampling.hpp:496-537).
+struct SampleCfg {
+  int fm;        // Filter
+  int mm;        // Mip
+  int levels;    // usable mip levels
+  float bias;    // mip bias
+  float offset;  // cell-centre offset subtracted before the tap
+};
+
+// One (batch, camera) view of a transform resample. Forward coordinate modes
+// (idx -> object space) use A = P^-1, B = (V M)^-1; reverse modes (object
+// space -> idx) use A = V M, B = P. p22/p32 are P's entries (col 2, row 2)
+// and (col 3, row 2) for the linear-depth remap.
+struct SampleView {
+  float A[16];
+  float B[16];
+  float p22, p32, nearp, farp;
+  int in_batch;
+  int out_slot;
+};
+constexpr int kMaxSampleViews = 8;
+struct SampleLaunch {
+  SampleView view[kMaxSampleViews];
+  int num_views;
+  int cm;          // Coord
+  int accumulate;  // sum-over-views launch: add to the existing output
+};
+
+// One SampleGridLut launch: lookup-table shape and flags; blockIdx.y walks
+// the (batch, table) pairs starting at batch0.
+struct LutLaunch {
+  Dims3 dout;     // lookup table / output cells
+  int num_cams;   // lookup tables
+  int global;     // separate_camera_batch
+  int relative;
+  int normalized;
+  int batch0;
+};
+
 // idx -> ndc -> view (divide by w) -> object space (transformations_v2.hpp:166-179)
 __device__ __forceinline__ float3 idx_to_os(const RayView& v, float ix, float iy, float iz, float dinvx, float dinvy,
                                             float dinvz) {

@@ -1,0 +1,691 @@
+// Frustum and lookup-table resampling on gfx950: SampleGridTransform and
+// SampleGridLut forward, their gradients (the reference's count-normalised
+// scatter and the exact adjoint), mip pyramids and the LoD transform.
+//
+// One thread per output cell, x fastest so that a wave writes 64 consecutive
+// cells; views of one launch are the grid's y dimension and their camera
+// records travel as kernel arguments (no __constant__ state, re-entrant).
+// Sampler semantics follow the v1 sampler of the reference
+// (phitest/render/cuda/src/sampling.hpp:158-252,356-491,539-764).
+#include "gft_device.hpp"
+
+namespace gft {
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ const float* level_ptr(const float* base, const float* atlas, const MipTable& mt, int b,
+                                                  int m) {
+  return m == 0 ? base + (size_t)b * mt.base_stride : atlas + (size_t)b * mt.atlas_stride + mt.offset[m];
+}
+__device__ __forceinline__ float* level_ptr_mut(float* base, float* atlas, const MipTable& mt, int b, int m) {
+  return m == 0 ? base + (size_t)b * mt.base_stride : atlas + (size_t)b * mt.atlas_stride + mt.offset[m];
+}
+
+__device__ __forceinline__ void unflat(size_t f, Dims3 d, int& x, int& y, int& z) {
+  x = (int)(f % (size_t)d.x);
+  size_t r = f / (size_t)d.x;
+  y = (int)(r % (size_t)d.y);
+  z = (int)(r / (size_t)d.y);
+}
+
+// v1 MIN/MAX over the 8 corners (read3DMin/read3DMax, sampling.hpp:372-491):
+// BORDER outside -> 0; if any corner is invalid the extremum starts at 0;
+// floor and ceil corners (ceil == floor at integers). arg receives the flat
+// index of the winning corner per channel (-1: the start value won).
+template <int C, int BM>
+__device__ __forceinline__ Vec<C> read_minmax_v1(const float* __restrict__ buf, Dims3 d, float px, float py, float pz,
+                                                 bool is_max, long long* arg) {
+  Taps t = make_taps<BM>(px, py, pz, d);
+  Vec<C> m;
+  if (arg)
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch) arg[ch] = -1;
+  if (BM == B_BORDER && !t.inside) return vzero<C>();
+  int ex[2] = {t.ix[0], t.wx == 0.f ? t.ix[0] : t.ix[1]};
+  int ey[2] = {t.iy[0], t.wy == 0.f ? t.iy[0] : t.iy[1]};
+  int ez[2] = {t.iz[0], t.wz == 0.f ? t.iz[0] : t.iz[1]};
+  bool evx[2] = {t.vx[0], t.wx == 0.f ? t.vx[0] : t.vx[1]};
+  bool evy[2] = {t.vy[0], t.wy == 0.f ? t.vy[0] : t.vy[1]};
+  bool evz[2] = {t.vz[0], t.wz == 0.f ? t.vz[0] : t.vz[1]};
+  float init = is_max ? -kFloatMax : kFloatMax;
+  if (BM == B_BORDER && !(evx[0] & evx[1] & evy[0] & evy[1] & evz[0] & evz[1])) init = 0.f;
+#pragma unroll
+  for (int ch = 0; ch < C; ++ch) m.v[ch] = init;
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        if (BM == B_BORDER && !(evx[i] & evy[j] & evz[k])) continue;
+        size_t f = flat3(ex[i], ey[j], ez[k], d);
+        Vec<C> v = vload<C>(buf, f);
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+          bool better = is_max ? (v.v[ch] > m.v[ch]) : (v.v[ch] < m.v[ch]);
+          if (better) {
+            m.v[ch] = v.v[ch];
+            if (arg) arg[ch] = (long long)f;
+          }
+        }
+      }
+  return m;
+}
+
+// One tap of the v1 sampler at a position with the offset removed.
+template <int C, int BM>
+__device__ __forceinline__ Vec<C> fetch_v1(const float* __restrict__ buf, Dims3 d, int fm, float px, float py,
+                                           float pz) {
+  if (fm == F_LINEAR) return read_linear<C, BM>(buf, d, px, py, pz);
+  if (fm == F_NEAREST) return read_nearest<C, BM>(buf, d, px, py, pz);
+  return read_minmax_v1<C, BM>(buf, d, px, py, pz, fm == F_MAX, nullptr);
+}
+
+// Mip level selection (sampling.hpp:551-571,603-620): returns the number of
+// taps (1 or 2) with their levels and weights.
+__device__ __forceinline__ int level_taps(const SampleCfg& s, float lod, int* lv, float* w) {
+  if (s.mm == MIP_NONE) {
+    lv[0] = 0;
+    w[0] = 1.f;
+    return 1;
+  }
+  float l = fminf(fmaxf(lod + s.bias, 0.f), (float)s.levels);
+  int fl = min((int)(l + (s.mm == MIP_NEAREST ? 0.5f : 0.f)), s.levels);
+  if (s.mm == MIP_LINEAR) {
+    int cl = min((int)ceilf(l), s.levels);
+    if (cl > fl) {
+      float fr = l - floorf(l);
+      lv[0] = fl;
+      w[0] = 1.f - fr;
+      lv[1] = cl;
+      w[1] = fr;
+      return 2;
+    }
+  }
+  lv[0] = fl;
+  w[0] = 1.f;
+  return 1;
+}
+
+template <int C, int BM>
+__device__ __forceinline__ Vec<C> sample_v1(const float* base, const float* atlas, const MipTable& mt, int b,
+                                            const SampleCfg& s, float3 p, float lod) {
+  int lv[2];
+  float w[2];
+  int n = level_taps(s, lod, lv, w);
+  float sc = 1.f / (float)(1 << lv[0]);
+  Vec<C> a = fetch_v1<C, BM>(level_ptr(base, atlas, mt, b, lv[0]), mt.dims[lv[0]], s.fm, p.x * sc - s.offset,
+                             p.y * sc - s.offset, p.z * sc - s.offset);
+  if (n == 2) {
+    float sc1 = 1.f / (float)(1 << lv[1]);
+    Vec<C> c = fetch_v1<C, BM>(level_ptr(base, atlas, mt, b, lv[1]), mt.dims[lv[1]], s.fm, p.x * sc1 - s.offset,
+                               p.y * sc1 - s.offset, p.z * sc1 - s.offset);
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch) a.v[ch] = lerpf(a.v[ch], c.v[ch], w[1]);
+  }
+  return a;
+}
+
+// Coordinate maps of samplePos3D (sampling.hpp:59-136) with the v1
+// transforms (transformations.hpp:86-145). q is an output index + 0.5.
+__device__ __forceinline__ float3 map_pos(const SampleView& v, int cm, float qx, float qy, float qz, float3 oinv,
+                                          float3 idims) {
+  if (cm == CM_TRANSFORM || cm == CM_LINDEPTH) {
+    float nx = qx * oinv.x * 2.f - 1.f, ny = qy * oinv.y * 2.f - 1.f, nz = qz * oinv.z * 2.f - 1.f;
+    if (cm == CM_LINDEPTH) {
+      float t = qz * oinv.z;
+      float z = -(v.nearp + (v.farp - v.nearp) * t);
+      nz = -v.p22 - v.p32 / z;
+    }
+    float4 vs = mat_vec(v.A, nx, ny, nz, 1.f);
+    float4 os = mat_vec(v.B, vs.x / vs.w, vs.y / vs.w, vs.z / vs.w, 1.f);
+    return make_float3(os.x, os.y, os.z);
+  }
+  float4 vs = mat_vec(v.A, qx, qy, qz, 1.f);
+  float4 ndc = mat_vec(v.B, vs.x, vs.y, vs.z, vs.w);
+  float ndx = ndc.x / ndc.w, ndy = ndc.y / ndc.w, ndz = ndc.z / ndc.w;
+  float3 r = make_float3((ndx * 0.5f + 0.5f) * idims.x, (ndy * 0.5f + 0.5f) * idims.y, (ndz * 0.5f + 0.5f) * idims.z);
+  if (cm == CM_LINDEPTH_REVERSE) {
+    float z = -v.p32 / (ndz + v.p22);
+    float t = -(z + v.nearp) / (v.farp - v.nearp);
+    r.z = t * idims.z;
+  }
+  return r;
+}
+
+__device__ __forceinline__ float len3f(float a, float b, float c) { return sqrtf(a * a + b * b + c * c); }
+
+// Position and LoD of one output cell (forward differences; Jacobian rows for
+// the forward modes, columns for the reverse modes).
+__device__ __forceinline__ float3 sample_pos(const SampleView& v, int cm, bool lod, float qx, float qy, float qz,
+                                             float3 oinv, float3 idims, float& lod_out) {
+  float3 p = map_pos(v, cm, qx, qy, qz, oinv, idims);
+  lod_out = 0.f;
+  if (lod) {
+    float3 a = map_pos(v, cm, qx + 1.f, qy, qz, oinv, idims);
+    float3 b = map_pos(v, cm, qx, qy + 1.f, qz, oinv, idims);
+    float3 c = map_pos(v, cm, qx, qy, qz + 1.f, oinv, idims);
+    float3 dx = make_float3(p.x - a.x, p.y - a.y, p.z - a.z);
+    float3 dy = make_float3(p.x - b.x, p.y - b.y, p.z - b.z);
+    float3 dz = make_float3(p.x - c.x, p.y - c.y, p.z - c.z);
+    float fx, fy, fz;
+    if (cm == CM_LINDEPTH || cm == CM_TRANSFORM) {
+      fx = len3f(dx.x, dy.x, dz.x);
+      fy = len3f(dx.y, dy.y, dz.y);
+      fz = len3f(dx.z, dy.z, dz.z);
+    } else {
+      fx = len3f(dx.x, dx.y, dx.z);
+      fy = len3f(dy.x, dy.y, dy.z);
+      fz = len3f(dz.x, dz.y, dz.z);
+    }
+    lod_out = fmaxf(0.f, log2f(fmaxf(fx, fmaxf(fy, fz))));
+  }
+  return p;
+}
+
+// ---------------------------------------------------------------- mips
+// kGenerateMip3D (resample_grid.cu.cc:46-72): pairwise means in z, then y,
+// then x. One thread per coarse voxel; blockIdx.y = batch.
+template <int C>
+__global__ __launch_bounds__(256) void k_mip_down(const float* __restrict__ src, size_t src_stride, Dims3 s,
+                                                  float* __restrict__ dst, size_t dst_stride, Dims3 o) {
+  size_t n = (size_t)o.x * o.y * o.z;
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const float* in = src + (size_t)blockIdx.y * src_stride;
+  float* out = dst + (size_t)blockIdx.y * dst_stride;
+  int x, y, z;
+  unflat(f, o, x, y, z);
+  Vec<C> ya[2];
+#pragma unroll
+  for (int xi = 0; xi < 2; ++xi) {
+    Vec<C> za[2];
+#pragma unroll
+    for (int yi = 0; yi < 2; ++yi) {
+      Vec<C> a = vload<C>(in, flat3(2 * x + xi, 2 * y + yi, 2 * z, s));
+      Vec<C> b = vload<C>(in, flat3(2 * x + xi, 2 * y + yi, 2 * z + 1, s));
+#pragma unroll
+      for (int ch = 0; ch < C; ++ch) za[yi].v[ch] = (a.v[ch] + b.v[ch]) * 0.5f;
+    }
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch) ya[xi].v[ch] = (za[0].v[ch] + za[1].v[ch]) * 0.5f;
+  }
+  Vec<C> r;
+#pragma unroll
+  for (int ch = 0; ch < C; ++ch) r.v[ch] = (ya[0].v[ch] + ya[1].v[ch]) * 0.5f;
+  vstore<C>(out, f, r);
+}
+
+// Adjoint of one mip step: every fine voxel under a coarse voxel receives 1/8
+// of its gradient (gather form, no atomics).
+template <int C>
+__global__ __launch_bounds__(256) void k_mip_collapse(const float* __restrict__ gparent, size_t parent_stride, Dims3 o,
+                                                      float* __restrict__ gchild, size_t child_stride, Dims3 s) {
+  size_t n = (size_t)s.x * s.y * s.z;
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int x, y, z;
+  unflat(f, s, x, y, z);
+  int px = x >> 1, py = y >> 1, pz = z >> 1;
+  if (px >= o.x || py >= o.y || pz >= o.z) return;
+  const float* gp = gparent + (size_t)blockIdx.y * parent_stride;
+  float* gc = gchild + (size_t)blockIdx.y * child_stride;
+  Vec<C> g = vload<C>(gp, flat3(px, py, pz, o));
+  Vec<C> c = vload<C>(gc, f);
+#pragma unroll
+  for (int ch = 0; ch < C; ++ch) c.v[ch] += 0.125f * g.v[ch];
+  vstore<C>(gc, f, c);
+}
+
+// ------------------------------------------------------ transform forward
+// kSample3D (resample_grid.cu.cc:240-252). blockIdx.y = view of the launch.
+// SUM: one thread loops over the launch's views and writes (or adds) the sum
+// into slot 0, the view-summed gather of the production sampling gradient.
+template <int C, int BM, bool SUM>
+__global__ __launch_bounds__(256) void k_sample_transform(const float* __restrict__ in,
+                                                          const float* __restrict__ atlas, MipTable mt, SampleCfg s,
+                                                          Dims3 dout, SampleLaunch L, float* __restrict__ out) {
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int x, y, z;
+  unflat(f, dout, x, y, z);
+  const float3 oinv = make_float3(1.f / dout.x, 1.f / dout.y, 1.f / dout.z);
+  const float3 idims = make_float3((float)mt.dims[0].x, (float)mt.dims[0].y, (float)mt.dims[0].z);
+  const bool lod = s.mm != MIP_NONE;
+  if constexpr (SUM) {
+    Vec<C> acc = vzero<C>();
+    for (int vi = 0; vi < L.num_views; ++vi) {
+      const SampleView& v = L.view[vi];
+      float l;
+      float3 p = sample_pos(v, L.cm, lod, x + 0.5f, y + 0.5f, z + 0.5f, oinv, idims, l);
+      Vec<C> r = sample_v1<C, BM>(in, atlas, mt, v.in_batch, s, p, l);
+#pragma unroll
+      for (int ch = 0; ch < C; ++ch) acc.v[ch] += r.v[ch];
+    }
+    if (L.accumulate) {
+      Vec<C> o = vload<C>(out, f);
+#pragma unroll
+      for (int ch = 0; ch < C; ++ch) acc.v[ch] += o.v[ch];
+    }
+    vstore<C>(out, f, acc);
+  } else {
+    const SampleView& v = L.view[blockIdx.y];
+    float l;
+    float3 p = sample_pos(v, L.cm, lod, x + 0.5f, y + 0.5f, z + 0.5f, oinv, idims, l);
+    Vec<C> r = sample_v1<C, BM>(in, atlas, mt, v.in_batch, s, p, l);
+    vstore<C>(out + (size_t)v.out_slot * n * C, f, r);
+  }
+}
+
+// ------------------------------------------------------ gradient scatters
+// Reference scatter (scatterGradInterpolated, resample_grid.cu.cc:761-812,
+// sampling_v2.hpp:459-486): BORDER accepts p in (-1, dims), floor and ceil
+// corners, each valid corner counted once per visit.
+template <int C, int BM>
+__device__ __forceinline__ void scatter_reference(float* __restrict__ grad, unsigned* __restrict__ cnt, Dims3 d,
+                                                  float px, float py, float pz, const Vec<C>& g) {
+  if (BM == B_BORDER) {
+    if (!((-1.f < px) & (px < (float)d.x) & (-1.f < py) & (py < (float)d.y) & (-1.f < pz) & (pz < (float)d.z))) return;
+  }
+  float fxf = floorf(px), fyf = floorf(py), fzf = floorf(pz);
+  float cwx = px - fxf, cwy = py - fyf, cwz = pz - fzf;
+  int fx = (int)fxf, fy = (int)fyf, fz = (int)fzf;
+  int cx = (int)ceilf(px), cy = (int)ceilf(py), cz = (int)ceilf(pz);
+  if (BM == B_CLAMP) {
+    fx = clampi(fx, 0, d.x - 1); cx = clampi(cx, 0, d.x - 1);
+    fy = clampi(fy, 0, d.y - 1); cy = clampi(cy, 0, d.y - 1);
+    fz = clampi(fz, 0, d.z - 1); cz = clampi(cz, 0, d.z - 1);
+  } else if (BM == B_WRAP) {
+    fx = pmodi(fx, d.x); cx = pmodi(cx, d.x);
+    fy = pmodi(fy, d.y); cy = pmodi(cy, d.y);
+    fz = pmodi(fz, d.z); cz = pmodi(cz, d.z);
+  }
+  int ix[2] = {fx, cx}, iy[2] = {fy, cy}, iz[2] = {fz, cz};
+  float wx[2] = {1.f - cwx, cwx}, wy[2] = {1.f - cwy, cwy}, wz[2] = {1.f - cwz, cwz};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        if (BM == B_BORDER &&
+            !((ix[i] >= 0) & (ix[i] < d.x) & (iy[j] >= 0) & (iy[j] < d.y) & (iz[k] >= 0) & (iz[k] < d.z)))
+          continue;
+        size_t f = flat3(ix[i], iy[j], iz[k], d);
+        vatomic_add<C>(grad, f, g, wx[i] * wy[j] * wz[k]);
+        if (cnt) atomicAdd(cnt + f, 1u);
+      }
+}
+
+// Exact adjoint of one v1 tap: trilinear with the forward read's domain and
+// corners, nearest, or the winning corner of MIN/MAX.
+template <int C, int BM>
+__device__ __forceinline__ void fetch_v1_adjoint(const float* __restrict__ buf, float* __restrict__ grad, Dims3 d,
+                                                 int fm, float px, float py, float pz, const Vec<C>& g, float scale) {
+  if (fm == F_LINEAR) {
+    Taps t = make_taps<BM>(px, py, pz, d);
+    if (BM == B_BORDER && !t.inside) return;
+    float wx[2] = {1.f - t.wx, t.wx}, wy[2] = {1.f - t.wy, t.wy}, wz[2] = {1.f - t.wz, t.wz};
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          if (BM == B_BORDER && !(t.vx[i] & t.vy[j] & t.vz[k])) continue;
+          float w = wx[i] * wy[j] * wz[k];
+          if (w == 0.f) continue;
+          vatomic_add<C>(grad, flat3(t.ix[i], t.iy[j], t.iz[k], d), g, w * scale);
+        }
+  } else if (fm == F_NEAREST) {
+    size_t f;
+    if (nearest_index<BM>(px, py, pz, d, f)) vatomic_add<C>(grad, f, g, scale);
+  } else {
+    long long arg[C];
+    (void)read_minmax_v1<C, BM>(buf, d, px, py, pz, fm == F_MAX, arg);
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch)
+      if (arg[ch] >= 0) atomic_addf(grad + (size_t)arg[ch] * C + ch, g.v[ch] * scale);
+  }
+}
+
+// SampleGridTransformGrad, reference flavour: scatter of every view of one
+// batch into its gradient and per-voxel counter (kScatter3DGradients,
+// resample_grid.cu.cc:841-861). grad/cnt point at the batch.
+template <int C, int BM>
+__global__ __launch_bounds__(256) void k_sample_transform_scatter_ref(const float* __restrict__ og, Dims3 din,
+                                                                      Dims3 dout, SampleLaunch L,
+                                                                      float* __restrict__ grad,
+                                                                      unsigned* __restrict__ cnt) {
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const SampleView& v = L.view[blockIdx.y];
+  int x, y, z;
+  unflat(f, dout, x, y, z);
+  const float3 oinv = make_float3(1.f / dout.x, 1.f / dout.y, 1.f / dout.z);
+  const float3 idims = make_float3((float)din.x, (float)din.y, (float)din.z);
+  float l;
+  float3 p = sample_pos(v, L.cm, false, x + 0.5f, y + 0.5f, z + 0.5f, oinv, idims, l);
+  Vec<C> g = vload<C>(og + (size_t)v.out_slot * n * C, f);
+  scatter_reference<C, BM>(grad, cnt, din, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, g);
+}
+
+// Exact adjoint of the transform forward, mips included: level m taps land
+// in the level-m gradient (level 0 = the input gradient).
+template <int C, int BM>
+__global__ __launch_bounds__(256) void k_sample_transform_adjoint(const float* __restrict__ in,
+                                                                  const float* __restrict__ atlas, MipTable mt,
+                                                                  SampleCfg s, Dims3 dout, SampleLaunch L,
+                                                                  const float* __restrict__ og,
+                                                                  float* __restrict__ grad,
+                                                                  float* __restrict__ gatlas) {
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const SampleView& v = L.view[blockIdx.y];
+  int x, y, z;
+  unflat(f, dout, x, y, z);
+  const float3 oinv = make_float3(1.f / dout.x, 1.f / dout.y, 1.f / dout.z);
+  const float3 idims = make_float3((float)mt.dims[0].x, (float)mt.dims[0].y, (float)mt.dims[0].z);
+  float l;
+  float3 p = sample_pos(v, L.cm, s.mm != MIP_NONE, x + 0.5f, y + 0.5f, z + 0.5f, oinv, idims, l);
+  Vec<C> g = vload<C>(og + (size_t)v.out_slot * n * C, f);
+  int lv[2];
+  float w[2];
+  int nt = level_taps(s, l, lv, w);
+  for (int t = 0; t < nt; ++t) {
+    int m = lv[t];
+    float sc = 1.f / (float)(1 << m);
+    fetch_v1_adjoint<C, BM>(level_ptr(in, atlas, mt, v.in_batch, m), level_ptr_mut(grad, gatlas, mt, v.in_batch, m),
+                            mt.dims[m], s.fm, p.x * sc - s.offset, p.y * sc - s.offset, p.z * sc - s.offset, g, w[t]);
+  }
+}
+
+// kNormalize3DGradients (resample_grid.cu.cc:814-832): divide by the count
+// where it is positive.
+template <int C>
+__global__ __launch_bounds__(256) void k_normalize_count(float* __restrict__ grad, const unsigned* __restrict__ cnt,
+                                                         size_t nvox) {
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nvox) return;
+  unsigned c = cnt[f];
+  if (c == 0) return;
+  Vec<C> g = vload<C>(grad, f);
+  float inv = 1.f / (float)c;
+#pragma unroll
+  for (int ch = 0; ch < C; ++ch) g.v[ch] = g.v[ch] * inv;
+  vstore<C>(grad, f, g);
+}
+
+// ------------------------------------------------------------------- LuT
+
+// kSampleGrid3DLuT (resample_grid.cu.cc:465-484): position = lut.xyz
+// (* dims if normalised) (+ idx if relative), lod = lut.w.
+__device__ __forceinline__ float3 lut_pos(const float* __restrict__ lut, const LutLaunch& L, int cam, size_t f, int x,
+                                          int y, int z, float& lod) {
+  size_t n = (size_t)L.dout.x * L.dout.y * L.dout.z;
+  float4 e = *reinterpret_cast<const float4*>(lut + ((size_t)cam * n + f) * 4);
+  float3 p = make_float3(e.x, e.y, e.z);
+  if (L.normalized) p = make_float3(p.x * L.dout.x, p.y * L.dout.y, p.z * L.dout.z);
+  if (L.relative) p = make_float3(p.x + x, p.y + y, p.z + z);
+  lod = e.w;
+  return p;
+}
+
+// blockIdx.y = (batch - batch0) * views_per_batch + view.
+template <int C, int BM>
+__global__ __launch_bounds__(256) void k_sample_lut(const float* __restrict__ in, const float* __restrict__ atlas,
+                                                    MipTable mt, SampleCfg s, LutLaunch L,
+                                                    const float* __restrict__ lut, float* __restrict__ out) {
+  const Dims3 dout = L.dout;
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int per = L.global ? L.num_cams : 1;
+  int b = L.batch0 + (int)blockIdx.y / per;
+  int cam = L.global ? (int)blockIdx.y % per : b;
+  int slot = b * per + (L.global ? cam : 0);
+  int x, y, z;
+  unflat(f, dout, x, y, z);
+  float lod;
+  float3 p = lut_pos(lut, L, cam, f, x, y, z, lod);
+  Vec<C> r = sample_v1<C, BM>(in, atlas, mt, b, s, p, lod);
+  vstore<C>(out + (size_t)slot * n * C, f, r);
+}
+
+// Spatial derivative of the trilinear read (read3DGrad, sampling.hpp:257-352).
+// EXACT = false: floor/ceil corners (zero derivative at integer positions);
+// EXACT = true: floor/floor+1, the right derivative. Same boundary rules.
+template <int C, int BM, bool EXACT>
+__device__ __forceinline__ void read_grad(const float* __restrict__ buf, Dims3 d, float px, float py, float pz,
+                                          Vec<C>& gx, Vec<C>& gy, Vec<C>& gz) {
+  gx = vzero<C>();
+  gy = vzero<C>();
+  gz = vzero<C>();
+  float fxf = floorf(px), fyf = floorf(py), fzf = floorf(pz);
+  float wx = px - fxf, wy = py - fyf, wz = pz - fzf;
+  int fx = (int)fxf, fy = (int)fyf, fz = (int)fzf;
+  int cx = EXACT ? fx + 1 : (int)ceilf(px), cy = EXACT ? fy + 1 : (int)ceilf(py), cz = EXACT ? fz + 1 : (int)ceilf(pz);
+  bool vx[2] = {true, true}, vy[2] = {true, true}, vz[2] = {true, true};
+  if (BM == B_BORDER) {
+    float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+    if (!((0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) & (qz < (float)d.z))) return;
+    vx[0] = (fx > -1) & (fx < d.x); vx[1] = (cx > -1) & (cx < d.x);
+    vy[0] = (fy > -1) & (fy < d.y); vy[1] = (cy > -1) & (cy < d.y);
+    vz[0] = (fz > -1) & (fz < d.z); vz[1] = (cz > -1) & (cz < d.z);
+    fx = clampi(fx, 0, d.x - 1); cx = clampi(cx, 0, d.x - 1);
+    fy = clampi(fy, 0, d.y - 1); cy = clampi(cy, 0, d.y - 1);
+    fz = clampi(fz, 0, d.z - 1); cz = clampi(cz, 0, d.z - 1);
+  } else if (BM == B_CLAMP) {
+    fx = clampi(fx, 0, d.x - 1); cx = clampi(cx, 0, d.x - 1);
+    fy = clampi(fy, 0, d.y - 1); cy = clampi(cy, 0, d.y - 1);
+    fz = clampi(fz, 0, d.z - 1); cz = clampi(cz, 0, d.z - 1);
+  } else {
+    fx = pmodi(fx, d.x); cx = pmodi(cx, d.x);
+    fy = pmodi(fy, d.y); cy = pmodi(cy, d.y);
+    fz = pmodi(fz, d.z); cz = pmodi(cz, d.z);
+  }
+  int ix[2] = {fx, cx}, iy[2] = {fy, cy}, iz[2] = {fz, cz};
+  Vec<C> v[2][2][2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        v[k][j][i] = vload<C>(buf, flat3(ix[i], iy[j], iz[k], d));
+        if (BM == B_BORDER && !(vx[i] & vy[j] & vz[k])) v[k][j][i] = vzero<C>();
+      }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    gx.v[c] = lerpf(lerpf(v[0][0][1].v[c] - v[0][0][0].v[c], v[0][1][1].v[c] - v[0][1][0].v[c], wy),
+                    lerpf(v[1][0][1].v[c] - v[1][0][0].v[c], v[1][1][1].v[c] - v[1][1][0].v[c], wy), wz);
+    gy.v[c] = lerpf(lerpf(v[0][1][0].v[c] - v[0][0][0].v[c], v[0][1][1].v[c] - v[0][0][1].v[c], wx),
+                    lerpf(v[1][1][0].v[c] - v[1][0][0].v[c], v[1][1][1].v[c] - v[1][0][1].v[c], wx), wz);
+    gz.v[c] = lerpf(lerpf(v[1][0][0].v[c] - v[0][0][0].v[c], v[1][0][1].v[c] - v[0][0][1].v[c], wx),
+                    lerpf(v[1][1][0].v[c] - v[0][1][0].v[c], v[1][1][1].v[c] - v[0][1][1].v[c], wx), wy);
+  }
+}
+
+// SampleGridLutGrad for the views of one data batch (blockIdx.y = view).
+// EXACT = false: kScatterLuT3DGradients (resample_grid.cu.cc:1051-1081): lut
+// gradient written per view (not scaled for normalised coordinates), input
+// gradient as a counted trilinear scatter whatever the filter.
+// EXACT = true: the lut gradient is added (one thread per lut cell, so no
+// atomics), scaled by the output dims for normalised coordinates; the input
+// gradient is the adjoint of the filter.
+template <int C, int BM, bool EXACT>
+__global__ __launch_bounds__(256) void k_sample_lut_bwd(const float* __restrict__ in_b, int fm, float offset,
+                                                        LutLaunch L, const float* __restrict__ lut,
+                                                        const float* __restrict__ og, Dims3 din,
+                                                        float* __restrict__ grad_b, unsigned* __restrict__ cnt,
+                                                        float* __restrict__ lut_grad) {
+  const Dims3 dout = L.dout;
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int per = L.global ? L.num_cams : 1;
+  int b = L.batch0;
+  int cam = L.global ? (int)blockIdx.y : b;
+  int slot = b * per + (L.global ? cam : 0);
+  int x, y, z;
+  unflat(f, dout, x, y, z);
+  float lod;
+  float3 p = lut_pos(lut, L, cam, f, x, y, z, lod);
+  Vec<C> g = vload<C>(og + (size_t)slot * n * C, f);
+  float px = p.x - offset, py = p.y - offset, pz = p.z - offset;
+  if (fm == F_LINEAR) {
+    Vec<C> gx, gy, gz;
+    read_grad<C, BM, EXACT>(in_b, din, px, py, pz, gx, gy, gz);
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      lx += g.v[c] * gx.v[c];
+      ly += g.v[c] * gy.v[c];
+      lz += g.v[c] * gz.v[c];
+    }
+    float4* dst = reinterpret_cast<float4*>(lut_grad + ((size_t)cam * n + f) * 4);
+    if (EXACT) {
+      if (L.normalized) {
+        lx *= dout.x;
+        ly *= dout.y;
+        lz *= dout.z;
+      }
+      float4 o = *dst;
+      *dst = make_float4(o.x + lx, o.y + ly, o.z + lz, o.w);
+    } else {
+      *dst = make_float4(lx, ly, lz, 0.f);
+    }
+  }
+  if (EXACT) fetch_v1_adjoint<C, BM>(in_b, grad_b, din, fm, px, py, pz, g, 1.f);
+  else scatter_reference<C, BM>(grad_b, cnt, din, px, py, pz, g);
+}
+
+// ------------------------------------------------------------------- LoD
+// kComputeLoD3D (resample_grid.cu.cc:1288-1300): position and LoD of every
+// output cell (central differences, Jacobian rows). blockIdx.y = camera.
+__global__ __launch_bounds__(256) void k_lod_transform(Dims3 din, Dims3 dout, SampleLaunch L,
+                                                       float* __restrict__ out) {
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const SampleView& v = L.view[blockIdx.y];
+  int x, y, z;
+  unflat(f, dout, x, y, z);
+  const float3 oinv = make_float3(1.f / dout.x, 1.f / dout.y, 1.f / dout.z);
+  const float3 idims = make_float3((float)din.x, (float)din.y, (float)din.z);
+  float qx = x + 0.5f, qy = y + 0.5f, qz = z + 0.5f;
+  float3 p = map_pos(v, L.cm, qx, qy, qz, oinv, idims);
+  float3 xm = map_pos(v, L.cm, qx - 1.f, qy, qz, oinv, idims), xp = map_pos(v, L.cm, qx + 1.f, qy, qz, oinv, idims);
+  float3 ym = map_pos(v, L.cm, qx, qy - 1.f, qz, oinv, idims), yp = map_pos(v, L.cm, qx, qy + 1.f, qz, oinv, idims);
+  float3 zm = map_pos(v, L.cm, qx, qy, qz - 1.f, oinv, idims), zp = map_pos(v, L.cm, qx, qy, qz + 1.f, oinv, idims);
+  float3 dx = make_float3((xm.x - xp.x) * 0.5f, (xm.y - xp.y) * 0.5f, (xm.z - xp.z) * 0.5f);
+  float3 dy = make_float3((ym.x - yp.x) * 0.5f, (ym.y - yp.y) * 0.5f, (ym.z - yp.z) * 0.5f);
+  float3 dz = make_float3((zm.x - zp.x) * 0.5f, (zm.y - zp.y) * 0.5f, (zm.z - zp.z) * 0.5f);
+  float fx = len3f(dx.x, dy.x, dz.x), fy = len3f(dx.y, dy.y, dz.y), fz = len3f(dx.z, dy.z, dz.z);
+  float lod = fmaxf(0.f, log2f(fmaxf(fx, fmaxf(fy, fz))));
+  *reinterpret_cast<float4*>(out + ((size_t)v.out_slot * n + f) * 4) = make_float4(p.x, p.y, p.z, lod);
+}
+
+// --------------------------------------------------------------- launchers
+static inline dim3 grid_for(size_t n, unsigned views) { return dim3((unsigned)((n + 255) / 256), views); }
+
+#define GFT_DISPATCH_C(C, ...)                     \
+  switch (C) {                                     \
+    case 1: { constexpr int CC = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int CC = 2; __VA_ARGS__; } break; \
+    default: { constexpr int CC = 4; __VA_ARGS__; } break; \
+  }
+#define GFT_DISPATCH_BM(bm, ...)                                 \
+  switch (bm) {                                                  \
+    case B_BORDER: { constexpr int BB = B_BORDER; __VA_ARGS__; } break; \
+    case B_CLAMP: { constexpr int BB = B_CLAMP; __VA_ARGS__; } break;   \
+    default: { constexpr int BB = B_WRAP; __VA_ARGS__; } break;         \
+  }
+
+hipError_t mip_down_launch(int C, hipStream_t st, const float* src, size_t src_stride, Dims3 s, float* dst,
+                           size_t dst_stride, Dims3 o, int batches) {
+  size_t n = (size_t)o.x * o.y * o.z;
+  GFT_DISPATCH_C(C, hipLaunchKernelGGL(k_mip_down<CC>, grid_for(n, batches), dim3(256), 0, st, src, src_stride, s,
+                                       dst, dst_stride, o));
+  return hipGetLastError();
+}
+
+hipError_t mip_collapse_launch(int C, hipStream_t st, const float* gp, size_t gp_stride, Dims3 o, float* gc,
+                               size_t gc_stride, Dims3 s, int batches) {
+  size_t n = (size_t)s.x * s.y * s.z;
+  GFT_DISPATCH_C(C, hipLaunchKernelGGL(k_mip_collapse<CC>, grid_for(n, batches), dim3(256), 0, st, gp, gp_stride, o,
+                                       gc, gc_stride, s));
+  return hipGetLastError();
+}
+
+hipError_t sample_transform_launch(int C, int bm, bool sum, hipStream_t st, const float* in, const float* atlas,
+                                   const MipTable& mt, const SampleCfg& s, Dims3 dout, const SampleLaunch& L,
+                                   float* out) {
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  GFT_DISPATCH_C(C, GFT_DISPATCH_BM(bm, {
+    if (sum)
+      hipLaunchKernelGGL((k_sample_transform<CC, BB, true>), grid_for(n, 1), dim3(256), 0, st, in, atlas, mt, s, dout,
+                         L, out);
+    else
+      hipLaunchKernelGGL((k_sample_transform<CC, BB, false>), grid_for(n, L.num_views), dim3(256), 0, st, in, atlas,
+                         mt, s, dout, L, out);
+  }));
+  return hipGetLastError();
+}
+
+hipError_t sample_transform_scatter_ref_launch(int C, int bm, hipStream_t st, const float* og, Dims3 din, Dims3 dout,
+                                               const SampleLaunch& L, float* grad, unsigned* cnt) {
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  GFT_DISPATCH_C(C, GFT_DISPATCH_BM(bm, hipLaunchKernelGGL((k_sample_transform_scatter_ref<CC, BB>),
+                                                           grid_for(n, L.num_views), dim3(256), 0, st, og, din, dout,
+                                                           L, grad, cnt)));
+  return hipGetLastError();
+}
+
+hipError_t sample_transform_adjoint_launch(int C, int bm, hipStream_t st, const float* in, const float* atlas,
+                                           const MipTable& mt, const SampleCfg& s, Dims3 dout, const SampleLaunch& L,
+                                           const float* og, float* grad, float* gatlas) {
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  GFT_DISPATCH_C(C, GFT_DISPATCH_BM(bm, hipLaunchKernelGGL((k_sample_transform_adjoint<CC, BB>),
+                                                           grid_for(n, L.num_views), dim3(256), 0, st, in, atlas, mt,
+                                                           s, dout, L, og, grad, gatlas)));
+  return hipGetLastError();
+}
+
+hipError_t normalize_count_launch(int C, hipStream_t st, float* grad, const unsigned* cnt, size_t nvox) {
+  GFT_DISPATCH_C(C, hipLaunchKernelGGL(k_normalize_count<CC>, grid_for(nvox, 1), dim3(256), 0, st, grad, cnt, nvox));
+  return hipGetLastError();
+}
+
+hipError_t sample_lut_launch(int C, int bm, hipStream_t st, const float* in, const float* atlas, const MipTable& mt,
+                             const SampleCfg& s, const LutLaunch& L, int pairs, const float* lut, float* out) {
+  size_t n = (size_t)L.dout.x * L.dout.y * L.dout.z;
+  GFT_DISPATCH_C(C, GFT_DISPATCH_BM(bm, hipLaunchKernelGGL((k_sample_lut<CC, BB>), grid_for(n, pairs), dim3(256), 0,
+                                                           st, in, atlas, mt, s, L, lut, out)));
+  return hipGetLastError();
+}
+
+hipError_t sample_lut_bwd_launch(int C, int bm, bool exact, hipStream_t st, const float* in_b, int fm, float offset,
+                                 const LutLaunch& L, int views, const float* lut, const float* og, Dims3 din,
+                                 float* grad_b, unsigned* cnt, float* lut_grad) {
+  size_t n = (size_t)L.dout.x * L.dout.y * L.dout.z;
+  GFT_DISPATCH_C(C, GFT_DISPATCH_BM(bm, {
+    if (exact)
+      hipLaunchKernelGGL((k_sample_lut_bwd<CC, BB, true>), grid_for(n, views), dim3(256), 0, st, in_b, fm, offset, L,
+                         lut, og, din, grad_b, cnt, lut_grad);
+    else
+      hipLaunchKernelGGL((k_sample_lut_bwd<CC, BB, false>), grid_for(n, views), dim3(256), 0, st, in_b, fm, offset, L,
+                         lut, og, din, grad_b, cnt, lut_grad);
+  }));
+  return hipGetLastError();
+}
+
+hipError_t lod_transform_launch(hipStream_t st, Dims3 din, Dims3 dout, const SampleLaunch& L, float* out) {
+  size_t n = (size_t)dout.x * dout.y * dout.z;
+  hipLaunchKernelGGL(k_lod_transform, grid_for(n, L.num_views), dim3(256), 0, st, din, dout, L, out);
+  return hipGetLastError();
+}
+
+}  // namespace gft

@@ -60,6 +60,26 @@ _SIGNATURES = {
                                                  _i32, _vp, _vp, _f32, _vp, _sz, _vp, _vp]),
     "gft_render_density_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
                                               _i32, _vp, _vp, _vp, _sz, _vp]),
+    "gft_mip_atlas_bytes": (_sz, [_i64p, _i32]),
+    "gft_build_mips": (ctypes.c_int, [_vp, _vp, _i64p, _i32, _vp, _sz]),
+    "gft_sample_grid_transform_workspace": (_sz, [_i64p, _i32, _i32]),
+    "gft_sample_grid_transform_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _i32, _i32,
+                                                     _i32, _i32, _f32, _i32, _f32, _i32, _vp, _sz, _vp]),
+    "gft_sample_grid_transform_bwd_workspace": (_sz, [_i64p, _i32, _i32, _i32]),
+    "gft_sample_grid_transform_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _i32,
+                                                     _i32, _i32, _i32, _f32, _i32, _f32, _i32, _i32, _vp, _sz, _vp]),
+    "gft_sample_grid_transform_gather_grad_workspace": (_sz, [_i64p, _i32, _i32]),
+    "gft_sample_grid_transform_gather_grad": (ctypes.c_int, [_vp, _vp, _i64p, _i32p, _vp, _i32, _vp, _vp, _vp, _i32,
+                                                             _i32, _i32, _i32, _i32, _f32, _i32, _vp, _sz, _vp]),
+    "gft_sample_grid_lut_workspace": (_sz, [_i64p, _i32, _i32]),
+    "gft_sample_grid_lut_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i64p, _i32, _i32, _i32, _i32, _f32, _f32, _i32,
+                                               _i32, _i32, _vp, _sz, _vp]),
+    "gft_sample_grid_lut_bwd_workspace": (_sz, [_i64p, _i32]),
+    "gft_sample_grid_lut_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i64p, _vp, _i32, _i32, _i32, _f32, _i32, _i32,
+                                               _i32, _i32, _vp, _sz, _vp, _vp]),
+    "gft_lod_transform": (ctypes.c_int, [_vp, _i32p, _vp, _vp, _vp, _i32, _i32p, _i32, _vp]),
+    "gft_reduce_grid_blend_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _i32, _i32, _vp]),
+    "gft_reduce_grid_blend_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _i32, _i32, _i32, _vp]),
 }
 
 

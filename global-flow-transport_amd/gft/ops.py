@@ -199,3 +199,384 @@ class _RaymarchingOps(object):
 
 raymarching_ops = _RaymarchingOps()
 raymarch_grid_transform = raymarching_ops.raymarch_grid_transform
+
+
+# ------------------------------------------------------------------- sampling
+def _workspace(nbytes, device):
+    return torch.empty((max(int(nbytes), 1),), dtype=torch.uint8, device=device)
+
+
+def _check_transform_attrs(interpolation, boundary, coordinate_mode, cell_center_offset, mipmapping, num_mipmaps):
+    fm = _enum(FILTER, interpolation, "interpolation")
+    if fm not in (FILTER["NEAREST"], FILTER["LINEAR"]):
+        raise _lib.InvalidArgumentError("interpolation must be NEAREST or LINEAR")
+    bm = _enum(BOUNDARY, boundary, "boundary")
+    if bm not in (BOUNDARY["BORDER"], BOUNDARY["CLAMP"]):
+        raise _lib.InvalidArgumentError("boundary must be BORDER or CLAMP")
+    cm = _enum(COORD, coordinate_mode, "coordinate_mode")
+    if float(cell_center_offset) != 0.5:
+        raise _lib.InvalidArgumentError("Invalid cell center offset for given coordinate mode.")
+    mm = _enum(MIP, mipmapping, "mipmapping")
+    if mm != MIP["NONE"] and int(num_mipmaps) <= 0:
+        raise _lib.InvalidArgumentError("when using mipmaps num_mipmaps must be larger than 0.")
+    return fm, bm, cm, mm
+
+
+def _sample_transform_fwd_impl(input, cams, out_dhw, fm, bm, mm, num_mipmaps, mip_bias, cm, sep):
+    M, nM, V, P, F, nCam = cams
+    N, D, H, W, C = input.shape
+    if nM != N:
+        raise _lib.InvalidArgumentError("model matrix batch size mismatch")
+    if not sep and nCam != N:
+        raise _lib.InvalidArgumentError("camera batch must match data batch when not using global sampling.")
+    nv = nCam if sep else 1
+    out = torch.empty((N, nv) + tuple(out_dhw) + (C,), device=input.device, dtype=torch.float32)
+    L = _lib.lib()
+    shape = _lib.i64_array(input.shape)
+    wsb = L.gft_sample_grid_transform_workspace(shape, mm, int(num_mipmaps))
+    ws = _workspace(wsb, input.device)
+    _lib.check(L.gft_sample_grid_transform_fwd(
+        _lib.stream_of(input), _lib.ptr(input), shape, _np_ptr(M), nM, _np_ptr(V), _np_ptr(P), _np_ptr(F), nCam,
+        _lib.i32_array(out_dhw), fm, bm, mm, int(num_mipmaps), float(mip_bias), cm, 0.5, int(bool(sep)),
+        _lib.ptr(ws), wsb, _lib.ptr(out)))
+    return out
+
+
+def _sample_transform_bwd_impl(input, output_grad, cams, fm, bm, mm, num_mipmaps, mip_bias, cm, sep, gm):
+    M, nM, V, P, F, nCam = cams
+    output_grad = output_grad.contiguous()
+    N = input.shape[0]
+    if output_grad.dim() != 6 or output_grad.shape[5] > 4:
+        raise _lib.InvalidArgumentError("Invalid output_grads shape (NVDHWC): %s" % list(output_grad.shape))
+    if output_grad.shape[0] != N:
+        raise _lib.InvalidArgumentError("output_grads batch size does not match input batch size.")
+    if not sep and output_grad.shape[1] != 1:
+        raise _lib.InvalidArgumentError("output_grads views size must be 1 if not using global sampling.")
+    if sep and output_grad.shape[1] != nCam:
+        raise _lib.InvalidArgumentError("output_grads views size does not match cameras.")
+    L = _lib.lib()
+    shape = _lib.i64_array(input.shape)
+    wsb = L.gft_sample_grid_transform_bwd_workspace(shape, mm, int(num_mipmaps), gm)
+    ws = _workspace(wsb, input.device)
+    grad = torch.empty_like(input)
+    _lib.check(L.gft_sample_grid_transform_bwd(
+        _lib.stream_of(input), _lib.ptr(input), shape, _lib.ptr(output_grad), _np_ptr(M), nM, _np_ptr(V), _np_ptr(P),
+        _np_ptr(F), nCam, _lib.i32_array(output_grad.shape[2:5]), fm, bm, mm, int(num_mipmaps), float(mip_bias), cm,
+        0.5, int(bool(sep)), gm, _lib.ptr(ws), wsb, _lib.ptr(grad)))
+    return grad
+
+
+def _gather_grad_impl(output_grad, cams, in_dhw, fm, bm, mm, num_mipmaps, mip_bias, cm):
+    M, nM, V, P, F, nCam = cams
+    output_grad = output_grad.contiguous()
+    N, Vc = output_grad.shape[0], output_grad.shape[1]
+    C = output_grad.shape[5]
+    L = _lib.lib()
+    gshape = _lib.i64_array(output_grad.shape)
+    wsb = L.gft_sample_grid_transform_gather_grad_workspace(gshape, mm, int(num_mipmaps))
+    ws = _workspace(wsb, output_grad.device)
+    grad = torch.empty((N,) + tuple(in_dhw) + (C,), device=output_grad.device, dtype=torch.float32)
+    _lib.check(L.gft_sample_grid_transform_gather_grad(
+        _lib.stream_of(output_grad), _lib.ptr(output_grad), gshape, _lib.i32_array(in_dhw), _np_ptr(M), nM,
+        _np_ptr(V), _np_ptr(P), _np_ptr(F), nCam, fm, bm, mm, int(num_mipmaps), float(mip_bias), cm, _lib.ptr(ws),
+        wsb, _lib.ptr(grad)))
+    return grad
+
+
+class _SampleTransformFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, input, cams, out_dhw, attrs):
+        fm, bm, mm, nmip, bias, cm, sep, gm, gather_bias_add = attrs
+        out = _sample_transform_fwd_impl(input, cams, out_dhw, fm, bm, mm, nmip, bias, cm, sep)
+        ctx.save_for_backward(input)
+        ctx.cams, ctx.attrs = cams, attrs
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (input,) = ctx.saved_tensors
+        fm, bm, mm, nmip, bias, cm, sep, gm, gather_bias_add = ctx.attrs
+        if gm == "GATHER":
+            g = _gather_grad_impl(grad_out, ctx.cams, tuple(input.shape[1:4]), fm, bm, mm, nmip,
+                                  bias + gather_bias_add, cm)
+        else:
+            g = _sample_transform_bwd_impl(input, grad_out, ctx.cams, fm, bm, mm, nmip, bias, cm, sep, gm)
+        return g, None, None, None
+
+
+def _lut_check(input, lookup, cell_center_offset, separate_camera_batch, normalized_coords):
+    lookup = _grid(lookup, "lookup")
+    if lookup.shape[4] != 4:
+        raise _lib.InvalidArgumentError("Invalid lut shape (NDHWC) with C=4: %s" % list(lookup.shape))
+    if normalized_coords and float(cell_center_offset) != 0.5:
+        raise _lib.InvalidArgumentError("Invalid cell center offset must be 0.5 when using normalized coordinates.")
+    if not separate_camera_batch and lookup.shape[0] != input.shape[0]:
+        raise _lib.InvalidArgumentError("camera batch must match data batch when not using global sampling.")
+    return lookup
+
+
+def _sample_lut_fwd_impl(input, lookup, fm, bm, mm, num_mipmaps, mip_bias, offset, sep, rel, norm):
+    N, C = input.shape[0], input.shape[4]
+    nv = lookup.shape[0] if sep else 1
+    out = torch.empty((N, nv) + tuple(lookup.shape[1:4]) + (C,), device=input.device, dtype=torch.float32)
+    L = _lib.lib()
+    shape = _lib.i64_array(input.shape)
+    wsb = L.gft_sample_grid_lut_workspace(shape, mm, int(num_mipmaps))
+    ws = _workspace(wsb, input.device)
+    _lib.check(L.gft_sample_grid_lut_fwd(
+        _lib.stream_of(input), _lib.ptr(input), shape, _lib.ptr(lookup), _lib.i64_array(lookup.shape), fm, bm, mm,
+        int(num_mipmaps), float(mip_bias), float(offset), int(bool(sep)), int(bool(rel)), int(bool(norm)),
+        _lib.ptr(ws), wsb, _lib.ptr(out)))
+    return out
+
+
+def _sample_lut_bwd_impl(input, lookup, output_grad, fm, bm, mm, offset, sep, rel, norm, gm):
+    output_grad = output_grad.contiguous()
+    N = input.shape[0]
+    if output_grad.dim() != 6 or output_grad.shape[0] != N:
+        raise _lib.InvalidArgumentError("Invalid output_grads shape (NVDHWC): %s" % list(output_grad.shape))
+    if tuple(output_grad.shape[2:5]) != tuple(lookup.shape[1:4]):
+        raise _lib.InvalidArgumentError("Invalid lut shape (VDHWC): %s, DHW must macht output shape: %s"
+                                        % (list(lookup.shape), list(output_grad.shape)))
+    if not sep and output_grad.shape[1] != 1:
+        raise _lib.InvalidArgumentError("output_grads views size must be 1 if not using global sampling.")
+    if sep and output_grad.shape[1] != lookup.shape[0]:
+        raise _lib.InvalidArgumentError("output_grads views size does not match cameras.")
+    if mm != MIP["NONE"]:
+        raise _lib.InvalidArgumentError("lut gradient supports mipmapping NONE only")
+    L = _lib.lib()
+    shape = _lib.i64_array(input.shape)
+    wsb = L.gft_sample_grid_lut_bwd_workspace(shape, gm)
+    ws = _workspace(wsb, input.device)
+    gin = torch.empty_like(input)
+    glut = torch.empty_like(lookup)
+    _lib.check(L.gft_sample_grid_lut_bwd(
+        _lib.stream_of(input), _lib.ptr(input), shape, _lib.ptr(lookup), _lib.i64_array(lookup.shape),
+        _lib.ptr(output_grad), fm, bm, mm, float(offset), int(bool(sep)), int(bool(rel)), int(bool(norm)), gm,
+        _lib.ptr(ws), wsb, _lib.ptr(gin), _lib.ptr(glut)))
+    return gin, glut
+
+
+class _SampleLutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, input, lookup, attrs):
+        fm, bm, mm, nmip, bias, offset, sep, rel, norm, gm = attrs
+        out = _sample_lut_fwd_impl(input, lookup, fm, bm, mm, nmip, bias, offset, sep, rel, norm)
+        ctx.save_for_backward(input, lookup)
+        ctx.attrs = attrs
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        input, lookup = ctx.saved_tensors
+        fm, bm, mm, nmip, bias, offset, sep, rel, norm, gm = ctx.attrs
+        gin, glut = _sample_lut_bwd_impl(input, lookup, grad_out, fm, bm, mm, offset, sep, rel, norm, gm)
+        return gin, glut, None
+
+
+class _SamplingOps(object):
+    """``sampling_ops`` of ops_loader.py:11-51 (resample_grid.cc:23-154)."""
+
+    @staticmethod
+    def sample_grid_transform(input, matrix_m, matrix_v, matrix_p, frustum_params, output_shape,
+                              interpolation="LINEAR", boundary="BORDER", mipmapping="NONE", num_mipmaps=0,
+                              mip_bias=0.0, coordinate_mode="TRANSFORM_LINDEPTH", cell_center_offset=0.5,
+                              separate_camera_batch=True, grad_mode="REFERENCE", gradient_mip_bias_add=0.0):
+        """SampleGridTransform: NDHWC -> [N,V,Do,Ho,Wo,C] ([N,1,...] without
+        separate_camera_batch). grad_mode "REFERENCE" registers the op's own
+        gradient (ops_loader.py:14-31), "EXACT" the true adjoint, "GATHER"
+        the renderer's view-summed reverse resample (renderer.py:246-272) with
+        mip_bias + gradient_mip_bias_add."""
+        input = _grid(input)
+        fm, bm, cm, mm = _check_transform_attrs(interpolation, boundary, coordinate_mode, cell_center_offset,
+                                                mipmapping, num_mipmaps)
+        cams = _cams(matrix_m, matrix_v, matrix_p, frustum_params)
+        gm = "GATHER" if str(grad_mode).upper() == "GATHER" else _enum(GRAD, grad_mode, "grad_mode")
+        attrs = (fm, bm, mm, int(num_mipmaps), float(mip_bias), cm, bool(separate_camera_batch), gm,
+                 float(gradient_mip_bias_add))
+        return _SampleTransformFn.apply(input, cams, _shape3(output_shape), attrs)
+
+    @staticmethod
+    def sample_grid_transform_grad(input, output_grad, matrix_m, matrix_v, matrix_p, frustum_params,
+                                   interpolation="LINEAR", boundary="BORDER", mipmapping="NONE", num_mipmaps=0,
+                                   mip_bias=0.0, coordinate_mode="TRANSFORM_LINDEPTH", cell_center_offset=0.5,
+                                   separate_camera_batch=True, grad_mode="REFERENCE"):
+        """SampleGridTransformGrad (resample_grid.cc:64-84)."""
+        input = _grid(input)
+        fm, bm, cm, mm = _check_transform_attrs(interpolation, boundary, coordinate_mode, cell_center_offset,
+                                                mipmapping, num_mipmaps)
+        gm = _enum(GRAD, grad_mode, "grad_mode")
+        if gm == GRAD["REFERENCE"] and mm != MIP["NONE"]:
+            raise _lib.InvalidArgumentError("the REFERENCE gradient supports mipmapping NONE only")
+        return _sample_transform_bwd_impl(input, output_grad.detach(), _cams(matrix_m, matrix_v, matrix_p,
+                                                                               frustum_params),
+                                          fm, bm, mm, num_mipmaps, mip_bias, cm, bool(separate_camera_batch), gm)
+
+    @staticmethod
+    def sample_grid_transform_gather_grad(output_grad, matrix_m, matrix_v, matrix_p, frustum_params, input_shape,
+                                          interpolation="LINEAR", boundary="BORDER", mipmapping="NONE",
+                                          num_mipmaps=0, mip_bias=0.0, coordinate_mode="TRANSFORM_LINDEPTH"):
+        """The renderer's gather gradient (renderer.py:246-272) in one fused
+        pass: dx[i] = sum_j resample(dy[i, j]) with the reverse coordinate
+        mode of ``coordinate_mode``; ``mip_bias`` includes the gradient add."""
+        fm, bm, cm, mm = _check_transform_attrs(interpolation, boundary, coordinate_mode, 0.5, mipmapping,
+                                                num_mipmaps)
+        og = output_grad.detach().contiguous()
+        if og.dim() != 6:
+            raise _lib.InvalidArgumentError("Invalid output_grads shape (NVDHWC): %s" % list(og.shape))
+        return _gather_grad_impl(og, _cams(matrix_m, matrix_v, matrix_p, frustum_params), _shape3(input_shape), fm,
+                                 bm, mm, num_mipmaps, mip_bias, cm)
+
+    @staticmethod
+    def sample_grid_lut(input, lookup, interpolation="LINEAR", boundary="BORDER", mipmapping="NONE", num_mipmaps=0,
+                        mip_bias=0.0, coordinate_mode="LOOKUP", cell_center_offset=0.0, separate_camera_batch=True,
+                        relative_coords=True, normalized_coords=False, grad_mode="REFERENCE"):
+        """SampleGridLut (resample_grid.cc:86-109): NDHWC sampled at a lookup
+        table [V,Do,Ho,Wo,4] -> [N,V,Do,Ho,Wo,C]."""
+        input = _grid(input)
+        if str(coordinate_mode).upper() != "LOOKUP":
+            raise _lib.InvalidArgumentError("Invalid coorindate_mode")
+        lookup = _lut_check(input, lookup, cell_center_offset, separate_camera_batch, normalized_coords)
+        mm = _enum(MIP, mipmapping, "mipmapping")
+        if mm != MIP["NONE"] and int(num_mipmaps) <= 0:
+            raise _lib.InvalidArgumentError("when using mipmaps num_mipmaps must be larger than 0.")
+        attrs = (_enum(FILTER, interpolation, "interpolation"), _enum(BOUNDARY, boundary, "boundary"), mm,
+                 int(num_mipmaps), float(mip_bias), float(cell_center_offset), bool(separate_camera_batch),
+                 bool(relative_coords), bool(normalized_coords), _enum(GRAD, grad_mode, "grad_mode"))
+        return _SampleLutFn.apply(input, lookup, attrs)
+
+    @staticmethod
+    def sample_grid_lut_grad(input, output_grad, lookup, interpolation="LINEAR", boundary="BORDER",
+                             mipmapping="NONE", num_mipmaps=0, mip_bias=0.0, coordinate_mode="LOOKUP",
+                             cell_center_offset=0.0, separate_camera_batch=True, relative_coords=True,
+                             normalized_coords=False, grad_mode="REFERENCE"):
+        """SampleGridLutGrad (resample_grid.cc:113-134): (input_grad, lookup_grad)."""
+        input = _grid(input)
+        lookup = _lut_check(input, lookup, cell_center_offset, separate_camera_batch, normalized_coords)
+        return _sample_lut_bwd_impl(input, lookup, output_grad.detach(), _enum(FILTER, interpolation, "interpolation"),
+                                    _enum(BOUNDARY, boundary, "boundary"), _enum(MIP, mipmapping, "mipmapping"),
+                                    float(cell_center_offset), bool(separate_camera_batch), bool(relative_coords),
+                                    bool(normalized_coords), _enum(GRAD, grad_mode, "grad_mode"))
+
+    @staticmethod
+    def lod_transform(input_shape, matrix_mv, matrix_p, frustum_params, output_shape, inverse_transform=False,
+                      device="cuda"):
+        """LodTransform (resample_grid.cc:136-154): [V,Do,Ho,Wo,4] sampling
+        positions (x,y,z) and LoD per output cell."""
+        MV, nMV = _matrices(matrix_mv, "matrix_mv")
+        P, nP = _matrices(matrix_p, "matrix_p")
+        F, nF = _frustum(frustum_params)
+        if not (nMV == nP == nF):
+            raise _lib.InvalidArgumentError("camera batch size mismatch")
+        in_dhw = _shape3(input_shape)
+        out_dhw = _shape3(output_shape)
+        out = torch.empty((nMV,) + tuple(out_dhw) + (4,), device=device, dtype=torch.float32)
+        L = _lib.lib()
+        _lib.check(L.gft_lod_transform(_lib.stream_of(out), _lib.i32_array(in_dhw), _np_ptr(MV), _np_ptr(P),
+                                       _np_ptr(F), nMV, _lib.i32_array(out_dhw), int(bool(inverse_transform)),
+                                       _lib.ptr(out)))
+        return out
+
+    @staticmethod
+    def build_mips(input, num_mipmaps):
+        """Mip pyramid (kGenerateMip3D): list of the levels 1..L as
+        [N,D>>m,H>>m,W>>m,C] tensors (views into one atlas)."""
+        input = _grid(input)
+        L = _lib.lib()
+        shape = _lib.i64_array(input.shape)
+        nb = L.gft_mip_atlas_bytes(shape, int(num_mipmaps))
+        atlas = torch.empty((max(int(nb) // 4, 1),), device=input.device, dtype=torch.float32)
+        _lib.check(L.gft_build_mips(_lib.stream_of(input), _lib.ptr(input), shape, int(num_mipmaps), _lib.ptr(atlas),
+                                    nb))
+        N, D, H, W, C = input.shape
+        levels, off, dims = [], 0, []
+        m = 1
+        while m <= int(num_mipmaps) and (D >> m) > 0 and (H >> m) > 0 and (W >> m) > 0 and m <= 15:
+            dims.append((D >> m, H >> m, W >> m))
+            m += 1
+        per = sum(d * h * w * C for d, h, w in dims)
+        stride = (per + 63) // 64 * 64
+        for d, h, w in dims:
+            n = d * h * w * C
+            lv = torch.as_strided(atlas, (N, d, h, w, C), (stride, h * w * C, w * C, C, 1), off)
+            levels.append(lv)
+            off += n
+        return levels
+
+
+sampling_ops = _SamplingOps()
+
+
+# ------------------------------------------------------------------- blending
+def _blend_check(input, blending_mode):
+    input = _grid(input)
+    if input.shape[4] > 4:
+        raise _lib.InvalidArgumentError("Invalid input shape %s" % list(input.shape))
+    return input, _enum(BLEND, blending_mode, "blending mode")
+
+
+def _reduce_blend_fwd_impl(input, bl, keep_dims):
+    N, D, H, W, C = input.shape
+    out = torch.empty(input.shape if keep_dims else (N, H, W, C), device=input.device, dtype=torch.float32)
+    L = _lib.lib()
+    _lib.check(L.gft_reduce_grid_blend_fwd(_lib.stream_of(input), _lib.ptr(input), _lib.i64_array(input.shape), bl,
+                                           int(bool(keep_dims)), _lib.ptr(out)))
+    return out
+
+
+def _reduce_blend_bwd_impl(output_grad, output, input, bl, keep_dims, gm):
+    output_grad = output_grad.contiguous()
+    output = output.contiguous()
+    if tuple(output_grad.shape) != tuple(output.shape):
+        raise _lib.InvalidArgumentError("Mismatch between output gradients and output shape: %s %s"
+                                        % (list(output_grad.shape), list(output.shape)))
+    N, D, H, W, C = input.shape
+    expect = tuple(input.shape) if keep_dims else (N, H, W, C)
+    if tuple(output.shape) != expect:
+        raise _lib.InvalidArgumentError("output gradients shape does not match input shape %s" % list(input.shape))
+    grad = torch.empty_like(input)
+    L = _lib.lib()
+    _lib.check(L.gft_reduce_grid_blend_bwd(_lib.stream_of(input), _lib.ptr(input), _lib.i64_array(input.shape),
+                                           _lib.ptr(output), _lib.ptr(output_grad), bl, int(bool(keep_dims)), gm,
+                                           _lib.ptr(grad)))
+    return grad
+
+
+class _ReduceBlendFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, input, bl, keep_dims, gm):
+        out = _reduce_blend_fwd_impl(input, bl, keep_dims)
+        ctx.save_for_backward(input, out)
+        ctx.attrs = (bl, keep_dims, gm)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        input, out = ctx.saved_tensors
+        bl, keep_dims, gm = ctx.attrs
+        return _reduce_blend_bwd_impl(grad_out, out, input, bl, keep_dims, gm), None, None, None
+
+
+class _BlendingOps(object):
+    """``blending_ops`` of ops_loader.py:53-60 (reduce_blend.cc:23-48)."""
+
+    @staticmethod
+    def reduce_grid_blend(input, blending_mode="BEER_LAMBERT", keep_dims=False, grad_mode="REFERENCE"):
+        """ReduceGridBlend: blend NDHWC along D -> [N,H,W,C] (every prefix
+        [N,D,H,W,C] with keep_dims)."""
+        input, bl = _blend_check(input, blending_mode)
+        return _ReduceBlendFn.apply(input, bl, bool(keep_dims), _enum(GRAD, grad_mode, "grad_mode"))
+
+    @staticmethod
+    def reduce_grid_blend_grad(output_grad, output, input, blending_mode="BEER_LAMBERT", keep_dims=False,
+                               grad_mode="REFERENCE"):
+        """ReduceGridBlendGrad (input order of reduce_blend.cc:37-47)."""
+        input, bl = _blend_check(input, blending_mode)
+        return _reduce_blend_bwd_impl(output_grad.detach(), output.detach(), input, bl, bool(keep_dims),
+                                      _enum(GRAD, grad_mode, "grad_mode"))
+
+
+blending_ops = _BlendingOps()
+sample_grid_transform = sampling_ops.sample_grid_transform
+sample_grid_lut = sampling_ops.sample_grid_lut
+reduce_grid_blend = blending_ops.reduce_grid_blend

@@ -139,6 +139,127 @@ int gft_render_density_bwd(void* stream, const float* density, const int64_t sha
                            const float* images, const float* images_grad, void* workspace, size_t workspace_bytes,
                            float* density_grad);
 
+/*
+ * Mip pyramid of a grid batch [N,D,H,W,C]: levels 1..L (L = num_mipmaps,
+ * reduced while a level would have an empty dimension) of pairwise means,
+ * level m dims = base dims >> m. Layout: per batch a 256-byte aligned block
+ * holding levels 1..L back to back, each NDHWC. Replaces MipAtlas3D and
+ * kGenerateMip3D (resample_grid.cu.cc:46-236, setupMipAtlas
+ * resample_grid.cc:201-220) without the per-call pinned pointer table.
+ */
+size_t gft_mip_atlas_bytes(const int64_t shape[5], int32_t num_mipmaps);
+int gft_build_mips(void* stream, const float* input, const int64_t shape[5], int32_t num_mipmaps, float* atlas,
+                   size_t atlas_bytes);
+
+/*
+ * SampleGridTransform forward. Replaces SampleGridTransformOp::Compute
+ * (resample_grid.cc:268-447) and SampleGridKernelLauncher / kSample3D
+ * (resample_grid.cu.cc:240-438).
+ *   input  [N,D,H,W,C]; output [N,V',Do,Ho,Wo,C], V' = num_cameras if
+ *   separate_camera_batch else 1 (camera i samples batch i).
+ *   interpolation NEAREST|LINEAR, boundary BORDER|CLAMP, coordinate_mode one
+ *   of the four transform modes, cell_center_offset must be 0.5.
+ *   The workspace (gft_sample_grid_transform_workspace) holds the mip atlas.
+ */
+size_t gft_sample_grid_transform_workspace(const int64_t input_shape[5], int32_t mipmapping, int32_t num_mipmaps);
+int gft_sample_grid_transform_fwd(void* stream, const float* input, const int64_t input_shape[5], const float* M,
+                                  int32_t num_M, const float* V, const float* P, const float* frustum,
+                                  int32_t num_cameras, const int32_t output_shape[3], int32_t interpolation,
+                                  int32_t boundary, int32_t mipmapping, int32_t num_mipmaps, float mip_bias,
+                                  int32_t coordinate_mode, float cell_center_offset, int32_t separate_camera_batch,
+                                  void* workspace, size_t workspace_bytes, float* output);
+
+/*
+ * SampleGridTransformGrad. Replaces SampleGridTransformGradOp::Compute
+ * (resample_grid.cc:469-625) and SampleGridGradKernelLauncher
+ * (resample_grid.cu.cc:761-1018). grad_mode REFERENCE: trilinear scatter
+ * with a per-voxel counter and division by the count per data batch
+ * (mipmapping must be NONE, as the reference op registers); EXACT: the true
+ * adjoint of the forward, mip levels included. input_grad is overwritten.
+ */
+size_t gft_sample_grid_transform_bwd_workspace(const int64_t input_shape[5], int32_t mipmapping, int32_t num_mipmaps,
+                                               int32_t grad_mode);
+int gft_sample_grid_transform_bwd(void* stream, const float* input, const int64_t input_shape[5],
+                                  const float* output_grad, const float* M, int32_t num_M, const float* V,
+                                  const float* P, const float* frustum, int32_t num_cameras,
+                                  const int32_t output_shape[3], int32_t interpolation, int32_t boundary,
+                                  int32_t mipmapping, int32_t num_mipmaps, float mip_bias, int32_t coordinate_mode,
+                                  float cell_center_offset, int32_t separate_camera_batch, int32_t grad_mode,
+                                  void* workspace, size_t workspace_bytes, float* input_grad);
+
+/*
+ * Gather gradient of the renderer's sampling: replaces the custom gradient
+ * of Renderer._sample_transform / _sample_camera_transform
+ * (renderer.py:200-227, 246-272), i.e. per data batch i
+ *   input_grad[i] = sum_j sample_grid_transform(output_grad[i, j], M[i],
+ *                   camera j, reverse coordinate mode, mips on output_grad)
+ * fused into one pass with the sum over views. output_grad
+ * [N,V,Do,Ho,Wo,C]; input_grad [N,D,H,W,C] with DHW = input_dhw;
+ * coordinate_mode is the forward op's mode; mip_bias already includes the
+ * gradient bias add.
+ */
+size_t gft_sample_grid_transform_gather_grad_workspace(const int64_t output_grad_shape[6], int32_t mipmapping,
+                                                       int32_t num_mipmaps);
+int gft_sample_grid_transform_gather_grad(void* stream, const float* output_grad, const int64_t output_grad_shape[6],
+                                          const int32_t input_dhw[3], const float* M, int32_t num_M, const float* V,
+                                          const float* P, const float* frustum, int32_t num_cameras,
+                                          int32_t interpolation, int32_t boundary, int32_t mipmapping,
+                                          int32_t num_mipmaps, float mip_bias, int32_t coordinate_mode,
+                                          void* workspace, size_t workspace_bytes, float* input_grad);
+
+/*
+ * SampleGridLut forward. Replaces SampleGridLutOp::Compute
+ * (resample_grid.cc:650-802) and SampleGridLuTKernelLauncher /
+ * kSampleGrid3DLuT (resample_grid.cu.cc:465-660).
+ *   lookup [V,Do,Ho,Wo,4] = (x, y, z, lod); position = xyz (* out dims if
+ *   normalized) (+ cell index if relative). interpolation NEAREST|LINEAR|
+ *   MIN|MAX, boundary BORDER|CLAMP|WRAP. output [N,V',Do,Ho,Wo,C].
+ */
+size_t gft_sample_grid_lut_workspace(const int64_t input_shape[5], int32_t mipmapping, int32_t num_mipmaps);
+int gft_sample_grid_lut_fwd(void* stream, const float* input, const int64_t input_shape[5], const float* lookup,
+                            const int64_t lookup_shape[5], int32_t interpolation, int32_t boundary, int32_t mipmapping,
+                            int32_t num_mipmaps, float mip_bias, float cell_center_offset,
+                            int32_t separate_camera_batch, int32_t relative_coords, int32_t normalized_coords,
+                            void* workspace, size_t workspace_bytes, float* output);
+
+/*
+ * SampleGridLutGrad. Replaces SampleGridLutGradOp::Compute
+ * (resample_grid.cc:828-979) and SampleGridLuTGradKernelLauncher /
+ * kScatterLuT3DGradients (resample_grid.cu.cc:1051-1251). REFERENCE: lookup
+ * gradient (LINEAR only) written per view, input gradient a counted
+ * trilinear scatter for every filter; EXACT: lookup gradient summed over
+ * batches (scaled for normalised coordinates), input gradient the adjoint of
+ * the filter. mipmapping must be NONE. Both outputs are overwritten.
+ */
+size_t gft_sample_grid_lut_bwd_workspace(const int64_t input_shape[5], int32_t grad_mode);
+int gft_sample_grid_lut_bwd(void* stream, const float* input, const int64_t input_shape[5], const float* lookup,
+                            const int64_t lookup_shape[5], const float* output_grad, int32_t interpolation,
+                            int32_t boundary, int32_t mipmapping, float cell_center_offset,
+                            int32_t separate_camera_batch, int32_t relative_coords, int32_t normalized_coords,
+                            int32_t grad_mode, void* workspace, size_t workspace_bytes, float* input_grad,
+                            float* lookup_grad);
+
+/*
+ * LodTransform: per camera the sampling position and LoD of every output
+ * cell, output [num_cameras,Do,Ho,Wo,4]. Replaces LodTransformOp::Compute
+ * (resample_grid.cc:986-1105) and kComputeLoD3D (resample_grid.cu.cc:1288-1300).
+ */
+int gft_lod_transform(void* stream, const int32_t input_dhw[3], const float* MV, const float* P, const float* frustum,
+                      int32_t num_cameras, const int32_t output_shape[3], int32_t inverse_transform, float* output);
+
+/*
+ * ReduceGridBlend / ReduceGridBlendGrad: blend along depth, output
+ * [N,H,W,C] or every prefix [N,D,H,W,C] with keep_dims. Replace
+ * ReduceGridBlendOp / ReduceGridBlendGradOp (reduce_blend.cc:60-242) and
+ * kBlend / kBlendGrad (reduce_blend.cu.cc:34-227). REFERENCE rejects
+ * ALPHA_ADDITIVE gradients as the reference launcher does.
+ */
+int gft_reduce_grid_blend_fwd(void* stream, const float* input, const int64_t shape[5], int32_t blending_mode,
+                              int32_t keep_dims, float* output);
+int gft_reduce_grid_blend_bwd(void* stream, const float* input, const int64_t shape[5], const float* output,
+                              const float* output_grad, int32_t blending_mode, int32_t keep_dims, int32_t grad_mode,
+                              float* input_grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -346,3 +346,24 @@ def step_fwd_bwd(d0, vel_c, target, M, V, P, F, out_dhw, depth_step, order=2, cl
                              grad_mode=grad_mode, prec=prec)
     gd, gv = advect_bwd(d0, vel_c, gd1, order, clamp, 1.0, "CLAMP", grad_mode, prec)
     return dict(loss=loss, img=img, d1=d1, gd=gd, gv=gv, gd1=gd1)
+
+
+# ------------------------------------------- renderer gather gradient (A4)
+def sample_transform_gather_grad(output_grad, M, V, P, F, in_dhw, interpolation="LINEAR", boundary="BORDER",
+                                 mipmapping="NONE", num_mipmaps=0, mip_bias=0.0,
+                                 coordinate_mode="TRANSFORM_LINDEPTH", prec=1):
+    """Custom gradient of Renderer._sample_camera_transform / _sample_transform
+    (renderer.py:200-227,246-272): per data batch i, resample every view's
+    gradient grid dy[i, j] back with camera j and the reverse coordinate mode
+    (mips built on dy) and sum over the views."""
+    rev = {"TRANSFORM_LINDEPTH": "TRANSFORM_LINDEPTH_REVERSE", "TRANSFORM_LINDEPTH_REVERSE": "TRANSFORM_LINDEPTH",
+           "TRANSFORM": "TRANSFORM_REVERSE", "TRANSFORM_REVERSE": "TRANSFORM"}[coordinate_mode]
+    og = _d(output_grad)
+    N, Vc = og.shape[:2]
+    M = _f(M).reshape(-1, 16)
+    out = []
+    for i in range(N):
+        dxv = sample_transform_fwd(og[i], np.repeat(M[i:i + 1], Vc, axis=0), V, P, F, in_dhw, interpolation, boundary,
+                                   mipmapping, num_mipmaps, mip_bias, rev, separate_camera_batch=False, prec=prec)
+        out.append(dxv.sum(axis=(0, 1)))
+    return np.stack(out)
