@@ -1,0 +1,190 @@
+"""Host-side checks that need no GPU: the C ABI library exports every entry
+point declared in include/gft.h, the ctypes binding covers them, workspace
+queries, argument validation (which returns before any device work), the op
+layer's refusal of host tensors, camera/matrix conventions and the workload
+geometry."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gft.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gft_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from gft import _lib
+    return _lib.lib()
+
+
+def test_every_declared_symbol_is_exported_and_bound(lib):
+    from gft import _lib
+    names = _declared()
+    assert len(names) >= 28
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_lib.exported_symbols()) == names
+
+
+def test_abi_version_and_error_slot(lib):
+    assert lib.gft_abi_version() == 1
+    assert isinstance(lib.gft_last_error(), bytes)
+
+
+def _i64(*v):
+    return (ctypes.c_int64 * len(v))(*v)
+
+
+def _i32(*v):
+    return (ctypes.c_int32 * len(v))(*v)
+
+
+def test_workspace_queries(lib):
+    # mip atlas: per batch levels 1..L of (D>>m, H>>m, W>>m, C), 64-float aligned blocks
+    shape = _i64(2, 13, 18, 22, 4)
+    per = sum((13 >> m) * (18 >> m) * (22 >> m) * 4 for m in (1, 2, 3))  # level 4 has D = 0
+    stride = (per + 63) // 64 * 64
+    assert lib.gft_mip_atlas_bytes(shape, 6) == ((stride * 4 * 2 + 255) // 256) * 256
+    assert lib.gft_sample_grid_transform_workspace(shape, 0, 3) == 0
+    assert lib.gft_sample_grid_transform_bwd_workspace(shape, 0, 0, 0) == ((13 * 18 * 22 * 4 + 255) // 256) * 256
+    assert lib.gft_sample_grid_transform_bwd_workspace(shape, 2, 6, 1) == 2 * lib.gft_mip_atlas_bytes(shape, 6)
+    assert lib.gft_sample_grid_lut_bwd_workspace(shape, 1) == 0
+    assert lib.gft_raymarch_grid_transform_bwd_workspace(shape, 1) == 0
+    vox = 13 * 18 * 22
+    assert lib.gft_raymarch_grid_transform_bwd_workspace(shape, 0) == ((vox * 4 * 4 + 255) // 256) * 256 + vox * 4
+    assert lib.gft_advect_saved_bytes(_i64(1, 4, 5, 6, 1)) > 0
+
+
+def _eye(n=1):
+    return np.ascontiguousarray(np.broadcast_to(np.eye(4, dtype=np.float32), (n, 4, 4)))
+
+
+def test_validation_returns_before_device_work(lib):
+    from gft import _lib
+    M, F = _eye(), np.array([[1, 10, -1, 1, 1, -1]], dtype=np.float32)
+    vp = M.ctypes.data_as(ctypes.c_void_p)
+    fp = F.ctypes.data_as(ctypes.c_void_p)
+    null = ctypes.c_void_p(0)
+    # C == 3 is Unimplemented (resample_grid.cc:330-331, raymarch_grid.cc)
+    rc = lib.gft_raymarch_grid_transform_fwd(null, null, _i64(1, 4, 4, 4, 3), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), 1,
+                                             0, 0, 0, 1, null)
+    assert rc == _lib.GFT_ERR_UNIMPLEMENTED
+    assert b"Channel" in lib.gft_last_error()
+    # keep_dims=False with a data batch of 2 is rejected
+    M2 = _eye(2)
+    rc = lib.gft_raymarch_grid_transform_fwd(null, null, _i64(2, 4, 4, 4, 1), M2.ctypes.data_as(ctypes.c_void_p), 2,
+                                             vp, vp, fp, 1, _i32(4, 4, 4), 1, 0, 0, 0, 1, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT
+    # model matrix batch mismatch
+    rc = lib.gft_sample_grid_transform_fwd(null, null, _i64(2, 4, 4, 4, 1), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), 1, 0,
+                                           0, 0, 0.0, 0, 0.5, 1, null, 0, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT and b"model matrix" in lib.gft_last_error()
+    # transform op: WRAP boundary and a cell-centre offset other than 0.5 are invalid
+    rc = lib.gft_sample_grid_transform_fwd(null, null, _i64(1, 4, 4, 4, 1), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), 1, 2,
+                                           0, 0, 0.0, 0, 0.5, 1, null, 0, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT
+    rc = lib.gft_sample_grid_transform_fwd(null, null, _i64(1, 4, 4, 4, 1), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), 1, 0,
+                                           0, 0, 0.0, 0, 0.0, 1, null, 0, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT and b"cell center" in lib.gft_last_error()
+    # mips need num_mipmaps > 0
+    rc = lib.gft_sample_grid_transform_fwd(null, null, _i64(1, 4, 4, 4, 1), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), 1, 0,
+                                           2, 0, 0.0, 0, 0.5, 1, null, 0, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT and b"num_mipmaps" in lib.gft_last_error()
+    # REFERENCE transform gradient with mips is rejected (the op registers NONE only)
+    rc = lib.gft_sample_grid_transform_bwd(null, null, _i64(1, 4, 4, 4, 1), null, vp, 1, vp, vp, fp, 1, _i32(4, 4, 4),
+                                           1, 0, 2, 2, 0.0, 0, 0.5, 1, 0, null, 0, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT
+    # LuT: normalised coordinates need offset 0.5; the table must have 4 channels
+    rc = lib.gft_sample_grid_lut_fwd(null, null, _i64(1, 4, 4, 4, 1), null, _i64(1, 3, 3, 3, 4), 1, 0, 0, 0, 0.0, 0.0,
+                                     1, 1, 1, null, 0, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT
+    rc = lib.gft_sample_grid_lut_fwd(null, null, _i64(1, 4, 4, 4, 1), null, _i64(1, 3, 3, 3, 3), 1, 0, 0, 0, 0.0, 0.0,
+                                     1, 1, 0, null, 0, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT
+    # reduce blend: REFERENCE has no ALPHA_ADDITIVE gradient
+    rc = lib.gft_reduce_grid_blend_bwd(null, null, _i64(1, 4, 4, 4, 4), null, null, 2, 0, 0, null)
+    assert rc == _lib.GFT_ERR_UNIMPLEMENTED
+    # workspace too small is its own status
+    rc = lib.gft_raymarch_grid_transform_bwd(null, ctypes.c_void_p(16), _i64(1, 4, 4, 4, 1), ctypes.c_void_p(16),
+                                             ctypes.c_void_p(16), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), 1, 0, 0, 0, 1,
+                                             0, null, 0, ctypes.c_void_p(16))
+    assert rc == _lib.GFT_ERR_WORKSPACE
+    # fused advection: scalar grids only
+    rc = lib.gft_advect_fwd(null, null, _i64(1, 4, 4, 4, 2), null, ctypes.c_float(1.0), 2, 1, null, null, 0)
+    assert rc == _lib.GFT_ERR_UNIMPLEMENTED
+
+
+def test_ops_refuse_host_tensors():
+    import torch
+    from gft import _lib
+    from gft.ops import blending_ops, raymarching_ops, sampling_ops
+    x = torch.zeros((1, 4, 4, 4, 1))
+    M = np.eye(4, dtype=np.float32)[None]
+    F = np.array([[1, 10, -1, 1, 1, -1]], dtype=np.float32)
+    with pytest.raises(_lib.InvalidArgumentError):
+        raymarching_ops.raymarch_grid_transform(x, M, M, M, F, (4, 4, 4))
+    with pytest.raises(_lib.InvalidArgumentError):
+        sampling_ops.sample_grid_transform(x, M, M, M, F, (4, 4, 4))
+    with pytest.raises(_lib.InvalidArgumentError):
+        blending_ops.reduce_grid_blend(x)
+
+
+def test_projection_and_view_matrices_follow_opengl():
+    from gft.camera import Camera, camera_params_batch, model_matrices
+    from gft.transform import GridTransform, Transform
+    t = GridTransform([10, 20, 30], translation=[1.0, 2.0, 3.0], rotation_deg=[0, 30, 0])
+    cam = Camera(t, nearFar=[2.0, 12.0], fov=40.0, aspect=16 / 9.0)
+    n, f = 2.0, 12.0
+    r = np.tan(np.radians(20.0)) * n          # horizontal fov (camera.py:64-70)
+    top = r / (16 / 9.0)
+    P = cam.projection_matrix()
+    assert np.allclose(P, [[n / r, 0, 0, 0], [0, n / top, 0, 0], [0, 0, -(f + n) / (f - n), -2 * f * n / (f - n)],
+                           [0, 0, -1, 0]], atol=1e-6)
+    assert np.allclose(cam.frustum(), [n, f, -r, r, top, -top], atol=1e-6)
+    # view = inverse of the camera transform; the ops receive transposes
+    V, Pc, F = camera_params_batch([cam])
+    assert np.allclose(V[0].T @ t.get_transform_matrix(), np.eye(4), atol=1e-5)
+    assert np.allclose(Pc[0], P.T)
+    # grid model matrix: centring translation then rotation then translation
+    g = GridTransform([4, 6, 8], translation=[5, 0, 0], center=True)
+    Mg = model_matrices([g])[0].T
+    assert np.allclose(Mg @ [4, 3, 2, 1], [5, 0, 0, 1])  # the grid centre (x,y,z) = (4,3,2) maps to the translation
+    rot = Transform(rotation_deg=[0, 90, 0]).get_transform_matrix()
+    assert np.allclose(rot @ [1, 0, 0, 1], [0, 0, -1, 1], atol=1e-12)
+
+
+def test_workload_geometry():
+    """Sample counts and pixel totals of the benchmark workloads (one depth
+    sample per grid cell across the grid's diagonal in xz)."""
+    from gft import synthetic as syn
+    expect = {"c1": (91, 145920), "c2": (182, 583680)}
+    for name, (S, npx) in expect.items():
+        w = syn.WORKLOADS[name]
+        cams, s = syn.arc_cameras(w["grid"], w["image"], w["views"])
+        assert s == S
+        assert w["views"] * w["image"][0] * w["image"][1] == npx
+        assert abs(cams[0].far - cams[0].near - np.sqrt(w["grid"][2] ** 2 + w["grid"][0] ** 2)) < 1e-4
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_graft_entry_points_exist():
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    assert callable(ge.build) and callable(ge.smoke)

@@ -1,0 +1,561 @@
+"""Pins of the CPU oracle (no GPU).
+
+The reference ships no golden vectors, so each oracle function is checked
+against what the mathematics fixes independently of the restatement:
+closed forms, exact reproduction cases (grid points, linear fields, identity
+cameras), brute-force evaluation on tiny grids, adjoint identities and FP64
+central finite differences for the EXACT gradients, and structural
+invariants of the reference's normalised (REFERENCE) gradients.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+RNG = np.random.Generator(np.random.PCG64(1234))
+
+
+# ------------------------------------------------------------ small helpers
+def _abs_lut(pos, lod=None):
+    """Absolute-coordinate lookup table [1, n, 1, 1, 4] for positions (n, 3)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    lut = np.zeros((1, pos.shape[0], 1, 1, 4))
+    lut[0, :, 0, 0, :3] = pos
+    if lod is not None:
+        lut[0, :, 0, 0, 3] = lod
+    return lut
+
+
+def _lut_sample(data, pos, lod=None, **kw):
+    kw.setdefault("relative_coords", False)
+    out = orc.sample_lut_fwd(data, _abs_lut(pos, lod), **kw)
+    return out[0, 0, :, 0, 0, :]
+
+
+def _identity_camera(dhw):
+    """Camera for which TRANSFORM maps output index q straight to grid index q
+    (P = I, V = (index -> ndc), M = I), as renderer.py:197-198 packs an
+    identity projection with frustum [-1,1,-1,1,1,-1]."""
+    D, H, W = dhw
+    A = np.array([[W / 2.0, 0, 0, W / 2.0], [0, H / 2.0, 0, H / 2.0], [0, 0, D / 2.0, D / 2.0], [0, 0, 0, 1]])
+    V = np.linalg.inv(A).T.astype(np.float32)[None]
+    P = np.eye(4, dtype=np.float32)[None]
+    M = np.eye(4, dtype=np.float32)[None]
+    F = np.array([[-1, 1, -1, 1, 1, -1]], dtype=np.float32)
+    return M, V, P, F
+
+
+def _trilinear_bruteforce(data, p, boundary):
+    """Textbook trilinear interpolation of data[D,H,W,C] at index position p
+    (x, y, z; cell centres at integers): sum over the 8 corners of the
+    product weights. BORDER: 0 outside [-0.5, dims-0.5), corners outside read
+    0; CLAMP clamps corner indices; WRAP wraps them."""
+    D, H, W, C = data.shape
+    dims = (W, H, D)
+    if boundary == "BORDER" and not all(0.0 <= p[a] + 0.5 < dims[a] for a in range(3)):
+        return np.zeros(C)
+    base = [int(np.floor(p[a])) for a in range(3)]
+    t = [p[a] - base[a] for a in range(3)]
+    acc = np.zeros(C)
+    for dx, dy, dz in itertools.product((0, 1), repeat=3):
+        idx = [base[0] + dx, base[1] + dy, base[2] + dz]
+        w = (t[0] if dx else 1 - t[0]) * (t[1] if dy else 1 - t[1]) * (t[2] if dz else 1 - t[2])
+        if boundary == "BORDER":
+            if not all(0 <= idx[a] < dims[a] for a in range(3)):
+                continue
+        elif boundary == "CLAMP":
+            idx = [min(max(idx[a], 0), dims[a] - 1) for a in range(3)]
+        else:
+            idx = [idx[a] % dims[a] for a in range(3)]
+        acc += w * data[idx[2], idx[1], idx[0]]
+    return acc
+
+
+# ------------------------------------------------------- sampling (A1, A8)
+@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP", "WRAP"])
+def test_trilinear_matches_bruteforce(boundary):
+    data = RNG.uniform(-1, 1, size=(1, 3, 4, 5, 2))
+    pos = RNG.uniform(-1.5, 6.0, size=(200, 3))
+    got = _lut_sample(data, pos, boundary=boundary)
+    for i in range(len(pos)):
+        ref = _trilinear_bruteforce(data[0], pos[i], boundary)
+        assert np.allclose(got[i], ref, atol=1e-12), (pos[i], got[i], ref)
+
+
+@pytest.mark.parametrize("filt", ["NEAREST", "LINEAR", "MIN", "MAX"])
+@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP", "WRAP"])
+def test_grid_points_are_exact(filt, boundary):
+    data = RNG.uniform(-1, 1, size=(1, 4, 5, 6, 4))
+    idx = np.array([(x, y, z) for z in range(4) for y in range(5) for x in range(6)], dtype=np.float64)
+    got = _lut_sample(data, idx, interpolation=filt, boundary=boundary)
+    ref = data[0].reshape(-1, 4)
+    assert np.array_equal(got, ref)
+
+
+def test_linear_field_is_reproduced():
+    D, H, W = 5, 6, 7
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    f = (0.3 * x - 0.7 * y + 1.1 * z + 2.0)[None, ..., None]
+    pos = RNG.uniform(0, 1, size=(300, 3)) * np.array([W - 1, H - 1, D - 1])
+    for boundary in ("BORDER", "CLAMP"):
+        got = _lut_sample(f, pos, boundary=boundary)[:, 0]
+        assert np.allclose(got, 0.3 * pos[:, 0] - 0.7 * pos[:, 1] + 1.1 * pos[:, 2] + 2.0, atol=1e-12)
+
+
+def test_border_clamp_wrap_outside():
+    data = RNG.uniform(0.5, 1, size=(1, 3, 3, 3, 1))
+    far = np.array([[-5.0, 1.0, 1.0], [1.0, 9.0, 1.0], [1.0, 1.0, -3.2]])
+    assert np.all(_lut_sample(data, far, boundary="BORDER") == 0)
+    clamp = _lut_sample(data, far, boundary="CLAMP")[:, 0]
+    assert clamp[0] == data[0, 1, 1, 0, 0] and clamp[1] == data[0, 1, 2, 1, 0] and clamp[2] == data[0, 0, 1, 1, 0]
+    p = RNG.uniform(0, 3, size=(50, 3))
+    for filt in ("LINEAR", "NEAREST"):
+        a = _lut_sample(data, p, interpolation=filt, boundary="WRAP")
+        # positive shifts only: the nearest index truncates toward zero
+        # (int cast of p + 0.5), so negative coordinates are not periodic
+        shift = np.array([3.0, -6.0, 9.0]) if filt == "LINEAR" else np.array([3.0, 6.0, 9.0])
+        b = _lut_sample(data, p + shift, interpolation=filt, boundary="WRAP")
+        assert np.allclose(a, b, atol=1e-12)
+
+
+def test_nearest_rounds_half_up_and_truncates():
+    data = np.arange(5, dtype=np.float64).reshape(1, 1, 1, 5, 1)
+    pos = np.array([[1.49, 0, 0], [1.5, 0, 0], [2.51, 0, 0], [-0.4, 0, 0], [-0.6, 0, 0], [4.49, 0, 0]])
+    got = _lut_sample(data, pos, interpolation="NEAREST", boundary="BORDER")[:, 0]
+    # index = trunc(p + 0.5): -0.6 -> trunc(-0.1) = 0 (C truncation toward zero)
+    assert list(got) == [1, 2, 3, 0, 0, 4]
+
+
+def test_min_max_are_corner_extrema():
+    data = RNG.uniform(0.1, 1, size=(1, 4, 4, 4, 2))
+    pos = RNG.uniform(-0.7, 3.7, size=(300, 3))
+    lo = _lut_sample(data, pos, interpolation="MIN", boundary="BORDER")
+    hi = _lut_sample(data, pos, interpolation="MAX", boundary="BORDER")
+    for i, p in enumerate(pos):
+        if not all(0 <= p[a] + 0.5 < 4 for a in range(3)):
+            assert np.all(lo[i] == 0) and np.all(hi[i] == 0)
+            continue
+        corners, invalid = [], False
+        for c in itertools.product(*[sorted({int(np.floor(p[a])), int(np.ceil(p[a]))}) for a in range(3)]):
+            if all(0 <= c[a] < 4 for a in range(3)):
+                corners.append(data[0, c[2], c[1], c[0]])
+            else:
+                invalid = True
+        corners = np.array(corners)
+        start = 0.0 if invalid else None
+        mn = corners.min(axis=0) if start is None else np.minimum(corners.min(axis=0), 0.0)
+        mx = corners.max(axis=0) if start is None else np.maximum(corners.max(axis=0), 0.0)
+        assert np.array_equal(lo[i], mn) and np.array_equal(hi[i], mx)
+
+
+@pytest.mark.parametrize("mode", ["TRANSFORM", "TRANSFORM_REVERSE"])
+def test_identity_camera_resample_is_identity(mode):
+    dhw = (4, 8, 16)
+    data = RNG.uniform(-1, 1, size=(1,) + dhw + (2,))
+    M, V, P, F = _identity_camera(dhw)
+    for filt in ("LINEAR", "NEAREST"):
+        out = orc.sample_transform_fwd(data, M, V, P, F, dhw, filt, "BORDER", coordinate_mode=mode)
+        assert np.allclose(out[:, 0], data, atol=1e-12)
+
+
+def _numpy_geometry(M_cm, V_cm, P_cm, F, out_dhw, mode):
+    """Object-space position of every output cell centre, computed with
+    numpy's own inverse from the row-major OpenGL matrices (an independent
+    restatement of transformations.hpp:120-145 for the forward modes)."""
+    M, V, P = M_cm.T.astype(np.float64), V_cm.T.astype(np.float64), P_cm.T.astype(np.float64)
+    D, H, W = out_dhw
+    z, y, x = np.meshgrid(np.arange(D) + 0.5, np.arange(H) + 0.5, np.arange(W) + 0.5, indexing="ij")
+    ndc = np.stack([x / W * 2 - 1, y / H * 2 - 1, z / D * 2 - 1, np.ones_like(x)], -1)
+    if mode == "TRANSFORM_LINDEPTH":
+        n, f = float(F[0]), float(F[1])
+        zv = -(n + z / D * (f - n))
+        ndc[..., 2] = -P[2, 2] - P[2, 3] / zv
+    vs = ndc @ np.linalg.inv(P).T
+    vs = vs / vs[..., 3:4]
+    return (vs @ np.linalg.inv(V @ M).T)[..., :3]
+
+
+@pytest.mark.parametrize("mode", ["TRANSFORM_LINDEPTH", "TRANSFORM"])
+def test_forward_modes_match_numpy_geometry(mode):
+    """An affine field a.p + b is reproduced exactly by trilinear sampling,
+    so the resample at every in-hull cell equals the field at the position
+    numpy computes from the camera."""
+    from helpers import scene
+    grid = (8, 10, 9)
+    sc = scene(grid=grid, image=(7, 6), views=2, seed=2)
+    a, b = np.array([0.3, -0.2, 0.45]), 0.7
+    D, H, W = grid
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    field = (a[0] * (x + 0.5) + a[1] * (y + 0.5) + a[2] * (z + 0.5) + b)[None, ..., None]
+    out_dhw = (11, 7, 6)
+    out = orc.sample_transform_fwd(field, sc["M"], sc["V"], sc["P"], sc["F"], out_dhw, "LINEAR", "BORDER",
+                                   coordinate_mode=mode)
+    checked = 0
+    for v in range(2):
+        pos = _numpy_geometry(sc["M"][0], sc["V"][v], sc["P"][v], sc["F"][v], out_dhw, mode)
+        inside = np.all((pos - 0.5 >= 0) & (pos - 0.5 <= np.array([W - 1, H - 1, D - 1])), axis=-1)
+        expect = pos @ a + b
+        assert np.allclose(out[0, v, ..., 0][inside], expect[inside], atol=1e-9)
+        assert np.all(out[0, v, ..., 0][~inside & np.all((pos + 0.5 < 0) | (pos - 0.5 > np.array([W, H, D])),
+                                                        axis=-1)] == 0)
+        checked += inside.sum()
+    assert checked > 50
+
+
+def test_raymarch_equals_depth_sum_of_resample_for_ortho():
+    """With an orthographic camera the raymarch samples (uniform in object
+    space between the near and far cell centres) coincide with the TRANSFORM
+    resample's cell centres, so ADDITIVE raymarch == sum over depth."""
+    from gft import synthetic as syn
+    from gft.camera import camera_params_batch, model_matrices
+    from helpers import ortho_camera
+    grid = (6, 8, 7)
+    cam = ortho_camera(grid, (8, 7), 9, yaw=25.0)
+    M = model_matrices([syn.grid_transform(grid)])
+    V, P, F = camera_params_batch([cam])
+    out_dhw = (9, 8, 7)
+    data = RNG.uniform(0, 1, size=(1,) + grid + (1,))
+    ray = orc.raymarch_fwd(data, M, V, P, F, out_dhw, "LINEAR", "BORDER", "ADDITIVE")
+    fr = orc.sample_transform_fwd(data, M, V, P, F, out_dhw, "LINEAR", "BORDER", coordinate_mode="TRANSFORM")
+    assert np.abs(fr).max() > 0.1
+    assert np.allclose(ray[0, 0], fr[0, 0].sum(0), atol=1e-10)
+
+
+# ------------------------------------------------------------ mips (A2)
+def test_mips_are_block_means():
+    data = RNG.uniform(0, 1, size=(9, 8, 12, 3))
+    lv = orc.build_mips(data, 3)
+    assert [l.shape for l in lv] == [(4, 4, 6, 3), (2, 2, 3, 3), (1, 1, 1, 3)]
+    for m, l in enumerate(lv, start=1):
+        s = 2 ** m
+        for k, j, i in itertools.product(range(l.shape[0]), range(l.shape[1]), range(l.shape[2])):
+            block = data[k * s:(k + 1) * s, j * s:(j + 1) * s, i * s:(i + 1) * s]
+            assert np.allclose(l[k, j, i], block.reshape(-1, 3).mean(0), atol=1e-12)
+    const = np.full((8, 8, 8, 1), 0.37)
+    assert all(np.allclose(l, 0.37, atol=1e-15) for l in orc.build_mips(const, 3))
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_lod_selects_level_of_downscaled_identity(k):
+    """Output cells 2^k times larger than the grid cells: LoD = k exactly, so
+    NEAREST mips read level k at its own cell centres."""
+    dhw = (8, 16, 16)
+    data = RNG.uniform(0, 1, size=(1,) + dhw + (1,))
+    out_dhw = tuple(s >> k for s in dhw)
+    M, V, P, F = _identity_camera(dhw)
+    # index map of the output: A_out maps output ndc to grid index, so reuse
+    # the grid's camera; the output grid is coarser by 2^k per axis
+    out = orc.sample_transform_fwd(data, M, V, P, F, out_dhw, "LINEAR", "CLAMP", "NEAREST", 3, 0.0, "TRANSFORM")
+    ref = orc.build_mips(data[0], 3)[k - 1]
+    assert np.allclose(out[0, 0], ref, atol=1e-12)
+
+
+def test_linear_mip_lerp_weight_is_fract_lod():
+    """Checkerboard along x: level 1 is 0 everywhere, so a LINEAR-mip sample
+    at a level-0 cell centre is (1 - fract(lod)) * value; NEAREST mips switch
+    level at lod 0.5."""
+    D, H, W = 4, 4, 8
+    x = np.arange(W)
+    data = np.broadcast_to(np.where(x % 2 == 0, 1.0, -1.0), (1, D, H, W))[..., None].copy()
+    pos = np.array([[i + 0.5, 1.5, 1.5] for i in range(W)])
+    for lod in (0.0, 0.25, 0.5, 0.8):
+        got = _lut_sample(data, pos, lod=np.full(W, lod), boundary="CLAMP", mipmapping="LINEAR", num_mipmaps=1,
+                          cell_center_offset=0.5)[:, 0]
+        assert np.allclose(got, (1 - lod) * data[0, 1, 1, :, 0], atol=1e-12)
+        nearest = _lut_sample(data, pos, lod=np.full(W, lod), boundary="CLAMP", mipmapping="NEAREST", num_mipmaps=1,
+                              cell_center_offset=0.5)[:, 0]
+        assert np.allclose(nearest, data[0, 1, 1, :, 0] if lod < 0.5 else 0.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_lod_transform_matches_its_own_jacobian(inverse):
+    from helpers import scene
+    sc = scene(grid=(8, 10, 12), image=(9, 7), views=2, seed=3)
+    MV = np.stack([(sc["V"][i].T @ sc["M"][0].T).T for i in range(2)]).astype(np.float32)
+    grid_dhw, fr_dhw = (8, 10, 12), tuple(sc["out_dhw"])
+    in_dhw, out_dhw = (fr_dhw, grid_dhw) if inverse else (grid_dhw, fr_dhw)
+    out = orc.lod_transform(in_dhw, MV, sc["P"], sc["F"], out_dhw, inverse)
+    pos = out[..., :3]
+    # central differences of the position output at interior cells
+    jx = (pos[:, 1:-1, 1:-1, 2:] - pos[:, 1:-1, 1:-1, :-2]) / 2
+    jy = (pos[:, 1:-1, 2:, 1:-1] - pos[:, 1:-1, :-2, 1:-1]) / 2
+    jz = (pos[:, 2:, 1:-1, 1:-1] - pos[:, :-2, 1:-1, 1:-1]) / 2
+    foot = np.sqrt(jx ** 2 + jy ** 2 + jz ** 2)  # per component a: |d p_a / d (x, y, z)|
+    lod = np.maximum(0.0, np.log2(foot.max(-1)))
+    assert np.allclose(out[:, 1:-1, 1:-1, 1:-1, 3], lod, atol=1e-9)
+
+
+# ---------------------------------------------------------- raymarch (A5)
+def _cams(views=2, seed=0):
+    from helpers import scene
+    return scene(grid=(6, 8, 7), image=(5, 4), views=views, seed=seed)
+
+
+def test_raymarch_constant_density_closed_forms():
+    sc = _cams()
+    S = sc["out_dhw"][0]
+    e, s = np.array([0.3, 0.5, 0.9]), 0.07
+    data = np.broadcast_to(np.concatenate([e, [s]]), (1, 6, 8, 7, 4)).copy()
+    i = np.arange(1, S + 1)
+    expect = {"BEER_LAMBERT": np.concatenate([e * np.exp(-i * s).sum(), [S * s]]),
+              "ADDITIVE": S * np.concatenate([e, [s]]),
+              "ALPHA": np.concatenate([e * ((1 - s) ** (i - 1)).sum(), [1 - (1 - s) ** S]]),
+              "ALPHA_ADDITIVE": np.concatenate([S * (1 - s) * e, [S * s]])}
+    for blend, ref in expect.items():
+        out = orc.raymarch_fwd(data, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], "LINEAR", "CLAMP", blend)
+        assert np.allclose(out, ref, rtol=1e-12, atol=1e-12), blend
+    d1 = np.full((1, 6, 8, 7, 1), s)
+    assert np.allclose(orc.raymarch_fwd(d1, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], boundary="CLAMP"),
+                       S * s, atol=1e-12)
+    d2 = np.broadcast_to(np.array([0.4, s]), (1, 6, 8, 7, 2)).copy()
+    out2 = orc.raymarch_fwd(d2, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], boundary="CLAMP")
+    assert np.allclose(out2[..., 0], 0.4 * np.exp(-i * s).sum(), atol=1e-12)
+
+
+# ---------------------------------------------------- EXACT adjoints (FD64)
+def _fd(fun, x, idx, h):
+    xp, xm = x.copy(), x.copy()
+    xp.flat[idx] += h
+    xm.flat[idx] -= h
+    return (fun(xp) - fun(xm)) / (2 * h)
+
+
+@pytest.mark.parametrize("blend,channels", [("BEER_LAMBERT", 4), ("BEER_LAMBERT", 2), ("ALPHA", 4),
+                                            ("ALPHA_ADDITIVE", 4), ("ADDITIVE", 1)])
+@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP"])
+def test_raymarch_exact_gradient_fd(blend, channels, boundary):
+    sc = _cams(seed=5)
+    data = RNG.uniform(0.0, 0.15, size=(1, 6, 8, 7, channels))
+    out = orc.raymarch_fwd(data, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], "LINEAR", boundary, blend)
+    y = RNG.uniform(-1, 1, size=out.shape)
+    g = orc.raymarch_bwd(data, out, y, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], boundary, blend,
+                         grad_mode="EXACT")
+
+    def loss(d):
+        return float((orc.raymarch_fwd(d, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], "LINEAR", boundary,
+                                       blend) * y).sum())
+    cand = np.argsort(-np.abs(g).ravel())[:6].tolist() + RNG.integers(0, data.size, 4).tolist()
+    for idx in cand:
+        assert abs(_fd(loss, data, idx, 1e-5) - g.flat[idx]) <= 1e-6 * max(np.abs(g).max(), 1e-12)
+
+
+@pytest.mark.parametrize("mode", ["TRANSFORM_LINDEPTH", "TRANSFORM_REVERSE"])
+@pytest.mark.parametrize("filt,mip", [("LINEAR", "NONE"), ("NEAREST", "NONE"), ("LINEAR", "LINEAR"),
+                                      ("NEAREST", "NEAREST")])
+def test_resample_exact_adjoint_identity(mode, filt, mip):
+    sc = _cams(seed=6)
+    rev = mode.endswith("REVERSE")
+    in_dhw = tuple(sc["out_dhw"]) if rev else (6, 8, 7)
+    out_dhw = (6, 8, 7) if rev else tuple(sc["out_dhw"])
+    x = RNG.uniform(-1, 1, size=(1,) + in_dhw + (2,))
+    kw = dict(interpolation=filt, boundary="BORDER", mipmapping=mip, num_mipmaps=2 if mip != "NONE" else 0,
+              mip_bias=0.3, coordinate_mode=mode)
+    ax = orc.sample_transform_fwd(x, sc["M"], sc["V"], sc["P"], sc["F"], out_dhw, **kw)
+    y = RNG.uniform(-1, 1, size=ax.shape)
+    aty = orc.sample_transform_bwd(x, y, sc["M"], sc["V"], sc["P"], sc["F"], grad_mode="EXACT", **kw)
+    lhs, rhs = float((ax * y).sum()), float((x * aty).sum())
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1.0)
+
+
+@pytest.mark.parametrize("filt", ["LINEAR", "NEAREST", "MIN", "MAX"])
+@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP", "WRAP"])
+def test_lut_exact_input_adjoint(filt, boundary):
+    """The sample is linear in the data for LINEAR/NEAREST and piecewise
+    linear (a selection) for MIN/MAX: <A x, y> == <x, A^T y>."""
+    x = RNG.uniform(0.1, 1, size=(1, 4, 5, 6, 2))
+    lut = RNG.uniform(-2, 2, size=(2, 3, 4, 5, 4))
+    kw = dict(interpolation=filt, boundary=boundary)
+    ax = orc.sample_lut_fwd(x, lut, **kw)
+    y = RNG.uniform(-1, 1, size=ax.shape)
+    gin, _ = orc.sample_lut_bwd(x, lut, y, grad_mode="EXACT", **kw)
+    assert abs(float((ax * y).sum()) - float((x * gin).sum())) <= 1e-10 * max(abs(float((ax * y).sum())), 1)
+
+
+@pytest.mark.parametrize("normalized", [False, True])
+def test_lut_exact_lookup_gradient_fd(normalized):
+    x = RNG.uniform(0.1, 1, size=(1, 4, 5, 6, 2))
+    lut = RNG.uniform(-2, 2, size=(1, 3, 4, 5, 4))
+    kw = dict(interpolation="LINEAR", boundary="CLAMP")
+    if normalized:
+        lut[..., :3] = RNG.uniform(0.0, 1.0, size=lut[..., :3].shape)
+        kw.update(relative_coords=False, normalized_coords=True, cell_center_offset=0.5)
+    y = RNG.uniform(-1, 1, size=(1, 1, 3, 4, 5, 2))
+    _, glut = orc.sample_lut_bwd(x, lut, y, grad_mode="EXACT", **kw)
+
+    def loss(l):
+        return float((orc.sample_lut_fwd(x, l, **kw) * y).sum())
+    for idx in RNG.integers(0, lut.size, 12):
+        if idx % 4 == 3:
+            continue
+        assert abs(_fd(loss, lut, int(idx), 1e-7) - glut.flat[idx]) <= 1e-5 * max(np.abs(glut).max(), 1e-9)
+
+
+@pytest.mark.parametrize("blend", ["BEER_LAMBERT", "ALPHA", "ALPHA_ADDITIVE", "ADDITIVE"])
+@pytest.mark.parametrize("keep", [False, True])
+def test_reduce_blend_exact_gradient_fd(blend, keep):
+    x = RNG.uniform(0.0, 0.3, size=(1, 5, 2, 3, 4))
+    out = orc.reduce_blend_fwd(x, blend, keep)
+    y = RNG.uniform(-1, 1, size=out.shape)
+    g = orc.reduce_blend_bwd(x, out, y, blend, keep, grad_mode="EXACT")
+
+    def loss(d):
+        return float((orc.reduce_blend_fwd(d, blend, keep) * y).sum())
+    for idx in range(0, x.size, 7):
+        assert abs(_fd(loss, x, idx, 1e-6) - g.flat[idx]) <= 1e-7 * max(np.abs(g).max(), 1)
+
+
+def test_reduce_blend_beer_lambert_closed_form():
+    s = RNG.uniform(0, 0.3, size=(1, 6, 2, 2, 1))
+    e = RNG.uniform(0, 1, size=(1, 6, 2, 2, 3))
+    out = orc.reduce_blend_fwd(np.concatenate([e, s], -1), "BEER_LAMBERT", False)
+    tau = np.cumsum(s, axis=1)
+    assert np.allclose(out[..., 3], tau[:, -1, ..., 0], atol=1e-12)
+    assert np.allclose(out[..., :3], (np.exp(-tau) * e).sum(1), atol=1e-12)
+
+
+# -------------------------------------------- REFERENCE-mode invariants
+def test_reference_raymarch_gradient_normalises_per_camera():
+    """Constant output gradient c with ADDITIVE: each camera's scatter is
+    divided by its own weight sum, so every touched voxel gets c per camera."""
+    sc = _cams(views=1, seed=7)
+    data = RNG.uniform(0, 1, size=(1, 6, 8, 7, 1))
+    out = orc.raymarch_fwd(data, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], "LINEAR", "BORDER", "ADDITIVE")
+    g = orc.raymarch_bwd(data, out, np.full(out.shape, 0.7), sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"],
+                         "BORDER", "ADDITIVE", grad_mode="REFERENCE")
+    touched = g != 0
+    assert touched.mean() > 0.3
+    assert np.allclose(g[touched], 0.7, atol=1e-12)
+
+
+def test_reference_resample_gradient_counts_every_corner():
+    """Identity camera, constant output gradient c: every output sample sits
+    on a grid point, where floor == ceil, so each of the 8 corner visits hits
+    the same voxel: weight sum 1, count 8, gradient c / 8 (weight-independent
+    counting of the reference)."""
+    dhw = (4, 8, 8)
+    M, V, P, F = _identity_camera(dhw)
+    x = RNG.uniform(size=(1,) + dhw + (1,))
+    g = orc.sample_transform_bwd(x, np.full((1, 1) + dhw + (1,), 0.8), M, V, P, F, coordinate_mode="TRANSFORM",
+                                 grad_mode="REFERENCE")
+    assert np.allclose(g, 0.1, atol=1e-12)
+    ge = orc.sample_transform_bwd(x, np.full((1, 1) + dhw + (1,), 0.8), M, V, P, F, coordinate_mode="TRANSFORM",
+                                  grad_mode="EXACT")
+    assert np.allclose(ge, 0.8, atol=1e-12)
+
+
+def test_reference_lut_gradient_vanishes_at_grid_points():
+    """read3DGrad uses floor/ceil corners, so at integer positions its spatial
+    derivative is 0; the EXACT gradient is the one-sided slope."""
+    D, H, W = 4, 5, 6
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    f = (0.5 * x + 0.25 * y - 1.0 * z)[None, ..., None]
+    lut = np.zeros((1, D, H, W, 4))
+    og = np.ones((1, 1, D, H, W, 1))
+    _, gref = orc.sample_lut_bwd(f, lut, og, boundary="CLAMP", grad_mode="REFERENCE")
+    _, gex = orc.sample_lut_bwd(f, lut, og, boundary="CLAMP", grad_mode="EXACT")
+    assert np.all(gref == 0)
+    inner = gex[0, :-1, :-1, :-1]
+    assert np.allclose(inner[..., 0], 0.5) and np.allclose(inner[..., 1], 0.25) and np.allclose(inner[..., 2], -1.0)
+
+
+# ------------------------------------------------------- advection (A10)
+def _linear_field(D, H, W):
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    return (0.2 * x - 0.3 * y + 0.4 * z + 1.0)[None, ..., None]
+
+
+@pytest.mark.parametrize("order,clamp", [(1, "NONE"), (2, "NONE"), (2, "MC_SMOOTH"), (1, "NEGATIVE")])
+def test_advect_zero_velocity_is_identity(order, clamp):
+    d = RNG.uniform(0, 1, size=(1, 5, 6, 7, 1))
+    out = orc.advect_fwd(d, np.zeros((1, 5, 6, 7, 3)), order, clamp)
+    assert np.allclose(out, d, atol=1e-15)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_advect_integer_translation_is_a_shift(order):
+    d = RNG.uniform(0, 1, size=(1, 5, 6, 7, 1))
+    v = np.zeros((1, 5, 6, 7, 3))
+    v[..., 0], v[..., 2] = 2.0, -1.0  # x by +2 cells, z by -1 cell
+    out = orc.advect_fwd(d, v, order, "MC_SMOOTH")
+    # out(x, y, z) = d(x - 2, y, z + 1) with CLAMP edge replication
+    xs = np.clip(np.arange(7) - 2, 0, 6)
+    zs = np.clip(np.arange(5) + 1, 0, 4)
+    ref = d[:, zs][:, :, :, xs]
+    # MacCormack's back-trace b(x) = f(x + v) returns d(x) wherever neither
+    # trace leaves the grid: x <= 4, z >= 1
+    inner = (slice(None), slice(1, 5), slice(None), slice(0, 5))
+    assert np.allclose(out[inner], ref[inner], atol=1e-12)
+    if order == 1:
+        assert np.allclose(out, ref, atol=1e-12)
+
+
+def test_maccormack_is_exact_on_linear_fields():
+    d = _linear_field(8, 9, 10)
+    v = np.broadcast_to(np.array([0.37, -0.61, 0.23]), (1, 8, 9, 10, 3)).copy()
+    for order in (1, 2):
+        out = orc.advect_fwd(d, v, order, "NONE")
+        ref = _linear_field(8, 9, 10) - (0.2 * 0.37 - 0.3 * -0.61 + 0.4 * 0.23)
+        inner = (slice(None), slice(2, -2), slice(2, -2), slice(2, -2))
+        assert np.allclose(out[inner], ref[inner], atol=1e-12)
+
+
+def test_mc_smooth_is_bounded_or_first_order():
+    d = RNG.uniform(0, 1, size=(1, 8, 8, 8, 1))
+    v = RNG.uniform(-1.5, 1.5, size=(1, 8, 8, 8, 3))
+    out, parts = orc.advect_fwd(d, v, 2, "MC_SMOOTH", return_parts=True)
+    inside = (out >= parts["mn"]) & (out <= parts["mx"])
+    first = out == parts["f"]
+    assert np.all(inside | first)
+    assert np.array_equal(out[parts["mask"]], parts["f"][parts["mask"]])
+    assert parts["mask"].any() and (~parts["mask"]).any()
+
+
+@pytest.mark.parametrize("order,clamp", [(1, "NONE"), (2, "NONE"), (2, "MC_SMOOTH")])
+def test_advect_exact_gradient_fd(order, clamp):
+    d = RNG.uniform(0, 1, size=(1, 5, 6, 7, 1))
+    v = RNG.uniform(-1.2, 1.2, size=(1, 5, 6, 7, 3))
+    y = RNG.uniform(-1, 1, size=d.shape)
+    gd, gv = orc.advect_bwd(d, v, y, order, clamp, grad_mode="EXACT")
+    mask0 = orc.advect_fwd(d, v, order, clamp, return_parts=True)[1].get("mask")
+    h = 1e-6
+
+    def central(x, idx, fun):
+        """Central difference, or None where the MC_SMOOTH mask switches."""
+        xp, xm = x.copy(), x.copy()
+        xp.flat[idx] += h
+        xm.flat[idx] -= h
+        op, pp = fun(xp)
+        om, pm = fun(xm)
+        if mask0 is not None and not (np.array_equal(pp["mask"], mask0) and np.array_equal(pm["mask"], mask0)):
+            return None
+        return float(((op - om) * y).sum()) / (2 * h)
+
+    checked = 0
+    for idx in RNG.integers(0, d.size, 12):
+        fd = central(d, int(idx), lambda dd: orc.advect_fwd(dd, v, order, clamp, return_parts=True))
+        if fd is not None:
+            assert abs(fd - gd.flat[idx]) <= 1e-6 * np.abs(gd).max()
+            checked += 1
+    for idx in RNG.integers(0, v.size, 40):
+        fd = central(v, int(idx), lambda vv: orc.advect_fwd(d, vv, order, clamp, return_parts=True))
+        if fd is not None:
+            assert abs(fd - gv.flat[idx]) <= 1e-5 * np.abs(gv).max()
+            checked += 1
+    assert checked >= 20
+
+
+def test_transport_render_step_gradient_fd():
+    from helpers import scene
+    sc = scene(grid=(6, 8, 7), image=(5, 4), views=2, seed=9)
+    d0 = sc["data"].astype(np.float64)
+    v = RNG.uniform(-0.8, 0.8, size=d0.shape[:-1] + (3,))
+    tgt = RNG.uniform(0, 0.5, size=(1, 2) + tuple(sc["out_dhw"][1:]) + (3,))
+    args = (sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"])
+    r = orc.step_fwd_bwd(d0, v, tgt, *args, order=1, clamp="NONE")
+
+    def loss_d(dd):
+        return orc.step_fwd_bwd(dd, v, tgt, *args, order=1, clamp="NONE")["loss"]
+    for idx in np.argsort(-np.abs(r["gd"]).ravel())[:5]:
+        assert abs(_fd(loss_d, d0, int(idx), 1e-6) - r["gd"].flat[idx]) <= 1e-5 * np.abs(r["gd"]).max()
