@@ -11,12 +11,16 @@ second blob density (seed 2).
 
 Prints one JSON line: voxel-samples/s (one 8-corner trilinear evaluation = 1
 voxel-sample; N_vs = 6V + 2 sum W*H*S per step), the HBM roofline of the
-dominant kernel from HIP events in the timed region, and a CPU baseline (the
+dominant kernel from HIP events around its launches in a second pass over the
+same steps, the plain per-op FP32 path (the reference's op composition on
+this build's per-op kernels) timed on the same inputs, and a CPU baseline (the
 FP32 oracle, one full step on the host cores).
+
+``--copy-kernel`` instead runs a 1 GiB device copy: the calibration kernel of
+known traffic for the HBM counters (FETCH_SIZE / WRITE_SIZE) under rocprofv3.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -27,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "voxel-samples/s fwd+bwd (advect+resample+raymarch), HBM GB/s vs plain FP32"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c2.txt")
 
 
 def build_inputs(name):
@@ -57,22 +62,53 @@ def cpu_baseline(inp, target_np):
     return time.perf_counter() - t0, threads
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration),
+    or None when the summary has no line for it."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        parts = line.split()
+        if len(parts) >= 4 and parts[0] == "traffic" and parts[1] == workload and parts[2] == kernel:
+            return float(parts[3])
+    return None
+
+
+def copy_kernel():
+    import torch
+    n = (1 << 30) // 4
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-kernel", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_kernel:
+        copy_kernel()
+        return
 
     import numpy as np
     import torch
 
-    from gft.pipeline import TransportRenderStep, render_density_forward, _Cameras
+    from gft._lib import KernelTimers
+    from gft.pipeline import TransportRenderStep, _Cameras, render_density_forward
 
     inp = build_inputs(args.workload)
     dev = torch.device("cuda:0")
@@ -89,29 +125,38 @@ def main():
         step.run(d0, vel, target)
     torch.cuda.synchronize()
 
-    timers = {}
+    # timed region: exactly K fused steps, inputs resident in HBM
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
     for _ in range(args.steps):
-        step.run(d0, vel, target, timers)
+        step.run(d0, vel, target)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_total = start.elapsed_time(stop)
-    ms_per_step = ms_total / args.steps
+    ms_per_step = start.elapsed_time(stop) / args.steps
 
-    phase_ms = {}
-    for name, evs in timers.items():
-        pairs = list(zip(evs[0::2], evs[1::2]))
-        phase_ms[name] = sum(a.elapsed_time(b) for a, b in pairs) / len(pairs)
-    alg = step.algorithmic_bytes()
-    dominant = max(phase_ms, key=phase_ms.get)
-    achieved = alg[dominant] / (phase_ms[dominant] * 1e-3) / 1e9
+    # second pass over K steps with HIP events around every kernel launch
+    with KernelTimers("*") as kt:
+        for _ in range(args.steps):
+            step.run(d0, vel, target)
+        torch.cuda.synchronize()
+    kbytes = step.kernel_algorithmic_bytes()
+    kernels = {}
+    for name, (n, tot) in kt.result.items():
+        avg = tot / max(n, 1)
+        b = kbytes.get(name, 0)
+        kernels[name] = {"launches_per_step": n / args.steps, "avg_ms": avg, "alg_bytes": b,
+                         "GBps": b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0,
+                         "traffic": measured_traffic(name, args.workload)}
+    dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches_per_step"])
+    dk = kernels[dominant]
+    traffic = measured_traffic(dominant, args.workload)
+
     nvs = step.voxel_samples()
     value = nvs / (ms_per_step * 1e-3)
-    b_alg = sum(alg.values())
+    b_alg = sum(step.algorithmic_bytes().values())
 
     result = {
         "metric": METRIC,
@@ -137,20 +182,41 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
-            "achieved": achieved,
+            "achieved": dk["GBps"],
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
-            "algorithmic_bytes": alg[dominant],
-            "avg_ms": phase_ms[dominant],
+            "frac": dk["GBps"] / HBM_PEAK_GBPS,
+            "traffic": traffic,
+            "algorithmic_bytes": dk["alg_bytes"],
+            "avg_ms": dk["avg_ms"],
         },
-        "phases_ms": phase_ms,
+        "kernels": kernels,
         "step_algorithmic_bytes": b_alg,
         "step_hbm_alg_GBps": b_alg / (ms_per_step * 1e-3) / 1e9,
         "loss": float(step.loss.item()),
         "wall_s_timed": wall,
     }
+
+    if not args.no_plain:
+        from gft.plain import PlainTransportRenderStep
+        plain = PlainTransportRenderStep(inp["grid"], inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"],
+                                         inp["step"])
+        plain.run(d0, vel, target)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.plain_steps):
+            plain.run(d0, vel, target)
+        b.record()
+        torch.cuda.synchronize()
+        ms_plain = a.elapsed_time(b) / args.plain_steps
+        b_plain = plain.plain_bytes()
+        result["plain_fp32"] = {"ms_per_step": ms_plain, "value": nvs / (ms_plain * 1e-3), "steps": args.plain_steps,
+                                "plain_bytes": b_plain, "hbm_plain_GBps": b_plain / (ms_plain * 1e-3) / 1e9,
+                                "hbm_alg_GBps": b_alg / (ms_plain * 1e-3) / 1e9,
+                                "peak_allocated_GiB": torch.cuda.max_memory_allocated() / 2 ** 30}
+        result["vs_plain"] = ms_plain / ms_per_step
+
     if not args.no_cpu_baseline:
         tgt_np = target.cpu().numpy().astype(np.float64)
         secs, threads = cpu_baseline(inp, tgt_np)

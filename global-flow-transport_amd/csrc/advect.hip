@@ -242,12 +242,17 @@ static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel, float* f, float* out,
                              uint8_t* mask, const AdvectDims& a) {
   size_t n = size_t(a.d.x) * a.d.y * a.d.z;
-  hipLaunchKernelGGL(k_advect_sl, grid1(n), dim3(256), 0, s, dens, vel, f, a, n);
+  {
+    KernelTimer t(s, "advect_sl");
+    hipLaunchKernelGGL(k_advect_sl, grid1(n), dim3(256), 0, s, dens, vel, f, a, n);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.order == 2) {
+    KernelTimer t(s, "advect_mc");
     hipLaunchKernelGGL(k_advect_mc, grid1(n), dim3(256), 0, s, dens, vel, f, out, mask, a, n);
   } else {
+    KernelTimer t(s, "advect_o1_out");
     hipLaunchKernelGGL(k_advect_o1_out, grid1(n), dim3(256), 0, s, f, out, mask, a, n);
   }
   return hipGetLastError();
@@ -257,12 +262,17 @@ hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel,
                              const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a) {
   size_t n = size_t(a.d.x) * a.d.y * a.d.z;
   if (a.order == 2) {
-    hipLaunchKernelGGL(k_advect_bwd_a, grid1(n), dim3(256), 0, s, vel, f, mask, gout, gf_acc, gd, gv, a, n);
+    {
+      KernelTimer t(s, "advect_bwd_a");
+      hipLaunchKernelGGL(k_advect_bwd_a, grid1(n), dim3(256), 0, s, vel, f, mask, gout, gf_acc, gd, gv, a, n);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    KernelTimer t(s, "advect_bwd_b");
     hipLaunchKernelGGL(k_advect_bwd_b<false>, grid1(n), dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a, n);
   } else {
     // gd must be zero on entry (the caller clears it)
+    KernelTimer t(s, "advect_bwd_b");
     hipLaunchKernelGGL(k_advect_bwd_b<true>, grid1(n), dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a, n);
   }
   return hipGetLastError();

@@ -1,5 +1,7 @@
 // Shared host code of the C ABI: errors, 4x4 algebra in double precision and
 // the view-pair enumeration.
+#include <algorithm>
+
 #include "gft_host.hpp"
 
 namespace gft {
@@ -92,9 +94,117 @@ std::vector<ViewPair> view_pairs(int N, int num_cameras, bool separate_camera_ba
   return v;
 }
 
+// ------------------------------------------------------------ kernel timers
+// Per thread: the set of timed names ("*" = all) and, per name, the event
+// pairs recorded since gft_timer_start.
+struct TimerState {
+  std::string names;
+  bool all = false;
+  std::vector<std::string> order;
+  std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> events;
+  std::vector<hipEvent_t> pending;
+};
+static thread_local TimerState g_timer;
+
+static int timer_slot(const char* name) {
+  for (size_t i = 0; i < g_timer.order.size(); ++i)
+    if (g_timer.order[i] == name) return (int)i;
+  g_timer.order.push_back(name);
+  g_timer.events.emplace_back();
+  g_timer.pending.push_back(nullptr);
+  return (int)g_timer.order.size() - 1;
+}
+
+bool timer_active(const char* name) {
+  if (g_timer.all) return true;
+  if (g_timer.names.empty()) return false;
+  const std::string n(name);
+  size_t pos = 0;
+  while (pos <= g_timer.names.size()) {
+    size_t end = g_timer.names.find(',', pos);
+    if (end == std::string::npos) end = g_timer.names.size();
+    if (g_timer.names.compare(pos, end - pos, n) == 0) return true;
+    pos = end + 1;
+  }
+  return false;
+}
+
+void timer_record(hipStream_t s, const char* name, bool begin) {
+  int i = timer_slot(name);
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  if (hipEventRecord(e, s) != hipSuccess) {
+    (void)hipEventDestroy(e);
+    return;
+  }
+  if (begin) {
+    if (g_timer.pending[i]) (void)hipEventDestroy(g_timer.pending[i]);
+    g_timer.pending[i] = e;
+  } else if (g_timer.pending[i]) {
+    g_timer.events[i].push_back({g_timer.pending[i], e});
+    g_timer.pending[i] = nullptr;
+  } else {
+    (void)hipEventDestroy(e);
+  }
+}
+
+static void timer_release() {
+  for (auto& v : g_timer.events)
+    for (auto& p : v) {
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+  for (hipEvent_t e : g_timer.pending)
+    if (e) (void)hipEventDestroy(e);
+  g_timer = TimerState();
+}
+
 }  // namespace gft
 
 extern "C" {
+
+int gft_timer_start(const char* kernels) {
+  gft::timer_release();
+  if (!kernels) return gft::fail(GFT_ERR_INVALID_ARGUMENT, "null kernel list");
+  gft::g_timer.names = kernels;
+  gft::g_timer.all = gft::g_timer.names == "*";
+  return GFT_OK;
+}
+
+int gft_timer_read(const char* kernel, int32_t* launches, double* total_ms) {
+  if (!kernel || !launches || !total_ms) return gft::fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  *launches = 0;
+  *total_ms = 0.0;
+  for (size_t i = 0; i < gft::g_timer.order.size(); ++i) {
+    if (gft::g_timer.order[i] != kernel) continue;
+    for (auto& p : gft::g_timer.events[i]) {
+      int rc = gft::check_hip(hipEventSynchronize(p.second), "timer event");
+      if (rc) return rc;
+      float ms = 0.f;
+      if ((rc = gft::check_hip(hipEventElapsedTime(&ms, p.first, p.second), "timer elapsed"))) return rc;
+      *total_ms += ms;
+      *launches += 1;
+    }
+  }
+  return GFT_OK;
+}
+
+int gft_timer_names(char* buf, size_t buf_bytes) {
+  std::string all;
+  for (size_t i = 0; i < gft::g_timer.order.size(); ++i) {
+    if (i) all += ",";
+    all += gft::g_timer.order[i];
+  }
+  if (buf && buf_bytes) {
+    size_t n = std::min(all.size(), buf_bytes - 1);
+    all.copy(buf, n);
+    buf[n] = 0;
+  }
+  return (int)all.size();
+}
+
+void gft_timer_stop(void) { gft::timer_release(); }
+
 
 const char* gft_last_error(void) { return gft::g_last_error.c_str(); }
 

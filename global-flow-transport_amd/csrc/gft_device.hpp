@@ -401,6 +401,24 @@ struct LutLaunch {
   int batch0;
 };
 
+// ------------------------------------------------------ kernel timers
+// Optional HIP-event timing of named launches (gft_timer_* in the C ABI).
+// A scope records an event before and after the launches it encloses when
+// its name is being timed on the calling thread; otherwise it does nothing.
+bool timer_active(const char* name);
+void timer_record(hipStream_t s, const char* name, bool begin);
+struct KernelTimer {
+  hipStream_t s;
+  const char* name;
+  bool on;
+  KernelTimer(hipStream_t s_, const char* n) : s(s_), name(n), on(timer_active(n)) {
+    if (on) timer_record(s, name, true);
+  }
+  ~KernelTimer() {
+    if (on) timer_record(s, name, false);
+  }
+};
+
 // idx -> ndc -> view (divide by w) -> object space (transformations_v2.hpp:166-179)
 __device__ __forceinline__ float3 idx_to_os(const RayView& v, float ix, float iy, float iz, float dinvx, float dinvy,
                                             float dinvz) {

@@ -578,6 +578,7 @@ static dim3 tile_grid(const RayDims& rd, int views) {
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
                                      const RayLaunch& L, const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
+  KernelTimer t(s, "render_fwd");
   switch (bm) {
     case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp); break;
     case B_CLAMP: hipLaunchKernelGGL(k_render_density_fwd<B_CLAMP>, grid, dim3(256), 0, s, dens, img, rd, L, rp); break;
@@ -591,6 +592,7 @@ hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* dens,
                                       const float* second, float* gs, float4* rays, float* loss, const RayDims& rd,
                                       const RayLaunch& L, const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
+  KernelTimer t(s, "render_bwd_walk");
   if (l2) hipLaunchKernelGGL(k_render_density_bwd_walk<true>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp);
   else hipLaunchKernelGGL(k_render_density_bwd_walk<false>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp);
   return hipGetLastError();
@@ -599,6 +601,7 @@ hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* dens,
 hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
                                         const RayDims& rd, const ProjLaunch& PL) {
   const dim3 grid((rd.in.x + 63) / 64, (rd.in.y + 3) / 4, rd.in.z);
+  KernelTimer t(s, "render_bwd_gather");
   hipLaunchKernelGGL(k_render_density_bwd_gather, grid, dim3(256), 0, s, gs, rays, grad, rd, PL);
   return hipGetLastError();
 }
@@ -608,6 +611,7 @@ hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const f
                                          const float* second, float* grad, float* loss, const RayDims& rd,
                                          const RayLaunch& L, const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
+  KernelTimer t(s, "render_bwd_scatter");
   if (bm == B_CLAMP) {
     if (l2) hipLaunchKernelGGL(k_render_density_bwd_clamp<true>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
     else hipLaunchKernelGGL(k_render_density_bwd_clamp<false>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);

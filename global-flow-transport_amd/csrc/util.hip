@@ -15,6 +15,7 @@ __global__ __launch_bounds__(256) void k_fill1(float* __restrict__ p, size_t n, 
 
 hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v) {
   if (n == 0) return hipSuccess;
+  KernelTimer t(s, "fill");
   size_t head = 0;
   if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
     size_t n4 = n / 4;

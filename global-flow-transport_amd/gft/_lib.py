@@ -80,6 +80,10 @@ _SIGNATURES = {
     "gft_lod_transform": (ctypes.c_int, [_vp, _i32p, _vp, _vp, _vp, _i32, _i32p, _i32, _vp]),
     "gft_reduce_grid_blend_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _i32, _i32, _vp]),
     "gft_reduce_grid_blend_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _i32, _i32, _i32, _vp]),
+    "gft_timer_start": (ctypes.c_int, [ctypes.c_char_p]),
+    "gft_timer_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
+    "gft_timer_names": (ctypes.c_int, [ctypes.c_char_p, _sz]),
+    "gft_timer_stop": (None, []),
 }
 
 
@@ -131,3 +135,37 @@ def stream_of(tensor):
 
 def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+class KernelTimers(object):
+    """HIP-event timing of the library's named kernel launches on this thread
+    (gft_timer_*): ``with KernelTimers("*") as t: ...`` then ``t.read()``
+    -> {name: (launches, total_ms)}."""
+
+    def __init__(self, kernels="*"):
+        self.kernels = kernels
+        self.result = None
+
+    def __enter__(self):
+        check(lib().gft_timer_start(self.kernels.encode()))
+        return self
+
+    def __exit__(self, *exc):
+        try:
+            if exc[0] is None:
+                self.result = self.read()
+        finally:
+            lib().gft_timer_stop()
+        return False
+
+    def read(self):
+        L = lib()
+        n = L.gft_timer_names(None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        L.gft_timer_names(buf, n + 1)
+        out = {}
+        for name in [x for x in buf.value.decode().split(",") if x]:
+            cnt, tot = ctypes.c_int32(0), ctypes.c_double(0.0)
+            check(L.gft_timer_read(name.encode(), ctypes.byref(cnt), ctypes.byref(tot)))
+            out[name] = (int(cnt.value), float(tot.value))
+        return out

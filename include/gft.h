@@ -260,6 +260,21 @@ int gft_reduce_grid_blend_bwd(void* stream, const float* input, const int64_t sh
                               const float* output_grad, int32_t blending_mode, int32_t keep_dims, int32_t grad_mode,
                               float* input_grad);
 
+/*
+ * Kernel timers (instrumentation; no counterpart in the reference, which
+ * has none). gft_timer_start("a,b" or "*") makes every later launch of the
+ * named kernels on the calling thread record a HIP event before and after
+ * it on its stream; gft_timer_read synchronises those events and returns
+ * the launch count and summed duration of one name; gft_timer_names lists
+ * the names seen (returns the full length); gft_timer_stop frees the events.
+ * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
+ * render_fwd, render_bwd_walk, render_bwd_gather, render_bwd_scatter, fill.
+ */
+int gft_timer_start(const char* kernels);
+int gft_timer_read(const char* kernel, int32_t* launches, double* total_ms);
+int gft_timer_names(char* buf, size_t buf_bytes);
+void gft_timer_stop(void);
+
 #ifdef __cplusplus
 }
 #endif
