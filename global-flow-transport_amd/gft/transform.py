@@ -125,6 +125,24 @@ class GridTransform(Transform):
         M = M_norm_scale @ M if self.normalize != 'NONE' else M
         return M_trans @ (M_rot @ M)
 
+    @classmethod
+    def from_transform(cls, transform, grid_size, center=False, normalize='NONE'):
+        """transform.py:211-212"""
+        return cls(grid_size, transform.translation, transform.rotation_deg, transform.scale, center, normalize,
+                   transform.parent)
+
+    def grid_size_world(self):
+        """World-space edge lengths of the grid along its x, y, z axes
+        (transform.py:273-278)."""
+        D, H, W = [float(v) for v in self.grid_size]
+        M = np.asarray(self.get_transform_matrix(), dtype=np.float64)
+        return np.array([np.linalg.norm(M[:3, 0] * W), np.linalg.norm(M[:3, 1] * H), np.linalg.norm(M[:3, 2] * D)])
+
+    def cell_size_world(self):
+        """World-space size of one cell along x, y, z (transform.py:280-284)."""
+        M = np.asarray(self.get_transform_matrix(), dtype=np.float64)
+        return np.array([np.linalg.norm(M[:3, 0]), np.linalg.norm(M[:3, 1]), np.linalg.norm(M[:3, 2])])
+
     def set_data(self, data):
         shape = list(data.shape)
         self.grid_size = shape[1:4]

@@ -367,3 +367,99 @@ def sample_transform_gather_grad(output_grad, M, V, P, F, in_dhw, interpolation=
                                    mipmapping, num_mipmaps, mip_bias, rev, separate_camera_batch=False, prec=prec)
         out.append(dxv.sum(axis=(0, 1)))
     return np.stack(out)
+
+
+# ----------------------------------------------- staggered velocity (A10)
+_FACE_OFFSETS = {  # (component sampled, target component) -> offset xyz (data_structures.py:835-862)
+    ("X", "Y"): (0.5, -0.5, 0.0), ("X", "Z"): (0.5, 0.0, -0.5),
+    ("Y", "X"): (-0.5, 0.5, 0.0), ("Y", "Z"): (0.0, 0.5, -0.5),
+    ("Z", "X"): (-0.5, 0.0, 0.5), ("Z", "Y"): (0.0, -0.5, 0.5),
+}
+
+
+def _offset_lut(shape_dhw, offset):
+    lut = np.zeros((1,) + tuple(shape_dhw) + (4,))
+    lut[..., 0], lut[..., 1], lut[..., 2] = offset
+    return lut
+
+
+def face_velocity(comps, target, prec=1):
+    """Velocity at the faces of component ``target``: its own values and the
+    other two resampled with half-cell offsets (LINEAR, CLAMP, relative LuT)."""
+    names = ("X", "Y", "Z")
+    tshape = comps[names.index(target)].shape[1:4]
+    out = []
+    for n, c in zip(names, comps):
+        if n == target:
+            out.append(_d(c))
+        else:
+            out.append(sample_lut_fwd(c, _offset_lut(tshape, _FACE_OFFSETS[(n, target)]), boundary="CLAMP",
+                                      separate_camera_batch=False, relative_coords=True, prec=prec)[:, 0])
+    return np.concatenate(out, axis=-1)
+
+
+def face_velocity_bwd(comps, target, g_faces, prec=1):
+    """Adjoint of face_velocity (EXACT LuT input adjoints) -> grads per component."""
+    names = ("X", "Y", "Z")
+    tshape = comps[names.index(target)].shape[1:4]
+    grads = []
+    for i, (n, c) in enumerate(zip(names, comps)):
+        g = _d(g_faces[..., i:i + 1])
+        if n == target:
+            grads.append(g)
+        else:
+            gin, _ = sample_lut_bwd(c, _offset_lut(tshape, _FACE_OFFSETS[(n, target)]), g[:, None], boundary="CLAMP",
+                                    separate_camera_batch=False, relative_coords=True, grad_mode="EXACT", prec=prec)
+            grads.append(gin)
+    return grads
+
+
+def velocity_self_advect(comps, order=2, clamp="MC_SMOOTH", dt=1.0, prec=1):
+    """VelocityGrid.warped() with no external velocity (data_structures.py:594-651):
+    each component warped by the velocity at its own faces; only the
+    MacCormack clamps apply to components."""
+    c = clamp if clamp in ("MC", "MC_SMOOTH") else "NONE"
+    return [advect_fwd(comp, face_velocity(comps, n, prec), order, c, dt, "CLAMP", prec)
+            for n, comp in zip(("X", "Y", "Z"), comps)]
+
+
+def velocity_self_advect_bwd(comps, out_grads, order=2, clamp="MC_SMOOTH", dt=1.0, prec=1):
+    """Exact gradient of sum_c <out_grads[c], warped_c> w.r.t. the components."""
+    c = clamp if clamp in ("MC", "MC_SMOOTH") else "NONE"
+    total = [np.zeros_like(_d(x)) for x in comps]
+    for i, (n, comp) in enumerate(zip(("X", "Y", "Z"), comps)):
+        vf = face_velocity(comps, n, prec)
+        gd, gv = advect_bwd(comp, vf, out_grads[i], order, c, dt, "CLAMP", "EXACT", prec)
+        total[i] += gd
+        for j, g in enumerate(face_velocity_bwd(comps, n, gv, prec)):
+            total[j] += g
+    return total
+
+
+# ------------------------------------------------- transport chain (A12)
+def chain_fwd_bwd(d0, velocities, targets, M, V, P, F, out_dhw, depth_step, order=2, clamp="MC_SMOOTH",
+                  grad_mode="EXACT", decay=0.0, light=(1.0, 1.0, 1.0), prec=1):
+    """Density chain d_{f+1} = advect(d_f, v_f), loss = sum_f mean((render(d_f)
+    - target_f)^2) and the backward of optStep_sequence in BWD order
+    (optimization.py:592-602,974-1047): EXACT accumulates lambda_f = g_f +
+    A_f^T lambda_{f+1}; REFERENCE with decay a > 0 uses lambda_f = (1-a) g_f
+    + a A_f^T lambda_{f+1} (lambda_{T-1} = (1-a) g_{T-1}) and with a = 0 the
+    plain sum; grad d0 = g_0 + A_0^T lambda_1."""
+    T = len(targets)
+    ds = [_d(d0)]
+    for f in range(T - 1):
+        ds.append(advect_fwd(ds[-1], velocities[f], order, clamp, 1.0, "CLAMP", prec))
+    losses, g_own = [], []
+    for f in range(T):
+        img = render_density_fwd(ds[f], M, V, P, F, out_dhw, depth_step, light, prec=prec)
+        loss, gimg = l2_loss_and_grad(img, _d(targets[f]))
+        losses.append(loss)
+        g_own.append(render_density_bwd(ds[f], img, gimg, M, V, P, F, out_dhw, depth_step, light,
+                                        grad_mode="EXACT", prec=prec))
+    exact = grad_mode == "EXACT" or decay <= 0  # the reference sums (l + d) when the decay is 0
+    lam = g_own[-1] if exact else (1.0 - decay) * g_own[-1]
+    gv = [None] * (T - 1)
+    for f in range(T - 2, -1, -1):
+        gd, gv[f] = advect_bwd(ds[f], velocities[f], lam, order, clamp, 1.0, "CLAMP", "EXACT", prec)
+        lam = g_own[f] + gd if (exact or f == 0) else (1.0 - decay) * g_own[f] + decay * gd
+    return dict(loss=float(np.sum(losses)), losses=losses, gd0=lam, gv=gv, densities=ds)

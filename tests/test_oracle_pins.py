@@ -559,3 +559,96 @@ def test_transport_render_step_gradient_fd():
         return orc.step_fwd_bwd(dd, v, tgt, *args, order=1, clamp="NONE")["loss"]
     for idx in np.argsort(-np.abs(r["gd"]).ravel())[:5]:
         assert abs(_fd(loss_d, d0, int(idx), 1e-6) - r["gd"].flat[idx]) <= 1e-5 * np.abs(r["gd"]).max()
+
+
+# ------------------------------------------------- staggered velocity (A10)
+def _stag_shapes(D, H, W):
+    return [(1, D, H, W + 1, 1), (1, D, H + 1, W, 1), (1, D + 1, H, W, 1)]
+
+
+def test_face_velocity_interpolates_linear_components_at_the_faces():
+    """Each component linear in its own face coordinates: resampled to the
+    faces of another component it must equal the same linear function at
+    those faces (x-face (i,j,k) sits at centred (i-0.5, j, k), y-face at
+    (i, j-0.5, k), z-face at (i, j, k-0.5))."""
+    D, H, W = 6, 7, 8
+    a = {"X": (0.3, -0.2, 0.1), "Y": (-0.4, 0.25, 0.05), "Z": (0.15, 0.1, -0.3)}
+    shift = {"X": (-0.5, 0, 0), "Y": (0, -0.5, 0), "Z": (0, 0, -0.5)}
+
+    def field(name, shape):
+        z, y, x = np.meshgrid(np.arange(shape[1]), np.arange(shape[2]), np.arange(shape[3]), indexing="ij")
+        s = shift[name]
+        cx, cy, cz = x + s[0], y + s[1], z + s[2]  # centred coordinates of the faces
+        return (a[name][0] * cx + a[name][1] * cy + a[name][2] * cz)[None, ..., None]
+    comps = [field(n, s) for n, s in zip("XYZ", _stag_shapes(D, H, W))]
+    for ti, target in enumerate("XYZ"):
+        fv = orc.face_velocity(comps, target)
+        shape = _stag_shapes(D, H, W)[ti]
+        z, y, x = np.meshgrid(np.arange(shape[1]), np.arange(shape[2]), np.arange(shape[3]), indexing="ij")
+        s = shift[target]
+        cx, cy, cz = x + s[0], y + s[1], z + s[2]
+        for ci, n in enumerate("XYZ"):
+            expect = a[n][0] * cx + a[n][1] * cy + a[n][2] * cz
+            # away from the CLAMP edges, where the half-cell offset leaves the component's grid
+            inner = (slice(1, -1),) * 3
+            assert np.allclose(fv[0, ..., ci][inner], expect[inner], atol=1e-12), (target, n)
+
+
+def test_velocity_self_advection_zero_and_uniform():
+    D, H, W = 5, 6, 7
+    zero = [np.zeros(s) for s in _stag_shapes(D, H, W)]
+    out = orc.velocity_self_advect(zero, 2, "MC_SMOOTH")
+    assert all(np.all(o == 0) for o in out)
+    # a uniform velocity (1, 0, 0) cell/step: every component field is constant, so it is unchanged
+    uni = [np.full(s, v) for s, v in zip(_stag_shapes(D, H, W), (1.0, 0.0, 0.0))]
+    out = orc.velocity_self_advect(uni, 2, "MC_SMOOTH")
+    assert all(np.allclose(o, u, atol=1e-14) for o, u in zip(out, uni))
+
+
+def test_velocity_self_advection_exact_gradient_fd():
+    D, H, W = 4, 5, 6
+    comps = [RNG.uniform(-0.6, 0.6, size=s) for s in _stag_shapes(D, H, W)]
+    ws = [RNG.uniform(-1, 1, size=s) for s in _stag_shapes(D, H, W)]
+    g = orc.velocity_self_advect_bwd(comps, ws, 2, "NONE")
+
+    def loss(cs):
+        return float(sum((o * w).sum() for o, w in zip(orc.velocity_self_advect(cs, 2, "NONE"), ws)))
+    h = 1e-6
+    gmax = max(np.abs(x).max() for x in g)
+    for ci in range(3):
+        for idx in RNG.integers(0, comps[ci].size, 6):
+            cp = [c.copy() for c in comps]
+            cm = [c.copy() for c in comps]
+            cp[ci].flat[idx] += h
+            cm[ci].flat[idx] -= h
+            fd = (loss(cp) - loss(cm)) / (2 * h)
+            assert abs(fd - g[ci].flat[idx]) <= 1e-5 * gmax
+
+
+# --------------------------------------------------- transport chain (A12)
+def test_chain_exact_gradient_is_the_fd_of_the_summed_loss():
+    from helpers import scene
+    sc = scene(grid=(5, 7, 6), image=(5, 4), views=2, seed=13)
+    d0 = sc["data"].astype(np.float64)
+    T = 3
+    vels = [RNG.uniform(-0.7, 0.7, size=d0.shape[:-1] + (3,)) for _ in range(T - 1)]
+    tg = [RNG.uniform(0, 0.4, size=(1, 2) + tuple(sc["out_dhw"][1:]) + (3,)) for _ in range(T)]
+    args = (sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"])
+    r = orc.chain_fwd_bwd(d0, vels, tg, *args, order=1, clamp="NONE")
+    assert len(r["losses"]) == T
+
+    def loss_d(dd):
+        return orc.chain_fwd_bwd(dd, vels, tg, *args, order=1, clamp="NONE")["loss"]
+    for idx in np.argsort(-np.abs(r["gd0"]).ravel())[:4]:
+        assert abs(_fd(loss_d, d0, int(idx), 1e-6) - r["gd0"].flat[idx]) <= 1e-5 * np.abs(r["gd0"]).max()
+    for f in range(T - 1):
+        def loss_v(vv, f=f):
+            vs = list(vels)
+            vs[f] = vv
+            return orc.chain_fwd_bwd(d0, vs, tg, *args, order=1, clamp="NONE")["loss"]
+        for idx in np.argsort(-np.abs(r["gv"][f]).ravel())[:3]:
+            assert abs(_fd(loss_v, vels[f], int(idx), 1e-6) - r["gv"][f].flat[idx]) <= \
+                1e-5 * np.abs(r["gv"][f]).max()
+    # the decayed (reference) recursion equals the exact one at decay 0
+    r0 = orc.chain_fwd_bwd(d0, vels, tg, *args, order=1, clamp="NONE", grad_mode="REFERENCE", decay=0.0)
+    assert np.allclose(r0["gd0"], r["gd0"]) and all(np.allclose(a, b) for a, b in zip(r0["gv"], r["gv"]))
