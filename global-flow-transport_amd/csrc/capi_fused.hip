@@ -23,6 +23,13 @@ hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const f
                                          const float* second, float* grad, float* loss, const RayDims& rd,
                                          const RayLaunch& L, const RenderParams& rp);
 hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v);
+hipError_t quad_copy_launch(hipStream_t s, const float* lin, float* out, Dims3 d);
+size_t quad_copy_floats(Dims3 d);
+
+// Bytes of the corner-quad density copy the BORDER render kernels read.
+static size_t quad_bytes(const int64_t* shape) {
+  return (quad_copy_floats(Dims3{(int)shape[3], (int)shape[2], (int)shape[1]}) * sizeof(float) + 255) & ~size_t(255);
+}
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -145,17 +152,30 @@ int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], con
                    "advect backward");
 }
 
+size_t gft_render_density_fwd_workspace(const int64_t shape[5], int32_t boundary) {
+  return boundary == GFT_BORDER ? quad_bytes(shape) : 0;
+}
+
 int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                           float* images) {
+                           void* workspace, size_t workspace_bytes, float* images) {
   (void)frustum;
   int rc;
   if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
   if (!density || !images || !M || !V || !P || !light) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  const size_t need = gft_render_density_fwd_workspace(shape, boundary);
+  if (need && (!workspace || workspace_bytes < need))
+    return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
+                need);
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
   auto pairs = view_pairs(1, num_cameras, true);
+  if (boundary == GFT_BORDER) {
+    if ((rc = check_hip(quad_copy_launch((hipStream_t)stream, density, (float*)workspace, rd.in), "quad copy")))
+      return rc;
+    density = (const float*)workspace;
+  }
   for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
     size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
     RayLaunch L;
@@ -169,11 +189,10 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
 
 size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                         int32_t boundary) {
-  (void)shape;
   if (boundary != GFT_BORDER) return 0;
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
   return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
-         align256(plane * num_cameras * 2 * sizeof(float4));
+         align256(plane * num_cameras * 2 * sizeof(float4)) + quad_bytes(shape);
 }
 
 static int render_bwd_common(void* stream, const float* density, const int64_t shape[5], const float* M,
@@ -197,14 +216,17 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
   if (boundary == GFT_BORDER) {
     const size_t plane = (size_t)output_shape[1] * output_shape[2];
     float* gs = (float*)workspace;
-    float4* rays = (float4*)((char*)workspace + align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)));
+    const size_t gs_bytes = align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float));
+    float4* rays = (float4*)((char*)workspace + gs_bytes);
+    float* quads = (float*)((char*)workspace + gs_bytes + align256(plane * num_cameras * 2 * sizeof(float4)));
+    if ((rc = check_hip(quad_copy_launch(s, density, quads, rd.in), "quad copy"))) return rc;
     for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
       size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
       RayLaunch L;
       ProjLaunch PL;
       if ((rc = ray_views(pairs, first, cnt, M, V, P, L, &PL))) return rc;
       PL.accumulate = first > 0;
-      if ((rc = check_hip(render_density_walk_launch(l2, s, density, images, second, gs, rays, loss, rd, L, rp),
+      if ((rc = check_hip(render_density_walk_launch(l2, s, quads, images, second, gs, rays, loss, rd, L, rp),
                           "render backward walk")))
         return rc;
       if ((rc = check_hip(render_density_gather_launch(s, gs, rays, density_grad, rd, PL), "render backward gather")))

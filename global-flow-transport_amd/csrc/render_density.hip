@@ -73,44 +73,45 @@ __device__ __forceinline__ void clip_ray(float3 n, float3 st, Dims3 d, int S, in
   }
 }
 
-// BORDER trilinear read of a scalar grid at p (cell-centre offset removed).
-// Returns false when the sample is outside [-0.5, dims-0.5) (it reads 0).
-// Interior footprints use 32-bit offsets and no per-corner checks.
-__device__ __forceinline__ bool read_border(const float* __restrict__ buf, Dims3 d, float px, float py, float pz,
-                                            float& val) {
-  const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
-  if (!((0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) & (qz < (float)d.z)))
-    return false;
-  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
-  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-  if (((unsigned)ix < (unsigned)(d.x - 1)) & ((unsigned)iy < (unsigned)(d.y - 1)) &
-      ((unsigned)iz < (unsigned)(d.z - 1))) {
-    const float wx = px - fx, wy = py - fy, wz = pz - fz;
-    const unsigned sx = (unsigned)d.x, sxy = (unsigned)d.x * (unsigned)d.y;
-    const unsigned b = ((unsigned)iz * (unsigned)d.y + (unsigned)iy) * sx + (unsigned)ix;
-    const float c000 = buf[b], c001 = buf[b + 1], c010 = buf[b + sx], c011 = buf[b + sx + 1];
-    const float c100 = buf[b + sxy], c101 = buf[b + sxy + 1], c110 = buf[b + sxy + sx], c111 = buf[b + sxy + sx + 1];
-    const float v0 = lerpf(lerpf(c000, c001, wx), lerpf(c010, c011, wx), wy);
-    const float v1 = lerpf(lerpf(c100, c101, wx), lerpf(c110, c111, wx), wy);
-    val = lerpf(v0, v1, wz);
-  } else {
-    val = read_linear<1, B_BORDER>(buf, d, px, py, pz).v[0];
-  }
-  return true;
+// Corner-quad copy of a scalar grid for BORDER trilinear taps: entry
+// (x, y, z) for x in [-1, W-1], y in [-1, H-1], z in [-1, D] holds the float4
+// (v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z)) with v = 0 outside the grid
+// (the BORDER value of an invalid corner). A trilinear tap is then two 16-byte
+// loads (slices z and z+1) instead of eight 4-byte gathers, and needs no
+// per-corner validity test.
+struct QuadDims {
+  unsigned qx, qy, qz;  // W+1, H+1, D+2
+};
+__host__ __device__ inline QuadDims make_quad_dims(Dims3 d) {
+  return QuadDims{(unsigned)d.x + 1u, (unsigned)d.y + 1u, (unsigned)d.z + 2u};
+}
+
+__global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin, float4* __restrict__ quad, Dims3 d,
+                                                   QuadDims q) {
+  const size_t n = (size_t)q.qx * q.qy * q.qz;
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int x = (int)(j % q.qx) - 1, y = (int)((j / q.qx) % q.qy) - 1, z = (int)(j / ((size_t)q.qx * q.qy)) - 1;
+  auto v = [&](int xx, int yy) -> float {
+    return ((unsigned)xx < (unsigned)d.x) & ((unsigned)yy < (unsigned)d.y) & ((unsigned)z < (unsigned)d.z)
+               ? lin[flat3(xx, yy, z, d)]
+               : 0.f;
+  };
+  quad[j] = make_float4(v(x, y), v(x + 1, y), v(x, y + 1), v(x + 1, y + 1));
 }
 
 // Branch-free BORDER read of U consecutive samples i0, i0+dir, ... of one ray
-// (dir = +1 or -1): all 8U corner loads are issued before any value is used,
-// so a lane keeps 8U loads in flight. Samples outside [-0.5, dims-0.5) or past
-// `last` (in walk direction) read 0 and report inside = false. Corner indices
-// are clamped into the grid for the address and masked for BORDER validity.
+// (dir = +1 or -1) from the corner-quad copy: all 2U loads are issued before
+// any value is used. Samples outside [-0.5, dims-0.5) or past `last` (in walk
+// direction) read 0 and report inside = false; their addresses are clamped
+// into the copy.
 template <int U>
-__device__ __forceinline__ void read_border_batch(const float* __restrict__ buf, Dims3 d, float3 n, float3 st,
-                                                  int i0, int dir, int last, float val[U], bool inside[U]) {
-  const unsigned sx = (unsigned)d.x, sxy = (unsigned)d.x * (unsigned)d.y;
-  unsigned a[U][8];
+__device__ __forceinline__ void read_border_batch(const float4* __restrict__ quad, Dims3 d, QuadDims q, float3 n,
+                                                  float3 st, int i0, int dir, int last, float val[U],
+                                                  bool inside[U]) {
+  unsigned a[U];
   float w[U][3];
-  unsigned m[U];  // bit k: corner k valid
+  const unsigned slice = q.qx * q.qy;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int i = i0 + dir * u;
@@ -124,32 +125,20 @@ __device__ __forceinline__ void read_border_batch(const float* __restrict__ buf,
     w[u][0] = px - fx;
     w[u][1] = py - fy;
     w[u][2] = pz - fz;
-    const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-    const bool vx0 = ix >= 0, vx1 = ix + 1 < d.x, vy0 = iy >= 0, vy1 = iy + 1 < d.y, vz0 = iz >= 0,
-               vz1 = iz + 1 < d.z;
-    const unsigned x0 = (unsigned)clampi(ix, 0, d.x - 1), x1 = (unsigned)clampi(ix + 1, 0, d.x - 1);
-    const unsigned y0 = (unsigned)clampi(iy, 0, d.y - 1), y1 = (unsigned)clampi(iy + 1, 0, d.y - 1);
-    const unsigned z0 = (unsigned)clampi(iz, 0, d.z - 1), z1 = (unsigned)clampi(iz + 1, 0, d.z - 1);
-    const unsigned r00 = z0 * sxy + y0 * sx, r01 = z0 * sxy + y1 * sx, r10 = z1 * sxy + y0 * sx,
-                   r11 = z1 * sxy + y1 * sx;
-    a[u][0] = r00 + x0; a[u][1] = r00 + x1; a[u][2] = r01 + x0; a[u][3] = r01 + x1;
-    a[u][4] = r10 + x0; a[u][5] = r10 + x1; a[u][6] = r11 + x0; a[u][7] = r11 + x1;
-    m[u] = (unsigned)(vx0 & vy0 & vz0) | ((unsigned)(vx1 & vy0 & vz0) << 1) | ((unsigned)(vx0 & vy1 & vz0) << 2) |
-           ((unsigned)(vx1 & vy1 & vz0) << 3) | ((unsigned)(vx0 & vy0 & vz1) << 4) |
-           ((unsigned)(vx1 & vy0 & vz1) << 5) | ((unsigned)(vx0 & vy1 & vz1) << 6) |
-           ((unsigned)(vx1 & vy1 & vz1) << 7);
+    const unsigned ix = (unsigned)(clampi((int)fx, -1, d.x - 1) + 1), iy = (unsigned)(clampi((int)fy, -1, d.y - 1) + 1),
+                   iz = (unsigned)(clampi((int)fz, -1, d.z - 1) + 1);
+    a[u] = (iz * q.qy + iy) * q.qx + ix;
   }
-  float c[U][8];
-#pragma unroll
-  for (int u = 0; u < U; ++u)
-#pragma unroll
-    for (int k = 0; k < 8; ++k) c[u][k] = buf[a[u][k]];
+  float4 c0[U], c1[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
+    c0[u] = quad[a[u]];
+    c1[u] = quad[a[u] + slice];
+  }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) c[u][k] = ((m[u] >> k) & 1u) ? c[u][k] : 0.f;
-    const float v0 = lerpf(lerpf(c[u][0], c[u][1], w[u][0]), lerpf(c[u][2], c[u][3], w[u][0]), w[u][1]);
-    const float v1 = lerpf(lerpf(c[u][4], c[u][5], w[u][0]), lerpf(c[u][6], c[u][7], w[u][0]), w[u][1]);
+  for (int u = 0; u < U; ++u) {
+    const float v0 = lerpf(lerpf(c0[u].x, c0[u].y, w[u][0]), lerpf(c0[u].z, c0[u].w, w[u][0]), w[u][1]);
+    const float v1 = lerpf(lerpf(c1[u].x, c1[u].y, w[u][0]), lerpf(c1[u].z, c1[u].w, w[u][0]), w[u][1]);
     val[u] = inside[u] ? lerpf(v0, v1, w[u][2]) : 0.f;
   }
 }
@@ -157,9 +146,11 @@ __device__ __forceinline__ void read_border_batch(const float* __restrict__ buf,
 constexpr int kBatch = 4;
 
 // ----------------------------------------------------------------- forward
+// BORDER reads `dens` as the corner-quad copy (float4, QuadDims q); CLAMP
+// and WRAP read the x-fastest grid.
 template <int BM>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
-                                                            RayDims rd, RayLaunch L, RenderParams rp) {
+                                                            RayDims rd, RayLaunch L, RenderParams rp, QuadDims q) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
@@ -174,7 +165,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     for (int i = lo; i <= hi; i += kBatch) {
       float val[kBatch];
       bool in[kBatch];
-      read_border_batch<kBatch>(buf, rd.in, nearp, step, i, 1, hi, val, in);
+      read_border_batch<kBatch>(reinterpret_cast<const float4*>(dens), rd.in, q, nearp, step, i, 1, hi, val, in);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         // samples outside the grid read 0 and leave D and E unchanged
@@ -232,7 +223,7 @@ __device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, f
 }
 
 // -------------------------------------------------- BORDER: walk + gather
-// Walk: stores g_d of every inside sample at gs[((slot*S + i)*H + y)*W + x]
+// Walk (reads the corner-quad copy): stores g_d of every inside sample at gs[((slot*S + i)*H + y)*W + x]
 // and the ray of every pixel at rays[2*((slot*H + y)*W + x) + {0,1}] =
 // (near, lo), (step, hi); lo > hi marks a pixel with no stored sample.
 template <bool L2>
@@ -241,7 +232,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
                                                                  const float* __restrict__ second,
                                                                  float* __restrict__ gs, float4* __restrict__ rays,
                                                                  float* __restrict__ loss, RayDims rd, RayLaunch L,
-                                                                 RenderParams rp) {
+                                                                 RenderParams rp, QuadDims q) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
@@ -267,7 +258,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
     for (int i = hi; i >= lo; i -= kBatch) {
       float val[kBatch];
       bool in[kBatch];
-      read_border_batch<kBatch>(buf, rd.in, nearp, step, i, -1, lo, val, in);
+      read_border_batch<kBatch>(reinterpret_cast<const float4*>(dens), rd.in, q, nearp, step, i, -1, lo, val, in);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (!in[u]) continue;  // the sample read 0: nothing stored, D and g_D unchanged
@@ -575,26 +566,43 @@ static dim3 tile_grid(const RayDims& rd, int views) {
   return dim3((rd.out_w + kRTileX - 1) / kRTileX, (rd.out_h + kRTileY - 1) / kRTileY, views);
 }
 
+size_t quad_copy_floats(Dims3 d) {
+  const QuadDims q = make_quad_dims(d);
+  return (size_t)q.qx * q.qy * q.qz * 4;
+}
+
+hipError_t quad_copy_launch(hipStream_t s, const float* lin, float* out, Dims3 d) {
+  const QuadDims q = make_quad_dims(d);
+  const size_t n = (size_t)q.qx * q.qy * q.qz;
+  KernelTimer t(s, "quad_copy");
+  hipLaunchKernelGGL(k_quad_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin,
+                     reinterpret_cast<float4*>(out), d, q);
+  return hipGetLastError();
+}
+
+// BORDER: dens is the corner-quad copy.
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
                                      const RayLaunch& L, const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
+  const QuadDims q = make_quad_dims(rd.in);
   KernelTimer t(s, "render_fwd");
   switch (bm) {
-    case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp); break;
-    case B_CLAMP: hipLaunchKernelGGL(k_render_density_fwd<B_CLAMP>, grid, dim3(256), 0, s, dens, img, rd, L, rp); break;
-    default: hipLaunchKernelGGL(k_render_density_fwd<B_WRAP>, grid, dim3(256), 0, s, dens, img, rd, L, rp); break;
+    case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q); break;
+    case B_CLAMP: hipLaunchKernelGGL(k_render_density_fwd<B_CLAMP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q); break;
+    default: hipLaunchKernelGGL(k_render_density_fwd<B_WRAP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q); break;
   }
   return hipGetLastError();
 }
 
-// BORDER walk (writes gs, rays) for one chunk of views
+// BORDER walk (reads the corner-quad copy, writes gs, rays) for one chunk of views
 hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* dens, const float* img,
                                       const float* second, float* gs, float4* rays, float* loss, const RayDims& rd,
                                       const RayLaunch& L, const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
+  const QuadDims q = make_quad_dims(rd.in);
   KernelTimer t(s, "render_bwd_walk");
-  if (l2) hipLaunchKernelGGL(k_render_density_bwd_walk<true>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp);
-  else hipLaunchKernelGGL(k_render_density_bwd_walk<false>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp);
+  if (l2) hipLaunchKernelGGL(k_render_density_bwd_walk<true>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp, q);
+  else hipLaunchKernelGGL(k_render_density_bwd_walk<false>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp, q);
   return hipGetLastError();
 }
 

@@ -64,6 +64,8 @@ class TransportChain(object):
         self.images = torch.empty((1, nV, h, w, 4), device=dev)
         self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
         self.render_ws = torch.empty((max(self._nrws, 1),), dtype=torch.uint8, device=dev)
+        self._nfws = L.gft_render_density_fwd_workspace(self._shape, self.boundary)
+        self.render_fwd_ws = torch.empty((max(self._nfws, 1),), dtype=torch.uint8, device=dev)
         self.frame_loss = None
 
     # ---------------------------------------------------------------- kernels
@@ -82,7 +84,8 @@ class TransportChain(object):
         L, c, s = self._L, self.cams, _lib.stream_of(d)
         _lib.check(L.gft_render_density_fwd(s, _lib.ptr(d), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                             _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light, self.depth_step,
-                                            self.boundary, _lib.ptr(self.images)))
+                                            self.boundary, _lib.ptr(self.render_fwd_ws), self._nfws,
+                                            _lib.ptr(self.images)))
         _lib.check(L.gft_render_density_bwd_l2(s, _lib.ptr(d), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                                _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
                                                self.depth_step, self.boundary, _lib.ptr(self.images),

@@ -109,15 +109,19 @@ class _Cameras(object):
         self.c_out = _lib.i32_array(self.out_dhw)
 
 
-def render_density_forward(d, cams, depth_step, light=(1.0, 1.0, 1.0), boundary="BORDER", out=None):
+def render_density_forward(d, cams, depth_step, light=(1.0, 1.0, 1.0), boundary="BORDER", out=None, ws=None):
     L = _lib.lib()
     if out is None:
         out = torch.empty((1, cams.nCam, cams.out_dhw[1], cams.out_dhw[2], 4), device=d.device, dtype=torch.float32)
     lt = (ctypes.c_float * 3)(*[float(x) for x in light])
-    _lib.check(L.gft_render_density_fwd(_lib.stream_of(d), _lib.ptr(d), _lib.i64_array(d.shape), _np_ptr(cams.M),
-                                        cams.nM, _np_ptr(cams.V), _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam,
-                                        cams.c_out, lt, float(depth_step), _enum(BOUNDARY, boundary, "boundary"),
-                                        _lib.ptr(out)))
+    shape = _lib.i64_array(d.shape)
+    bm = _enum(BOUNDARY, boundary, "boundary")
+    nws = L.gft_render_density_fwd_workspace(shape, bm)
+    if ws is None:
+        ws = torch.empty((max(int(nws), 1),), dtype=torch.uint8, device=d.device)
+    _lib.check(L.gft_render_density_fwd(_lib.stream_of(d), _lib.ptr(d), shape, _np_ptr(cams.M), cams.nM,
+                                        _np_ptr(cams.V), _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt,
+                                        float(depth_step), bm, _lib.ptr(ws), nws, _lib.ptr(out)))
     return out
 
 
@@ -192,6 +196,8 @@ class TransportRenderStep(object):
         self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
         self._nrws = L.gft_render_density_bwd_workspace(self._shape, self.cams.nCam, self.cams.c_out, self.boundary)
         self.render_ws = torch.empty((max(self._nrws, 1),), dtype=torch.uint8, device=dev)
+        self._nfws = L.gft_render_density_fwd_workspace(self._shape, self.boundary)
+        self.render_fwd_ws = torch.empty((max(self._nfws, 1),), dtype=torch.uint8, device=dev)
 
     PHASES = ("advect_fwd", "render_fwd", "render_bwd", "advect_bwd")
 
@@ -210,7 +216,8 @@ class TransportRenderStep(object):
         self._mark(timers, "render_fwd", True)
         _lib.check(L.gft_render_density_fwd(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                             _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light, self.depth_step,
-                                            self.boundary, _lib.ptr(self.images)))
+                                            self.boundary, _lib.ptr(self.render_fwd_ws), self._nfws,
+                                            _lib.ptr(self.images)))
         self._mark(timers, "render_fwd", False)
         return self.images
 
@@ -255,7 +262,7 @@ class TransportRenderStep(object):
         N = self.n_pixels
         return {"advect_sl": 16 * V, "advect_mc": 4 * V, "advect_o1_out": 4 * V, "render_fwd": 4 * V + 16 * N,
                 "render_bwd_walk": 4 * V + 28 * N, "render_bwd_gather": 4 * V, "render_bwd_scatter": 8 * V + 28 * N,
-                "advect_bwd_a": 16 * V, "advect_bwd_b": 20 * V, "fill": 0}
+                "advect_bwd_a": 16 * V, "advect_bwd_b": 20 * V, "quad_copy": 0, "fill": 0}
 
     def voxel_samples(self):
         """Trilinear evaluations of one fwd+bwd step: advection fwd a*V

@@ -110,19 +110,24 @@ int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], con
  * depth). Replaces Renderer.render_density with an ambient Light
  * (renderer.py:603-651, lighting.py:23-26) over raymarch_grid_transform
  * (raymarch_grid.cu.cc:107-143) without the materialised 4-channel grid.
+ * BORDER reads a corner-quad copy of the density (one float4 of x/y corner
+ * neighbours per cell) that the call writes into the workspace
+ * (gft_render_density_fwd_workspace; 0 bytes for CLAMP/WRAP).
  */
+size_t gft_render_density_fwd_workspace(const int64_t shape[5], int32_t boundary);
 int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                           float* images);
+                           void* workspace, size_t workspace_bytes, float* images);
 /*
  * Backward of gft_render_density_fwd with the L2 image loss folded in:
  * loss = loss_scale * sum (rgb - target)^2 (targets [1,V,H,W,3]); writes the
  * scalar loss (device float) and the exact density gradient (overwritten).
  * Replaces loss_dens_target (optimization.py:396-412) + the raymarch gradient
  * (raymarch_grid.cu.cc:343-372). BORDER needs the workspace returned by
- * gft_render_density_bwd_workspace (per-sample gradients of every view and
- * the per-pixel rays) and is deterministic; CLAMP/WRAP need none.
+ * gft_render_density_bwd_workspace (per-sample gradients of every view, the
+ * per-pixel rays and a corner-quad density copy) and is deterministic; CLAMP/WRAP
+ * need none.
  */
 size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                         int32_t boundary);
@@ -268,7 +273,7 @@ int gft_reduce_grid_blend_bwd(void* stream, const float* input, const int64_t sh
  * the launch count and summed duration of one name; gft_timer_names lists
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
- * render_fwd, render_bwd_walk, render_bwd_gather, render_bwd_scatter, fill.
+ * render_fwd, render_bwd_walk, render_bwd_gather, render_bwd_scatter, quad_copy, fill.
  */
 int gft_timer_start(const char* kernels);
 int gft_timer_read(const char* kernel, int32_t* launches, double* total_ms);
