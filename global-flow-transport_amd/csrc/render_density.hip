@@ -223,9 +223,12 @@ __device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, f
 }
 
 // -------------------------------------------------- BORDER: walk + gather
-// Walk (reads the corner-quad copy): stores g_d of every inside sample at gs[((slot*S + i)*H + y)*W + x]
-// and the ray of every pixel at rays[2*((slot*H + y)*W + x) + {0,1}] =
+// Walk (reads the corner-quad copy): stores g_d of every inside sample of
+// the ray's clipped range [lo, hi] at gs[((slot*S + i)*H + y)*W + x] and the
+// ray of every pixel at rays[kRayRec*((slot*H + y)*W + x) + {0,1}] =
 // (near, lo), (step, hi); lo > hi marks a pixel with no stored sample.
+constexpr int kRayRec = 2;
+
 template <bool L2>
 __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __restrict__ dens,
                                                                  const float* __restrict__ img,
@@ -250,10 +253,9 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
       hi = 0;
     }
     const size_t plane = size_t(rd.out_w) * rd.out_h;
-    float4* r = rays + 2 * ((size_t)v.out_slot * plane + pix);
+    float4* r = rays + kRayRec * ((size_t)v.out_slot * plane + pix);
     r[0] = make_float4(nearp.x, nearp.y, nearp.z, (float)lo);
     r[1] = make_float4(step.x, step.y, step.z, (float)hi);
-    const float* buf = dens + rd.in_batch_stride * v.in_batch;
     float* gview = gs + (size_t)v.out_slot * rd.out_d * plane + pix;
     for (int i = hi; i >= lo; i -= kBatch) {
       float val[kBatch];
@@ -277,7 +279,11 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
 // Gather: one lane per voxel c, all views of the launch summed in order.
 // A sample q (cell-centre offset removed) reaches c with weight
 // prod_a max(0, 1 - |q_a - c_a|); its support box (c-1, c+1) is projected to
-// find the candidate pixels, and each candidate ray is cut by the box.
+// find the candidate pixels, and each candidate ray is cut by the box to the
+// samples strictly inside it (a sample on the box surface has weight 0; the
+// 1e-3 margin in the ray parameter only adds samples, whose weight test
+// rejects them). Only samples with positive weight that the walk stored
+// (inside the grid) are loaded.
 __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* __restrict__ gs,
                                                                    const float4* __restrict__ rays,
                                                                    float* __restrict__ grad, RayDims rd,
@@ -289,6 +295,8 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
   const Dims3 d = rd.in;
   const size_t plane = size_t(rd.out_w) * rd.out_h;
   const float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz;
+  // box corners in ray coordinates (which carry the +0.5 cell-centre offset)
+  const float bx0 = fcx - 0.5f, by0 = fcy - 0.5f, bz0 = fcz - 0.5f;
   float acc = 0.f;
   for (int vi = 0; vi < PL.num_views; ++vi) {
     const ProjView& pv = PL.view[vi];
@@ -322,35 +330,37 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
       ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
     }
     const float* gview = gs + (size_t)pv.out_slot * rd.out_d * plane;
-    const float4* rview = rays + 2 * (size_t)pv.out_slot * plane;
+    const float4* rview = rays + kRayRec * (size_t)pv.out_slot * plane;
     for (int py = ys; py <= ye; ++py) {
       for (int px = xs; px <= xe; ++px) {
         const size_t pix = size_t(py) * rd.out_w + px;
-        const float4 r0 = rview[2 * pix], r1 = rview[2 * pix + 1];
+        const float4 r0 = rview[kRayRec * pix], r1 = rview[kRayRec * pix + 1];
         const int lo = (int)r0.w, hi = (int)r1.w;
         if (lo > hi) continue;
-        // sample parameter range where the ray is inside the support box
-        // (c-0.5, c+1.5); an infinite reciprocal of a zero step component
-        // yields the unbounded or empty interval by itself
-        const float ax = (fcx - 0.5f - r0.x) * __builtin_amdgcn_rcpf(r1.x), bx = fmaf(2.f, __builtin_amdgcn_rcpf(r1.x), ax);
-        const float ay = (fcy - 0.5f - r0.y) * __builtin_amdgcn_rcpf(r1.y), by = fmaf(2.f, __builtin_amdgcn_rcpf(r1.y), ay);
-        const float az = (fcz - 0.5f - r0.z) * __builtin_amdgcn_rcpf(r1.z), bz = fmaf(2.f, __builtin_amdgcn_rcpf(r1.z), az);
+        const float4 ri = make_float4(__builtin_amdgcn_rcpf(r1.x), __builtin_amdgcn_rcpf(r1.y), __builtin_amdgcn_rcpf(r1.z), 0.f);
+        // parameter interval of the ray inside the open box (c-0.5, c+1.5);
+        // a zero step component gives an infinite (approximate) reciprocal,
+        // an unbounded or empty interval, or NaN bounds that fminf/fmaxf
+        // ignore (the weight test stays exact)
+        const float ax = (bx0 - r0.x) * ri.x, bx = fmaf(2.f, ri.x, ax);
+        const float ay = (by0 - r0.y) * ri.y, by = fmaf(2.f, ri.y, ay);
+        const float az = (bz0 - r0.z) * ri.z, bz = fmaf(2.f, ri.z, az);
         const float tlo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
         const float thi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
-        if (!(tlo <= thi + 1.f)) continue;
-        const int i0 = max(lo, (int)floorf(fmaxf(tlo, -1.f)) - 1), i1 = min(hi, (int)ceilf(fminf(thi, 1e9f)) + 1);
+        const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));
+        const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
         for (int i = i0; i <= i1; ++i) {
           const float fi = (float)i;
-          const float qx = fmaf(fi, r1.x, r0.x) - 0.5f, qy = fmaf(fi, r1.y, r0.y) - 0.5f,
-                      qz = fmaf(fi, r1.z, r0.z) - 0.5f;
-          const float wx = 1.f - fabsf(qx - fcx), wy = 1.f - fabsf(qy - fcy), wz = 1.f - fabsf(qz - fcz);
-          if ((wx <= 0.f) | (wy <= 0.f) | (wz <= 0.f)) continue;
+          // hat weights relative to the box corner: 1 - |q - c| with q - c = (ray - box0) - 1
+          const float wx = 1.f - fabsf(fmaf(fi, r1.x, r0.x - bx0) - 1.f);
+          const float wy = 1.f - fabsf(fmaf(fi, r1.y, r0.y - by0) - 1.f);
+          const float wz = 1.f - fabsf(fmaf(fi, r1.z, r0.z - bz0) - 1.f);
+          const float w = fmaxf(wx, 0.f) * fmaxf(wy, 0.f) * fmaxf(wz, 0.f);
           // the walk stored only samples inside [-0.5, dims-0.5)
-          const float ux = qx + 0.5f, uy = qy + 0.5f, uz = qz + 0.5f;
-          if (!((0.f <= ux) & (ux < (float)d.x) & (0.f <= uy) & (uy < (float)d.y) & (0.f <= uz) &
-                (uz < (float)d.z)))
-            continue;
-          acc = fmaf(gview[(size_t)i * plane + pix], wx * wy * wz, acc);
+          const float ux = fmaf(fi, r1.x, r0.x), uy = fmaf(fi, r1.y, r0.y), uz = fmaf(fi, r1.z, r0.z);
+          const bool ok = (w > 0.f) & (0.f <= ux) & (ux < (float)d.x) & (0.f <= uy) & (uy < (float)d.y) &
+                          (0.f <= uz) & (uz < (float)d.z);
+          if (ok) acc = fmaf(gview[(size_t)i * plane + pix], w, acc);
         }
       }
     }
