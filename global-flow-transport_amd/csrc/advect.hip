@@ -13,7 +13,8 @@
 //   out(x) = (c < mn || c > mx) ? f(x) : c     (MC_SMOOTH)
 // The composition is two launches: the first writes f, the second computes
 // b, the corner extrema and the selection, and stores the clamp mask for the
-// backward. The backward is two scatter launches (float atomics):
+// backward. The backward is two tiled scatter launches (LDS accumulation,
+// then float atomics into the gradient):
 //   A: gb = mask ? 0 : -g/2 is scattered into gf at x+v*dt; gd(x) = mask?0:g/2;
 //      gv(x) = gb * dt * grad f(x+v*dt)
 //   B: gf = g + scattered gb; gf is scattered into gd at x-v*dt;
@@ -75,19 +76,6 @@ __device__ __forceinline__ float3 tri_grad(const Tri& t) {
   float gz = lerpf(lerpf(v[1][0][0] - v[0][0][0], v[1][0][1] - v[0][0][1], t.wx),
                    lerpf(v[1][1][0] - v[0][1][0], v[1][1][1] - v[0][1][1], t.wx), t.wy);
   return make_float3(gx, gy, gz);
-}
-
-__device__ __forceinline__ void tri_scatter(float* __restrict__ g, Dims3 d, const Tri& t, float val) {
-  const float wx[2] = {1.f - t.wx, t.wx}, wy[2] = {1.f - t.wy, t.wy}, wz[2] = {1.f - t.wz, t.wz};
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        float w = wx[i] * wy[j] * wz[k];
-        if (w != 0.f) atomic_addf(g + flat3(t.ix[i], t.iy[j], t.iz[k], d), val * w);
-      }
 }
 
 __device__ __forceinline__ void voxel_of(size_t i, Dims3 d, int& x, int& y, int& z) {
@@ -170,31 +158,91 @@ __global__ __launch_bounds__(256) void k_advect_mc(const float* __restrict__ den
   mask[i] = m;
 }
 
+// ----------------------------------------------------------- backward
+// The adjoint scatters are accumulated per tile of kTX x kTY x kTZ voxels in
+// an LDS region covering the tile plus a kHalo-cell margin (positions moved by
+// less than kHalo cells in every axis have all 8 corners inside it), with LDS
+// atomics; the region is then added to the global gradient with one coalesced
+// global atomic per nonzero cell. Corners outside the region (faster motion)
+// go straight to a global atomic, so any velocity is handled exactly.
+constexpr int kTX = 32, kTY = 8, kTZ = 8, kHalo = 3;
+constexpr int kRX = kTX + 2 * kHalo, kRY = kTY + 2 * kHalo, kRZ = kTZ + 2 * kHalo;
+constexpr int kRegion = kRX * kRY * kRZ;
+
+struct TileScatter {
+  float* lds;
+  int x0, y0, z0;  // tile origin
+
+  __device__ __forceinline__ void clear() {
+    for (int k = threadIdx.x; k < kRegion; k += blockDim.x) lds[k] = 0.f;
+  }
+  __device__ __forceinline__ void add(float* __restrict__ g, Dims3 d, const Tri& t, float val) {
+    const float wx[2] = {1.f - t.wx, t.wx}, wy[2] = {1.f - t.wy, t.wy}, wz[2] = {1.f - t.wz, t.wz};
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const float w = wx[i] * wy[j] * wz[k];
+          if (w == 0.f) continue;
+          const int lx = t.ix[i] - x0 + kHalo, ly = t.iy[j] - y0 + kHalo, lz = t.iz[k] - z0 + kHalo;
+          if (((unsigned)lx < (unsigned)kRX) & ((unsigned)ly < (unsigned)kRY) & ((unsigned)lz < (unsigned)kRZ))
+            atomicAdd(lds + (lz * kRY + ly) * kRX + lx, val * w);
+          else
+            atomic_addf(g + flat3(t.ix[i], t.iy[j], t.iz[k], d), val * w);
+        }
+  }
+  __device__ __forceinline__ void flush(float* __restrict__ g, Dims3 d) {
+    for (int k = threadIdx.x; k < kRegion; k += blockDim.x) {
+      const float v = lds[k];
+      if (v == 0.f) continue;
+      const int x = x0 - kHalo + k % kRX, y = y0 - kHalo + (k / kRX) % kRY, z = z0 - kHalo + k / (kRX * kRY);
+      // cells outside the grid never receive a corner (indices are clamped)
+      if (((unsigned)x < (unsigned)d.x) & ((unsigned)y < (unsigned)d.y) & ((unsigned)z < (unsigned)d.z))
+        atomic_addf(g + flat3(x, y, z, d), v);
+    }
+  }
+};
+
+__device__ __forceinline__ bool tile_voxel(int k, Dims3 d, int x0, int y0, int z0, int& x, int& y, int& z) {
+  x = x0 + k % kTX;
+  y = y0 + (k / kTX) % kTY;
+  z = z0 + k / (kTX * kTY);
+  return (x < d.x) & (y < d.y) & (z < d.z);
+}
+
 // backward A (MacCormack only): scatter of the back-warp gradient into gf,
 // direct density term, velocity term of the back warp
 __global__ __launch_bounds__(256) void k_advect_bwd_a(const float* __restrict__ vel, const float* __restrict__ f,
                                                       const uint8_t* __restrict__ mask,
                                                       const float* __restrict__ gout, float* __restrict__ gf_acc,
-                                                      float* __restrict__ gd, float* __restrict__ gv, AdvectDims a,
-                                                      size_t n) {
-  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int x, y, z;
-  voxel_of(i, a.d, x, y, z);
-  float3 v = load_vel(vel, i);
-  float g = gout[i];
-  uint8_t m = mask[i];
-  // gradient of the corrected value c: zero where MC_SMOOTH reverted to f or
-  // NEGATIVE clamped to 0
-  float gc = m ? 0.f : g;
-  float gb = -0.5f * gc;
-  gd[i] = 0.5f * gc;
-  Tri tb = tri_clamp(f, a.d, x + v.x * a.dt, y + v.y * a.dt, z + v.z * a.dt);
-  float3 df = tri_grad(tb);
-  gv[3 * i] = gb * a.dt * df.x;
-  gv[3 * i + 1] = gb * a.dt * df.y;
-  gv[3 * i + 2] = gb * a.dt * df.z;
-  if (gb != 0.f) tri_scatter(gf_acc, a.d, tb, gb);
+                                                      float* __restrict__ gd, float* __restrict__ gv, AdvectDims a) {
+  __shared__ float region[kRegion];
+  TileScatter ts{region, (int)blockIdx.x * kTX, (int)blockIdx.y * kTY, (int)blockIdx.z * kTZ};
+  ts.clear();
+  __syncthreads();
+  for (int k = threadIdx.x; k < kTX * kTY * kTZ; k += blockDim.x) {
+    int x, y, z;
+    if (!tile_voxel(k, a.d, ts.x0, ts.y0, ts.z0, x, y, z)) continue;
+    const size_t i = flat3(x, y, z, a.d);
+    float3 v = load_vel(vel, i);
+    float g = gout[i];
+    uint8_t m = mask[i];
+    // gradient of the corrected value c: zero where MC_SMOOTH reverted to f or
+    // NEGATIVE clamped to 0
+    float gc = m ? 0.f : g;
+    float gb = -0.5f * gc;
+    gd[i] = 0.5f * gc;
+    Tri tb = tri_clamp(f, a.d, x + v.x * a.dt, y + v.y * a.dt, z + v.z * a.dt);
+    float3 df = tri_grad(tb);
+    gv[3 * i] = gb * a.dt * df.x;
+    gv[3 * i + 1] = gb * a.dt * df.y;
+    gv[3 * i + 2] = gb * a.dt * df.z;
+    if (gb != 0.f) ts.add(gf_acc, a.d, tb, gb);
+  }
+  __syncthreads();
+  ts.flush(gf_acc, a.d);
 }
 
 // backward B: total gf scattered into gd at x - v dt, velocity term of the
@@ -205,36 +253,43 @@ __global__ __launch_bounds__(256) void k_advect_bwd_b(const float* __restrict__ 
                                                       const float* __restrict__ gout,
                                                       const float* __restrict__ gf_acc,
                                                       const uint8_t* __restrict__ mask, float* __restrict__ gd,
-                                                      float* __restrict__ gv, AdvectDims a, size_t n) {
-  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int x, y, z;
-  voxel_of(i, a.d, x, y, z);
-  float3 v = load_vel(vel, i);
-  float g = gout[i];
-  float gf;
-  if (FIRST) {
-    // order 1: out = f (or max(f,0) with NEGATIVE)
-    gf = (a.clamp == CLAMP_NEGATIVE && mask[i]) ? 0.f : g;
-  } else {
-    // f(x) enters out directly in both clamp branches, plus the back-warp scatter
-    uint8_t m = mask[i];
-    float gc = m ? 0.f : g;
-    gf = (a.clamp == CLAMP_MC_SMOOTH ? g : gc) + gf_acc[i];
+                                                      float* __restrict__ gv, AdvectDims a) {
+  __shared__ float region[kRegion];
+  TileScatter ts{region, (int)blockIdx.x * kTX, (int)blockIdx.y * kTY, (int)blockIdx.z * kTZ};
+  ts.clear();
+  __syncthreads();
+  for (int k = threadIdx.x; k < kTX * kTY * kTZ; k += blockDim.x) {
+    int x, y, z;
+    if (!tile_voxel(k, a.d, ts.x0, ts.y0, ts.z0, x, y, z)) continue;
+    const size_t i = flat3(x, y, z, a.d);
+    float3 v = load_vel(vel, i);
+    float g = gout[i];
+    float gf;
+    if (FIRST) {
+      // order 1: out = f (or max(f,0) with NEGATIVE)
+      gf = (a.clamp == CLAMP_NEGATIVE && mask[i]) ? 0.f : g;
+    } else {
+      // f(x) enters out directly in both clamp branches, plus the back-warp scatter
+      uint8_t m = mask[i];
+      float gc = m ? 0.f : g;
+      gf = (a.clamp == CLAMP_MC_SMOOTH ? g : gc) + gf_acc[i];
+    }
+    Tri tf = tri_clamp(dens, a.d, x - v.x * a.dt, y - v.y * a.dt, z - v.z * a.dt);
+    float3 dd = tri_grad(tf);
+    float3 gvv = make_float3(-gf * a.dt * dd.x, -gf * a.dt * dd.y, -gf * a.dt * dd.z);
+    if (FIRST) {
+      gv[3 * i] = gvv.x;
+      gv[3 * i + 1] = gvv.y;
+      gv[3 * i + 2] = gvv.z;
+    } else {
+      gv[3 * i] += gvv.x;
+      gv[3 * i + 1] += gvv.y;
+      gv[3 * i + 2] += gvv.z;
+    }
+    if (gf != 0.f) ts.add(gd, a.d, tf, gf);
   }
-  Tri tf = tri_clamp(dens, a.d, x - v.x * a.dt, y - v.y * a.dt, z - v.z * a.dt);
-  float3 dd = tri_grad(tf);
-  float3 gvv = make_float3(-gf * a.dt * dd.x, -gf * a.dt * dd.y, -gf * a.dt * dd.z);
-  if (FIRST) {
-    gv[3 * i] = gvv.x;
-    gv[3 * i + 1] = gvv.y;
-    gv[3 * i + 2] = gvv.z;
-  } else {
-    gv[3 * i] += gvv.x;
-    gv[3 * i + 1] += gvv.y;
-    gv[3 * i + 2] += gvv.z;
-  }
-  if (gf != 0.f) tri_scatter(gd, a.d, tf, gf);
+  __syncthreads();
+  ts.flush(gd, a.d);
 }
 
 static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -260,20 +315,21 @@ hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel,
 
 hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
                              const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a) {
-  size_t n = size_t(a.d.x) * a.d.y * a.d.z;
+  const dim3 tiles((unsigned)((a.d.x + kTX - 1) / kTX), (unsigned)((a.d.y + kTY - 1) / kTY),
+                   (unsigned)((a.d.z + kTZ - 1) / kTZ));
   if (a.order == 2) {
     {
       KernelTimer t(s, "advect_bwd_a");
-      hipLaunchKernelGGL(k_advect_bwd_a, grid1(n), dim3(256), 0, s, vel, f, mask, gout, gf_acc, gd, gv, a, n);
+      hipLaunchKernelGGL(k_advect_bwd_a, tiles, dim3(256), 0, s, vel, f, mask, gout, gf_acc, gd, gv, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     KernelTimer t(s, "advect_bwd_b");
-    hipLaunchKernelGGL(k_advect_bwd_b<false>, grid1(n), dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a, n);
+    hipLaunchKernelGGL(k_advect_bwd_b<false>, tiles, dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a);
   } else {
     // gd must be zero on entry (the caller clears it)
     KernelTimer t(s, "advect_bwd_b");
-    hipLaunchKernelGGL(k_advect_bwd_b<true>, grid1(n), dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a, n);
+    hipLaunchKernelGGL(k_advect_bwd_b<true>, tiles, dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a);
   }
   return hipGetLastError();
 }
