@@ -276,39 +276,45 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
   if (L2) block_loss(sq, loss, rp.loss_scale);
 }
 
-// Gather: one lane per voxel c, all views of the launch summed in order.
-// A sample q (cell-centre offset removed) reaches c with weight
-// prod_a max(0, 1 - |q_a - c_a|); its support box (c-1, c+1) is projected to
-// find the candidate pixels, and each candidate ray is cut by the box to the
-// samples strictly inside it (a sample on the box surface has weight 0; the
-// 1e-3 margin in the ray parameter only adds samples, whose weight test
-// rejects them). Only samples with positive weight that the walk stored
-// (inside the grid) are loaded.
+// Gather: one lane per run of ZR voxels c = (x, y, z0..z0+ZR-1), all views of
+// the launch summed in order. A sample q (cell-centre offset removed) reaches
+// c with weight prod_a max(0, 1 - |q_a - c_a|). The run's support box
+// (x-1, x+1) x (y-1, y+1) x (z0-1, z0+ZR) is projected to find the candidate
+// pixels, and each candidate ray is cut by the box to the samples strictly
+// inside it (a sample on the box surface has weight 0; the 1e-3 margin in the
+// ray parameter only adds samples, whose weight test rejects them). A sample
+// is loaded once for the whole run; only samples with positive weight that the
+// walk stored (inside the grid) are loaded.
+template <int ZR>
 __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* __restrict__ gs,
                                                                    const float4* __restrict__ rays,
                                                                    float* __restrict__ grad, RayDims rd,
                                                                    ProjLaunch PL) {
   const int cx = blockIdx.x * 64 + (threadIdx.x & 63);
   const int cy = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int cz = blockIdx.z;
+  const int cz0 = blockIdx.z * ZR;
   if (cx >= rd.in.x || cy >= rd.in.y) return;
   const Dims3 d = rd.in;
   const size_t plane = size_t(rd.out_w) * rd.out_h;
-  const float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz;
-  // box corners in ray coordinates (which carry the +0.5 cell-centre offset)
+  const float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz0;
+  // box corner in ray coordinates (which carry the +0.5 cell-centre offset);
+  // the box spans 2 x 2 x (ZR + 1)
   const float bx0 = fcx - 0.5f, by0 = fcy - 0.5f, bz0 = fcz - 0.5f;
-  float acc = 0.f;
+  const float hz = 0.5f * (float)(ZR + 1);
+  float acc[ZR];
+#pragma unroll
+  for (int r = 0; r < ZR; ++r) acc[r] = 0.f;
   for (int vi = 0; vi < PL.num_views; ++vi) {
     const ProjView& pv = PL.view[vi];
     const float* A = pv.pmv;
-    // clip coordinates of the support box corners c+0.5 +- 1 from the centre
-    // and the three column vectors (the map is linear before the divide)
-    const float4 cc = mat_vec(A, fcx + 0.5f, fcy + 0.5f, fcz + 0.5f, 1.f);
+    // clip coordinates of the box corners from its centre and the three
+    // column vectors (the map is linear before the divide)
+    const float4 cc = mat_vec(A, fcx + 0.5f, fcy + 0.5f, fcz + 0.5f * (float)ZR, 1.f);
     float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
     bool behind = false;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float sx = (k & 1) ? 1.f : -1.f, sy = (k & 2) ? 1.f : -1.f, sz = (k & 4) ? 1.f : -1.f;
+      const float sx = (k & 1) ? 1.f : -1.f, sy = (k & 2) ? 1.f : -1.f, sz = (k & 4) ? hz : -hz;
       const float cxk = fmaf(sz, A[8], fmaf(sy, A[4], fmaf(sx, A[0], cc.x)));
       const float cyk = fmaf(sz, A[9], fmaf(sy, A[5], fmaf(sx, A[1], cc.y)));
       const float cwk = fmaf(sz, A[11], fmaf(sy, A[7], fmaf(sx, A[3], cc.w)));
@@ -337,36 +343,49 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
         const float4 r0 = rview[kRayRec * pix], r1 = rview[kRayRec * pix + 1];
         const int lo = (int)r0.w, hi = (int)r1.w;
         if (lo > hi) continue;
-        const float4 ri = make_float4(__builtin_amdgcn_rcpf(r1.x), __builtin_amdgcn_rcpf(r1.y), __builtin_amdgcn_rcpf(r1.z), 0.f);
-        // parameter interval of the ray inside the open box (c-0.5, c+1.5);
-        // a zero step component gives an infinite (approximate) reciprocal,
-        // an unbounded or empty interval, or NaN bounds that fminf/fmaxf
-        // ignore (the weight test stays exact)
-        const float ax = (bx0 - r0.x) * ri.x, bx = fmaf(2.f, ri.x, ax);
-        const float ay = (by0 - r0.y) * ri.y, by = fmaf(2.f, ri.y, ay);
-        const float az = (bz0 - r0.z) * ri.z, bz = fmaf(2.f, ri.z, az);
+        const float irx = __builtin_amdgcn_rcpf(r1.x), iry = __builtin_amdgcn_rcpf(r1.y),
+                    irz = __builtin_amdgcn_rcpf(r1.z);
+        // parameter interval of the ray inside the open box; a zero step
+        // component gives an infinite (approximate) reciprocal, an unbounded
+        // or empty interval, or NaN bounds that fminf/fmaxf ignore (the weight
+        // test stays exact)
+        const float ax = (bx0 - r0.x) * irx, bx = fmaf(2.f, irx, ax);
+        const float ay = (by0 - r0.y) * iry, by = fmaf(2.f, iry, ay);
+        const float az = (bz0 - r0.z) * irz, bz = fmaf((float)(ZR + 1), irz, az);
         const float tlo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
         const float thi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
         const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));
         const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
         for (int i = i0; i <= i1; ++i) {
           const float fi = (float)i;
-          // hat weights relative to the box corner: 1 - |q - c| with q - c = (ray - box0) - 1
-          const float wx = 1.f - fabsf(fmaf(fi, r1.x, r0.x - bx0) - 1.f);
-          const float wy = 1.f - fabsf(fmaf(fi, r1.y, r0.y - by0) - 1.f);
-          const float wz = 1.f - fabsf(fmaf(fi, r1.z, r0.z - bz0) - 1.f);
-          const float w = fmaxf(wx, 0.f) * fmaxf(wy, 0.f) * fmaxf(wz, 0.f);
-          // the walk stored only samples inside [-0.5, dims-0.5)
           const float ux = fmaf(fi, r1.x, r0.x), uy = fmaf(fi, r1.y, r0.y), uz = fmaf(fi, r1.z, r0.z);
-          const bool ok = (w > 0.f) & (0.f <= ux) & (ux < (float)d.x) & (0.f <= uy) & (uy < (float)d.y) &
-                          (0.f <= uz) & (uz < (float)d.z);
-          if (ok) acc = fmaf(gview[(size_t)i * plane + pix], w, acc);
+          // hat weights: 1 - |q - c| with q = ray - 0.5
+          const float wxy = fmaxf(1.f - fabsf(ux - 0.5f - fcx), 0.f) * fmaxf(1.f - fabsf(uy - 0.5f - fcy), 0.f);
+          float wz[ZR];
+          bool any = false;
+#pragma unroll
+          for (int r = 0; r < ZR; ++r) {
+            wz[r] = fmaxf(1.f - fabsf(uz - 0.5f - (fcz + (float)r)), 0.f) * wxy;
+            any |= wz[r] > 0.f;
+          }
+          // the walk stored only samples inside [-0.5, dims-0.5)
+          const bool ok = any & (0.f <= ux) & (ux < (float)d.x) & (0.f <= uy) & (uy < (float)d.y) & (0.f <= uz) &
+                          (uz < (float)d.z);
+          if (ok) {
+            const float g = gview[(size_t)i * plane + pix];
+#pragma unroll
+            for (int r = 0; r < ZR; ++r) acc[r] = fmaf(g, wz[r], acc[r]);
+          }
         }
       }
     }
   }
-  float* g = grad + flat3(cx, cy, cz, d);
-  *g = PL.accumulate ? *g + acc : acc;
+#pragma unroll
+  for (int r = 0; r < ZR; ++r) {
+    if (cz0 + r >= d.z) break;
+    float* g = grad + flat3(cx, cy, cz0 + r, d);
+    *g = PL.accumulate ? *g + acc[r] : acc[r];
+  }
 }
 
 // ----------------------------------------- CLAMP: tiled LDS pre-reduction
@@ -616,11 +635,13 @@ hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* dens,
   return hipGetLastError();
 }
 
+constexpr int kGatherZRun = 4;
+
 hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
                                         const RayDims& rd, const ProjLaunch& PL) {
-  const dim3 grid((rd.in.x + 63) / 64, (rd.in.y + 3) / 4, rd.in.z);
+  const dim3 grid((rd.in.x + 63) / 64, (rd.in.y + 3) / 4, (rd.in.z + kGatherZRun - 1) / kGatherZRun);
   KernelTimer t(s, "render_bwd_gather");
-  hipLaunchKernelGGL(k_render_density_bwd_gather, grid, dim3(256), 0, s, gs, rays, grad, rd, PL);
+  hipLaunchKernelGGL(k_render_density_bwd_gather<kGatherZRun>, grid, dim3(256), 0, s, gs, rays, grad, rd, PL);
   return hipGetLastError();
 }
 
