@@ -208,6 +208,6 @@ void gft_timer_stop(void) { gft::timer_release(); }
 
 const char* gft_last_error(void) { return gft::g_last_error.c_str(); }
 
-int gft_abi_version(void) { return 2; }
+int gft_abi_version(void) { return 3; }
 
 }  // extern "C"

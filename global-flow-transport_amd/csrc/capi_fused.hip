@@ -13,10 +13,10 @@ hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel,
 hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
                              const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a);
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp);
-hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* dens, const float* img,
-                                      const float* second, float* gs, float4* rays, float* loss, const RayDims& rd,
-                                      const RayLaunch& L, const RenderParams& rp);
+                                     const RayLaunch& L, const RenderParams& rp, float* gs, float4* rays);
+hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* img, const float* second, float* gs,
+                                      float4* rays, float* loss, const RayDims& rd, const RayLaunch& L,
+                                      const RenderParams& rp);
 hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
                                         const RayDims& rd, const ProjLaunch& PL);
 hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
@@ -156,10 +156,26 @@ size_t gft_render_density_fwd_workspace(const int64_t shape[5], int32_t boundary
   return boundary == GFT_BORDER ? quad_bytes(shape) : 0;
 }
 
+size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
+                                      int32_t boundary) {
+  (void)shape;
+  if (boundary != GFT_BORDER) return 0;
+  const size_t plane = (size_t)output_shape[1] * output_shape[2];
+  return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
+         align256(plane * num_cameras * 2 * sizeof(float4));
+}
+
+// saved = per-sample values [views][S][H][W] followed by the ray records
+static void saved_parts(void* saved, const int32_t* output_shape, int32_t num_cameras, float** gs, float4** rays) {
+  const size_t plane = (size_t)output_shape[1] * output_shape[2];
+  *gs = (float*)saved;
+  *rays = (float4*)((char*)saved + align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)));
+}
+
 int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                           void* workspace, size_t workspace_bytes, float* images) {
+                           void* workspace, size_t workspace_bytes, void* saved, size_t saved_bytes, float* images) {
   (void)frustum;
   int rc;
   if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
@@ -168,6 +184,13 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
   if (need && (!workspace || workspace_bytes < need))
     return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
                 need);
+  const size_t need_saved = gft_render_density_saved_bytes(shape, num_cameras, output_shape, boundary);
+  if (saved && saved_bytes < need_saved)
+    return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
+                need_saved);
+  float* gs = nullptr;
+  float4* rays = nullptr;
+  if (saved && need_saved) saved_parts(saved, output_shape, num_cameras, &gs, &rays);
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
   auto pairs = view_pairs(1, num_cameras, true);
@@ -180,7 +203,7 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
     size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
     RayLaunch L;
     if ((rc = ray_views(pairs, first, cnt, M, V, P, L))) return rc;
-    if ((rc = check_hip(render_density_fwd_launch(boundary, (hipStream_t)stream, density, images, rd, L, rp),
+    if ((rc = check_hip(render_density_fwd_launch(boundary, (hipStream_t)stream, density, images, rd, L, rp, gs, rays),
                         "render forward")))
       return rc;
   }
@@ -189,24 +212,25 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
 
 size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                         int32_t boundary) {
-  if (boundary != GFT_BORDER) return 0;
-  const size_t plane = (size_t)output_shape[1] * output_shape[2];
-  return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
-         align256(plane * num_cameras * 2 * sizeof(float4)) + quad_bytes(shape);
+  (void)shape;
+  (void)num_cameras;
+  (void)output_shape;
+  (void)boundary;
+  return 0;
 }
 
 static int render_bwd_common(void* stream, const float* density, const int64_t shape[5], const float* M,
                              int32_t num_M, const float* V, const float* P, int32_t num_cameras,
                              const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                             const float* images, const float* second, bool l2, float loss_scale, void* workspace,
-                             size_t workspace_bytes, float* density_grad, float* loss) {
+                             const float* images, const float* second, bool l2, float loss_scale, void* saved,
+                             size_t saved_bytes, float* density_grad, float* loss) {
   int rc;
   if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
   if (!density || !images || !second || !density_grad || (l2 && !loss) || !M || !V || !P || !light)
     return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
-  const size_t need = gft_render_density_bwd_workspace(shape, num_cameras, output_shape, boundary);
-  if (need && (!workspace || workspace_bytes < need))
-    return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
+  const size_t need = gft_render_density_saved_bytes(shape, num_cameras, output_shape, boundary);
+  if (need && (!saved || saved_bytes < need))
+    return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
                 need);
   hipStream_t s = (hipStream_t)stream;
   const RayDims rd = render_dims(shape, output_shape);
@@ -214,19 +238,16 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
   if (l2 && (rc = check_hip(fill_launch(s, loss, 1, 0.f), "clear loss"))) return rc;
   auto pairs = view_pairs(1, num_cameras, true);
   if (boundary == GFT_BORDER) {
-    const size_t plane = (size_t)output_shape[1] * output_shape[2];
-    float* gs = (float*)workspace;
-    const size_t gs_bytes = align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float));
-    float4* rays = (float4*)((char*)workspace + gs_bytes);
-    float* quads = (float*)((char*)workspace + gs_bytes + align256(plane * num_cameras * 2 * sizeof(float4)));
-    if ((rc = check_hip(quad_copy_launch(s, density, quads, rd.in), "quad copy"))) return rc;
+    float* gs;
+    float4* rays;
+    saved_parts(saved, output_shape, num_cameras, &gs, &rays);
     for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
       size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
       RayLaunch L;
       ProjLaunch PL;
       if ((rc = ray_views(pairs, first, cnt, M, V, P, L, &PL))) return rc;
       PL.accumulate = first > 0;
-      if ((rc = check_hip(render_density_walk_launch(l2, s, quads, images, second, gs, rays, loss, rd, L, rp),
+      if ((rc = check_hip(render_density_walk_launch(l2, s, images, second, gs, rays, loss, rd, L, rp),
                           "render backward walk")))
         return rc;
       if ((rc = check_hip(render_density_gather_launch(s, gs, rays, density_grad, rd, PL), "render backward gather")))
@@ -251,22 +272,25 @@ int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t 
                               int32_t num_M, const float* V, const float* P, const float* frustum,
                               int32_t num_cameras, const int32_t output_shape[3], const float light[3],
                               float depth_step, int32_t boundary, const float* images, const float* targets,
-                              float loss_scale, void* workspace, size_t workspace_bytes, float* density_grad,
-                              float* loss) {
+                              float loss_scale, void* saved, size_t saved_bytes, void* workspace,
+                              size_t workspace_bytes, float* density_grad, float* loss) {
   (void)frustum;
+  (void)workspace;
+  (void)workspace_bytes;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
-                           boundary, images, targets, true, loss_scale, workspace, workspace_bytes, density_grad, loss);
+                           boundary, images, targets, true, loss_scale, saved, saved_bytes, density_grad, loss);
 }
 
 int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                           const float* images, const float* images_grad, void* workspace, size_t workspace_bytes,
-                           float* density_grad) {
+                           const float* images, const float* images_grad, void* saved, size_t saved_bytes,
+                           void* workspace, size_t workspace_bytes, float* density_grad) {
   (void)frustum;
+  (void)workspace;
+  (void)workspace_bytes;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
-                           boundary, images, images_grad, false, 0.f, workspace, workspace_bytes, density_grad,
-                           nullptr);
+                           boundary, images, images_grad, false, 0.f, saved, saved_bytes, density_grad, nullptr);
 }
 
 }  // extern "C"

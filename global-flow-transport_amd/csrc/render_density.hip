@@ -144,37 +144,71 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
 }
 
 constexpr int kBatch = 4;
+constexpr int kRayRec = 2;  // float4 records per pixel ray: (near, lo), (step, hi)
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int m = 32; m > 0; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int m = 32; m > 0; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+  return v;
+}
 
 // ----------------------------------------------------------------- forward
 // BORDER reads `dens` as the corner-quad copy (float4, QuadDims q); CLAMP
-// and WRAP read the x-fastest grid.
+// and WRAP read the x-fastest grid. With gs != nullptr (BORDER, training) the
+// forward also saves what the backward walk needs: the ray of every pixel at
+// rays[2*((slot*H + y)*W + x) + {0,1}] = (near, lo), (step, hi) and sigma of
+// every sample of the clipped range [lo, hi] at gs[((slot*S + i)*H + y)*W + x]
+// (0 for a sample outside the grid).
 template <int BM>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
-                                                            RayDims rd, RayLaunch L, RenderParams rp, QuadDims q) {
+                                                            RayDims rd, RayLaunch L, RenderParams rp, QuadDims q,
+                                                            float* __restrict__ gs, float4* __restrict__ rays) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
-  if (x >= rd.out_w || y >= rd.out_h) return;
-  float3 nearp, step;
-  pixel_ray(v, rd, x, y, nearp, step);
-  const float* buf = dens + rd.in_batch_stride * v.in_batch;
+  const bool valid = x < rd.out_w && y < rd.out_h;
   float D = 0.f, E = 0.f;
   if (BM == B_BORDER) {
-    int lo, hi;
-    clip_ray(nearp, step, rd.in, rd.out_d, lo, hi);
-    for (int i = lo; i <= hi; i += kBatch) {
+    // every lane of the wave takes part in the range reduction
+    float3 nearp = make_float3(0.f, 0.f, 0.f), step = make_float3(0.f, 0.f, 0.f);
+    int lo = 1, hi = 0;
+    if (valid) {
+      pixel_ray(v, rd, x, y, nearp, step);
+      clip_ray(nearp, step, rd.in, rd.out_d, lo, hi);
+    }
+    const size_t plane = size_t(rd.out_w) * rd.out_h;
+    const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
+    float* gview = (gs && valid) ? gs + (size_t)v.out_slot * rd.out_d * plane + pix : nullptr;
+    if (gs && valid) {
+      float4* r = rays + kRayRec * ((size_t)v.out_slot * plane + pix);
+      r[0] = make_float4(nearp.x, nearp.y, nearp.z, (float)lo);
+      r[1] = make_float4(step.x, step.y, step.z, (float)hi);
+    }
+    // march the wave's union of ranges with one sample index for all lanes, so
+    // that the saved-sigma stores of one instruction fill one sample plane
+    const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
+    for (int i = wlo; i <= whi; i += kBatch) {
       float val[kBatch];
       bool in[kBatch];
       read_border_batch<kBatch>(reinterpret_cast<const float4*>(dens), rd.in, q, nearp, step, i, 1, hi, val, in);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        // samples outside the grid read 0 and leave D and E unchanged
-        const float sigma = rp.step * val[u];
-        D = in[u] ? fmaxf(D + sigma, 0.f) : D;
+        // samples outside the grid or the lane's range read 0 and leave D and E unchanged
+        const bool act = in[u] & (i + u >= lo);
+        const float sigma = act ? rp.step * val[u] : 0.f;
+        D = act ? fmaxf(D + sigma, 0.f) : D;
         E = fmaf(__expf(-D), sigma, E);
+        if (gview && i + u <= whi) gview[(size_t)(i + u) * plane] = sigma;
       }
     }
   } else {
+    if (!valid) return;
+    float3 nearp, step;
+    pixel_ray(v, rd, x, y, nearp, step);
+    const float* buf = dens + rd.in_batch_stride * v.in_batch;
     for (int i = 0; i < rd.out_d; ++i) {
       const float fi = (float)i;
       const float sigma = rp.step * read_linear<1, BM>(buf, rd.in, fmaf(fi, step.x, nearp.x) - 0.5f,
@@ -184,6 +218,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
       E = fmaf(__expf(-D), sigma, E);
     }
   }
+  if (!valid) return;
   const float4 o = make_float4(rp.light[0] * E, rp.light[1] * E, rp.light[2] * E, D);
   reinterpret_cast<float4*>(img + rd.out_view_stride * v.out_slot)[size_t(y) * rd.out_w + x] = o;
 }
@@ -223,48 +258,51 @@ __device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, f
 }
 
 // -------------------------------------------------- BORDER: walk + gather
-// Walk (reads the corner-quad copy): stores g_d of every inside sample of
-// the ray's clipped range [lo, hi] at gs[((slot*S + i)*H + y)*W + x] and the
-// ray of every pixel at rays[kRayRec*((slot*H + y)*W + x) + {0,1}] =
-// (near, lo), (step, hi); lo > hi marks a pixel with no stored sample.
-constexpr int kRayRec = 2;
+// Walk over the forward's saved sigma: for every pixel of the chunk, from far
+// to near over its clipped range, D_{i-1} = max(D_i - sigma_i, 0) and
+// g_i = s * (exp(-D_i) G + g_D), g_D -= exp(-D_i) sigma_i G; g_i overwrites
+// sigma_i in place (samples outside the grid get a value the gather never
+// reads). A pixel with no gradient gets lo > hi so the gather skips it.
+constexpr int kWalkBatch = 8;
 
 template <bool L2>
-__global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __restrict__ dens,
-                                                                 const float* __restrict__ img,
+__global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __restrict__ img,
                                                                  const float* __restrict__ second,
                                                                  float* __restrict__ gs, float4* __restrict__ rays,
                                                                  float* __restrict__ loss, RayDims rd, RayLaunch L,
-                                                                 RenderParams rp, QuadDims q) {
+                                                                 RenderParams rp) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
-  float sq = 0.f;
-  if (x < rd.out_w && y < rd.out_h) {
-    const size_t pix = size_t(y) * rd.out_w + x;
-    float G, gD, D;
+  const bool valid = x < rd.out_w && y < rd.out_h;
+  float sq = 0.f, G = 0.f, gD = 0.f, D = 0.f;
+  int lo = 1, hi = 0;
+  const size_t plane = size_t(rd.out_w) * rd.out_h;
+  const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
+  float* gview = gs + (size_t)v.out_slot * rd.out_d * plane + pix;
+  if (valid) {
     pixel_grad<L2>(img, second, rd, v, pix, rp, G, gD, D, sq);
-    float3 nearp, step;
-    pixel_ray(v, rd, x, y, nearp, step);
-    int lo, hi;
-    clip_ray(nearp, step, rd.in, rd.out_d, lo, hi);
-    if (G == 0.f && gD == 0.f) {
+    float4* r = rays + kRayRec * ((size_t)v.out_slot * plane + pix);
+    lo = (int)r[0].w;
+    hi = (int)r[1].w;
+    if (G == 0.f && gD == 0.f && lo <= hi) {
+      reinterpret_cast<float*>(r)[3] = 1.f;
+      reinterpret_cast<float*>(r + 1)[3] = 0.f;
       lo = 1;
       hi = 0;
     }
-    const size_t plane = size_t(rd.out_w) * rd.out_h;
-    float4* r = rays + kRayRec * ((size_t)v.out_slot * plane + pix);
-    r[0] = make_float4(nearp.x, nearp.y, nearp.z, (float)lo);
-    r[1] = make_float4(step.x, step.y, step.z, (float)hi);
-    float* gview = gs + (size_t)v.out_slot * rd.out_d * plane + pix;
-    for (int i = hi; i >= lo; i -= kBatch) {
-      float val[kBatch];
-      bool in[kBatch];
-      read_border_batch<kBatch>(reinterpret_cast<const float4*>(dens), rd.in, q, nearp, step, i, -1, lo, val, in);
+  }
+  // one sample index for the whole wave (coalesced plane loads and stores);
+  // each lane acts only inside its own range
+  const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
+  for (int i = whi; i >= wlo; i -= kWalkBatch) {
+    float sg[kWalkBatch];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        if (!in[u]) continue;  // the sample read 0: nothing stored, D and g_D unchanged
-        const float sigma = rp.step * val[u];
+    for (int u = 0; u < kWalkBatch; ++u) sg[u] = (i - u >= lo && i - u <= hi) ? gview[(size_t)(i - u) * plane] : 0.f;
+#pragma unroll
+    for (int u = 0; u < kWalkBatch; ++u) {
+      if (i - u >= lo && i - u <= hi) {
+        const float sigma = sg[u];
         const float e = __expf(-D);
         D = fmaxf(D - sigma, 0.f);
         const float esg = e * sigma * G;
@@ -611,27 +649,26 @@ hipError_t quad_copy_launch(hipStream_t s, const float* lin, float* out, Dims3 d
 
 // BORDER: dens is the corner-quad copy.
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp) {
+                                     const RayLaunch& L, const RenderParams& rp, float* gs, float4* rays) {
   const dim3 grid = tile_grid(rd, L.num_views);
   const QuadDims q = make_quad_dims(rd.in);
   KernelTimer t(s, "render_fwd");
   switch (bm) {
-    case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q); break;
-    case B_CLAMP: hipLaunchKernelGGL(k_render_density_fwd<B_CLAMP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q); break;
-    default: hipLaunchKernelGGL(k_render_density_fwd<B_WRAP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q); break;
+    case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, gs, rays); break;
+    case B_CLAMP: hipLaunchKernelGGL(k_render_density_fwd<B_CLAMP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, nullptr, nullptr); break;
+    default: hipLaunchKernelGGL(k_render_density_fwd<B_WRAP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, nullptr, nullptr); break;
   }
   return hipGetLastError();
 }
 
-// BORDER walk (reads the corner-quad copy, writes gs, rays) for one chunk of views
-hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* dens, const float* img,
-                                      const float* second, float* gs, float4* rays, float* loss, const RayDims& rd,
-                                      const RayLaunch& L, const RenderParams& rp) {
+// BORDER walk over the saved sigma (rewrites gs in place) for one chunk of views
+hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* img, const float* second, float* gs,
+                                      float4* rays, float* loss, const RayDims& rd, const RayLaunch& L,
+                                      const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
-  const QuadDims q = make_quad_dims(rd.in);
   KernelTimer t(s, "render_bwd_walk");
-  if (l2) hipLaunchKernelGGL(k_render_density_bwd_walk<true>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp, q);
-  else hipLaunchKernelGGL(k_render_density_bwd_walk<false>, grid, dim3(256), 0, s, dens, img, second, gs, rays, loss, rd, L, rp, q);
+  if (l2) hipLaunchKernelGGL(k_render_density_bwd_walk<true>, grid, dim3(256), 0, s, img, second, gs, rays, loss, rd, L, rp);
+  else hipLaunchKernelGGL(k_render_density_bwd_walk<false>, grid, dim3(256), 0, s, img, second, gs, rays, loss, rd, L, rp);
   return hipGetLastError();
 }
 

@@ -57,13 +57,13 @@ class TransportChain(object):
         self._shape = _lib.i64_array(self.shape)
         self._nsaved = L.gft_advect_saved_bytes(self._shape)
         self._nws = L.gft_advect_bwd_workspace(self._shape)
-        self._nrws = L.gft_render_density_bwd_workspace(self._shape, self.cams.nCam, self.cams.c_out, self.boundary)
+        self._nrsaved = L.gft_render_density_saved_bytes(self._shape, self.cams.nCam, self.cams.c_out, self.boundary)
         nV, h, w = self.cams.nCam, self.cams.out_dhw[1], self.cams.out_dhw[2]
         self.loss_scale = 1.0 / (nV * h * w * 3)
         dev = self.device
         self.images = torch.empty((1, nV, h, w, 4), device=dev)
         self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
-        self.render_ws = torch.empty((max(self._nrws, 1),), dtype=torch.uint8, device=dev)
+        self.render_saved = torch.empty((max(self._nrsaved, 1),), dtype=torch.uint8, device=dev)
         self._nfws = L.gft_render_density_fwd_workspace(self._shape, self.boundary)
         self.render_fwd_ws = torch.empty((max(self._nfws, 1),), dtype=torch.uint8, device=dev)
         self.frame_loss = None
@@ -85,12 +85,12 @@ class TransportChain(object):
         _lib.check(L.gft_render_density_fwd(s, _lib.ptr(d), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                             _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light, self.depth_step,
                                             self.boundary, _lib.ptr(self.render_fwd_ws), self._nfws,
-                                            _lib.ptr(self.images)))
+                                            _lib.ptr(self.render_saved), self._nrsaved, _lib.ptr(self.images)))
         _lib.check(L.gft_render_density_bwd_l2(s, _lib.ptr(d), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                                _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
                                                self.depth_step, self.boundary, _lib.ptr(self.images),
-                                               _lib.ptr(target), self.loss_scale, _lib.ptr(self.render_ws),
-                                               self._nrws, _lib.ptr(grad), _lib.ptr(loss)))
+                                               _lib.ptr(target), self.loss_scale, _lib.ptr(self.render_saved),
+                                               self._nrsaved, None, 0, _lib.ptr(grad), _lib.ptr(loss)))
 
     # ------------------------------------------------------------------- run
     def run(self, d0, velocities, targets):

@@ -109,7 +109,16 @@ class _Cameras(object):
         self.c_out = _lib.i32_array(self.out_dhw)
 
 
-def render_density_forward(d, cams, depth_step, light=(1.0, 1.0, 1.0), boundary="BORDER", out=None, ws=None):
+def render_saved_bytes(shape, cams, boundary="BORDER"):
+    """Bytes of the buffer a forward fills for its backward (0 unless BORDER)."""
+    return int(_lib.lib().gft_render_density_saved_bytes(_lib.i64_array(shape), cams.nCam, cams.c_out,
+                                                         _enum(BOUNDARY, boundary, "boundary")))
+
+
+def render_density_forward(d, cams, depth_step, light=(1.0, 1.0, 1.0), boundary="BORDER", out=None, ws=None,
+                           saved=None):
+    """``saved``: optional uint8 device buffer of render_saved_bytes() that
+    receives what the backward of this forward needs."""
     L = _lib.lib()
     if out is None:
         out = torch.empty((1, cams.nCam, cams.out_dhw[1], cams.out_dhw[2], 4), device=d.device, dtype=torch.float32)
@@ -119,35 +128,39 @@ def render_density_forward(d, cams, depth_step, light=(1.0, 1.0, 1.0), boundary=
     nws = L.gft_render_density_fwd_workspace(shape, bm)
     if ws is None:
         ws = torch.empty((max(int(nws), 1),), dtype=torch.uint8, device=d.device)
+    sp, sn = (_lib.ptr(saved), saved.numel()) if saved is not None else (None, 0)
     _lib.check(L.gft_render_density_fwd(_lib.stream_of(d), _lib.ptr(d), shape, _np_ptr(cams.M), cams.nM,
                                         _np_ptr(cams.V), _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt,
-                                        float(depth_step), bm, _lib.ptr(ws), nws, _lib.ptr(out)))
+                                        float(depth_step), bm, _lib.ptr(ws), nws, sp, sn, _lib.ptr(out)))
     return out
 
 
 class _RenderFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, d, cams, depth_step, light, boundary):
-        img = render_density_forward(d, cams, depth_step, light, boundary)
-        ctx.save_for_backward(d, img)
+        nsaved = render_saved_bytes(d.shape, cams, boundary)
+        saved = torch.empty((max(nsaved, 1),), dtype=torch.uint8, device=d.device)
+        img = render_density_forward(d, cams, depth_step, light, boundary, saved=saved)
+        ctx.save_for_backward(d, img, saved)
         ctx.cfg = (cams, depth_step, light, boundary)
         return img
 
     @staticmethod
     def backward(ctx, g):
-        d, img = ctx.saved_tensors
+        d, img, saved = ctx.saved_tensors
         cams, depth_step, light, boundary = ctx.cfg
         gd = torch.empty_like(d)
         lt = (ctypes.c_float * 3)(*[float(x) for x in light])
         L = _lib.lib()
         shape = _lib.i64_array(d.shape)
         bm = _enum(BOUNDARY, boundary, "boundary")
-        nws = L.gft_render_density_bwd_workspace(shape, cams.nCam, cams.c_out, bm)
-        ws = torch.empty((max(int(nws), 1),), dtype=torch.uint8, device=d.device)
+        # the backward overwrites the saved buffer: work on a copy so that a
+        # retained graph can run backward again
+        saved = saved.clone()
         _lib.check(L.gft_render_density_bwd(
             _lib.stream_of(d), _lib.ptr(d), shape, _np_ptr(cams.M), cams.nM, _np_ptr(cams.V),
             _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt, float(depth_step), bm, _lib.ptr(img),
-            _lib.ptr(g.contiguous()), _lib.ptr(ws), nws, _lib.ptr(gd)))
+            _lib.ptr(g.contiguous()), _lib.ptr(saved), saved.numel(), None, 0, _lib.ptr(gd)))
         return gd, None, None, None, None
 
 
@@ -194,8 +207,8 @@ class TransportRenderStep(object):
         self._nws = L.gft_advect_bwd_workspace(self._shape)
         self.saved = torch.empty((self._nsaved,), dtype=torch.uint8, device=dev)
         self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
-        self._nrws = L.gft_render_density_bwd_workspace(self._shape, self.cams.nCam, self.cams.c_out, self.boundary)
-        self.render_ws = torch.empty((max(self._nrws, 1),), dtype=torch.uint8, device=dev)
+        self._nrsaved = L.gft_render_density_saved_bytes(self._shape, self.cams.nCam, self.cams.c_out, self.boundary)
+        self.render_saved = torch.empty((max(self._nrsaved, 1),), dtype=torch.uint8, device=dev)
         self._nfws = L.gft_render_density_fwd_workspace(self._shape, self.boundary)
         self.render_fwd_ws = torch.empty((max(self._nfws, 1),), dtype=torch.uint8, device=dev)
 
@@ -217,18 +230,20 @@ class TransportRenderStep(object):
         _lib.check(L.gft_render_density_fwd(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                             _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light, self.depth_step,
                                             self.boundary, _lib.ptr(self.render_fwd_ws), self._nfws,
-                                            _lib.ptr(self.images)))
+                                            _lib.ptr(self.render_saved), self._nrsaved, _lib.ptr(self.images)))
         self._mark(timers, "render_fwd", False)
         return self.images
 
     def backward(self, d0, vel, target, timers=None):
+        """Needs the forward just before it: with BORDER the render backward
+        consumes (overwrites) the per-sample record the forward saved."""
         L, c, s = self._L, self.cams, _lib.stream_of(d0)
         self._mark(timers, "render_bwd", True)
         _lib.check(L.gft_render_density_bwd_l2(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                                _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
                                                self.depth_step, self.boundary, _lib.ptr(self.images),
-                                               _lib.ptr(target), self.loss_scale, _lib.ptr(self.render_ws),
-                                               self._nrws, _lib.ptr(self.grad_d1), _lib.ptr(self.loss)))
+                                               _lib.ptr(target), self.loss_scale, _lib.ptr(self.render_saved),
+                                               self._nrsaved, None, 0, _lib.ptr(self.grad_d1), _lib.ptr(self.loss)))
         self._mark(timers, "render_bwd", False)
         self._mark(timers, "advect_bwd", True)
         _lib.check(L.gft_advect_bwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,
@@ -255,8 +270,9 @@ class TransportRenderStep(object):
         """The per-phase compulsory bytes split over the kernels that touch
         those tensors: the first advection pass reads d and v, the second
         writes d'; the render forward reads d' and writes the images; the
-        BORDER walk reads d', images and targets and the gather writes g_d'
-        (the CLAMP/WRAP scatter does both); advect_bwd_a reads g_d' and v,
+        BORDER walk reads images and targets and stands for the read of d'
+        (it consumes the forward's per-sample record instead) and the gather
+        writes g_d' (the CLAMP/WRAP scatter does both); advect_bwd_a reads g_d' and v,
         advect_bwd_b reads d and writes g_d and g_v."""
         V = int(np.prod(self.shape))
         N = self.n_pixels

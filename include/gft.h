@@ -112,22 +112,28 @@ int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], con
  * (raymarch_grid.cu.cc:107-143) without the materialised 4-channel grid.
  * BORDER reads a corner-quad copy of the density (one float4 of x/y corner
  * neighbours per cell) that the call writes into the workspace
- * (gft_render_density_fwd_workspace; 0 bytes for CLAMP/WRAP).
+ * (gft_render_density_fwd_workspace; 0 bytes for CLAMP/WRAP). With BORDER a
+ * non-null `saved` buffer (gft_render_density_saved_bytes) receives every
+ * ray's sample range and per-sample extinction for the backward; it may be
+ * null when no backward follows.
  */
 size_t gft_render_density_fwd_workspace(const int64_t shape[5], int32_t boundary);
+size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
+                                      int32_t boundary);
 int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                           void* workspace, size_t workspace_bytes, float* images);
+                           void* workspace, size_t workspace_bytes, void* saved, size_t saved_bytes, float* images);
 /*
  * Backward of gft_render_density_fwd with the L2 image loss folded in:
  * loss = loss_scale * sum (rgb - target)^2 (targets [1,V,H,W,3]); writes the
  * scalar loss (device float) and the exact density gradient (overwritten).
  * Replaces loss_dens_target (optimization.py:396-412) + the raymarch gradient
- * (raymarch_grid.cu.cc:343-372). BORDER needs the workspace returned by
- * gft_render_density_bwd_workspace (per-sample gradients of every view, the
- * per-pixel rays and a corner-quad density copy) and is deterministic; CLAMP/WRAP
- * need none.
+ * (raymarch_grid.cu.cc:343-372). BORDER needs the `saved` buffer that the
+ * forward filled from the same density and cameras; the backward overwrites it
+ * with per-sample gradients and is deterministic. CLAMP/WRAP need none.
+ * gft_render_density_bwd_workspace is 0 for every boundary; the workspace
+ * arguments are kept for callers that size one.
  */
 size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                         int32_t boundary);
@@ -135,14 +141,14 @@ int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t 
                               int32_t num_M, const float* V, const float* P, const float* frustum,
                               int32_t num_cameras, const int32_t output_shape[3], const float light[3],
                               float depth_step, int32_t boundary, const float* images, const float* targets,
-                              float loss_scale, void* workspace, size_t workspace_bytes, float* density_grad,
-                              float* loss);
+                              float loss_scale, void* saved, size_t saved_bytes, void* workspace,
+                              size_t workspace_bytes, float* density_grad, float* loss);
 /* Backward of gft_render_density_fwd from an image gradient [1,V,H,W,4]. */
 int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                           const float* images, const float* images_grad, void* workspace, size_t workspace_bytes,
-                           float* density_grad);
+                           const float* images, const float* images_grad, void* saved, size_t saved_bytes,
+                           void* workspace, size_t workspace_bytes, float* density_grad);
 
 /*
  * Mip pyramid of a grid batch [N,D,H,W,C]: levels 1..L (L = num_mipmaps,

@@ -38,7 +38,7 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
 
 
 def test_abi_version_and_error_slot(lib):
-    assert lib.gft_abi_version() == 2
+    assert lib.gft_abi_version() == 3
     assert isinstance(lib.gft_last_error(), bytes)
 
 

@@ -137,5 +137,7 @@ def test_border_backward_is_deterministic(cuda):
     step.backward(td, tv, tgt)
     first = step.grad_d1.clone()
     for _ in range(3):
+        # the backward consumes the forward's saved samples: run the pair again
+        step.forward(td, tv)
         step.backward(td, tv, tgt)
         assert torch.equal(step.grad_d1, first)
