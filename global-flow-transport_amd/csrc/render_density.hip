@@ -314,45 +314,52 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
   if (L2) block_loss(sq, loss, rp.loss_scale);
 }
 
-// Gather: one lane per run of ZR voxels c = (x, y, z0..z0+ZR-1), all views of
-// the launch summed in order. A sample q (cell-centre offset removed) reaches
-// c with weight prod_a max(0, 1 - |q_a - c_a|). The run's support box
-// (x-1, x+1) x (y-1, y+1) x (z0-1, z0+ZR) is projected to find the candidate
-// pixels, and each candidate ray is cut by the box to the samples strictly
-// inside it (a sample on the box surface has weight 0; the 1e-3 margin in the
-// ray parameter only adds samples, whose weight test rejects them). A sample
-// is loaded once for the whole run; only samples with positive weight that the
-// walk stored (inside the grid) are loaded.
-template <int ZR>
+// Gather: one lane per block of BX x BY x BZ voxels c = (x0.., y0.., z0..),
+// all views of the launch summed in order. A sample q (cell-centre offset
+// removed) reaches c with weight prod_a max(0, 1 - |q_a - c_a|). The block's
+// support box (x0-1, x0+BX) x (y0-1, y0+BY) x (z0-1, z0+BZ) is projected to
+// find the candidate pixels, and each candidate ray is cut by the box to the
+// samples strictly inside it (a sample on the box surface has weight 0; the
+// 1e-3 margin in the ray parameter only adds samples, whose weight test
+// rejects them). A sample is loaded once for the whole block; only samples
+// with positive weight that the walk stored (inside the grid) are loaded.
+// Larger blocks visit fewer candidate rays per voxel and load each sample for
+// more voxels, at the price of fewer lanes.
+template <int BX, int BY, int BZ, int LX>
 __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* __restrict__ gs,
                                                                    const float4* __restrict__ rays,
                                                                    float* __restrict__ grad, RayDims rd,
                                                                    ProjLaunch PL) {
-  const int cx = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int cy = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int cz0 = blockIdx.z * ZR;
+  // LX lanes along x, 256 / LX along y
+  const int cx = (blockIdx.x * LX + (threadIdx.x % LX)) * BX;
+  const int cy = (blockIdx.y * (256 / LX) + (threadIdx.x / LX)) * BY;
+  const int cz0 = blockIdx.z * BZ;
   if (cx >= rd.in.x || cy >= rd.in.y) return;
   const Dims3 d = rd.in;
   const size_t plane = size_t(rd.out_w) * rd.out_h;
   const float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz0;
   // box corner in ray coordinates (which carry the +0.5 cell-centre offset);
-  // the box spans 2 x 2 x (ZR + 1)
+  // the box spans (BX + 1) x (BY + 1) x (BZ + 1)
   const float bx0 = fcx - 0.5f, by0 = fcy - 0.5f, bz0 = fcz - 0.5f;
-  const float hz = 0.5f * (float)(ZR + 1);
-  float acc[ZR];
+  const float hx = 0.5f * (float)(BX + 1), hy = 0.5f * (float)(BY + 1), hz = 0.5f * (float)(BZ + 1);
+  float acc[BZ][BY][BX];
 #pragma unroll
-  for (int r = 0; r < ZR; ++r) acc[r] = 0.f;
+  for (int k = 0; k < BZ; ++k)
+#pragma unroll
+    for (int j = 0; j < BY; ++j)
+#pragma unroll
+      for (int i = 0; i < BX; ++i) acc[k][j][i] = 0.f;
   for (int vi = 0; vi < PL.num_views; ++vi) {
     const ProjView& pv = PL.view[vi];
     const float* A = pv.pmv;
     // clip coordinates of the box corners from its centre and the three
     // column vectors (the map is linear before the divide)
-    const float4 cc = mat_vec(A, fcx + 0.5f, fcy + 0.5f, fcz + 0.5f * (float)ZR, 1.f);
+    const float4 cc = mat_vec(A, fcx + 0.5f * (float)BX, fcy + 0.5f * (float)BY, fcz + 0.5f * (float)BZ, 1.f);
     float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
     bool behind = false;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float sx = (k & 1) ? 1.f : -1.f, sy = (k & 2) ? 1.f : -1.f, sz = (k & 4) ? hz : -hz;
+      const float sx = (k & 1) ? hx : -hx, sy = (k & 2) ? hy : -hy, sz = (k & 4) ? hz : -hz;
       const float cxk = fmaf(sz, A[8], fmaf(sy, A[4], fmaf(sx, A[0], cc.x)));
       const float cyk = fmaf(sz, A[9], fmaf(sy, A[5], fmaf(sx, A[1], cc.y)));
       const float cwk = fmaf(sz, A[11], fmaf(sy, A[7], fmaf(sx, A[3], cc.w)));
@@ -387,43 +394,66 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
         // component gives an infinite (approximate) reciprocal, an unbounded
         // or empty interval, or NaN bounds that fminf/fmaxf ignore (the weight
         // test stays exact)
-        const float ax = (bx0 - r0.x) * irx, bx = fmaf(2.f, irx, ax);
-        const float ay = (by0 - r0.y) * iry, by = fmaf(2.f, iry, ay);
-        const float az = (bz0 - r0.z) * irz, bz = fmaf((float)(ZR + 1), irz, az);
+        const float ax = (bx0 - r0.x) * irx, bx = fmaf((float)(BX + 1), irx, ax);
+        const float ay = (by0 - r0.y) * iry, by = fmaf((float)(BY + 1), iry, ay);
+        const float az = (bz0 - r0.z) * irz, bz = fmaf((float)(BZ + 1), irz, az);
         const float tlo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
         const float thi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
         const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));
         const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
         for (int i = i0; i <= i1; ++i) {
           const float fi = (float)i;
-          const float ux = fmaf(fi, r1.x, r0.x), uy = fmaf(fi, r1.y, r0.y), uz = fmaf(fi, r1.z, r0.z);
-          // hat weights: 1 - |q - c| with q = ray - 0.5
-          const float wxy = fmaxf(1.f - fabsf(ux - 0.5f - fcx), 0.f) * fmaxf(1.f - fabsf(uy - 0.5f - fcy), 0.f);
-          float wz[ZR];
-          bool any = false;
+          // hat weights 1 - |q - c| with q = ray - 0.5
+          const float qx = fmaf(fi, r1.x, r0.x) - 0.5f, qy = fmaf(fi, r1.y, r0.y) - 0.5f,
+                      qz = fmaf(fi, r1.z, r0.z) - 0.5f;
+          float wx[BX], wy[BY], wz[BZ];
+          bool anyx = false, anyy = false, anyz = false;
 #pragma unroll
-          for (int r = 0; r < ZR; ++r) {
-            wz[r] = fmaxf(1.f - fabsf(uz - 0.5f - (fcz + (float)r)), 0.f) * wxy;
-            any |= wz[r] > 0.f;
+          for (int t = 0; t < BX; ++t) {
+            wx[t] = fmaxf(1.f - fabsf(qx - (fcx + (float)t)), 0.f);
+            anyx |= wx[t] > 0.f;
           }
-          // the walk stored only samples inside [-0.5, dims-0.5)
-          const bool ok = any & (0.f <= ux) & (ux < (float)d.x) & (0.f <= uy) & (uy < (float)d.y) & (0.f <= uz) &
-                          (uz < (float)d.z);
+#pragma unroll
+          for (int t = 0; t < BY; ++t) {
+            wy[t] = fmaxf(1.f - fabsf(qy - (fcy + (float)t)), 0.f);
+            anyy |= wy[t] > 0.f;
+          }
+#pragma unroll
+          for (int t = 0; t < BZ; ++t) {
+            wz[t] = fmaxf(1.f - fabsf(qz - (fcz + (float)t)), 0.f);
+            anyz |= wz[t] > 0.f;
+          }
+          // the walk's value counts only for samples the forward found inside
+          // the grid (the same test on the same positions)
+          const float ix = qx + 0.5f, iy = qy + 0.5f, iz = qz + 0.5f;
+          const bool ok = anyx & anyy & anyz & (0.f <= ix) & (ix < (float)d.x) & (0.f <= iy) & (iy < (float)d.y) &
+                          (0.f <= iz) & (iz < (float)d.z);
           if (ok) {
             const float g = gview[(size_t)i * plane + pix];
 #pragma unroll
-            for (int r = 0; r < ZR; ++r) acc[r] = fmaf(g, wz[r], acc[r]);
+            for (int k = 0; k < BZ; ++k)
+#pragma unroll
+              for (int j = 0; j < BY; ++j) {
+                const float gyz = g * wy[j] * wz[k];
+#pragma unroll
+                for (int t = 0; t < BX; ++t) acc[k][j][t] = fmaf(gyz, wx[t], acc[k][j][t]);
+              }
           }
         }
       }
     }
   }
 #pragma unroll
-  for (int r = 0; r < ZR; ++r) {
-    if (cz0 + r >= d.z) break;
-    float* g = grad + flat3(cx, cy, cz0 + r, d);
-    *g = PL.accumulate ? *g + acc[r] : acc[r];
-  }
+  for (int k = 0; k < BZ; ++k)
+#pragma unroll
+    for (int j = 0; j < BY; ++j)
+#pragma unroll
+      for (int t = 0; t < BX; ++t) {
+        if (cz0 + k < d.z && cy + j < d.y && cx + t < d.x) {
+          float* g = grad + flat3(cx + t, cy + j, cz0 + k, d);
+          *g = PL.accumulate ? *g + acc[k][j][t] : acc[k][j][t];
+        }
+      }
 }
 
 // ----------------------------------------- CLAMP: tiled LDS pre-reduction
@@ -672,13 +702,17 @@ hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* img, 
   return hipGetLastError();
 }
 
-constexpr int kGatherZRun = 4;
+constexpr int kGatherBX = 1, kGatherBY = 2, kGatherBZ = 4, kGatherLX = 64;
 
 hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
                                         const RayDims& rd, const ProjLaunch& PL) {
-  const dim3 grid((rd.in.x + 63) / 64, (rd.in.y + 3) / 4, (rd.in.z + kGatherZRun - 1) / kGatherZRun);
+  constexpr int ly = 256 / kGatherLX;
+  const dim3 grid((rd.in.x + kGatherLX * kGatherBX - 1) / (kGatherLX * kGatherBX),
+                  (rd.in.y + ly * kGatherBY - 1) / (ly * kGatherBY),
+                  (rd.in.z + kGatherBZ - 1) / kGatherBZ);
   KernelTimer t(s, "render_bwd_gather");
-  hipLaunchKernelGGL(k_render_density_bwd_gather<kGatherZRun>, grid, dim3(256), 0, s, gs, rays, grad, rd, PL);
+  auto kern = k_render_density_bwd_gather<kGatherBX, kGatherBY, kGatherBZ, kGatherLX>;
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, gs, rays, grad, rd, PL);
   return hipGetLastError();
 }
 
