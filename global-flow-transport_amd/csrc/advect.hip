@@ -160,21 +160,28 @@ __global__ __launch_bounds__(256) void k_advect_mc(const float* __restrict__ den
 
 // ----------------------------------------------------------- backward
 // The adjoint scatters are accumulated per tile of kTX x kTY x kTZ voxels in
-// an LDS region covering the tile plus a kHalo-cell margin (positions moved by
-// less than kHalo cells in every axis have all 8 corners inside it), with LDS
-// atomics; the region is then added to the global gradient with one coalesced
+// an LDS region covering the tile plus a kHalo-cell margin (a position moved
+// by s with -kHalo <= s < kHalo - 1 in every axis has all 8 corners inside
+// it), with LDS atomics; the region is then added to the global gradient with one coalesced
 // global atomic per nonzero cell. Corners outside the region (faster motion)
 // go straight to a global atomic, so any velocity is handled exactly.
-constexpr int kTX = 32, kTY = 8, kTZ = 8, kHalo = 3;
+constexpr int kTX = 32, kTY = 4, kTZ = 4, kHalo = 3;
 constexpr int kRX = kTX + 2 * kHalo, kRY = kTY + 2 * kHalo, kRZ = kTZ + 2 * kHalo;
 constexpr int kRegion = kRX * kRY * kRZ;
 
+// The region accumulates in FP64: ds_add_f64 runs at about 20x the rate of
+// ds_add_f32 on gfx950 (measured 7 vs 0.33 lane-ops per clock per CU), and the
+// FP64 sum of the FP32 addends is rounded once when it is flushed. The pointer
+// carries the LDS address space (a generic pointer held in the struct would
+// make the compiler emit flat atomics).
+typedef __attribute__((address_space(3))) double lds_double;
+
 struct TileScatter {
-  float* lds;
+  lds_double* lds;
   int x0, y0, z0;  // tile origin
 
   __device__ __forceinline__ void clear() {
-    for (int k = threadIdx.x; k < kRegion; k += blockDim.x) lds[k] = 0.f;
+    for (int k = threadIdx.x; k < kRegion; k += blockDim.x) lds[k] = 0.0;
   }
   __device__ __forceinline__ void add(float* __restrict__ g, Dims3 d, const Tri& t, float val) {
     const float wx[2] = {1.f - t.wx, t.wx}, wy[2] = {1.f - t.wy, t.wy}, wz[2] = {1.f - t.wz, t.wz};
@@ -188,14 +195,15 @@ struct TileScatter {
           if (w == 0.f) continue;
           const int lx = t.ix[i] - x0 + kHalo, ly = t.iy[j] - y0 + kHalo, lz = t.iz[k] - z0 + kHalo;
           if (((unsigned)lx < (unsigned)kRX) & ((unsigned)ly < (unsigned)kRY) & ((unsigned)lz < (unsigned)kRZ))
-            atomicAdd(lds + (lz * kRY + ly) * kRX + lx, val * w);
+            __hip_atomic_fetch_add(lds + (lz * kRY + ly) * kRX + lx, (double)(val * w), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
           else
             atomic_addf(g + flat3(t.ix[i], t.iy[j], t.iz[k], d), val * w);
         }
   }
   __device__ __forceinline__ void flush(float* __restrict__ g, Dims3 d) {
     for (int k = threadIdx.x; k < kRegion; k += blockDim.x) {
-      const float v = lds[k];
+      const float v = (float)lds[k];
       if (v == 0.f) continue;
       const int x = x0 - kHalo + k % kRX, y = y0 - kHalo + (k / kRX) % kRY, z = z0 - kHalo + k / (kRX * kRY);
       // cells outside the grid never receive a corner (indices are clamped)
@@ -218,8 +226,8 @@ __global__ __launch_bounds__(256) void k_advect_bwd_a(const float* __restrict__ 
                                                       const uint8_t* __restrict__ mask,
                                                       const float* __restrict__ gout, float* __restrict__ gf_acc,
                                                       float* __restrict__ gd, float* __restrict__ gv, AdvectDims a) {
-  __shared__ float region[kRegion];
-  TileScatter ts{region, (int)blockIdx.x * kTX, (int)blockIdx.y * kTY, (int)blockIdx.z * kTZ};
+  __shared__ double region[kRegion];
+  TileScatter ts{(lds_double*)region, (int)blockIdx.x * kTX, (int)blockIdx.y * kTY, (int)blockIdx.z * kTZ};
   ts.clear();
   __syncthreads();
   for (int k = threadIdx.x; k < kTX * kTY * kTZ; k += blockDim.x) {
@@ -254,8 +262,8 @@ __global__ __launch_bounds__(256) void k_advect_bwd_b(const float* __restrict__ 
                                                       const float* __restrict__ gf_acc,
                                                       const uint8_t* __restrict__ mask, float* __restrict__ gd,
                                                       float* __restrict__ gv, AdvectDims a) {
-  __shared__ float region[kRegion];
-  TileScatter ts{region, (int)blockIdx.x * kTX, (int)blockIdx.y * kTY, (int)blockIdx.z * kTZ};
+  __shared__ double region[kRegion];
+  TileScatter ts{(lds_double*)region, (int)blockIdx.x * kTX, (int)blockIdx.y * kTY, (int)blockIdx.z * kTZ};
   ts.clear();
   __syncthreads();
   for (int k = threadIdx.x; k < kTX * kTY * kTZ; k += blockDim.x) {
