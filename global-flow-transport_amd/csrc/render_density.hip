@@ -145,6 +145,9 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
 
 constexpr int kBatch = 4;
 constexpr int kRayRec = 2;  // float4 records per pixel ray: (near, lo), (step, hi)
+// saved value of a sample outside the grid (quiet NaN; never a valid extinction
+// of a finite density)
+#define kOutsideMark __builtin_nanf("")
 
 __device__ __forceinline__ int wave_min(int v) {
   for (int m = 32; m > 0; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
@@ -201,7 +204,9 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
         const float sigma = act ? rp.step * val[u] : 0.f;
         D = act ? fmaxf(D + sigma, 0.f) : D;
         E = fmaf(__expf(-D), sigma, E);
-        if (gview && i + u <= whi) gview[(size_t)(i + u) * plane] = sigma;
+        // the saved value of a sample outside the grid is a NaN mark: the walk
+        // stores a zero gradient there, so the gather needs no inside test
+        if (gview && i + u <= whi) gview[(size_t)(i + u) * plane] = act ? sigma : kOutsideMark;
       }
     }
   } else {
@@ -261,8 +266,8 @@ __device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, f
 // Walk over the forward's saved sigma: for every pixel of the chunk, from far
 // to near over its clipped range, D_{i-1} = max(D_i - sigma_i, 0) and
 // g_i = s * (exp(-D_i) G + g_D), g_D -= exp(-D_i) sigma_i G; g_i overwrites
-// sigma_i in place (samples outside the grid get a value the gather never
-// reads). A pixel with no gradient gets lo > hi so the gather skips it.
+// sigma_i in place (0 for samples outside the grid, which the forward marked).
+// A pixel with no gradient gets lo > hi so the gather skips it.
 constexpr int kWalkBatch = 8;
 
 template <bool L2>
@@ -303,11 +308,15 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
     for (int u = 0; u < kWalkBatch; ++u) {
       if (i - u >= lo && i - u <= hi) {
         const float sigma = sg[u];
-        const float e = __expf(-D);
-        D = fmaxf(D - sigma, 0.f);
-        const float esg = e * sigma * G;
-        gview[(size_t)(i - u) * plane] = rp.step * (e * G + (gD - esg));
-        gD -= esg;
+        float gval = 0.f;  // a sample outside the grid gets a zero gradient
+        if (sigma == sigma) {
+          const float e = __expf(-D);
+          D = fmaxf(D - sigma, 0.f);
+          const float esg = e * sigma * G;
+          gval = rp.step * (e * G + (gD - esg));
+          gD -= esg;
+        }
+        gview[(size_t)(i - u) * plane] = gval;
       }
     }
   }
@@ -392,8 +401,8 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
                     irz = __builtin_amdgcn_rcpf(r1.z);
         // parameter interval of the ray inside the open box; a zero step
         // component gives an infinite (approximate) reciprocal, an unbounded
-        // or empty interval, or NaN bounds that fminf/fmaxf ignore (the weight
-        // test stays exact)
+        // or empty interval, or NaN bounds that fminf/fmaxf ignore (the
+        // weights stay exact)
         const float ax = (bx0 - r0.x) * irx, bx = fmaf((float)(BX + 1), irx, ax);
         const float ay = (by0 - r0.y) * iry, by = fmaf((float)(BY + 1), iry, ay);
         const float az = (bz0 - r0.z) * irz, bz = fmaf((float)(BZ + 1), irz, az);
@@ -401,44 +410,46 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
         const float thi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
         const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));
         const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
+        if (i0 > i1) continue;
+        // sample position relative to the block's first voxel, stepped along
+        // the ray; samples outside the grid carry a zero gradient
+        const float fi0 = (float)i0;
+        float ux = fmaf(fi0, r1.x, r0.x) - (fcx + 0.5f), uy = fmaf(fi0, r1.y, r0.y) - (fcy + 0.5f),
+              uz = fmaf(fi0, r1.z, r0.z) - (fcz + 0.5f);
+        const float* gp = gview + (size_t)i0 * plane + pix;
         for (int i = i0; i <= i1; ++i) {
-          const float fi = (float)i;
-          // hat weights 1 - |q - c| with q = ray - 0.5
-          const float qx = fmaf(fi, r1.x, r0.x) - 0.5f, qy = fmaf(fi, r1.y, r0.y) - 0.5f,
-                      qz = fmaf(fi, r1.z, r0.z) - 0.5f;
-          float wx[BX], wy[BY], wz[BZ];
-          bool anyx = false, anyy = false, anyz = false;
+          const float g = *gp;
+          gp += plane;
+          // hat weights max(0, 1 - |u - t|)
+          float wy[BY], wz[BZ];
+          float gx = g;
 #pragma unroll
-          for (int t = 0; t < BX; ++t) {
-            wx[t] = fmaxf(1.f - fabsf(qx - (fcx + (float)t)), 0.f);
-            anyx |= wx[t] > 0.f;
-          }
+          for (int t = 0; t < BX; ++t) gx = t == 0 ? g * fmaxf(1.f - fabsf(ux), 0.f) : gx;
 #pragma unroll
-          for (int t = 0; t < BY; ++t) {
-            wy[t] = fmaxf(1.f - fabsf(qy - (fcy + (float)t)), 0.f);
-            anyy |= wy[t] > 0.f;
-          }
+          for (int t = 0; t < BY; ++t) wy[t] = fmaxf(1.f - fabsf(uy - (float)t), 0.f);
 #pragma unroll
-          for (int t = 0; t < BZ; ++t) {
-            wz[t] = fmaxf(1.f - fabsf(qz - (fcz + (float)t)), 0.f);
-            anyz |= wz[t] > 0.f;
-          }
-          // the walk's value counts only for samples the forward found inside
-          // the grid (the same test on the same positions)
-          const float ix = qx + 0.5f, iy = qy + 0.5f, iz = qz + 0.5f;
-          const bool ok = anyx & anyy & anyz & (0.f <= ix) & (ix < (float)d.x) & (0.f <= iy) & (iy < (float)d.y) &
-                          (0.f <= iz) & (iz < (float)d.z);
-          if (ok) {
-            const float g = gview[(size_t)i * plane + pix];
+          for (int t = 0; t < BZ; ++t) wz[t] = fmaxf(1.f - fabsf(uz - (float)t), 0.f);
+          if (BX == 1) {
+#pragma unroll
+            for (int j = 0; j < BY; ++j) {
+              const float gxy = gx * wy[j];
+#pragma unroll
+              for (int k = 0; k < BZ; ++k) acc[k][j][0] = fmaf(gxy, wz[k], acc[k][j][0]);
+            }
+          } else {
 #pragma unroll
             for (int k = 0; k < BZ; ++k)
 #pragma unroll
               for (int j = 0; j < BY; ++j) {
                 const float gyz = g * wy[j] * wz[k];
 #pragma unroll
-                for (int t = 0; t < BX; ++t) acc[k][j][t] = fmaf(gyz, wx[t], acc[k][j][t]);
+                for (int t = 0; t < BX; ++t)
+                  acc[k][j][t] = fmaf(gyz, fmaxf(1.f - fabsf(ux - (float)t), 0.f), acc[k][j][t]);
               }
           }
+          ux += r1.x;
+          uy += r1.y;
+          uz += r1.z;
         }
       }
     }
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
 // skipped). A brick over the LDS capacity, or a corner outside the brick
 // through rounding, goes straight to a global atomic.
 constexpr int kSegK = 8;
-constexpr int kCap = 7168;
+constexpr int kCap = 6144;  // FP64 brick cells: ds_add_f64 is ~20x faster than ds_add_f32 on gfx950
 
 struct Brick {
   int lo[3], n[3];
@@ -510,7 +521,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float* _
                                                                   const float* __restrict__ second,
                                                                   float* __restrict__ grad, float* __restrict__ loss,
                                                                   RayDims rd, RayLaunch L, RenderParams rp) {
-  __shared__ float lg[kCap];
+  __shared__ double lg[kCap];
   const RayView& v = L.view[blockIdx.z];
   const int x0 = blockIdx.x * kRTileX, y0 = blockIdx.y * kRTileY;
   const int x = x0 + (threadIdx.x % kRTileX), y = y0 + (threadIdx.x / kRTileX);
@@ -531,7 +542,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float* _
     pixel_ray(v, rd, x, y, nearp, step);
   }
   const bool walk = active && (G != 0.f || gD != 0.f);
-  for (int i = threadIdx.x; i < kCap; i += 256) lg[i] = 0.f;
+  for (int i = threadIdx.x; i < kCap; i += 256) lg[i] = 0.0;
   __syncthreads();
   const int S = rd.out_d;
   for (int sgi = (S + kSegK - 1) / kSegK - 1; sgi >= 0; --sgi) {
@@ -570,7 +581,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float* _
               const int bx = tp.ix[ii] - br.lo[0], by = tp.iy[j] - br.lo[1], bz = tp.iz[k] - br.lo[2];
               const bool in = use_lds & ((unsigned)bx < (unsigned)br.n[0]) & ((unsigned)by < (unsigned)br.n[1]) &
                               ((unsigned)bz < (unsigned)br.n[2]);
-              if (in) atomicAdd(&lg[(bz * br.n[1] + by) * br.n[0] + bx], gds * w);
+              if (in) atomicAdd(&lg[(bz * br.n[1] + by) * br.n[0] + bx], (double)(gds * w));
               else atomic_addf(gout + flat3(tp.ix[ii], tp.iy[j], tp.iz[k], rd.in), gds * w);
             }
       }
@@ -582,10 +593,10 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float* _
       int bx = threadIdx.x % nx, t = threadIdx.x / nx;
       int by = t % ny, bz = t / ny;
       for (int idx = threadIdx.x; idx < br.size; idx += 256) {
-        const float val = lg[idx];
+        const float val = (float)lg[idx];
         if (val != 0.f) {
           atomic_addf(gout + flat3(br.lo[0] + bx, br.lo[1] + by, br.lo[2] + bz, rd.in), val);
-          lg[idx] = 0.f;
+          lg[idx] = 0.0;
         }
         bx += r;
         by += q;
