@@ -13,11 +13,11 @@ hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel,
 hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
                              const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a);
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp, float* gs, float4* rays);
+                                     const RayLaunch& L, const RenderParams& rp, float* gs, int2* rays);
 hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* img, const float* second, float* gs,
-                                      float4* rays, float* loss, const RayDims& rd, const RayLaunch& L,
+                                      int2* rays, float* loss, const RayDims& rd, const RayLaunch& L,
                                       const RenderParams& rp);
-hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
+hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const int2* rays, float* grad,
                                         const RayDims& rd, const ProjLaunch& PL);
 hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
                                          const float* second, float* grad, float* loss, const RayDims& rd,
@@ -48,8 +48,43 @@ static int check_advect_mode(int32_t order, int32_t clamp, float dt) {
   return GFT_OK;
 }
 
+// Homogeneous object-space point of the raymarch's index->object map at pixel
+// (px, py) and depth index iz as g0 + px gx + py gy: ndc = (2 (i + 0.5) / n - 1)
+// per axis, h = Pinv ndc, object = MVinv (h.xyz / h.w) = (A h).xyz / h.w with A
+// the affine part of MVinv (its last row is not used, as in the device map).
+static void ray_coefficients(const Mat4d& pinv, const Mat4d& mvinv, const RayDims& rd, ProjView& pv) {
+  const double W = rd.out_w, H = rd.out_h, S = rd.out_d;
+  const double zmax = rd.out_d > 1 ? rd.out_d - 1 : 1;
+  auto hcol = [&](int c, double* out) {
+    for (int r = 0; r < 4; ++r) out[r] = pinv.m[4 * c + r];  // column-major
+  };
+  auto affine = [&](const double* h, float* out) {
+    for (int r = 0; r < 3; ++r)
+      out[r] = (float)(mvinv.m[r] * h[0] + mvinv.m[4 + r] * h[1] + mvinv.m[8 + r] * h[2] + mvinv.m[12 + r] * h[3]);
+    out[3] = (float)h[3];
+  };
+  double c0[4], c1[4], c2[4], c3[4];
+  hcol(0, c0);
+  hcol(1, c1);
+  hcol(2, c2);
+  hcol(3, c3);
+  double hx[4], hy[4], hn[4], hf[4];
+  const double zn = 2.0 * 0.5 / S - 1.0, zf = 2.0 * (zmax + 0.5) / S - 1.0;
+  for (int r = 0; r < 4; ++r) {
+    hx[r] = c0[r] * (2.0 / W);
+    hy[r] = c1[r] * (2.0 / H);
+    const double base = c0[r] * (1.0 / W - 1.0) + c1[r] * (1.0 / H - 1.0) + c3[r];
+    hn[r] = base + c2[r] * zn;
+    hf[r] = base + c2[r] * zf;
+  }
+  affine(hx, pv.gx);
+  affine(hy, pv.gy);
+  affine(hn, pv.near0);
+  affine(hf, pv.far0);
+}
+
 static int ray_views(const std::vector<ViewPair>& pairs, size_t first, size_t count, const float* M, const float* V,
-                     const float* P, RayLaunch& L, ProjLaunch* PL = nullptr) {
+                     const float* P, RayLaunch& L, ProjLaunch* PL = nullptr, const RayDims* rd = nullptr) {
   L.num_views = (int)count;
   if (PL) PL->num_views = (int)count;
   for (size_t i = 0; i < count; ++i) {
@@ -64,7 +99,9 @@ static int ray_views(const std::vector<ViewPair>& pairs, size_t first, size_t co
     L.view[i].out_slot = vp.slot;
     if (PL) {
       mat_to_float(mat_mul(p, mv), PL->view[i].pmv);
+      ray_coefficients(pinv, mvinv, *rd, PL->view[i]);
       PL->view[i].out_slot = vp.slot;
+      PL->inv_zmax = (float)(1.0 / (rd->out_d > 1 ? rd->out_d - 1 : 1));
     }
   }
   return GFT_OK;
@@ -162,14 +199,14 @@ size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_camera
   if (boundary != GFT_BORDER) return 0;
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
   return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
-         align256(plane * num_cameras * 2 * sizeof(float4));
+         align256(plane * num_cameras * sizeof(int2));
 }
 
-// saved = per-sample values [views][S][H][W] followed by the ray records
-static void saved_parts(void* saved, const int32_t* output_shape, int32_t num_cameras, float** gs, float4** rays) {
+// saved = per-sample values [views][S][H][W] followed by the per-pixel sample ranges [views][H][W]
+static void saved_parts(void* saved, const int32_t* output_shape, int32_t num_cameras, float** gs, int2** rays) {
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
   *gs = (float*)saved;
-  *rays = (float4*)((char*)saved + align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)));
+  *rays = (int2*)((char*)saved + align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)));
 }
 
 int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
@@ -189,7 +226,7 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
     return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
                 need_saved);
   float* gs = nullptr;
-  float4* rays = nullptr;
+  int2* rays = nullptr;
   if (saved && need_saved) saved_parts(saved, output_shape, num_cameras, &gs, &rays);
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
@@ -239,13 +276,13 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
   auto pairs = view_pairs(1, num_cameras, true);
   if (boundary == GFT_BORDER) {
     float* gs;
-    float4* rays;
+    int2* rays;
     saved_parts(saved, output_shape, num_cameras, &gs, &rays);
     for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
       size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
       RayLaunch L;
       ProjLaunch PL;
-      if ((rc = ray_views(pairs, first, cnt, M, V, P, L, &PL))) return rc;
+      if ((rc = ray_views(pairs, first, cnt, M, V, P, L, &PL, &rd))) return rc;
       PL.accumulate = first > 0;
       if ((rc = check_hip(render_density_walk_launch(l2, s, images, second, gs, rays, loss, rd, L, rp),
                           "render backward walk")))

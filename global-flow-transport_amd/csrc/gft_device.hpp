@@ -315,11 +315,18 @@ struct RayLaunch {
 // -> clip space (P * V * M) and the output slot of the view.
 struct ProjView {
   float pmv[16];
+  // pixel rays without a per-pixel record: with g(px, py) = g0 + px gx + py gy
+  // (homogeneous, object space), the ray of pixel (px, py) starts at
+  // near = gn.xyz / gn.w (g0 = near0) and ends at far = gf.xyz / gf.w (g0 =
+  // far0), both the index->object map of the raymarch at depth 0.5 and
+  // S - 1 + 0.5 (for S > 1)
+  float near0[4], far0[4], gx[4], gy[4];
   int out_slot;
 };
 struct ProjLaunch {
   ProjView view[kMaxViewsPerLaunch];
   int num_views;
+  float inv_zmax;  // 1 / max(S - 1, 1): step = (far - near) * inv_zmax
   int accumulate;  // 0: write the sum, 1: add it to the existing gradient
 };
 

@@ -144,7 +144,8 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
 }
 
 constexpr int kBatch = 4;
-constexpr int kRayRec = 2;  // float4 records per pixel ray: (near, lo), (step, hi)
+// Per-pixel record: the clipped sample range (lo, hi) of the ray; lo > hi: no
+// sample (or, after the walk, no gradient).
 // saved value of a sample outside the grid (quiet NaN; never a valid extinction
 // of a finite density)
 #define kOutsideMark __builtin_nanf("")
@@ -161,14 +162,14 @@ __device__ __forceinline__ int wave_max(int v) {
 // ----------------------------------------------------------------- forward
 // BORDER reads `dens` as the corner-quad copy (float4, QuadDims q); CLAMP
 // and WRAP read the x-fastest grid. With gs != nullptr (BORDER, training) the
-// forward also saves what the backward walk needs: the ray of every pixel at
-// rays[2*((slot*H + y)*W + x) + {0,1}] = (near, lo), (step, hi) and sigma of
-// every sample of the clipped range [lo, hi] at gs[((slot*S + i)*H + y)*W + x]
-// (0 for a sample outside the grid).
+// forward also saves what the backward needs: the sample range of every pixel
+// at rays[(slot*H + y)*W + x] = (lo, hi) and sigma of every sample of the
+// wave's clipped range at gs[((slot*S + i)*H + y)*W + x] (a NaN mark for a
+// sample outside the grid).
 template <int BM>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
                                                             RayDims rd, RayLaunch L, RenderParams rp, QuadDims q,
-                                                            float* __restrict__ gs, float4* __restrict__ rays) {
+                                                            float* __restrict__ gs, int2* __restrict__ rays) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
@@ -185,11 +186,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     const size_t plane = size_t(rd.out_w) * rd.out_h;
     const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
     float* gview = (gs && valid) ? gs + (size_t)v.out_slot * rd.out_d * plane + pix : nullptr;
-    if (gs && valid) {
-      float4* r = rays + kRayRec * ((size_t)v.out_slot * plane + pix);
-      r[0] = make_float4(nearp.x, nearp.y, nearp.z, (float)lo);
-      r[1] = make_float4(step.x, step.y, step.z, (float)hi);
-    }
+    if (gs && valid) rays[(size_t)v.out_slot * plane + pix] = make_int2(lo, hi);
     // march the wave's union of ranges with one sample index for all lanes, so
     // that the saved-sigma stores of one instruction fill one sample plane
     const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
@@ -273,7 +270,7 @@ constexpr int kWalkBatch = 8;
 template <bool L2>
 __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __restrict__ img,
                                                                  const float* __restrict__ second,
-                                                                 float* __restrict__ gs, float4* __restrict__ rays,
+                                                                 float* __restrict__ gs, int2* __restrict__ rays,
                                                                  float* __restrict__ loss, RayDims rd, RayLaunch L,
                                                                  RenderParams rp) {
   const RayView& v = L.view[blockIdx.z];
@@ -287,12 +284,12 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
   float* gview = gs + (size_t)v.out_slot * rd.out_d * plane + pix;
   if (valid) {
     pixel_grad<L2>(img, second, rd, v, pix, rp, G, gD, D, sq);
-    float4* r = rays + kRayRec * ((size_t)v.out_slot * plane + pix);
-    lo = (int)r[0].w;
-    hi = (int)r[1].w;
+    int2* r = rays + (size_t)v.out_slot * plane + pix;
+    const int2 range = *r;
+    lo = range.x;
+    hi = range.y;
     if (G == 0.f && gD == 0.f && lo <= hi) {
-      reinterpret_cast<float*>(r)[3] = 1.f;
-      reinterpret_cast<float*>(r + 1)[3] = 0.f;
+      *r = make_int2(1, 0);
       lo = 1;
       hi = 0;
     }
@@ -336,7 +333,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
 // more voxels, at the price of fewer lanes.
 template <int BX, int BY, int BZ, int LX>
 __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* __restrict__ gs,
-                                                                   const float4* __restrict__ rays,
+                                                                   const int2* __restrict__ rays,
                                                                    float* __restrict__ grad, RayDims rd,
                                                                    ProjLaunch PL) {
   // LX lanes along x, 256 / LX along y
@@ -390,13 +387,30 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
       ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
     }
     const float* gview = gs + (size_t)pv.out_slot * rd.out_d * plane;
-    const float4* rview = rays + kRayRec * (size_t)pv.out_slot * plane;
+    const int2* rview = rays + (size_t)pv.out_slot * plane;
     for (int py = ys; py <= ye; ++py) {
+      const float fpy = (float)py;
+      // homogeneous near / far points of the row's pixel 0
+      const float4 rn = make_float4(fmaf(fpy, pv.gy[0], pv.near0[0]), fmaf(fpy, pv.gy[1], pv.near0[1]),
+                                    fmaf(fpy, pv.gy[2], pv.near0[2]), fmaf(fpy, pv.gy[3], pv.near0[3]));
+      const float4 rf = make_float4(fmaf(fpy, pv.gy[0], pv.far0[0]), fmaf(fpy, pv.gy[1], pv.far0[1]),
+                                    fmaf(fpy, pv.gy[2], pv.far0[2]), fmaf(fpy, pv.gy[3], pv.far0[3]));
       for (int px = xs; px <= xe; ++px) {
         const size_t pix = size_t(py) * rd.out_w + px;
-        const float4 r0 = rview[kRayRec * pix], r1 = rview[kRayRec * pix + 1];
-        const int lo = (int)r0.w, hi = (int)r1.w;
+        const int2 range = rview[pix];
+        const int lo = range.x, hi = range.y;
         if (lo > hi) continue;
+        // the pixel's ray from the view's projective map (the forward's ray up
+        // to rounding; positions only feed the hat weights)
+        const float fpx = (float)px;
+        const float nw = __builtin_amdgcn_rcpf(fmaf(fpx, pv.gx[3], rn.w)),
+                    fw = __builtin_amdgcn_rcpf(fmaf(fpx, pv.gx[3], rf.w));
+        const float nx = fmaf(fpx, pv.gx[0], rn.x) * nw, ny = fmaf(fpx, pv.gx[1], rn.y) * nw,
+                    nz = fmaf(fpx, pv.gx[2], rn.z) * nw;
+        const float sx = (fmaf(fpx, pv.gx[0], rf.x) * fw - nx) * PL.inv_zmax,
+                    sy = (fmaf(fpx, pv.gx[1], rf.y) * fw - ny) * PL.inv_zmax,
+                    sz = (fmaf(fpx, pv.gx[2], rf.z) * fw - nz) * PL.inv_zmax;
+        const float4 r0 = make_float4(nx, ny, nz, 0.f), r1 = make_float4(sx, sy, sz, 0.f);
         const float irx = __builtin_amdgcn_rcpf(r1.x), iry = __builtin_amdgcn_rcpf(r1.y),
                     irz = __builtin_amdgcn_rcpf(r1.z);
         // parameter interval of the ray inside the open box; a zero step
@@ -690,7 +704,7 @@ hipError_t quad_copy_launch(hipStream_t s, const float* lin, float* out, Dims3 d
 
 // BORDER: dens is the corner-quad copy.
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp, float* gs, float4* rays) {
+                                     const RayLaunch& L, const RenderParams& rp, float* gs, int2* rays) {
   const dim3 grid = tile_grid(rd, L.num_views);
   const QuadDims q = make_quad_dims(rd.in);
   KernelTimer t(s, "render_fwd");
@@ -704,7 +718,7 @@ hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, f
 
 // BORDER walk over the saved sigma (rewrites gs in place) for one chunk of views
 hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* img, const float* second, float* gs,
-                                      float4* rays, float* loss, const RayDims& rd, const RayLaunch& L,
+                                      int2* rays, float* loss, const RayDims& rd, const RayLaunch& L,
                                       const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
   KernelTimer t(s, "render_bwd_walk");
@@ -715,7 +729,7 @@ hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* img, 
 
 constexpr int kGatherBX = 1, kGatherBY = 2, kGatherBZ = 4, kGatherLX = 64;
 
-hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
+hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const int2* rays, float* grad,
                                         const RayDims& rd, const ProjLaunch& PL) {
   constexpr int ly = 256 / kGatherLX;
   const dim3 grid((rd.in.x + kGatherLX * kGatherBX - 1) / (kGatherLX * kGatherBX),
