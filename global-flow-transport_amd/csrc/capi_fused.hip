@@ -13,11 +13,10 @@ hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel,
 hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
                              const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a);
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp, float* gs, int2* rays);
-hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* img, const float* second, float* gs,
-                                      int2* rays, float* loss, const RayDims& rd, const RayLaunch& L,
-                                      const RenderParams& rp);
-hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const int2* rays, float* grad,
+                                     const RayLaunch& L, const RenderParams& rp, float* gs, float4* rays);
+hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img, const float* second, float4* rays,
+                                       float* loss, const RayDims& rd, const RayLaunch& L, const RenderParams& rp);
+hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
                                         const RayDims& rd, const ProjLaunch& PL);
 hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
                                          const float* second, float* grad, float* loss, const RayDims& rd,
@@ -199,14 +198,14 @@ size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_camera
   if (boundary != GFT_BORDER) return 0;
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
   return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
-         align256(plane * num_cameras * sizeof(int2));
+         align256(plane * num_cameras * sizeof(float4));
 }
 
-// saved = per-sample values [views][S][H][W] followed by the per-pixel sample ranges [views][H][W]
-static void saved_parts(void* saved, const int32_t* output_shape, int32_t num_cameras, float** gs, int2** rays) {
+// saved = per-sample values [views][S][H][W] followed by the per-pixel records [views][H][W]
+static void saved_parts(void* saved, const int32_t* output_shape, int32_t num_cameras, float** gs, float4** rays) {
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
   *gs = (float*)saved;
-  *rays = (int2*)((char*)saved + align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)));
+  *rays = (float4*)((char*)saved + align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)));
 }
 
 int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
@@ -226,7 +225,7 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
     return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
                 need_saved);
   float* gs = nullptr;
-  int2* rays = nullptr;
+  float4* rays = nullptr;
   if (saved && need_saved) saved_parts(saved, output_shape, num_cameras, &gs, &rays);
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
@@ -276,7 +275,7 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
   auto pairs = view_pairs(1, num_cameras, true);
   if (boundary == GFT_BORDER) {
     float* gs;
-    int2* rays;
+    float4* rays;
     saved_parts(saved, output_shape, num_cameras, &gs, &rays);
     for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
       size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
@@ -284,8 +283,8 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
       ProjLaunch PL;
       if ((rc = ray_views(pairs, first, cnt, M, V, P, L, &PL, &rd))) return rc;
       PL.accumulate = first > 0;
-      if ((rc = check_hip(render_density_walk_launch(l2, s, images, second, gs, rays, loss, rd, L, rp),
-                          "render backward walk")))
+      if ((rc = check_hip(render_density_pixel_launch(l2, s, images, second, rays, loss, rd, L, rp),
+                          "render backward pixel pass")))
         return rc;
       if ((rc = check_hip(render_density_gather_launch(s, gs, rays, density_grad, rd, PL), "render backward gather")))
         return rc;

@@ -12,22 +12,24 @@
 //
 // Backward, from an image gradient or with the L2 loss
 // loss = scale * sum (rgb - target)^2 folded in (each lane then forms g_rgb
-// from its own pixel): the ray is walked from far to near recovering
-// D_{i-1} = max(D_i - sigma_i, 0) (no forward state is stored), and the
-// density gradient of each sample is
-//   g_d = s * (exp(-D_i) * G + g_D),  G = sum_c (c*i)_c g_rgb_c,
-//   g_D <- g_D - exp(-D_i) * sigma_i * G.
-// Spreading g_d to the 8 trilinear corners is done in one of three ways:
-//   BORDER : atomic-free transpose. The walk stores g_d per sample (and the
-//            ray of every pixel); a voxel-parallel gather then enumerates, for
-//            each view, exactly the samples inside the voxel's trilinear
-//            support (box -> projected pixel range -> ray/box slab) and sums
-//            g_d * w in a fixed order: deterministic, one write per voxel.
+// from its own pixel). The density gradient of each sample is
+//   g_d = s * (exp(-D_i) * G - G * sum_{j>=i} exp(-D_j) sigma_j + g_D),
+//   G = sum_c (c*i)_c g_rgb_c,
+// and spreading g_d to the 8 trilinear corners is done in one of three ways:
+//   BORDER : atomic-free transpose. The forward saves, per sample,
+//            c_i = exp(-D_i) + E_{i-1} (so that g_d = alpha c_i + beta with
+//            per-pixel alpha, beta) and per pixel the clipped sample range and
+//            E; a pixel pass forms alpha, beta from the loss gradient; a
+//            voxel-parallel gather then enumerates, for each view, exactly the
+//            samples inside the voxel's trilinear support (box -> projected
+//            pixel range -> ray/box slab) and sums g_d * w in a fixed order:
+//            deterministic, one write per voxel.
 //   CLAMP  : samples outside the grid land on boundary voxels, so the support
-//            is unbounded; a tiled walk accumulates the scatter in an LDS
+//            is unbounded; a tiled walk from far to near (recovering
+//            D_{i-1} = max(D_i - sigma_i, 0)) accumulates the scatter in an LDS
 //            brick per 16x16-pixel tile and K-sample segment and flushes it
 //            with coalesced global atomics.
-//   WRAP   : per-sample global atomics.
+//   WRAP   : the same walk with per-sample global atomics.
 // Samples of a BORDER ray outside the grid read 0 and are skipped (exact).
 #include "gft_device.hpp"
 
@@ -143,11 +145,16 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
   }
 }
 
-constexpr int kBatch = 4;
-// Per-pixel record: the clipped sample range (lo, hi) of the ray; lo > hi: no
-// sample (or, after the walk, no gradient).
-// saved value of a sample outside the grid (quiet NaN; never a valid extinction
-// of a finite density)
+constexpr int kBatch = 8;
+// Per-pixel record (float4): the clipped sample range lo, hi (int bits in x, y;
+// lo > hi: no sample, or after the backward's pixel pass, no gradient) and
+// z = E, the pixel's emission sum, written by the forward; the pixel pass
+// replaces (z, w) by the coefficients (alpha, beta) of its sample gradients.
+__device__ __forceinline__ float4 make_ray_rec(int lo, int hi, float a, float b) {
+  return make_float4(__int_as_float(lo), __int_as_float(hi), a, b);
+}
+// saved value of a sample outside the grid (quiet NaN; never the value of a
+// sample of a finite density)
 #define kOutsideMark __builtin_nanf("")
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -169,7 +176,7 @@ __device__ __forceinline__ int wave_max(int v) {
 template <int BM>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
                                                             RayDims rd, RayLaunch L, RenderParams rp, QuadDims q,
-                                                            float* __restrict__ gs, int2* __restrict__ rays) {
+                                                            float* __restrict__ gs, float4* __restrict__ rays) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
@@ -186,7 +193,6 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     const size_t plane = size_t(rd.out_w) * rd.out_h;
     const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
     float* gview = (gs && valid) ? gs + (size_t)v.out_slot * rd.out_d * plane + pix : nullptr;
-    if (gs && valid) rays[(size_t)v.out_slot * plane + pix] = make_int2(lo, hi);
     // march the wave's union of ranges with one sample index for all lanes, so
     // that the saved-sigma stores of one instruction fill one sample plane
     const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
@@ -200,12 +206,14 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
         const bool act = in[u] & (i + u >= lo);
         const float sigma = act ? rp.step * val[u] : 0.f;
         D = act ? fmaxf(D + sigma, 0.f) : D;
-        E = fmaf(__expf(-D), sigma, E);
-        // the saved value of a sample outside the grid is a NaN mark: the walk
-        // stores a zero gradient there, so the gather needs no inside test
-        if (gview && i + u <= whi) gview[(size_t)(i + u) * plane] = act ? sigma : kOutsideMark;
+        const float e = __expf(-D);
+        // saved for the backward: c_i = exp(-D_i) + E_{i-1}; a NaN mark for a
+        // sample outside the grid (its gradient is 0)
+        if (gview && i + u <= whi) gview[(size_t)(i + u) * plane] = act ? e + E : kOutsideMark;
+        E = fmaf(e, sigma, E);
       }
     }
+    if (gs && valid) rays[(size_t)v.out_slot * plane + pix] = make_ray_rec(lo, hi, E, 0.f);
   } else {
     if (!valid) return;
     float3 nearp, step;
@@ -259,62 +267,36 @@ __device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, f
   }
 }
 
-// -------------------------------------------------- BORDER: walk + gather
-// Walk over the forward's saved sigma: for every pixel of the chunk, from far
-// to near over its clipped range, D_{i-1} = max(D_i - sigma_i, 0) and
-// g_i = s * (exp(-D_i) G + g_D), g_D -= exp(-D_i) sigma_i G; g_i overwrites
-// sigma_i in place (0 for samples outside the grid, which the forward marked).
+// -------------------------------------------------- BORDER: pixel pass + gather
+// With the forward's D_i = max(D_{i-1} + sigma_i, 0) (no clamp active),
+// E = sum_i exp(-D_i) sigma_i and the pixel gradients G (light-weighted rgb)
+// and g_D (optical depth), the gradient of sample k is
+//   dL/dd_k = s * (G * (exp(-D_k) - sum_{i>=k} exp(-D_i) sigma_i) + g_D)
+//           = alpha * c_k + beta,  c_k = exp(-D_k) + E_{k-1},
+//   alpha = s * G,  beta = s * (g_D - G * E).
+// The forward saves c_k per sample and E per pixel, so the backward needs no
+// walk along the ray: this pass turns each pixel's loss gradient into (alpha,
+// beta) (and the L2 loss), the gather evaluates alpha * c_k + beta per sample.
 // A pixel with no gradient gets lo > hi so the gather skips it.
-constexpr int kWalkBatch = 8;
-
 template <bool L2>
-__global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __restrict__ img,
-                                                                 const float* __restrict__ second,
-                                                                 float* __restrict__ gs, int2* __restrict__ rays,
-                                                                 float* __restrict__ loss, RayDims rd, RayLaunch L,
-                                                                 RenderParams rp) {
+__global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* __restrict__ img,
+                                                                  const float* __restrict__ second,
+                                                                  float4* __restrict__ rays, float* __restrict__ loss,
+                                                                  RayDims rd, RayLaunch L, RenderParams rp) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
-  const bool valid = x < rd.out_w && y < rd.out_h;
-  float sq = 0.f, G = 0.f, gD = 0.f, D = 0.f;
-  int lo = 1, hi = 0;
-  const size_t plane = size_t(rd.out_w) * rd.out_h;
-  const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
-  float* gview = gs + (size_t)v.out_slot * rd.out_d * plane + pix;
-  if (valid) {
+  float sq = 0.f;
+  if (x < rd.out_w && y < rd.out_h) {
+    const size_t pix = size_t(y) * rd.out_w + x;
+    float G, gD, D;
     pixel_grad<L2>(img, second, rd, v, pix, rp, G, gD, D, sq);
-    int2* r = rays + (size_t)v.out_slot * plane + pix;
-    const int2 range = *r;
-    lo = range.x;
-    hi = range.y;
-    if (G == 0.f && gD == 0.f && lo <= hi) {
-      *r = make_int2(1, 0);
-      lo = 1;
-      hi = 0;
-    }
-  }
-  // one sample index for the whole wave (coalesced plane loads and stores);
-  // each lane acts only inside its own range
-  const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
-  for (int i = whi; i >= wlo; i -= kWalkBatch) {
-    float sg[kWalkBatch];
-#pragma unroll
-    for (int u = 0; u < kWalkBatch; ++u) sg[u] = (i - u >= lo && i - u <= hi) ? gview[(size_t)(i - u) * plane] : 0.f;
-#pragma unroll
-    for (int u = 0; u < kWalkBatch; ++u) {
-      if (i - u >= lo && i - u <= hi) {
-        const float sigma = sg[u];
-        float gval = 0.f;  // a sample outside the grid gets a zero gradient
-        if (sigma == sigma) {
-          const float e = __expf(-D);
-          D = fmaxf(D - sigma, 0.f);
-          const float esg = e * sigma * G;
-          gval = rp.step * (e * G + (gD - esg));
-          gD -= esg;
-        }
-        gview[(size_t)(i - u) * plane] = gval;
-      }
+    float4* r = rays + (size_t)v.out_slot * rd.out_w * rd.out_h + pix;
+    const float4 rec = *r;
+    const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
+    if (lo <= hi) {
+      const bool none = G == 0.f && gD == 0.f;
+      *r = none ? make_ray_rec(1, 0, 0.f, 0.f) : make_ray_rec(lo, hi, rp.step * G, rp.step * fmaf(-G, rec.z, gD));
     }
   }
   if (L2) block_loss(sq, loss, rp.loss_scale);
@@ -333,7 +315,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_walk(const float* __
 // more voxels, at the price of fewer lanes.
 template <int BX, int BY, int BZ, int LX>
 __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* __restrict__ gs,
-                                                                   const int2* __restrict__ rays,
+                                                                   const float4* __restrict__ rays,
                                                                    float* __restrict__ grad, RayDims rd,
                                                                    ProjLaunch PL) {
   // LX lanes along x, 256 / LX along y
@@ -387,7 +369,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
       ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
     }
     const float* gview = gs + (size_t)pv.out_slot * rd.out_d * plane;
-    const int2* rview = rays + (size_t)pv.out_slot * plane;
+    const float4* rview = rays + (size_t)pv.out_slot * plane;
     for (int py = ys; py <= ye; ++py) {
       const float fpy = (float)py;
       // homogeneous near / far points of the row's pixel 0
@@ -397,9 +379,10 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
                                     fmaf(fpy, pv.gy[2], pv.far0[2]), fmaf(fpy, pv.gy[3], pv.far0[3]));
       for (int px = xs; px <= xe; ++px) {
         const size_t pix = size_t(py) * rd.out_w + px;
-        const int2 range = rview[pix];
-        const int lo = range.x, hi = range.y;
+        const float4 rec = rview[pix];
+        const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
         if (lo > hi) continue;
+        const float alpha = rec.z, beta = rec.w;
         // the pixel's ray from the view's projective map (the forward's ray up
         // to rounding; positions only feed the hat weights)
         const float fpx = (float)px;
@@ -426,14 +409,16 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
         const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
         if (i0 > i1) continue;
         // sample position relative to the block's first voxel, stepped along
-        // the ray; samples outside the grid carry a zero gradient
+        // the ray
         const float fi0 = (float)i0;
         float ux = fmaf(fi0, r1.x, r0.x) - (fcx + 0.5f), uy = fmaf(fi0, r1.y, r0.y) - (fcy + 0.5f),
               uz = fmaf(fi0, r1.z, r0.z) - (fcz + 0.5f);
         const float* gp = gview + (size_t)i0 * plane + pix;
         for (int i = i0; i <= i1; ++i) {
-          const float g = *gp;
+          const float c = *gp;
           gp += plane;
+          // NaN mark: sample outside the grid, no gradient
+          const float g = c == c ? fmaf(alpha, c, beta) : 0.f;
           // hat weights max(0, 1 - |u - t|)
           float wy[BY], wz[BZ];
           float gx = g;
@@ -704,7 +689,7 @@ hipError_t quad_copy_launch(hipStream_t s, const float* lin, float* out, Dims3 d
 
 // BORDER: dens is the corner-quad copy.
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp, float* gs, int2* rays) {
+                                     const RayLaunch& L, const RenderParams& rp, float* gs, float4* rays) {
   const dim3 grid = tile_grid(rd, L.num_views);
   const QuadDims q = make_quad_dims(rd.in);
   KernelTimer t(s, "render_fwd");
@@ -716,20 +701,19 @@ hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, f
   return hipGetLastError();
 }
 
-// BORDER walk over the saved sigma (rewrites gs in place) for one chunk of views
-hipError_t render_density_walk_launch(bool l2, hipStream_t s, const float* img, const float* second, float* gs,
-                                      int2* rays, float* loss, const RayDims& rd, const RayLaunch& L,
-                                      const RenderParams& rp) {
+// BORDER pixel pass (loss and per-pixel gradient coefficients) for one chunk of views
+hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img, const float* second, float4* rays,
+                                       float* loss, const RayDims& rd, const RayLaunch& L, const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
-  KernelTimer t(s, "render_bwd_walk");
-  if (l2) hipLaunchKernelGGL(k_render_density_bwd_walk<true>, grid, dim3(256), 0, s, img, second, gs, rays, loss, rd, L, rp);
-  else hipLaunchKernelGGL(k_render_density_bwd_walk<false>, grid, dim3(256), 0, s, img, second, gs, rays, loss, rd, L, rp);
+  KernelTimer t(s, "render_bwd_pixel");
+  if (l2) hipLaunchKernelGGL(k_render_density_bwd_pixel<true>, grid, dim3(256), 0, s, img, second, rays, loss, rd, L, rp);
+  else hipLaunchKernelGGL(k_render_density_bwd_pixel<false>, grid, dim3(256), 0, s, img, second, rays, loss, rd, L, rp);
   return hipGetLastError();
 }
 
 constexpr int kGatherBX = 1, kGatherBY = 2, kGatherBZ = 4, kGatherLX = 64;
 
-hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const int2* rays, float* grad,
+hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
                                         const RayDims& rd, const ProjLaunch& PL) {
   constexpr int ly = 256 / kGatherLX;
   const dim3 grid((rd.in.x + kGatherLX * kGatherBX - 1) / (kGatherLX * kGatherBX),

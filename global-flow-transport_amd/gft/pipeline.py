@@ -270,14 +270,14 @@ class TransportRenderStep(object):
         """The per-phase compulsory bytes split over the kernels that touch
         those tensors: the first advection pass reads d and v, the second
         writes d'; the render forward reads d' and writes the images; the
-        BORDER walk reads images and targets and stands for the read of d'
-        (it consumes the forward's per-sample record instead) and the gather
-        writes g_d' (the CLAMP/WRAP scatter does both); advect_bwd_a reads g_d' and v,
-        advect_bwd_b reads d and writes g_d and g_v."""
+        BORDER pixel pass reads images and targets, the gather writes g_d' and
+        stands for the read of d' (it consumes the forward's per-sample record
+        instead); the CLAMP/WRAP scatter does both; advect_bwd_a reads g_d' and
+        v, advect_bwd_b reads d and writes g_d and g_v."""
         V = int(np.prod(self.shape))
         N = self.n_pixels
         return {"advect_sl": 16 * V, "advect_mc": 4 * V, "advect_o1_out": 4 * V, "render_fwd": 4 * V + 16 * N,
-                "render_bwd_walk": 4 * V + 28 * N, "render_bwd_gather": 4 * V, "render_bwd_scatter": 8 * V + 28 * N,
+                "render_bwd_pixel": 28 * N, "render_bwd_gather": 8 * V, "render_bwd_scatter": 8 * V + 28 * N,
                 "advect_bwd_a": 16 * V, "advect_bwd_b": 20 * V, "quad_copy": 0, "fill": 0}
 
     def voxel_samples(self):

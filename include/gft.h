@@ -279,7 +279,7 @@ int gft_reduce_grid_blend_bwd(void* stream, const float* input, const int64_t sh
  * the launch count and summed duration of one name; gft_timer_names lists
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
- * render_fwd, render_bwd_walk, render_bwd_gather, render_bwd_scatter, quad_copy, fill.
+ * render_fwd, render_bwd_pixel, render_bwd_gather, render_bwd_scatter, quad_copy, fill.
  */
 int gft_timer_start(const char* kernels);
 int gft_timer_read(const char* kernel, int32_t* launches, double* total_ms);
