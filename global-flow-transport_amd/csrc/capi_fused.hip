@@ -13,11 +13,14 @@ hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel,
 hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
                              const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a);
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp, float* gs, float4* rays);
-hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img, const float* second, float4* rays,
-                                       float* loss, const RayDims& rd, const RayLaunch& L, const RenderParams& rp);
-hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
-                                        const RayDims& rd, const ProjLaunch& PL);
+                                     const RayLaunch& L, const RenderParams& rp, const RenderSaved& sv);
+size_t render_tiles_per_view(const RayDims& rd);
+hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img, const float* second,
+                                       const RenderSaved& sv, const RayDims& rd, const RayLaunch& L,
+                                       const RenderParams& rp);
+hipError_t loss_sum_launch(hipStream_t s, const float* partials, int n, float scale, float* loss);
+hipError_t render_density_gather_launch(hipStream_t s, const RenderSaved& sv, float* grad, const RayDims& rd,
+                                        const ProjLaunch& PL);
 hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
                                          const float* second, float* grad, float* loss, const RayDims& rd,
                                          const RayLaunch& L, const RenderParams& rp);
@@ -194,18 +197,30 @@ size_t gft_render_density_fwd_workspace(const int64_t shape[5], int32_t boundary
 
 size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                       int32_t boundary) {
-  (void)shape;
   if (boundary != GFT_BORDER) return 0;
+  const RayDims rd = render_dims(shape, output_shape);
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
   return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
-         align256(plane * num_cameras * sizeof(float4));
+         align256(plane * num_cameras * sizeof(int2)) + align256(plane * num_cameras * sizeof(float2)) +
+         align256(render_tiles_per_view(rd) * num_cameras * sizeof(float));
 }
 
-// saved = per-sample values [views][S][H][W] followed by the per-pixel records [views][H][W]
-static void saved_parts(void* saved, const int32_t* output_shape, int32_t num_cameras, float** gs, float4** rays) {
+// the saved buffer: per-sample values, per-pixel ranges and coefficients,
+// per-tile loss partials (the layout of RenderSaved)
+static RenderSaved saved_parts(void* saved, const int64_t* shape, const int32_t* output_shape, int32_t num_cameras) {
+  const RayDims rd = render_dims(shape, output_shape);
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
-  *gs = (float*)saved;
-  *rays = (float4*)((char*)saved + align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)));
+  char* p = (char*)saved;
+  RenderSaved sv;
+  sv.samples = (float*)p;
+  p += align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float));
+  sv.ranges = (int2*)p;
+  p += align256(plane * num_cameras * sizeof(int2));
+  sv.coefs = (float2*)p;
+  p += align256(plane * num_cameras * sizeof(float2));
+  sv.partials = (float*)p;
+  (void)rd;
+  return sv;
 }
 
 int gft_render_density_fwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
@@ -224,9 +239,8 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
   if (saved && saved_bytes < need_saved)
     return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
                 need_saved);
-  float* gs = nullptr;
-  float4* rays = nullptr;
-  if (saved && need_saved) saved_parts(saved, output_shape, num_cameras, &gs, &rays);
+  RenderSaved sv{nullptr, nullptr, nullptr, nullptr};
+  if (saved && need_saved) sv = saved_parts(saved, shape, output_shape, num_cameras);
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
   auto pairs = view_pairs(1, num_cameras, true);
@@ -239,7 +253,7 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
     size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
     RayLaunch L;
     if ((rc = ray_views(pairs, first, cnt, M, V, P, L))) return rc;
-    if ((rc = check_hip(render_density_fwd_launch(boundary, (hipStream_t)stream, density, images, rd, L, rp, gs, rays),
+    if ((rc = check_hip(render_density_fwd_launch(boundary, (hipStream_t)stream, density, images, rd, L, rp, sv),
                         "render forward")))
       return rc;
   }
@@ -271,26 +285,29 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
   hipStream_t s = (hipStream_t)stream;
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, loss_scale};
-  if (l2 && (rc = check_hip(fill_launch(s, loss, 1, 0.f), "clear loss"))) return rc;
   auto pairs = view_pairs(1, num_cameras, true);
   if (boundary == GFT_BORDER) {
-    float* gs;
-    float4* rays;
-    saved_parts(saved, output_shape, num_cameras, &gs, &rays);
+    const RenderSaved sv = saved_parts(saved, shape, output_shape, num_cameras);
     for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
       size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
       RayLaunch L;
       ProjLaunch PL;
       if ((rc = ray_views(pairs, first, cnt, M, V, P, L, &PL, &rd))) return rc;
       PL.accumulate = first > 0;
-      if ((rc = check_hip(render_density_pixel_launch(l2, s, images, second, rays, loss, rd, L, rp),
+      if ((rc = check_hip(render_density_pixel_launch(l2, s, images, second, sv, rd, L, rp),
                           "render backward pixel pass")))
         return rc;
-      if ((rc = check_hip(render_density_gather_launch(s, gs, rays, density_grad, rd, PL), "render backward gather")))
+      if ((rc = check_hip(render_density_gather_launch(s, sv, density_grad, rd, PL), "render backward gather")))
         return rc;
     }
+    // the loss from the per-tile partials of every view, in a fixed order
+    if (l2 && (rc = check_hip(loss_sum_launch(s, sv.partials, (int)(render_tiles_per_view(rd) * num_cameras),
+                                              loss_scale, loss),
+                              "loss sum")))
+      return rc;
     return GFT_OK;
   }
+  if (l2 && (rc = check_hip(fill_launch(s, loss, 1, 0.f), "clear loss"))) return rc;
   if ((rc = check_hip(fill_launch(s, density_grad, rd.in_batch_stride, 0.f), "clear density_grad"))) return rc;
   for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
     size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);

@@ -330,6 +330,17 @@ struct ProjLaunch {
   int accumulate;  // 0: write the sum, 1: add it to the existing gradient
 };
 
+// What the BORDER density render forward saves for its backward (device
+// pointers, indexed by output view slot): per-sample values [views][S][H][W],
+// per-pixel sample ranges (lo, hi) and coefficients [views][H][W], per-tile
+// partial sums of the L2 loss [views][tiles].
+struct RenderSaved {
+  float* samples;
+  int2* ranges;
+  float2* coefs;
+  float* partials;
+};
+
 // Fused advection launch parameters.
 struct AdvectDims {
   Dims3 d;

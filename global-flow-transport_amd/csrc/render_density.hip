@@ -146,13 +146,9 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
 }
 
 constexpr int kBatch = 8;
-// Per-pixel record (float4): the clipped sample range lo, hi (int bits in x, y;
-// lo > hi: no sample, or after the backward's pixel pass, no gradient) and
-// z = E, the pixel's emission sum, written by the forward; the pixel pass
-// replaces (z, w) by the coefficients (alpha, beta) of its sample gradients.
-__device__ __forceinline__ float4 make_ray_rec(int lo, int hi, float a, float b) {
-  return make_float4(__int_as_float(lo), __int_as_float(hi), a, b);
-}
+// Per-pixel records: the clipped sample range (lo, hi) (lo > hi: no sample, or
+// after the backward's pixel pass, no gradient) and coefficients, (E, 0) from
+// the forward (E the pixel's emission sum), (alpha, beta) after the pixel pass.
 // saved value of a sample outside the grid (quiet NaN; never the value of a
 // sample of a finite density)
 #define kOutsideMark __builtin_nanf("")
@@ -169,14 +165,14 @@ __device__ __forceinline__ int wave_max(int v) {
 // ----------------------------------------------------------------- forward
 // BORDER reads `dens` as the corner-quad copy (float4, QuadDims q); CLAMP
 // and WRAP read the x-fastest grid. With gs != nullptr (BORDER, training) the
-// forward also saves what the backward needs: the sample range of every pixel
-// at rays[(slot*H + y)*W + x] = (lo, hi) and sigma of every sample of the
-// wave's clipped range at gs[((slot*S + i)*H + y)*W + x] (a NaN mark for a
-// sample outside the grid).
+// forward also saves what the backward needs (RenderSaved): per pixel the
+// sample range and E, per sample of the wave's clipped range a value c_i at
+// samples[((slot*S + i)*H + y)*W + x] (a NaN mark for a sample outside the
+// grid).
 template <int BM>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
                                                             RayDims rd, RayLaunch L, RenderParams rp, QuadDims q,
-                                                            float* __restrict__ gs, float4* __restrict__ rays) {
+                                                            RenderSaved sv) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
@@ -192,7 +188,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     }
     const size_t plane = size_t(rd.out_w) * rd.out_h;
     const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
-    float* gview = (gs && valid) ? gs + (size_t)v.out_slot * rd.out_d * plane + pix : nullptr;
+    float* gview = (sv.samples && valid) ? sv.samples + (size_t)v.out_slot * rd.out_d * plane + pix : nullptr;
     // march the wave's union of ranges with one sample index for all lanes, so
     // that the saved-sigma stores of one instruction fill one sample plane
     const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
@@ -213,7 +209,10 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
         E = fmaf(e, sigma, E);
       }
     }
-    if (gs && valid) rays[(size_t)v.out_slot * plane + pix] = make_ray_rec(lo, hi, E, 0.f);
+    if (gview) {
+      sv.ranges[(size_t)v.out_slot * plane + pix] = make_int2(lo, hi);
+      sv.coefs[(size_t)v.out_slot * plane + pix] = make_float2(E, 0.f);
+    }
   } else {
     if (!valid) return;
     float3 nearp, step;
@@ -280,8 +279,7 @@ __device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, f
 // A pixel with no gradient gets lo > hi so the gather skips it.
 template <bool L2>
 __global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* __restrict__ img,
-                                                                  const float* __restrict__ second,
-                                                                  float4* __restrict__ rays, float* __restrict__ loss,
+                                                                  const float* __restrict__ second, RenderSaved sv,
                                                                   RayDims rd, RayLaunch L, RenderParams rp) {
   const RayView& v = L.view[blockIdx.z];
   const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
@@ -291,15 +289,44 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* _
     const size_t pix = size_t(y) * rd.out_w + x;
     float G, gD, D;
     pixel_grad<L2>(img, second, rd, v, pix, rp, G, gD, D, sq);
-    float4* r = rays + (size_t)v.out_slot * rd.out_w * rd.out_h + pix;
-    const float4 rec = *r;
-    const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
-    if (lo <= hi) {
-      const bool none = G == 0.f && gD == 0.f;
-      *r = none ? make_ray_rec(1, 0, 0.f, 0.f) : make_ray_rec(lo, hi, rp.step * G, rp.step * fmaf(-G, rec.z, gD));
+    const size_t at = (size_t)v.out_slot * rd.out_w * rd.out_h + pix;
+    const int2 range = sv.ranges[at];
+    if (range.x <= range.y) {
+      if (G == 0.f && gD == 0.f) {
+        sv.ranges[at] = make_int2(1, 0);
+      } else {
+        const float E = sv.coefs[at].x;
+        sv.coefs[at] = make_float2(rp.step * G, rp.step * fmaf(-G, E, gD));
+      }
     }
   }
-  if (L2) block_loss(sq, loss, rp.loss_scale);
+  if (L2) {
+    // per-tile partial sum (fixed order); a single-block pass adds the tiles
+    __shared__ float red[256 / 64];
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float t = 0.f;
+      for (int w = 0; w < 256 / 64; ++w) t += red[w];
+      sv.partials[(size_t)v.out_slot * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
+  }
+}
+
+// loss = scale * sum of the n tile partials, in a fixed order (deterministic)
+__global__ __launch_bounds__(256) void k_loss_sum(const float* __restrict__ partials, int n, float scale,
+                                                  float* __restrict__ loss) {
+  __shared__ float red[256];
+  float t = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) t += partials[i];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[0] = red[0] * scale;
 }
 
 // Gather: one lane per block of BX x BY x BZ voxels c = (x0.., y0.., z0..),
@@ -314,10 +341,8 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* _
 // Larger blocks visit fewer candidate rays per voxel and load each sample for
 // more voxels, at the price of fewer lanes.
 template <int BX, int BY, int BZ, int LX>
-__global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* __restrict__ gs,
-                                                                   const float4* __restrict__ rays,
-                                                                   float* __restrict__ grad, RayDims rd,
-                                                                   ProjLaunch PL) {
+__global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved sv, float* __restrict__ grad,
+                                                                   RayDims rd, ProjLaunch PL) {
   // LX lanes along x, 256 / LX along y
   const int cx = (blockIdx.x * LX + (threadIdx.x % LX)) * BX;
   const int cy = (blockIdx.y * (256 / LX) + (threadIdx.x / LX)) * BY;
@@ -368,8 +393,9 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
       ys = max(ys, (int)ceilf(mny - 0.5f - 1e-2f));
       ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
     }
-    const float* gview = gs + (size_t)pv.out_slot * rd.out_d * plane;
-    const float4* rview = rays + (size_t)pv.out_slot * plane;
+    const float* gview = sv.samples + (size_t)pv.out_slot * rd.out_d * plane;
+    const int2* rview = sv.ranges + (size_t)pv.out_slot * plane;
+    const float2* cview = sv.coefs + (size_t)pv.out_slot * plane;
     for (int py = ys; py <= ye; ++py) {
       const float fpy = (float)py;
       // homogeneous near / far points of the row's pixel 0
@@ -379,10 +405,9 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
                                     fmaf(fpy, pv.gy[2], pv.far0[2]), fmaf(fpy, pv.gy[3], pv.far0[3]));
       for (int px = xs; px <= xe; ++px) {
         const size_t pix = size_t(py) * rd.out_w + px;
-        const float4 rec = rview[pix];
-        const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
+        const int2 range = rview[pix];
+        const int lo = range.x, hi = range.y;
         if (lo > hi) continue;
-        const float alpha = rec.z, beta = rec.w;
         // the pixel's ray from the view's projective map (the forward's ray up
         // to rounding; positions only feed the hat weights)
         const float fpx = (float)px;
@@ -408,6 +433,8 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(const float* 
         const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));
         const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
         if (i0 > i1) continue;
+        const float2 coef = cview[pix];
+        const float alpha = coef.x, beta = coef.y;
         // sample position relative to the block's first voxel, stepped along
         // the ray
         const float fi0 = (float)i0;
@@ -689,39 +716,54 @@ hipError_t quad_copy_launch(hipStream_t s, const float* lin, float* out, Dims3 d
 
 // BORDER: dens is the corner-quad copy.
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
-                                     const RayLaunch& L, const RenderParams& rp, float* gs, float4* rays) {
+                                     const RayLaunch& L, const RenderParams& rp, const RenderSaved& sv) {
   const dim3 grid = tile_grid(rd, L.num_views);
   const QuadDims q = make_quad_dims(rd.in);
+  const RenderSaved none{nullptr, nullptr, nullptr, nullptr};
   KernelTimer t(s, "render_fwd");
   switch (bm) {
-    case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, gs, rays); break;
-    case B_CLAMP: hipLaunchKernelGGL(k_render_density_fwd<B_CLAMP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, nullptr, nullptr); break;
-    default: hipLaunchKernelGGL(k_render_density_fwd<B_WRAP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, nullptr, nullptr); break;
+    case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, sv); break;
+    case B_CLAMP: hipLaunchKernelGGL(k_render_density_fwd<B_CLAMP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, none); break;
+    default: hipLaunchKernelGGL(k_render_density_fwd<B_WRAP>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, none); break;
   }
   return hipGetLastError();
 }
 
-// BORDER pixel pass (loss and per-pixel gradient coefficients) for one chunk of views
-hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img, const float* second, float4* rays,
-                                       float* loss, const RayDims& rd, const RayLaunch& L, const RenderParams& rp) {
+size_t render_tiles_per_view(const RayDims& rd) {
+  const dim3 g = tile_grid(rd, 1);
+  return (size_t)g.x * g.y;
+}
+
+// BORDER pixel pass (per-tile loss partials and per-pixel gradient coefficients)
+// for one chunk of views
+hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img, const float* second,
+                                       const RenderSaved& sv, const RayDims& rd, const RayLaunch& L,
+                                       const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
   KernelTimer t(s, "render_bwd_pixel");
-  if (l2) hipLaunchKernelGGL(k_render_density_bwd_pixel<true>, grid, dim3(256), 0, s, img, second, rays, loss, rd, L, rp);
-  else hipLaunchKernelGGL(k_render_density_bwd_pixel<false>, grid, dim3(256), 0, s, img, second, rays, loss, rd, L, rp);
+  if (l2) hipLaunchKernelGGL(k_render_density_bwd_pixel<true>, grid, dim3(256), 0, s, img, second, sv, rd, L, rp);
+  else hipLaunchKernelGGL(k_render_density_bwd_pixel<false>, grid, dim3(256), 0, s, img, second, sv, rd, L, rp);
+  return hipGetLastError();
+}
+
+// loss = scale * (sum of the n partials)
+hipError_t loss_sum_launch(hipStream_t s, const float* partials, int n, float scale, float* loss) {
+  KernelTimer t(s, "loss_sum");
+  hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, partials, n, scale, loss);
   return hipGetLastError();
 }
 
 constexpr int kGatherBX = 1, kGatherBY = 2, kGatherBZ = 4, kGatherLX = 64;
 
-hipError_t render_density_gather_launch(hipStream_t s, const float* gs, const float4* rays, float* grad,
-                                        const RayDims& rd, const ProjLaunch& PL) {
+hipError_t render_density_gather_launch(hipStream_t s, const RenderSaved& sv, float* grad, const RayDims& rd,
+                                        const ProjLaunch& PL) {
   constexpr int ly = 256 / kGatherLX;
   const dim3 grid((rd.in.x + kGatherLX * kGatherBX - 1) / (kGatherLX * kGatherBX),
                   (rd.in.y + ly * kGatherBY - 1) / (ly * kGatherBY),
                   (rd.in.z + kGatherBZ - 1) / kGatherBZ);
   KernelTimer t(s, "render_bwd_gather");
   auto kern = k_render_density_bwd_gather<kGatherBX, kGatherBY, kGatherBZ, kGatherLX>;
-  hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, gs, rays, grad, rd, PL);
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, sv, grad, rd, PL);
   return hipGetLastError();
 }
 
