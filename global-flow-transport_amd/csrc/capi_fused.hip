@@ -201,11 +201,11 @@ size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_camera
   const RayDims rd = render_dims(shape, output_shape);
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
   return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
-         align256(plane * num_cameras * sizeof(int2)) + align256(plane * num_cameras * sizeof(float2)) +
+         align256(plane * num_cameras * sizeof(float4)) +
          align256(render_tiles_per_view(rd) * num_cameras * sizeof(float));
 }
 
-// the saved buffer: per-sample values, per-pixel ranges and coefficients,
+// the saved buffer: per-sample values, per-pixel records,
 // per-tile loss partials (the layout of RenderSaved)
 static RenderSaved saved_parts(void* saved, const int64_t* shape, const int32_t* output_shape, int32_t num_cameras) {
   const RayDims rd = render_dims(shape, output_shape);
@@ -214,10 +214,8 @@ static RenderSaved saved_parts(void* saved, const int64_t* shape, const int32_t*
   RenderSaved sv;
   sv.samples = (float*)p;
   p += align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float));
-  sv.ranges = (int2*)p;
-  p += align256(plane * num_cameras * sizeof(int2));
-  sv.coefs = (float2*)p;
-  p += align256(plane * num_cameras * sizeof(float2));
+  sv.records = (float4*)p;
+  p += align256(plane * num_cameras * sizeof(float4));
   sv.partials = (float*)p;
   (void)rd;
   return sv;
@@ -239,7 +237,7 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
   if (saved && saved_bytes < need_saved)
     return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
                 need_saved);
-  RenderSaved sv{nullptr, nullptr, nullptr, nullptr};
+  RenderSaved sv{nullptr, nullptr, nullptr};
   if (saved && need_saved) sv = saved_parts(saved, shape, output_shape, num_cameras);
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};

@@ -332,12 +332,12 @@ struct ProjLaunch {
 
 // What the BORDER density render forward saves for its backward (device
 // pointers, indexed by output view slot): per-sample values [views][S][H][W],
-// per-pixel sample ranges (lo, hi) and coefficients [views][H][W], per-tile
-// partial sums of the L2 loss [views][tiles].
+// per-pixel records [views][H][W] (sample range lo, hi as int bits in x, y;
+// coefficients in z, w) and per-tile partial sums of the L2 loss
+// [views][tiles].
 struct RenderSaved {
   float* samples;
-  int2* ranges;
-  float2* coefs;
+  float4* records;
   float* partials;
 };
 

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin
   if (j >= n) return;
   const int x = (int)(j % q.qx) - 1, y = (int)((j / q.qx) % q.qy) - 1, z = (int)(j / ((size_t)q.qx * q.qy)) - 1;
   auto v = [&](int xx, int yy) -> float {
-    return ((unsigned)xx < (unsigned)d.x) & ((unsigned)yy < (unsigned)d.y) & ((unsigned)z < (unsigned)d.z)
+    return (((unsigned)xx < (unsigned)d.x) & ((unsigned)yy < (unsigned)d.y) & ((unsigned)z < (unsigned)d.z))
                ? lin[flat3(xx, yy, z, d)]
                : 0.f;
   };
@@ -146,9 +146,14 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
 }
 
 constexpr int kBatch = 8;
-// Per-pixel records: the clipped sample range (lo, hi) (lo > hi: no sample, or
-// after the backward's pixel pass, no gradient) and coefficients, (E, 0) from
-// the forward (E the pixel's emission sum), (alpha, beta) after the pixel pass.
+// Per-pixel record (float4): the clipped sample range lo, hi (int bits in x, y;
+// lo > hi: no sample, or after the backward's pixel pass, no gradient) and
+// coefficients in (z, w): (E, 0) from the forward (E the pixel's emission
+// sum), (alpha, beta) after the pixel pass. One 16-byte load per candidate
+// pixel in the gather.
+__device__ __forceinline__ float4 make_record(int lo, int hi, float a, float b) {
+  return make_float4(__int_as_float(lo), __int_as_float(hi), a, b);
+}
 // saved value of a sample outside the grid (quiet NaN; never the value of a
 // sample of a finite density)
 #define kOutsideMark __builtin_nanf("")
@@ -209,10 +214,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
         E = fmaf(e, sigma, E);
       }
     }
-    if (gview) {
-      sv.ranges[(size_t)v.out_slot * plane + pix] = make_int2(lo, hi);
-      sv.coefs[(size_t)v.out_slot * plane + pix] = make_float2(E, 0.f);
-    }
+    if (gview) sv.records[(size_t)v.out_slot * plane + pix] = make_record(lo, hi, E, 0.f);
   } else {
     if (!valid) return;
     float3 nearp, step;
@@ -290,15 +292,11 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* _
     float G, gD, D;
     pixel_grad<L2>(img, second, rd, v, pix, rp, G, gD, D, sq);
     const size_t at = (size_t)v.out_slot * rd.out_w * rd.out_h + pix;
-    const int2 range = sv.ranges[at];
-    if (range.x <= range.y) {
-      if (G == 0.f && gD == 0.f) {
-        sv.ranges[at] = make_int2(1, 0);
-      } else {
-        const float E = sv.coefs[at].x;
-        sv.coefs[at] = make_float2(rp.step * G, rp.step * fmaf(-G, E, gD));
-      }
-    }
+    const float4 rec = sv.records[at];
+    const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
+    if (lo <= hi)
+      sv.records[at] = (G == 0.f && gD == 0.f) ? make_record(1, 0, 0.f, 0.f)
+                                               : make_record(lo, hi, rp.step * G, rp.step * fmaf(-G, rec.z, gD));
   }
   if (L2) {
     // per-tile partial sum (fixed order); a single-block pass adds the tiles
@@ -394,8 +392,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
       ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
     }
     const float* gview = sv.samples + (size_t)pv.out_slot * rd.out_d * plane;
-    const int2* rview = sv.ranges + (size_t)pv.out_slot * plane;
-    const float2* cview = sv.coefs + (size_t)pv.out_slot * plane;
+    const float4* rview = sv.records + (size_t)pv.out_slot * plane;
     for (int py = ys; py <= ye; ++py) {
       const float fpy = (float)py;
       // homogeneous near / far points of the row's pixel 0
@@ -405,8 +402,8 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
                                     fmaf(fpy, pv.gy[2], pv.far0[2]), fmaf(fpy, pv.gy[3], pv.far0[3]));
       for (int px = xs; px <= xe; ++px) {
         const size_t pix = size_t(py) * rd.out_w + px;
-        const int2 range = rview[pix];
-        const int lo = range.x, hi = range.y;
+        const float4 rec = rview[pix];
+        const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
         if (lo > hi) continue;
         // the pixel's ray from the view's projective map (the forward's ray up
         // to rounding; positions only feed the hat weights)
@@ -433,8 +430,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
         const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));
         const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
         if (i0 > i1) continue;
-        const float2 coef = cview[pix];
-        const float alpha = coef.x, beta = coef.y;
+        const float alpha = rec.z, beta = rec.w;
         // sample position relative to the block's first voxel, stepped along
         // the ray
         const float fi0 = (float)i0;
@@ -719,7 +715,7 @@ hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, f
                                      const RayLaunch& L, const RenderParams& rp, const RenderSaved& sv) {
   const dim3 grid = tile_grid(rd, L.num_views);
   const QuadDims q = make_quad_dims(rd.in);
-  const RenderSaved none{nullptr, nullptr, nullptr, nullptr};
+  const RenderSaved none{nullptr, nullptr, nullptr};
   KernelTimer t(s, "render_fwd");
   switch (bm) {
     case B_BORDER: hipLaunchKernelGGL(k_render_density_fwd<B_BORDER>, grid, dim3(256), 0, s, dens, img, rd, L, rp, q, sv); break;
