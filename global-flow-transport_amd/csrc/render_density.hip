@@ -327,45 +327,38 @@ __global__ __launch_bounds__(256) void k_loss_sum(const float* __restrict__ part
   if (threadIdx.x == 0) loss[0] = red[0] * scale;
 }
 
-// Gather: one lane per block of BX x BY x BZ voxels c = (x0.., y0.., z0..),
-// all views of the launch summed in order. A sample q (cell-centre offset
-// removed) reaches c with weight prod_a max(0, 1 - |q_a - c_a|). The block's
-// support box (x0-1, x0+BX) x (y0-1, y0+BY) x (z0-1, z0+BZ) is projected to
-// find the candidate pixels, and each candidate ray is cut by the box to the
-// samples strictly inside it (a sample on the box surface has weight 0; the
-// 1e-3 margin in the ray parameter only adds samples, whose weight test
-// rejects them). A sample is loaded once for the whole block; only samples
-// with positive weight that the walk stored (inside the grid) are loaded.
-// Larger blocks visit fewer candidate rays per voxel and load each sample for
-// more voxels, at the price of fewer lanes.
-template <int BX, int BY, int BZ, int LX>
+// Gather: one wave per block of WX x WY x WZ voxels, accumulated in a
+// wave-private FP64 LDS region. For each view the wave enumerates the pixels
+// whose rays can cross the block's support box (x0-1, x0+WX) x ... (the
+// projected box; lanes take consecutive pixels in row order), cuts each ray by
+// the box and adds every inside sample's gradient alpha c_i + beta,
+// trilinear-weighted, to the corner voxels that lie in the block (a sample
+// q reaches voxel c with weight prod_a max(0, 1 - |q_a - c_a|)). The pixel loop
+// is amortised over a whole block. Only this wave touches its region, so the
+// sums run in a fixed order (deterministic) and one store per voxel follows.
+template <int WX, int WY, int WZ>
 __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved sv, float* __restrict__ grad,
                                                                    RayDims rd, ProjLaunch PL) {
-  // LX lanes along x, 256 / LX along y
-  const int cx = (blockIdx.x * LX + (threadIdx.x % LX)) * BX;
-  const int cy = (blockIdx.y * (256 / LX) + (threadIdx.x / LX)) * BY;
-  const int cz0 = blockIdx.z * BZ;
-  if (cx >= rd.in.x || cy >= rd.in.y) return;
+  constexpr int kVox = WX * WY * WZ;
+  __shared__ double region[4][kVox];
+  typedef __attribute__((address_space(3))) double lds_double;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  lds_double* acc = (lds_double*)region[wave];
   const Dims3 d = rd.in;
+  const int nbx = (d.x + WX - 1) / WX, nby = (d.y + WY - 1) / WY, nbz = (d.z + WZ - 1) / WZ;
+  const int blk = blockIdx.x * 4 + wave;
+  const bool live = blk < nbx * nby * nbz;  // a wave past the last block only joins the barriers
+  const int x0 = (blk % nbx) * WX, y0 = ((blk / nbx) % nby) * WY, z0 = (blk / (nbx * nby)) * WZ;
+  for (int k = lane; k < kVox; k += 64) acc[k] = 0.0;
   const size_t plane = size_t(rd.out_w) * rd.out_h;
-  const float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz0;
-  // box corner in ray coordinates (which carry the +0.5 cell-centre offset);
-  // the box spans (BX + 1) x (BY + 1) x (BZ + 1)
-  const float bx0 = fcx - 0.5f, by0 = fcy - 0.5f, bz0 = fcz - 0.5f;
-  const float hx = 0.5f * (float)(BX + 1), hy = 0.5f * (float)(BY + 1), hz = 0.5f * (float)(BZ + 1);
-  float acc[BZ][BY][BX];
-#pragma unroll
-  for (int k = 0; k < BZ; ++k)
-#pragma unroll
-    for (int j = 0; j < BY; ++j)
-#pragma unroll
-      for (int i = 0; i < BX; ++i) acc[k][j][i] = 0.f;
-  for (int vi = 0; vi < PL.num_views; ++vi) {
+  const float fx0 = (float)x0, fy0 = (float)y0, fz0 = (float)z0;
+  // support box in ray coordinates (which carry the +0.5 cell-centre offset)
+  const float bx0 = fx0 - 0.5f, by0 = fy0 - 0.5f, bz0 = fz0 - 0.5f;
+  const float hx = 0.5f * (float)(WX + 1), hy = 0.5f * (float)(WY + 1), hz = 0.5f * (float)(WZ + 1);
+  for (int vi = 0; vi < PL.num_views && live; ++vi) {
     const ProjView& pv = PL.view[vi];
     const float* A = pv.pmv;
-    // clip coordinates of the box corners from its centre and the three
-    // column vectors (the map is linear before the divide)
-    const float4 cc = mat_vec(A, fcx + 0.5f * (float)BX, fcy + 0.5f * (float)BY, fcz + 0.5f * (float)BZ, 1.f);
+    const float4 cc = mat_vec(A, fx0 + 0.5f * (float)WX, fy0 + 0.5f * (float)WY, fz0 + 0.5f * (float)WZ, 1.f);
     float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
     bool behind = false;
 #pragma unroll
@@ -384,109 +377,97 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
     }
     int xs = 0, xe = rd.out_w - 1, ys = 0, ye = rd.out_h - 1;
     if (!behind) {
-      // the ray of pixel x passes at x + 0.5; slack covers the approximate
-      // reciprocal and rounding
       xs = max(xs, (int)ceilf(mnx - 0.5f - 1e-2f));
       xe = min(xe, (int)floorf(mxx - 0.5f + 1e-2f));
       ys = max(ys, (int)ceilf(mny - 0.5f - 1e-2f));
       ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
     }
+    if (xs > xe || ys > ye) continue;
     const float* gview = sv.samples + (size_t)pv.out_slot * rd.out_d * plane;
     const float4* rview = sv.records + (size_t)pv.out_slot * plane;
-    for (int py = ys; py <= ye; ++py) {
-      const float fpy = (float)py;
-      // homogeneous near / far points of the row's pixel 0
-      const float4 rn = make_float4(fmaf(fpy, pv.gy[0], pv.near0[0]), fmaf(fpy, pv.gy[1], pv.near0[1]),
-                                    fmaf(fpy, pv.gy[2], pv.near0[2]), fmaf(fpy, pv.gy[3], pv.near0[3]));
-      const float4 rf = make_float4(fmaf(fpy, pv.gy[0], pv.far0[0]), fmaf(fpy, pv.gy[1], pv.far0[1]),
-                                    fmaf(fpy, pv.gy[2], pv.far0[2]), fmaf(fpy, pv.gy[3], pv.far0[3]));
-      for (int px = xs; px <= xe; ++px) {
+    {
+      const int nx = xe - xs + 1, cnt = nx * (ye - ys + 1);
+      for (int k0 = 0; k0 < cnt; k0 += 64) {
+        const int k = k0 + lane;
+        if (k >= cnt) break;
+        const int px = xs + k % nx, py = ys + k / nx;
         const size_t pix = size_t(py) * rd.out_w + px;
         const float4 rec = rview[pix];
         const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
         if (lo > hi) continue;
-        // the pixel's ray from the view's projective map (the forward's ray up
-        // to rounding; positions only feed the hat weights)
-        const float fpx = (float)px;
-        const float nw = __builtin_amdgcn_rcpf(fmaf(fpx, pv.gx[3], rn.w)),
-                    fw = __builtin_amdgcn_rcpf(fmaf(fpx, pv.gx[3], rf.w));
-        const float nx = fmaf(fpx, pv.gx[0], rn.x) * nw, ny = fmaf(fpx, pv.gx[1], rn.y) * nw,
-                    nz = fmaf(fpx, pv.gx[2], rn.z) * nw;
-        const float sx = (fmaf(fpx, pv.gx[0], rf.x) * fw - nx) * PL.inv_zmax,
-                    sy = (fmaf(fpx, pv.gx[1], rf.y) * fw - ny) * PL.inv_zmax,
-                    sz = (fmaf(fpx, pv.gx[2], rf.z) * fw - nz) * PL.inv_zmax;
-        const float4 r0 = make_float4(nx, ny, nz, 0.f), r1 = make_float4(sx, sy, sz, 0.f);
-        const float irx = __builtin_amdgcn_rcpf(r1.x), iry = __builtin_amdgcn_rcpf(r1.y),
-                    irz = __builtin_amdgcn_rcpf(r1.z);
-        // parameter interval of the ray inside the open box; a zero step
-        // component gives an infinite (approximate) reciprocal, an unbounded
-        // or empty interval, or NaN bounds that fminf/fmaxf ignore (the
-        // weights stay exact)
-        const float ax = (bx0 - r0.x) * irx, bx = fmaf((float)(BX + 1), irx, ax);
-        const float ay = (by0 - r0.y) * iry, by = fmaf((float)(BY + 1), iry, ay);
-        const float az = (bz0 - r0.z) * irz, bz = fmaf((float)(BZ + 1), irz, az);
+        const float fpx = (float)px, fpy = (float)py;
+        const float4 gn = make_float4(fmaf(fpy, pv.gy[0], fmaf(fpx, pv.gx[0], pv.near0[0])),
+                                      fmaf(fpy, pv.gy[1], fmaf(fpx, pv.gx[1], pv.near0[1])),
+                                      fmaf(fpy, pv.gy[2], fmaf(fpx, pv.gx[2], pv.near0[2])),
+                                      fmaf(fpy, pv.gy[3], fmaf(fpx, pv.gx[3], pv.near0[3])));
+        const float4 gf = make_float4(fmaf(fpy, pv.gy[0], fmaf(fpx, pv.gx[0], pv.far0[0])),
+                                      fmaf(fpy, pv.gy[1], fmaf(fpx, pv.gx[1], pv.far0[1])),
+                                      fmaf(fpy, pv.gy[2], fmaf(fpx, pv.gx[2], pv.far0[2])),
+                                      fmaf(fpy, pv.gy[3], fmaf(fpx, pv.gx[3], pv.far0[3])));
+        const float nw = __builtin_amdgcn_rcpf(gn.w), fw = __builtin_amdgcn_rcpf(gf.w);
+        const float nx_ = gn.x * nw, ny_ = gn.y * nw, nz_ = gn.z * nw;
+        const float sx = (gf.x * fw - nx_) * PL.inv_zmax, sy = (gf.y * fw - ny_) * PL.inv_zmax,
+                    sz = (gf.z * fw - nz_) * PL.inv_zmax;
+        const float irx = __builtin_amdgcn_rcpf(sx), iry = __builtin_amdgcn_rcpf(sy), irz = __builtin_amdgcn_rcpf(sz);
+        const float ax = (bx0 - nx_) * irx, bx = fmaf((float)(WX + 1), irx, ax);
+        const float ay = (by0 - ny_) * iry, by = fmaf((float)(WY + 1), iry, ay);
+        const float az = (bz0 - nz_) * irz, bz = fmaf((float)(WZ + 1), irz, az);
         const float tlo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
         const float thi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
         const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));
         const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
         if (i0 > i1) continue;
         const float alpha = rec.z, beta = rec.w;
-        // sample position relative to the block's first voxel, stepped along
-        // the ray
+        // sample position in cell coordinates relative to the block origin
         const float fi0 = (float)i0;
-        float ux = fmaf(fi0, r1.x, r0.x) - (fcx + 0.5f), uy = fmaf(fi0, r1.y, r0.y) - (fcy + 0.5f),
-              uz = fmaf(fi0, r1.z, r0.z) - (fcz + 0.5f);
+        float qx = fmaf(fi0, sx, nx_) - (fx0 + 0.5f), qy = fmaf(fi0, sy, ny_) - (fy0 + 0.5f),
+              qz = fmaf(fi0, sz, nz_) - (fz0 + 0.5f);
         const float* gp = gview + (size_t)i0 * plane + pix;
         for (int i = i0; i <= i1; ++i) {
           const float c = *gp;
           gp += plane;
-          // NaN mark: sample outside the grid, no gradient
-          const float g = c == c ? fmaf(alpha, c, beta) : 0.f;
-          // hat weights max(0, 1 - |u - t|)
-          float wy[BY], wz[BZ];
-          float gx = g;
+          if (c == c) {  // NaN mark: sample outside the grid
+            const float g = fmaf(alpha, c, beta);
+            const float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+            const float tx = qx - flx, ty = qy - fly, tz = qz - flz;
+            const int lx = (int)flx, ly = (int)fly, lz = (int)flz;
+            const float wx[2] = {g * (1.f - tx), g * tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
 #pragma unroll
-          for (int t = 0; t < BX; ++t) gx = t == 0 ? g * fmaxf(1.f - fabsf(ux), 0.f) : gx;
+            for (int kz = 0; kz < 2; ++kz) {
+              const int cz = lz + kz;
+              if ((unsigned)cz >= (unsigned)WZ) continue;
 #pragma unroll
-          for (int t = 0; t < BY; ++t) wy[t] = fmaxf(1.f - fabsf(uy - (float)t), 0.f);
+              for (int ky = 0; ky < 2; ++ky) {
+                const int cy = ly + ky;
+                if ((unsigned)cy >= (unsigned)WY) continue;
+                const float wyz = wy[ky] * wz[kz];
 #pragma unroll
-          for (int t = 0; t < BZ; ++t) wz[t] = fmaxf(1.f - fabsf(uz - (float)t), 0.f);
-          if (BX == 1) {
-#pragma unroll
-            for (int j = 0; j < BY; ++j) {
-              const float gxy = gx * wy[j];
-#pragma unroll
-              for (int k = 0; k < BZ; ++k) acc[k][j][0] = fmaf(gxy, wz[k], acc[k][j][0]);
-            }
-          } else {
-#pragma unroll
-            for (int k = 0; k < BZ; ++k)
-#pragma unroll
-              for (int j = 0; j < BY; ++j) {
-                const float gyz = g * wy[j] * wz[k];
-#pragma unroll
-                for (int t = 0; t < BX; ++t)
-                  acc[k][j][t] = fmaf(gyz, fmaxf(1.f - fabsf(ux - (float)t), 0.f), acc[k][j][t]);
+                for (int kx = 0; kx < 2; ++kx) {
+                  const int cx = lx + kx;
+                  if ((unsigned)cx >= (unsigned)WX) continue;
+                  __hip_atomic_fetch_add(acc + (cz * WY + cy) * WX + cx, (double)(wx[kx] * wyz), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
               }
+            }
           }
-          ux += r1.x;
-          uy += r1.y;
-          uz += r1.z;
+          qx += sx;
+          qy += sy;
+          qz += sz;
         }
       }
     }
   }
-#pragma unroll
-  for (int k = 0; k < BZ; ++k)
-#pragma unroll
-    for (int j = 0; j < BY; ++j)
-#pragma unroll
-      for (int t = 0; t < BX; ++t) {
-        if (cz0 + k < d.z && cy + j < d.y && cx + t < d.x) {
-          float* g = grad + flat3(cx + t, cy + j, cz0 + k, d);
-          *g = PL.accumulate ? *g + acc[k][j][t] : acc[k][j][t];
-        }
-      }
+  __syncthreads();
+  if (!live) return;
+  for (int k = lane; k < kVox; k += 64) {
+    const int x = x0 + k % WX, y = y0 + (k / WX) % WY, z = z0 + k / (WX * WY);
+    if (x < d.x && y < d.y && z < d.z) {
+      float* g = grad + flat3(x, y, z, d);
+      const float v = (float)acc[k];
+      *g = PL.accumulate ? *g + v : v;
+    }
+  }
 }
 
 // ----------------------------------------- CLAMP: tiled LDS pre-reduction
@@ -749,17 +730,15 @@ hipError_t loss_sum_launch(hipStream_t s, const float* partials, int n, float sc
   return hipGetLastError();
 }
 
-constexpr int kGatherBX = 1, kGatherBY = 2, kGatherBZ = 4, kGatherLX = 64;
+constexpr int kGatherWX = 16, kGatherWY = 16, kGatherWZ = 2;
 
 hipError_t render_density_gather_launch(hipStream_t s, const RenderSaved& sv, float* grad, const RayDims& rd,
                                         const ProjLaunch& PL) {
-  constexpr int ly = 256 / kGatherLX;
-  const dim3 grid((rd.in.x + kGatherLX * kGatherBX - 1) / (kGatherLX * kGatherBX),
-                  (rd.in.y + ly * kGatherBY - 1) / (ly * kGatherBY),
-                  (rd.in.z + kGatherBZ - 1) / kGatherBZ);
+  const int nb = ((rd.in.x + kGatherWX - 1) / kGatherWX) * ((rd.in.y + kGatherWY - 1) / kGatherWY) *
+                 ((rd.in.z + kGatherWZ - 1) / kGatherWZ);
   KernelTimer t(s, "render_bwd_gather");
-  auto kern = k_render_density_bwd_gather<kGatherBX, kGatherBY, kGatherBZ, kGatherLX>;
-  hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, sv, grad, rd, PL);
+  auto kern = k_render_density_bwd_gather<kGatherWX, kGatherWY, kGatherWZ>;
+  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL);
   return hipGetLastError();
 }
 
