@@ -354,6 +354,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
   const float fx0 = (float)x0, fy0 = (float)y0, fz0 = (float)z0;
   // support box in ray coordinates (which carry the +0.5 cell-centre offset)
   const float bx0 = fx0 - 0.5f, by0 = fy0 - 0.5f, bz0 = fz0 - 0.5f;
+  const float bx1 = bx0 + (float)(WX + 1), by1 = by0 + (float)(WY + 1), bz1 = bz0 + (float)(WZ + 1);
   const float hx = 0.5f * (float)(WX + 1), hy = 0.5f * (float)(WY + 1), hz = 0.5f * (float)(WZ + 1);
   for (int vi = 0; vi < PL.num_views && live; ++vi) {
     const ProjView& pv = PL.view[vi];
@@ -409,9 +410,12 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
         const float sx = (gf.x * fw - nx_) * PL.inv_zmax, sy = (gf.y * fw - ny_) * PL.inv_zmax,
                     sz = (gf.z * fw - nz_) * PL.inv_zmax;
         const float irx = __builtin_amdgcn_rcpf(sx), iry = __builtin_amdgcn_rcpf(sy), irz = __builtin_amdgcn_rcpf(sz);
-        const float ax = (bx0 - nx_) * irx, bx = fmaf((float)(WX + 1), irx, ax);
-        const float ay = (by0 - ny_) * iry, by = fmaf((float)(WY + 1), iry, ay);
-        const float az = (bz0 - nz_) * irz, bz = fmaf((float)(WZ + 1), irz, az);
+        // both slab ends as products: a zero step component (reciprocal inf)
+        // then gives (-inf, inf) for a ray inside that slab and an empty
+        // interval outside it
+        const float ax = (bx0 - nx_) * irx, bx = (bx1 - nx_) * irx;
+        const float ay = (by0 - ny_) * iry, by = (by1 - ny_) * iry;
+        const float az = (bz0 - nz_) * irz, bz = (bz1 - nz_) * irz;
         const float tlo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
         const float thi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
         const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));

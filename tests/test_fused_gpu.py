@@ -141,3 +141,76 @@ def test_border_backward_is_deterministic(cuda):
         step.forward(td, tv)
         step.backward(td, tv, tgt)
         assert torch.equal(step.grad_d1, first)
+
+
+def _render_case(d, sc, light, boundary="BORDER", seed=5):
+    """GPU render forward/backward through the autograd op vs the oracle."""
+    from gft.pipeline import render_density
+    td = torch.from_numpy(d).cuda().requires_grad_(True)
+    img = render_density(td, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], light, boundary)
+    ref = orc.render_density_fwd(d, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], light,
+                                 boundary=boundary)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = rng.uniform(-1, 1, size=ref.shape).astype(np.float32)
+    img.backward(torch.from_numpy(g).cuda())
+    gref = orc.render_density_bwd(d, img.detach().cpu().numpy(), g, sc["M"], sc["V"], sc["P"], sc["F"],
+                                  sc["out_dhw"], sc["step"], light, boundary=boundary)
+    return img.detach().cpu().numpy(), ref, td.grad.cpu().numpy(), gref
+
+
+@pytest.mark.parametrize("views,yaw", [(18, (-60.0, 60.0)), (7, (-180.0, 154.0))])
+def test_render_density_many_views_and_full_circle(cuda, views, yaw):
+    """18 views cross the 16-view launch chunk (the second chunk adds to the
+    first's gradient); a full circle of cameras sees the grid from behind and
+    from the sides (negative and near-zero ray step components)."""
+    sc = scene(grid=(13, 21, 17), image=(19, 12), views=views, seed=9, yaw_range=yaw)
+    img, ref, gd, gref = _render_case(sc["data"], sc, (1.0, 0.8, 1.2))
+    assert norm_err(img, ref) < 1e-5
+    assert norm_err(gd, gref) < 1e-4
+
+
+@pytest.mark.parametrize("yaw", [0.0, 90.0, 33.0])
+def test_render_density_orthographic_axis_aligned_rays(cuda, yaw):
+    """Orthographic cameras with rays along a grid axis (yaw 0 and 90): two
+    step components are exactly zero, so the gather's ray/box cut runs on
+    infinite reciprocals."""
+    from gft.camera import camera_params_batch, model_matrices
+    from helpers import ortho_camera
+    grid = (10, 12, 9)
+    cam = ortho_camera(grid, (12, 9), 11, yaw=yaw)
+    V, P, F = camera_params_batch([cam])
+    sc = dict(M=model_matrices([syn.grid_transform(grid)]), V=V, P=P, F=F, out_dhw=(11, 12, 9), step=cam.depth_step)
+    d = syn.gaussian_blobs(grid, 4, seed=3).astype(np.float32)
+    img, ref, gd, gref = _render_case(d, sc, (1.0, 1.0, 1.0), seed=8)
+    assert np.abs(ref).max() > 0.05
+    assert norm_err(img, ref) < 1e-5
+    assert norm_err(gd, gref) < 1e-4
+
+
+def test_render_density_saved_buffer_is_checked(cuda):
+    """The BORDER backward needs the forward's saved buffer: a missing or short
+    buffer is a workspace error, not a fault."""
+    import ctypes
+    from gft import _lib
+    from gft.ops import BOUNDARY, _np_ptr
+    from gft.pipeline import _Cameras, render_density_forward, render_saved_bytes
+    sc = scene(grid=(8, 10, 9), image=(10, 7), views=2, seed=4)
+    cams = _Cameras(sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"])
+    d = torch.from_numpy(sc["data"]).cuda()
+    n = render_saved_bytes(d.shape, cams)
+    assert n > 0 and render_saved_bytes(d.shape, cams, "CLAMP") == 0
+    saved = torch.empty((n,), dtype=torch.uint8, device="cuda")
+    img = render_density_forward(d, cams, sc["step"], saved=saved)
+    L = _lib.lib()
+    lt = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
+    gd = torch.empty_like(d)
+    g = torch.ones_like(img)
+    args = lambda sp, sn: (_lib.stream_of(d), _lib.ptr(d), _lib.i64_array(d.shape), _np_ptr(cams.M), cams.nM,
+                           _np_ptr(cams.V), _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt,
+                           float(sc["step"]), BOUNDARY["BORDER"], _lib.ptr(img), _lib.ptr(g), sp, sn, None, 0,
+                           _lib.ptr(gd))
+    assert L.gft_render_density_bwd(*args(None, 0)) == _lib.GFT_ERR_WORKSPACE
+    assert L.gft_render_density_bwd(*args(_lib.ptr(saved), n - 1)) == _lib.GFT_ERR_WORKSPACE
+    assert L.gft_render_density_bwd(*args(_lib.ptr(saved), n)) == 0
+    torch.cuda.synchronize()
+    assert torch.isfinite(gd).all()
