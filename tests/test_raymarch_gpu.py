@@ -119,3 +119,38 @@ def test_invalid_arguments(cuda):
                                     sc["out_dhw"])
     with pytest.raises(InvalidArgumentError):
         ops.raymarch_grid_transform(t, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], blending_mode="FOO")
+
+
+def _ortho_scene(yaw, channels=4):
+    from gft import synthetic as syn
+    from gft.camera import camera_params_batch, model_matrices
+    from helpers import ortho_camera
+    grid = (9, 11, 8)
+    cam = ortho_camera(grid, (11, 8), 10, yaw=yaw)
+    V, P, F = camera_params_batch([cam])
+    d = syn.gaussian_blobs(grid, 4, seed=6)
+    rng = np.random.Generator(np.random.PCG64(17))
+    d = (d * rng.uniform(0.3, 1.0, size=(1, 1, 1, 1, channels))).astype(np.float32)
+    return dict(data=d, M=model_matrices([syn.grid_transform(grid)]), V=V, P=P, F=F, out_dhw=(10, 11, 8),
+                step=cam.depth_step)
+
+
+@pytest.mark.parametrize("yaw", [0.0, 90.0, 180.0])
+@pytest.mark.parametrize("grad_mode", ["EXACT", "REFERENCE"])
+def test_orthographic_axis_aligned(cuda, yaw, grad_mode):
+    """Rays along a grid axis (two step components exactly zero) through the
+    op's forward and both gradient modes."""
+    sc = _ortho_scene(yaw)
+    data = sc["data"] * sc["step"]
+    ops = _ops()
+    t = torch.from_numpy(data).cuda()
+    out = ops.raymarch_grid_transform(t, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"])
+    ref_out = orc.raymarch_fwd(data, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"])
+    assert np.abs(ref_out).max() > 0.05
+    assert norm_err(out.cpu().numpy(), ref_out) < 1e-5
+    og = np.random.Generator(np.random.PCG64(2)).uniform(-1, 1, size=ref_out.shape).astype(np.float32)
+    g = ops.raymarch_grid_transform_grad(t, out, torch.from_numpy(og).cuda(), sc["M"], sc["V"], sc["P"], sc["F"],
+                                         sc["out_dhw"], grad_mode=grad_mode).cpu().numpy()
+    ref = orc.raymarch_bwd(data, out.cpu().numpy(), og, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], "BORDER",
+                           "BEER_LAMBERT", True, grad_mode)
+    assert norm_err(g, ref) < 1e-4
