@@ -18,6 +18,12 @@ FP32 oracle, one full step on the host cores).
 
 ``--copy-kernel`` instead runs a 1 GiB device copy: the calibration kernel of
 known traffic for the HBM counters (FETCH_SIZE / WRITE_SIZE) under rocprofv3.
+
+``--chain`` instead measures the global transport chain (workload c3): an
+initial density advected through 20 per-frame velocity fields, every frame
+rendered to 5 views plus one unseen view and compared with its targets, the
+summed loss back-propagated through the whole chain with checkpointing every
+4 frames; peak device memory is reported against keeping every frame.
 """
 import argparse
 import json
@@ -86,6 +92,72 @@ def copy_kernel():
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
 
 
+def chain_bench(args):
+    """One JSON line for the checkpointed transport chain at c3."""
+    import numpy as np
+    import torch
+
+    from gft import synthetic as syn
+    from gft.camera import camera_params_batch, model_matrices
+    from gft.chain import TransportChain
+    from gft.pipeline import _Cameras, advect_forward, render_density_forward
+
+    w = syn.WORKLOADS["c3"]
+    grid, image, frames, k = w["grid"], w["image"], 20, 4
+    cams, S = syn.arc_cameras(grid, image, w["views"], extra_yaws=(w["unseen_yaw"],))
+    M = model_matrices([syn.grid_transform(grid)])
+    V, P, F = camera_params_batch(cams)
+    step = float(cams[0].depth_step)
+    out_dhw = (S,) + tuple(image)
+    dev = torch.device("cuda:0")
+    d0 = torch.from_numpy(syn.gaussian_blobs(grid, w["blobs"], seed=0)).to(dev)
+    vels = [torch.from_numpy(syn.curl_noise_velocity(grid, seed=100 + f, max_speed=w["max_speed"])).to(dev)
+            for f in range(frames - 1)]
+    # targets: a second density (seed 2) advected through the same velocities and rendered
+    c = _Cameras(M, V, P, F, out_dhw)
+    dt = torch.from_numpy(syn.gaussian_blobs(grid, w["blobs"], seed=2)).to(dev)
+    targets = []
+    for f in range(frames):
+        targets.append(render_density_forward(dt, c, step)[..., :3].contiguous())
+        if f < frames - 1:
+            dt, _ = advect_forward(dt, vels[f])
+    del dt
+    torch.cuda.synchronize()
+
+    def run(every, steps):
+        chain = TransportChain(grid, M, V, P, F, out_dhw, step, checkpoint_every=every)
+        chain.run(d0, vels, targets)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            loss, gd0, gv = chain.run(d0, vels, targets)
+            del gd0, gv
+        b.record()
+        torch.cuda.synchronize()
+        peak = torch.cuda.max_memory_allocated() - base
+        return a.elapsed_time(b) / steps, peak, float(loss.item())
+
+    ms, peak_k, loss = run(k, args.steps)
+    ms_all, peak_all, _ = run(1, 1)
+    Vox = int(np.prod(grid))
+    px = len(cams) * image[0] * image[1]
+    # per frame: render fwd + bwd (2 sum W*H*S); per advection: MacCormack fwd 4V + bwd 2V
+    nvs = frames * 2 * px * S + (frames - 1) * 6 * Vox
+    print(json.dumps({
+        "metric": "voxel-samples/s fwd+bwd of the checkpointed transport chain",
+        "value": nvs / (ms * 1e-3), "unit": "voxel-samples/s", "n_gpus": 1, "steps": args.steps,
+        "ms_per_chain": ms, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c3: grid %dx%dx%d (x,y,z), %d frames, %d views (5 + unseen yaw 90) %dx%d px, "
+                               "%d samples/ray, MacCormack+MC_SMOOTH, EXACT adjoint"
+                               % (grid[2], grid[1], grid[0], frames, len(cams), image[1], image[0], S),
+                   "checkpoint_every": k, "voxel_samples_per_chain": nvs},
+        "peak_bytes_checkpointed": peak_k, "peak_bytes_every_frame": peak_all,
+        "ms_per_chain_every_frame": ms_all, "loss": loss}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,12 +168,16 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-kernel", action="store_true")
+    ap.add_argument("--chain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.copy_kernel:
         copy_kernel()
+        return
+    if args.chain:
+        chain_bench(args)
         return
 
     import numpy as np
