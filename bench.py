@@ -19,6 +19,10 @@ FP32 oracle, one full step on the host cores).
 ``--copy-kernel`` instead runs a 1 GiB device copy: the calibration kernel of
 known traffic for the HBM counters (FETCH_SIZE / WRITE_SIZE) under rocprofv3.
 
+``--ops`` instead times every per-op kernel of the reference's op API
+(resample, mips, LoD, their gradients, the gather gradient, raymarch, depth
+blend, LuT sampling) at the c2 sizes, each against its compulsory bytes.
+
 ``--chain`` instead measures the global transport chain (workload c3): an
 initial density advected through 20 per-frame velocity fields, every frame
 rendered to 5 views plus one unseen view and compared with its targets, the
@@ -90,6 +94,98 @@ def copy_kernel():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+
+
+def ops_bench(args):
+    """One JSON line with the per-op kernels at c2 sizes: HIP-event time per
+    call (median of args.steps calls after one warm-up) and the achieved rate
+    on the op's compulsory bytes (inputs read once, outputs written once)."""
+    import numpy as np
+    import torch
+
+    from gft.ops import blending_ops, raymarching_ops, sampling_ops
+
+    inp = build_inputs("c2")
+    dev = torch.device("cuda:0")
+    M, V, P, F, out_dhw = inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"]
+    d = torch.from_numpy(inp["d0"]).to(dev)
+    vel = torch.from_numpy(inp["vel"]).to(dev)
+    grid4 = torch.cat([d, d, d, d], dim=-1) * inp["step"]  # the renderer's [light rgb, density] grid
+    Vx = int(d.numel())
+    nv = int(np.asarray(V).reshape(-1, 16).shape[0])
+    cells = nv * int(np.prod(out_dhw))
+    px = nv * out_dhw[1] * out_dhw[2]
+    lut = torch.cat([-vel, torch.zeros_like(vel[..., :1])], dim=-1).contiguous()
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(args.steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts))
+
+    res = {}
+
+    def op(name, fn, alg_bytes, what):
+        ms = timeit(fn)
+        res[name] = {"ms": ms, "alg_bytes": int(alg_bytes), "GBps": alg_bytes / (ms * 1e-3) / 1e9, "work": what}
+
+    tf = dict(interpolation="LINEAR", boundary="BORDER", coordinate_mode="TRANSFORM_LINDEPTH")
+    fr = sampling_ops.sample_grid_transform(grid4, M, V, P, F, out_dhw, **tf)
+    op("A1 sample_grid_transform fwd (C=4)", lambda: sampling_ops.sample_grid_transform(grid4, M, V, P, F, out_dhw,
+                                                                                      **tf),
+       Vx * 16 + cells * 16, "%d frustum cells" % cells)
+    op("A2 build_mips (3 levels, C=4)", lambda: sampling_ops.build_mips(grid4, 3), Vx * 16 * (1 + 1.0 / 7),
+       "levels 1-3")
+    tfm = dict(tf, mipmapping="LINEAR", num_mipmaps=3)
+    op("A2 sample_grid_transform fwd, LINEAR mips (C=4)",
+       lambda: sampling_ops.sample_grid_transform(grid4, M, V, P, F, out_dhw, **tfm),
+       Vx * 16 * (1 + 1.0 / 7) + cells * 16, "%d frustum cells, per-sample LoD" % cells)
+    mv = np.stack([V[i] @ M[0] for i in range(len(V))]).astype(np.float32)  # as the op's callers form it
+    op("A2 lod_transform", lambda: sampling_ops.lod_transform(tuple(d.shape[1:4]), mv, P, F, out_dhw),
+       cells * 16, "%d frustum cells" % cells)
+    g_fr = torch.rand_like(fr)
+    for gm in ("REFERENCE", "EXACT"):
+        op("A3 sample_grid_transform_grad %s (C=4)" % gm,
+           lambda gm=gm: sampling_ops.sample_grid_transform_grad(grid4, g_fr, M, V, P, F, grad_mode=gm, **tf),
+           cells * 16 + Vx * 16, "%d frustum cells" % cells)
+    op("A4 gather gradient (C=4)", lambda: sampling_ops.sample_grid_transform_gather_grad(
+        g_fr, M, V, P, F, tuple(d.shape[1:4]), interpolation="LINEAR", boundary="BORDER"),
+       cells * 16 + Vx * 16, "%d frustum cells" % cells)
+    img = raymarching_ops.raymarch_grid_transform(grid4, M, V, P, F, out_dhw)
+    op("A5 raymarch_grid_transform fwd (C=4)", lambda: raymarching_ops.raymarch_grid_transform(grid4, M, V, P, F,
+                                                                                            out_dhw),
+       Vx * 16 + px * 16, "%d rays x %d samples" % (px, out_dhw[0]))
+    g_img = torch.rand_like(img)
+    for gm in ("REFERENCE", "EXACT"):
+        op("A6 raymarch_grid_transform_grad %s (C=4)" % gm,
+           lambda gm=gm: raymarching_ops.raymarch_grid_transform_grad(grid4, img, g_img, M, V, P, F, out_dhw,
+                                                                       grad_mode=gm),
+           Vx * 16 * 2 + px * 32, "%d rays x %d samples" % (px, out_dhw[0]))
+    fr5 = fr.reshape((nv,) + tuple(fr.shape[2:]))
+    bl = blending_ops.reduce_grid_blend(fr5, "BEER_LAMBERT")
+    op("A7 reduce_grid_blend fwd", lambda: blending_ops.reduce_grid_blend(fr5, "BEER_LAMBERT"), cells * 16 + px * 16,
+       "%d cells" % cells)
+    g_bl = torch.rand_like(bl)
+    op("A7 reduce_grid_blend_grad", lambda: blending_ops.reduce_grid_blend_grad(g_bl, bl, fr5, "BEER_LAMBERT"),
+       cells * 32 + px * 32, "%d cells" % cells)
+    lk = dict(interpolation="LINEAR", boundary="CLAMP", separate_camera_batch=False, relative_coords=True)
+    op("A8 sample_grid_lut fwd (advection LuT)", lambda: sampling_ops.sample_grid_lut(d, lut, **lk),
+       Vx * (4 + 16 + 4), "%d voxels" % Vx)
+    g_l = torch.rand((1, 1) + tuple(d.shape[1:]), device=dev)
+    for gm in ("REFERENCE", "EXACT"):
+        op("A9 sample_grid_lut_grad %s" % gm,
+           lambda gm=gm: sampling_ops.sample_grid_lut_grad(d, g_l, lut, grad_mode=gm, **lk),
+           Vx * (4 + 4 + 16 + 4 + 16), "%d voxels" % Vx)
+    print(json.dumps({"metric": "per-op kernel time at c2 sizes", "unit": "ms", "n_gpus": 1, "steps": args.steps,
+                      "config": {"workload": "c2 sizes: grid 128x224x128, 5 views 256x456 px, 182 samples"},
+                      "ops": res}))
 
 
 def chain_bench(args):
@@ -169,6 +265,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-kernel", action="store_true")
     ap.add_argument("--chain", action="store_true")
+    ap.add_argument("--ops", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -178,6 +275,9 @@ def main():
         return
     if args.chain:
         chain_bench(args)
+        return
+    if args.ops:
+        ops_bench(args)
         return
 
     import numpy as np

@@ -50,41 +50,6 @@ static int check_advect_mode(int32_t order, int32_t clamp, float dt) {
   return GFT_OK;
 }
 
-// Homogeneous object-space point of the raymarch's index->object map at pixel
-// (px, py) and depth index iz as g0 + px gx + py gy: ndc = (2 (i + 0.5) / n - 1)
-// per axis, h = Pinv ndc, object = MVinv (h.xyz / h.w) = (A h).xyz / h.w with A
-// the affine part of MVinv (its last row is not used, as in the device map).
-static void ray_coefficients(const Mat4d& pinv, const Mat4d& mvinv, const RayDims& rd, ProjView& pv) {
-  const double W = rd.out_w, H = rd.out_h, S = rd.out_d;
-  const double zmax = rd.out_d > 1 ? rd.out_d - 1 : 1;
-  auto hcol = [&](int c, double* out) {
-    for (int r = 0; r < 4; ++r) out[r] = pinv.m[4 * c + r];  // column-major
-  };
-  auto affine = [&](const double* h, float* out) {
-    for (int r = 0; r < 3; ++r)
-      out[r] = (float)(mvinv.m[r] * h[0] + mvinv.m[4 + r] * h[1] + mvinv.m[8 + r] * h[2] + mvinv.m[12 + r] * h[3]);
-    out[3] = (float)h[3];
-  };
-  double c0[4], c1[4], c2[4], c3[4];
-  hcol(0, c0);
-  hcol(1, c1);
-  hcol(2, c2);
-  hcol(3, c3);
-  double hx[4], hy[4], hn[4], hf[4];
-  const double zn = 2.0 * 0.5 / S - 1.0, zf = 2.0 * (zmax + 0.5) / S - 1.0;
-  for (int r = 0; r < 4; ++r) {
-    hx[r] = c0[r] * (2.0 / W);
-    hy[r] = c1[r] * (2.0 / H);
-    const double base = c0[r] * (1.0 / W - 1.0) + c1[r] * (1.0 / H - 1.0) + c3[r];
-    hn[r] = base + c2[r] * zn;
-    hf[r] = base + c2[r] * zf;
-  }
-  affine(hx, pv.gx);
-  affine(hy, pv.gy);
-  affine(hn, pv.near0);
-  affine(hf, pv.far0);
-}
-
 static int ray_views(const std::vector<ViewPair>& pairs, size_t first, size_t count, const float* M, const float* V,
                      const float* P, RayLaunch& L, ProjLaunch* PL = nullptr, const RayDims* rd = nullptr) {
   L.num_views = (int)count;

@@ -9,6 +9,11 @@ hipError_t raymarch_fwd_launch(int C, int fm, int bm, int bl, hipStream_t s, con
                                const RayDims& rd, const RayLaunch& L);
 hipError_t raymarch_bwd_launch(int C, bool exact, int bm, int bl, hipStream_t s, const float* in, const float* o,
                                const float* og, float* g, float* w, const RayDims& rd, const RayLaunch& L);
+hipError_t raymarch_bwd_walk_launch(int C, bool exact, int bl, hipStream_t s, const float* in, const float* o,
+                                    const float* og, float* gsamp, int2* ranges, const RayDims& rd,
+                                    const RayLaunch& L);
+hipError_t raymarch_bwd_gather_launch(int C, bool exact, hipStream_t s, const float* gsamp, const int2* ranges,
+                                      float* grad, const RayDims& rd, const ProjLaunch& PL);
 hipError_t normalize_add_launch(int C, hipStream_t s, float* total, float* buf, float* wsum, size_t nvox);
 hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v);
 
@@ -100,7 +105,25 @@ int gft_raymarch_grid_transform_fwd(void* stream, const float* input, const int6
   return GFT_OK;
 }
 
-size_t gft_raymarch_grid_transform_bwd_workspace(const int64_t input_shape[5], int32_t grad_mode) {
+// BORDER walk + gather workspace for one chunk of at most kMaxViewsPerLaunch
+// views of one input batch: per-sample gradients [view][S][H][W][C] and the
+// per-pixel sample ranges.
+static size_t walk_chunk_views(int32_t num_cameras, int32_t separate_camera_batch) {
+  return (size_t)std::min<int32_t>(separate_camera_batch ? num_cameras : 1, kMaxViewsPerLaunch);
+}
+static size_t walk_samples_bytes(const int64_t* input_shape, const int32_t* output_shape, size_t views) {
+  const size_t plane = (size_t)output_shape[1] * output_shape[2];
+  return (views * output_shape[0] * plane * input_shape[4] * sizeof(float) + 255) & ~size_t(255);
+}
+
+size_t gft_raymarch_grid_transform_bwd_workspace(const int64_t input_shape[5], int32_t num_cameras,
+                                                 const int32_t output_shape[3], int32_t boundary,
+                                                 int32_t separate_camera_batch, int32_t grad_mode) {
+  if (boundary == GFT_BORDER) {
+    const size_t views = walk_chunk_views(num_cameras, separate_camera_batch);
+    const size_t plane = (size_t)output_shape[1] * output_shape[2];
+    return walk_samples_bytes(input_shape, output_shape, views) + views * plane * sizeof(int2);
+  }
   if (grad_mode == GFT_GRAD_EXACT) return 0;
   size_t vox = (size_t)input_shape[1] * input_shape[2] * input_shape[3];
   size_t a = ((vox * input_shape[4] * sizeof(float)) + 255) & ~size_t(255);
@@ -123,7 +146,8 @@ int gft_raymarch_grid_transform_bwd(void* stream, const float* input, const int6
     return fail(GFT_ERR_INVALID_ARGUMENT, "invalid grad_mode");
   if (!input || !output || !output_grad || !input_grad || !M || !V || !P)
     return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
-  size_t need = gft_raymarch_grid_transform_bwd_workspace(input_shape, grad_mode);
+  size_t need = gft_raymarch_grid_transform_bwd_workspace(input_shape, num_cameras, output_shape, boundary,
+                                                          separate_camera_batch, grad_mode);
   if (workspace_bytes < need || (need && !workspace))
     return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
                 need);
@@ -133,6 +157,45 @@ int gft_raymarch_grid_transform_bwd(void* stream, const float* input, const int6
   if ((rc = check_hip(fill_launch(s, input_grad, total, 0.f), "zero input_grad"))) return rc;
   auto pairs = view_pairs((int)input_shape[0], num_cameras, separate_camera_batch != 0);
   const bool exact = grad_mode == GFT_GRAD_EXACT;
+  if (boundary == GFT_BORDER) {
+    // walk + gather, per input batch and chunk of views of that batch
+    const size_t views = walk_chunk_views(num_cameras, separate_camera_batch);
+    float* gsamp = (float*)workspace;
+    int2* ranges = (int2*)((char*)workspace + walk_samples_bytes(input_shape, output_shape, views));
+    size_t first = 0;
+    while (first < pairs.size()) {
+      size_t cnt = 1;
+      while (first + cnt < pairs.size() && cnt < (size_t)kMaxViewsPerLaunch &&
+             pairs[first + cnt].batch == pairs[first].batch)
+        ++cnt;
+      RayLaunch L;
+      if ((rc = build_views(pairs, first, cnt, M, V, P, L))) return rc;
+      ProjLaunch PL;
+      PL.num_views = (int)cnt;
+      PL.accumulate = 1;
+      PL.inv_zmax = (float)(1.0 / (rd.out_d > 1 ? rd.out_d - 1 : 1));
+      for (size_t i = 0; i < cnt; ++i) {
+        const ViewPair& vp = pairs[first + i];
+        Mat4d m = mat_from_colmajor(M + 16 * vp.batch), v = mat_from_colmajor(V + 16 * vp.cam);
+        Mat4d p = mat_from_colmajor(P + 16 * vp.cam), mv = mat_mul(v, m), mvinv, pinv;
+        if (!mat_inverse(mv, mvinv)) return fail(GFT_ERR_INVALID_ARGUMENT, "model-view matrix is singular");
+        if (!mat_inverse(p, pinv)) return fail(GFT_ERR_INVALID_ARGUMENT, "projection matrix is singular");
+        mat_to_float(mat_mul(p, mv), PL.view[i].pmv);
+        ray_coefficients(pinv, mvinv, rd, PL.view[i]);
+        PL.view[i].out_slot = (int)i;  // launch-local index into the walk's buffers
+      }
+      if ((rc = check_hip(raymarch_bwd_walk_launch(rd.C, exact, blending_mode, s, input, output, output_grad, gsamp,
+                                                   ranges, rd, L),
+                          "raymarch backward walk")))
+        return rc;
+      if ((rc = check_hip(raymarch_bwd_gather_launch(rd.C, exact, s, gsamp, ranges,
+                                                     input_grad + rd.in_batch_stride * pairs[first].batch, rd, PL),
+                          "raymarch backward gather")))
+        return rc;
+      first += cnt;
+    }
+    return GFT_OK;
+  }
   if (exact) {
     for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
       size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
@@ -145,8 +208,8 @@ int gft_raymarch_grid_transform_bwd(void* stream, const float* input, const int6
     }
     return GFT_OK;
   }
-  // REFERENCE: per camera scatter into a buffer with weights, then normalise
-  // by the weight sum and add (raymarch_grid.cu.cc:504-570).
+  // REFERENCE (CLAMP / WRAP): per camera scatter into a buffer with weights,
+  // then normalise by the weight sum and add (raymarch_grid.cu.cc:504-570).
   const size_t vox = (size_t)input_shape[1] * input_shape[2] * input_shape[3];
   float* buf = (float*)workspace;
   float* wsum = (float*)((char*)workspace + (((vox * rd.C * sizeof(float)) + 255) & ~size_t(255)));

@@ -126,6 +126,34 @@ template <int BM> __device__ __forceinline__ Taps make_taps(float px, float py, 
   return t;
 }
 
+// Sample range [lo, hi] of the ray n + i st, i in [0, S), that can hold
+// positions in the box [-m, dims + m] (ray coordinates, which carry the +0.5
+// cell-centre offset: m = 0 is the grid box, inside meaning 0 <= p < dims);
+// one sample of slack on each end, so per-sample tests stay exact.
+// lo > hi: no sample.
+__device__ __forceinline__ void clip_ray(float3 n, float3 st, Dims3 d, int S, float m, int& lo, int& hi) {
+  float tlo = 0.f, thi = (float)(S - 1);
+  const float nn[3] = {n.x, n.y, n.z}, ss[3] = {st.x, st.y, st.z};
+  const float dd[3] = {(float)d.x, (float)d.y, (float)d.z};
+  bool miss = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (ss[a] != 0.f) {
+      float t0 = (-m - nn[a]) / ss[a], t1 = (dd[a] + m - nn[a]) / ss[a];
+      tlo = fmaxf(tlo, fminf(t0, t1));
+      thi = fminf(thi, fmaxf(t0, t1));
+    } else if (!(-m <= nn[a] && nn[a] < dd[a] + m)) {
+      miss = true;
+    }
+  }
+  lo = max(0, (int)floorf(tlo) - 1);
+  hi = min(S - 1, (int)ceilf(thi) + 1);
+  if (miss || thi < tlo - 2.f) {
+    lo = 1;
+    hi = 0;
+  }
+}
+
 // Trilinear value, lerp order x, y, z as in read3DInterpolated.
 template <int C, int BM>
 __device__ __forceinline__ Vec<C> read_linear(const float* __restrict__ buf, Dims3 d, float px, float py, float pz) {

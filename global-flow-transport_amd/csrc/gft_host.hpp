@@ -37,4 +37,8 @@ struct ViewPair {
 };
 std::vector<ViewPair> view_pairs(int N, int num_cameras, bool separate_camera_batch);
 
+// Per-view projective coefficients of the raymarch's pixel rays (near0, far0,
+// gx, gy of ProjView) from the inverse projection and inverse model-view.
+void ray_coefficients(const Mat4d& pinv, const Mat4d& mvinv, const RayDims& rd, ProjView& pv);
+
 }  // namespace gft

@@ -148,6 +148,242 @@ __global__ __launch_bounds__(256) void k_normalize_add(float* __restrict__ total
   wsum[i] = 0.f;
 }
 
+// ------------------------------------------------ BORDER: walk + gather
+// The BORDER backward without atomics: a walk stores every sample's gradient,
+// a voxel-block gather sums them (the transpose of the trilinear scatter).
+//   walk   : one lane per pixel, all lanes walk the whole ray in lock-step
+//            (the blend recurrences see every sample, as the scatter walk does)
+//            and store, for the samples of the pixel's clipped range, the
+//            gradient vector that the scatter would spread, or a NaN mark for
+//            a sample the scatter skips (EXACT: outside the grid; REFERENCE:
+//            outside the band (-1, dims)). Layout [view][S][H][W][C] plus the
+//            range (lo, hi) per pixel.
+//   gather : one wave per block of voxels with a wave-private FP64 LDS
+//            accumulator; per view, the candidate pixels of the block's
+//            support box, each ray cut by the box, hat weights per sample.
+//            REFERENCE also sums the weights and divides per view, as
+//            kNormalize3DGradients does per camera.
+template <int C, int BL, bool EXACT>
+__global__ __launch_bounds__(256) void k_raymarch_bwd_walk(const float* __restrict__ input,
+                                                           const float* __restrict__ output,
+                                                           const float* __restrict__ output_grad,
+                                                           float* __restrict__ gsamp, int2* __restrict__ ranges,
+                                                           RayDims rd, RayLaunch L) {
+  const int vi = blockIdx.z;
+  const RayView& v = L.view[vi];
+  const int x = blockIdx.x * kTileX + (threadIdx.x % kTileX);
+  const int y = blockIdx.y * kTileY + (threadIdx.x / kTileX);
+  if (x >= rd.out_w || y >= rd.out_h) return;
+  float3 nearp, farp, step;
+  ray_endpoints(v, x, y, rd, nearp, farp, step);
+  int lo, hi;
+  clip_ray(nearp, step, rd.in, rd.out_d, EXACT ? 0.f : 0.5f, lo, hi);
+  const size_t plane = size_t(rd.out_w) * rd.out_h;
+  const size_t pix = size_t(y) * rd.out_w + x;
+  ranges[(size_t)vi * plane + pix] = make_int2(lo, hi);
+  const float* buf = input + rd.in_batch_stride * v.in_batch;
+  Vec<C> acc = vload<C>(output + rd.out_view_stride * v.out_slot, pix);
+  Vec<C> g = vload<C>(output_grad + rd.out_view_stride * v.out_slot, pix);
+  const int S = rd.out_d;
+  float* gview = gsamp + (size_t)vi * S * plane * C;
+  for (int i = S - 1; i >= 0; --i) {
+    float3 p;
+    if (EXACT) {
+      const float fi = (float)i;
+      p = make_float3(fmaf(fi, step.x, nearp.x), fmaf(fi, step.y, nearp.y), fmaf(fi, step.z, nearp.z));
+    } else {
+      const float fi = (float)(S - 1 - i);
+      p = make_float3(fmaf(-fi, step.x, farp.x), fmaf(-fi, step.y, farp.y), fmaf(-fi, step.z, farp.z));
+    }
+    const bool in_range = (lo <= i) & (i <= hi);
+    // outside the clipped range the BORDER sample reads 0 (the blend still runs)
+    const Vec<C> smp = in_range ? sample_ray<C, F_LINEAR, B_BORDER>(buf, rd.in, p) : vzero<C>();
+    const Vec<C> gs = blend_grad<C, BL, EXACT>(g, smp, acc);
+    if (in_range) {
+      const float px = p.x - 0.5f, py = p.y - 0.5f, pz = p.z - 0.5f;
+      bool counted;
+      if (EXACT) {
+        const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+        counted = (0.f <= qx) & (qx < (float)rd.in.x) & (0.f <= qy) & (qy < (float)rd.in.y) & (0.f <= qz) &
+                  (qz < (float)rd.in.z);
+      } else {
+        counted = (-1.f < px) & (px < (float)rd.in.x) & (-1.f < py) & (py < (float)rd.in.y) & (-1.f < pz) &
+                  (pz < (float)rd.in.z);
+      }
+      Vec<C> out = gs;
+      if (!counted) out.v[0] = __builtin_nanf("");
+      vstore<C>(gview, (size_t)i * plane + pix, out);
+    }
+  }
+}
+
+template <int C, int WX, int WY, int WZ, bool EXACT>
+__global__ __launch_bounds__(256) void k_raymarch_bwd_gather(const float* __restrict__ gsamp,
+                                                             const int2* __restrict__ ranges,
+                                                             float* __restrict__ grad, RayDims rd, ProjLaunch PL) {
+  constexpr int kVox = WX * WY * WZ;
+  constexpr int kCh = EXACT ? C : C + 1;  // REFERENCE keeps the weight sum as the last channel
+  constexpr int kOwn = kVox / 64;         // voxels per lane in the per-view normalisation
+  static_assert(kVox % 64 == 0, "block must be a multiple of the wave");
+  __shared__ double region[4][kVox * kCh];
+  typedef __attribute__((address_space(3))) double lds_double;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  lds_double* acc = (lds_double*)region[wave];
+  const Dims3 d = rd.in;
+  const int nbx = (d.x + WX - 1) / WX, nby = (d.y + WY - 1) / WY, nbz = (d.z + WZ - 1) / WZ;
+  const int blk = blockIdx.x * 4 + wave;
+  const bool live = blk < nbx * nby * nbz;
+  const int x0 = (blk % nbx) * WX, y0 = ((blk / nbx) % nby) * WY, z0 = (blk / (nbx * nby)) * WZ;
+  for (int k = lane; k < kVox * kCh; k += 64) acc[k] = 0.0;
+  float tot[kOwn][C];
+#pragma unroll
+  for (int j = 0; j < kOwn; ++j)
+#pragma unroll
+    for (int c = 0; c < C; ++c) tot[j][c] = 0.f;
+  const size_t plane = size_t(rd.out_w) * rd.out_h;
+  const float fx0 = (float)x0, fy0 = (float)y0, fz0 = (float)z0;
+  const float bx0 = fx0 - 0.5f, by0 = fy0 - 0.5f, bz0 = fz0 - 0.5f;
+  const float bx1 = bx0 + (float)(WX + 1), by1 = by0 + (float)(WY + 1), bz1 = bz0 + (float)(WZ + 1);
+  const float hx = 0.5f * (float)(WX + 1), hy = 0.5f * (float)(WY + 1), hz = 0.5f * (float)(WZ + 1);
+  for (int vi = 0; vi < PL.num_views && live; ++vi) {
+    const ProjView& pv = PL.view[vi];
+    const float* A = pv.pmv;
+    const float4 cc = mat_vec(A, fx0 + 0.5f * (float)WX, fy0 + 0.5f * (float)WY, fz0 + 0.5f * (float)WZ, 1.f);
+    float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
+    bool behind = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float sx = (k & 1) ? hx : -hx, sy = (k & 2) ? hy : -hy, sz = (k & 4) ? hz : -hz;
+      const float cxk = fmaf(sz, A[8], fmaf(sy, A[4], fmaf(sx, A[0], cc.x)));
+      const float cyk = fmaf(sz, A[9], fmaf(sy, A[5], fmaf(sx, A[1], cc.y)));
+      const float cwk = fmaf(sz, A[11], fmaf(sy, A[7], fmaf(sx, A[3], cc.w)));
+      behind |= !(cwk > 0.f);
+      const float iw = __builtin_amdgcn_rcpf(cwk);
+      const float px = fmaf(cxk * iw, 0.5f, 0.5f) * (float)rd.out_w, py = fmaf(cyk * iw, 0.5f, 0.5f) * (float)rd.out_h;
+      mnx = fminf(mnx, px);
+      mxx = fmaxf(mxx, px);
+      mny = fminf(mny, py);
+      mxy = fmaxf(mxy, py);
+    }
+    int xs = 0, xe = rd.out_w - 1, ys = 0, ye = rd.out_h - 1;
+    if (!behind) {
+      xs = max(xs, (int)ceilf(mnx - 0.5f - 1e-2f));
+      xe = min(xe, (int)floorf(mxx - 0.5f + 1e-2f));
+      ys = max(ys, (int)ceilf(mny - 0.5f - 1e-2f));
+      ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
+    }
+    const float* gview = gsamp + (size_t)vi * rd.out_d * plane * C;
+    const int2* rview = ranges + (size_t)vi * plane;
+    if (xs <= xe && ys <= ye) {
+      const int nx = xe - xs + 1, cnt = nx * (ye - ys + 1);
+      for (int k0 = 0; k0 < cnt; k0 += 64) {
+        const int k = k0 + lane;
+        if (k >= cnt) break;
+        const int px = xs + k % nx, py = ys + k / nx;
+        const size_t pix = size_t(py) * rd.out_w + px;
+        const int2 range = rview[pix];
+        const int lo = range.x, hi = range.y;
+        if (lo > hi) continue;
+        const float fpx = (float)px, fpy = (float)py;
+        const float4 gn = make_float4(fmaf(fpy, pv.gy[0], fmaf(fpx, pv.gx[0], pv.near0[0])),
+                                      fmaf(fpy, pv.gy[1], fmaf(fpx, pv.gx[1], pv.near0[1])),
+                                      fmaf(fpy, pv.gy[2], fmaf(fpx, pv.gx[2], pv.near0[2])),
+                                      fmaf(fpy, pv.gy[3], fmaf(fpx, pv.gx[3], pv.near0[3])));
+        const float4 gf = make_float4(fmaf(fpy, pv.gy[0], fmaf(fpx, pv.gx[0], pv.far0[0])),
+                                      fmaf(fpy, pv.gy[1], fmaf(fpx, pv.gx[1], pv.far0[1])),
+                                      fmaf(fpy, pv.gy[2], fmaf(fpx, pv.gx[2], pv.far0[2])),
+                                      fmaf(fpy, pv.gy[3], fmaf(fpx, pv.gx[3], pv.far0[3])));
+        const float nw = __builtin_amdgcn_rcpf(gn.w), fw = __builtin_amdgcn_rcpf(gf.w);
+        const float nx_ = gn.x * nw, ny_ = gn.y * nw, nz_ = gn.z * nw;
+        const float sx = (gf.x * fw - nx_) * PL.inv_zmax, sy = (gf.y * fw - ny_) * PL.inv_zmax,
+                    sz = (gf.z * fw - nz_) * PL.inv_zmax;
+        const float irx = __builtin_amdgcn_rcpf(sx), iry = __builtin_amdgcn_rcpf(sy), irz = __builtin_amdgcn_rcpf(sz);
+        // slab ends as products: a zero step component keeps a ray inside its slab
+        const float ax = (bx0 - nx_) * irx, bx = (bx1 - nx_) * irx;
+        const float ay = (by0 - ny_) * iry, by = (by1 - ny_) * iry;
+        const float az = (bz0 - nz_) * irz, bz = (bz1 - nz_) * irz;
+        const float tlo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+        const float thi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+        const int i0 = max(lo, (int)ceilf(fmaxf(tlo, -1.f) - 1e-3f));
+        const int i1 = min(hi, (int)floorf(fminf(thi, 1e9f) + 1e-3f));
+        if (i0 > i1) continue;
+        const float fi0 = (float)i0;
+        float qx = fmaf(fi0, sx, nx_) - (fx0 + 0.5f), qy = fmaf(fi0, sy, ny_) - (fy0 + 0.5f),
+              qz = fmaf(fi0, sz, nz_) - (fz0 + 0.5f);
+        const float* gp = gview + ((size_t)i0 * plane + pix) * C;
+        for (int i = i0; i <= i1; ++i) {
+          const Vec<C> gs = vload<C>(gp, 0);
+          gp += plane * C;
+          if (gs.v[0] == gs.v[0]) {  // NaN mark: a sample the scatter skips
+            const float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+            const float tx = qx - flx, ty = qy - fly, tz = qz - flz;
+            const int lx = (int)flx, ly = (int)fly, lz = (int)flz;
+            const float wx[2] = {1.f - tx, tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
+#pragma unroll
+            for (int kz = 0; kz < 2; ++kz) {
+              const int cz = lz + kz;
+              if ((unsigned)cz >= (unsigned)WZ) continue;
+#pragma unroll
+              for (int ky = 0; ky < 2; ++ky) {
+                const int cy = ly + ky;
+                if ((unsigned)cy >= (unsigned)WY) continue;
+                const float wyz = wy[ky] * wz[kz];
+#pragma unroll
+                for (int kx = 0; kx < 2; ++kx) {
+                  const int cx = lx + kx;
+                  if ((unsigned)cx >= (unsigned)WX) continue;
+                  const float w = wx[kx] * wyz;
+                  lds_double* a = acc + ((cz * WY + cy) * WX + cx) * kCh;
+#pragma unroll
+                  for (int c = 0; c < C; ++c)
+                    __hip_atomic_fetch_add(a + c, (double)(gs.v[c] * w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                  if (!EXACT)
+                    __hip_atomic_fetch_add(a + C, (double)w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+              }
+            }
+          }
+          qx += sx;
+          qy += sy;
+          qz += sz;
+        }
+      }
+    }
+    if (!EXACT) {
+      // per-view normalisation by the weight sum (where positive), then clear
+#pragma unroll
+      for (int j = 0; j < kOwn; ++j) {
+        lds_double* a = acc + (lane + 64 * j) * kCh;
+        const float w = (float)a[C];
+        if (w > 0.f) {
+          const float inv = 1.0f / w;
+#pragma unroll
+          for (int c = 0; c < C; ++c) tot[j][c] += (float)a[c] * inv;
+        }
+#pragma unroll
+        for (int c = 0; c < kCh; ++c) a[c] = 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+#pragma unroll
+  for (int j = 0; j < kOwn; ++j) {
+    const int k = lane + 64 * j;
+    const int x = x0 + k % WX, y = y0 + (k / WX) % WY, z = z0 + k / (WX * WY);
+    if (x < d.x && y < d.y && z < d.z) {
+      Vec<C> r;
+#pragma unroll
+      for (int c = 0; c < C; ++c) r.v[c] = EXACT ? (float)acc[k * kCh + c] : tot[j][c];
+      // the caller cleared grad; chunks of views add up
+      Vec<C> o = vload<C>(grad, flat3(x, y, z, d));
+#pragma unroll
+      for (int c = 0; c < C; ++c) o.v[c] += r.v[c];
+      vstore<C>(grad, flat3(x, y, z, d), o);
+    }
+  }
+}
+
 // ------------------------------------------------------------- dispatch
 template <int C, int FM, int BM>
 static hipError_t launch_fwd_bl(int bl, dim3 grid, hipStream_t s, const float* in, float* out, const RayDims& rd,
@@ -235,6 +471,55 @@ hipError_t normalize_add_launch(int C, hipStream_t s, float* total, float* buf, 
     default: hipLaunchKernelGGL((k_normalize_add<4>), grid, dim3(256), 0, s, total, buf, wsum, nvox); break;
   }
   return hipGetLastError();
+}
+
+// BORDER walk for one chunk of views (launch-local view index vi)
+template <int C, bool EX>
+static hipError_t launch_walk_bl(int bl, dim3 grid, hipStream_t s, const float* in, const float* o, const float* og,
+                                 float* gs, int2* rg, const RayDims& rd, const RayLaunch& L) {
+  switch (bl) {
+    case BL_BEER_LAMBERT: hipLaunchKernelGGL((k_raymarch_bwd_walk<C, BL_BEER_LAMBERT, EX>), grid, dim3(256), 0, s, in, o, og, gs, rg, rd, L); break;
+    case BL_ALPHA: hipLaunchKernelGGL((k_raymarch_bwd_walk<C, BL_ALPHA, EX>), grid, dim3(256), 0, s, in, o, og, gs, rg, rd, L); break;
+    case BL_ALPHA_ADDITIVE: hipLaunchKernelGGL((k_raymarch_bwd_walk<C, BL_ALPHA_ADDITIVE, EX>), grid, dim3(256), 0, s, in, o, og, gs, rg, rd, L); break;
+    default: hipLaunchKernelGGL((k_raymarch_bwd_walk<C, BL_ADDITIVE, EX>), grid, dim3(256), 0, s, in, o, og, gs, rg, rd, L); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t raymarch_bwd_walk_launch(int C, bool exact, int bl, hipStream_t s, const float* in, const float* o,
+                                    const float* og, float* gsamp, int2* ranges, const RayDims& rd,
+                                    const RayLaunch& L) {
+  dim3 grid((rd.out_w + kTileX - 1) / kTileX, (rd.out_h + kTileY - 1) / kTileY, L.num_views);
+  KernelTimer t(s, "raymarch_bwd_walk");
+  switch (C) {
+    case 1: return exact ? launch_walk_bl<1, true>(bl, grid, s, in, o, og, gsamp, ranges, rd, L)
+                         : launch_walk_bl<1, false>(bl, grid, s, in, o, og, gsamp, ranges, rd, L);
+    case 2: return exact ? launch_walk_bl<2, true>(bl, grid, s, in, o, og, gsamp, ranges, rd, L)
+                         : launch_walk_bl<2, false>(bl, grid, s, in, o, og, gsamp, ranges, rd, L);
+    default: return exact ? launch_walk_bl<4, true>(bl, grid, s, in, o, og, gsamp, ranges, rd, L)
+                          : launch_walk_bl<4, false>(bl, grid, s, in, o, og, gsamp, ranges, rd, L);
+  }
+}
+
+// voxel blocks per channel count: 512, 256, 128 voxels (FP64 LDS per wave <= 5 KB)
+template <int C, int WX, int WY, int WZ>
+static hipError_t launch_gather(bool exact, hipStream_t s, const float* gsamp, const int2* ranges, float* grad,
+                                const RayDims& rd, const ProjLaunch& PL) {
+  const int nb = ((rd.in.x + WX - 1) / WX) * ((rd.in.y + WY - 1) / WY) * ((rd.in.z + WZ - 1) / WZ);
+  const dim3 grid((nb + 3) / 4);
+  if (exact) hipLaunchKernelGGL((k_raymarch_bwd_gather<C, WX, WY, WZ, true>), grid, dim3(256), 0, s, gsamp, ranges, grad, rd, PL);
+  else hipLaunchKernelGGL((k_raymarch_bwd_gather<C, WX, WY, WZ, false>), grid, dim3(256), 0, s, gsamp, ranges, grad, rd, PL);
+  return hipGetLastError();
+}
+
+hipError_t raymarch_bwd_gather_launch(int C, bool exact, hipStream_t s, const float* gsamp, const int2* ranges,
+                                      float* grad, const RayDims& rd, const ProjLaunch& PL) {
+  KernelTimer t(s, "raymarch_bwd_gather");
+  switch (C) {
+    case 1: return launch_gather<1, 16, 16, 2>(exact, s, gsamp, ranges, grad, rd, PL);
+    case 2: return launch_gather<2, 16, 8, 2>(exact, s, gsamp, ranges, grad, rd, PL);
+    default: return launch_gather<4, 8, 8, 2>(exact, s, gsamp, ranges, grad, rd, PL);
+  }
 }
 
 }  // namespace gft

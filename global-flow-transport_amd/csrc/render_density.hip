@@ -49,32 +49,6 @@ __device__ __forceinline__ void pixel_ray(const RayView& v, const RayDims& rd, i
   step = make_float3((f.x - nearp.x) * izm, (f.y - nearp.y) * izm, (f.z - nearp.z) * izm);
 }
 
-// Sample range of a ray that can hold samples inside the grid (positions carry
-// the +0.5 cell-centre offset, so inside means 0 <= p < dims); one sample of
-// slack on each end, the per-sample test stays exact. lo > hi: no sample.
-__device__ __forceinline__ void clip_ray(float3 n, float3 st, Dims3 d, int S, int& lo, int& hi) {
-  float tlo = 0.f, thi = (float)(S - 1);
-  const float nn[3] = {n.x, n.y, n.z}, ss[3] = {st.x, st.y, st.z};
-  const float dd[3] = {(float)d.x, (float)d.y, (float)d.z};
-  bool miss = false;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (ss[a] != 0.f) {
-      float t0 = (0.f - nn[a]) / ss[a], t1 = (dd[a] - nn[a]) / ss[a];
-      tlo = fmaxf(tlo, fminf(t0, t1));
-      thi = fminf(thi, fmaxf(t0, t1));
-    } else if (!(0.f <= nn[a] && nn[a] < dd[a])) {
-      miss = true;
-    }
-  }
-  lo = max(0, (int)floorf(tlo) - 1);
-  hi = min(S - 1, (int)ceilf(thi) + 1);
-  if (miss || thi < tlo - 2.f) {
-    lo = 1;
-    hi = 0;
-  }
-}
-
 // Corner-quad copy of a scalar grid for BORDER trilinear taps: entry
 // (x, y, z) for x in [-1, W-1], y in [-1, H-1], z in [-1, D] holds the float4
 // (v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z)) with v = 0 outside the grid
@@ -189,7 +163,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     int lo = 1, hi = 0;
     if (valid) {
       pixel_ray(v, rd, x, y, nearp, step);
-      clip_ray(nearp, step, rd.in, rd.out_d, lo, hi);
+      clip_ray(nearp, step, rd.in, rd.out_d, 0.f, lo, hi);
     }
     const size_t plane = size_t(rd.out_w) * rd.out_h;
     const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;

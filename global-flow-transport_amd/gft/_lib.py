@@ -46,7 +46,7 @@ _SIGNATURES = {
     "gft_abi_version": (ctypes.c_int, []),
     "gft_raymarch_grid_transform_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _i32,
                                                        _i32, _i32, _i32, _i32, _vp]),
-    "gft_raymarch_grid_transform_bwd_workspace": (_sz, [_i64p, _i32]),
+    "gft_raymarch_grid_transform_bwd_workspace": (_sz, [_i64p, _i32, _i32p, _i32, _i32, _i32]),
     "gft_raymarch_grid_transform_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _i32p,
                                                        _i32, _i32, _i32, _i32, _i32, _i32, _vp, _sz, _vp]),
     "gft_advect_saved_bytes": (_sz, [_i64p]),

@@ -132,7 +132,8 @@ def _raymarch_bwd_impl(input, output, output_grad, M, nM, V, P, F, nCam, out_dhw
                                         % (list(output.shape), list(output_grad.shape)))
     L = _lib.lib()
     shape = _lib.i64_array(input.shape)
-    ws_bytes = L.gft_raymarch_grid_transform_bwd_workspace(shape, grad_mode)
+    ws_bytes = L.gft_raymarch_grid_transform_bwd_workspace(shape, nCam, _lib.i32_array(out_dhw), boundary,
+                                                           int(bool(separate_camera_batch)), grad_mode)
     ws = torch.empty((max(int(ws_bytes), 1),), dtype=torch.uint8, device=input.device)
     grad = torch.empty_like(input)
     _lib.check(L.gft_raymarch_grid_transform_bwd(

@@ -74,9 +74,14 @@ int gft_raymarch_grid_transform_fwd(void* stream, const float* input, const int6
  * RaymarchGridGradKernel (raymarch_grid.hpp:45-53, raymarch_grid.cu.cc:451-594).
  * interpolation must be LINEAR (the reference's grad op attr, :66).
  * grad_mode REFERENCE reproduces the per-camera weight normalisation;
- * EXACT is the true adjoint. input_grad is fully overwritten.
+ * EXACT is the true adjoint. input_grad is fully overwritten. BORDER runs
+ * without atomics (a walk stores each sample's gradient, a voxel-block
+ * gather sums them in a fixed order) and needs per-sample workspace for up to
+ * 16 views of one batch; CLAMP/WRAP scatter with atomics.
  */
-size_t gft_raymarch_grid_transform_bwd_workspace(const int64_t input_shape[5], int32_t grad_mode);
+size_t gft_raymarch_grid_transform_bwd_workspace(const int64_t input_shape[5], int32_t num_cameras,
+                                                 const int32_t output_shape[3], int32_t boundary,
+                                                 int32_t separate_camera_batch, int32_t grad_mode);
 int gft_raymarch_grid_transform_bwd(void* stream, const float* input, const int64_t input_shape[5],
                                     const float* output, const float* output_grad, const float* M, int32_t num_M,
                                     const float* V, const float* P, const float* frustum, int32_t num_cameras,

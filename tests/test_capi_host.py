@@ -38,7 +38,7 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
 
 
 def test_abi_version_and_error_slot(lib):
-    assert lib.gft_abi_version() == 3
+    assert lib.gft_abi_version() == 4
     assert isinstance(lib.gft_last_error(), bytes)
 
 
@@ -60,9 +60,17 @@ def test_workspace_queries(lib):
     assert lib.gft_sample_grid_transform_bwd_workspace(shape, 0, 0, 0) == ((13 * 18 * 22 * 4 + 255) // 256) * 256
     assert lib.gft_sample_grid_transform_bwd_workspace(shape, 2, 6, 1) == 2 * lib.gft_mip_atlas_bytes(shape, 6)
     assert lib.gft_sample_grid_lut_bwd_workspace(shape, 1) == 0
-    assert lib.gft_raymarch_grid_transform_bwd_workspace(shape, 1) == 0
+    out = _i32(7, 5, 6)
+    CLAMP, BORDER = 1, 0
+    assert lib.gft_raymarch_grid_transform_bwd_workspace(shape, 3, out, CLAMP, 1, 1) == 0
     vox = 13 * 18 * 22
-    assert lib.gft_raymarch_grid_transform_bwd_workspace(shape, 0) == ((vox * 4 * 4 + 255) // 256) * 256 + vox * 4
+    assert (lib.gft_raymarch_grid_transform_bwd_workspace(shape, 3, out, CLAMP, 1, 0)
+            == ((vox * 4 * 4 + 255) // 256) * 256 + vox * 4)
+    # BORDER: per-sample gradients of one batch's views (at most 16 per chunk) plus the sample ranges
+    for cams, sep, views in ((3, 1, 3), (20, 1, 16), (3, 0, 1)):
+        for gm in (0, 1):
+            want = ((views * 7 * 30 * 4 * 4 + 255) // 256) * 256 + views * 30 * 8
+            assert lib.gft_raymarch_grid_transform_bwd_workspace(shape, cams, out, BORDER, sep, gm) == want
     assert lib.gft_advect_saved_bytes(_i64(1, 4, 5, 6, 1)) > 0
 
 
