@@ -53,7 +53,14 @@ __global__ __launch_bounds__(256) void k_raymarch_fwd(const float* __restrict__ 
   ray_endpoints(v, x, y, rd, nearp, farp, step);
   const float* buf = input + rd.in_batch_stride * v.in_batch;
   Vec<C> acc = vzero<C>();
-  for (int i = 0; i < rd.out_d; ++i) {
+  // BORDER LINEAR / NEAREST: samples outside the box that any corner or the
+  // truncated nearest index can reach read 0, and blending a 0 sample leaves the
+  // accumulator unchanged in every mode, so only the clipped range is read
+  // (MIN/MAX return +-FLT_MAX outside and are not clipped)
+  int lo = 0, hi = rd.out_d - 1;
+  if (BM == B_BORDER && (FM == F_LINEAR || FM == F_NEAREST))
+    clip_ray(nearp, step, rd.in, rd.out_d, FM == F_LINEAR ? 0.f : 1.f, lo, hi);
+  for (int i = lo; i <= hi; ++i) {
     float fi = (float)i;
     float3 p = make_float3(fmaf(fi, step.x, nearp.x), fmaf(fi, step.y, nearp.y), fmaf(fi, step.z, nearp.z));
     Vec<C> s = sample_ray<C, FM, BM>(buf, rd.in, p);
