@@ -21,6 +21,8 @@ w.r.t. the data and the velocity) when ``grad_mode`` is "EXACT"; with
 "REFERENCE" they are composed of SampleGridLut passes exactly as the reference
 composes them, so their gradients are the reference's count-normalised ones.
 """
+import os
+
 import numpy as np
 import torch
 
@@ -29,6 +31,10 @@ from .ops import sampling_ops
 from .pipeline import advect as fused_advect
 
 WARP_CLAMPS = ("NONE", "MC", "MC_SMOOTH", "NEGATIVE")
+
+
+def _to_numpy(t):
+    return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
 
 
 def _check_clamp(clamp):
@@ -82,8 +88,10 @@ def _fused_warp(data, vel, order, dt, clamp):
 
 class DensityGrid(object):
     def __init__(self, shape, constant=0.1, d=None, inflow=None, inflow_offset=None, inflow_mask=None, device="cuda",
-                 grad_mode="EXACT"):
+                 grad_mode="EXACT", hull=None):
         self.shape = [int(s) for s in shape]
+        self.hull = hull
+        self.inflow_offset = None if inflow_offset is None else [int(o) for o in inflow_offset]
         if d is None:
             d = torch.full([1] + self.shape + [1], float(constant), dtype=torch.float32, device=device)
         if tuple(d.shape) != tuple([1] + self.shape + [1]):
@@ -125,6 +133,45 @@ class DensityGrid(object):
 
     def copy_warped(self, vel_grid, order=1, dt=1.0, clamp="NONE"):
         return DensityGrid(self.shape, d=self.warped(vel_grid, order, dt, clamp), grad_mode=self.grad_mode)
+
+    # ------------------------------------------------------------ state I/O
+    def save(self, path):
+        """DensityGrid.save (data_structures.py:332-353): compressed npz with the
+        density as the first positional array (``arr_0``) and the optional
+        ``hull``, ``inflow``, ``inflow_mask``, ``inflow_offset``."""
+        save = {}
+        if self.hull is not None:
+            save["hull"] = _to_numpy(self.hull)
+        if self._inflow is not None:
+            save["inflow"] = _to_numpy(self._inflow)
+            if self.inflow_mask is not None:
+                save["inflow_mask"] = _to_numpy(self.inflow_mask)
+            save["inflow_offset"] = np.asarray(self.inflow_offset if self.inflow_offset is not None else [0, 0, 0])
+        np.savez_compressed(path, _to_numpy(self._d), **save)
+
+    @classmethod
+    def from_file(cls, path, hull=None, inflow=None, inflow_offset=None, inflow_mask=None, device="cuda",
+                  grad_mode="EXACT"):
+        """DensityGrid.from_file (data_structures.py:174-193); the stored hull
+        and inflow fields are used where the caller passes none. Returns None
+        (as the reference does) when the file cannot be read."""
+        try:
+            with np.load(path) as f:
+                d = f["arr_0"]
+                if "hull" in f and hull is None:
+                    hull = torch.from_numpy(np.ascontiguousarray(f["hull"])).to(device)
+                if "inflow" in f and inflow is None:
+                    inflow = torch.from_numpy(np.ascontiguousarray(f["inflow"], dtype=np.float32)).to(device)
+                    if "inflow_mask" in f and inflow_mask is None:
+                        inflow_mask = torch.from_numpy(np.ascontiguousarray(f["inflow_mask"],
+                                                                            dtype=np.float32)).to(device)
+                    if "inflow_offset" in f and inflow_offset is None:
+                        inflow_offset = f["inflow_offset"].tolist()
+        except (OSError, KeyError, ValueError):
+            return None
+        d = torch.from_numpy(np.ascontiguousarray(d, dtype=np.float32)).to(device)
+        return cls(list(d.shape[1:4]), d=d, inflow=inflow, inflow_offset=inflow_offset, inflow_mask=inflow_mask,
+                   device=device, grad_mode=grad_mode, hull=hull)
 
 
 class VelocityGrid(object):
@@ -262,6 +309,34 @@ class VelocityGrid(object):
         return torch.linalg.norm(v, dim=-1, keepdim=True)
 
 
+    # ------------------------------------------------------------ state I/O
+    def save(self, path):
+        """VelocityGrid.save (data_structures.py:680-681): compressed npz with
+        ``centered_shape`` and the staggered components ``vel_x``, ``vel_y``,
+        ``vel_z`` (NDHWC, N = C = 1)."""
+        np.savez_compressed(path, centered_shape=np.asarray(self.centered_shape), vel_x=_to_numpy(self.x),
+                            vel_y=_to_numpy(self.y), vel_z=_to_numpy(self.z))
+
+    @classmethod
+    def from_file(cls, path, boundary="CLAMP", device="cuda", grad_mode="EXACT"):
+        """VelocityGrid.from_file (data_structures.py:487-503), including the
+        legacy files without ``centered_shape`` (shape from vel_x, one fewer
+        cell in x). Returns None when the file cannot be read."""
+        try:
+            with np.load(path) as f:
+                if "centered_shape" not in f:
+                    shape = list(f["vel_x"].shape)
+                    shape[-2] -= 1
+                    shape = shape[1:-1]
+                else:
+                    shape = f["centered_shape"].tolist()
+                comps = [torch.from_numpy(np.ascontiguousarray(f[k], dtype=np.float32)).to(device)
+                         for k in ("vel_x", "vel_y", "vel_z")]
+        except (OSError, KeyError, ValueError):
+            return None
+        return cls(shape, x=comps[0], y=comps[1], z=comps[2], boundary=boundary, device=device, grad_mode=grad_mode)
+
+
 class State(object):
     def __init__(self, density, velocity, frame, prev=None, next=None, targets=None):
         self.density = density
@@ -277,8 +352,48 @@ class State(object):
     def velocity_advected(self, dt=1.0, order=1, clamp="NONE"):
         return self.velocity.copy_warped(order=order, dt=dt, clamp=clamp)
 
+    def save(self, path, suffix=None):
+        """State.save (data_structures.py:1213-1215): density.npz and
+        velocity.npz (or density_<suffix>.npz, velocity_<suffix>.npz) in path."""
+        self.density.save(os.path.join(path, "density.npz" if suffix is None else "density_" + suffix + ".npz"))
+        self.velocity.save(os.path.join(path, "velocity.npz" if suffix is None else "velocity_" + suffix + ".npz"))
+
+    @classmethod
+    def from_file(cls, path, frame, boundary="CLAMP", device="cuda", density_filename="density.npz",
+                  velocity_filename="velocity.npz"):
+        """State.from_file (data_structures.py:1123-1128)."""
+        density = DensityGrid.from_file(os.path.join(path, density_filename), device=device)
+        velocity = VelocityGrid.from_file(os.path.join(path, velocity_filename), boundary=boundary, device=device)
+        return cls(density, velocity, frame)
+
 
 class Sequence(list):
+    @classmethod
+    def from_file(cls, load_path, frames, boundary="CLAMP", device="cuda", density_filename="density.npz",
+                  velocity_filename="velocity.npz"):
+        """Sequence.from_file (data_structures.py:1241-1258): one State per
+        frame from load_path/frame_{:06d}, linked prev/next."""
+        seq = cls()
+        prev = None
+        for frame in frames:
+            st = State.from_file(os.path.join(load_path, "frame_{:06d}".format(frame)), frame, boundary=boundary,
+                                 device=device, density_filename=density_filename,
+                                 velocity_filename=velocity_filename)
+            st.prev = prev
+            if prev is not None:
+                prev.next = st
+            prev = st
+            seq.append(st)
+        return seq
+
+    def save(self, path, suffix=None):
+        """Sequence.save (data_structures.py:1302-1307): each state into
+        path/frame_{:06d}."""
+        for st in self:
+            d = os.path.join(path, "frame_{:06d}".format(st.frame))
+            os.makedirs(d, exist_ok=True)
+            st.save(d, suffix)
+
     def densities_advect_fwd(self, dt=1.0, order=1, clamp="NONE"):
         """data_structures.py:1309-1320; LOCAL clamps negatives after each
         warp, GLOBAL after the whole chain."""
