@@ -23,6 +23,10 @@ known traffic for the HBM counters (FETCH_SIZE / WRITE_SIZE) under rocprofv3.
 (resample, mips, LoD, their gradients, the gather gradient, raymarch, depth
 blend, LuT sampling) at the c2 sizes, each against its compulsory bytes.
 
+``--shadow`` instead times the spot-light shadow map at the production
+settings (256x196x196, LINEAR mips, gather gradient) over the c2 density:
+the fused shadow op against the op sequence it replaces.
+
 ``--chain`` instead measures the global transport chain (workload c3): an
 initial density advected through 20 per-frame velocity fields, every frame
 rendered to 5 views plus one unseen view and compared with its targets, the
@@ -254,6 +258,108 @@ def chain_bench(args):
         "ms_per_chain_every_frame": ms_all, "loss": loss}))
 
 
+def shadow_bench(args):
+    """One JSON line for the spot-light shadow map at the production settings
+    (256x196x196 shadow map, LINEAR mips 4 levels, gather sampling gradient,
+    no cone mask) over the c2 density: the fused shadow op against the op
+    sequence it replaces, forward and forward+backward, and the whole
+    shadow map (with the reverse resample to the grid)."""
+    import numpy as np
+    import torch
+
+    from gft import synthetic as syn
+    from gft._lib import KernelTimers
+    from gft.lighting import SpotLight
+    from gft.renderer import Renderer
+    from gft.transform import Transform
+
+    grid = syn.WORKLOADS["c2"]["grid"]
+    res = [256, 196, 196]
+    dev = torch.device("cuda:0")
+    gt = syn.grid_transform(grid)
+    d = torch.from_numpy(syn.gaussian_blobs(grid, 48, seed=0) * 0.05).to(dev)
+    gt.set_data(d)
+    # the production light (reconstruct_sequence.py:698-700) in cell units: 300 cells from the
+    # grid centre, tilted 40 degrees down, clip 140..460, 50 degree cone
+    sl = SpotLight(Transform(translation=[0, 0, 300], parent=Transform(rotation_deg=[-40, 0, 0])), intensity=0.85,
+                   cast_shadows=True, shadow_clip=[140, 460], range_scale=0.825, angle_deg=50.0,
+                   shadow_resolution=res, cone_mask=False)
+    r = Renderer(filter_mode="LINEAR", boundary_mode="BORDER", mipmapping="LINEAR", num_mips=4, mip_bias=0.0,
+                 blend_mode="BEER_LAMBERT", sample_gradients=True)
+    cam = sl.shadow_cam
+    g_frustum = torch.rand([1] + res + [1], device=dev)
+    g_grid = torch.rand_like(d)
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(args.steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts))
+
+    def frustum(fused, backward):
+        def run():
+            x = d.detach().requires_grad_(backward)
+            gt.set_data(x)
+            t = r.shadow_transmission(gt, cam) if fused else sl._shadow_ops(gt, r)
+            if backward:
+                t.backward(g_frustum)
+        return run
+
+    def whole(fused):
+        def run():
+            x = d.detach().requires_grad_(True)
+            gt.set_data(x)
+            sl.render_shadow_map(gt, r, fused=fused).backward(g_grid)
+        return run
+
+    out = {}
+    for name, fn in (("fused fwd", frustum(True, False)), ("op sequence fwd", frustum(False, False)),
+                     ("fused fwd+bwd", frustum(True, True)), ("op sequence fwd+bwd", frustum(False, True)),
+                     ("shadow map fused fwd+bwd", whole(True)), ("shadow map op sequence fwd+bwd", whole(False))):
+        out[name] = timeit(fn)
+    # kernel timers are per thread and autograd runs the backward on its own thread: call the
+    # fused forward and the scan backward directly here
+    from gft import ops
+    M = np.asarray(gt.get_transform_matrix(), dtype=np.float32).T[None]
+    V, P, F = r._get_camera_params_batch([cam])
+    cams = ops._cams(M, V, P, F)
+    with KernelTimers("shadow_fwd,shadow_bwd") as kt:
+        for _ in range(args.steps):
+            _, saved = ops._shadow_fwd_impl(d, cams, tuple(res), cam.depth_step, ops.FILTER["LINEAR"],
+                                            ops.BOUNDARY["BORDER"], ops.MIP["LINEAR"], 4, 0.0,
+                                            ops.BLEND["BEER_LAMBERT"], None, save=True)
+            ops._shadow_bwd_impl(saved, g_frustum, tuple(res), ops.BOUNDARY["BORDER"], ops.BLEND["BEER_LAMBERT"],
+                                 None)
+        torch.cuda.synchronize()
+    kern = {k: v[1] / max(v[0], 1) for k, v in kt.result.items()}
+    cells = int(np.prod(res))
+    vox = int(d.numel())
+    # compulsory bytes: fwd reads the density and its pyramid once, writes transmittance + prefix sums;
+    # bwd reads prefix sums + transmittance gradient, writes the frustum gradient
+    fwd_bytes = vox * 4 * (1 + 1.0 / 7) + cells * 8
+    bwd_bytes = cells * 12
+    gt.set_data(d)
+    print(json.dumps({
+        "metric": "spot-light shadow map time, fused vs op sequence", "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c2 density grid 128x224x128 (x,y,z); shadow map 256x196x196 (D,H,W); "
+                               "LINEAR filter, BORDER, LINEAR mips 4 levels, BEER_LAMBERT, gather gradient, "
+                               "no cone mask"},
+        "ms": out, "speedup_fwd": out["op sequence fwd"] / out["fused fwd"],
+        "speedup_fwd_bwd": out["op sequence fwd+bwd"] / out["fused fwd+bwd"],
+        "kernels_ms": kern,
+        "shadow_fwd": {"voxel_samples_per_s": cells / (kern["shadow_fwd"] * 1e-3),
+                       "GBps": fwd_bytes / (kern["shadow_fwd"] * 1e-3) / 1e9, "alg_bytes": int(fwd_bytes)},
+        "shadow_bwd": {"GBps": bwd_bytes / (kern["shadow_bwd"] * 1e-3) / 1e9, "alg_bytes": int(bwd_bytes)}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +372,7 @@ def main():
     ap.add_argument("--copy-kernel", action="store_true")
     ap.add_argument("--chain", action="store_true")
     ap.add_argument("--ops", action="store_true")
+    ap.add_argument("--shadow", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -278,6 +385,9 @@ def main():
         return
     if args.ops:
         ops_bench(args)
+        return
+    if args.shadow:
+        shadow_bench(args)
         return
 
     import numpy as np

@@ -11,7 +11,7 @@ namespace gft {
 
 hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v);
 hipError_t mip_down_launch(int C, hipStream_t st, const float* src, size_t src_stride, Dims3 s, float* dst,
-                           size_t dst_stride, Dims3 o, int batches);
+                           size_t dst_stride, Dims3 o, int batches, float scale);
 hipError_t mip_collapse_launch(int C, hipStream_t st, const float* gp, size_t gp_stride, Dims3 o, float* gc,
                                size_t gc_stride, Dims3 s, int batches);
 hipError_t sample_transform_launch(int C, int bm, bool sum, hipStream_t st, const float* in, const float* atlas,
@@ -22,6 +22,11 @@ hipError_t sample_transform_scatter_ref_launch(int C, int bm, hipStream_t st, co
 hipError_t sample_transform_adjoint_launch(int C, int bm, hipStream_t st, const float* in, const float* atlas,
                                            const MipTable& mt, const SampleCfg& s, Dims3 dout, const SampleLaunch& L,
                                            const float* og, float* grad, float* gatlas);
+hipError_t shadow_fwd_launch(int bm, int fm, int bl, hipStream_t st, const float* dens, const float* atlas,
+                             const MipTable& mt, const SampleCfg& cfg, const SampleView& v, Dims3 dout, float step,
+                             const float* mask, float* saved, float* trans);
+hipError_t shadow_bwd_launch(int bm, int bl, hipStream_t st, const float* saved, const float* gt, const float* mask,
+                             Dims3 dout, float* gs);
 hipError_t normalize_count_launch(int C, hipStream_t st, float* grad, const unsigned* cnt, size_t nvox);
 hipError_t sample_lut_launch(int C, int bm, hipStream_t st, const float* in, const float* atlas, const MipTable& mt,
                              const SampleCfg& s, const LutLaunch& L, int pairs, const float* lut, float* out);
@@ -67,12 +72,15 @@ static size_t atlas_bytes(Dims3 d, int C, int levels, int batches) {
   return align256(mt.atlas_stride * sizeof(float) * (size_t)batches);
 }
 
-static hipError_t build_atlas(hipStream_t st, const float* base, const MipTable& mt, int batches, float* atlas) {
+// Mip levels 1.. of `base` into the atlas; base_scale multiplies the base
+// grid's values as they are read (the pyramid of base * base_scale).
+static hipError_t build_atlas(hipStream_t st, const float* base, const MipTable& mt, int batches, float* atlas,
+                              float base_scale = 1.f) {
   for (int m = 1; m <= mt.levels; ++m) {
     const float* src = m == 1 ? base : atlas + mt.offset[m - 1];
     size_t src_stride = m == 1 ? mt.base_stride : mt.atlas_stride;
     hipError_t e = mip_down_launch(mt.C, st, src, src_stride, mt.dims[m - 1], atlas + mt.offset[m], mt.atlas_stride,
-                                   mt.dims[m], batches);
+                                   mt.dims[m], batches, m == 1 ? base_scale : 1.f);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -517,6 +525,76 @@ int gft_lod_transform(void* stream, const int32_t input_dhw[3], const float* MV,
       return rc;
   }
   return GFT_OK;
+}
+
+// ------------------------------------------------- shadow transmittance
+size_t gft_shadow_transmission_saved_bytes(const int32_t shadow_shape[3]) {
+  if (!shadow_shape || shadow_shape[0] < 1 || shadow_shape[1] < 1 || shadow_shape[2] < 1) return 0;
+  return nvox(dims_dhw(shadow_shape)) * sizeof(float);
+}
+
+size_t gft_shadow_transmission_fwd_workspace(const int64_t density_shape[5], int32_t mipmapping,
+                                             int32_t num_mipmaps) {
+  return gft_sample_grid_transform_workspace(density_shape, mipmapping, num_mipmaps);
+}
+
+static int check_shadow_attrs(const int32_t* shadow_shape, int32_t boundary, int32_t blending_mode) {
+  int rc;
+  if (!shadow_shape) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  if ((rc = check_out_shape(shadow_shape))) return rc;
+  if (boundary != GFT_BORDER && boundary != GFT_CLAMP)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "boundary must be BORDER or CLAMP");
+  if (blending_mode != GFT_BEER_LAMBERT && blending_mode != GFT_ALPHA)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "Unknown blend_mode: the shadow map supports BEER_LAMBERT and ALPHA");
+  return GFT_OK;
+}
+
+int gft_shadow_transmission_fwd(void* stream, const float* density, const int64_t density_shape[5], const float* M,
+                                const float* V, const float* P, const float* frustum, const int32_t shadow_shape[3],
+                                float depth_step, int32_t interpolation, int32_t boundary, int32_t mipmapping,
+                                int32_t num_mipmaps, float mip_bias, int32_t blending_mode, const float* mask,
+                                void* workspace, size_t workspace_bytes, float* saved, float* transmission) {
+  int rc;
+  if (!density_shape) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  if ((rc = check_grid(density_shape, "density"))) return rc;
+  if (density_shape[0] != 1 || density_shape[4] != 1)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "density must be [1,D,H,W,1], is [%lld,%lld,%lld,%lld,%lld]",
+                (long long)density_shape[0], (long long)density_shape[1], (long long)density_shape[2],
+                (long long)density_shape[3], (long long)density_shape[4]);
+  if ((rc = check_shadow_attrs(shadow_shape, boundary, blending_mode))) return rc;
+  if (interpolation != GFT_NEAREST && interpolation != GFT_LINEAR)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "interpolation must be NEAREST or LINEAR");
+  if ((rc = check_mips(mipmapping, num_mipmaps))) return rc;
+  if (!density || !M || !V || !P || !frustum || !transmission)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  size_t need = gft_shadow_transmission_fwd_workspace(density_shape, mipmapping, num_mipmaps);
+  if (workspace_bytes < need || (need && !workspace))
+    return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
+                need);
+  hipStream_t st = (hipStream_t)stream;
+  Dims3 din = dims_of(density_shape);
+  int levels = mipmapping == GFT_MIP_NONE ? 0 : usable_levels(din, num_mipmaps);
+  MipTable mt = make_mip_table(din, 1, levels, nvox(din));
+  float* atlas = (float*)workspace;
+  if (levels && (rc = check_hip(build_atlas(st, density, mt, 1, atlas, depth_step), "mip generation"))) return rc;
+  SampleCfg cfg{interpolation, mipmapping, levels, mip_bias, 0.5f};
+  std::vector<ViewPair> pairs{{0, 0, 0}};
+  SampleLaunch L;
+  if ((rc = build_sample_views(pairs, 0, 1, M, -1, V, P, frustum, CM_LINDEPTH, L))) return rc;
+  return check_hip(shadow_fwd_launch(boundary, interpolation, blending_mode, st, density, atlas, mt, cfg, L.view[0],
+                                     dims_dhw(shadow_shape), depth_step, mask, saved, transmission),
+                   "shadow transmission launch");
+}
+
+int gft_shadow_transmission_bwd(void* stream, const float* saved, const int32_t shadow_shape[3], int32_t boundary,
+                                int32_t blending_mode, const float* mask, const float* transmission_grad,
+                                float* frustum_grad) {
+  int rc;
+  if ((rc = check_shadow_attrs(shadow_shape, boundary, blending_mode))) return rc;
+  if (!saved || !transmission_grad || !frustum_grad) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  return check_hip(shadow_bwd_launch(boundary, blending_mode, (hipStream_t)stream, saved, transmission_grad, mask,
+                                     dims_dhw(shadow_shape), frustum_grad),
+                   "shadow transmission gradient launch");
 }
 
 }  // extern "C"

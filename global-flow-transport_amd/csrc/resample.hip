@@ -184,10 +184,13 @@ __device__ __forceinline__ float3 sample_pos(const SampleView& v, int cm, bool l
 
 // ---------------------------------------------------------------- mips
 // kGenerateMip3D (resample_grid.cu.cc:46-72): pairwise means in z, then y,
-// then x. One thread per coarse voxel; blockIdx.y = batch.
+// then x. One thread per coarse voxel; blockIdx.y = batch. Every source value
+// is multiplied by `scale` first (1 leaves it unchanged): the pyramid of a
+// scaled grid without materialising the scaled grid.
 template <int C>
 __global__ __launch_bounds__(256) void k_mip_down(const float* __restrict__ src, size_t src_stride, Dims3 s,
-                                                  float* __restrict__ dst, size_t dst_stride, Dims3 o) {
+                                                  float* __restrict__ dst, size_t dst_stride, Dims3 o, float scale) {
+#pragma clang fp contract(off)
   size_t n = (size_t)o.x * o.y * o.z;
   size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n) return;
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(256) void k_mip_down(const float* __restrict__ src,
       Vec<C> a = vload<C>(in, flat3(2 * x + xi, 2 * y + yi, 2 * z, s));
       Vec<C> b = vload<C>(in, flat3(2 * x + xi, 2 * y + yi, 2 * z + 1, s));
 #pragma unroll
-      for (int ch = 0; ch < C; ++ch) za[yi].v[ch] = (a.v[ch] + b.v[ch]) * 0.5f;
+      for (int ch = 0; ch < C; ++ch) za[yi].v[ch] = (a.v[ch] * scale + b.v[ch] * scale) * 0.5f;
     }
 #pragma unroll
     for (int ch = 0; ch < C; ++ch) ya[xi].v[ch] = (za[0].v[ch] + za[1].v[ch]) * 0.5f;
@@ -588,6 +591,220 @@ __global__ __launch_bounds__(256) void k_lod_transform(Dims3 din, Dims3 dout, Sa
   *reinterpret_cast<float4*>(out + ((size_t)v.out_slot * n + f) * 4) = make_float4(p.x, p.y, p.z, lod);
 }
 
+// ------------------------------------------------ shadow transmittance
+// Fused spot-light shadow map (phitest/render/lighting.py:106-141): the
+// density resampled into the light frustum (TRANSFORM_LINDEPTH), each tap
+// scaled by the depth step as the step-premultiplied grid would hold it
+// (renderer.py:365-385), its running sum along depth (reduce_grid_blend
+// keep_dims on one channel is a plain prefix sum), the one-cell shift with
+// the boundary's pad, the transmittance exp(-D) or 1 - clip(D, 0, 1) and
+// the cone mask, in one pass that writes only the transmittance and,
+// optionally, the prefix sums the backward needs.
+//
+// A block is 64 shadow-map pixels (one per lane, x fastest) by kShadowWaves
+// waves; a round covers kShadowWaves * kShadowSeg depth cells, wave w
+// sampling cells [w seg, (w + 1) seg) of it into LDS. Every wave then runs
+// the round's serial sum over LDS in depth order, the order of the unfused
+// running sum, so the prefix sums match it bit for bit, and keeps the values
+// of its own cells.
+constexpr int kShadowWaves = 8, kShadowSeg = 8, kShadowRound = kShadowWaves * kShadowSeg;
+constexpr int kShadowPitch = kShadowRound + 4;  // floats per pixel row; +4 spreads lanes over banks
+constexpr int kShadowUnroll = 1;
+
+// A grid value times the step, rounded on its own (never fused into the
+// lerp that follows), as the step-premultiplied grid stores it.
+__device__ __forceinline__ float scaled_load(const float* __restrict__ p, float sc) {
+#pragma clang fp contract(off)
+  return *p * sc;
+}
+
+// One step-scaled tap of the v1 sampler (read_linear / read_nearest with
+// every corner multiplied by `sc` before the lerps).
+template <int BM, int FM>
+__device__ __forceinline__ float fetch_scaled(const float* __restrict__ buf, Dims3 d, float px, float py, float pz,
+                                              float sc) {
+  if constexpr (FM == F_NEAREST) {
+    size_t f;
+    bool ok = nearest_index<BM>(px, py, pz, d, f);
+    return ok ? scaled_load(buf + f, sc) : 0.f;
+  } else {
+    Taps t = make_taps<BM>(px, py, pz, d);
+    if (BM == B_BORDER && !t.inside) return 0.f;
+    float c[2][2][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          c[k][j][i] = scaled_load(buf + flat3(t.ix[i], t.iy[j], t.iz[k], d), sc);
+          if (BM == B_BORDER && !(t.vx[i] & t.vy[j] & t.vz[k])) c[k][j][i] = 0.f;
+        }
+    float v0 = lerpf(lerpf(c[0][0][0], c[0][0][1], t.wx), lerpf(c[0][1][0], c[0][1][1], t.wx), t.wy);
+    float v1 = lerpf(lerpf(c[1][0][0], c[1][0][1], t.wx), lerpf(c[1][1][0], c[1][1][1], t.wx), t.wy);
+    return lerpf(v0, v1, t.wz);
+  }
+}
+
+template <int BL> __device__ __forceinline__ float shadow_transmit(float d) {
+  if constexpr (BL == BL_BEER_LAMBERT) return expf(-d);
+  else return 1.f - fminf(fmaxf(d, 0.f), 1.f);
+}
+
+// sample_v1 with step-scaled taps: level 0 is the unscaled grid, read with
+// `step`; the mip levels come from the pyramid of the scaled grid.
+template <int BM, int FM>
+__device__ __forceinline__ float sample_scaled(const float* __restrict__ base, const float* __restrict__ atlas,
+                                               const MipTable& mt, const SampleCfg& s, float3 p, float lod,
+                                               float step) {
+  int lv[2];
+  float w[2];
+  int n = level_taps(s, lod, lv, w);
+  float sc = 1.f / (float)(1 << lv[0]);
+  float a = fetch_scaled<BM, FM>(level_ptr(base, atlas, mt, 0, lv[0]), mt.dims[lv[0]], p.x * sc - s.offset,
+                                 p.y * sc - s.offset, p.z * sc - s.offset, lv[0] == 0 ? step : 1.f);
+  if (n == 2) {
+    float sc1 = 1.f / (float)(1 << lv[1]);
+    float c = fetch_scaled<BM, FM>(level_ptr(base, atlas, mt, 0, lv[1]), mt.dims[lv[1]], p.x * sc1 - s.offset,
+                                   p.y * sc1 - s.offset, p.z * sc1 - s.offset, lv[1] == 0 ? step : 1.f);
+    a = lerpf(a, c, w[1]);
+  }
+  return a;
+}
+
+template <int BM, int FM, int BL, bool LOD>
+__global__ __launch_bounds__(kShadowWaves * 64) void k_shadow_fwd(const float* __restrict__ dens,
+                                                                  const float* __restrict__ atlas, MipTable mt,
+                                                                  SampleCfg cfg, SampleView v, Dims3 dout, float step,
+                                                                  const float* __restrict__ mask,
+                                                                  float* __restrict__ saved,
+                                                                  float* __restrict__ trans) {
+  __shared__ float s_lds[64 * kShadowPitch];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const size_t plane = (size_t)dout.x * dout.y;
+  const size_t p = (size_t)blockIdx.x * 64 + lane;
+  const bool live = p < plane;
+  const int x = live ? (int)(p % (size_t)dout.x) : 0, y = live ? (int)(p / (size_t)dout.x) : 0;
+  const float3 oinv = make_float3(1.f / dout.x, 1.f / dout.y, 1.f / dout.z);
+  const float3 idims = make_float3((float)mt.dims[0].x, (float)mt.dims[0].y, (float)mt.dims[0].z);
+  const float mk = (mask && live) ? mask[p] : 1.f;
+  float* row = s_lds + lane * kShadowPitch;
+  float carry = 0.f;
+  for (int r0 = 0; r0 < dout.z; r0 += kShadowRound) {
+    // one sample per iteration keeps the kernel at the register budget of
+    // the plain resample (8 waves per SIMD); the waves hide each other's latency
+#pragma unroll kShadowUnroll
+    for (int i = 0; i < kShadowSeg; ++i) {
+      int z = r0 + w * kShadowSeg + i;
+      float val = 0.f;
+      if (live && z < dout.z) {
+        float l;
+        float3 q = sample_pos(v, CM_LINDEPTH, LOD, x + 0.5f, y + 0.5f, z + 0.5f, oinv, idims, l);
+        val = sample_scaled<BM, FM>(dens, atlas, mt, cfg, q, l, step);
+      }
+      row[w * kShadowSeg + i] = val;
+    }
+    __syncthreads();
+    float acc = carry;
+    float mine[kShadowSeg];
+    for (int sg = 0; sg < kShadowWaves; ++sg) {
+#pragma unroll
+      for (int i = 0; i < kShadowSeg; i += 4) {
+        float4 q = *reinterpret_cast<const float4*>(row + sg * kShadowSeg + i);
+        const float e[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          acc += e[u];
+          mine[i + u] = sg == w ? acc : mine[i + u];
+        }
+      }
+    }
+    carry = acc;
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < kShadowSeg; ++i) {
+        int z = r0 + w * kShadowSeg + i;
+        if (z >= dout.z) break;
+        float dz = mine[i];
+        if (saved) saved[(size_t)z * plane + p] = dz;
+        if (z + 1 < dout.z) trans[(size_t)(z + 1) * plane + p] = shadow_transmit<BL>(dz) * mk;
+        if (z == 0) trans[p] = shadow_transmit<BL>(BM == B_CLAMP ? dz : 0.f) * mk;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Reverse of k_shadow_fwd without the resample: u_z = dL/dD_z from the
+// transmittance gradient of the cell it shades (and, under CLAMP, of the pad
+// cell 0), and the running-sum adjoint g_s[z] = sum_{k >= z} u_k accumulated
+// from the back, the order of the blend gradient (reduce_blend.cu.cc:34-79),
+// so the result matches the op sequence's bit for bit given the same u.
+// Same block shape as the forward, rounds walked from the last: waves store
+// the u of their cells in LDS and every wave runs the round's serial sum.
+template <int BL> __device__ __forceinline__ float shadow_transmit_grad(float gm, float d) {
+  if constexpr (BL == BL_BEER_LAMBERT) return -(gm * expf(-d));
+  else return (0.f <= d && d <= 1.f) ? -gm : 0.f;
+}
+
+template <int BL, bool CLAMP_PAD>
+__global__ __launch_bounds__(kShadowWaves * 64) void k_shadow_bwd(const float* __restrict__ saved,
+                                                                  const float* __restrict__ gt,
+                                                                  const float* __restrict__ mask, Dims3 dout,
+                                                                  float* __restrict__ gs) {
+  __shared__ float s_lds[64 * kShadowPitch];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const size_t plane = (size_t)dout.x * dout.y;
+  const size_t p = (size_t)blockIdx.x * 64 + lane;
+  const bool live = p < plane;
+  const float mk = (mask && live) ? mask[p] : 1.f;
+  float* row = s_lds + lane * kShadowPitch;
+  float carry = 0.f;
+  const int rounds = (dout.z + kShadowRound - 1) / kShadowRound;
+  for (int r = rounds - 1; r >= 0; --r) {
+    const int r0 = r * kShadowRound;
+    float uv[kShadowSeg];
+#pragma unroll
+    for (int i = 0; i < kShadowSeg; ++i) {
+      int z = r0 + w * kShadowSeg + i;
+      float u = 0.f;
+      if (live && z < dout.z) {
+        float dz = saved[(size_t)z * plane + p];
+        if (z + 1 < dout.z) u = shadow_transmit_grad<BL>(gt[(size_t)(z + 1) * plane + p] * mk, dz);
+        if (CLAMP_PAD && z == 0) u = u + shadow_transmit_grad<BL>(gt[p] * mk, dz);
+      }
+      uv[i] = u;
+    }
+#pragma unroll
+    for (int i = 0; i < kShadowSeg; i += 4)
+      *reinterpret_cast<float4*>(row + w * kShadowSeg + i) = make_float4(uv[i], uv[i + 1], uv[i + 2], uv[i + 3]);
+    __syncthreads();
+    float acc = carry;
+    float mine[kShadowSeg];
+    for (int sg = kShadowWaves - 1; sg >= 0; --sg) {
+#pragma unroll
+      for (int i = kShadowSeg - 4; i >= 0; i -= 4) {
+        float4 q = *reinterpret_cast<const float4*>(row + sg * kShadowSeg + i);
+        const float e[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int u = 3; u >= 0; --u) {
+          acc += e[u];
+          mine[i + u] = sg == w ? acc : mine[i + u];
+        }
+      }
+    }
+    carry = acc;
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < kShadowSeg; ++i) {
+        int z = r0 + w * kShadowSeg + i;
+        if (z < dout.z) gs[(size_t)z * plane + p] = mine[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // --------------------------------------------------------------- launchers
 static inline dim3 grid_for(size_t n, unsigned views) { return dim3((unsigned)((n + 255) / 256), views); }
 
@@ -605,10 +822,10 @@ static inline dim3 grid_for(size_t n, unsigned views) { return dim3((unsigned)((
   }
 
 hipError_t mip_down_launch(int C, hipStream_t st, const float* src, size_t src_stride, Dims3 s, float* dst,
-                           size_t dst_stride, Dims3 o, int batches) {
+                           size_t dst_stride, Dims3 o, int batches, float scale) {
   size_t n = (size_t)o.x * o.y * o.z;
   GFT_DISPATCH_C(C, hipLaunchKernelGGL(k_mip_down<CC>, grid_for(n, batches), dim3(256), 0, st, src, src_stride, s,
-                                       dst, dst_stride, o));
+                                       dst, dst_stride, o, scale));
   return hipGetLastError();
 }
 
@@ -685,6 +902,57 @@ hipError_t sample_lut_bwd_launch(int C, int bm, bool exact, hipStream_t st, cons
 hipError_t lod_transform_launch(hipStream_t st, Dims3 din, Dims3 dout, const SampleLaunch& L, float* out) {
   size_t n = (size_t)dout.x * dout.y * dout.z;
   hipLaunchKernelGGL(k_lod_transform, grid_for(n, L.num_views), dim3(256), 0, st, din, dout, L, out);
+  return hipGetLastError();
+}
+
+hipError_t shadow_fwd_launch(int bm, int fm, int bl, hipStream_t st, const float* dens, const float* atlas,
+                             const MipTable& mt, const SampleCfg& cfg, const SampleView& v, Dims3 dout, float step,
+                             const float* mask, float* saved, float* trans) {
+  KernelTimer t(st, "shadow_fwd");
+  size_t plane = (size_t)dout.x * dout.y;
+  dim3 grid((unsigned)((plane + 63) / 64)), block(kShadowWaves * 64);
+  const bool lod = cfg.mm != MIP_NONE;
+#define GFT_SHADOW_FWD(B, F, L)                                                                                    \
+  do {                                                                                                             \
+    if (lod)                                                                                                       \
+      hipLaunchKernelGGL((k_shadow_fwd<B, F, L, true>), grid, block, 0, st, dens, atlas, mt, cfg, v, dout, step,   \
+                         mask, saved, trans);                                                                      \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_shadow_fwd<B, F, L, false>), grid, block, 0, st, dens, atlas, mt, cfg, v, dout, step,  \
+                         mask, saved, trans);                                                                      \
+  } while (0)
+#define GFT_SHADOW_FWD_BL(B, F)              \
+  do {                                       \
+    if (bl == BL_ALPHA)                      \
+      GFT_SHADOW_FWD(B, F, BL_ALPHA);        \
+    else                                     \
+      GFT_SHADOW_FWD(B, F, BL_BEER_LAMBERT); \
+  } while (0)
+  if (bm == B_CLAMP) {
+    if (fm == F_NEAREST) GFT_SHADOW_FWD_BL(B_CLAMP, F_NEAREST);
+    else GFT_SHADOW_FWD_BL(B_CLAMP, F_LINEAR);
+  } else {
+    if (fm == F_NEAREST) GFT_SHADOW_FWD_BL(B_BORDER, F_NEAREST);
+    else GFT_SHADOW_FWD_BL(B_BORDER, F_LINEAR);
+  }
+#undef GFT_SHADOW_FWD_BL
+#undef GFT_SHADOW_FWD
+  return hipGetLastError();
+}
+
+hipError_t shadow_bwd_launch(int bm, int bl, hipStream_t st, const float* saved, const float* gt, const float* mask,
+                             Dims3 dout, float* gs) {
+  KernelTimer t(st, "shadow_bwd");
+  size_t plane = (size_t)dout.x * dout.y;
+  dim3 grid((unsigned)((plane + 63) / 64)), block(kShadowWaves * 64);
+  if (bl == BL_ALPHA) {
+    if (bm == B_CLAMP) hipLaunchKernelGGL((k_shadow_bwd<BL_ALPHA, true>), grid, block, 0, st, saved, gt, mask, dout, gs);
+    else hipLaunchKernelGGL((k_shadow_bwd<BL_ALPHA, false>), grid, block, 0, st, saved, gt, mask, dout, gs);
+  } else {
+    if (bm == B_CLAMP)
+      hipLaunchKernelGGL((k_shadow_bwd<BL_BEER_LAMBERT, true>), grid, block, 0, st, saved, gt, mask, dout, gs);
+    else hipLaunchKernelGGL((k_shadow_bwd<BL_BEER_LAMBERT, false>), grid, block, 0, st, saved, gt, mask, dout, gs);
+  }
   return hipGetLastError();
 }
 

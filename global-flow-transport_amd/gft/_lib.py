@@ -82,6 +82,11 @@ _SIGNATURES = {
     "gft_lod_transform": (ctypes.c_int, [_vp, _i32p, _vp, _vp, _vp, _i32, _i32p, _i32, _vp]),
     "gft_reduce_grid_blend_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _i32, _i32, _vp]),
     "gft_reduce_grid_blend_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _i32, _i32, _i32, _vp]),
+    "gft_shadow_transmission_saved_bytes": (_sz, [_i32p]),
+    "gft_shadow_transmission_fwd_workspace": (_sz, [_i64p, _i32, _i32]),
+    "gft_shadow_transmission_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _vp, _vp, _i32p, _f32, _i32, _i32, _i32,
+                                                   _i32, _f32, _i32, _vp, _vp, _sz, _vp, _vp]),
+    "gft_shadow_transmission_bwd": (ctypes.c_int, [_vp, _vp, _i32p, _i32, _i32, _vp, _vp, _vp]),
     "gft_timer_start": (ctypes.c_int, [ctypes.c_char_p]),
     "gft_timer_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
     "gft_timer_names": (ctypes.c_int, [ctypes.c_char_p, _sz]),

@@ -7,7 +7,9 @@ falloff 1 / (1 + (|p - l| s)^2) in world-scaled grid coordinates
 the density resampled into the light's frustum, blended along depth with
 every prefix kept, shifted by one cell, exponentiated (BEER_LAMBERT) or
 clipped (ALPHA), cone-masked and resampled back to the grid
-(lighting.py:77-147). All grid work runs on the HIP ops through the renderer.
+(lighting.py:77-147). All grid work runs on the HIP ops through the renderer;
+the frustum part runs as one fused shadow op where the renderer's settings
+allow (the op sequence otherwise).
 """
 import numpy as np
 import torch
@@ -77,32 +79,48 @@ class SpotLight(PointLight):
         gy, gx = torch.meshgrid(a, b, indexing="ij")
         return (torch.sqrt(gy * gy + gx * gx) <= 1).to(torch.float32)[None, None, :, :, None]
 
-    def render_shadow_map(self, density_transform, renderer):
+    def _shadow_ops(self, density_transform, renderer):
+        """The op sequence of lighting.py:108-135: frustum resample, depth
+        prefix blend, one-cell shift, transmittance."""
+        shadow = renderer.sample_camera(density_transform.data, density_transform, self.shadow_cam, inverse=False,
+                                        use_step_channel=[0])
+        shadow = renderer.blending.reduce_grid_blend(shadow, renderer.blend_mode, keep_dims=True,
+                                                     grad_mode=renderer.grad_mode)
+        if renderer.boundary_mode == "BORDER":
+            pad = torch.zeros_like(shadow[:, :1])
+        elif renderer.boundary_mode == "CLAMP":
+            pad = shadow[:, :1]
+        elif renderer.boundary_mode == "WRAP":
+            pad = shadow[:, -1:]
+        else:
+            raise ValueError("Unknow boundary_mode %s" % renderer.boundary_mode)
+        shadow = torch.cat([pad, shadow[:, :-1]], dim=1)
+        if renderer.blend_mode == "BEER_LAMBERT":
+            return torch.exp(-shadow)
+        if renderer.blend_mode == "ALPHA":
+            return 1 - torch.clamp(shadow, 0, 1)
+        raise ValueError("Unknown blend_mode '%s'" % renderer.blend_mode)
+
+    def render_shadow_map(self, density_transform, renderer, fused=None):
+        """Shadow transmittance on the density grid (lighting.py:106-141).
+        With shadows cast, the fused shadow op computes the light-frustum
+        transmittance in one pass where the renderer's settings allow it
+        (``fused`` None: the renderer decides; False: always the op
+        sequence); the reverse resample to the grid follows either way."""
         dev = density_transform.data.device
+        mask = self._shadow_mask(dev) if self.cone_mask else None
         if self.cast_shadows:
-            shadow = renderer.sample_camera(density_transform.data, density_transform, self.shadow_cam, inverse=False,
-                                            use_step_channel=[0])
-            shadow = renderer.blending.reduce_grid_blend(shadow, renderer.blend_mode, keep_dims=True,
-                                                         grad_mode=renderer.grad_mode)
-            if renderer.boundary_mode == "BORDER":
-                pad = torch.zeros_like(shadow[:, :1])
-            elif renderer.boundary_mode == "CLAMP":
-                pad = shadow[:, :1]
-            elif renderer.boundary_mode == "WRAP":
-                pad = shadow[:, -1:]
-            else:
-                raise ValueError("Unknow boundary_mode %s" % renderer.boundary_mode)
-            shadow = torch.cat([pad, shadow[:, :-1]], dim=1)
-            if renderer.blend_mode == "BEER_LAMBERT":
-                transmission = torch.exp(-shadow)
-            elif renderer.blend_mode == "ALPHA":
-                transmission = 1 - torch.clamp(shadow, 0, 1)
-            else:
-                raise ValueError("Unknown blend_mode '%s'" % renderer.blend_mode)
+            transmission = None
+            if fused is not False:
+                transmission = renderer.shadow_transmission(density_transform, self.shadow_cam, mask)
+            if transmission is None:
+                transmission = self._shadow_ops(density_transform, renderer)
+                if mask is not None:
+                    transmission = transmission * mask
         else:
             transmission = torch.ones([1] + self.shadow_resolution + [1], dtype=torch.float32, device=dev)
-        if self.cone_mask:
-            transmission = transmission * self._shadow_mask(dev)
+            if mask is not None:
+                transmission = transmission * mask
         return renderer.sample_camera(transmission.contiguous(), density_transform, self.shadow_cam, inverse=True,
                                       use_step_channel=None)
 

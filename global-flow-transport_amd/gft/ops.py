@@ -578,6 +578,105 @@ class _BlendingOps(object):
 
 
 blending_ops = _BlendingOps()
+
+
+# ------------------------------------------------------------------- lighting
+def _shadow_fwd_impl(density, cams, shadow_dhw, step, fm, bm, mm, num_mipmaps, mip_bias, bl, mask, save):
+    M, nM, V, P, F, nCam = cams
+    if nM != 1 or nCam != 1:
+        raise _lib.InvalidArgumentError("the shadow map takes one model matrix and one light camera")
+    L = _lib.lib()
+    shape = _lib.i64_array(density.shape)
+    wsb = L.gft_shadow_transmission_fwd_workspace(shape, mm, int(num_mipmaps))
+    ws = _workspace(wsb, density.device)
+    out = torch.empty((1,) + tuple(shadow_dhw) + (1,), device=density.device, dtype=torch.float32)
+    saved = torch.empty(tuple(shadow_dhw), device=density.device, dtype=torch.float32) if save else None
+    _lib.check(L.gft_shadow_transmission_fwd(
+        _lib.stream_of(density), _lib.ptr(density), shape, _np_ptr(M), _np_ptr(V), _np_ptr(P), _np_ptr(F),
+        _lib.i32_array(shadow_dhw), float(step), fm, bm, mm, int(num_mipmaps), float(mip_bias), bl,
+        _lib.ptr(mask) if mask is not None else None, _lib.ptr(ws), wsb,
+        _lib.ptr(saved) if saved is not None else None, _lib.ptr(out)))
+    return out, saved
+
+
+def _shadow_bwd_impl(saved, grad_t, shadow_dhw, bm, bl, mask):
+    grad_t = grad_t.contiguous()
+    if tuple(grad_t.shape) != (1,) + tuple(shadow_dhw) + (1,):
+        raise _lib.InvalidArgumentError("transmission gradient shape %s does not match the shadow map %s"
+                                        % (list(grad_t.shape), list(shadow_dhw)))
+    gs = torch.empty((1, 1) + tuple(shadow_dhw) + (1,), device=saved.device, dtype=torch.float32)
+    L = _lib.lib()
+    _lib.check(L.gft_shadow_transmission_bwd(
+        _lib.stream_of(saved), _lib.ptr(saved), _lib.i32_array(shadow_dhw), bm, bl,
+        _lib.ptr(mask) if mask is not None else None, _lib.ptr(grad_t), _lib.ptr(gs)))
+    return gs
+
+
+class _ShadowFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, density, cams, shadow_dhw, attrs, mask):
+        step, fm, bm, mm, nmip, bias, bl, gm, gather_bias_add = attrs
+        out, saved = _shadow_fwd_impl(density, cams, shadow_dhw, step, fm, bm, mm, nmip, bias, bl, mask,
+                                      save=ctx.needs_input_grad[0])
+        if saved is not None:
+            ctx.save_for_backward(density, saved)
+        ctx.cams, ctx.shadow_dhw, ctx.attrs, ctx.mask = cams, shadow_dhw, attrs, mask
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        density, saved = ctx.saved_tensors
+        step, fm, bm, mm, nmip, bias, bl, gm, gather_bias_add = ctx.attrs
+        gs = _shadow_bwd_impl(saved, grad_out, ctx.shadow_dhw, bm, bl, ctx.mask)
+        cm = COORD["TRANSFORM_LINDEPTH"]
+        if gm == "GATHER":
+            g = _gather_grad_impl(gs, ctx.cams, tuple(density.shape[1:4]), fm, bm, mm, nmip, bias + gather_bias_add,
+                                  cm)
+        else:
+            g = _sample_transform_bwd_impl(density, gs, ctx.cams, fm, bm, mm, nmip, bias, cm, True, gm)
+        # the op sequence scales the grid by the step before sampling: d/d density = step * d/d(scaled grid)
+        g = g * torch.tensor([step], dtype=torch.float32, device=g.device)
+        return g, None, None, None, None
+
+
+class _LightingOps(object):
+    """The fused spot-light shadow transmittance (lighting.py:106-141 up to
+    the reverse resample) as one op."""
+
+    @staticmethod
+    def shadow_transmission(density, matrix_m, matrix_v, matrix_p, frustum_params, shadow_shape, depth_step,
+                            interpolation="LINEAR", boundary="BORDER", mipmapping="NONE", num_mipmaps=0,
+                            mip_bias=0.0, blending_mode="BEER_LAMBERT", cone_mask=None, grad_mode="REFERENCE",
+                            gradient_mip_bias_add=0.0):
+        """Transmittance [1,Ds,Hs,Ws,1] of a density [1,D,H,W,1] in the light
+        frustum: sample_grid_transform (TRANSFORM_LINDEPTH) of density *
+        depth_step, reduce_grid_blend keep_dims, the one-cell depth shift
+        padded by 0 (BORDER) or the first cell (CLAMP), exp(-D) for
+        BEER_LAMBERT or 1 - clip(D, 0, 1) for ALPHA, times cone_mask
+        ([Hs,Ws] or any shape with Hs*Ws elements; None for no mask).
+        grad_mode selects the sampling gradient as in sample_grid_transform
+        ("REFERENCE", "EXACT" or "GATHER"); mipmapping, num_mipmaps and
+        mip_bias as there (the REFERENCE gradient takes no mips)."""
+        density = _grid(density, "density")
+        fm, bm, cm, mm = _check_transform_attrs(interpolation, boundary, "TRANSFORM_LINDEPTH", 0.5, mipmapping,
+                                                num_mipmaps)
+        bl = _enum(BLEND, blending_mode, "blending mode")
+        if bl not in (BLEND["BEER_LAMBERT"], BLEND["ALPHA"]):
+            raise _lib.InvalidArgumentError("Unknown blend_mode '%s'" % blending_mode)
+        dhw = _shape3(shadow_shape)
+        if cone_mask is not None:
+            cone_mask = cone_mask.detach().to(device=density.device, dtype=torch.float32).contiguous()
+            if cone_mask.numel() != dhw[1] * dhw[2]:
+                raise _lib.InvalidArgumentError("cone mask of %d values for a %dx%d shadow map"
+                                                % (cone_mask.numel(), dhw[1], dhw[2]))
+        gm = "GATHER" if str(grad_mode).upper() == "GATHER" else _enum(GRAD, grad_mode, "grad_mode")
+        attrs = (float(depth_step), fm, bm, mm, int(num_mipmaps), float(mip_bias), bl, gm,
+                 float(gradient_mip_bias_add))
+        return _ShadowFn.apply(density, _cams(matrix_m, matrix_v, matrix_p, frustum_params), tuple(dhw), attrs,
+                               cone_mask)
+
+
+lighting_ops = _LightingOps()
 sample_grid_transform = sampling_ops.sample_grid_transform
 sample_grid_lut = sampling_ops.sample_grid_lut
 reduce_grid_blend = blending_ops.reduce_grid_blend

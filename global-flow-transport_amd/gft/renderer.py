@@ -23,7 +23,7 @@ import torch
 from . import _lib
 from .camera import camera_params_batch
 from .lighting import Light
-from .ops import blending_ops, raymarching_ops, sampling_ops
+from .ops import blending_ops, lighting_ops, raymarching_ops, sampling_ops
 from .pipeline import render_density as fused_render_density
 
 
@@ -47,6 +47,7 @@ class Renderer(object):
         self.grad_mode = grad_mode
         self.luma = kwargs.get("luma", [0.2126, 0.7152, 0.0722])
         self.allow_fused = kwargs.get("fused", False)
+        self.fused_shadow = kwargs.get("fused_shadow", True)
         self.scattering_ratio = kwargs.get("scattering_ratio", 1.0)
 
     @property
@@ -156,6 +157,30 @@ class Renderer(object):
                                                        boundary=kwargs.get("boundary_mode", self.boundary_mode),
                                                        blending_mode=kwargs.get("blend_mode", self.blend_mode),
                                                        grad_mode=self.grad_mode)
+
+    def shadow_transmission(self, density_transform, camera, cone_mask=None):
+        """Light-frustum transmittance of a spot light's shadow map
+        (lighting.py:108-135 and the cone mask) as one fused op, or None
+        where the renderer's settings fall outside it (fused_shadow off,
+        batch or channels other than 1, MIN/MAX filters, WRAP, blend modes
+        other than BEER_LAMBERT and ALPHA)."""
+        data = density_transform.data
+        if not self.fused_shadow or data.dim() != 5 or data.shape[0] != 1 or data.shape[-1] != 1:
+            return None
+        if str(self.filter_mode).upper() not in ("LINEAR", "NEAREST"):
+            return None
+        if str(self.boundary_mode).upper() not in ("BORDER", "CLAMP"):
+            return None
+        if str(self.blend_mode).upper() not in ("BEER_LAMBERT", "ALPHA"):
+            return None
+        M = np.asarray(density_transform.get_transform_matrix(), dtype=np.float32).T[None]
+        V, P, F = self._get_camera_params_batch([camera])
+        return lighting_ops.shadow_transmission(data, M, V, P, F, camera.transform.grid_size, camera.depth_step,
+                                                interpolation=self.filter_mode, boundary=self.boundary_mode,
+                                                mipmapping=self.mip_mode, num_mipmaps=self.num_mips,
+                                                mip_bias=self.mip_bias, blending_mode=self.blend_mode,
+                                                cone_mask=cone_mask, grad_mode=self._sample_grad_mode(),
+                                                gradient_mip_bias_add=self.gradient_mip_bias_add)
 
     def raymarch_camera(self, data, transformations, cameras, use_step_channel=None, **raymarch_kwargs):
         transformations, no_batch = self._check_batch(data, transformations)

@@ -277,6 +277,45 @@ int gft_reduce_grid_blend_bwd(void* stream, const float* input, const int64_t sh
                               float* input_grad);
 
 /*
+ * Fused spot-light shadow transmittance. Replaces the op sequence of
+ * SpotLight._render_shadow_map up to the reverse resample (phitest/render/
+ * lighting.py:106-141): sample_camera of the density into the light frustum
+ * with the depth step on channel 0 (renderer.py:365-385, SampleGridTransform
+ * TRANSFORM_LINDEPTH, resample_grid.cc:268-447), reduce_grid_blend
+ * keep_dims (a running sum on one channel, reduce_blend.cc:60-124), the
+ * one-cell depth shift padded by 0 (BORDER) or the first cell (CLAMP), then
+ * exp(-D) (BEER_LAMBERT) or 1 - clip(D, 0, 1) (ALPHA), times the cone mask.
+ *   density       [1,D,H,W,1] device; M, V, P [4,4] host; frustum [6] host
+ *   shadow_shape  [Ds,Hs,Ws]; transmission [Ds,Hs,Ws] device
+ *   mask          [Hs,Ws] device, or NULL for no cone mask
+ *   workspace     gft_shadow_transmission_fwd_workspace() bytes (the mip
+ *                 pyramid of density * depth_step; 0 without mips)
+ *   saved         NULL, or gft_shadow_transmission_saved_bytes() of device
+ *                 memory that receives the prefix sums D for the backward
+ * interpolation NEAREST or LINEAR and boundary BORDER or CLAMP (the sampling
+ * op's own limits); mipmapping, num_mipmaps and mip_bias as in
+ * gft_sample_grid_transform_fwd.
+ *
+ * The backward is the reverse of the shift, transmittance and running sum:
+ * frustum_grad [Ds,Hs,Ws] receives dL/d(step-scaled frustum sample), i.e. the
+ * gradient the op sequence hands to SampleGridTransform's gradient; the
+ * caller passes it, as a [1,1,Ds,Hs,Ws,1] output gradient, to
+ * gft_sample_grid_transform_bwd or gft_sample_grid_transform_gather_grad
+ * and multiplies the result by depth_step.
+ */
+size_t gft_shadow_transmission_saved_bytes(const int32_t shadow_shape[3]);
+size_t gft_shadow_transmission_fwd_workspace(const int64_t density_shape[5], int32_t mipmapping,
+                                             int32_t num_mipmaps);
+int gft_shadow_transmission_fwd(void* stream, const float* density, const int64_t density_shape[5], const float* M,
+                                const float* V, const float* P, const float* frustum, const int32_t shadow_shape[3],
+                                float depth_step, int32_t interpolation, int32_t boundary, int32_t mipmapping,
+                                int32_t num_mipmaps, float mip_bias, int32_t blending_mode, const float* mask,
+                                void* workspace, size_t workspace_bytes, float* saved, float* transmission);
+int gft_shadow_transmission_bwd(void* stream, const float* saved, const int32_t shadow_shape[3], int32_t boundary,
+                                int32_t blending_mode, const float* mask, const float* transmission_grad,
+                                float* frustum_grad);
+
+/*
  * Kernel timers (instrumentation; no counterpart in the reference, which
  * has none). gft_timer_start("a,b" or "*") makes every later launch of the
  * named kernels on the calling thread record a HIP event before and after
@@ -284,7 +323,8 @@ int gft_reduce_grid_blend_bwd(void* stream, const float* input, const int64_t sh
  * the launch count and summed duration of one name; gft_timer_names lists
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
- * render_fwd, render_bwd_pixel, render_bwd_gather, render_bwd_scatter, quad_copy, fill.
+ * render_fwd, render_bwd_pixel, render_bwd_gather, render_bwd_scatter, quad_copy, fill,
+ * shadow_fwd, shadow_bwd.
  */
 int gft_timer_start(const char* kernels);
 int gft_timer_read(const char* kernel, int32_t* launches, double* total_ms);

@@ -72,6 +72,12 @@ def test_workspace_queries(lib):
             want = ((views * 7 * 30 * 4 * 4 + 255) // 256) * 256 + views * 30 * 8
             assert lib.gft_raymarch_grid_transform_bwd_workspace(shape, cams, out, BORDER, sep, gm) == want
     assert lib.gft_advect_saved_bytes(_i64(1, 4, 5, 6, 1)) > 0
+    # shadow map: prefix sums of the light frustum; the pyramid of the step-scaled density
+    assert lib.gft_shadow_transmission_saved_bytes(_i32(70, 13, 11)) == 70 * 13 * 11 * 4
+    assert lib.gft_shadow_transmission_saved_bytes(_i32(0, 13, 11)) == 0
+    dshape = _i64(1, 13, 18, 22, 1)
+    assert lib.gft_shadow_transmission_fwd_workspace(dshape, 0, 4) == 0
+    assert lib.gft_shadow_transmission_fwd_workspace(dshape, 2, 3) == lib.gft_mip_atlas_bytes(dshape, 3)
 
 
 def _eye(n=1):
@@ -128,6 +134,19 @@ def test_validation_returns_before_device_work(lib):
                                              ctypes.c_void_p(16), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), 1, 0, 0, 0, 1,
                                              0, null, 0, ctypes.c_void_p(16))
     assert rc == _lib.GFT_ERR_WORKSPACE
+    # shadow map: one-channel, batch-1 density; BORDER/CLAMP; BEER_LAMBERT/ALPHA; workspace for mips
+    def shadow(shape, boundary=0, blend=0, mm=0, nm=0, ws=0, out=(8, 8, 8)):
+        return lib.gft_shadow_transmission_fwd(null, ctypes.c_void_p(16), _i64(*shape), vp, vp, vp, fp, _i32(*out),
+                                               ctypes.c_float(0.5), 1, boundary, mm, nm, ctypes.c_float(0.0), blend,
+                                               null, null, ws, null, ctypes.c_void_p(16))
+    assert shadow((1, 4, 4, 4, 2)) == _lib.GFT_ERR_INVALID_ARGUMENT
+    assert shadow((2, 4, 4, 4, 1)) == _lib.GFT_ERR_INVALID_ARGUMENT
+    assert shadow((1, 4, 4, 4, 1), boundary=2) == _lib.GFT_ERR_INVALID_ARGUMENT
+    assert shadow((1, 4, 4, 4, 1), blend=3) == _lib.GFT_ERR_INVALID_ARGUMENT and b"blend" in lib.gft_last_error()
+    assert shadow((1, 4, 4, 4, 1), out=(8, 0, 8)) == _lib.GFT_ERR_INVALID_ARGUMENT
+    assert shadow((1, 8, 8, 8, 1), mm=2, nm=2) == _lib.GFT_ERR_WORKSPACE
+    rc = lib.gft_shadow_transmission_bwd(null, null, _i32(8, 8, 8), 0, 0, null, null, null)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT
     # fused advection: scalar grids only
     rc = lib.gft_advect_fwd(null, null, _i64(1, 4, 4, 4, 2), null, ctypes.c_float(1.0), 2, 1, null, null, 0)
     assert rc == _lib.GFT_ERR_UNIMPLEMENTED
