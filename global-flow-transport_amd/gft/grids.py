@@ -86,11 +86,23 @@ def _fused_warp(data, vel, order, dt, clamp):
     return fused_advect(data.contiguous(), vel.contiguous(), dt, order, clamp)
 
 
+def _scale_renderer(renderer, boundary):
+    """The renderer that rescales grids at grow steps; by default the
+    reference's scale renderer without mips (LINEAR filter, the velocity
+    boundary, reconstruct_sequence.py:528-535)."""
+    if renderer is not None:
+        return renderer
+    from .renderer import Renderer
+    return Renderer(filter_mode="LINEAR", boundary_mode=str(boundary).upper(), mipmapping="NONE",
+                    blend_mode="ADDITIVE")
+
+
 class DensityGrid(object):
     def __init__(self, shape, constant=0.1, d=None, inflow=None, inflow_offset=None, inflow_mask=None, device="cuda",
-                 grad_mode="EXACT", hull=None):
+                 grad_mode="EXACT", hull=None, scale_renderer=None):
         self.shape = [int(s) for s in shape]
         self.hull = hull
+        self.scale_renderer = scale_renderer
         self.inflow_offset = None if inflow_offset is None else [int(o) for o in inflow_offset]
         if d is None:
             d = torch.full([1] + self.shape + [1], float(constant), dtype=torch.float32, device=device)
@@ -132,7 +144,27 @@ class DensityGrid(object):
         return vel_grid.warp(self.with_inflow(), order=order, dt=dt, clamp=clamp)
 
     def copy_warped(self, vel_grid, order=1, dt=1.0, clamp="NONE"):
-        return DensityGrid(self.shape, d=self.warped(vel_grid, order, dt, clamp), grad_mode=self.grad_mode)
+        return DensityGrid(self.shape, d=self.warped(vel_grid, order, dt, clamp), grad_mode=self.grad_mode,
+                           scale_renderer=self.scale_renderer)
+
+    # ------------------------------------------------------------ grid growth
+    def scaled(self, new_shape, with_inflow=False):
+        """The density resampled to new_shape with border alignment
+        (data_structures.py:236-247); the same shape returns a copy."""
+        if not (isinstance(new_shape, (list, tuple)) and len(new_shape) == 3):
+            raise ValueError("Invalid shape")
+        new_shape = [int(s) for s in new_shape]
+        density = self.d if not with_inflow else self.with_inflow()
+        if new_shape != self.shape:
+            return _scale_renderer(self.scale_renderer, "CLAMP").resample_grid3D_aligned(density, new_shape)
+        return density.clone()
+
+    def copy_scaled(self, new_shape):
+        """A new grid of new_shape holding scaled(); inflow and hull are not
+        carried over (data_structures.py:249-261)."""
+        d = self.scaled(new_shape)
+        return DensityGrid(new_shape, d=d, device=d.device, grad_mode=self.grad_mode,
+                           scale_renderer=self.scale_renderer)
 
     # ------------------------------------------------------------ state I/O
     def save(self, path):
@@ -186,8 +218,9 @@ class VelocityGrid(object):
         return x_shape, y_shape, z_shape
 
     def __init__(self, centered_shape, std=0.1, x=None, y=None, z=None, boundary="CLAMP", device="cuda",
-                 grad_mode="EXACT", seed=None):
+                 grad_mode="EXACT", seed=None, scale_renderer=None):
         self.centered_shape = [int(s) for s in centered_shape]
+        self.scale_renderer = scale_renderer
         self.x_shape, self.y_shape, self.z_shape = VelocityGrid.component_shapes(self.centered_shape)
         self.boundary = boundary
         self.grad_mode = grad_mode
@@ -296,7 +329,33 @@ class VelocityGrid(object):
     def copy_warped(self, vel_grid=None, order=1, dt=1.0, clamp="NONE"):
         x, y, z = self.warped(vel_grid, order, dt, clamp)
         return VelocityGrid(self.centered_shape, x=x, y=y, z=z, boundary=self.boundary, device=self._x.device,
-                            grad_mode=self.grad_mode)
+                            grad_mode=self.grad_mode, scale_renderer=self.scale_renderer)
+
+    # ------------------------------------------------------------ grid growth
+    def scaled(self, centered_shape, scale_magnitude=True):
+        """Components resampled to the staggered shapes of centered_shape,
+        each aligned at the outer cell centres along its own axis and at the
+        borders along the others, and with scale_magnitude multiplied by
+        new/old size per axis (data_structures.py:551-573)."""
+        if not (isinstance(centered_shape, (list, tuple)) and len(centered_shape) == 3):
+            raise ValueError("Invalid shape")
+        centered_shape = [int(s) for s in centered_shape]
+        if centered_shape == self.centered_shape:
+            return self.x.clone(), self.y.clone(), self.z.clone()
+        r = _scale_renderer(self.scale_renderer, self.boundary)
+        x_shape, y_shape, z_shape = VelocityGrid.component_shapes(centered_shape)
+        x = r.resample_grid3D_aligned(self.x, x_shape, align_x='center')
+        y = r.resample_grid3D_aligned(self.y, y_shape, align_y='center')
+        z = r.resample_grid3D_aligned(self.z, z_shape, align_z='center')
+        if scale_magnitude:
+            s = [o / i for i, o in zip(self.centered_shape, centered_shape)]  # z, y, x
+            x, y, z = x * s[2], y * s[1], z * s[0]
+        return x, y, z
+
+    def copy_scaled(self, centered_shape, scale_magnitude=True):
+        x, y, z = self.scaled(centered_shape, scale_magnitude)
+        return VelocityGrid(centered_shape, x=x, y=y, z=z, boundary=self.boundary, device=x.device,
+                            grad_mode=self.grad_mode, scale_renderer=self.scale_renderer)
 
     def divergence(self, world_scale=(1, 1, 1)):
         x_div = self.x[:, :, :, 1:, :] - self.x[:, :, :, :-1, :]
@@ -351,6 +410,17 @@ class State(object):
 
     def velocity_advected(self, dt=1.0, order=1, clamp="NONE"):
         return self.velocity.copy_warped(order=order, dt=dt, clamp=clamp)
+
+    def rescale_density(self, shape):
+        """Grow step (data_structures.py:1178-1179)."""
+        self.density = self.density.copy_scaled(shape)
+
+    def rescale_velocity(self, shape, scale_magnitude=True):
+        self.velocity = self.velocity.copy_scaled(shape, scale_magnitude=scale_magnitude)
+
+    def rescale(self, dens_shape, vel_shape):
+        self.rescale_density(dens_shape)
+        self.rescale_velocity(vel_shape)
 
     def save(self, path, suffix=None):
         """State.save (data_structures.py:1213-1215): density.npz and

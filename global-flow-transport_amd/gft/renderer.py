@@ -72,8 +72,19 @@ class Renderer(object):
     def _sample_transform(self, data, from_transformations, to_transformations, inverse=False, fix_scale_center=False):
         """Resample between grids (TRANSFORM / TRANSFORM_REVERSE, identity
         projection, frustum [-1,1,-1,1,1,-1]); renderer.py:188-227."""
-        from .transform import GridTransform
         cm = "TRANSFORM_REVERSE" if inverse else "TRANSFORM"
+        M, V, P, F = self._sample_transform_params(from_transformations, to_transformations, fix_scale_center)
+        out_shape = from_transformations[0].grid_size if inverse else to_transformations[0].grid_size
+        return sampling_ops.sample_grid_transform(data, M, V, P, F, out_shape, interpolation=self.filter_mode,
+                                                  boundary=self.boundary_mode, mipmapping=self.mip_mode,
+                                                  num_mipmaps=self.num_mips, mip_bias=self.mip_bias,
+                                                  coordinate_mode=cm, grad_mode=self._sample_grad_mode(),
+                                                  gradient_mip_bias_add=self.gradient_mip_bias_add)
+
+    @staticmethod
+    def _sample_transform_params(from_transformations, to_transformations, fix_scale_center=False):
+        """M, V, P, F of a grid-to-grid resample (renderer.py:191-198)."""
+        from .transform import GridTransform
         M = np.stack([np.asarray(t.get_transform_matrix(), dtype=np.float32).T for t in from_transformations])
         if fix_scale_center:
             V = [(GridTransform(t.grid_size, scale=[2, 2, 2], center=True, normalize='ALL').get_transform_matrix()
@@ -83,12 +94,7 @@ class Renderer(object):
         V = np.stack(V).astype(np.float32)
         P = np.stack([np.eye(4, dtype=np.float32) for _ in to_transformations])
         F = np.array([[-1, 1, -1, 1, 1, -1] for _ in to_transformations], dtype=np.float32)
-        out_shape = from_transformations[0].grid_size if inverse else to_transformations[0].grid_size
-        return sampling_ops.sample_grid_transform(data, M, V, P, F, out_shape, interpolation=self.filter_mode,
-                                                  boundary=self.boundary_mode, mipmapping=self.mip_mode,
-                                                  num_mipmaps=self.num_mips, mip_bias=self.mip_bias,
-                                                  coordinate_mode=cm, grad_mode=self._sample_grad_mode(),
-                                                  gradient_mip_bias_add=self.gradient_mip_bias_add)
+        return M, V, P, F
 
     def _sample_camera_transform(self, data, transformations, cameras, inverse=False):
         cm = "TRANSFORM_LINDEPTH_REVERSE" if inverse else "TRANSFORM_LINDEPTH"
@@ -201,6 +207,48 @@ class Renderer(object):
                                            separate_camera_batch=not combined_batch, relative_coords=relative,
                                            normalized_coords=normalized, grad_mode=self.grad_mode)
         return out[:, 0] if combined_batch else out
+
+    def resample_grid3D_aligned(self, data, target_shape, align_x='BORDER', align_y='BORDER', align_z='BORDER'):
+        """Resample NDHWC data to target_shape (D,H,W) with per-axis alignment
+        (renderer.py:653-692): BORDER aligns the outer cell borders, CENTER
+        the outer cell centres, STAGGER_INPUT input centres to output
+        borders, STAGGER_OUTPUT input borders to output centres. The input
+        is scaled into the centred [-1,1] cube that the TRANSFORM resample
+        maps the output grid onto; the sample position of output cell q is
+        then (q+0.5) in/out - 0.5 (BORDER), q (in-1)/(out-1) (CENTER),
+        (q+0.5) (in-1)/out (STAGGER_INPUT), (q+0.5) (in+1)/out - 1
+        (STAGGER_OUTPUT)."""
+        if data.dim() != 5:
+            raise ValueError("input shape must be NDHWC, is : %s" % list(data.shape))
+        if len(target_shape) != 3:
+            raise ValueError("target_shape must be (D,H,W)")
+        in_transform, target_transform = self.aligned_transforms(data.shape[1:4], target_shape, align_x, align_y,
+                                                                 align_z)
+        return self._sample_transform(data.contiguous(), [in_transform] * int(data.shape[0]),
+                                      [target_transform])[:, 0]
+
+    @staticmethod
+    def aligned_transforms(in_shape, target_shape, align_x='BORDER', align_y='BORDER', align_z='BORDER'):
+        """Input and target grid transforms of resample_grid3D_aligned."""
+        from .transform import GridTransform
+        in_dhw = [int(s) for s in in_shape]
+        out_dhw = [int(s) for s in target_shape]
+
+        def get_scale(align, n_in, n_out):
+            a = str(align).upper()
+            if a == 'BORDER':
+                return 2. / n_in
+            if a == 'CENTER':
+                return 2. / n_out * (n_out - 1.) / (n_in - 1.)
+            if a == 'STAGGER_INPUT':
+                return 2. / (n_in - 1.)
+            if a == 'STAGGER_OUTPUT':
+                return 2. / (n_in + 1.)
+            raise ValueError("unknown alignment {}".format(align))
+
+        scale = [get_scale(align_x, in_dhw[2], out_dhw[2]), get_scale(align_y, in_dhw[1], out_dhw[1]),
+                 get_scale(align_z, in_dhw[0], out_dhw[0])]
+        return GridTransform(in_dhw, scale=scale, center=True), GridTransform(out_dhw)
 
     def resample_grid3D_offset(self, data, offsets, target_shape):
         offsets = torch.as_tensor(np.asarray(offsets, dtype=np.float32), device=data.device)
