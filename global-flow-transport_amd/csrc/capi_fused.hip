@@ -235,10 +235,12 @@ size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_came
 static int render_bwd_common(void* stream, const float* density, const int64_t shape[5], const float* M,
                              int32_t num_M, const float* V, const float* P, int32_t num_cameras,
                              const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
-                             const float* images, const float* second, bool l2, float loss_scale, void* saved,
-                             size_t saved_bytes, float* density_grad, float* loss) {
+                             const float* images, const float* second, bool l2, float loss_scale,
+                             int32_t loss_kind, const float* backgrounds, void* saved, size_t saved_bytes,
+                             float* density_grad, float* loss) {
   int rc;
   if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
+  if (loss_kind != GFT_LOSS_SE && loss_kind != GFT_LOSS_AE) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid loss_kind");
   if (!density || !images || !second || !density_grad || (l2 && !loss) || !M || !V || !P || !light)
     return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
   const size_t need = gft_render_density_saved_bytes(shape, num_cameras, output_shape, boundary);
@@ -247,7 +249,7 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
                 need);
   hipStream_t s = (hipStream_t)stream;
   const RayDims rd = render_dims(shape, output_shape);
-  const RenderParams rp{{light[0], light[1], light[2]}, depth_step, loss_scale};
+  const RenderParams rp{{light[0], light[1], light[2]}, depth_step, loss_scale, loss_kind, backgrounds};
   auto pairs = view_pairs(1, num_cameras, true);
   if (boundary == GFT_BORDER) {
     const RenderSaved sv = saved_parts(saved, shape, output_shape, num_cameras);
@@ -294,7 +296,23 @@ int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t 
   (void)workspace;
   (void)workspace_bytes;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
-                           boundary, images, targets, true, loss_scale, saved, saved_bytes, density_grad, loss);
+                           boundary, images, targets, true, loss_scale, GFT_LOSS_SE, nullptr, saved, saved_bytes,
+                           density_grad, loss);
+}
+
+int gft_render_density_bwd_loss(void* stream, const float* density, const int64_t shape[5], const float* M,
+                                int32_t num_M, const float* V, const float* P, const float* frustum,
+                                int32_t num_cameras, const int32_t output_shape[3], const float light[3],
+                                float depth_step, int32_t boundary, const float* images, const float* targets,
+                                const float* backgrounds, int32_t loss_kind, float loss_scale, void* saved,
+                                size_t saved_bytes, void* workspace, size_t workspace_bytes, float* density_grad,
+                                float* loss) {
+  (void)frustum;
+  (void)workspace;
+  (void)workspace_bytes;
+  return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
+                           boundary, images, targets, true, loss_scale, loss_kind, backgrounds, saved, saved_bytes,
+                           density_grad, loss);
 }
 
 int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
@@ -306,7 +324,8 @@ int gft_render_density_bwd(void* stream, const float* density, const int64_t sha
   (void)workspace;
   (void)workspace_bytes;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
-                           boundary, images, images_grad, false, 0.f, saved, saved_bytes, density_grad, nullptr);
+                           boundary, images, images_grad, false, 0.f, GFT_LOSS_SE, nullptr, saved, saved_bytes,
+                           density_grad, nullptr);
 }
 
 }  // extern "C"

@@ -378,10 +378,13 @@ struct AdvectDims {
 };
 
 // Fused density render parameters.
+enum LossKind { LOSS_SE = 0, LOSS_AE = 1 };
 struct RenderParams {
   float light[3];  // light colour * intensity
   float step;      // camera depth step (renderer.py:420-437)
   float loss_scale;
+  int loss_kind;       // LossKind of the folded image loss
+  const float* bkg;    // [V,H,W,3] background composited as rgb + bkg * exp(-depth), or null
 };
 
 // Shape of one raymarch launch.

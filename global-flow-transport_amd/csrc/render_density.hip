@@ -209,7 +209,12 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
 }
 
 // Per-pixel loss gradient: G (light-weighted rgb gradient), g_D (optical depth
-// gradient) and the squared error for the loss.
+// gradient) and the pixel's loss term. The folded loss compares
+// e = rgb (+ bkg * exp(-D)) - target per channel: SE sums e^2 (gradient 2e),
+// AE sums |e| (gradient sign(e), 0 at 0), as loss_dens_target and
+// loss_dens_target_raw with the reference's SE / AE loss functions
+// (optimization.py:94-130,396-430); the composited background makes the
+// optical depth channel receive -exp(-D) sum_c bkg_c dL/de_c.
 template <bool L2>
 __device__ __forceinline__ void pixel_grad(const float* __restrict__ img, const float* __restrict__ second,
                                            const RayDims& rd, const RayView& v, size_t pix, const RenderParams& rp,
@@ -217,11 +222,33 @@ __device__ __forceinline__ void pixel_grad(const float* __restrict__ img, const 
   const float4 o = reinterpret_cast<const float4*>(img + rd.out_view_stride * v.out_slot)[pix];
   D = o.w;
   if (L2) {
-    const float* t = second + (size_t)v.out_slot * rd.out_w * rd.out_h * 3 + pix * 3;
-    const float e0 = o.x - t[0], e1 = o.y - t[1], e2 = o.z - t[2];
-    sq = e0 * e0 + e1 * e1 + e2 * e2;
-    G = 2.f * rp.loss_scale * (rp.light[0] * e0 + rp.light[1] * e1 + rp.light[2] * e2);
-    gD = 0.f;  // no loss term on the optical depth channel
+    const size_t at3 = ((size_t)v.out_slot * rd.out_w * rd.out_h + pix) * 3;
+    const float* t = second + at3;
+    float e0 = o.x - t[0], e1 = o.y - t[1], e2 = o.z - t[2];
+    float tr = 0.f, b0 = 0.f, b1 = 0.f, b2 = 0.f;
+    if (rp.bkg) {
+      tr = expf(-D);
+      b0 = rp.bkg[at3];
+      b1 = rp.bkg[at3 + 1];
+      b2 = rp.bkg[at3 + 2];
+      e0 = (o.x + b0 * tr) - t[0];
+      e1 = (o.y + b1 * tr) - t[1];
+      e2 = (o.z + b2 * tr) - t[2];
+    }
+    float f0, f1, f2;
+    if (rp.loss_kind == LOSS_AE) {
+      sq = fabsf(e0) + fabsf(e1) + fabsf(e2);
+      f0 = (float)(e0 > 0.f) - (float)(e0 < 0.f);
+      f1 = (float)(e1 > 0.f) - (float)(e1 < 0.f);
+      f2 = (float)(e2 > 0.f) - (float)(e2 < 0.f);
+    } else {
+      sq = e0 * e0 + e1 * e1 + e2 * e2;
+      f0 = 2.f * e0;
+      f1 = 2.f * e1;
+      f2 = 2.f * e2;
+    }
+    G = rp.loss_scale * (rp.light[0] * f0 + rp.light[1] * f1 + rp.light[2] * f2);
+    gD = rp.bkg ? -(rp.loss_scale * tr) * (b0 * f0 + b1 * f1 + b2 * f2) : 0.f;
   } else {
     const float4 go = reinterpret_cast<const float4*>(second + rd.out_view_stride * v.out_slot)[pix];
     G = rp.light[0] * go.x + rp.light[1] * go.y + rp.light[2] * go.z;

@@ -60,6 +60,9 @@ _SIGNATURES = {
     "gft_render_density_bwd_workspace": (_sz, [_i64p, _i32, _i32p, _i32]),
     "gft_render_density_bwd_l2": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
                                                  _i32, _vp, _vp, _f32, _vp, _sz, _vp, _sz, _vp, _vp]),
+    "gft_render_density_bwd_loss": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p,
+                                                   _f32, _i32, _vp, _vp, _vp, _i32, _f32, _vp, _sz, _vp, _sz, _vp,
+                                                   _vp]),
     "gft_render_density_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
                                               _i32, _vp, _vp, _vp, _sz, _vp, _sz, _vp]),
     "gft_mip_atlas_bytes": (_sz, [_i64p, _i32]),
@@ -87,6 +90,12 @@ _SIGNATURES = {
     "gft_shadow_transmission_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _vp, _vp, _vp, _i32p, _f32, _i32, _i32, _i32,
                                                    _i32, _f32, _i32, _vp, _vp, _sz, _vp, _vp]),
     "gft_shadow_transmission_bwd": (ctypes.c_int, [_vp, _vp, _i32p, _i32, _i32, _vp, _vp, _vp]),
+    "gft_divergence_loss_workspace": (_sz, [_i32p]),
+    "gft_divergence_loss": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32p, _f32p, _f32, _i32, _vp, _sz, _vp, _vp, _vp,
+                                           _vp]),
+    "gft_adam_step": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                     ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int64), _f32, _f32, _f32, _f32,
+                                     ctypes.c_int64]),
     "gft_timer_start": (ctypes.c_int, [ctypes.c_char_p]),
     "gft_timer_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
     "gft_timer_names": (ctypes.c_int, [ctypes.c_char_p, _sz]),

@@ -174,14 +174,26 @@ def render_density(d, matrix_m, matrix_v, matrix_p, frustum_params, output_shape
 
 
 # ------------------------------------------------------------- fused step
+LOSS = {"SE": 0, "AE": 1}
+
+
 class TransportRenderStep(object):
-    """advect (MacCormack + MC_SMOOTH) -> ambient render -> L2 loss, forward
-    and exact backward, on fixed buffers. ``run(d0, vel, target)`` leaves the
-    loss in ``self.loss`` and the gradients in ``self.grad_density`` /
-    ``self.grad_velocity``."""
+    """advect (MacCormack + MC_SMOOTH) -> ambient render -> image loss,
+    forward and exact backward, on fixed buffers. ``run(d0, vel, target)``
+    leaves the loss in ``self.loss`` and the gradients in
+    ``self.grad_density`` / ``self.grad_velocity``.
+
+    The image loss is folded into the render backward: ``loss`` "SE" (squared
+    error, the default) or "AE" (absolute error, the reference's default for
+    the target losses, optimization.py:114-116), summed and multiplied by
+    ``loss_scale`` (default 1 / (views * pixels * 3), the mean). With a
+    ``background`` ([1,V,H,W,3] tensor or an rgb triple) the loss compares
+    rgb + background * exp(-depth) with the targets (loss_dens_target_raw,
+    optimization.py:414-430)."""
 
     def __init__(self, grid_dhw, matrix_m, matrix_v, matrix_p, frustum_params, output_shape, depth_step,
-                 light=(1.0, 1.0, 1.0), order=2, clamp="MC_SMOOTH", dt=1.0, boundary="BORDER", device="cuda"):
+                 light=(1.0, 1.0, 1.0), order=2, clamp="MC_SMOOTH", dt=1.0, boundary="BORDER", device="cuda",
+                 loss="SE", background=None, loss_scale=None):
         D, H, W = grid_dhw
         self.shape = (1, D, H, W, 1)
         self.cams = _Cameras(matrix_m, matrix_v, matrix_p, frustum_params, output_shape)
@@ -196,7 +208,15 @@ class TransportRenderStep(object):
         nV = self.cams.nCam
         h, w = self.cams.out_dhw[1], self.cams.out_dhw[2]
         self.n_pixels = nV * h * w
-        self.loss_scale = 1.0 / (self.n_pixels * 3)
+        self.loss_scale = 1.0 / (self.n_pixels * 3) if loss_scale is None else float(loss_scale)
+        self.loss_kind = _enum(LOSS, loss, "loss")
+        self.background = None
+        if background is not None:
+            if isinstance(background, torch.Tensor):
+                bg = background.to(device=dev, dtype=torch.float32)
+            else:
+                bg = torch.tensor([float(c) for c in background], dtype=torch.float32, device=dev)
+            self.background = bg.expand(1, nV, h, w, 3).contiguous()
         self.d1 = torch.empty(self.shape, device=dev)
         self.images = torch.empty((1, nV, h, w, 4), device=dev)
         self.loss = torch.zeros((1,), device=dev)
@@ -239,11 +259,13 @@ class TransportRenderStep(object):
         consumes (overwrites) the per-sample record the forward saved."""
         L, c, s = self._L, self.cams, _lib.stream_of(d0)
         self._mark(timers, "render_bwd", True)
-        _lib.check(L.gft_render_density_bwd_l2(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
-                                               _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
-                                               self.depth_step, self.boundary, _lib.ptr(self.images),
-                                               _lib.ptr(target), self.loss_scale, _lib.ptr(self.render_saved),
-                                               self._nrsaved, None, 0, _lib.ptr(self.grad_d1), _lib.ptr(self.loss)))
+        bg = _lib.ptr(self.background) if self.background is not None else None
+        _lib.check(L.gft_render_density_bwd_loss(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
+                                                 _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
+                                                 self.depth_step, self.boundary, _lib.ptr(self.images),
+                                                 _lib.ptr(target), bg, self.loss_kind, self.loss_scale,
+                                                 _lib.ptr(self.render_saved), self._nrsaved, None, 0,
+                                                 _lib.ptr(self.grad_d1), _lib.ptr(self.loss)))
         self._mark(timers, "render_bwd", False)
         self._mark(timers, "advect_bwd", True)
         _lib.check(L.gft_advect_bwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,

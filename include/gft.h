@@ -46,6 +46,7 @@ enum {
   GFT_TRANSFORM_REVERSE = 3
 }; /* coordinate_mode */
 enum { GFT_GRAD_REFERENCE = 0, GFT_GRAD_EXACT = 1 }; /* gradient flavour */
+enum { GFT_LOSS_SE = 0, GFT_LOSS_AE = 1 };           /* folded image loss: squared / absolute error */
 
 const char* gft_last_error(void);
 int gft_abi_version(void);
@@ -148,6 +149,23 @@ int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t 
                               float depth_step, int32_t boundary, const float* images, const float* targets,
                               float loss_scale, void* saved, size_t saved_bytes, void* workspace,
                               size_t workspace_bytes, float* density_grad, float* loss);
+/*
+ * As gft_render_density_bwd_l2 with the reference's image loss choices:
+ * loss = loss_scale * sum_c f(rgb_c + bkg_c * exp(-depth) - target_c), f the
+ * squared (GFT_LOSS_SE) or absolute (GFT_LOSS_AE, gradient sign(e), 0 at 0)
+ * error. backgrounds [1,V,H,W,3] (device) or NULL: with a background the loss
+ * is loss_dens_target_raw (optimization.py:414-430), without it
+ * loss_dens_target (optimization.py:396-412); the reference's default loss
+ * function for both is AE (optimization.py:114-116) and its mean over views
+ * is loss_scale = scale / V.
+ */
+int gft_render_density_bwd_loss(void* stream, const float* density, const int64_t shape[5], const float* M,
+                                int32_t num_M, const float* V, const float* P, const float* frustum,
+                                int32_t num_cameras, const int32_t output_shape[3], const float light[3],
+                                float depth_step, int32_t boundary, const float* images, const float* targets,
+                                const float* backgrounds, int32_t loss_kind, float loss_scale, void* saved,
+                                size_t saved_bytes, void* workspace, size_t workspace_bytes, float* density_grad,
+                                float* loss);
 /* Backward of gft_render_density_fwd from an image gradient [1,V,H,W,4]. */
 int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
                            const float* V, const float* P, const float* frustum, int32_t num_cameras,
@@ -316,6 +334,36 @@ int gft_shadow_transmission_bwd(void* stream, const float* saved, const int32_t 
                                 float* frustum_grad);
 
 /*
+ * Divergence loss of a staggered velocity: replaces loss_vel_divergence
+ * (optimization.py:824-838) over VelocityGrid.divergence
+ * (data_structures.py:929-936) with its SE loss function:
+ *   div = wx dx(vel_x) + wy dy(vel_y) + wz dz(vel_z) per centred cell,
+ *   loss = loss_scale * sum div^2 (device float).
+ * x [D,H,W+1], y [D,H+1,W], z [D+1,H,W] (device, one channel) for
+ * centered_shape [D,H,W]; world_scale host [3] (x, y, z). The gradients
+ * (all three or none) are overwritten, or added to when accumulate != 0.
+ * Deterministic (no atomics).
+ */
+size_t gft_divergence_loss_workspace(const int32_t centered_shape[3]);
+int gft_divergence_loss(void* stream, const float* x, const float* y, const float* z,
+                        const int32_t centered_shape[3], const float world_scale[3], float loss_scale,
+                        int32_t accumulate, void* workspace, size_t workspace_bytes, float* grad_x, float* grad_y,
+                        float* grad_z, float* loss);
+
+/*
+ * One Adam update of up to 8 parameter tensors in a single launch, with the
+ * update tf.train.AdamOptimizer documents (the reference's optimizer,
+ * reconstruct_sequence.py:1899-1905): lr_t = lr sqrt(1-b2^t)/(1-b1^t),
+ * m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, p -= lr_t m / (sqrt(v) + eps).
+ * Host arrays of num_tensors device pointers (params, grads, first and second
+ * moments) and element counts; step = t >= 1, the update count including this
+ * one. Parameters and moments are updated in place.
+ */
+int gft_adam_step(void* stream, int32_t num_tensors, float* const* params, const float* const* grads,
+                  float* const* exp_avg, float* const* exp_avg_sq, const int64_t* sizes, float learning_rate,
+                  float beta1, float beta2, float epsilon, int64_t step);
+
+/*
  * Kernel timers (instrumentation; no counterpart in the reference, which
  * has none). gft_timer_start("a,b" or "*") makes every later launch of the
  * named kernels on the calling thread record a HIP event before and after
@@ -324,7 +372,7 @@ int gft_shadow_transmission_bwd(void* stream, const float* saved, const int32_t 
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
  * render_fwd, render_bwd_pixel, render_bwd_gather, render_bwd_scatter, quad_copy, fill,
- * shadow_fwd, shadow_bwd.
+ * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam.
  */
 int gft_timer_start(const char* kernels);
 int gft_timer_read(const char* kernel, int32_t* launches, double* total_ms);

@@ -335,13 +335,76 @@ def l2_loss_and_grad(img, target):
     return loss, g
 
 
+def image_loss_and_grad(img, target, kind="SE", background=None, scale=1.0):
+    """Image loss of loss_dens_target / loss_dens_target_raw
+    (optimization.py:396-430) with the SE or AE loss function
+    (optimization.py:94-130): loss = scale * sum_c f(e_c), e = rgb + bkg *
+    exp(-D) - target (no bkg term without a background), f = e^2 or |e|
+    (d|e|/de = sign(e), 0 at 0). Returns (loss, d loss / d img [..., 4])."""
+    img = np.asarray(img, dtype=np.float64)
+    rgb, D = img[..., :3], img[..., 3:4]
+    e = rgb - np.asarray(target, dtype=np.float64)
+    tr = None
+    if background is not None:
+        tr = np.exp(-D)
+        e = e + np.asarray(background, dtype=np.float64) * tr
+    if kind == "AE":
+        loss, fp = float(np.sum(np.abs(e))), np.sign(e)
+    else:
+        loss, fp = float(np.sum(e * e)), 2.0 * e
+    g = np.zeros_like(img)
+    g[..., :3] = scale * fp
+    if background is not None:
+        g[..., 3] = -scale * np.sum(np.asarray(background, dtype=np.float64) * fp, axis=-1) * tr[..., 0]
+    return scale * loss, g
+
+
+def divergence_loss_and_grad(x, y, z, world_scale=(1.0, 1.0, 1.0), scale=1.0):
+    """VelocityGrid.divergence (data_structures.py:929-936) of staggered
+    components x [1,D,H,W+1,1], y [1,D,H+1,W,1], z [1,D+1,H,W,1] and the SE
+    divergence loss scale * sum(div^2) (optimization.py:824-838). The
+    gradient is the adjoint of the forward differences: a face receives
+    2 scale w (div of the cell before it - div of the cell after it)."""
+    x, y, z = (np.asarray(a, dtype=np.float64) for a in (x, y, z))
+    div = ((x[:, :, :, 1:] - x[:, :, :, :-1]) * world_scale[0] + (y[:, :, 1:] - y[:, :, :-1]) * world_scale[1]
+           + (z[:, 1:] - z[:, :-1]) * world_scale[2])
+    loss = scale * float(np.sum(div * div))
+    gx, gy, gz = np.zeros_like(x), np.zeros_like(y), np.zeros_like(z)
+    t = 2.0 * scale * div
+    gx[:, :, :, 1:] += t * world_scale[0]
+    gx[:, :, :, :-1] -= t * world_scale[0]
+    gy[:, :, 1:] += t * world_scale[1]
+    gy[:, :, :-1] -= t * world_scale[1]
+    gz[:, 1:] += t * world_scale[2]
+    gz[:, :-1] -= t * world_scale[2]
+    return loss, div, (gx, gy, gz)
+
+
+def adam_tf(p, g, m, v, lr, beta1, beta2, epsilon, t):
+    """One update as tf.train.AdamOptimizer documents it (the reference's
+    optimizer, reconstruct_sequence.py:1899-1905), in float64:
+    lr_t = lr sqrt(1 - b2^t) / (1 - b1^t); m = b1 m + (1 - b1) g;
+    v = b2 v + (1 - b2) g^2; p = p - lr_t m / (sqrt(v) + eps)."""
+    p, g, m, v = (np.asarray(a, dtype=np.float64) for a in (p, g, m, v))
+    lr_t = lr * np.sqrt(1.0 - beta2 ** t) / (1.0 - beta1 ** t)
+    m = beta1 * m + (1.0 - beta1) * g
+    v = beta2 * v + (1.0 - beta2) * g * g
+    return p - lr_t * m / (np.sqrt(v) + epsilon), m, v
+
+
 def step_fwd_bwd(d0, vel_c, target, M, V, P, F, out_dhw, depth_step, order=2, clamp="MC_SMOOTH", prec=1,
-                 grad_mode="EXACT", boundary_render="BORDER"):
-    """One advect -> render -> L2 loss step and its gradients w.r.t. d0 and
-    the centred velocity. Returns dict(loss, img, d1, gd, gv)."""
+                 grad_mode="EXACT", boundary_render="BORDER", loss_kind=None, background=None, loss_scale=None):
+    """One advect -> render -> image loss step and its gradients w.r.t. d0
+    and the centred velocity. Default: the mean squared error
+    (l2_loss_and_grad); with loss_kind / background / loss_scale the image
+    loss of image_loss_and_grad. Returns dict(loss, img, d1, gd, gv)."""
     d1 = advect_fwd(d0, vel_c, order, clamp, 1.0, "CLAMP", prec)
     img = render_density_fwd(d1, M, V, P, F, out_dhw, depth_step, boundary=boundary_render, prec=prec)
-    loss, gimg = l2_loss_and_grad(img, target)
+    if loss_kind is None and background is None and loss_scale is None:
+        loss, gimg = l2_loss_and_grad(img, target)
+    else:
+        scale = loss_scale if loss_scale is not None else 1.0 / np.asarray(target).size
+        loss, gimg = image_loss_and_grad(img, target, loss_kind or "SE", background, scale)
     gd1 = render_density_bwd(d1, img, gimg, M, V, P, F, out_dhw, depth_step, boundary=boundary_render,
                              grad_mode=grad_mode, prec=prec)
     gd, gv = advect_bwd(d0, vel_c, gd1, order, clamp, 1.0, "CLAMP", grad_mode, prec)

@@ -652,3 +652,74 @@ def test_chain_exact_gradient_is_the_fd_of_the_summed_loss():
     # the decayed (reference) recursion equals the exact one at decay 0
     r0 = orc.chain_fwd_bwd(d0, vels, tg, *args, order=1, clamp="NONE", grad_mode="REFERENCE", decay=0.0)
     assert np.allclose(r0["gd0"], r["gd0"]) and all(np.allclose(a, b) for a, b in zip(r0["gv"], r["gv"]))
+
+
+# ------------------------------------------------- loss / optimizer epilogue
+@pytest.mark.parametrize("kind", ["SE", "AE"])
+@pytest.mark.parametrize("with_bkg", [False, True])
+def test_image_loss_gradient_fd(kind, with_bkg):
+    """The image loss gradient (rgb and, with a background, the optical depth
+    channel) is the central difference of the loss; errors are kept away from
+    the AE kink so the difference is exact to O(h^2)."""
+    rng = np.random.Generator(np.random.PCG64(21))
+    img = rng.uniform(0.1, 1.0, size=(1, 2, 3, 4, 4))
+    tgt = img[..., :3] + rng.choice([-1.0, 1.0], size=img[..., :3].shape) * rng.uniform(0.05, 0.3,
+                                                                                          size=img[..., :3].shape)
+    bkg = rng.uniform(0.0, 1.0, size=tgt.shape) if with_bkg else None
+    if with_bkg:
+        tgt = tgt + bkg * np.exp(-img[..., 3:4])
+    loss, g = orc.image_loss_and_grad(img, tgt, kind, bkg, scale=0.37)
+    h = 1e-6
+    for idx in [(0, 0, 0, 0, 0), (0, 1, 2, 3, 2), (0, 1, 1, 2, 3), (0, 0, 2, 1, 3)]:
+        a, b = img.copy(), img.copy()
+        a[idx] += h
+        b[idx] -= h
+        fd = (orc.image_loss_and_grad(a, tgt, kind, bkg, 0.37)[0] - orc.image_loss_and_grad(b, tgt, kind, bkg,
+                                                                                           0.37)[0]) / (2 * h)
+        assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(fd)), (idx, fd, g[idx])
+    if not with_bkg:
+        assert np.all(g[..., 3] == 0)
+
+
+def test_divergence_loss_linear_field_and_fd():
+    """Components linear along their own axis (slopes a, b, c per cell) have
+    div = wx a + wy b + wz c in every cell; the gradient is the central
+    difference of the loss."""
+    D, H, W = 4, 5, 6
+    ws = (0.5, 2.0, 1.5)
+    a, b, c = 0.3, -0.7, 0.2
+    x = np.broadcast_to(a * np.arange(W + 1, dtype=np.float64)[None, None, None, :, None], (1, D, H, W + 1, 1))
+    y = np.broadcast_to(b * np.arange(H + 1, dtype=np.float64)[None, None, :, None, None], (1, D, H + 1, W, 1))
+    z = np.broadcast_to(c * np.arange(D + 1, dtype=np.float64)[None, :, None, None, None], (1, D + 1, H, W, 1))
+    loss, div, _ = orc.divergence_loss_and_grad(x, y, z, ws, scale=0.25)
+    want = ws[0] * a + ws[1] * b + ws[2] * c
+    assert np.allclose(div, want, atol=1e-12)
+    assert abs(loss - 0.25 * D * H * W * want * want) <= 1e-12
+    rng = np.random.Generator(np.random.PCG64(5))
+    comps = [rng.uniform(-1, 1, size=s) for s in ((1, D, H, W + 1, 1), (1, D, H + 1, W, 1), (1, D + 1, H, W, 1))]
+    loss, _, grads = orc.divergence_loss_and_grad(*comps, ws, scale=0.25)
+    h = 1e-6
+    for ci, idx in [(0, (0, 1, 2, 0, 0)), (0, (0, 3, 4, 6, 0)), (1, (0, 2, 5, 3, 0)), (2, (0, 4, 0, 5, 0)),
+                    (2, (0, 0, 3, 2, 0))]:
+        up = [cc.copy() for cc in comps]
+        dn = [cc.copy() for cc in comps]
+        up[ci][idx] += h
+        dn[ci][idx] -= h
+        fd = (orc.divergence_loss_and_grad(*up, ws, 0.25)[0] - orc.divergence_loss_and_grad(*dn, ws, 0.25)[0]) / (2 * h)
+        assert abs(fd - grads[ci][idx]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_adam_tf_closed_forms():
+    """First step: p - lr g / (|g| + eps / sqrt(1 - b2)). A constant gradient
+    moves every step by lr g / (|g| + eps / sqrt(1 - b2^t)) (m_t = (1-b1^t) g,
+    v_t = (1-b2^t) g^2)."""
+    lr, b1, b2, eps = 0.01, 0.9, 0.999, 1e-3
+    p0 = np.array([1.0, -2.0, 0.5, 0.0])
+    g = np.array([0.5, -1e-3, 2.0, 0.0])
+    p, m, v = p0, np.zeros(4), np.zeros(4)
+    for t in range(1, 6):
+        prev = p
+        p, m, v = orc.adam_tf(p, g, m, v, lr, b1, b2, eps, t)
+        want = prev - lr * g / (np.abs(g) + eps / np.sqrt(1.0 - b2 ** t))
+        assert np.allclose(p, want, rtol=0, atol=1e-13), t
+    assert p[3] == p0[3]
