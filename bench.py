@@ -27,6 +27,10 @@ blend, LuT sampling) at the c2 sizes, each against its compulsory bytes.
 settings (256x196x196, LINEAR mips, gather gradient) over the c2 density:
 the fused shadow op against the op sequence it replaces.
 
+``--epilogue`` instead times render + AE/background image loss + divergence loss
++ Adam at c2: the fused kernels against torch autograd / torch.optim.Adam over this
+build's render kernels.
+
 ``--chain`` instead measures the global transport chain (workload c3): an
 initial density advected through 20 per-frame velocity fields, every frame
 rendered to 5 views plus one unseen view and compared with its targets, the
@@ -360,6 +364,101 @@ def shadow_bench(args):
         "shadow_bwd": {"GBps": bwd_bytes / (kern["shadow_bwd"] * 1e-3) / 1e9, "alg_bytes": int(bwd_bytes)}}))
 
 
+def epilogue_bench(args):
+    """One JSON line for the loss / optimizer epilogue at c2: render forward,
+    AE image loss against targets with the background composited
+    (loss_dens_target_raw), render backward, divergence loss of a staggered
+    velocity of the grid's shape, and one Adam update of the density and the
+    three velocity components. Fused: the loss folded into the render
+    backward, the divergence kernel pair and the multi-tensor Adam. Op
+    composition: torch autograd for the image loss and the divergence on this
+    build's render kernels, torch.optim.Adam (foreach) for the update."""
+    import numpy as np
+    import torch
+
+    from gft._lib import KernelTimers
+    from gft.grids import VelocityGrid
+    from gft.optim import AdamOptimizer, divergence_loss_into
+    from gft.pipeline import TransportRenderStep, render_density
+
+    inp = build_inputs("c2")
+    dev = torch.device("cuda:0")
+    grid = inp["grid"]
+    nV, h, w = inp["views"], inp["out_dhw"][1], inp["out_dhw"][2]
+    d = torch.from_numpy(inp["d0"]).to(dev)
+    vg = VelocityGrid(list(grid), std=0.1, seed=3, device=dev)
+    bkg = (0.0, 0.5, 1.0)
+    bkg_t = torch.tensor(bkg, device=dev).expand(1, nV, h, w, 3).contiguous()
+    target = (torch.rand((1, nV, h, w, 3), device=dev) * 0.5).contiguous()
+    scale = 1.0 / (nV * h * w * 3)
+    # fused: the fused step's render buffers and folded AE + background loss, on the density itself
+    from gft import _lib
+    from gft.ops import _np_ptr
+    step = TransportRenderStep(grid, inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"],
+                               loss="AE", background=bkg, loss_scale=scale)
+    params_f = [d.clone(), vg.x.clone(), vg.y.clone(), vg.z.clone()]
+    gvel = [torch.empty_like(c) for c in params_f[1:]]
+    opt_f = AdamOptimizer(1e-3)
+    L, c = step._L, step.cams
+
+    def fused():
+        dd, s = params_f[0], _lib.stream_of(params_f[0])
+        _lib.check(L.gft_render_density_fwd(s, _lib.ptr(dd), step._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
+                                            _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, step.light, step.depth_step,
+                                            step.boundary, _lib.ptr(step.render_fwd_ws), step._nfws,
+                                            _lib.ptr(step.render_saved), step._nrsaved, _lib.ptr(step.images)))
+        _lib.check(L.gft_render_density_bwd_loss(
+            s, _lib.ptr(dd), step._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V), _np_ptr(c.P), _np_ptr(c.F), c.nCam,
+            c.c_out, step.light, step.depth_step, step.boundary, _lib.ptr(step.images), _lib.ptr(target),
+            _lib.ptr(step.background), step.loss_kind, step.loss_scale, _lib.ptr(step.render_saved), step._nrsaved,
+            None, 0, _lib.ptr(step.grad_d1), _lib.ptr(step.loss)))
+        divergence_loss_into(params_f[1:], (1.0, 1.0, 1.0), 1e-3, grads=gvel)
+        opt_f.apply_gradients([(step.grad_d1, params_f[0])] + list(zip(gvel, params_f[1:])))
+
+    params_c = [p.clone().requires_grad_(True) for p in params_f]
+    opt_c = torch.optim.Adam(params_c, lr=1e-3)
+
+    def composed():
+        opt_c.zero_grad(set_to_none=True)
+        img = render_density(params_c[0], inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"])
+        comp = img[..., :3] + bkg_t * torch.exp(-img[..., 3:4])
+        loss = torch.abs(comp - target).sum() * scale
+        v = VelocityGrid(list(grid), x=params_c[1], y=params_c[2], z=params_c[3])
+        loss = loss + 1e-3 * (v.divergence((1.0, 1.0, 1.0)) ** 2).sum()
+        loss.backward()
+        opt_c.step()
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(args.steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts))
+
+    ms_f, ms_c = timeit(fused), timeit(composed)
+    with KernelTimers("render_fwd,render_bwd_pixel,render_bwd_gather,loss_sum,divergence,divergence_grad,adam,"
+                      "quad_copy") as kt:
+        for _ in range(args.steps):
+            fused()
+        torch.cuda.synchronize()
+    kern = {k: v[1] / max(v[0], 1) for k, v in kt.result.items()}
+    nparam = sum(p.numel() for p in params_f)
+    print(json.dumps({
+        "metric": "render + loss + divergence + Adam epilogue time, fused vs op composition", "unit": "ms",
+        "n_gpus": 1, "steps": args.steps, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c2: grid 128x224x128, 5 views 256x456, AE loss with background, divergence "
+                               "loss of a staggered velocity, Adam over %d parameters" % nparam},
+        "ms": {"fused": ms_f, "op composition": ms_c}, "speedup": ms_c / ms_f, "kernels_ms": kern,
+        "adam": {"GBps": nparam * 28 / (kern["adam"] * 1e-3) / 1e9 if "adam" in kern else None,
+                 "alg_bytes": nparam * 28}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,6 +472,7 @@ def main():
     ap.add_argument("--chain", action="store_true")
     ap.add_argument("--ops", action="store_true")
     ap.add_argument("--shadow", action="store_true")
+    ap.add_argument("--epilogue", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -388,6 +488,9 @@ def main():
         return
     if args.shadow:
         shadow_bench(args)
+        return
+    if args.epilogue:
+        epilogue_bench(args)
         return
 
     import numpy as np
