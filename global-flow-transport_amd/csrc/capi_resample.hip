@@ -27,6 +27,9 @@ hipError_t shadow_fwd_launch(int bm, int fm, int bl, hipStream_t st, const float
                              const float* mask, float* saved, float* trans);
 hipError_t shadow_bwd_launch(int bm, int bl, hipStream_t st, const float* saved, const float* gt, const float* mask,
                              Dims3 dout, float* gs);
+hipError_t sample_transform_gather_launch(int C, bool ref, int fm, hipStream_t st, const float* og, Dims3 din,
+                                          Dims3 dout, const SampleLaunch& L, const SampleLaunch& Linv, int accumulate,
+                                          float* grad);
 hipError_t normalize_count_launch(int C, hipStream_t st, float* grad, const unsigned* cnt, size_t nvox);
 hipError_t sample_lut_launch(int C, int bm, hipStream_t st, const float* in, const float* atlas, const MipTable& mt,
                              const SampleCfg& s, const LutLaunch& L, int pairs, const float* lut, float* out);
@@ -275,8 +278,42 @@ int gft_sample_grid_transform_bwd(void* stream, const float* input, const int64_
   int C = (int)input_shape[4];
   const int N = (int)input_shape[0];
   const size_t vox = nvox(din), bstride = vox * C;
-  if ((rc = check_hip(fill_launch(st, input_grad, bstride * N, 0.f), "zero input_grad"))) return rc;
   auto pairs = view_pairs(N, num_cameras, separate_camera_batch != 0);
+  // Gather form (no atomics, one store per voxel) for BORDER without mips in
+  // the forward coordinate modes: per batch, views in launches of up to
+  // kMaxSampleViews (REFERENCE needs all of a batch's views in one launch for
+  // its counts).
+  const bool gather = boundary == GFT_BORDER && mipmapping == GFT_MIP_NONE &&
+                      (coordinate_mode == CM_LINDEPTH || coordinate_mode == CM_TRANSFORM) &&
+                      (grad_mode == GFT_GRAD_REFERENCE || interpolation == GFT_LINEAR ||
+                       interpolation == GFT_NEAREST) &&
+                      nvox(dout) < (size_t(1) << 31);
+  if (gather) {
+    std::vector<int> per_batch(N, 0);
+    for (const ViewPair& vp : pairs) per_batch[vp.batch]++;
+    bool fits = true;
+    for (int b = 0; b < N && grad_mode == GFT_GRAD_REFERENCE; ++b) fits &= per_batch[b] <= kMaxSampleViews;
+    if (fits) {
+      for (int b = 0; b < N; ++b) {
+        std::vector<ViewPair> mine;
+        for (const ViewPair& vp : pairs)
+          if (vp.batch == b) mine.push_back(vp);
+        for (size_t first = 0; first < mine.size(); first += kMaxSampleViews) {
+          size_t n = std::min(mine.size() - first, (size_t)kMaxSampleViews);
+          SampleLaunch L, Linv;
+          if ((rc = build_sample_views(mine, first, n, M, -1, V, P, frustum, coordinate_mode, L))) return rc;
+          if ((rc = build_sample_views(mine, first, n, M, -1, V, P, frustum, coordinate_mode ^ 1, Linv))) return rc;
+          if ((rc = check_hip(sample_transform_gather_launch(C, grad_mode == GFT_GRAD_REFERENCE, interpolation, st,
+                                                             output_grad, din, dout, L, Linv, first > 0,
+                                                             input_grad + bstride * b),
+                              "gradient gather launch")))
+            return rc;
+        }
+      }
+      return GFT_OK;
+    }
+  }
+  if ((rc = check_hip(fill_launch(st, input_grad, bstride * N, 0.f), "zero input_grad"))) return rc;
   if (grad_mode == GFT_GRAD_REFERENCE) {
     // per batch: counted scatter of all its views, then normalisation
     unsigned* cnt = (unsigned*)workspace;

@@ -405,6 +405,203 @@ __global__ __launch_bounds__(256) void k_sample_transform_adjoint(const float* _
   }
 }
 
+// ----------------------------------------- gradient gather (BORDER, no mips)
+// SampleGridTransformGrad in gather form: one block per kGBX x kGBY x kGBZ
+// voxel block of one batch. For each view the block bounds the frustum cells
+// whose sample can reach its voxels: the corners of the position box
+// [x0-1, x0+kGBX] x ... (every tap of a sample inside it, floor/ceil or
+// floor/floor+1, lies in the block only if the sample does) are mapped back
+// to frustum index space with the inverse coordinate mode (a projective map,
+// so the image of the box lies in the box of its corner images; one cell of
+// margin covers rounding), and the block walks those cells, recomputes each
+// sample position exactly as the forward op does and adds the contributions
+// of the taps that fall in the block to FP64 LDS accumulators (plus the
+// per-voxel visit count of the REFERENCE flavour). A corner image behind the
+// camera widens the walk to the whole frustum. No global atomics; one store
+// per voxel: the REFERENCE flavour stores sum / count
+// (kNormalize3DGradients), EXACT stores or adds the sum.
+constexpr int kGBX = 16, kGBY = 8, kGBZ = 4, kGBVox = kGBX * kGBY * kGBZ;
+
+// frustum-space bounds [lo, hi] (cell indices) of the cells whose sample can
+// land in the position box b0..b1 (vinv: the view in the reverse coordinate
+// mode, grid position -> frustum position, as map_pos computes it). The x and
+// y images are projective in the position (extremes at the corners while
+// every corner is in front of the camera, w > 0); the linear-depth image
+// -p32 / (ndz + p22) is monotonic on each side of its pole, and frustum cells
+// only produce depths on the negative side: all corners there -> the corner
+// hull, all on the positive side -> no cell, straddling the pole (possible
+// for orthographic projections) -> every depth. Out-of-front corners or NaN
+// widen to every cell.
+__device__ __forceinline__ void gather_cell_bounds(const SampleView& vinv, int cm_inv, float3 b0, float3 b1,
+                                                   float3 idims_out, Dims3 dout, int lo[3], int hi[3]) {
+  float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  bool ok = true;
+  int neg = 0, pos = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    // positions are in sample units; the coordinate maps take cell-centre units (+0.5)
+    const float qx = ((k & 1) ? b1.x : b0.x) + 0.5f, qy = ((k & 2) ? b1.y : b0.y) + 0.5f,
+                qz = ((k & 4) ? b1.z : b0.z) + 0.5f;
+    const float4 vs = mat_vec(vinv.A, qx, qy, qz, 1.f);
+    const float4 ndc = mat_vec(vinv.B, vs.x, vs.y, vs.z, vs.w);
+    ok &= ndc.w > 0.f;
+    const float ndx = ndc.x / ndc.w, ndy = ndc.y / ndc.w, ndz = ndc.z / ndc.w;
+    float r[3] = {(ndx * 0.5f + 0.5f) * idims_out.x, (ndy * 0.5f + 0.5f) * idims_out.y,
+                  (ndz * 0.5f + 0.5f) * idims_out.z};
+    if (cm_inv == CM_LINDEPTH_REVERSE) {
+      const float z = -vinv.p32 / (ndz + vinv.p22);
+      neg += z < 0.f;
+      pos += z > 0.f;
+      r[2] = -(z + vinv.nearp) / (vinv.farp - vinv.nearp) * idims_out.z;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      ok &= r[a] == r[a];
+      mn[a] = fminf(mn[a], r[a]);
+      mx[a] = fmaxf(mx[a], r[a]);
+    }
+  }
+  const int dd[3] = {dout.x, dout.y, dout.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    // cell c has its centre at c + 0.5
+    lo[a] = (int)fmaxf(ceilf(fmaxf(mn[a], -4.f) - 0.5f) - 1.f, 0.f);
+    hi[a] = (int)fminf(floorf(fminf(mx[a], (float)dd[a] + 4.f) - 0.5f) + 1.f, (float)(dd[a] - 1));
+  }
+  if (cm_inv == CM_LINDEPTH_REVERSE && neg + pos < 8) ok = false;  // a corner on the pole
+  if (cm_inv == CM_LINDEPTH_REVERSE && ok && pos == 8) {
+    lo[2] = 1;  // no frustum cell has a sample there
+    hi[2] = 0;
+  } else if (cm_inv == CM_LINDEPTH_REVERSE && ok && pos > 0) {
+    lo[2] = 0;
+    hi[2] = dd[2] - 1;
+  }
+  if (!ok) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = 0;
+      hi[a] = dd[a] - 1;
+    }
+  }
+}
+
+template <int C, bool REF, int FM>
+__global__ __launch_bounds__(256) void k_sample_transform_gather_adj(const float* __restrict__ og, Dims3 din,
+                                                                     Dims3 dout, SampleLaunch L, SampleLaunch Linv,
+                                                                     int accumulate, float* __restrict__ grad) {
+  __shared__ double acc[kGBVox * C];
+  __shared__ unsigned cnt[REF ? kGBVox : 1];
+  typedef __attribute__((address_space(3))) double lds_double;
+  typedef __attribute__((address_space(3))) unsigned lds_uint;
+  lds_double* a = (lds_double*)acc;
+  lds_uint* cn = (lds_uint*)cnt;
+  const int nbx = (din.x + kGBX - 1) / kGBX, nby = (din.y + kGBY - 1) / kGBY;
+  const int x0 = (blockIdx.x % nbx) * kGBX, y0 = ((blockIdx.x / nbx) % nby) * kGBY,
+            z0 = (blockIdx.x / (nbx * nby)) * kGBZ;
+  for (int k = threadIdx.x; k < kGBVox * C; k += 256) acc[k] = 0.0;
+  if (REF)
+    for (int k = threadIdx.x; k < kGBVox; k += 256) cnt[k] = 0u;
+  __syncthreads();
+  const size_t n = (size_t)dout.x * dout.y * dout.z;
+  const float3 oinv = make_float3(1.f / dout.x, 1.f / dout.y, 1.f / dout.z);
+  const float3 idims = make_float3((float)din.x, (float)din.y, (float)din.z);
+  const float3 idims_out = make_float3((float)dout.x, (float)dout.y, (float)dout.z);
+  // nearest truncates p + 0.5 toward zero, so p down to x0 - 1.5 can reach x0 = 0
+  const float3 b0 = make_float3((float)x0 - 2.f, (float)y0 - 2.f, (float)z0 - 2.f);
+  const float3 b1 = make_float3((float)(x0 + kGBX), (float)(y0 + kGBY), (float)(z0 + kGBZ));
+  for (int vi = 0; vi < L.num_views; ++vi) {
+    const SampleView& v = L.view[vi];
+    int lo[3], hi[3];
+    gather_cell_bounds(Linv.view[vi], Linv.cm, b0, b1, idims_out, dout, lo, hi);
+    if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) continue;
+    const unsigned nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+    const unsigned cells = nx * ny * (unsigned)(hi[2] - lo[2] + 1);  // the caller keeps n below 2^31
+    const float* ogv = og + (size_t)v.out_slot * n * C;
+    for (unsigned k = threadIdx.x; k < cells; k += 256) {
+      const unsigned r = k / nx;
+      const int cx = lo[0] + (int)(k - r * nx), cy = lo[1] + (int)(r % ny), cz = lo[2] + (int)(r / ny);
+      float l;
+      const float3 q = sample_pos(v, L.cm, false, cx + 0.5f, cy + 0.5f, cz + 0.5f, oinv, idims, l);
+      const float px = q.x - 0.5f, py = q.y - 0.5f, pz = q.z - 0.5f;
+      if (!((px >= b0.x) & (px <= b1.x) & (py >= b0.y) & (py <= b1.y) & (pz >= b0.z) & (pz <= b1.z))) continue;
+      const Vec<C> g = vload<C>(ogv, ((size_t)cz * dout.y + cy) * dout.x + cx);
+      int ix[2], iy[2], iz[2];
+      float wx[2], wy[2], wz[2];
+      bool take = true;
+      if (REF) {
+        // scatterGradInterpolated: p in (-1, dims), floor and ceil corners
+        take = (-1.f < px) & (px < (float)din.x) & (-1.f < py) & (py < (float)din.y) & (-1.f < pz) &
+               (pz < (float)din.z);
+        const float fxf = floorf(px), fyf = floorf(py), fzf = floorf(pz);
+        ix[0] = (int)fxf; ix[1] = (int)ceilf(px);
+        iy[0] = (int)fyf; iy[1] = (int)ceilf(py);
+        iz[0] = (int)fzf; iz[1] = (int)ceilf(pz);
+        wx[1] = px - fxf; wx[0] = 1.f - wx[1];
+        wy[1] = py - fyf; wy[0] = 1.f - wy[1];
+        wz[1] = pz - fzf; wz[0] = 1.f - wz[1];
+      } else if (FM == F_LINEAR) {
+        const Taps t = make_taps<B_BORDER>(px, py, pz, din);
+        take = t.inside;
+        ix[0] = t.vx[0] ? t.ix[0] : -1000; ix[1] = t.vx[1] ? t.ix[1] : -1000;
+        iy[0] = t.vy[0] ? t.iy[0] : -1000; iy[1] = t.vy[1] ? t.iy[1] : -1000;
+        iz[0] = t.vz[0] ? t.iz[0] : -1000; iz[1] = t.vz[1] ? t.iz[1] : -1000;
+        wx[0] = 1.f - t.wx; wx[1] = t.wx;
+        wy[0] = 1.f - t.wy; wy[1] = t.wy;
+        wz[0] = 1.f - t.wz; wz[1] = t.wz;
+      } else {
+        size_t f;
+        take = nearest_index<B_BORDER>(px, py, pz, din, f);
+        ix[0] = ix[1] = (int)(px + 0.5f);
+        iy[0] = iy[1] = (int)(py + 0.5f);
+        iz[0] = iz[1] = (int)(pz + 0.5f);
+        wx[0] = wy[0] = wz[0] = 1.f;
+        wx[1] = wy[1] = wz[1] = 0.f;
+      }
+      if (!take) continue;
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            if (FM == F_NEAREST && !REF && (i | j | kk)) continue;
+            const int lx = ix[i] - x0, ly = iy[j] - y0, lz = iz[kk] - z0;
+            if (!(((unsigned)lx < (unsigned)kGBX) & ((unsigned)ly < (unsigned)kGBY) & ((unsigned)lz < (unsigned)kGBZ)))
+              continue;
+            // the reference also drops corners outside the grid (BORDER)
+            if (REF && !((ix[i] < din.x) & (iy[j] < din.y) & (iz[kk] < din.z))) continue;
+            if (!REF && FM == F_LINEAR && !((ix[i] < din.x) & (iy[j] < din.y) & (iz[kk] < din.z))) continue;
+            const float w = wx[i] * wy[j] * wz[kk];
+            if (!REF && FM == F_LINEAR && w == 0.f) continue;
+            const int vox = (lz * kGBY + ly) * kGBX + lx;
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch)
+              __hip_atomic_fetch_add(a + vox * C + ch, (double)(g.v[ch] * w), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (REF) __hip_atomic_fetch_add(cn + vox, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kGBVox; k += 256) {
+    const int x = x0 + k % kGBX, y = y0 + (k / kGBX) % kGBY, z = z0 + k / (kGBX * kGBY);
+    if (x >= din.x || y >= din.y || z >= din.z) continue;
+    const size_t f = flat3(x, y, z, din);
+    Vec<C> r;
+    if (REF) {
+      const unsigned c = cnt[k];
+      const float inv = c ? 1.f / (float)c : 0.f;
+#pragma unroll
+      for (int ch = 0; ch < C; ++ch) r.v[ch] = c ? (float)acc[k * C + ch] * inv : 0.f;
+    } else {
+      Vec<C> old = accumulate ? vload<C>(grad, f) : vzero<C>();
+#pragma unroll
+      for (int ch = 0; ch < C; ++ch) r.v[ch] = old.v[ch] + (float)acc[k * C + ch];
+    }
+    vstore<C>(grad, f, r);
+  }
+}
+
 // kNormalize3DGradients (resample_grid.cu.cc:814-832): divide by the count
 // where it is positive.
 template <int C>
@@ -849,6 +1046,27 @@ hipError_t sample_transform_launch(int C, int bm, bool sum, hipStream_t st, cons
       hipLaunchKernelGGL((k_sample_transform<CC, BB, false>), grid_for(n, L.num_views), dim3(256), 0, st, in, atlas,
                          mt, s, dout, L, out);
   }));
+  return hipGetLastError();
+}
+
+// gather-form gradient over the voxel blocks of one batch (grad points at it)
+hipError_t sample_transform_gather_launch(int C, bool ref, int fm, hipStream_t st, const float* og, Dims3 din,
+                                          Dims3 dout, const SampleLaunch& L, const SampleLaunch& Linv, int accumulate,
+                                          float* grad) {
+  const unsigned nb = (unsigned)(((din.x + kGBX - 1) / kGBX) * ((din.y + kGBY - 1) / kGBY) *
+                                 ((din.z + kGBZ - 1) / kGBZ));
+  KernelTimer t(st, "transform_grad_gather");
+  GFT_DISPATCH_C(C, {
+    if (ref)
+      hipLaunchKernelGGL((k_sample_transform_gather_adj<CC, true, F_LINEAR>), dim3(nb), dim3(256), 0, st, og, din,
+                         dout, L, Linv, accumulate, grad);
+    else if (fm == F_NEAREST)
+      hipLaunchKernelGGL((k_sample_transform_gather_adj<CC, false, F_NEAREST>), dim3(nb), dim3(256), 0, st, og, din,
+                         dout, L, Linv, accumulate, grad);
+    else
+      hipLaunchKernelGGL((k_sample_transform_gather_adj<CC, false, F_LINEAR>), dim3(nb), dim3(256), 0, st, og, din,
+                         dout, L, Linv, accumulate, grad);
+  });
   return hipGetLastError();
 }
 

@@ -210,6 +210,10 @@ int gft_sample_grid_transform_fwd(void* stream, const float* input, const int64_
  * with a per-voxel counter and division by the count per data batch
  * (mipmapping must be NONE, as the reference op registers); EXACT: the true
  * adjoint of the forward, mip levels included. input_grad is overwritten.
+ * BORDER without mips in the forward coordinate modes runs in gather form
+ * (per voxel block, no global atomics; REFERENCE when a batch has at most 8
+ * views); the other cases scatter with atomics. The workspace query covers
+ * both.
  */
 size_t gft_sample_grid_transform_bwd_workspace(const int64_t input_shape[5], int32_t mipmapping, int32_t num_mipmaps,
                                                int32_t grad_mode);
@@ -372,7 +376,7 @@ int gft_adam_step(void* stream, int32_t num_tensors, float* const* params, const
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
  * render_fwd, render_bwd_pixel, render_bwd_gather, render_bwd_scatter, quad_copy, fill,
- * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam.
+ * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam, transform_grad_gather.
  */
 int gft_timer_start(const char* kernels);
 int gft_timer_read(const char* kernel, int32_t* launches, double* total_ms);

@@ -320,3 +320,62 @@ def test_transform_orthographic_axis_aligned(cuda, yaw, coord):
                                           **kw).cpu().numpy()
     gref = orc.sample_transform_bwd(data, og, M, V, P, F, grad_mode="EXACT", **kw)
     assert norm_err(g, gref) < 1e-4
+
+
+@pytest.mark.parametrize("grad_mode,filt", [("EXACT", "NEAREST"), ("EXACT", "LINEAR"), ("REFERENCE", "LINEAR")])
+@pytest.mark.parametrize("views,sep", [(11, True), (2, False)])
+def test_transform_backward_gather_form(cuda, grad_mode, filt, views, sep):
+    """BORDER without mips runs the gradient in gather form: eleven views
+    (EXACT adds over two launches, REFERENCE takes the counted scatter because
+    its counts span all views of a batch), a camera per batch, NEAREST taps
+    (against the FP32 oracle: index rounding at exact halves)."""
+    sc = scene(grid=(10, 18, 14), image=(12, 9), views=views, seed=12, channels=2)
+    data, M = sc["data"], sc["M"]
+    if not sep:
+        data = np.concatenate([data, 0.7 * data[:, ::-1]], axis=0).astype(np.float32)
+        M = np.concatenate([M, M], axis=0)
+    rng = np.random.Generator(np.random.PCG64(13))
+    og = rng.uniform(-1, 1, size=(data.shape[0], views if sep else 1) + tuple(sc["out_dhw"]) + (2,)).astype(np.float32)
+    kw = dict(interpolation=filt, boundary="BORDER", coordinate_mode="TRANSFORM_LINDEPTH", separate_camera_batch=sep)
+    got = _ops().sample_grid_transform_grad(torch.from_numpy(data).cuda(), torch.from_numpy(og).cuda(), M, sc["V"],
+                                            sc["P"], sc["F"], grad_mode=grad_mode, **kw).cpu().numpy()
+    ref = orc.sample_transform_bwd(data, og, M, sc["V"], sc["P"], sc["F"], grad_mode=grad_mode,
+                                   prec=0 if filt == "NEAREST" else 1, **kw)
+    assert np.abs(ref).max() > 0
+    assert norm_err(got, ref) < 1e-4
+
+
+@pytest.mark.parametrize("grad_mode", ["REFERENCE", "EXACT"])
+def test_transform_backward_camera_inside_grid(cuda, grad_mode):
+    """A camera inside the volume: voxel blocks behind it have corner images
+    behind the camera, and the gather walks every frustum cell for them."""
+    from gft.camera import Camera, camera_params_batch, model_matrices
+    from gft import synthetic as syn
+    from gft.transform import GridTransform
+    grid = (12, 14, 16)
+    rng = np.random.Generator(np.random.PCG64(14))
+    data = rng.uniform(0, 1, size=(1,) + grid + (1,)).astype(np.float32)
+    cam = Camera(GridTransform([10, 9, 11], translation=[0.5, -0.5, 1.0]), nearFar=[0.5, 9.0], fov=70.0, aspect=11 / 9)
+    V, P, F = camera_params_batch([cam])
+    M = model_matrices([syn.grid_transform(grid)])
+    og = rng.uniform(-1, 1, size=(1, 1, 10, 9, 11, 1)).astype(np.float32)
+    kw = dict(interpolation="LINEAR", boundary="BORDER", coordinate_mode="TRANSFORM_LINDEPTH")
+    got = _ops().sample_grid_transform_grad(torch.from_numpy(data).cuda(), torch.from_numpy(og).cuda(), M, V, P, F,
+                                            grad_mode=grad_mode, **kw).cpu().numpy()
+    ref = orc.sample_transform_bwd(data, og, M, V, P, F, grad_mode=grad_mode, **kw)
+    assert np.abs(ref).max() > 0 and (np.abs(ref) > 0).mean() > 0.05
+    assert norm_err(got, ref) < 1e-4
+
+
+def test_transform_backward_gather_adjoint_identity(cuda):
+    """<A x, y> == <x, A^T y> for the gather-form EXACT gradient."""
+    sc, data, out_dhw = _setup(4, seed=15, views=4)
+    rng = np.random.Generator(np.random.PCG64(16))
+    y = rng.uniform(-1, 1, size=(1, 4) + tuple(out_dhw) + (4,)).astype(np.float32)
+    x = torch.from_numpy(data).cuda().requires_grad_(True)
+    ax = _ops().sample_grid_transform(x, sc["M"], sc["V"], sc["P"], sc["F"], out_dhw, grad_mode="EXACT",
+                                      boundary="BORDER", coordinate_mode="TRANSFORM")
+    ax.backward(torch.from_numpy(y).cuda())
+    lhs = float((ax.detach().double().cpu() * torch.from_numpy(y).double()).sum())
+    rhs = float((x.detach().double().cpu() * x.grad.double().cpu()).sum())
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
