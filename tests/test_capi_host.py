@@ -147,6 +147,32 @@ def test_validation_returns_before_device_work(lib):
     assert shadow((1, 8, 8, 8, 1), mm=2, nm=2) == _lib.GFT_ERR_WORKSPACE
     rc = lib.gft_shadow_transmission_bwd(null, null, _i32(8, 8, 8), 0, 0, null, null, null)
     assert rc == _lib.GFT_ERR_INVALID_ARGUMENT
+    # epilogue: divergence shape and gradient triple; Adam tensor count, step and sizes
+    ws = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
+    assert lib.gft_divergence_loss(null, null, null, null, _i32(4, 0, 4), ws, ctypes.c_float(1.0), 0, null, 0, null,
+                                   null, null, null) == _lib.GFT_ERR_INVALID_ARGUMENT
+    p16 = ctypes.c_void_p(16)
+    assert lib.gft_divergence_loss(null, p16, p16, p16, _i32(4, 4, 4), ws, ctypes.c_float(1.0), 0, p16, 1 << 20, p16,
+                                   null, null, p16) == _lib.GFT_ERR_INVALID_ARGUMENT
+    need = lib.gft_divergence_loss_workspace(_i32(4, 4, 4))
+    assert need > 0 and lib.gft_divergence_loss(null, p16, p16, p16, _i32(4, 4, 4), ws, ctypes.c_float(1.0), 0, p16,
+                                                need - 1, null, null, null, p16) == _lib.GFT_ERR_WORKSPACE
+    ptrs = (ctypes.c_void_p * 9)()
+    sizes = (ctypes.c_int64 * 9)()
+    assert lib.gft_adam_step(null, 9, ptrs, ptrs, ptrs, ptrs, sizes, 1e-3, 0.9, 0.999, 1e-8, 1) == \
+        _lib.GFT_ERR_INVALID_ARGUMENT
+    assert lib.gft_adam_step(null, 1, ptrs, ptrs, ptrs, ptrs, sizes, 1e-3, 0.9, 0.999, 1e-8, 0) == \
+        _lib.GFT_ERR_INVALID_ARGUMENT
+    sizes[0] = 5
+    assert lib.gft_adam_step(null, 1, ptrs, ptrs, ptrs, ptrs, sizes, 1e-3, 0.9, 0.999, 1e-8, 1) == \
+        _lib.GFT_ERR_INVALID_ARGUMENT and b"tensor 0" in lib.gft_last_error()
+    assert lib.gft_adam_step(null, 0, None, None, None, None, None, 1e-3, 0.9, 0.999, 1e-8, 1) == _lib.GFT_OK
+    # folded image loss: an unknown loss kind is rejected before any launch
+    light = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
+    rc = lib.gft_render_density_bwd_loss(null, p16, _i64(1, 4, 4, 4, 1), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), light,
+                                         ctypes.c_float(1.0), 1, p16, p16, null, 7, ctypes.c_float(1.0), null, 0, null,
+                                         0, p16, p16)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT and b"loss_kind" in lib.gft_last_error()
     # fused advection: scalar grids only
     rc = lib.gft_advect_fwd(null, null, _i64(1, 4, 4, 4, 2), null, ctypes.c_float(1.0), 2, 1, null, null, 0)
     assert rc == _lib.GFT_ERR_UNIMPLEMENTED
