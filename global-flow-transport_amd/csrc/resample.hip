@@ -762,6 +762,157 @@ __global__ __launch_bounds__(256) void k_sample_lut_bwd(const float* __restrict_
   else scatter_reference<C, BM>(grad_b, cnt, din, px, py, pz, g);
 }
 
+// Tiled form of k_sample_lut_bwd for one-channel grids and relative lookups
+// (the advection LuT, whose positions stay near their cells): a block takes a
+// kLTX x kLTY x kLTZ tile of lookup cells and accumulates the input-gradient
+// taps (and the REFERENCE visit counts) that fall in the tile plus a
+// kLHalo-cell margin in FP64 / u32 LDS, then adds each nonzero region cell to
+// the global gradient (and counter) with one atomic. Taps outside the region
+// go to global atomics directly, so any lookup is handled exactly. Tap
+// positions, weights, validity and counts are those of scatter_reference and
+// fetch_v1_adjoint; the lookup gradient is written per cell as in
+// k_sample_lut_bwd. MIN/MAX taps keep the untiled kernel.
+constexpr int kLTX = 32, kLTY = 4, kLTZ = 4, kLHalo = 3;
+constexpr int kLRX = kLTX + 2 * kLHalo, kLRY = kLTY + 2 * kLHalo, kLRZ = kLTZ + 2 * kLHalo;
+constexpr int kLRegion = kLRX * kLRY * kLRZ;
+
+template <int BM, bool EXACT>
+__global__ __launch_bounds__(256) void k_sample_lut_bwd_tiled(const float* __restrict__ in_b, int fm, float offset,
+                                                              LutLaunch L, const float* __restrict__ lut,
+                                                              const float* __restrict__ og, Dims3 din,
+                                                              float* __restrict__ grad_b, unsigned* __restrict__ cnt,
+                                                              float* __restrict__ lut_grad) {
+  __shared__ double acc[kLRegion];
+  __shared__ unsigned cts[EXACT ? 1 : kLRegion];
+  typedef __attribute__((address_space(3))) double lds_double;
+  typedef __attribute__((address_space(3))) unsigned lds_uint;
+  lds_double* a = (lds_double*)acc;
+  lds_uint* cn = (lds_uint*)cts;
+  const Dims3 dout = L.dout;
+  const size_t n = (size_t)dout.x * dout.y * dout.z;
+  const int ntx = (dout.x + kLTX - 1) / kLTX, nty = (dout.y + kLTY - 1) / kLTY;
+  const int x0 = (blockIdx.x % ntx) * kLTX, y0 = ((blockIdx.x / ntx) % nty) * kLTY,
+            z0 = (blockIdx.x / (ntx * nty)) * kLTZ;
+  const int rx0 = x0 - kLHalo, ry0 = y0 - kLHalo, rz0 = z0 - kLHalo;
+  const int per = L.global ? L.num_cams : 1;
+  const int b = L.batch0;
+  const int cam = L.global ? (int)blockIdx.y : b;
+  const int slot = b * per + (L.global ? cam : 0);
+  for (int k = threadIdx.x; k < kLRegion; k += 256) {
+    acc[k] = 0.0;
+    if (!EXACT) cts[k] = 0u;
+  }
+  __syncthreads();
+  // one tap of weight w at grid cell (ix, iy, iz): the region when it is there, global otherwise
+  auto emit = [&](int ix, int iy, int iz, float v, bool count) {
+    const int lx = ix - rx0, ly = iy - ry0, lz = iz - rz0;
+    if (((unsigned)lx < (unsigned)kLRX) & ((unsigned)ly < (unsigned)kLRY) & ((unsigned)lz < (unsigned)kLRZ)) {
+      const int r = (lz * kLRY + ly) * kLRX + lx;
+      __hip_atomic_fetch_add(a + r, (double)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (count) __hip_atomic_fetch_add(cn + r, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      const size_t f = flat3(ix, iy, iz, din);
+      atomic_addf(grad_b + f, v);
+      if (count) atomicAdd(cnt + f, 1u);
+    }
+  };
+  for (int k = threadIdx.x; k < kLTX * kLTY * kLTZ; k += 256) {
+    const int x = x0 + k % kLTX, y = y0 + (k / kLTX) % kLTY, z = z0 + k / (kLTX * kLTY);
+    if (x >= dout.x || y >= dout.y || z >= dout.z) continue;
+    const size_t f = ((size_t)z * dout.y + y) * dout.x + x;
+    float lod;
+    const float3 p = lut_pos(lut, L, cam, f, x, y, z, lod);
+    const float g = og[(size_t)slot * n + f];
+    const float px = p.x - offset, py = p.y - offset, pz = p.z - offset;
+    if (fm == F_LINEAR) {
+      Vec<1> gx, gy, gz;
+      read_grad<1, BM, EXACT>(in_b, din, px, py, pz, gx, gy, gz);
+      float lx = g * gx.v[0], ly = g * gy.v[0], lz = g * gz.v[0];
+      float4* dst = reinterpret_cast<float4*>(lut_grad + ((size_t)cam * n + f) * 4);
+      if (EXACT) {
+        if (L.normalized) {
+          lx *= dout.x;
+          ly *= dout.y;
+          lz *= dout.z;
+        }
+        const float4 o = *dst;
+        *dst = make_float4(o.x + lx, o.y + ly, o.z + lz, o.w);
+      } else {
+        *dst = make_float4(lx, ly, lz, 0.f);
+      }
+    }
+    if (EXACT) {
+      if (fm == F_LINEAR) {
+        const Taps t = make_taps<BM>(px, py, pz, din);
+        if (BM == B_BORDER && !t.inside) continue;
+        const float wx[2] = {1.f - t.wx, t.wx}, wy[2] = {1.f - t.wy, t.wy}, wz[2] = {1.f - t.wz, t.wz};
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+              if (BM == B_BORDER && !(t.vx[i] & t.vy[j] & t.vz[kk])) continue;
+              const float w = wx[i] * wy[j] * wz[kk];
+              if (w == 0.f) continue;
+              emit(t.ix[i], t.iy[j], t.iz[kk], g * w, false);
+            }
+      } else {
+        size_t fi;
+        if (nearest_index<BM>(px, py, pz, din, fi)) {
+          const int ix = (int)(fi % (size_t)din.x), iy = (int)((fi / (size_t)din.x) % (size_t)din.y),
+                    iz = (int)(fi / ((size_t)din.x * din.y));
+          emit(ix, iy, iz, g, false);
+        }
+      }
+    } else {
+      // scatterGradInterpolated (as scatter_reference)
+      if (BM == B_BORDER &&
+          !((-1.f < px) & (px < (float)din.x) & (-1.f < py) & (py < (float)din.y) & (-1.f < pz) & (pz < (float)din.z)))
+        continue;
+      const float fxf = floorf(px), fyf = floorf(py), fzf = floorf(pz);
+      const float cwx = px - fxf, cwy = py - fyf, cwz = pz - fzf;
+      int fx = (int)fxf, fy = (int)fyf, fz = (int)fzf;
+      int cx = (int)ceilf(px), cy = (int)ceilf(py), cz = (int)ceilf(pz);
+      if (BM == B_CLAMP) {
+        fx = clampi(fx, 0, din.x - 1); cx = clampi(cx, 0, din.x - 1);
+        fy = clampi(fy, 0, din.y - 1); cy = clampi(cy, 0, din.y - 1);
+        fz = clampi(fz, 0, din.z - 1); cz = clampi(cz, 0, din.z - 1);
+      } else if (BM == B_WRAP) {
+        fx = pmodi(fx, din.x); cx = pmodi(cx, din.x);
+        fy = pmodi(fy, din.y); cy = pmodi(cy, din.y);
+        fz = pmodi(fz, din.z); cz = pmodi(cz, din.z);
+      }
+      const int ix[2] = {fx, cx}, iy[2] = {fy, cy}, iz[2] = {fz, cz};
+      const float wx[2] = {1.f - cwx, cwx}, wy[2] = {1.f - cwy, cwy}, wz[2] = {1.f - cwz, cwz};
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            if (BM == B_BORDER &&
+                !((ix[i] >= 0) & (ix[i] < din.x) & (iy[j] >= 0) & (iy[j] < din.y) & (iz[kk] >= 0) & (iz[kk] < din.z)))
+              continue;
+            emit(ix[i], iy[j], iz[kk], g * (wx[i] * wy[j] * wz[kk]), true);
+          }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kLRegion; k += 256) {
+    const double v = acc[k];
+    const unsigned c = EXACT ? 0u : cts[k];
+    if (v == 0.0 && c == 0u) continue;
+    const int x = rx0 + k % kLRX, y = ry0 + (k / kLRX) % kLRY, z = rz0 + k / (kLRX * kLRY);
+    // taps always land inside the grid (BORDER drops, CLAMP/WRAP fold them in)
+    if (((unsigned)x < (unsigned)din.x) & ((unsigned)y < (unsigned)din.y) & ((unsigned)z < (unsigned)din.z)) {
+      const size_t f = flat3(x, y, z, din);
+      if (v != 0.0) atomic_addf(grad_b + f, (float)v);
+      if (c) atomicAdd(cnt + f, c);
+    }
+  }
+}
+
 // ------------------------------------------------------------------- LoD
 // kComputeLoD3D (resample_grid.cu.cc:1288-1300): position and LoD of every
 // output cell (central differences, Jacobian rows). blockIdx.y = camera.
@@ -1106,6 +1257,21 @@ hipError_t sample_lut_bwd_launch(int C, int bm, bool exact, hipStream_t st, cons
                                  const LutLaunch& L, int views, const float* lut, const float* og, Dims3 din,
                                  float* grad_b, unsigned* cnt, float* lut_grad) {
   size_t n = (size_t)L.dout.x * L.dout.y * L.dout.z;
+  KernelTimer t(st, "lut_grad");
+  if (C == 1 && L.relative && (fm == F_LINEAR || fm == F_NEAREST || !exact)) {
+    const unsigned tiles = (unsigned)(((L.dout.x + kLTX - 1) / kLTX) * ((L.dout.y + kLTY - 1) / kLTY) *
+                                      ((L.dout.z + kLTZ - 1) / kLTZ));
+    const dim3 grid(tiles, (unsigned)views);
+    GFT_DISPATCH_BM(bm, {
+      if (exact)
+        hipLaunchKernelGGL((k_sample_lut_bwd_tiled<BB, true>), grid, dim3(256), 0, st, in_b, fm, offset, L, lut, og,
+                           din, grad_b, cnt, lut_grad);
+      else
+        hipLaunchKernelGGL((k_sample_lut_bwd_tiled<BB, false>), grid, dim3(256), 0, st, in_b, fm, offset, L, lut, og,
+                           din, grad_b, cnt, lut_grad);
+    });
+    return hipGetLastError();
+  }
   GFT_DISPATCH_C(C, GFT_DISPATCH_BM(bm, {
     if (exact)
       hipLaunchKernelGGL((k_sample_lut_bwd<CC, BB, true>), grid_for(n, views), dim3(256), 0, st, in_b, fm, offset, L,

@@ -376,7 +376,7 @@ int gft_adam_step(void* stream, int32_t num_tensors, float* const* params, const
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
  * render_fwd, render_bwd_pixel, render_bwd_gather, render_bwd_scatter, quad_copy, fill,
- * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam, transform_grad_gather.
+ * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam, transform_grad_gather, lut_grad.
  */
 int gft_timer_start(const char* kernels);
 int gft_timer_read(const char* kernel, int32_t* launches, double* total_ms);

@@ -379,3 +379,25 @@ def test_transform_backward_gather_adjoint_identity(cuda):
     lhs = float((ax.detach().double().cpu() * torch.from_numpy(y).double()).sum())
     rhs = float((x.detach().double().cpu() * x.grad.double().cpu()).sum())
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+@pytest.mark.parametrize("grad_mode", ["REFERENCE", "EXACT"])
+@pytest.mark.parametrize("filt", ["NEAREST", "LINEAR", "MAX"])
+@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP", "WRAP"])
+@pytest.mark.parametrize("spread", [1.5, 8.0])
+def test_lut_backward_one_channel_tiled(cuda, grad_mode, filt, boundary, spread):
+    """One-channel relative lookups take the tiled gradient (LDS region of the
+    tile plus a 3-cell margin; taps beyond it, as with spread 8, go to global
+    atomics); EXACT MAX keeps the untiled kernel."""
+    sc, data, _ = _setup(1, seed=22)
+    lut = _lut((9, 13, 40), 2, seed=23, spread=spread)
+    rng = np.random.Generator(np.random.PCG64(24))
+    og = rng.uniform(-1, 1, size=(1, 2, 9, 13, 40, 1)).astype(np.float32)
+    kw = dict(interpolation=filt, boundary=boundary, grad_mode=grad_mode)
+    gin, glut = _ops().sample_grid_lut_grad(torch.from_numpy(data).cuda(), torch.from_numpy(og).cuda(),
+                                            torch.from_numpy(lut).cuda(), **kw)
+    p = 0 if filt != "LINEAR" else 1
+    rin, rlut = orc.sample_lut_bwd(data, lut, og, prec=p, **kw)
+    assert np.abs(rin).max() > 0
+    assert norm_err(gin.cpu().numpy(), rin) < 1e-4
+    assert norm_err(glut.cpu().numpy(), rlut) < 1e-4
