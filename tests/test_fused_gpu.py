@@ -214,3 +214,51 @@ def test_render_density_saved_buffer_is_checked(cuda):
     assert L.gft_render_density_bwd(*args(_lib.ptr(saved), n)) == 0
     torch.cuda.synchronize()
     assert torch.isfinite(gd).all()
+
+
+def test_uneven_reconstructions_in_one_graph(cuda):
+    """Independent reconstructions of uneven grid sizes and view counts (the
+    batched-uneven workload at test size) captured together in one graph:
+    every replay reproduces each reconstruction's eager loss, images and
+    gradients, and each matches the FP64 oracle."""
+    from gft.pipeline import TransportRenderStep
+    cases = [((16, 26, 18), 3, (24, 14)), ((12, 20, 14), 1, (18, 11)), ((20, 16, 24), 4, (16, 20)),
+             ((10, 10, 10), 2, (12, 12))]
+    steps, inputs, eager = [], [], []
+    for k, (grid, views, image) in enumerate(cases):
+        sc = scene(grid=grid, image=image, views=views, seed=30 + k)
+        d, v = _fields(grid, seed=40 + k)
+        tgt = orc.render_density_fwd(syn.gaussian_blobs(grid, 4, seed=50 + k), sc["M"], sc["V"], sc["P"], sc["F"],
+                                     sc["out_dhw"], sc["step"])[..., :3]
+        st = TransportRenderStep(grid, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"])
+        ins = (torch.from_numpy(d).cuda(), torch.from_numpy(v).cuda(), torch.from_numpy(tgt.astype(np.float32)).cuda())
+        st.run(*ins)
+        torch.cuda.synchronize()
+        r = orc.step_fwd_bwd(d, v, tgt, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"])
+        assert abs(st.loss.item() - r["loss"]) <= 1e-5 * abs(r["loss"])
+        assert norm_err(st.grad_density.cpu().numpy(), r["gd"]) < 1e-3
+        steps.append(st)
+        inputs.append(ins)
+        eager.append((st.loss.clone(), st.images.clone(), st.grad_density.clone(), st.grad_velocity.clone()))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for st, ins in zip(steps, inputs):
+            st.run(*ins)
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for st, ins in zip(steps, inputs):
+            st.run(*ins)
+    for _ in range(2):
+        for st in steps:
+            st.grad_density.fill_(7.0)
+            st.images.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for st, (lo, img, gd, gv) in zip(steps, eager):
+            assert torch.equal(st.images, img)
+            assert abs(st.loss.item() - lo.item()) <= 1e-6 * abs(lo.item())
+            assert norm_err(st.grad_density.cpu().numpy(), gd.cpu().numpy()) < 1e-5
+            assert norm_err(st.grad_velocity.cpu().numpy(), gv.cpu().numpy()) < 1e-5
