@@ -109,14 +109,31 @@ struct AdamLaunch {
   float lr_t, b1, b2, eps;
 };
 
-__device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, const AdamLaunch& L) {
+__device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, const AdamLaunch& L, float lr_t) {
   m = L.b1 * m + (1.f - L.b1) * g;
   v = L.b2 * v + (1.f - L.b2) * (g * g);
-  p = p - L.lr_t * m / (sqrtf(v) + L.eps);
+  p = p - lr_t * m / (sqrtf(v) + L.eps);
+}
+
+// lr_t of update t = *counter + 1 (device counter; the host form passes lr_t in L)
+__device__ __forceinline__ float adam_lr_t(const AdamLaunch& L, const long long* counter, float lr) {
+  const double t = (double)(*counter + 1);
+  return (float)(lr * sqrt(1.0 - pow((double)L.b2, t)) / (1.0 - pow((double)L.b1, t)));
+}
+
+__global__ void k_adam_count(long long* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += 1;
 }
 
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_adam(AdamLaunch L) {
+__global__ __launch_bounds__(256) void k_adam(AdamLaunch L, const long long* __restrict__ counter, float lr) {
+  // one bias correction per block from the device counter
+  __shared__ float s_lr;
+  if (counter) {
+    if (threadIdx.x == 0) s_lr = adam_lr_t(L, counter, lr);
+    __syncthreads();
+  }
+  const float lr_t = counter ? s_lr : L.lr_t;
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= L.first_quad[L.count]) return;
   int t = 0;
@@ -132,15 +149,15 @@ __global__ __launch_bounds__(256) void k_adam(AdamLaunch L) {
     const float4 gg = *reinterpret_cast<const float4*>(g + e);
     float4 mm = *reinterpret_cast<float4*>(m + e);
     float4 vv = *reinterpret_cast<float4*>(v + e);
-    adam_one(pp.x, gg.x, mm.x, vv.x, L);
-    adam_one(pp.y, gg.y, mm.y, vv.y, L);
-    adam_one(pp.z, gg.z, mm.z, vv.z, L);
-    adam_one(pp.w, gg.w, mm.w, vv.w, L);
+    adam_one(pp.x, gg.x, mm.x, vv.x, L, lr_t);
+    adam_one(pp.y, gg.y, mm.y, vv.y, L, lr_t);
+    adam_one(pp.z, gg.z, mm.z, vv.z, L, lr_t);
+    adam_one(pp.w, gg.w, mm.w, vv.w, L, lr_t);
     *reinterpret_cast<float4*>(p + e) = pp;
     *reinterpret_cast<float4*>(m + e) = mm;
     *reinterpret_cast<float4*>(v + e) = vv;
   } else {
-    for (size_t k = e; k < e + 4 && k < n; ++k) adam_one(p[k], g[k], m[k], v[k], L);
+    for (size_t k = e; k < e + 4 && k < n; ++k) adam_one(p[k], g[k], m[k], v[k], L, lr_t);
   }
 }
 
@@ -196,22 +213,25 @@ int gft_divergence_loss(void* stream, const float* x, const float* y, const floa
   return check_hip(hipGetLastError(), "divergence loss sum");
 }
 
-int gft_adam_step(void* stream, int32_t num_tensors, float* const* params, const float* const* grads,
-                  float* const* exp_avg, float* const* exp_avg_sq, const int64_t* sizes, float learning_rate,
-                  float beta1, float beta2, float epsilon, int64_t step) {
+static int adam_common(void* stream, int32_t num_tensors, float* const* params, const float* const* grads,
+                       float* const* exp_avg, float* const* exp_avg_sq, const int64_t* sizes, float learning_rate,
+                       float beta1, float beta2, float epsilon, int64_t step, int64_t* step_counter) {
   if (num_tensors < 0 || num_tensors > kMaxAdamTensors)
     return fail(GFT_ERR_INVALID_ARGUMENT, "num_tensors must be in [0, %d], is %d", kMaxAdamTensors, num_tensors);
-  if (step < 1) return fail(GFT_ERR_INVALID_ARGUMENT, "step must be >= 1 (the update count including this one)");
+  if (!step_counter && step < 1)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "step must be >= 1 (the update count including this one)");
   if (num_tensors && (!params || !grads || !exp_avg || !exp_avg_sq || !sizes))
     return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  if (!(beta1 < 1.f)) return fail(GFT_ERR_INVALID_ARGUMENT, "beta1 must be < 1");
   AdamLaunch L{};
   L.count = num_tensors;
   L.b1 = beta1;
   L.b2 = beta2;
   L.eps = epsilon;
-  const double bc1 = 1.0 - std::pow((double)beta1, (double)step), bc2 = 1.0 - std::pow((double)beta2, (double)step);
-  if (bc1 == 0.0) return fail(GFT_ERR_INVALID_ARGUMENT, "beta1 must be < 1");
-  L.lr_t = (float)(learning_rate * std::sqrt(bc2) / bc1);
+  if (!step_counter) {
+    const double bc1 = 1.0 - std::pow((double)beta1, (double)step), bc2 = 1.0 - std::pow((double)beta2, (double)step);
+    L.lr_t = (float)(learning_rate * std::sqrt(bc2) / bc1);
+  }
   bool vec = true;
   L.first_quad[0] = 0;
   for (int t = 0; t < num_tensors; ++t) {
@@ -229,13 +249,36 @@ int gft_adam_step(void* stream, int32_t num_tensors, float* const* params, const
       if (reinterpret_cast<uintptr_t>(ptr) & 15) vec = false;
   }
   const size_t quads = L.first_quad[num_tensors];
-  if (!quads) return GFT_OK;
   hipStream_t s = (hipStream_t)stream;
-  KernelTimer t(s, "adam");
-  dim3 grid((unsigned)((quads + 255) / 256));
-  if (vec) hipLaunchKernelGGL(k_adam<true>, grid, dim3(256), 0, s, L);
-  else hipLaunchKernelGGL(k_adam<false>, grid, dim3(256), 0, s, L);
-  return check_hip(hipGetLastError(), "adam launch");
+  const long long* counter = reinterpret_cast<const long long*>(step_counter);
+  if (quads) {
+    KernelTimer t(s, "adam");
+    dim3 grid((unsigned)((quads + 255) / 256));
+    if (vec) hipLaunchKernelGGL(k_adam<true>, grid, dim3(256), 0, s, L, counter, learning_rate);
+    else hipLaunchKernelGGL(k_adam<false>, grid, dim3(256), 0, s, L, counter, learning_rate);
+    int rc = check_hip(hipGetLastError(), "adam launch");
+    if (rc) return rc;
+  }
+  if (step_counter) {
+    hipLaunchKernelGGL(k_adam_count, dim3(1), dim3(64), 0, s, reinterpret_cast<long long*>(step_counter));
+    return check_hip(hipGetLastError(), "adam step count");
+  }
+  return GFT_OK;
+}
+
+int gft_adam_step(void* stream, int32_t num_tensors, float* const* params, const float* const* grads,
+                  float* const* exp_avg, float* const* exp_avg_sq, const int64_t* sizes, float learning_rate,
+                  float beta1, float beta2, float epsilon, int64_t step) {
+  return adam_common(stream, num_tensors, params, grads, exp_avg, exp_avg_sq, sizes, learning_rate, beta1, beta2,
+                     epsilon, step, nullptr);
+}
+
+int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params, const float* const* grads,
+                         float* const* exp_avg, float* const* exp_avg_sq, const int64_t* sizes, float learning_rate,
+                         float beta1, float beta2, float epsilon, int64_t* step_counter) {
+  if (!step_counter) return fail(GFT_ERR_INVALID_ARGUMENT, "null step counter");
+  return adam_common(stream, num_tensors, params, grads, exp_avg, exp_avg_sq, sizes, learning_rate, beta1, beta2,
+                     epsilon, 0, step_counter);
 }
 
 }  // extern "C"

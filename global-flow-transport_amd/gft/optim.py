@@ -92,11 +92,17 @@ class AdamOptimizer(object):
     place, 8 tensors per launch; the moments live here, zero-initialised, and
     one step counter serves all vars as TF's beta-power accumulators do."""
 
-    def __init__(self, learning_rate=0.001, beta1=0.9, beta2=0.999, epsilon=1e-8):
+    def __init__(self, learning_rate=0.001, beta1=0.9, beta2=0.999, epsilon=1e-8, device_step=False):
+        """device_step: keep the update count in a device counter (read and
+        advanced by the kernels), so that a captured graph of the optimisation
+        step advances it on every replay; ``iterations`` then counts host
+        calls only."""
         self.learning_rate = learning_rate
         self.beta1, self.beta2, self.epsilon = float(beta1), float(beta2), float(epsilon)
         self.iterations = 0
         self._slots = {}
+        self.device_step = bool(device_step)
+        self._counter = None
 
     def _slot(self, var):
         key = id(var)
@@ -128,7 +134,26 @@ class AdamOptimizer(object):
             arr = lambda ts: (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])  # noqa: E731
             sizes = (ctypes.c_int64 * n)(*[v.numel() for v in vs])
             with torch.no_grad():
-                _lib.check(L.gft_adam_step(_lib.stream_of(vs[0]), n, arr(vs), arr(grads), arr([m for m, _ in ms]),
-                                           arr([s for _, s in ms]), sizes, float(lr), self.beta1, self.beta2,
-                                           self.epsilon, self.iterations))
+                if self.device_step:
+                    groups = (len(pairs) + _MAX_ADAM_TENSORS - 1) // _MAX_ADAM_TENSORS
+                    if self._counter is None:
+                        self._counter = torch.zeros(groups, dtype=torch.int64, device=vs[0].device)
+                    if groups > self._counter.numel():
+                        raise _lib.InvalidArgumentError("device_step keeps one counter per group of 8 variables: "
+                                                        "pass the same variables on every call")
+                    # one counter per launch group, all advancing together
+                    cnt = self._counter[first // _MAX_ADAM_TENSORS:]
+                    _lib.check(L.gft_adam_step_device(_lib.stream_of(vs[0]), n, arr(vs), arr(grads),
+                                                      arr([m for m, _ in ms]), arr([s for _, s in ms]), sizes,
+                                                      float(lr), self.beta1, self.beta2, self.epsilon,
+                                                      _lib.ptr(cnt)))
+                else:
+                    _lib.check(L.gft_adam_step(_lib.stream_of(vs[0]), n, arr(vs), arr(grads),
+                                               arr([m for m, _ in ms]), arr([s for _, s in ms]), sizes, float(lr),
+                                               self.beta1, self.beta2, self.epsilon, self.iterations))
         return self.iterations
+
+    @property
+    def device_iterations(self):
+        """The device update count (device_step only)."""
+        return int(self._counter[0].item()) if self._counter is not None else 0

@@ -366,6 +366,15 @@ int gft_divergence_loss(void* stream, const float* x, const float* y, const floa
 int gft_adam_step(void* stream, int32_t num_tensors, float* const* params, const float* const* grads,
                   float* const* exp_avg, float* const* exp_avg_sq, const int64_t* sizes, float learning_rate,
                   float beta1, float beta2, float epsilon, int64_t step);
+/*
+ * As gft_adam_step with the update count on the device: t = *step_counter + 1
+ * is read by the update kernel and the counter is incremented after it, so a
+ * captured graph of the optimisation step advances t on every replay.
+ * step_counter: one device int64, 0 before the first update.
+ */
+int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params, const float* const* grads,
+                         float* const* exp_avg, float* const* exp_avg_sq, const int64_t* sizes, float learning_rate,
+                         float beta1, float beta2, float epsilon, int64_t* step_counter);
 
 /*
  * Kernel timers (instrumentation; no counterpart in the reference, which

@@ -167,6 +167,12 @@ def test_validation_returns_before_device_work(lib):
     assert lib.gft_adam_step(null, 1, ptrs, ptrs, ptrs, ptrs, sizes, 1e-3, 0.9, 0.999, 1e-8, 1) == \
         _lib.GFT_ERR_INVALID_ARGUMENT and b"tensor 0" in lib.gft_last_error()
     assert lib.gft_adam_step(null, 0, None, None, None, None, None, 1e-3, 0.9, 0.999, 1e-8, 1) == _lib.GFT_OK
+    # the device-counter form needs its counter, and keeps the tensor limit
+    assert lib.gft_adam_step_device(null, 0, None, None, None, None, None, 1e-3, 0.9, 0.999, 1e-8, None) == \
+        _lib.GFT_ERR_INVALID_ARGUMENT and b"step counter" in lib.gft_last_error()
+    cnt = (ctypes.c_int64 * 1)()
+    assert lib.gft_adam_step_device(null, 9, ptrs, ptrs, ptrs, ptrs, sizes, 1e-3, 0.9, 0.999, 1e-8, cnt) == \
+        _lib.GFT_ERR_INVALID_ARGUMENT
     # folded image loss: an unknown loss kind is rejected before any launch
     light = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
     rc = lib.gft_render_density_bwd_loss(null, p16, _i64(1, 4, 4, 4, 1), vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), light,

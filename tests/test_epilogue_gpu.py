@@ -127,3 +127,48 @@ def test_adam_multi_tensor(cuda):
         m = opt.get_slot(p, "m")
         assert np.allclose(m.cpu().numpy(), mr, rtol=1e-5, atol=1e-7)
     assert big[0].item() == 0.0  # the element before the view is untouched
+
+
+def test_optimisation_step_in_one_graph(cuda):
+    """The fused step and the Adam update captured in one graph with the
+    device step counter: replays continue the eager sequence (same
+    parameters as an eager run of the same number of updates with the host
+    counter)."""
+    from gft.optim import AdamOptimizer
+    from gft.pipeline import TransportRenderStep
+    grid = (16, 26, 18)
+    sc = scene(grid=grid, image=(24, 14), views=3, seed=8)
+    d, v = _fields(grid, seed=12)
+    tgt = torch.from_numpy(orc.render_density_fwd(syn.gaussian_blobs(grid, 5, seed=3), sc["M"], sc["V"], sc["P"],
+                                                  sc["F"], sc["out_dhw"], sc["step"])[..., :3].astype(np.float32)).cuda()
+
+    def make():
+        st = TransportRenderStep(grid, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], loss="AE")
+        return st, _t(d), _t(v)
+
+    def iterate(st, d0, vel, opt):
+        st.run(d0, vel, tgt)
+        opt.apply_gradients([(st.grad_density, d0), (st.grad_velocity, vel)])
+
+    st, d0, vel = make()
+    opt = AdamOptimizer(0.01, device_step=True)
+    iterate(st, d0, vel, opt)  # eager first update (allocates the moments and the counter)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        iterate(st, d0, vel, opt)
+    for _ in range(3):
+        g.replay()
+    torch.cuda.synchronize()
+    assert opt.device_iterations == 4
+
+    rs, rd0, rvel = make()
+    ropt = AdamOptimizer(0.01)
+    for _ in range(4):
+        iterate(rs, rd0, rvel, ropt)
+    torch.cuda.synchronize()
+    assert norm_err(d0.cpu().numpy(), rd0.cpu().numpy()) < 1e-5
+    assert norm_err(vel.cpu().numpy(), rvel.cpu().numpy()) < 1e-5
+    assert not torch.equal(rd0, _t(d))  # the parameters moved
