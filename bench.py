@@ -31,6 +31,13 @@ the fused shadow op against the op sequence it replaces.
 + Adam at c2: the fused kernels against torch autograd / torch.optim.Adam over this
 build's render kernels.
 
+``--precision`` instead runs workload c5 (200x356x200, 8 views of 540x960)
+with the density and velocity stored in bfloat16: ms per fused step against
+float32 storage, and the errors of images, loss and gradients against the
+FP64 oracle on the bfloat16 values (the float32 arithmetic) and on the
+unrounded inputs (arithmetic and storage), for the BORDER render's
+deterministic segmented gather and the CLAMP render's float32-atomic scatter.
+
 ``--chain`` instead measures the global transport chain (workload c3): an
 initial density advected through 20 per-frame velocity fields, every frame
 rendered to 5 views plus one unseen view and compared with its targets, the
@@ -459,6 +466,84 @@ def epilogue_bench(args):
                  "alg_bytes": nparam * 28}}))
 
 
+def precision_bench(args):
+    """One JSON line for the bfloat16-storage step at c5 (see the module
+    docstring). Progress goes to stderr: each FP64 oracle step takes tens of
+    seconds on the host cores."""
+    import numpy as np
+    import torch
+
+    from gft.pipeline import TransportRenderStep, _Cameras, render_density_forward
+    from oracle import oracle as orc
+
+    def say(msg):
+        print("precision: " + msg, file=sys.stderr, flush=True)
+
+    inp = build_inputs("c5")
+    say("inputs built")
+    dev = torch.device("cuda:0")
+    d32 = torch.from_numpy(inp["d0"]).to(dev)
+    v32 = torch.from_numpy(inp["vel"]).to(dev)
+    d16, v16 = d32.to(torch.bfloat16), v32.to(torch.bfloat16)
+    cams = _Cameras(inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"])
+    target = render_density_forward(torch.from_numpy(inp["d_target"]).to(dev), cams, inp["step"])[..., :3]
+    target = target.contiguous()
+    tgt64 = target.cpu().numpy().astype(np.float64)
+    rounded = (d16.float().cpu().numpy().astype(np.float64), v16.float().cpu().numpy().astype(np.float64))
+    exact = (inp["d0"].astype(np.float64), inp["vel"].astype(np.float64))
+    cam_args = (inp["grid"], inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"])
+
+    def timed(step, d, v):
+        for _ in range(args.warmup):
+            step.run(d, v, target)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(args.steps):
+            step.run(d, v, target)
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / args.steps
+
+    def nerr(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+    def frac(a, b, tol=1e-3):
+        return float(np.mean(np.abs(a - b) > tol * np.abs(b).max()))
+
+    res = {}
+    for boundary, reduction in (("BORDER", "deterministic segmented gather"), ("CLAMP", "float32-atomic scatter")):
+        step16 = TransportRenderStep(*cam_args, boundary=boundary, storage="bfloat16")
+        step32 = TransportRenderStep(*cam_args, boundary=boundary)
+        ms16, ms32 = timed(step16, d16, v16), timed(step32, d32, v32)
+        loss, gd, gv = step16.run(d16, v16, target)
+        torch.cuda.synchronize()
+        got = dict(loss=float(loss.item()), img=step16.images.cpu().numpy(), gd=gd.cpu().numpy(),
+                   gv=gv.cpu().numpy())
+        say("%s timed: %.3f ms (bf16), %.3f ms (fp32)" % (boundary, ms16, ms32))
+        errs = {}
+        for name, (d64, v64) in (("vs_fp64_on_bf16_values", rounded), ("vs_fp64_on_unrounded_inputs", exact)):
+            r = orc.step_fwd_bwd(d64, v64, tgt64, inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"],
+                                 boundary_render=boundary)
+            errs[name] = {"images": nerr(got["img"], r["img"]), "loss": abs(got["loss"] - r["loss"]) / abs(r["loss"]),
+                          "grad_density": nerr(got["gd"], r["gd"]), "grad_velocity": nerr(got["gv"], r["gv"]),
+                          "grad_density_frac_beyond_1e-3": frac(got["gd"], r["gd"]),
+                          "grad_velocity_frac_beyond_1e-3": frac(got["gv"], r["gv"])}
+            say("%s oracle %s done" % (boundary, name))
+        res[boundary] = {"reduction": reduction, "ms_bf16": ms16, "ms_fp32": ms32, "errors": errs}
+        del step16, step32
+        torch.cuda.empty_cache()
+    nvs = TransportRenderStep(*cam_args, storage="bfloat16").voxel_samples()
+    ms = res["BORDER"]["ms_bf16"]
+    print(json.dumps({
+        "metric": "voxel-samples/s fwd+bwd, density and velocity stored in bfloat16", "value": nvs / (ms * 1e-3),
+        "unit": "voxel-samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "dtype": "bf16 storage, f32 arithmetic", "data": "synthetic",
+        "config": {"workload": "c5: grid 200x356x200 (x,y,z), 8 views 540x960 px, %d samples/ray, MacCormack+"
+                               "MC_SMOOTH, L2 loss" % inp["S"], "voxel_samples_per_step": nvs},
+        "boundaries": res}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -473,6 +558,7 @@ def main():
     ap.add_argument("--ops", action="store_true")
     ap.add_argument("--shadow", action="store_true")
     ap.add_argument("--epilogue", action="store_true")
+    ap.add_argument("--precision", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -491,6 +577,9 @@ def main():
         return
     if args.epilogue:
         epilogue_bench(args)
+        return
+    if args.precision:
+        precision_bench(args)
         return
 
     import numpy as np

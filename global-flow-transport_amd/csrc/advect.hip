@@ -22,6 +22,11 @@
 // Trilinear derivatives use the floor/floor+1 corners (right derivative at
 // integer positions) with CLAMP indices, so the derivative vanishes where the
 // clamped read is constant.
+//
+// Storage: the density and the velocity are read as float or as bfloat16
+// (uint16_t bit patterns, the upper half of a float); every value is widened
+// to float on load and all arithmetic, the intermediate f, the output and the
+// gradients are float.
 #include "gft_device.hpp"
 
 namespace gft {
@@ -29,8 +34,14 @@ namespace gft {
 
 enum { CLAMP_NONE = 0, CLAMP_MC_SMOOTH = 1, CLAMP_NEGATIVE = 2 };
 
-__device__ __forceinline__ float3 load_vel(const float* __restrict__ vel, size_t i) {
-  return make_float3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+__device__ __forceinline__ float ld(const float* __restrict__ p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ld(const uint16_t* __restrict__ p, size_t i) {
+  return __uint_as_float((unsigned)p[i] << 16);
+}
+
+template <typename T>
+__device__ __forceinline__ float3 load_vel(const T* __restrict__ vel, size_t i) {
+  return make_float3(ld(vel, 3 * i), ld(vel, 3 * i + 1), ld(vel, 3 * i + 2));
 }
 
 // Clamped trilinear read of a scalar grid plus its 8 corner values.
@@ -40,7 +51,8 @@ struct Tri {
   int ix[2], iy[2], iz[2];
 };
 
-__device__ __forceinline__ Tri tri_clamp(const float* __restrict__ buf, Dims3 d, float px, float py, float pz) {
+template <typename T>
+__device__ __forceinline__ Tri tri_clamp(const T* __restrict__ buf, Dims3 d, float px, float py, float pz) {
   Tri t;
   float fxf = floorf(px), fyf = floorf(py), fzf = floorf(pz);
   t.wx = px - fxf; t.wy = py - fyf; t.wz = pz - fzf;
@@ -53,7 +65,7 @@ __device__ __forceinline__ Tri tri_clamp(const float* __restrict__ buf, Dims3 d,
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int i = 0; i < 2; ++i) t.v[k][j][i] = buf[flat3(t.ix[i], t.iy[j], t.iz[k], d)];
+      for (int i = 0; i < 2; ++i) t.v[k][j][i] = ld(buf, flat3(t.ix[i], t.iy[j], t.iz[k], d));
   return t;
 }
 
@@ -86,7 +98,8 @@ __device__ __forceinline__ void voxel_of(size_t i, Dims3 d, int& x, int& y, int&
 }
 
 // pass 1: f = trilinear(d, x - v dt)
-__global__ __launch_bounds__(256) void k_advect_sl(const float* __restrict__ dens, const float* __restrict__ vel,
+template <typename T>
+__global__ __launch_bounds__(256) void k_advect_sl(const T* __restrict__ dens, const T* __restrict__ vel,
                                                    float* __restrict__ f, AdvectDims a, size_t n) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -114,7 +127,8 @@ __global__ __launch_bounds__(256) void k_advect_o1_out(const float* __restrict__
 }
 
 // pass 2: MacCormack correction with the MC_SMOOTH revert clamp
-__global__ __launch_bounds__(256) void k_advect_mc(const float* __restrict__ dens, const float* __restrict__ vel,
+template <typename T>
+__global__ __launch_bounds__(256) void k_advect_mc(const T* __restrict__ dens, const T* __restrict__ vel,
                                                    const float* __restrict__ f, float* __restrict__ out,
                                                    uint8_t* __restrict__ mask, AdvectDims a, size_t n) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -125,7 +139,7 @@ __global__ __launch_bounds__(256) void k_advect_mc(const float* __restrict__ den
   float fx = f[i];
   Tri tb = tri_clamp(f, a.d, x + v.x * a.dt, y + v.y * a.dt, z + v.z * a.dt);
   float b = tri_value(tb);
-  float c = fx + 0.5f * (dens[i] - b);
+  float c = fx + 0.5f * (ld(dens, i) - b);
   uint8_t m = 0;
   if (a.clamp == CLAMP_MC_SMOOTH) {
     // corner extrema with the reference's ceil corners (ceil == floor at
@@ -143,7 +157,7 @@ __global__ __launch_bounds__(256) void k_advect_mc(const float* __restrict__ den
       for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int ii = 0; ii < 2; ++ii) {
-          float s = dens[flat3(ix[ii], iy[j], iz[k], a.d)];
+          float s = ld(dens, flat3(ix[ii], iy[j], iz[k], a.d));
           mn = fminf(mn, s);
           mx = fmaxf(mx, s);
         }
@@ -222,7 +236,8 @@ __device__ __forceinline__ bool tile_voxel(int k, Dims3 d, int x0, int y0, int z
 
 // backward A (MacCormack only): scatter of the back-warp gradient into gf,
 // direct density term, velocity term of the back warp
-__global__ __launch_bounds__(256) void k_advect_bwd_a(const float* __restrict__ vel, const float* __restrict__ f,
+template <typename T>
+__global__ __launch_bounds__(256) void k_advect_bwd_a(const T* __restrict__ vel, const float* __restrict__ f,
                                                       const uint8_t* __restrict__ mask,
                                                       const float* __restrict__ gout, float* __restrict__ gf_acc,
                                                       float* __restrict__ gd, float* __restrict__ gv, AdvectDims a) {
@@ -256,8 +271,8 @@ __global__ __launch_bounds__(256) void k_advect_bwd_a(const float* __restrict__ 
 // backward B: total gf scattered into gd at x - v dt, velocity term of the
 // forward warp. FIRST = true when pass A did not run (order 1): gv is
 // initialised here and gd must have been cleared.
-template <bool FIRST>
-__global__ __launch_bounds__(256) void k_advect_bwd_b(const float* __restrict__ dens, const float* __restrict__ vel,
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(256) void k_advect_bwd_b(const T* __restrict__ dens, const T* __restrict__ vel,
                                                       const float* __restrict__ gout,
                                                       const float* __restrict__ gf_acc,
                                                       const uint8_t* __restrict__ mask, float* __restrict__ gd,
@@ -302,18 +317,19 @@ __global__ __launch_bounds__(256) void k_advect_bwd_b(const float* __restrict__ 
 
 static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel, float* f, float* out,
-                             uint8_t* mask, const AdvectDims& a) {
+template <typename T>
+static hipError_t advect_fwd_typed(hipStream_t s, const T* dens, const T* vel, float* f, float* out, uint8_t* mask,
+                                   const AdvectDims& a) {
   size_t n = size_t(a.d.x) * a.d.y * a.d.z;
   {
     KernelTimer t(s, "advect_sl");
-    hipLaunchKernelGGL(k_advect_sl, grid1(n), dim3(256), 0, s, dens, vel, f, a, n);
+    hipLaunchKernelGGL(k_advect_sl<T>, grid1(n), dim3(256), 0, s, dens, vel, f, a, n);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.order == 2) {
     KernelTimer t(s, "advect_mc");
-    hipLaunchKernelGGL(k_advect_mc, grid1(n), dim3(256), 0, s, dens, vel, f, out, mask, a, n);
+    hipLaunchKernelGGL(k_advect_mc<T>, grid1(n), dim3(256), 0, s, dens, vel, f, out, mask, a, n);
   } else {
     KernelTimer t(s, "advect_o1_out");
     hipLaunchKernelGGL(k_advect_o1_out, grid1(n), dim3(256), 0, s, f, out, mask, a, n);
@@ -321,25 +337,42 @@ hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel,
   return hipGetLastError();
 }
 
-hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
-                             const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a) {
+template <typename T>
+static hipError_t advect_bwd_typed(hipStream_t s, const T* dens, const T* vel, const float* f, const uint8_t* mask,
+                                   const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a) {
   const dim3 tiles((unsigned)((a.d.x + kTX - 1) / kTX), (unsigned)((a.d.y + kTY - 1) / kTY),
                    (unsigned)((a.d.z + kTZ - 1) / kTZ));
   if (a.order == 2) {
     {
       KernelTimer t(s, "advect_bwd_a");
-      hipLaunchKernelGGL(k_advect_bwd_a, tiles, dim3(256), 0, s, vel, f, mask, gout, gf_acc, gd, gv, a);
+      hipLaunchKernelGGL(k_advect_bwd_a<T>, tiles, dim3(256), 0, s, vel, f, mask, gout, gf_acc, gd, gv, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     KernelTimer t(s, "advect_bwd_b");
-    hipLaunchKernelGGL(k_advect_bwd_b<false>, tiles, dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a);
+    hipLaunchKernelGGL((k_advect_bwd_b<T, false>), tiles, dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a);
   } else {
     // gd must be zero on entry (the caller clears it)
     KernelTimer t(s, "advect_bwd_b");
-    hipLaunchKernelGGL(k_advect_bwd_b<true>, tiles, dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a);
+    hipLaunchKernelGGL((k_advect_bwd_b<T, true>), tiles, dim3(256), 0, s, dens, vel, gout, gf_acc, mask, gd, gv, a);
   }
   return hipGetLastError();
+}
+
+// dens and vel are float, or bfloat16 bit patterns with `bf16`
+hipError_t advect_fwd_launch(hipStream_t s, const void* dens, const void* vel, bool bf16, float* f, float* out,
+                             uint8_t* mask, const AdvectDims& a) {
+  if (bf16)
+    return advect_fwd_typed(s, (const uint16_t*)dens, (const uint16_t*)vel, f, out, mask, a);
+  return advect_fwd_typed(s, (const float*)dens, (const float*)vel, f, out, mask, a);
+}
+
+hipError_t advect_bwd_launch(hipStream_t s, const void* dens, const void* vel, bool bf16, const float* f,
+                             const uint8_t* mask, const float* gout, float* gf_acc, float* gd, float* gv,
+                             const AdvectDims& a) {
+  if (bf16)
+    return advect_bwd_typed(s, (const uint16_t*)dens, (const uint16_t*)vel, f, mask, gout, gf_acc, gd, gv, a);
+  return advect_bwd_typed(s, (const float*)dens, (const float*)vel, f, mask, gout, gf_acc, gd, gv, a);
 }
 
 }  // namespace gft

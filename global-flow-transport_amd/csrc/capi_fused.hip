@@ -8,10 +8,11 @@
 namespace gft {
 
 
-hipError_t advect_fwd_launch(hipStream_t s, const float* dens, const float* vel, float* f, float* out, uint8_t* mask,
+hipError_t advect_fwd_launch(hipStream_t s, const void* dens, const void* vel, bool bf16, float* f, float* out,
+                             uint8_t* mask, const AdvectDims& a);
+hipError_t advect_bwd_launch(hipStream_t s, const void* dens, const void* vel, bool bf16, const float* f,
+                             const uint8_t* mask, const float* gout, float* gf_acc, float* gd, float* gv,
                              const AdvectDims& a);
-hipError_t advect_bwd_launch(hipStream_t s, const float* dens, const float* vel, const float* f, const uint8_t* mask,
-                             const float* gout, float* gf_acc, float* gd, float* gv, const AdvectDims& a);
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
                                      const RayLaunch& L, const RenderParams& rp, const RenderSaved& sv);
 size_t render_tiles_per_view(const RayDims& rd);
@@ -112,8 +113,12 @@ size_t gft_advect_saved_bytes(const int64_t shape[5]) {
   return align256(vox * sizeof(float)) + align256(vox);
 }
 
-int gft_advect_fwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
-                   int32_t order, int32_t clamp, float* output, void* saved, size_t saved_bytes) {
+}  // extern "C"
+
+namespace gft {
+
+static int advect_fwd_common(void* stream, const void* input, const int64_t shape[5], const void* velocity, bool bf16,
+                             float dt, int32_t order, int32_t clamp, float* output, void* saved, size_t saved_bytes) {
   int rc;
   if ((rc = check_scalar_grid(shape)) || (rc = check_advect_mode(order, clamp, dt))) return rc;
   if (!input || !velocity || !output) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
@@ -124,16 +129,13 @@ int gft_advect_fwd(void* stream, const float* input, const int64_t shape[5], con
   AdvectDims a{Dims3{(int)shape[3], (int)shape[2], (int)shape[1]}, dt, order, clamp};
   float* f = (float*)saved;
   uint8_t* mask = (uint8_t*)saved + align256(vox * sizeof(float));
-  return check_hip(advect_fwd_launch((hipStream_t)stream, input, velocity, f, output, mask, a), "advect forward");
+  return check_hip(advect_fwd_launch((hipStream_t)stream, input, velocity, bf16, f, output, mask, a),
+                   "advect forward");
 }
 
-size_t gft_advect_bwd_workspace(const int64_t shape[5]) {
-  return align256((size_t)shape[1] * shape[2] * shape[3] * sizeof(float));
-}
-
-int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
-                   int32_t order, int32_t clamp, const void* saved, const float* output_grad, float* input_grad,
-                   float* velocity_grad, void* workspace, size_t workspace_bytes) {
+static int advect_bwd_common(void* stream, const void* input, const int64_t shape[5], const void* velocity, bool bf16,
+                             float dt, int32_t order, int32_t clamp, const void* saved, const float* output_grad,
+                             float* input_grad, float* velocity_grad, void* workspace, size_t workspace_bytes) {
   int rc;
   if ((rc = check_scalar_grid(shape)) || (rc = check_advect_mode(order, clamp, dt))) return rc;
   if (!input || !velocity || !saved || !output_grad || !input_grad || !velocity_grad)
@@ -152,8 +154,41 @@ int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], con
   } else {
     if ((rc = check_hip(fill_launch(s, input_grad, vox, 0.f), "clear input_grad"))) return rc;
   }
-  return check_hip(advect_bwd_launch(s, input, velocity, f, mask, output_grad, gf_acc, input_grad, velocity_grad, a),
-                   "advect backward");
+  return check_hip(
+      advect_bwd_launch(s, input, velocity, bf16, f, mask, output_grad, gf_acc, input_grad, velocity_grad, a),
+      "advect backward");
+}
+
+}  // namespace gft
+
+extern "C" {
+
+int gft_advect_fwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
+                   int32_t order, int32_t clamp, float* output, void* saved, size_t saved_bytes) {
+  return advect_fwd_common(stream, input, shape, velocity, false, dt, order, clamp, output, saved, saved_bytes);
+}
+
+int gft_advect_fwd_bf16(void* stream, const uint16_t* input, const int64_t shape[5], const uint16_t* velocity,
+                        float dt, int32_t order, int32_t clamp, float* output, void* saved, size_t saved_bytes) {
+  return advect_fwd_common(stream, input, shape, velocity, true, dt, order, clamp, output, saved, saved_bytes);
+}
+
+size_t gft_advect_bwd_workspace(const int64_t shape[5]) {
+  return align256((size_t)shape[1] * shape[2] * shape[3] * sizeof(float));
+}
+
+int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
+                   int32_t order, int32_t clamp, const void* saved, const float* output_grad, float* input_grad,
+                   float* velocity_grad, void* workspace, size_t workspace_bytes) {
+  return advect_bwd_common(stream, input, shape, velocity, false, dt, order, clamp, saved, output_grad, input_grad,
+                           velocity_grad, workspace, workspace_bytes);
+}
+
+int gft_advect_bwd_bf16(void* stream, const uint16_t* input, const int64_t shape[5], const uint16_t* velocity,
+                        float dt, int32_t order, int32_t clamp, const void* saved, const float* output_grad,
+                        float* input_grad, float* velocity_grad, void* workspace, size_t workspace_bytes) {
+  return advect_bwd_common(stream, input, shape, velocity, true, dt, order, clamp, saved, output_grad, input_grad,
+                           velocity_grad, workspace, workspace_bytes);
 }
 
 size_t gft_render_density_fwd_workspace(const int64_t shape[5], int32_t boundary) {

@@ -24,47 +24,63 @@ from .ops import BOUNDARY, _cams, _enum, _np_ptr, _shape3
 CLAMP = {"NONE": 0, "MC_SMOOTH": 1, "NEGATIVE": 2}
 
 
-def _scalar_grid(t, name):
-    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32:
-        raise _lib.InvalidArgumentError("%s must be a float32 CUDA tensor" % name)
+def _scalar_grid(t, name, dtypes=(torch.float32,)):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype not in dtypes:
+        raise _lib.InvalidArgumentError("%s must be a %s CUDA tensor" % (name, " or ".join(str(d)[6:] for d in dtypes)))
     if t.dim() != 5 or t.shape[0] != 1 or t.shape[4] != 1:
         raise _lib.InvalidArgumentError("%s must be [1,D,H,W,1], got %s" % (name, list(t.shape)))
     return t.contiguous()
 
 
 def _velocity(v, dens):
-    if not isinstance(v, torch.Tensor) or not v.is_cuda or v.dtype != torch.float32:
-        raise _lib.InvalidArgumentError("velocity must be a float32 CUDA tensor")
+    """A centred velocity of the density's grid and storage type."""
+    if not isinstance(v, torch.Tensor) or not v.is_cuda or v.dtype != dens.dtype:
+        raise _lib.InvalidArgumentError("velocity must be a %s CUDA tensor (the density's type)" % str(dens.dtype)[6:])
     if tuple(v.shape) != tuple(dens.shape[:4]) + (3,):
         raise _lib.InvalidArgumentError("Shape mismatch: velocity %s for grid %s" % (list(v.shape), list(dens.shape)))
     return v.contiguous()
 
 
+STORAGE = {"float32": torch.float32, "bfloat16": torch.bfloat16}
+
+
+def _advect_calls(dtype):
+    """The forward and backward entry points for density/velocity storage."""
+    L = _lib.lib()
+    if dtype == torch.bfloat16:
+        return L.gft_advect_fwd_bf16, L.gft_advect_bwd_bf16
+    return L.gft_advect_fwd, L.gft_advect_bwd
+
+
 # -------------------------------------------------------------- advection
 def advect_forward(d, vel, dt=1.0, order=2, clamp="MC_SMOOTH", out=None, saved=None):
+    """d, vel float32 or bfloat16 (both the same); the output is float32."""
     L = _lib.lib()
     shape = _lib.i64_array(d.shape)
     nsaved = L.gft_advect_saved_bytes(shape)
     if saved is None:
         saved = torch.empty((nsaved,), dtype=torch.uint8, device=d.device)
     if out is None:
-        out = torch.empty_like(d)
-    _lib.check(L.gft_advect_fwd(_lib.stream_of(d), _lib.ptr(d), shape, _lib.ptr(vel), float(dt), int(order),
-                                CLAMP[clamp], _lib.ptr(out), _lib.ptr(saved), nsaved))
+        out = torch.empty(d.shape, dtype=torch.float32, device=d.device)
+    fwd, _ = _advect_calls(d.dtype)
+    _lib.check(fwd(_lib.stream_of(d), _lib.ptr(d), shape, _lib.ptr(vel), float(dt), int(order), CLAMP[clamp],
+                   _lib.ptr(out), _lib.ptr(saved), nsaved))
     return out, saved
 
 
 def advect_backward(d, vel, saved, grad_out, dt=1.0, order=2, clamp="MC_SMOOTH", gd=None, gv=None, ws=None):
+    """Gradients are float32 whatever the storage of d and vel."""
     L = _lib.lib()
     shape = _lib.i64_array(d.shape)
     nws = L.gft_advect_bwd_workspace(shape)
     if ws is None:
         ws = torch.empty((nws,), dtype=torch.uint8, device=d.device)
-    gd = torch.empty_like(d) if gd is None else gd
-    gv = torch.empty_like(vel) if gv is None else gv
-    _lib.check(L.gft_advect_bwd(_lib.stream_of(d), _lib.ptr(d), shape, _lib.ptr(vel), float(dt), int(order),
-                                CLAMP[clamp], _lib.ptr(saved), _lib.ptr(grad_out.contiguous()), _lib.ptr(gd),
-                                _lib.ptr(gv), _lib.ptr(ws), nws))
+    gd = torch.empty(d.shape, dtype=torch.float32, device=d.device) if gd is None else gd
+    gv = torch.empty(vel.shape, dtype=torch.float32, device=d.device) if gv is None else gv
+    _, bwd = _advect_calls(d.dtype)
+    _lib.check(bwd(_lib.stream_of(d), _lib.ptr(d), shape, _lib.ptr(vel), float(dt), int(order), CLAMP[clamp],
+                   _lib.ptr(saved), _lib.ptr(grad_out.float().contiguous()), _lib.ptr(gd), _lib.ptr(gv),
+                   _lib.ptr(ws), nws))
     return gd, gv
 
 
@@ -86,8 +102,10 @@ class _AdvectFn(torch.autograd.Function):
 
 def advect(d, velocity, dt=1.0, order=2, clamp="MC_SMOOTH"):
     """Warp a scalar grid [1,D,H,W,1] by a centred velocity [1,D,H,W,3]
-    (VelocityGrid.warp with a CLAMP LuT sampler); exact gradients for both."""
-    d = _scalar_grid(d, "density")
+    (VelocityGrid.warp with a CLAMP LuT sampler); exact gradients for both.
+    Both may be stored in bfloat16 (then sampled in float32); the result is
+    float32 and autograd casts the float32 gradients to the inputs' type."""
+    d = _scalar_grid(d, "density", (torch.float32, torch.bfloat16))
     velocity = _velocity(velocity, d)
     if order not in (1, 2):
         raise _lib.InvalidArgumentError("Unsupported warp order '%s'" % order)
@@ -193,8 +211,12 @@ class TransportRenderStep(object):
 
     def __init__(self, grid_dhw, matrix_m, matrix_v, matrix_p, frustum_params, output_shape, depth_step,
                  light=(1.0, 1.0, 1.0), order=2, clamp="MC_SMOOTH", dt=1.0, boundary="BORDER", device="cuda",
-                 loss="SE", background=None, loss_scale=None):
+                 loss="SE", background=None, loss_scale=None, storage="float32"):
         D, H, W = grid_dhw
+        if storage not in STORAGE:
+            raise _lib.InvalidArgumentError("storage must be one of %s" % sorted(STORAGE))
+        self.storage = STORAGE[storage]
+        self._advect_fwd, self._advect_bwd = _advect_calls(self.storage)
         self.shape = (1, D, H, W, 1)
         self.cams = _Cameras(matrix_m, matrix_v, matrix_p, frustum_params, output_shape)
         self.depth_step = float(depth_step)
@@ -243,7 +265,9 @@ class TransportRenderStep(object):
     def forward(self, d0, vel, timers=None):
         L, c, s = self._L, self.cams, _lib.stream_of(d0)
         self._mark(timers, "advect_fwd", True)
-        _lib.check(L.gft_advect_fwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,
+        if d0.dtype != self.storage or vel.dtype != self.storage:
+            raise _lib.InvalidArgumentError("d0 and vel must be %s (the step's storage)" % str(self.storage)[6:])
+        _lib.check(self._advect_fwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,
                                     _lib.ptr(self.d1), _lib.ptr(self.saved), self._nsaved))
         self._mark(timers, "advect_fwd", False)
         self._mark(timers, "render_fwd", True)
@@ -268,7 +292,7 @@ class TransportRenderStep(object):
                                                  _lib.ptr(self.grad_d1), _lib.ptr(self.loss)))
         self._mark(timers, "render_bwd", False)
         self._mark(timers, "advect_bwd", True)
-        _lib.check(L.gft_advect_bwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,
+        _lib.check(self._advect_bwd(s, _lib.ptr(d0), self._shape, _lib.ptr(vel), self.dt, self.order, self.clamp,
                                     _lib.ptr(self.saved), _lib.ptr(self.grad_d1), _lib.ptr(self.grad_density),
                                     _lib.ptr(self.grad_velocity), _lib.ptr(self.ws), self._nws))
         self._mark(timers, "advect_bwd", False)
@@ -281,12 +305,14 @@ class TransportRenderStep(object):
         return self.loss, self.grad_density, self.grad_velocity
 
     def algorithmic_bytes(self):
-        """Compulsory HBM bytes per phase at FP32 storage: each persistent
-        tensor read once and written once (intermediates excluded)."""
+        """Compulsory HBM bytes per phase: each persistent tensor read once
+        and written once (intermediates excluded); d0 and v at their storage
+        size s (4 or 2 bytes), d', the images and every gradient float32."""
         V = int(np.prod(self.shape))
         N = self.n_pixels
-        return {"advect_fwd": 20 * V, "render_fwd": 4 * V + 16 * N, "render_bwd": 8 * V + 28 * N,
-                "advect_bwd": 36 * V}
+        st = self.storage.itemsize
+        return {"advect_fwd": (4 * st + 4) * V, "render_fwd": 4 * V + 16 * N, "render_bwd": 8 * V + 28 * N,
+                "advect_bwd": (20 + 4 * st) * V}
 
     def kernel_algorithmic_bytes(self):
         """The per-phase compulsory bytes split over the kernels that touch
@@ -298,9 +324,10 @@ class TransportRenderStep(object):
         v, advect_bwd_b reads d and writes g_d and g_v."""
         V = int(np.prod(self.shape))
         N = self.n_pixels
-        return {"advect_sl": 16 * V, "advect_mc": 4 * V, "advect_o1_out": 4 * V, "render_fwd": 4 * V + 16 * N,
+        st = self.storage.itemsize
+        return {"advect_sl": 4 * st * V, "advect_mc": 4 * V, "advect_o1_out": 4 * V, "render_fwd": 4 * V + 16 * N,
                 "render_bwd_pixel": 28 * N, "render_bwd_gather": 8 * V, "render_bwd_scatter": 8 * V + 28 * N,
-                "advect_bwd_a": 16 * V, "advect_bwd_b": 20 * V, "quad_copy": 0, "fill": 0}
+                "advect_bwd_a": (4 + 3 * st) * V, "advect_bwd_b": (16 + st) * V, "quad_copy": 0, "fill": 0}
 
     def voxel_samples(self):
         """Trilinear evaluations of one fwd+bwd step: advection fwd a*V

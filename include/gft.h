@@ -108,6 +108,17 @@ size_t gft_advect_bwd_workspace(const int64_t shape[5]);
 int gft_advect_bwd(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
                    int32_t order, int32_t clamp, const void* saved, const float* output_grad, float* input_grad,
                    float* velocity_grad, void* workspace, size_t workspace_bytes);
+/*
+ * The same advection with the scalar and the velocity stored in bfloat16
+ * (uint16_t bit patterns, the upper half of an IEEE float): values are widened
+ * to float on load and everything else (the output, `saved`, the gradients)
+ * is float, as in gft_advect_fwd / gft_advect_bwd. Same buffer sizes.
+ */
+int gft_advect_fwd_bf16(void* stream, const uint16_t* input, const int64_t shape[5], const uint16_t* velocity,
+                        float dt, int32_t order, int32_t clamp, float* output, void* saved, size_t saved_bytes);
+int gft_advect_bwd_bf16(void* stream, const uint16_t* input, const int64_t shape[5], const uint16_t* velocity,
+                        float dt, int32_t order, int32_t clamp, const void* saved, const float* output_grad,
+                        float* input_grad, float* velocity_grad, void* workspace, size_t workspace_bytes);
 
 /*
  * Fused density render: BEER_LAMBERT raymarch of a density [1,D,H,W,1] lit

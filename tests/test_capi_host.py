@@ -182,6 +182,16 @@ def test_validation_returns_before_device_work(lib):
     # fused advection: scalar grids only
     rc = lib.gft_advect_fwd(null, null, _i64(1, 4, 4, 4, 2), null, ctypes.c_float(1.0), 2, 1, null, null, 0)
     assert rc == _lib.GFT_ERR_UNIMPLEMENTED
+    # the bfloat16-storage forms validate as the float32 ones, before any launch
+    rc = lib.gft_advect_fwd_bf16(null, null, _i64(1, 4, 4, 4, 2), null, ctypes.c_float(1.0), 2, 1, null, null, 0)
+    assert rc == _lib.GFT_ERR_UNIMPLEMENTED
+    rc = lib.gft_advect_fwd_bf16(null, null, _i64(1, 4, 4, 4, 1), null, ctypes.c_float(1.0), 2, 1, null, null, 0)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT and b"null pointer" in lib.gft_last_error()
+    rc = lib.gft_advect_fwd_bf16(null, p16, _i64(1, 4, 4, 4, 1), p16, ctypes.c_float(1.0), 3, 1, p16, p16, 1 << 20)
+    assert rc == _lib.GFT_ERR_INVALID_ARGUMENT
+    rc = lib.gft_advect_bwd_bf16(null, p16, _i64(1, 4, 4, 4, 1), p16, ctypes.c_float(1.0), 2, 1, p16, p16, p16, p16,
+                                 null, 0)
+    assert rc == _lib.GFT_ERR_WORKSPACE
 
 
 def test_ops_refuse_host_tensors():
