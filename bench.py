@@ -38,6 +38,13 @@ FP64 oracle on the bfloat16 values (the float32 arithmetic) and on the
 unrounded inputs (arithmetic and storage), for the BORDER render's
 deterministic segmented gather and the CLAMP render's float32-atomic scatter.
 
+``--batched`` instead runs workload c4: 32 independent reconstructions of
+uneven grid sizes (32^3 to 128x224x128) and view counts (1-8), each a fused
+fwd+bwd step, issued eagerly one after another, replayed from one captured
+graph on one stream, and replayed from one graph in which every
+reconstruction is a branch on its own stream (forked from and joined to the
+capture stream), so that the small reconstructions run side by side.
+
 ``--chain`` instead measures the global transport chain (workload c3): an
 initial density advected through 20 per-frame velocity fields, every frame
 rendered to 5 views plus one unseen view and compared with its targets, the
@@ -466,6 +473,103 @@ def epilogue_bench(args):
                  "alg_bytes": nparam * 28}}))
 
 
+def batched_bench(args):
+    """One JSON line for the batched-uneven workload c4 (see the module
+    docstring); the replays are checked against the eager results."""
+    import numpy as np
+    import torch
+
+    from gft import synthetic as syn
+    from gft.camera import camera_params_batch, model_matrices
+    from gft.pipeline import TransportRenderStep, _Cameras, render_density_forward
+
+    dev = torch.device("cuda:0")
+    recs = syn.batched_uneven(32, seed=7)
+    steps, inputs = [], []
+    for r in recs:
+        grid = r["grid"]
+        cams, S = syn.arc_cameras(grid, r["image"], r["views"])
+        M = model_matrices([syn.grid_transform(grid)])
+        V, P, F = camera_params_batch(cams)
+        out_dhw = (S,) + tuple(r["image"])
+        depth_step = float(cams[0].depth_step)
+        d0 = torch.from_numpy(syn.gaussian_blobs(grid, r["blobs"], seed=r["seed"])).to(dev)
+        vel = torch.from_numpy(syn.curl_noise_velocity(grid, seed=r["seed"] + 1, max_speed=r["max_speed"])).to(dev)
+        dt = torch.from_numpy(syn.gaussian_blobs(grid, r["blobs"], seed=r["seed"] + 2)).to(dev)
+        tgt = render_density_forward(dt, _Cameras(M, V, P, F, out_dhw), depth_step)[..., :3].contiguous()
+        steps.append(TransportRenderStep(grid, M, V, P, F, out_dhw, depth_step))
+        inputs.append((d0, vel, tgt))
+    print("batched: %d reconstructions built" % len(steps), file=sys.stderr, flush=True)
+    nvs = sum(st.voxel_samples() for st in steps)
+
+    def run_all():
+        for st, ins in zip(steps, inputs):
+            st.run(*ins)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / args.steps
+
+    ms_eager = timed(run_all)
+    eager = [(st.loss.clone(), st.images.clone(), st.grad_density.clone()) for st in steps]
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        run_all()
+    torch.cuda.current_stream().wait_stream(side)
+    g1 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g1):
+        run_all()
+    ms_graph = timed(g1.replay)
+
+    branches = [torch.cuda.Stream() for _ in steps]
+    for s_, st, ins in zip(branches, steps, inputs):  # warm each branch stream eagerly
+        s_.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s_):
+            st.run(*ins)
+        torch.cuda.current_stream().wait_stream(s_)
+    torch.cuda.synchronize()
+    gb = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gb):
+        cap = torch.cuda.current_stream()
+        for s_, st, ins in zip(branches, steps, inputs):
+            s_.wait_stream(cap)
+            with torch.cuda.stream(s_):
+                st.run(*ins)
+        for s_ in branches:
+            cap.wait_stream(s_)
+    ms_branches = timed(gb.replay)
+    worst_loss, images_equal, worst_gd = 0.0, True, 0.0
+    for st, (lo, img, gd) in zip(steps, eager):
+        worst_loss = max(worst_loss, abs(st.loss.item() - lo.item()) / abs(lo.item()))
+        images_equal &= bool(torch.equal(st.images, img))
+        worst_gd = max(worst_gd, float((st.grad_density - gd).abs().max() / gd.abs().max()))
+    best = min(ms_eager, ms_graph, ms_branches)
+    sizes = {}
+    for r in recs:
+        key = "x".join(str(x) for x in r["grid"][::-1])
+        sizes[key] = sizes.get(key, 0) + 1
+    print(json.dumps({
+        "metric": "voxel-samples/s fwd+bwd of 32 uneven reconstructions", "value": nvs / (best * 1e-3),
+        "unit": "voxel-samples/s", "n_gpus": 1, "steps": args.steps, "ms_per_batch": best,
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c4: 32 reconstructions, grids (x,y,z) %s, 1-8 views each, 120x214 to 256x456 px, "
+                               "MacCormack+MC_SMOOTH, L2 loss" % sizes,
+                   "views": [r["views"] for r in recs], "voxel_samples_per_batch": nvs},
+        "ms": {"eager, one stream": ms_eager, "one graph, one stream": ms_graph,
+               "one graph, a branch per reconstruction": ms_branches},
+        "replay_vs_eager": {"max_rel_loss": worst_loss, "images_equal": images_equal,
+                            "max_norm_grad_density": worst_gd}}))
+
+
 def precision_bench(args):
     """One JSON line for the bfloat16-storage step at c5 (see the module
     docstring). Progress goes to stderr: each FP64 oracle step takes tens of
@@ -559,6 +663,7 @@ def main():
     ap.add_argument("--shadow", action="store_true")
     ap.add_argument("--epilogue", action="store_true")
     ap.add_argument("--precision", action="store_true")
+    ap.add_argument("--batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -580,6 +685,9 @@ def main():
         return
     if args.precision:
         precision_bench(args)
+        return
+    if args.batched:
+        batched_bench(args)
         return
 
     import numpy as np

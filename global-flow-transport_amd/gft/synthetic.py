@@ -152,3 +152,21 @@ WORKLOADS = {
     "c3": dict(grid=(100, 178, 100), views=5, image=(480, 270), blobs=48, max_speed=2.0, unseen_yaw=90.0),
     "c5": dict(grid=(200, 356, 200), views=8, image=(960, 540), blobs=48, max_speed=2.0),
 }
+
+# Batched uneven workload c4: grid sizes (D,H,W) and the image (H,W) used with each
+C4_GRIDS = [(32, 32, 32), (48, 86, 48), (64, 112, 64), (96, 170, 96), (128, 224, 128)]
+C4_IMAGES = [(214, 120), (274, 154), (335, 188), (395, 222), (456, 256)]
+
+
+def batched_uneven(count=32, seed=7):
+    """``count`` independent reconstructions for the batched-uneven workload:
+    each draws a grid size from C4_GRIDS (with its image size from C4_IMAGES,
+    120x214 to 256x456 px) and 1-8 views, seeded. Returns a list of dicts
+    (grid, image, views, blobs, max_speed, seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(count):
+        i = int(rng.integers(0, len(C4_GRIDS)))
+        out.append(dict(grid=C4_GRIDS[i], image=C4_IMAGES[i], views=int(rng.integers(1, 9)),
+                        blobs=24 if i < 3 else 48, max_speed=1.5, seed=1000 + 10 * k))
+    return out

@@ -216,11 +216,14 @@ def test_render_density_saved_buffer_is_checked(cuda):
     assert torch.isfinite(gd).all()
 
 
-def test_uneven_reconstructions_in_one_graph(cuda):
+@pytest.mark.parametrize("branches", [False, True])
+def test_uneven_reconstructions_in_one_graph(cuda, branches):
     """Independent reconstructions of uneven grid sizes and view counts (the
-    batched-uneven workload at test size) captured together in one graph:
-    every replay reproduces each reconstruction's eager loss, images and
-    gradients, and each matches the FP64 oracle."""
+    batched-uneven workload at test size) captured together in one graph, in
+    sequence on the capture stream or each as a branch on its own stream
+    (forked from and joined to the capture stream): every replay reproduces
+    each reconstruction's eager loss, images and gradients, and each matches
+    the FP64 oracle."""
     from gft.pipeline import TransportRenderStep
     cases = [((16, 26, 18), 3, (24, 14)), ((12, 20, 14), 1, (18, 11)), ((20, 16, 24), 4, (16, 20)),
              ((10, 10, 10), 2, (12, 12))]
@@ -240,16 +243,26 @@ def test_uneven_reconstructions_in_one_graph(cuda):
         steps.append(st)
         inputs.append(ins)
         eager.append((st.loss.clone(), st.images.clone(), st.grad_density.clone(), st.grad_velocity.clone()))
-    s = torch.cuda.Stream()
-    s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
-        for st, ins in zip(steps, inputs):
+    streams = [torch.cuda.Stream() for _ in steps] if branches else [torch.cuda.Stream()] * len(steps)
+    for s, st, ins in zip(streams, steps, inputs):  # warm-up on the streams the capture uses
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
             st.run(*ins)
-    torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
-        for st, ins in zip(steps, inputs):
-            st.run(*ins)
+        if branches:
+            cap = torch.cuda.current_stream()
+            for s, st, ins in zip(streams, steps, inputs):
+                s.wait_stream(cap)
+                with torch.cuda.stream(s):
+                    st.run(*ins)
+            for s in streams:
+                cap.wait_stream(s)
+        else:
+            for st, ins in zip(steps, inputs):
+                st.run(*ins)
     for _ in range(2):
         for st in steps:
             st.grad_density.fill_(7.0)
