@@ -83,14 +83,14 @@ def build_inputs(name):
                 step=float(cams[0].depth_step), out_dhw=(S,) + tuple(image))
 
 
-def cpu_baseline(inp, target_np):
+def cpu_baseline(inp, target_np, boundary="BORDER"):
     """The FP32 oracle (C++/OpenMP restatement of the reference ops) timed on
     one full fwd+bwd step of the same workload."""
     from oracle import oracle as orc
     threads = int(orc.lib().orc_num_threads())
     t0 = time.perf_counter()
     orc.step_fwd_bwd(inp["d0"], inp["vel"], target_np, inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"],
-                     inp["step"], prec=0)
+                     inp["step"], prec=0, boundary_render=boundary)
     return time.perf_counter() - t0, threads
 
 
@@ -654,6 +654,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
+    ap.add_argument("--boundary", default="BORDER", choices=["BORDER", "CLAMP", "WRAP"],
+                    help="render boundary of the fused step (the reference renderer's default is BORDER)")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -700,7 +702,8 @@ def main():
     dev = torch.device("cuda:0")
     d0 = torch.from_numpy(inp["d0"]).to(dev)
     vel = torch.from_numpy(inp["vel"]).to(dev)
-    step = TransportRenderStep(inp["grid"], inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"])
+    step = TransportRenderStep(inp["grid"], inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"],
+                               boundary=args.boundary)
     # targets: render of the second density through the same kernels
     cams = _Cameras(inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"])
     tgt_img = render_density_forward(torch.from_numpy(inp["d_target"]).to(dev), cams, inp["step"])
@@ -764,6 +767,7 @@ def main():
                                                                inp["image"][0], inp["S"]),
             "voxel_samples_per_step": nvs,
             "grad_mode": "EXACT",
+            "render_boundary": args.boundary,
         },
         "roofline": {
             "bound": "hbm",
@@ -783,7 +787,7 @@ def main():
         "wall_s_timed": wall,
     }
 
-    if not args.no_plain:
+    if not args.no_plain and args.boundary == "BORDER":  # the plain composition renders with BORDER
         from gft.plain import PlainTransportRenderStep
         plain = PlainTransportRenderStep(inp["grid"], inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"],
                                          inp["step"])
@@ -805,7 +809,7 @@ def main():
 
     if not args.no_cpu_baseline:
         tgt_np = target.cpu().numpy().astype(np.float64)
-        secs, threads = cpu_baseline(inp, tgt_np)
+        secs, threads = cpu_baseline(inp, tgt_np, args.boundary)
         result["cpu_baseline"] = {"value": nvs / secs, "unit": "voxel-samples/s", "cores": threads, "kind": "port",
                                   "sample": "one full fwd+bwd step of the same workload with the FP32 oracle "
                                             "(%.1f s)" % secs}

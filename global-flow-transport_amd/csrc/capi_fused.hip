@@ -26,7 +26,7 @@ hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const f
                                          const float* second, float* grad, float* loss, const RayDims& rd,
                                          const RayLaunch& L, const RenderParams& rp);
 hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v);
-hipError_t quad_copy_launch(hipStream_t s, const float* lin, float* out, Dims3 d);
+hipError_t quad_copy_launch(hipStream_t s, int bm, const float* lin, float* out, Dims3 d);
 size_t quad_copy_floats(Dims3 d);
 
 // Bytes of the corner-quad density copy the BORDER render kernels read.
@@ -192,7 +192,7 @@ int gft_advect_bwd_bf16(void* stream, const uint16_t* input, const int64_t shape
 }
 
 size_t gft_render_density_fwd_workspace(const int64_t shape[5], int32_t boundary) {
-  return boundary == GFT_BORDER ? quad_bytes(shape) : 0;
+  return boundary == GFT_BORDER || boundary == GFT_CLAMP ? quad_bytes(shape) : 0;
 }
 
 size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
@@ -242,8 +242,10 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
   auto pairs = view_pairs(1, num_cameras, true);
-  if (boundary == GFT_BORDER) {
-    if ((rc = check_hip(quad_copy_launch((hipStream_t)stream, density, (float*)workspace, rd.in), "quad copy")))
+  if (boundary == GFT_BORDER || boundary == GFT_CLAMP) {
+    if ((rc = check_hip(quad_copy_launch((hipStream_t)stream, boundary == GFT_CLAMP ? B_CLAMP : B_BORDER, density,
+                                         (float*)workspace, rd.in),
+                        "quad copy")))
       return rc;
     density = (const float*)workspace;
   }
@@ -260,11 +262,9 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
 
 size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                         int32_t boundary) {
-  (void)shape;
   (void)num_cameras;
   (void)output_shape;
-  (void)boundary;
-  return 0;
+  return boundary == GFT_CLAMP ? quad_bytes(shape) : 0;  // the CLAMP corner-quad copy
 }
 
 static int render_bwd_common(void* stream, const float* density, const int64_t shape[5], const float* M,
@@ -272,7 +272,7 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
                              const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
                              const float* images, const float* second, bool l2, float loss_scale,
                              int32_t loss_kind, const float* backgrounds, void* saved, size_t saved_bytes,
-                             float* density_grad, float* loss) {
+                             void* workspace, size_t workspace_bytes, float* density_grad, float* loss) {
   int rc;
   if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
   if (loss_kind != GFT_LOSS_SE && loss_kind != GFT_LOSS_AE) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid loss_kind");
@@ -307,6 +307,14 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
       return rc;
     return GFT_OK;
   }
+  const size_t need_ws = gft_render_density_bwd_workspace(shape, num_cameras, output_shape, boundary);
+  if (need_ws && (!workspace || workspace_bytes < need_ws))
+    return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
+                need_ws);
+  if (boundary == GFT_CLAMP) {
+    if ((rc = check_hip(quad_copy_launch(s, B_CLAMP, density, (float*)workspace, rd.in), "quad copy"))) return rc;
+    density = (const float*)workspace;
+  }
   if (l2 && (rc = check_hip(fill_launch(s, loss, 1, 0.f), "clear loss"))) return rc;
   if ((rc = check_hip(fill_launch(s, density_grad, rd.in_batch_stride, 0.f), "clear density_grad"))) return rc;
   for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
@@ -328,11 +336,9 @@ int gft_render_density_bwd_l2(void* stream, const float* density, const int64_t 
                               float loss_scale, void* saved, size_t saved_bytes, void* workspace,
                               size_t workspace_bytes, float* density_grad, float* loss) {
   (void)frustum;
-  (void)workspace;
-  (void)workspace_bytes;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
                            boundary, images, targets, true, loss_scale, GFT_LOSS_SE, nullptr, saved, saved_bytes,
-                           density_grad, loss);
+                           workspace, workspace_bytes, density_grad, loss);
 }
 
 int gft_render_density_bwd_loss(void* stream, const float* density, const int64_t shape[5], const float* M,
@@ -343,11 +349,9 @@ int gft_render_density_bwd_loss(void* stream, const float* density, const int64_
                                 size_t saved_bytes, void* workspace, size_t workspace_bytes, float* density_grad,
                                 float* loss) {
   (void)frustum;
-  (void)workspace;
-  (void)workspace_bytes;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
                            boundary, images, targets, true, loss_scale, loss_kind, backgrounds, saved, saved_bytes,
-                           density_grad, loss);
+                           workspace, workspace_bytes, density_grad, loss);
 }
 
 int gft_render_density_bwd(void* stream, const float* density, const int64_t shape[5], const float* M, int32_t num_M,
@@ -356,11 +360,9 @@ int gft_render_density_bwd(void* stream, const float* density, const int64_t sha
                            const float* images, const float* images_grad, void* saved, size_t saved_bytes,
                            void* workspace, size_t workspace_bytes, float* density_grad) {
   (void)frustum;
-  (void)workspace;
-  (void)workspace_bytes;
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
                            boundary, images, images_grad, false, 0.f, GFT_LOSS_SE, nullptr, saved, saved_bytes,
-                           density_grad, nullptr);
+                           workspace, workspace_bytes, density_grad, nullptr);
 }
 
 }  // extern "C"

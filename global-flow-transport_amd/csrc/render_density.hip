@@ -82,7 +82,11 @@ __device__ __forceinline__ unsigned quad_index(const QuadDims& q, unsigned ix, u
 }
 
 // one thread per stored entry (coalesced writes); padding entries of partial
-// bricks are written as zeros and never read
+// bricks are written as zeros and never read. BORDER: a corner outside the
+// grid is 0. CLAMP: corners take the clamped voxel, so that an entry at the
+// copy's edge holds the edge value in all its slots and a tap whose index is
+// clamped into the copy reads the CLAMP value whatever its weights.
+template <int BM>
 __global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin, float4* __restrict__ quad, Dims3 d,
                                                    QuadDims q) {
   const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -93,6 +97,7 @@ __global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin
   const unsigned iz = (b / (q.nbx * q.nby)) * kQBZ + (o >> (kQBXl + kQBYl));
   const int x = (int)ix - 1, y = (int)iy - 1, z = (int)iz - 1;
   auto v = [&](int xx, int yy) -> float {
+    if (BM == B_CLAMP) return lin[flat3(clampi(xx, 0, d.x - 1), clampi(yy, 0, d.y - 1), clampi(z, 0, d.z - 1), d)];
     return (((unsigned)xx < (unsigned)d.x) & ((unsigned)yy < (unsigned)d.y) & ((unsigned)z < (unsigned)d.z))
                ? lin[flat3(xx, yy, z, d)]
                : 0.f;
@@ -100,12 +105,13 @@ __global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin
   quad[j] = make_float4(v(x, y), v(x + 1, y), v(x, y + 1), v(x + 1, y + 1));
 }
 
-// Branch-free BORDER read of U consecutive samples i0, i0+dir, ... of one ray
-// (dir = +1 or -1) from the corner-quad copy: all 2U loads are issued before
-// any value is used. Samples outside [-0.5, dims-0.5) or past `last` (in walk
-// direction) read 0 and report inside = false; their addresses are clamped
-// into the copy.
-template <int U>
+// Branch-free read of U consecutive samples i0, i0+dir, ... of one ray (dir =
+// +1 or -1) from the corner-quad copy: all 2U loads are issued before any
+// value is used, with the addresses clamped into the copy. Samples past
+// `last` (in walk direction) read 0 and report inside = false; so do, with
+// BORDER, samples outside [-0.5, dims-0.5). With CLAMP every other sample is
+// inside and reads the clamped value from the CLAMP copy.
+template <int U, int BM = B_BORDER>
 __device__ __forceinline__ void read_border_batch(const float4* __restrict__ quad, Dims3 d, QuadDims q, float3 n,
                                                   float3 st, int i0, int dir, int last, float val[U],
                                                   bool inside[U]) {
@@ -118,8 +124,9 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
     const float px = fmaf(fi, st.x, n.x) - 0.5f, py = fmaf(fi, st.y, n.y) - 0.5f, pz = fmaf(fi, st.z, n.z) - 0.5f;
     const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
     const bool in_range = dir > 0 ? i <= last : i >= last;
-    inside[u] = in_range & (0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) &
-                (qz < (float)d.z);
+    inside[u] = BM == B_CLAMP ? in_range
+                              : in_range & (0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) &
+                                    (0.f <= qz) & (qz < (float)d.z);
     const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
     w[u][0] = px - fx;
     w[u][1] = py - fy;
@@ -166,8 +173,9 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // ----------------------------------------------------------------- forward
-// BORDER reads `dens` as the corner-quad copy (float4, QuadDims q); CLAMP
-// and WRAP read the x-fastest grid. With gs != nullptr (BORDER, training) the
+// BORDER and CLAMP read `dens` as the corner-quad copy of that boundary
+// (float4, QuadDims q); WRAP reads the x-fastest grid. BORDER marches the
+// clipped range of each ray, CLAMP every sample. With gs != nullptr (BORDER, training) the
 // forward also saves what the backward needs (RenderSaved): per pixel the
 // sample range and E, per sample of the wave's clipped range a value c_i at
 // samples[((slot*S + i)*H + y)*W + x] (a NaN mark for a sample outside the
@@ -181,13 +189,18 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
   const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
   const bool valid = x < rd.out_w && y < rd.out_h;
   float D = 0.f, E = 0.f;
-  if (BM == B_BORDER) {
+  if (BM == B_BORDER || BM == B_CLAMP) {
     // every lane of the wave takes part in the range reduction
     float3 nearp = make_float3(0.f, 0.f, 0.f), step = make_float3(0.f, 0.f, 0.f);
     int lo = 1, hi = 0;
     if (valid) {
       pixel_ray(v, rd, x, y, nearp, step);
-      clip_ray(nearp, step, rd.in, rd.out_d, 0.f, lo, hi);
+      if (BM == B_BORDER) {
+        clip_ray(nearp, step, rd.in, rd.out_d, 0.f, lo, hi);
+      } else {
+        lo = 0;
+        hi = rd.out_d - 1;
+      }
     }
     const size_t plane = size_t(rd.out_w) * rd.out_h;
     const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
@@ -198,7 +211,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     for (int i = wlo; i <= whi; i += kBatch) {
       float val[kBatch];
       bool in[kBatch];
-      read_border_batch<kBatch>(reinterpret_cast<const float4*>(dens), rd.in, q, nearp, step, i, 1, hi, val, in);
+      read_border_batch<kBatch, BM>(reinterpret_cast<const float4*>(dens), rd.in, q, nearp, step, i, 1, hi, val, in);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         // samples outside the grid or the lane's range read 0 and leave D and E unchanged
@@ -566,8 +579,11 @@ __device__ __forceinline__ Brick segment_brick(const float3* cn, const float3* c
   return b;
 }
 
+// `quad` is the CLAMP corner-quad copy of the density (the forward's copy):
+// a segment's 8 samples are read with all their loads in flight, and give the
+// forward's sigma_i bit for bit (the same corners, weights and lerp order).
 template <bool L2>
-__global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float* __restrict__ dens,
+__global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float4* __restrict__ quad, QuadDims q,
                                                                   const float* __restrict__ img,
                                                                   const float* __restrict__ second,
                                                                   float* __restrict__ grad, float* __restrict__ loss,
@@ -577,7 +593,6 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float* _
   const int x0 = blockIdx.x * kRTileX, y0 = blockIdx.y * kRTileY;
   const int x = x0 + (threadIdx.x % kRTileX), y = y0 + (threadIdx.x / kRTileX);
   const bool active = x < rd.out_w && y < rd.out_h;
-  const float* buf = dens + rd.in_batch_stride * v.in_batch;
   float* gout = grad + rd.in_batch_stride * v.in_batch;
   float3 cn[4], cs[4];
   {
@@ -601,20 +616,17 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float* _
     const Brick br = segment_brick(cn, cs, ia, ib, rd.in);
     const bool use_lds = br.size <= kCap;
     if (walk) {
-      for (int i = ib; i >= ia; --i) {
+      float val[kSegK];
+      bool inr[kSegK];
+      read_border_batch<kSegK, B_CLAMP>(quad, rd.in, q, nearp, step, ib, -1, ia, val, inr);
+#pragma unroll
+      for (int u = 0; u < kSegK; ++u) {
+        const int i = ib - u;
+        if (i < ia) break;
         const float fi = (float)i;
         const Taps tp = make_taps<B_CLAMP>(fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
                                            fmaf(fi, step.z, nearp.z) - 0.5f, rd.in);
-        float c[2][2][2];
-#pragma unroll
-        for (int k = 0; k < 2; ++k)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int ii = 0; ii < 2; ++ii) c[k][j][ii] = buf[flat3(tp.ix[ii], tp.iy[j], tp.iz[k], rd.in)];
-        const float v0 = lerpf(lerpf(c[0][0][0], c[0][0][1], tp.wx), lerpf(c[0][1][0], c[0][1][1], tp.wx), tp.wy);
-        const float v1 = lerpf(lerpf(c[1][0][0], c[1][0][1], tp.wx), lerpf(c[1][1][0], c[1][1][1], tp.wx), tp.wy);
-        const float sigma = rp.step * lerpf(v0, v1, tp.wz);
+        const float sigma = rp.step * val[u];
         const float e = __expf(-D);
         D = fmaxf(D - sigma, 0.f);
         const float esg = e * sigma * G;
@@ -727,16 +739,21 @@ static dim3 tile_grid(const RayDims& rd, int views) {
 
 size_t quad_copy_floats(Dims3 d) { return quad_entries(make_quad_dims(d)) * 4; }
 
-hipError_t quad_copy_launch(hipStream_t s, const float* lin, float* out, Dims3 d) {
+// bm: B_BORDER or B_CLAMP
+hipError_t quad_copy_launch(hipStream_t s, int bm, const float* lin, float* out, Dims3 d) {
   const QuadDims q = make_quad_dims(d);
   const size_t n = quad_entries(q);
   KernelTimer t(s, "quad_copy");
-  hipLaunchKernelGGL(k_quad_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin,
-                     reinterpret_cast<float4*>(out), d, q);
+  if (bm == B_CLAMP)
+    hipLaunchKernelGGL(k_quad_copy<B_CLAMP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin,
+                       reinterpret_cast<float4*>(out), d, q);
+  else
+    hipLaunchKernelGGL(k_quad_copy<B_BORDER>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin,
+                       reinterpret_cast<float4*>(out), d, q);
   return hipGetLastError();
 }
 
-// BORDER: dens is the corner-quad copy.
+// BORDER, CLAMP: dens is the corner-quad copy of that boundary.
 hipError_t render_density_fwd_launch(int bm, hipStream_t s, const float* dens, float* img, const RayDims& rd,
                                      const RayLaunch& L, const RenderParams& rp, const RenderSaved& sv) {
   const dim3 grid = tile_grid(rd, L.num_views);
@@ -788,14 +805,17 @@ hipError_t render_density_gather_launch(hipStream_t s, const RenderSaved& sv, fl
 }
 
 // CLAMP / WRAP scatter backward (grad must be zero on entry)
+// CLAMP: dens is the CLAMP corner-quad copy; WRAP: the x-fastest grid.
 hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
                                          const float* second, float* grad, float* loss, const RayDims& rd,
                                          const RayLaunch& L, const RenderParams& rp) {
   const dim3 grid = tile_grid(rd, L.num_views);
   KernelTimer t(s, "render_bwd_scatter");
   if (bm == B_CLAMP) {
-    if (l2) hipLaunchKernelGGL(k_render_density_bwd_clamp<true>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
-    else hipLaunchKernelGGL(k_render_density_bwd_clamp<false>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
+    const float4* quad = reinterpret_cast<const float4*>(dens);
+    const QuadDims q = make_quad_dims(rd.in);
+    if (l2) hipLaunchKernelGGL(k_render_density_bwd_clamp<true>, grid, dim3(256), 0, s, quad, q, img, second, grad, loss, rd, L, rp);
+    else hipLaunchKernelGGL(k_render_density_bwd_clamp<false>, grid, dim3(256), 0, s, quad, q, img, second, grad, loss, rd, L, rp);
   } else {
     if (l2) hipLaunchKernelGGL(k_render_density_bwd_wrap<true>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
     else hipLaunchKernelGGL(k_render_density_bwd_wrap<false>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);

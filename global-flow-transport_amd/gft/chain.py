@@ -64,7 +64,9 @@ class TransportChain(object):
         self.images = torch.empty((1, nV, h, w, 4), device=dev)
         self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
         self.render_saved = torch.empty((max(self._nrsaved, 1),), dtype=torch.uint8, device=dev)
-        self._nfws = L.gft_render_density_fwd_workspace(self._shape, self.boundary)
+        self._nfws = max(L.gft_render_density_fwd_workspace(self._shape, self.boundary),
+                         L.gft_render_density_bwd_workspace(self._shape, self.cams.nCam, self.cams.c_out,
+                                                            self.boundary))
         self.render_fwd_ws = torch.empty((max(self._nfws, 1),), dtype=torch.uint8, device=dev)
         self.frame_loss = None
 
@@ -90,7 +92,8 @@ class TransportChain(object):
                                                _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
                                                self.depth_step, self.boundary, _lib.ptr(self.images),
                                                _lib.ptr(target), self.loss_scale, _lib.ptr(self.render_saved),
-                                               self._nrsaved, None, 0, _lib.ptr(grad), _lib.ptr(loss)))
+                                               self._nrsaved, _lib.ptr(self.render_fwd_ws), self._nfws,
+                                               _lib.ptr(grad), _lib.ptr(loss)))
 
     # ------------------------------------------------------------------- run
     def run(self, d0, velocities, targets):

@@ -175,10 +175,12 @@ class _RenderFn(torch.autograd.Function):
         # the backward overwrites the saved buffer: work on a copy so that a
         # retained graph can run backward again
         saved = saved.clone()
+        nws = int(L.gft_render_density_bwd_workspace(shape, cams.nCam, cams.c_out, bm))
+        ws = torch.empty((max(nws, 1),), dtype=torch.uint8, device=d.device)
         _lib.check(L.gft_render_density_bwd(
             _lib.stream_of(d), _lib.ptr(d), shape, _np_ptr(cams.M), cams.nM, _np_ptr(cams.V),
             _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt, float(depth_step), bm, _lib.ptr(img),
-            _lib.ptr(g.contiguous()), _lib.ptr(saved), saved.numel(), None, 0, _lib.ptr(gd)))
+            _lib.ptr(g.contiguous()), _lib.ptr(saved), saved.numel(), _lib.ptr(ws), nws, _lib.ptr(gd)))
         return gd, None, None, None, None
 
 
@@ -251,7 +253,11 @@ class TransportRenderStep(object):
         self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
         self._nrsaved = L.gft_render_density_saved_bytes(self._shape, self.cams.nCam, self.cams.c_out, self.boundary)
         self.render_saved = torch.empty((max(self._nrsaved, 1),), dtype=torch.uint8, device=dev)
-        self._nfws = L.gft_render_density_fwd_workspace(self._shape, self.boundary)
+        # one workspace for the render forward and backward (stream-ordered): the
+        # corner-quad copy each of them writes
+        self._nfws = max(L.gft_render_density_fwd_workspace(self._shape, self.boundary),
+                         L.gft_render_density_bwd_workspace(self._shape, self.cams.nCam, self.cams.c_out,
+                                                            self.boundary))
         self.render_fwd_ws = torch.empty((max(self._nfws, 1),), dtype=torch.uint8, device=dev)
 
     PHASES = ("advect_fwd", "render_fwd", "render_bwd", "advect_bwd")
@@ -288,7 +294,8 @@ class TransportRenderStep(object):
                                                  _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
                                                  self.depth_step, self.boundary, _lib.ptr(self.images),
                                                  _lib.ptr(target), bg, self.loss_kind, self.loss_scale,
-                                                 _lib.ptr(self.render_saved), self._nrsaved, None, 0,
+                                                 _lib.ptr(self.render_saved), self._nrsaved,
+                                                 _lib.ptr(self.render_fwd_ws), self._nfws,
                                                  _lib.ptr(self.grad_d1), _lib.ptr(self.loss)))
         self._mark(timers, "render_bwd", False)
         self._mark(timers, "advect_bwd", True)

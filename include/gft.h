@@ -127,9 +127,9 @@ int gft_advect_bwd_bf16(void* stream, const uint16_t* input, const int64_t shape
  * depth). Replaces Renderer.render_density with an ambient Light
  * (renderer.py:603-651, lighting.py:23-26) over raymarch_grid_transform
  * (raymarch_grid.cu.cc:107-143) without the materialised 4-channel grid.
- * BORDER reads a corner-quad copy of the density (one float4 of x/y corner
- * neighbours per cell) that the call writes into the workspace
- * (gft_render_density_fwd_workspace; 0 bytes for CLAMP/WRAP). With BORDER a
+ * BORDER and CLAMP read a corner-quad copy of the density (one float4 of x/y
+ * corner neighbours per cell) that the call writes into the workspace
+ * (gft_render_density_fwd_workspace; 0 bytes for WRAP). With BORDER a
  * non-null `saved` buffer (gft_render_density_saved_bytes) receives every
  * ray's sample range and per-sample extinction for the backward; it may be
  * null when no backward follows.
@@ -149,8 +149,9 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
  * (raymarch_grid.cu.cc:343-372). BORDER needs the `saved` buffer that the
  * forward filled from the same density and cameras; the backward overwrites it
  * with per-sample gradients and is deterministic. CLAMP/WRAP need none.
- * gft_render_density_bwd_workspace is 0 for every boundary; the workspace
- * arguments are kept for callers that size one.
+ * CLAMP needs a workspace of gft_render_density_bwd_workspace bytes (the
+ * call writes the density's CLAMP corner-quad copy there); it is 0 bytes for
+ * BORDER and WRAP.
  */
 size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                         int32_t boundary);
