@@ -183,6 +183,51 @@ __device__ __forceinline__ Vec<C> read_linear(const float* __restrict__ buf, Dim
   return r;
 }
 
+// U trilinear reads with all 8U loads issued before any value is used (the
+// indices make_taps returns are always inside the grid, so the loads need no
+// guard): the same corners, zeroing and lerp order as read_linear, so every
+// value equals read_linear's, and a BORDER sample outside the grid reads 0.
+template <int C, int BM, int U>
+__device__ __forceinline__ void read_linear_batch(const float* __restrict__ buf, Dims3 d, const float px[U],
+                                                  const float py[U], const float pz[U], Vec<C> out[U]) {
+  Taps t[U];
+  Vec<C> c[U][2][2][2];
+#pragma unroll
+  for (int u = 0; u < U; ++u) t[u] = make_taps<BM>(px[u], py[u], pz[u], d);
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) c[u][k][j][i] = vload<C>(buf, flat3(t[u].ix[i], t[u].iy[j], t[u].iz[k], d));
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (BM == B_BORDER) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+            if (!(t[u].vx[i] & t[u].vy[j] & t[u].vz[k])) c[u][k][j][i] = vzero<C>();
+    }
+    Vec<C> r;
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch) {
+      const float v00 = lerpf(c[u][0][0][0].v[ch], c[u][0][0][1].v[ch], t[u].wx);
+      const float v01 = lerpf(c[u][0][1][0].v[ch], c[u][0][1][1].v[ch], t[u].wx);
+      const float v0 = lerpf(v00, v01, t[u].wy);
+      const float v10 = lerpf(c[u][1][0][0].v[ch], c[u][1][0][1].v[ch], t[u].wx);
+      const float v11 = lerpf(c[u][1][1][0].v[ch], c[u][1][1][1].v[ch], t[u].wx);
+      const float v1 = lerpf(v10, v11, t[u].wy);
+      r.v[ch] = lerpf(v0, v1, t[u].wz);
+    }
+    out[u] = (BM == B_BORDER && !t[u].inside) ? vzero<C>() : r;
+  }
+}
+
 // Nearest tap: index = trunc(p + 0.5) as the reference's int cast.
 template <int BM> __device__ __forceinline__ bool nearest_index(float px, float py, float pz, Dims3 d, size_t& f) {
   int ix = (int)(px + 0.5f), iy = (int)(py + 0.5f), iz = (int)(pz + 0.5f);

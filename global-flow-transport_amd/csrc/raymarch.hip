@@ -22,6 +22,9 @@ namespace gft {
 
 constexpr int kTileX = 16;
 constexpr int kTileY = 16;
+// samples read per round trip by the LINEAR ray walks (8 float4 corners per
+// C=4 sample: two samples already hold 64 values in registers)
+template <int C> constexpr int kRayBatch = C == 4 ? 2 : (C == 2 ? 4 : 8);
 
 template <int C, int FM, int BM>
 __device__ __forceinline__ Vec<C> sample_ray(const float* __restrict__ buf, Dims3 d, float3 p) {
@@ -60,11 +63,31 @@ __global__ __launch_bounds__(256) void k_raymarch_fwd(const float* __restrict__ 
   int lo = 0, hi = rd.out_d - 1;
   if (BM == B_BORDER && (FM == F_LINEAR || FM == F_NEAREST))
     clip_ray(nearp, step, rd.in, rd.out_d, FM == F_LINEAR ? 0.f : 1.f, lo, hi);
-  for (int i = lo; i <= hi; ++i) {
-    float fi = (float)i;
-    float3 p = make_float3(fmaf(fi, step.x, nearp.x), fmaf(fi, step.y, nearp.y), fmaf(fi, step.z, nearp.z));
-    Vec<C> s = sample_ray<C, FM, BM>(buf, rd.in, p);
-    blend<C, BL>(acc, s);
+  if constexpr (FM == F_LINEAR) {
+    // kRayBatch<C> samples per round trip: all their corner loads in flight
+    constexpr int U = kRayBatch<C>;
+    for (int i = lo; i <= hi; i += U) {
+      float px[U], py[U], pz[U];
+      Vec<C> s[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float fi = (float)min(i + u, hi);  // past the range: a repeat, not blended
+        px[u] = fmaf(fi, step.x, nearp.x) - 0.5f;
+        py[u] = fmaf(fi, step.y, nearp.y) - 0.5f;
+        pz[u] = fmaf(fi, step.z, nearp.z) - 0.5f;
+      }
+      read_linear_batch<C, BM, U>(buf, rd.in, px, py, pz, s);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i + u <= hi) blend<C, BL>(acc, s[u]);
+    }
+  } else {
+    for (int i = lo; i <= hi; ++i) {
+      float fi = (float)i;
+      float3 p = make_float3(fmaf(fi, step.x, nearp.x), fmaf(fi, step.y, nearp.y), fmaf(fi, step.z, nearp.z));
+      Vec<C> s = sample_ray<C, FM, BM>(buf, rd.in, p);
+      blend<C, BL>(acc, s);
+    }
   }
   vstore<C>(output + rd.out_view_stride * v.out_slot, size_t(y) * rd.out_w + x, acc);
 }
@@ -193,18 +216,41 @@ __global__ __launch_bounds__(256) void k_raymarch_bwd_walk(const float* __restri
   Vec<C> g = vload<C>(output_grad + rd.out_view_stride * v.out_slot, pix);
   const int S = rd.out_d;
   float* gview = gsamp + (size_t)vi * S * plane * C;
-  for (int i = S - 1; i >= 0; --i) {
-    float3 p;
+  constexpr int U = kRayBatch<C>;
+  auto position = [&](int i) {
     if (EXACT) {
       const float fi = (float)i;
-      p = make_float3(fmaf(fi, step.x, nearp.x), fmaf(fi, step.y, nearp.y), fmaf(fi, step.z, nearp.z));
-    } else {
-      const float fi = (float)(S - 1 - i);
-      p = make_float3(fmaf(-fi, step.x, farp.x), fmaf(-fi, step.y, farp.y), fmaf(-fi, step.z, farp.z));
+      return make_float3(fmaf(fi, step.x, nearp.x), fmaf(fi, step.y, nearp.y), fmaf(fi, step.z, nearp.z));
     }
+    const float fi = (float)(S - 1 - i);
+    return make_float3(fmaf(-fi, step.x, farp.x), fmaf(-fi, step.y, farp.y), fmaf(-fi, step.z, farp.z));
+  };
+  for (int ib = S - 1; ib >= 0; ib -= U) {
+    // the batch's samples ib, ib-1, ... are read together when it meets the
+    // clipped range; outside the range the BORDER sample reads 0 (the blend
+    // still runs)
+    Vec<C> smps[U];
+    if ((ib - U + 1 <= hi) & (ib >= lo)) {
+      float px[U], py[U], pz[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float3 q = position(max(ib - u, 0));
+        px[u] = q.x - 0.5f;
+        py[u] = q.y - 0.5f;
+        pz[u] = q.z - 0.5f;
+      }
+      read_linear_batch<C, B_BORDER, U>(buf, rd.in, px, py, pz, smps);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) smps[u] = vzero<C>();
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+    const int i = ib - u;
+    if (i < 0) break;
+    const float3 p = position(i);
     const bool in_range = (lo <= i) & (i <= hi);
-    // outside the clipped range the BORDER sample reads 0 (the blend still runs)
-    const Vec<C> smp = in_range ? sample_ray<C, F_LINEAR, B_BORDER>(buf, rd.in, p) : vzero<C>();
+    const Vec<C> smp = in_range ? smps[u] : vzero<C>();
     const Vec<C> gs = blend_grad<C, BL, EXACT>(g, smp, acc);
     if (in_range) {
       const float px = p.x - 0.5f, py = p.y - 0.5f, pz = p.z - 0.5f;
@@ -220,6 +266,7 @@ __global__ __launch_bounds__(256) void k_raymarch_bwd_walk(const float* __restri
       Vec<C> out = gs;
       if (!counted) out.v[0] = __builtin_nanf("");
       vstore<C>(gview, (size_t)i * plane + pix, out);
+    }
     }
   }
 }
