@@ -2,7 +2,7 @@
 // the reference op semantics (tum-pbs/Global-Flow-Transport, phitest/render).
 //
 // TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg may load this library, and only as the checker or as the
+// cpu_baseline leg may load this library, and only as the reference results or as the
 // reported CPU baseline. The product path never links or calls it.
 //
 // Every routine is templated on the arithmetic type T (float = the reference's

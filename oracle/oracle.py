@@ -1,7 +1,7 @@
 """Python face of the CPU oracle (test infrastructure only).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module, as the checker or as the timed CPU baseline; the product path
+this module, as the reference results or as the timed CPU baseline; the product path
 never does. Wraps liboracle.so (oracle.cpp, built by oracle/Makefile) with
 numpy arrays and restates, in numpy, the Python-level compositions of the
 reference:
