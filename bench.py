@@ -13,7 +13,7 @@ Prints one JSON line: voxel-samples/s (one 8-corner trilinear evaluation = 1
 voxel-sample; N_vs = 6V + 2 sum W*H*S per step), the HBM roofline of the
 dominant kernel from HIP events around its launches in a second pass over the
 same steps, the plain per-op FP32 path (the reference's op composition on
-this build's per-op kernels) timed on the same inputs, and a CPU baseline (the
+libgft's per-op kernels) timed on the same inputs, and a CPU baseline (the
 FP32 oracle, one full step on the host cores).
 
 ``--copy-kernel`` instead runs a 1 GiB device copy: the calibration kernel of
@@ -28,8 +28,8 @@ settings (256x196x196, LINEAR mips, gather gradient) over the c2 density:
 the fused shadow op against the op sequence it replaces.
 
 ``--epilogue`` instead times render + AE/background image loss + divergence loss
-+ Adam at c2: the fused kernels against torch autograd / torch.optim.Adam over this
-build's render kernels.
++ Adam at c2: the fused kernels against torch autograd / torch.optim.Adam over
+libgft's render kernels.
 
 ``--precision`` instead runs workload c5 (200x356x200, 8 views of 540x960)
 with the density and velocity stored in bfloat16: ms per fused step against
@@ -385,8 +385,8 @@ def epilogue_bench(args):
     velocity of the grid's shape, and one Adam update of the density and the
     three velocity components. Fused: the loss folded into the render
     backward, the divergence kernel pair and the multi-tensor Adam. Op
-    composition: torch autograd for the image loss and the divergence on this
-    build's render kernels, torch.optim.Adam (foreach) for the update."""
+    composition: torch autograd for the image loss and the divergence on
+    libgft's render kernels, torch.optim.Adam (foreach) for the update."""
     import numpy as np
     import torch
 
@@ -668,7 +668,7 @@ def main():
     ap.add_argument("--batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: libgft has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.copy_kernel:
         copy_kernel()

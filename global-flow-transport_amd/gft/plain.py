@@ -1,5 +1,5 @@
 """Plain per-op FP32 path: the reference's op composition of one
-advect -> render -> L2 step, run on this build's per-op HIP kernels. It is
+advect -> render -> L2 step, run on libgft's per-op HIP kernels. It is
 the baseline the fused step is measured against.
 
 * advection: VelocityGrid.warp (data_structures.py:870-905) as separate

@@ -4,8 +4,8 @@
 The files are written here exactly as the reference's save methods write
 them (np.savez_compressed with the density as arr_0 and the optional hull /
 inflow fields; centered_shape + vel_x/y/z for velocities; the legacy velocity
-files without centered_shape) and read back through this build's loaders,
-and the build's own writers round-trip. CPU only.
+files without centered_shape) and read back through gft.grids' loaders,
+and gft's own writers round-trip. CPU only.
 """
 import os
 
