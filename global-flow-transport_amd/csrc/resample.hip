@@ -918,26 +918,48 @@ __global__ __launch_bounds__(256) void k_sample_lut_bwd_tiled(const float* __res
 // ------------------------------------------------------------------- LoD
 // kComputeLoD3D (resample_grid.cu.cc:1288-1300): position and LoD of every
 // output cell (central differences, Jacobian rows). blockIdx.y = camera.
-__global__ __launch_bounds__(256) void k_lod_transform(Dims3 din, Dims3 dout, SampleLaunch L,
-                                                       float* __restrict__ out) {
-  size_t n = (size_t)dout.x * dout.y * dout.z;
-  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= n) return;
+// One block per kLodTX x kLodTY x kLodTZ tile of output cells: the positions
+// of the tile and a one-cell margin are mapped once into LDS (about 2 maps per
+// cell instead of 7), then every cell takes its central differences from
+// there. A neighbour's position is the same map of the same coordinates
+// ((x - 1) + 0.5 = (x + 0.5) - 1 exactly), so the result is unchanged.
+constexpr int kLodTX = 16, kLodTY = 8, kLodTZ = 4;
+__global__ __launch_bounds__(kLodTX * kLodTY * kLodTZ) void k_lod_transform(Dims3 din, Dims3 dout, SampleLaunch L,
+                                                                            float* __restrict__ out) {
+  constexpr int HX = kLodTX + 2, HY = kLodTY + 2, HZ = kLodTZ + 2, NH = HX * HY * HZ;
+  __shared__ float hp[3][NH];
   const SampleView& v = L.view[blockIdx.y];
-  int x, y, z;
-  unflat(f, dout, x, y, z);
+  const int ntx = (dout.x + kLodTX - 1) / kLodTX, nty = (dout.y + kLodTY - 1) / kLodTY;
+  const int t = (int)blockIdx.x;
+  const int x0 = (t % ntx) * kLodTX, y0 = ((t / ntx) % nty) * kLodTY, z0 = (t / (ntx * nty)) * kLodTZ;
   const float3 oinv = make_float3(1.f / dout.x, 1.f / dout.y, 1.f / dout.z);
   const float3 idims = make_float3((float)din.x, (float)din.y, (float)din.z);
-  float qx = x + 0.5f, qy = y + 0.5f, qz = z + 0.5f;
-  float3 p = map_pos(v, L.cm, qx, qy, qz, oinv, idims);
-  float3 xm = map_pos(v, L.cm, qx - 1.f, qy, qz, oinv, idims), xp = map_pos(v, L.cm, qx + 1.f, qy, qz, oinv, idims);
-  float3 ym = map_pos(v, L.cm, qx, qy - 1.f, qz, oinv, idims), yp = map_pos(v, L.cm, qx, qy + 1.f, qz, oinv, idims);
-  float3 zm = map_pos(v, L.cm, qx, qy, qz - 1.f, oinv, idims), zp = map_pos(v, L.cm, qx, qy, qz + 1.f, oinv, idims);
+  for (int k = threadIdx.x; k < NH; k += blockDim.x) {
+    const int hx = k % HX, hy = (k / HX) % HY, hz = k / (HX * HY);
+    const float3 p = map_pos(v, L.cm, (float)(x0 + hx - 1) + 0.5f, (float)(y0 + hy - 1) + 0.5f,
+                             (float)(z0 + hz - 1) + 0.5f, oinv, idims);
+    hp[0][k] = p.x;
+    hp[1][k] = p.y;
+    hp[2][k] = p.z;
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % kLodTX, ly = (threadIdx.x / kLodTX) % kLodTY, lz = threadIdx.x / (kLodTX * kLodTY);
+  const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+  if (x >= dout.x || y >= dout.y || z >= dout.z) return;
+  auto at = [&](int dx, int dy, int dz) {
+    const int k = ((lz + 1 + dz) * HY + (ly + 1 + dy)) * HX + (lx + 1 + dx);
+    return make_float3(hp[0][k], hp[1][k], hp[2][k]);
+  };
+  const float3 p = at(0, 0, 0);
+  const float3 xm = at(-1, 0, 0), xp = at(1, 0, 0), ym = at(0, -1, 0), yp = at(0, 1, 0), zm = at(0, 0, -1),
+               zp = at(0, 0, 1);
   float3 dx = make_float3((xm.x - xp.x) * 0.5f, (xm.y - xp.y) * 0.5f, (xm.z - xp.z) * 0.5f);
   float3 dy = make_float3((ym.x - yp.x) * 0.5f, (ym.y - yp.y) * 0.5f, (ym.z - yp.z) * 0.5f);
   float3 dz = make_float3((zm.x - zp.x) * 0.5f, (zm.y - zp.y) * 0.5f, (zm.z - zp.z) * 0.5f);
   float fx = len3f(dx.x, dy.x, dz.x), fy = len3f(dx.y, dy.y, dz.y), fz = len3f(dx.z, dy.z, dz.z);
   float lod = fmaxf(0.f, log2f(fmaxf(fx, fmaxf(fy, fz))));
+  const size_t n = (size_t)dout.x * dout.y * dout.z;
+  const size_t f = ((size_t)z * dout.y + y) * dout.x + x;
   *reinterpret_cast<float4*>(out + ((size_t)v.out_slot * n + f) * 4) = make_float4(p.x, p.y, p.z, lod);
 }
 
@@ -1286,8 +1308,10 @@ hipError_t sample_lut_bwd_launch(int C, int bm, bool exact, hipStream_t st, cons
 }
 
 hipError_t lod_transform_launch(hipStream_t st, Dims3 din, Dims3 dout, const SampleLaunch& L, float* out) {
-  size_t n = (size_t)dout.x * dout.y * dout.z;
-  hipLaunchKernelGGL(k_lod_transform, grid_for(n, L.num_views), dim3(256), 0, st, din, dout, L, out);
+  const unsigned tiles = (unsigned)(((dout.x + kLodTX - 1) / kLodTX) * ((dout.y + kLodTY - 1) / kLodTY) *
+                                   ((dout.z + kLodTZ - 1) / kLodTZ));
+  hipLaunchKernelGGL(k_lod_transform, dim3(tiles, (unsigned)L.num_views), dim3(kLodTX * kLodTY * kLodTZ), 0, st, din,
+                     dout, L, out);
   return hipGetLastError();
 }
 
