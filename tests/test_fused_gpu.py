@@ -56,20 +56,24 @@ def test_advect_backward(cuda, order, clamp):
     assert norm_err(tv.grad.cpu().numpy(), gv_ref) < 1e-3
 
 
-def test_render_density_forward_backward(cuda):
+@pytest.mark.parametrize("boundary", ["BORDER", "CLAMP"])
+def test_render_density_forward_backward(cuda, boundary):
+    """The autograd render op (with CLAMP both directions build the CLAMP
+    corner-quad copy in their own workspace)."""
     from gft.pipeline import render_density
     sc = scene(grid=(14, 22, 18), image=(20, 13), views=4, seed=6)
     d = sc["data"]
     td = torch.from_numpy(d).cuda().requires_grad_(True)
     light = (0.9, 1.0, 1.1)
-    img = render_density(td, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], light)
-    ref = orc.render_density_fwd(d, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], light)
+    img = render_density(td, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], light, boundary=boundary)
+    ref = orc.render_density_fwd(d, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], light,
+                                 boundary=boundary)
     assert norm_err(img.detach().cpu().numpy(), ref) < 1e-5
     rng = np.random.Generator(np.random.PCG64(3))
     g = rng.uniform(-1, 1, size=ref.shape).astype(np.float32)
     img.backward(torch.from_numpy(g).cuda())
     gref = orc.render_density_bwd(d, img.detach().cpu().numpy(), g, sc["M"], sc["V"], sc["P"], sc["F"],
-                                  sc["out_dhw"], sc["step"], light)
+                                  sc["out_dhw"], sc["step"], light, boundary=boundary)
     assert norm_err(td.grad.cpu().numpy(), gref) < 1e-4
 
 
