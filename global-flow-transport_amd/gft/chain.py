@@ -9,15 +9,19 @@ targets. The backward follows the density-gradient warping of
 optStep_sequence in BWD order (phitest/render/optimization.py:592-602,
 974-1047): lambda_f, the gradient arriving at d_f, is
 
-  EXACT      lambda_f = g_f + A_f^T lambda_{f+1}                (the true chain rule)
+  EXACT      lambda_f = g_f + A_f^T lambda_{f+1}                 (the true chain rule)
   REFERENCE  lambda_f = (1-a) g_f + a A_f^T lambda_{f+1}, lambda_{T-1} = (1-a) g_{T-1}
-             for a decay a > 0, the EXACT sum for a = 0
+             for a decay a > 0; g_f + A_f^T lambda_{f+1} for a = 0
 
 with g_f the frame's own render-loss gradient, A_f^T the advection adjoint
-and a the decay (warp_dens_grads_decay). The gradient of d_0 is
-g_0 + A_0^T lambda_1 in both modes (the reference applies own plus warped
-gradients to the first frame's density); the velocity gradient of frame f is
-the velocity part of A_f^T lambda_{f+1}.
+and a the decay (warp_dens_grads_decay). The gradient applied to a frame is
+its own g_f plus the warped term; in REFERENCE mode at decay 0 the warped term
+is divided by num_step = T-1-f, the number of frames already visited in BWD
+order (optimization.py:1017-1018), while the accumulator stays unscaled
+(:1031). So grad d_0 = g_0 + s_0 (A_0^T lambda_1) and the velocity gradient of
+frame f is s_f times the velocity part of A_f^T lambda_{f+1}, with s_f =
+1/(T-1-f) at decay 0 in REFERENCE mode and 1 otherwise; the velocity uses its
+own decay (warp_vel_grads_decay, :1019-1020), which defaults to the density's.
 
 Checkpointing: the forward keeps d_f only at every k-th frame; the backward
 recomputes each k-frame segment (advection with its saved MacCormack state,
@@ -38,7 +42,7 @@ from .pipeline import CLAMP, _Cameras
 class TransportChain(object):
     def __init__(self, grid_dhw, matrix_m, matrix_v, matrix_p, frustum_params, output_shape, depth_step,
                  light=(1.0, 1.0, 1.0), order=2, clamp="MC_SMOOTH", dt=1.0, boundary="BORDER", checkpoint_every=4,
-                 grad_mode="EXACT", decay=0.0, device="cuda"):
+                 grad_mode="EXACT", decay=0.0, device="cuda", vel_decay=None):
         D, H, W = [int(s) for s in grid_dhw]
         self.shape = (1, D, H, W, 1)
         self.cams = _Cameras(matrix_m, matrix_v, matrix_p, frustum_params, output_shape)
@@ -51,6 +55,7 @@ class TransportChain(object):
         if self.grad_mode not in ("EXACT", "REFERENCE"):
             raise _lib.InvalidArgumentError("invalid grad_mode '%s'" % grad_mode)
         self.decay = float(decay)
+        self.vel_decay = self.decay if vel_decay is None else float(vel_decay)
         self.device = torch.device(device)
         L = _lib.lib()
         self._L = L
@@ -122,8 +127,10 @@ class TransportChain(object):
         losses = torch.zeros((T,), device=dev)
         grad_v = [None] * (T - 1)
         lam = None
-        exact = self.grad_mode == "EXACT" or self.decay <= 0  # the reference sums (l + d) at decay 0
+        gd0 = None
+        ref = self.grad_mode == "REFERENCE"
         a = self.decay
+        decayed = ref and a > 0  # optimization.py:1031: lerp(l, d, a) if a > 0 else l + d
         g_own = torch.empty(self.shape, device=dev)
         gd = torch.empty(self.shape, device=dev)
         starts = list(range(0, T, k))
@@ -145,15 +152,20 @@ class TransportChain(object):
                 df = ds[f - s0]
                 self._render_loss_grad(df, targets[f], g_own, losses[f:f + 1])
                 if f == T - 1:
-                    lam = g_own.clone() if exact else (1.0 - a) * g_own
+                    if T == 1:
+                        gd0 = g_own.clone()
+                    lam = (1.0 - a) * g_own if decayed else g_own.clone()
                     continue
                 gv = torch.empty(tuple(self.shape[:4]) + (3,), device=dev)
                 self._advect_bwd(df, velocities[f], saved[f - s0], lam, gd, gv)
+                num_step = T - 1 - f
+                if ref and self.vel_decay == 0:
+                    gv.mul_(1.0 / num_step)
                 grad_v[f] = gv
-                if exact or f == 0:
-                    lam = g_own + gd
-                else:
-                    lam = (1.0 - a) * g_own + a * gd
+                if f == 0:
+                    gd0 = g_own + gd * (1.0 / num_step if (ref and a == 0) else 1.0)
+                    continue
+                lam = (1.0 - a) * g_own + a * gd if decayed else g_own + gd
             del ds, saved
         self.frame_loss = losses
-        return losses.sum(), lam, grad_v
+        return losses.sum(), gd0, grad_v

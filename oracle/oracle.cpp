@@ -878,6 +878,31 @@ void sample_transform_fwd(const double* input, const int64_t* ishape, const floa
   }
 }
 
+// Raw sample positions and LoD of every output cell centre for one camera
+// (samplePos3D, sampling.hpp:59-136): what sample_transform_fwd feeds its
+// sampler, exported so the coordinate maps can be checked on their own.
+template <class T>
+void sample_positions(const float* M, const float* V, const float* P, const float* F, int nCam,
+                      const int64_t* in_dhw, const int64_t* out_dhw, int cm, int lod, double* out) {
+  Dims din{(int)in_dhw[2], (int)in_dhw[1], (int)in_dhw[0]};
+  Dims dout{(int)out_dhw[2], (int)out_dhw[1], (int)out_dhw[0]};
+  size_t ovox = size_t(dout.x) * dout.y * dout.z;
+  for (int c = 0; c < nCam; ++c) {
+    Xf<T> xf = make_xf<T>(M, V + 16 * c, P + 16 * c, F + 6 * c);
+    double* o = out + ovox * 4 * c;
+    for (int z = 0; z < dout.z; ++z)
+      for (int y = 0; y < dout.y; ++y)
+        for (int x = 0; x < dout.x; ++x) {
+          Pos4<T> pos = sample_pos(xf, cm, lod != 0, mk3<T>(x + T(0.5), y + T(0.5), z + T(0.5)), din, dout);
+          double* q = o + flat3(x, y, z, dout) * 4;
+          q[0] = pos.p.x;
+          q[1] = pos.p.y;
+          q[2] = pos.p.z;
+          q[3] = pos.lod;
+        }
+  }
+}
+
 // Adjoint helpers: scatter one tap of a v1 fetch into a level gradient.
 template <class T, class Add>
 void fetch_v1_adjoint(const SamplerCfg& s, const T* buf, Dims d, int C, v3<T> p, const T* g, T scale, Add&& add) {
@@ -1252,6 +1277,11 @@ int orc_sample_lut_bwd(int prec, int gm, const double* input, const int64_t* ish
                        int relative, int normalized, double* in_grad, double* lut_grad) {
   ORC_DISPATCH(prec, (orc::sample_lut_bwd<T>(gm, input, ishape, lut, lshape, out_grad, fm, bm, offset, global, relative,
                                              normalized, in_grad, lut_grad)));
+  return 0;
+}
+int orc_sample_positions(int prec, const float* M, const float* V, const float* P, const float* F, int nCam,
+                         const int64_t* in_dhw, const int64_t* out_dhw, int cm, int lod, double* out) {
+  ORC_DISPATCH(prec, (orc::sample_positions<T>(M, V, P, F, nCam, in_dhw, out_dhw, cm, lod, out)));
   return 0;
 }
 int orc_lod_transform(int prec, const int64_t* in_dhw, const float* MV, const float* P, const float* F, int nCam,

@@ -209,6 +209,22 @@ def lod_transform(in_dhw, MV, P, F, out_dhw, inverse_transform=False, prec=1):
     return out
 
 
+def sample_positions(M, V, P, F, in_dhw, out_dhw, coordinate_mode="TRANSFORM_LINDEPTH", lod=False, prec=1):
+    """Per output cell centre (index + 0.5) of every camera: the position the
+    coordinate mode maps it to (object space for the forward modes, input
+    index space for the reverse ones) and its LoD (samplePos3D,
+    sampling.hpp:59-136). Returns [nCam, D, H, W, 4] = (x, y, z, lod)."""
+    M = _f(M).reshape(-1, 16)[:1]
+    V = _f(V).reshape(-1, 16)
+    P = _f(P).reshape(-1, 16)
+    F = _f(F).reshape(-1, 6)
+    out = np.zeros((V.shape[0],) + tuple(int(s) for s in out_dhw) + (4,), dtype=np.float64)
+    _check(lib().orc_sample_positions(ctypes.c_int(prec), _p(M), _p(V), _p(P), _p(F), ctypes.c_int(V.shape[0]),
+                                      _p(_i64(in_dhw)), _p(_i64(out_dhw)), ctypes.c_int(COORD[coordinate_mode]),
+                                      ctypes.c_int(int(lod)), _p(out)), "sample_positions")
+    return out
+
+
 def build_mips(data_dhwc, levels, prec=1):
     data = _d(data_dhwc)
     D, H, W, C = data.shape
@@ -501,13 +517,24 @@ def velocity_self_advect_bwd(comps, out_grads, order=2, clamp="MC_SMOOTH", dt=1.
 
 # ------------------------------------------------- transport chain (A12)
 def chain_fwd_bwd(d0, velocities, targets, M, V, P, F, out_dhw, depth_step, order=2, clamp="MC_SMOOTH",
-                  grad_mode="EXACT", decay=0.0, light=(1.0, 1.0, 1.0), prec=1):
+                  grad_mode="EXACT", decay=0.0, light=(1.0, 1.0, 1.0), prec=1, vel_decay=None):
     """Density chain d_{f+1} = advect(d_f, v_f), loss = sum_f mean((render(d_f)
     - target_f)^2) and the backward of optStep_sequence in BWD order
-    (optimization.py:592-602,974-1047): EXACT accumulates lambda_f = g_f +
-    A_f^T lambda_{f+1}; REFERENCE with decay a > 0 uses lambda_f = (1-a) g_f
-    + a A_f^T lambda_{f+1} (lambda_{T-1} = (1-a) g_{T-1}) and with a = 0 the
-    plain sum; grad d0 = g_0 + A_0^T lambda_1."""
+    (optimization.py:592-602,974-1047), with g_f frame f's own render-loss
+    gradient and A_f^T the advection adjoint.
+
+    EXACT: lambda_f = g_f + A_f^T lambda_{f+1}; grad d0 = lambda_0 and the
+    velocity gradient of frame f is the velocity part of A_f^T lambda_{f+1}.
+
+    REFERENCE (the reference's warped-gradient accumulation, :1017-1031):
+    the accumulator starts at lambda_{T-1} = lerp(g_{T-1}, 0, a) and is
+    lambda_f = lerp(g_f, A_f^T lambda_{f+1}, a) = (1-a) g_f + a A_f^T lambda_{f+1}
+    for a decay a > 0 and g_f + A_f^T lambda_{f+1} for a = 0. The gradient
+    applied to frame f is its own g_f plus the warped term, which is divided by
+    num_step = T-1-f when the decay is 0 (:1017-1018): grad d0 = g_0 +
+    s_0 (A_0^T lambda_1)_d. The velocity gradient of frame f is
+    s_f (A_f^T lambda_{f+1})_v, with its own decay (warp_vel_grads_decay,
+    :1019-1020; vel_decay defaults to decay)."""
     T = len(targets)
     ds = [_d(d0)]
     for f in range(T - 1):
@@ -519,10 +546,24 @@ def chain_fwd_bwd(d0, velocities, targets, M, V, P, F, out_dhw, depth_step, orde
         losses.append(loss)
         g_own.append(render_density_bwd(ds[f], img, gimg, M, V, P, F, out_dhw, depth_step, light,
                                         grad_mode="EXACT", prec=prec))
-    exact = grad_mode == "EXACT" or decay <= 0  # the reference sums (l + d) when the decay is 0
-    lam = g_own[-1] if exact else (1.0 - decay) * g_own[-1]
+    ref = grad_mode == "REFERENCE"
+    a = float(decay)
+    av = a if vel_decay is None else float(vel_decay)
+
+    def lerp(l, d):  # optimization.py:1031: lerp(l, d, a) if a > 0 else l + d
+        return (1.0 - a) * l + a * d if (ref and a > 0) else l + d
+    if T == 1:
+        return dict(loss=float(np.sum(losses)), losses=losses, gd0=g_own[0], gv=[], densities=ds)
+    lam = lerp(g_own[-1], 0.0 * g_own[-1])
     gv = [None] * (T - 1)
+    gd0 = None
     for f in range(T - 2, -1, -1):
-        gd, gv[f] = advect_bwd(ds[f], velocities[f], lam, order, clamp, 1.0, "CLAMP", "EXACT", prec)
-        lam = g_own[f] + gd if (exact or f == 0) else (1.0 - decay) * g_own[f] + decay * gd
-    return dict(loss=float(np.sum(losses)), losses=losses, gd0=lam, gv=gv, densities=ds)
+        num_step = T - 1 - f
+        gd, gvf = advect_bwd(ds[f], velocities[f], lam, order, clamp, 1.0, "CLAMP", "EXACT", prec)
+        sd = 1.0 / num_step if (ref and a == 0) else 1.0
+        sv = 1.0 / num_step if (ref and av == 0) else 1.0
+        gv[f] = gvf * sv
+        if f == 0:
+            gd0 = g_own[0] + sd * gd
+        lam = lerp(g_own[f], gd)
+    return dict(loss=float(np.sum(losses)), losses=losses, gd0=gd0, gv=gv, densities=ds)

@@ -215,7 +215,7 @@ def _chain_inputs(T, seed=40):
 
 
 @pytest.mark.parametrize("k", [1, 2, 5])
-@pytest.mark.parametrize("grad_mode,decay", [("EXACT", 0.0), ("REFERENCE", 0.9)])
+@pytest.mark.parametrize("grad_mode,decay", [("EXACT", 0.0), ("REFERENCE", 0.9), ("REFERENCE", 0.0)])
 def test_transport_chain(cuda, k, grad_mode, decay):
     from gft.chain import TransportChain
     T = 5

@@ -204,6 +204,125 @@ def test_forward_modes_match_numpy_geometry(mode):
     assert checked > 50
 
 
+def _numpy_lindepth_forward(M_cm, V_cm, P_cm, F, q, dims):
+    """Object-space position of frustum index position q (any point, not only
+    cell centres) under TRANSFORM_LINDEPTH, from numpy's inverses: the same
+    restatement as _numpy_geometry, which the affine-field test pins."""
+    M, V, P = M_cm.T.astype(np.float64), V_cm.T.astype(np.float64), P_cm.T.astype(np.float64)
+    D, H, W = dims
+    n, f = float(F[0]), float(F[1])
+    ndc = np.stack([q[..., 0] / W * 2 - 1, q[..., 1] / H * 2 - 1, np.zeros_like(q[..., 0]), np.ones_like(q[..., 0])],
+                   -1)
+    zv = -(n + q[..., 2] / D * (f - n))
+    ndc[..., 2] = -P[2, 2] - P[2, 3] / zv
+    vs = ndc @ np.linalg.inv(P).T
+    vs = vs / vs[..., 3:4]
+    return (vs @ np.linalg.inv(V @ M).T)[..., :3]
+
+
+def _numpy_perspective_reverse(M_cm, V_cm, P_cm, F, c, dims):
+    """Frustum index position of object-space points c under a perspective
+    camera with depth linear in view space, computed directly: x, y from the
+    projected NDC, z from the view-space depth itself, t = (-z_view - near) /
+    (far - near). The reference recovers z_view from NDC depth instead
+    (transformations.hpp:106-117); for a perspective projection both agree."""
+    M, V, P = M_cm.T.astype(np.float64), V_cm.T.astype(np.float64), P_cm.T.astype(np.float64)
+    D, H, W = dims
+    hom = np.concatenate([c, np.ones_like(c[..., :1])], -1)
+    vs = hom @ (V @ M).T
+    clip = vs @ P.T
+    ndc = clip[..., :3] / clip[..., 3:4]
+    t = (-vs[..., 2] - float(F[0])) / (float(F[1]) - float(F[0]))
+    return np.stack([(ndc[..., 0] * 0.5 + 0.5) * W, (ndc[..., 1] * 0.5 + 0.5) * H, t * D], -1)
+
+
+def _cell_centres(dhw):
+    D, H, W = dhw
+    z, y, x = np.meshgrid(np.arange(D) + 0.5, np.arange(H) + 0.5, np.arange(W) + 0.5, indexing="ij")
+    return np.stack([x, y, z], -1)
+
+
+def test_lindepth_reverse_positions_match_numpy():
+    """TRANSFORM_LINDEPTH_REVERSE maps every grid cell centre to the frustum
+    index position numpy computes directly from the camera (perspective)."""
+    from helpers import scene
+    grid = (9, 12, 10)
+    sc = scene(grid=grid, image=(13, 7), views=3, seed=4)
+    fr = (17, 13, 7)
+    pos = orc.sample_positions(sc["M"], sc["V"], sc["P"], sc["F"], fr, grid, "TRANSFORM_LINDEPTH_REVERSE")
+    c = _cell_centres(grid)
+    for v in range(3):
+        want = _numpy_perspective_reverse(sc["M"][0], sc["V"][v], sc["P"][v], sc["F"][v], c, fr)
+        assert np.abs(pos[v, ..., :3] - want).max() <= 1e-9 * max(1.0, np.abs(want).max())
+        assert np.ptp(want[..., 2]) > 3  # the depth index really varies over the grid
+
+
+@pytest.mark.parametrize("perspective", [True, False])
+def test_lindepth_reverse_inverts_lindepth(perspective):
+    """LINDEPTH o LINDEPTH_REVERSE is the identity: the reverse position of a
+    cell centre, mapped forward again with the (pinned) numpy restatement of
+    TRANSFORM_LINDEPTH, is that cell centre. Orthographic cameras too, whose
+    NDC depth the reference's linear-depth formula treats as perspective."""
+    from gft import synthetic as syn
+    from gft.camera import camera_params_batch, model_matrices
+    from helpers import ortho_camera, scene
+    grid = (8, 11, 9)
+    fr = (15, 12, 10)
+    if perspective:
+        sc = scene(grid=grid, image=(10, 12), views=2, seed=6)
+        M, V, P, F = sc["M"], sc["V"], sc["P"], sc["F"]
+    else:
+        cams = [ortho_camera(grid, (11, 9), 15, yaw=y) for y in (0.0, 33.0)]
+        M = model_matrices([syn.grid_transform(grid)])
+        V, P, F = camera_params_batch(cams)
+    pos = orc.sample_positions(M, V, P, F, fr, grid, "TRANSFORM_LINDEPTH_REVERSE")
+    c = _cell_centres(grid)
+    for v in range(2):
+        back = _numpy_lindepth_forward(M[0], V[v], P[v], F[v], pos[v, ..., :3], fr)
+        assert np.abs(back - c).max() <= 1e-8 * np.abs(c).max()
+
+
+@pytest.mark.parametrize("k", [1, 2])
+@pytest.mark.parametrize("mode", ["TRANSFORM_REVERSE", "TRANSFORM_LINDEPTH_REVERSE"])
+def test_reverse_lod_of_a_finer_input_is_k(mode, k):
+    """Identity camera with an input 2^k times finer per axis than the output
+    grid: each output cell step moves 2^k input cells, so LoD = k."""
+    dhw = (4, 8, 8)
+    M, V, P, F = _identity_camera(dhw)
+    fine = tuple(s << k for s in dhw)
+    pos = orc.sample_positions(M, V, P, F, fine, dhw, mode, lod=True)
+    assert np.allclose(pos[..., 3], k, atol=1e-12)
+
+
+def test_reverse_lod_uses_jacobian_columns():
+    """The reverse modes' LoD is log2 of the largest Jacobian COLUMN norm of
+    the cell -> frustum-index map (sampling.hpp:124-136; the forward modes use
+    rows), with forward differences p(c) - p(c + e_a). Checked against numpy's
+    direct map under an anisotropic perspective camera, where rows and columns
+    give different LoDs."""
+    from gft.camera import Camera, camera_params_batch, model_matrices
+    from gft import synthetic as syn
+    from gft.transform import GridTransform, Transform
+    grid = (6, 7, 8)
+    fr = (40, 21, 33)  # fine, anisotropic frustum grid: LoD > 0 everywhere
+    t = GridTransform([fr[0], fr[1], fr[2]], translation=[0, 0, 14],
+                      parent=Transform(rotation_deg=[20, 35, 10], translation=[4, 3.5, 3]))
+    cam = Camera(t, nearFar=[8.0, 20.0], fov=40, aspect=fr[2] / fr[1])
+    M = model_matrices([syn.grid_transform(grid)])
+    V, P, F = camera_params_batch([cam])
+    pos = orc.sample_positions(M, V, P, F, fr, grid, "TRANSFORM_LINDEPTH_REVERSE", lod=True)
+    c = _cell_centres(grid)
+    p0 = _numpy_perspective_reverse(M[0], V[0], P[0], F[0], c, fr)
+    J = [p0 - _numpy_perspective_reverse(M[0], V[0], P[0], F[0], c + e, fr) for e in np.eye(3)]
+    cols = np.stack([np.linalg.norm(j, axis=-1) for j in J], -1)
+    rows = np.stack([np.linalg.norm(np.stack([j[..., a] for j in J], -1), axis=-1) for a in range(3)], -1)
+    lod_cols = np.maximum(0, np.log2(cols.max(-1)))
+    lod_rows = np.maximum(0, np.log2(rows.max(-1)))
+    assert np.abs(lod_cols - lod_rows).max() > 0.05
+    assert lod_cols.min() > 0.5
+    assert np.abs(pos[0, ..., 3] - lod_cols).max() <= 1e-8
+
+
 def test_raymarch_equals_depth_sum_of_resample_for_ortho():
     """With an orthographic camera the raymarch samples (uniform in object
     space between the near and far cell centres) coincide with the TRANSFORM
@@ -649,9 +768,48 @@ def test_chain_exact_gradient_is_the_fd_of_the_summed_loss():
         for idx in np.argsort(-np.abs(r["gv"][f]).ravel())[:3]:
             assert abs(_fd(loss_v, vels[f], int(idx), 1e-6) - r["gv"][f].flat[idx]) <= \
                 1e-5 * np.abs(r["gv"][f]).max()
-    # the decayed (reference) recursion equals the exact one at decay 0
-    r0 = orc.chain_fwd_bwd(d0, vels, tg, *args, order=1, clamp="NONE", grad_mode="REFERENCE", decay=0.0)
-    assert np.allclose(r0["gd0"], r["gd0"]) and all(np.allclose(a, b) for a, b in zip(r0["gv"], r["gv"]))
+
+
+@pytest.mark.parametrize("decay", [0.0, 0.9])
+def test_chain_reference_recursion_closed_form(decay):
+    """With zero velocity every advection is the identity (A_f = I for the
+    density), so the reference's warped-gradient recursion
+    (optimization.py:1017-1031, BWD order) has a closed form in the frames'
+    own render gradients g_f, here for T = 3 with three different targets:
+    decay a > 0: lambda_2 = (1-a) g_2, lambda_1 = (1-a) g_1 + a lambda_2,
+    grad d0 = g_0 + lambda_1; decay 0: lambda_1 = g_1 + g_2 and the warped term
+    applied to frame 0 is divided by num_step = 2: grad d0 = g_0 + (g_1 + g_2)/2.
+    The velocity gradients at decay 0 are the EXACT ones divided by
+    num_step = T-1-f (frame 1: 1, frame 0: 2), at decay 0.9 equal to the
+    EXACT adjoint of the decayed accumulator."""
+    from helpers import scene
+    sc = scene(grid=(5, 7, 6), image=(5, 4), views=2, seed=17)
+    d0 = sc["data"].astype(np.float64)
+    T = 3
+    zero = [np.zeros(d0.shape[:-1] + (3,)) for _ in range(T - 1)]
+    tg = [RNG.uniform(0, 0.4, size=(1, 2) + tuple(sc["out_dhw"][1:]) + (3,)) for _ in range(T)]
+    args = (sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"])
+    g = []
+    for f in range(T):  # each frame's own render-loss gradient of the same (unmoved) density
+        img = orc.render_density_fwd(d0, *args)
+        _, gimg = orc.l2_loss_and_grad(img, tg[f])
+        g.append(orc.render_density_bwd(d0, img, gimg, *args, grad_mode="EXACT"))
+    assert all(np.abs(g[f] - g[0]).max() > 1e-3 * np.abs(g[0]).max() for f in (1, 2))
+    r = orc.chain_fwd_bwd(d0, zero, tg, *args, order=2, clamp="MC_SMOOTH", grad_mode="REFERENCE", decay=decay)
+    a = decay
+    want = g[0] + (g[1] + g[2]) / 2 if a == 0 else g[0] + (1 - a) * g[1] + a * (1 - a) * g[2]
+    assert np.abs(r["gd0"] - want).max() <= 1e-12 * np.abs(want).max()
+    ex = orc.chain_fwd_bwd(d0, zero, tg, *args, order=2, clamp="MC_SMOOTH", grad_mode="EXACT")
+    assert np.abs(ex["gd0"] - (g[0] + g[1] + g[2])).max() <= 1e-12 * np.abs(ex["gd0"]).max()
+    if a == 0:
+        for f, n in ((0, 2), (1, 1)):
+            assert np.abs(r["gv"][f] - ex["gv"][f] / n).max() <= 1e-12 * np.abs(ex["gv"][f]).max()
+    else:
+        # frame 1's warped term is A_1^T lambda_2 = (1-a) A_1^T g_2: a scaled copy of EXACT's
+        assert np.abs(r["gv"][1] - (1 - a) * ex["gv"][1]).max() <= 1e-12 * np.abs(ex["gv"][1]).max()
+    # a single frame: nothing is warped, the gradient is the frame's own
+    r1 = orc.chain_fwd_bwd(d0, [], tg[:1], *args, grad_mode="REFERENCE", decay=decay)
+    assert np.array_equal(r1["gd0"], g[0])
 
 
 # ------------------------------------------------- loss / optimizer epilogue
