@@ -3,6 +3,8 @@
 // ambient-light density render with the L2 image loss
 // (Renderer.render_density, renderer.py:603-651; loss_dens_target,
 // optimization.py:396-412).
+#include <mutex>
+
 #include "gft_host.hpp"
 
 namespace gft {
@@ -189,6 +191,64 @@ int gft_advect_bwd_bf16(void* stream, const uint16_t* input, const int64_t shape
                         float* input_grad, float* velocity_grad, void* workspace, size_t workspace_bytes) {
   return advect_bwd_common(stream, input, shape, velocity, true, dt, order, clamp, saved, output_grad, input_grad,
                            velocity_grad, workspace, workspace_bytes);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ no-allocation advection adjoint
+// gft_advect_bwd_noalloc keeps the MacCormack adjoint's accumulator of the
+// forward-trace gradient (one float per voxel, the workspace of gft_advect_bwd)
+// in this file-scope array instead of caller memory, so the call allocates
+// nothing. Every process that loads the library holds it (64 MiB) on each GPU it
+// runs on until the process ends. The accumulator is cleared by a fill kernel at
+// the start of every call; concurrent calls on one GPU share it and race.
+static __device__ __align__(256) float scratch[GFT_ADVECT_NOALLOC_MAX_VOXELS];
+
+namespace gft {
+
+static constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kMaxScratchDevices];  // device address of `scratch` on each device
+
+// Device address of `scratch` on the current device, looked up once per device
+// with that device current. Host code never passes the symbol itself to a kernel.
+static int scratch_on_current_device(float** out) {
+  int dev = 0;
+  int rc;
+  if ((rc = check_hip(hipGetDevice(&dev), "hipGetDevice"))) return rc;
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(GFT_ERR_RUNTIME, "device %d beyond the %d devices the no-allocation scratch tracks", dev,
+                kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if ((rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))) return rc;
+    if (!p) return fail(GFT_ERR_RUNTIME, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (float*)g_scratch_ptr[dev];
+  return GFT_OK;
+}
+
+}  // namespace gft
+
+extern "C" {
+
+int gft_advect_bwd_noalloc(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
+                           int32_t order, int32_t clamp, const void* saved, const float* output_grad,
+                           float* input_grad, float* velocity_grad) {
+  int rc;
+  if ((rc = check_scalar_grid(shape))) return rc;
+  const size_t vox = (size_t)shape[1] * shape[2] * shape[3];
+  if (vox > (size_t)GFT_ADVECT_NOALLOC_MAX_VOXELS)
+    return fail(GFT_ERR_INVALID_ARGUMENT,
+                "grid of %zu voxels exceeds the %d voxels of the no-allocation scratch; use gft_advect_bwd with a "
+                "workspace",
+                vox, (int)GFT_ADVECT_NOALLOC_MAX_VOXELS);
+  float* ws = nullptr;
+  if ((rc = scratch_on_current_device(&ws))) return rc;
+  return advect_bwd_common(stream, input, shape, velocity, false, dt, order, clamp, saved, output_grad, input_grad,
+                           velocity_grad, ws, (size_t)GFT_ADVECT_NOALLOC_MAX_VOXELS * sizeof(float));
 }
 
 size_t gft_render_density_fwd_workspace(const int64_t shape[5], int32_t boundary) {

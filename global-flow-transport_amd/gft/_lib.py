@@ -53,6 +53,7 @@ _SIGNATURES = {
     "gft_advect_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _f32, _i32, _i32, _vp, _vp, _sz]),
     "gft_advect_bwd_workspace": (_sz, [_i64p]),
     "gft_advect_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _f32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "gft_advect_bwd_noalloc": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _f32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "gft_advect_fwd_bf16": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _f32, _i32, _i32, _vp, _vp, _sz]),
     "gft_advect_bwd_bf16": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _f32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _sz]),
     "gft_render_density_fwd_workspace": (_sz, [_i64p, _i32]),

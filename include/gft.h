@@ -121,6 +121,22 @@ int gft_advect_bwd_bf16(void* stream, const uint16_t* input, const int64_t shape
                         float* input_grad, float* velocity_grad, void* workspace, size_t workspace_bytes);
 
 /*
+ * gft_advect_bwd without a workspace argument: the accumulator the MacCormack
+ * adjoint needs (one float per voxel) lives in a device array inside the
+ * library, GFT_ADVECT_NOALLOC_MAX_VOXELS floats (64 MiB) per GPU, held from
+ * the library's load until the process ends. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc); the call makes no
+ * allocation. Grids above the limit are rejected (GFT_ERR_INVALID_ARGUMENT).
+ * Calls that overlap on one GPU share the array and race; it keeps the last
+ * call's contents, and each call clears what it uses first.
+ */
+#define GFT_ADVECT_NOALLOC_MAX_VOXELS (1 << 24)
+int gft_advect_bwd_noalloc(void* stream, const float* input, const int64_t shape[5], const float* velocity, float dt,
+                           int32_t order, int32_t clamp, const void* saved, const float* output_grad,
+                           float* input_grad, float* velocity_grad);
+
+/*
  * Fused density render: BEER_LAMBERT raymarch of a density [1,D,H,W,1] lit
  * by an ambient light (rgb emission = light * density * depth_step, optical
  * depth = density * depth_step), images [1,V,H,W,4] = (r, g, b, optical
