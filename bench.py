@@ -45,6 +45,13 @@ graph on one stream, and replayed from one graph in which every
 reconstruction is a branch on its own stream (forked from and joined to the
 capture stream), so that the small reconstructions run side by side.
 
+``--lit`` instead runs the c2 step under the production light (a spot light
+with a 256x196x196 shadow map, LINEAR mips with 4 levels and the gather
+sampling gradient, plus an ambient light): advect -> shadow map -> light
+factor -> render -> L2 loss -> backward through all of it, with the fused
+lit render against the renderer's generic path (the light grid materialised
+and the 4-channel raymarch op) on the same shadow ops.
+
 ``--chain`` instead measures the global transport chain (workload c3): an
 initial density advected through 20 per-frame velocity fields, every frame
 rendered to 5 views plus one unseen view and compared with its targets, the
@@ -648,6 +655,115 @@ def precision_bench(args):
         "boundaries": res}))
 
 
+def lit_bench(args):
+    """One JSON line for the production-lit c2 step (see the module docstring)."""
+    import numpy as np
+    import torch
+
+    from gft import synthetic as syn
+    from gft._lib import KernelTimers
+    from gft.lighting import Light, SpotLight
+    from gft.pipeline import _Cameras, advect, render_lit_forward, render_lit_saved_bytes
+    from gft.renderer import Renderer
+    from gft.transform import Transform
+
+    inp = build_inputs("c2")
+    dev = torch.device("cuda:0")
+    grid = inp["grid"]
+    cams, _ = syn.arc_cameras(grid, syn.WORKLOADS["c2"]["image"], syn.WORKLOADS["c2"]["views"])
+    d0 = torch.from_numpy(inp["d0"]).to(dev)
+    vel = torch.from_numpy(inp["vel"]).to(dev)
+    # the production light (reconstruct_sequence.py:698-704) in cell units: 300 cells from the grid
+    # centre, tilted 40 degrees down, clip 140..460, 50 degree cone; range_scale 0.825 per reference
+    # world unit, with the light ~2 units (300 cells here) away, is 0.0055 per cell
+    spot = SpotLight(Transform(translation=[0, 0, 300], parent=Transform(rotation_deg=[-40, 0, 0])), intensity=0.85,
+                     cast_shadows=True, shadow_clip=[140, 460], range_scale=0.825 / 150.0, angle_deg=50.0,
+                     shadow_resolution=[256, 196, 196], cone_mask=False)
+    lights = [spot, Light(intensity=0.64)]
+
+    def renderer(fused_lit):
+        return Renderer(filter_mode="LINEAR", boundary_mode="BORDER", mipmapping="LINEAR", num_mips=4, mip_bias=0.0,
+                        blend_mode="BEER_LAMBERT", sample_gradients=True, grad_mode="EXACT", fused=True,
+                        fused_lit=fused_lit)
+
+    gt = syn.grid_transform(grid)
+    gt.set_data(torch.from_numpy(inp["d_target"]).to(dev))
+    with torch.no_grad():
+        target = torch.cat(renderer(True).render_density(gt, lights, cams, cut_alpha=True), 0)[None].contiguous()
+
+    def step_fn(r):
+        def run():
+            d = d0.detach().requires_grad_(True)
+            v = vel.detach().requires_grad_(True)
+            d1 = advect(d, v, 1.0, 2, "MC_SMOOTH")
+            gt.set_data(d1)
+            img = torch.cat(r.render_density(gt, lights, cams, cut_alpha=True), 0)[None]
+            loss = torch.mean((img - target) ** 2)
+            loss.backward()
+            return loss, d.grad, v.grad
+        return run
+
+    def timeit(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(args.steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts))
+
+    res = {}
+    for name, fused_lit in (("fused_lit", True), ("generic", False)):
+        fn = step_fn(renderer(fused_lit))
+        res[name] = {"ms_per_step": timeit(fn)}
+        loss, gd, gv = fn()
+        torch.cuda.synchronize()
+        res[name]["loss"] = float(loss)
+        res[name]["gd"], res[name]["gv"] = gd.clone(), gv.clone()
+    gdiff = float((res["fused_lit"]["gd"] - res["generic"]["gd"]).abs().max() / res["generic"]["gd"].abs().max())
+    vdiff = float((res["fused_lit"]["gv"] - res["generic"]["gv"]).abs().max() / res["generic"]["gv"].abs().max())
+    for r_ in res.values():
+        del r_["gd"], r_["gv"]
+    # the lit render kernels alone, called directly (kernel timers are per thread and autograd runs
+    # the backward on its own thread)
+    import ctypes
+    from gft import _lib
+    from gft.ops import _np_ptr
+    r = renderer(True)
+    gt.set_data(d0)
+    lam = (spot.light_factor(gt, r) * 0.85).detach().contiguous()
+    rc = _Cameras(inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"])
+    saved = torch.empty((render_lit_saved_bytes(d0.shape, rc),), dtype=torch.uint8, device=dev)
+    gd, gl, loss = torch.empty_like(d0), torch.empty_like(d0), torch.zeros((1,), device=dev)
+    lc = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
+    tgt = target[0].contiguous()
+    with KernelTimers("*") as kt:
+        for _ in range(args.steps):
+            img = render_lit_forward(d0, lam, rc, inp["step"], (1.0, 1.0, 1.0), 0.64, saved=saved)
+            _lib.check(_lib.lib().gft_render_density_lit_bwd_loss(
+                _lib.stream_of(d0), _lib.ptr(d0), _lib.ptr(lam), _lib.i64_array(d0.shape), _np_ptr(rc.M), rc.nM,
+                _np_ptr(rc.V), _np_ptr(rc.P), _np_ptr(rc.F), rc.nCam, rc.c_out, lc, 0.64, float(inp["step"]), 0,
+                _lib.ptr(img), _lib.ptr(tgt), None, 0, 1.0 / tgt.numel(), _lib.ptr(saved), saved.numel(),
+                _lib.ptr(gd), _lib.ptr(gl), _lib.ptr(loss)))
+        torch.cuda.synchronize()
+    kern = {k: v[1] / max(v[0], 1) for k, v in kt.result.items()}
+    print(json.dumps({
+        "metric": "ms per fwd+bwd step under the production spot + ambient light (lower is better)",
+        "value": res["fused_lit"]["ms_per_step"], "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c2 grid 128x224x128, 5 views 256x456, MacCormack+MC_SMOOTH, spot light with a "
+                               "256x196x196 shadow map (LINEAR mips, 4 levels, gather gradient) + ambient 0.64, "
+                               "L2 loss, exact render gradient"},
+        "fused_lit": res["fused_lit"], "generic": res["generic"],
+        "speedup": res["generic"]["ms_per_step"] / res["fused_lit"]["ms_per_step"],
+        "grad_density_diff": gdiff, "grad_velocity_diff": vdiff, "lit_kernels_ms": kern}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -666,6 +782,7 @@ def main():
     ap.add_argument("--epilogue", action="store_true")
     ap.add_argument("--precision", action="store_true")
     ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--lit", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: libgft has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -690,6 +807,9 @@ def main():
         return
     if args.batched:
         batched_bench(args)
+        return
+    if args.lit:
+        lit_bench(args)
         return
 
     import numpy as np

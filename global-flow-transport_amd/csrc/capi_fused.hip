@@ -28,7 +28,13 @@ hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const f
                                          const float* second, float* grad, float* loss, const RayDims& rd,
                                          const RayLaunch& L, const RenderParams& rp);
 hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v);
-hipError_t quad_copy_launch(hipStream_t s, int bm, const float* lin, float* out, Dims3 d);
+hipError_t quad_copy_launch(hipStream_t s, int bm, const float* lin, float* out, Dims3 d, const float* fac = nullptr);
+hipError_t render_density_lit_fwd_launch(hipStream_t s, const float* dq, const float* eq, float* img,
+                                         const RayDims& rd, const RayLaunch& L, const RenderParams& rp,
+                                         const RenderSaved& sv);
+hipError_t render_density_lit_gather_launch(hipStream_t s, const RenderSaved& sv, const float* dens,
+                                            const float* lam, float* grad, float* lam_grad, const RayDims& rd,
+                                            const ProjLaunch& PL);
 size_t quad_copy_floats(Dims3 d);
 
 // Bytes of the corner-quad density copy the BORDER render kernels read.
@@ -271,7 +277,7 @@ static RenderSaved saved_parts(void* saved, const int64_t* shape, const int32_t*
   const RayDims rd = render_dims(shape, output_shape);
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
   char* p = (char*)saved;
-  RenderSaved sv;
+  RenderSaved sv{};
   sv.samples = (float*)p;
   p += align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float));
   sv.records = (float4*)p;
@@ -324,7 +330,8 @@ size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_came
                                         int32_t boundary) {
   (void)num_cameras;
   (void)output_shape;
-  return boundary == GFT_CLAMP ? quad_bytes(shape) : 0;  // the CLAMP corner-quad copy
+  // the corner-quad copy of the scatter walk (CLAMP, and BORDER without a saved buffer)
+  return boundary == GFT_CLAMP || boundary == GFT_BORDER ? quad_bytes(shape) : 0;
 }
 
 static int render_bwd_common(void* stream, const float* density, const int64_t shape[5], const float* M,
@@ -339,14 +346,14 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
   if (!density || !images || !second || !density_grad || (l2 && !loss) || !M || !V || !P || !light)
     return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
   const size_t need = gft_render_density_saved_bytes(shape, num_cameras, output_shape, boundary);
-  if (need && (!saved || saved_bytes < need))
+  if (saved && saved_bytes < need)
     return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
                 need);
   hipStream_t s = (hipStream_t)stream;
   const RayDims rd = render_dims(shape, output_shape);
   const RenderParams rp{{light[0], light[1], light[2]}, depth_step, loss_scale, loss_kind, backgrounds};
   auto pairs = view_pairs(1, num_cameras, true);
-  if (boundary == GFT_BORDER) {
+  if (boundary == GFT_BORDER && saved) {
     const RenderSaved sv = saved_parts(saved, shape, output_shape, num_cameras);
     for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
       size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
@@ -371,8 +378,9 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
   if (need_ws && (!workspace || workspace_bytes < need_ws))
     return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
                 need_ws);
-  if (boundary == GFT_CLAMP) {
-    if ((rc = check_hip(quad_copy_launch(s, B_CLAMP, density, (float*)workspace, rd.in), "quad copy"))) return rc;
+  if (boundary == GFT_CLAMP || boundary == GFT_BORDER) {
+    const int bm = boundary == GFT_CLAMP ? B_CLAMP : B_BORDER;
+    if ((rc = check_hip(quad_copy_launch(s, bm, density, (float*)workspace, rd.in), "quad copy"))) return rc;
     density = (const float*)workspace;
   }
   if (l2 && (rc = check_hip(fill_launch(s, loss, 1, 0.f), "clear loss"))) return rc;
@@ -423,6 +431,147 @@ int gft_render_density_bwd(void* stream, const float* density, const int64_t sha
   return render_bwd_common(stream, density, shape, M, num_M, V, P, num_cameras, output_shape, light, depth_step,
                            boundary, images, images_grad, false, 0.f, GFT_LOSS_SE, nullptr, saved, saved_bytes,
                            workspace, workspace_bytes, density_grad, nullptr);
+}
+
+// ------------------------------------------------------------- spot-lit render
+size_t gft_render_density_lit_fwd_workspace(const int64_t shape[5], int32_t boundary) {
+  return boundary == GFT_BORDER ? 2 * quad_bytes(shape) : 0;
+}
+
+size_t gft_render_density_lit_saved_bytes(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
+                                          int32_t boundary) {
+  if (boundary != GFT_BORDER) return 0;
+  const size_t plane = (size_t)output_shape[1] * output_shape[2];
+  return gft_render_density_saved_bytes(shape, num_cameras, output_shape, boundary) +
+         align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float));
+}
+
+// the lit saved buffer: the ambient render's layout followed by the u_i planes
+static RenderSaved saved_parts_lit(void* saved, const int64_t* shape, const int32_t* output_shape,
+                                   int32_t num_cameras) {
+  RenderSaved sv = saved_parts(saved, shape, output_shape, num_cameras);
+  sv.samples2 = (float*)((char*)saved + gft_render_density_saved_bytes(shape, num_cameras, output_shape, GFT_BORDER));
+  return sv;
+}
+
+static int check_lit_args(const int64_t* shape, int32_t num_M, int32_t num_cameras, const int32_t* output_shape,
+                          int32_t boundary, const float* color, float ambient) {
+  int rc;
+  if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
+  if (boundary != GFT_BORDER)
+    return fail(GFT_ERR_UNIMPLEMENTED, "the lit density render supports the BORDER boundary only");
+  if (!color) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  if (!std::isfinite(ambient)) return fail(GFT_ERR_INVALID_ARGUMENT, "ambient must be finite");
+  return GFT_OK;
+}
+
+int gft_render_density_lit_fwd(void* stream, const float* density, const float* light_factor,
+                               const int64_t shape[5], const float* M, int32_t num_M, const float* V,
+                               const float* P, const float* frustum, int32_t num_cameras,
+                               const int32_t output_shape[3], const float light_color[3], float ambient,
+                               float depth_step, int32_t boundary, void* workspace, size_t workspace_bytes,
+                               void* saved, size_t saved_bytes, float* images) {
+  (void)frustum;
+  int rc;
+  if ((rc = check_lit_args(shape, num_M, num_cameras, output_shape, boundary, light_color, ambient))) return rc;
+  if (!density || !light_factor || !images || !M || !V || !P)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  const size_t need = gft_render_density_lit_fwd_workspace(shape, boundary);
+  if (!workspace || workspace_bytes < need)
+    return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
+                need);
+  const size_t need_saved = gft_render_density_lit_saved_bytes(shape, num_cameras, output_shape, boundary);
+  if (saved && saved_bytes < need_saved)
+    return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
+                need_saved);
+  RenderSaved sv{nullptr, nullptr, nullptr, nullptr};
+  if (saved) sv = saved_parts_lit(saved, shape, output_shape, num_cameras);
+  hipStream_t s = (hipStream_t)stream;
+  const RayDims rd = render_dims(shape, output_shape);
+  RenderParams rp{{light_color[0], light_color[1], light_color[2]}, depth_step, 0.f};
+  rp.ambient = ambient;
+  float* dq = (float*)workspace;
+  float* eq = (float*)((char*)workspace + quad_bytes(shape));
+  if ((rc = check_hip(quad_copy_launch(s, B_BORDER, density, dq, rd.in), "quad copy"))) return rc;
+  if ((rc = check_hip(quad_copy_launch(s, B_BORDER, density, eq, rd.in, light_factor), "lit quad copy"))) return rc;
+  auto pairs = view_pairs(1, num_cameras, true);
+  for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
+    size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
+    RayLaunch L;
+    if ((rc = ray_views(pairs, first, cnt, M, V, P, L))) return rc;
+    if ((rc = check_hip(render_density_lit_fwd_launch(s, dq, eq, images, rd, L, rp, sv), "lit render forward")))
+      return rc;
+  }
+  return GFT_OK;
+}
+
+static int render_lit_bwd_common(void* stream, const float* density, const float* light_factor,
+                                 const int64_t shape[5], const float* M, int32_t num_M, const float* V,
+                                 const float* P, int32_t num_cameras, const int32_t output_shape[3],
+                                 const float light_color[3], float ambient, float depth_step, int32_t boundary,
+                                 const float* images, const float* second, bool l2, float loss_scale,
+                                 int32_t loss_kind, const float* backgrounds, void* saved, size_t saved_bytes,
+                                 float* density_grad, float* light_factor_grad, float* loss) {
+  int rc;
+  if ((rc = check_lit_args(shape, num_M, num_cameras, output_shape, boundary, light_color, ambient))) return rc;
+  if (loss_kind != GFT_LOSS_SE && loss_kind != GFT_LOSS_AE) return fail(GFT_ERR_INVALID_ARGUMENT, "invalid loss_kind");
+  if (!density || !light_factor || !images || !second || !density_grad || !light_factor_grad || (l2 && !loss) || !M ||
+      !V || !P)
+    return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  const size_t need = gft_render_density_lit_saved_bytes(shape, num_cameras, output_shape, boundary);
+  if (!saved || saved_bytes < need)
+    return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
+                need);
+  hipStream_t s = (hipStream_t)stream;
+  const RayDims rd = render_dims(shape, output_shape);
+  RenderParams rp{{light_color[0], light_color[1], light_color[2]}, depth_step, loss_scale, loss_kind, backgrounds};
+  rp.ambient = ambient;
+  const RenderSaved sv = saved_parts_lit(saved, shape, output_shape, num_cameras);
+  auto pairs = view_pairs(1, num_cameras, true);
+  for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
+    size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
+    RayLaunch L;
+    ProjLaunch PL;
+    if ((rc = ray_views(pairs, first, cnt, M, V, P, L, &PL, &rd))) return rc;
+    PL.accumulate = first > 0;
+    if ((rc = check_hip(render_density_pixel_launch(l2, s, images, second, sv, rd, L, rp),
+                        "lit render backward pixel pass")))
+      return rc;
+    if ((rc = check_hip(render_density_lit_gather_launch(s, sv, density, light_factor, density_grad,
+                                                         light_factor_grad, rd, PL),
+                        "lit render backward gather")))
+      return rc;
+  }
+  if (l2 && (rc = check_hip(loss_sum_launch(s, sv.partials, (int)(render_tiles_per_view(rd) * num_cameras),
+                                            loss_scale, loss),
+                            "loss sum")))
+    return rc;
+  return GFT_OK;
+}
+
+int gft_render_density_lit_bwd_loss(void* stream, const float* density, const float* light_factor,
+                                    const int64_t shape[5], const float* M, int32_t num_M, const float* V,
+                                    const float* P, const float* frustum, int32_t num_cameras,
+                                    const int32_t output_shape[3], const float light_color[3], float ambient,
+                                    float depth_step, int32_t boundary, const float* images, const float* targets,
+                                    const float* backgrounds, int32_t loss_kind, float loss_scale, void* saved,
+                                    size_t saved_bytes, float* density_grad, float* light_factor_grad, float* loss) {
+  (void)frustum;
+  return render_lit_bwd_common(stream, density, light_factor, shape, M, num_M, V, P, num_cameras, output_shape,
+                               light_color, ambient, depth_step, boundary, images, targets, true, loss_scale,
+                               loss_kind, backgrounds, saved, saved_bytes, density_grad, light_factor_grad, loss);
+}
+
+int gft_render_density_lit_bwd(void* stream, const float* density, const float* light_factor,
+                               const int64_t shape[5], const float* M, int32_t num_M, const float* V,
+                               const float* P, const float* frustum, int32_t num_cameras,
+                               const int32_t output_shape[3], const float light_color[3], float ambient,
+                               float depth_step, int32_t boundary, const float* images, const float* images_grad,
+                               void* saved, size_t saved_bytes, float* density_grad, float* light_factor_grad) {
+  (void)frustum;
+  return render_lit_bwd_common(stream, density, light_factor, shape, M, num_M, V, P, num_cameras, output_shape,
+                               light_color, ambient, depth_step, boundary, images, images_grad, false, 0.f,
+                               GFT_LOSS_SE, nullptr, saved, saved_bytes, density_grad, light_factor_grad, nullptr);
 }
 
 }  // extern "C"

@@ -412,6 +412,7 @@ struct RenderSaved {
   float* samples;
   float4* records;
   float* partials;
+  float* samples2;  // the lit render's second per-sample plane (u_i), else null
 };
 
 // Fused advection launch parameters.
@@ -430,6 +431,7 @@ struct RenderParams {
   float loss_scale;
   int loss_kind;       // LossKind of the folded image loss
   const float* bkg;    // [V,H,W,3] background composited as rgb + bkg * exp(-depth), or null
+  float ambient;       // lit render: ambient intensity relative to `light` (the colour)
 };
 
 // Shape of one raymarch launch.

@@ -86,9 +86,11 @@ __device__ __forceinline__ unsigned quad_index(const QuadDims& q, unsigned ix, u
 // grid is 0. CLAMP: corners take the clamped voxel, so that an entry at the
 // copy's edge holds the edge value in all its slots and a tap whose index is
 // clamped into the copy reads the CLAMP value whatever its weights.
+// With `fac` non-null the copy holds the product lin * fac per voxel (the
+// spot-lit emission grid d * lambda of the lit render).
 template <int BM>
-__global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin, float4* __restrict__ quad, Dims3 d,
-                                                   QuadDims q) {
+__global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin, const float* __restrict__ fac,
+                                                   float4* __restrict__ quad, Dims3 d, QuadDims q) {
   const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= quad_entries(q)) return;
   const unsigned b = (unsigned)(j / kQB), o = (unsigned)(j % kQB);
@@ -96,10 +98,11 @@ __global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin
   const unsigned iy = ((b / q.nbx) % q.nby) * kQBY + ((o >> kQBXl) & (kQBY - 1));
   const unsigned iz = (b / (q.nbx * q.nby)) * kQBZ + (o >> (kQBXl + kQBYl));
   const int x = (int)ix - 1, y = (int)iy - 1, z = (int)iz - 1;
+  auto at = [&](size_t f) -> float { return fac ? lin[f] * fac[f] : lin[f]; };
   auto v = [&](int xx, int yy) -> float {
-    if (BM == B_CLAMP) return lin[flat3(clampi(xx, 0, d.x - 1), clampi(yy, 0, d.y - 1), clampi(z, 0, d.z - 1), d)];
+    if (BM == B_CLAMP) return at(flat3(clampi(xx, 0, d.x - 1), clampi(yy, 0, d.y - 1), clampi(z, 0, d.z - 1), d));
     return (((unsigned)xx < (unsigned)d.x) & ((unsigned)yy < (unsigned)d.y) & ((unsigned)z < (unsigned)d.z))
-               ? lin[flat3(xx, yy, z, d)]
+               ? at(flat3(xx, yy, z, d))
                : 0.f;
   };
   quad[j] = make_float4(v(x, y), v(x + 1, y), v(x, y + 1), v(x + 1, y + 1));
@@ -147,6 +150,51 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
     const float v0 = lerpf(lerpf(c0[u].x, c0[u].y, w[u][0]), lerpf(c0[u].z, c0[u].w, w[u][0]), w[u][1]);
     const float v1 = lerpf(lerpf(c1[u].x, c1[u].y, w[u][0]), lerpf(c1[u].z, c1[u].w, w[u][0]), w[u][1]);
     val[u] = inside[u] ? lerpf(v0, v1, w[u][2]) : 0.f;
+  }
+}
+
+// As read_border_batch for two corner-quad copies of the same grid shape (the
+// density and the lit emission grid d * lambda), read with the same corners and
+// weights; all 4U loads are in flight before any value is used.
+template <int U>
+__device__ __forceinline__ void read_border_batch2(const float4* __restrict__ qa, const float4* __restrict__ qb,
+                                                   Dims3 d, QuadDims q, float3 n, float3 st, int i0, int last,
+                                                   float va[U], float vb[U], bool inside[U]) {
+  unsigned a[U], a1[U];
+  float w[U][3];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int i = i0 + u;
+    const float fi = (float)i;
+    const float px = fmaf(fi, st.x, n.x) - 0.5f, py = fmaf(fi, st.y, n.y) - 0.5f, pz = fmaf(fi, st.z, n.z) - 0.5f;
+    const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+    inside[u] = (i <= last) & (0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) &
+                (qz < (float)d.z);
+    const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    w[u][0] = px - fx;
+    w[u][1] = py - fy;
+    w[u][2] = pz - fz;
+    const unsigned ix = (unsigned)(clampi((int)fx, -1, d.x - 1) + 1), iy = (unsigned)(clampi((int)fy, -1, d.y - 1) + 1),
+                   iz = (unsigned)(clampi((int)fz, -1, d.z - 1) + 1);
+    a[u] = quad_index(q, ix, iy, iz);
+    a1[u] = quad_index(q, ix, iy, iz + 1);
+  }
+  float4 c0[U], c1[U], e0[U], e1[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    c0[u] = qa[a[u]];
+    c1[u] = qa[a1[u]];
+    e0[u] = qb[a[u]];
+    e1[u] = qb[a1[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const float v0 = lerpf(lerpf(c0[u].x, c0[u].y, w[u][0]), lerpf(c0[u].z, c0[u].w, w[u][0]), w[u][1]);
+    const float v1 = lerpf(lerpf(c1[u].x, c1[u].y, w[u][0]), lerpf(c1[u].z, c1[u].w, w[u][0]), w[u][1]);
+    const float g0 = lerpf(lerpf(e0[u].x, e0[u].y, w[u][0]), lerpf(e0[u].z, e0[u].w, w[u][0]), w[u][1]);
+    const float g1 = lerpf(lerpf(e1[u].x, e1[u].y, w[u][0]), lerpf(e1[u].z, e1[u].w, w[u][0]), w[u][1]);
+    va[u] = inside[u] ? lerpf(v0, v1, w[u][2]) : 0.f;
+    vb[u] = inside[u] ? lerpf(g0, g1, w[u][2]) : 0.f;
   }
 }
 
@@ -241,6 +289,66 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     }
   }
   if (!valid) return;
+  const float4 o = make_float4(rp.light[0] * E, rp.light[1] * E, rp.light[2] * E, D);
+  reinterpret_cast<float4*>(img + rd.out_view_stride * v.out_slot)[size_t(y) * rd.out_w + x] = o;
+}
+
+// ------------------------------------------------------------ lit forward
+// Density lit by a spot or point light and an ambient term (BORDER): per cell
+// the emission is color * d * (lambda + a), with lambda the light's per-cell
+// factor (intensity * shadow transmittance * falloff, lighting.py:106-147) and
+// a the ambient intensity relative to `color`; the absorption is d. Both are
+// scaled by the depth step and sampled trilinearly from the corner-quad copies
+// of d and of d * lambda:
+//   sigma_i = s d(p_i), eps_i = s (d lambda)(p_i),
+//   D_i = max(D_{i-1} + sigma_i, 0), u_i = exp(-D_i),
+//   E = sum_i u_i (eps_i + a sigma_i), rgb = color * E.
+// With sv.samples the forward saves per sample c_i = a u_i + E_{i-1} (a NaN
+// mark outside the grid) and u_i (sv.samples2): the backward's absorption
+// gradient is alpha c_i + beta and its emission gradient alpha u_i.
+constexpr int kLitBatch = 4;
+__global__ __launch_bounds__(256) void k_render_density_lit_fwd(const float4* __restrict__ dq,
+                                                                const float4* __restrict__ eq,
+                                                                float* __restrict__ img, RayDims rd, RayLaunch L,
+                                                                RenderParams rp, QuadDims q, RenderSaved sv) {
+  const RayView& v = L.view[blockIdx.z];
+  const int x = blockIdx.x * kRTileX + (threadIdx.x % kRTileX);
+  const int y = blockIdx.y * kRTileY + (threadIdx.x / kRTileX);
+  const bool valid = x < rd.out_w && y < rd.out_h;
+  float D = 0.f, E = 0.f;
+  float3 nearp = make_float3(0.f, 0.f, 0.f), step = make_float3(0.f, 0.f, 0.f);
+  int lo = 1, hi = 0;
+  if (valid) {
+    pixel_ray(v, rd, x, y, nearp, step);
+    clip_ray(nearp, step, rd.in, rd.out_d, 0.f, lo, hi);
+  }
+  const size_t plane = size_t(rd.out_w) * rd.out_h;
+  const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
+  const size_t vbase = (size_t)v.out_slot * rd.out_d * plane + pix;
+  float* cview = (sv.samples && valid) ? sv.samples + vbase : nullptr;
+  float* uview = (sv.samples2 && valid) ? sv.samples2 + vbase : nullptr;
+  const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
+  const float a = rp.ambient;
+  for (int i = wlo; i <= whi; i += kLitBatch) {
+    float vd[kLitBatch], ve[kLitBatch];
+    bool in[kLitBatch];
+    read_border_batch2<kLitBatch>(dq, eq, rd.in, q, nearp, step, i, hi, vd, ve, in);
+#pragma unroll
+    for (int u = 0; u < kLitBatch; ++u) {
+      const bool act = in[u] & (i + u >= lo);
+      const float sigma = act ? rp.step * vd[u] : 0.f;
+      const float eps = act ? rp.step * ve[u] : 0.f;
+      D = act ? fmaxf(D + sigma, 0.f) : D;
+      const float e = __expf(-D);
+      if (cview && i + u <= whi) {
+        cview[(size_t)(i + u) * plane] = act ? fmaf(a, e, E) : kOutsideMark;
+        uview[(size_t)(i + u) * plane] = act ? e : 0.f;
+      }
+      E = fmaf(e, fmaf(a, sigma, eps), E);
+    }
+  }
+  if (!valid) return;
+  if (cview) sv.records[(size_t)v.out_slot * plane + pix] = make_record(lo, hi, E, 0.f);
   const float4 o = make_float4(rp.light[0] * E, rp.light[1] * E, rp.light[2] * E, D);
   reinterpret_cast<float4*>(img + rd.out_view_stride * v.out_slot)[size_t(y) * rd.out_w + x] = o;
 }
@@ -375,20 +483,30 @@ __global__ __launch_bounds__(256) void k_loss_sum(const float* __restrict__ part
 // is amortised over a whole block. Only this wave touches its region, so the
 // sums run in a fixed order (deterministic) and one store per voxel follows.
 constexpr int kGatherBatch = 4;  // saved values loaded per round trip (2: 0.187 ms, 8: 0.211 ms at c2)
-template <int WX, int WY, int WZ>
+//
+// LIT (the spot-lit render): a second region sums the emission gradients
+// alpha u_i * w of the d * lambda grid; the store then writes
+// grad = sum_sigma + lambda * sum_eps and lambda_grad = d * sum_eps.
+template <int WX, int WY, int WZ, bool LIT = false>
 __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved sv, float* __restrict__ grad,
-                                                                   RayDims rd, ProjLaunch PL) {
+                                                                   RayDims rd, ProjLaunch PL,
+                                                                   const float* __restrict__ dens,
+                                                                   const float* __restrict__ lam,
+                                                                   float* __restrict__ lam_grad) {
   constexpr int kVox = WX * WY * WZ;
-  __shared__ double region[4][kVox];
+  __shared__ double region[LIT ? 8 : 4][kVox];
   typedef __attribute__((address_space(3))) double lds_double;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   lds_double* acc = (lds_double*)region[wave];
+  lds_double* acc2 = (lds_double*)region[LIT ? 4 + wave : wave];
   const Dims3 d = rd.in;
   const int nbx = (d.x + WX - 1) / WX, nby = (d.y + WY - 1) / WY, nbz = (d.z + WZ - 1) / WZ;
   const int blk = blockIdx.x * 4 + wave;
   const bool live = blk < nbx * nby * nbz;  // a wave past the last block only joins the barriers
   const int x0 = (blk % nbx) * WX, y0 = ((blk / nbx) % nby) * WY, z0 = (blk / (nbx * nby)) * WZ;
   for (int k = lane; k < kVox; k += 64) acc[k] = 0.0;
+  if (LIT)
+    for (int k = lane; k < kVox; k += 64) acc2[k] = 0.0;
   const size_t plane = size_t(rd.out_w) * rd.out_h;
   const float fx0 = (float)x0, fy0 = (float)y0, fz0 = (float)z0;
   // support box in ray coordinates (which carry the +0.5 cell-centre offset)
@@ -424,6 +542,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
     }
     if (xs > xe || ys > ye) continue;
     const float* gview = sv.samples + (size_t)pv.out_slot * rd.out_d * plane;
+    const float* uview = LIT ? sv.samples2 + (size_t)pv.out_slot * rd.out_d * plane : nullptr;
     const float4* rview = sv.records + (size_t)pv.out_slot * plane;
     {
       const int nx = xe - xs + 1, cnt = nx * (ye - ys + 1);
@@ -475,11 +594,17 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
         float qx = fmaf(fi0, sx, nx_) - (fx0 + 0.5f), qy = fmaf(fi0, sy, ny_) - (fy0 + 0.5f),
               qz = fmaf(fi0, sz, nz_) - (fz0 + 0.5f);
         const float* gp = gview + (size_t)i0 * plane + pix;
+        const float* up = LIT ? uview + (size_t)i0 * plane + pix : nullptr;
         for (int ib = i0; ib <= i1; ib += kGatherBatch) {
           // the batch's saved values are all loaded before the first is used
-          float cs[kGatherBatch];
+          float cs[kGatherBatch], us[kGatherBatch];
 #pragma unroll
           for (int u = 0; u < kGatherBatch; ++u) cs[u] = ib + u <= i1 ? gp[(size_t)u * plane] : kOutsideMark;
+          if (LIT) {
+#pragma unroll
+            for (int u = 0; u < kGatherBatch; ++u) us[u] = ib + u <= i1 ? up[(size_t)u * plane] : 0.f;
+            up += (size_t)kGatherBatch * plane;
+          }
           gp += (size_t)kGatherBatch * plane;
 #pragma unroll
           for (int u = 0; u < kGatherBatch; ++u) {
@@ -490,6 +615,8 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
             const float tx = qx - flx, ty = qy - fly, tz = qz - flz;
             const int lx = (int)flx, ly = (int)fly, lz = (int)flz;
             const float wx[2] = {g * (1.f - tx), g * tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
+            const float ge = LIT ? alpha * us[u] : 0.f;
+            const float ex[2] = {ge * (1.f - tx), ge * tx};
             // corners outside the block are skipped (a zero-weight LDS atomic
             // costs as much as any other: the LDS atomic rate bounds this loop)
 #pragma unroll
@@ -507,6 +634,9 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
                   if ((unsigned)cx >= (unsigned)WX) continue;
                   __hip_atomic_fetch_add(acc + (cz * WY + cy) * WX + cx, (double)(wx[kx] * wyz), __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                  if (LIT)
+                    __hip_atomic_fetch_add(acc2 + (cz * WY + cy) * WX + cx, (double)(ex[kx] * wyz),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
               }
             }
@@ -524,9 +654,17 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
   for (int k = lane; k < kVox; k += 64) {
     const int x = x0 + k % WX, y = y0 + (k / WX) % WY, z = z0 + k / (WX * WY);
     if (x < d.x && y < d.y && z < d.z) {
-      float* g = grad + flat3(x, y, z, d);
-      const float v = (float)acc[k];
-      *g = PL.accumulate ? *g + v : v;
+      const size_t f = flat3(x, y, z, d);
+      float* g = grad + f;
+      if (LIT) {
+        const double se = acc2[k];
+        const float v = (float)(acc[k] + (double)lam[f] * se), vl = (float)((double)dens[f] * se);
+        *g = PL.accumulate ? *g + v : v;
+        lam_grad[f] = PL.accumulate ? lam_grad[f] + vl : vl;
+      } else {
+        const float v = (float)acc[k];
+        *g = PL.accumulate ? *g + v : v;
+      }
     }
   }
 }
@@ -579,16 +717,23 @@ __device__ __forceinline__ Brick segment_brick(const float3* cn, const float3* c
   return b;
 }
 
-// `quad` is the CLAMP corner-quad copy of the density (the forward's copy):
-// a segment's 8 samples are read with all their loads in flight, and give the
-// forward's sigma_i bit for bit (the same corners, weights and lerp order).
-template <bool L2>
+// `quad` is the corner-quad copy of the density for the boundary (the
+// forward's copy): a segment's 8 samples are read with all their loads in
+// flight, and give the forward's sigma_i bit for bit (the same corners,
+// weights and lerp order).
+// BORDER (the recompute alternative to the saved-state gather: nothing is
+// kept from the forward but the images) walks only the segments of the
+// tile's clipped sample ranges; samples outside the grid read 0 and corners
+// outside it take no gradient. Its global float atomics make the summation
+// order, and so the last bits, vary from run to run.
+template <bool L2, int BM = B_CLAMP>
 __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float4* __restrict__ quad, QuadDims q,
                                                                   const float* __restrict__ img,
                                                                   const float* __restrict__ second,
                                                                   float* __restrict__ grad, float* __restrict__ loss,
                                                                   RayDims rd, RayLaunch L, RenderParams rp) {
   __shared__ double lg[kCap];
+  __shared__ int seg_range[2];
   const RayView& v = L.view[blockIdx.z];
   const int x0 = blockIdx.x * kRTileX, y0 = blockIdx.y * kRTileY;
   const int x = x0 + (threadIdx.x % kRTileX), y = y0 + (threadIdx.x / kRTileX);
@@ -609,29 +754,50 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float4* 
   }
   const bool walk = active && (G != 0.f || gD != 0.f);
   for (int i = threadIdx.x; i < kCap; i += 256) lg[i] = 0.0;
-  __syncthreads();
   const int S = rd.out_d;
-  for (int sgi = (S + kSegK - 1) / kSegK - 1; sgi >= 0; --sgi) {
+  int sg_lo = 0, sg_hi = (S + kSegK - 1) / kSegK - 1;
+  if (BM == B_BORDER) {
+    // the union of the tile's clipped ranges, in whole segments
+    int lo = 1, hi = 0;
+    if (walk) clip_ray(nearp, step, rd.in, S, 0.f, lo, hi);
+    const int wlo = wave_min(lo <= hi ? lo : S), whi = wave_max(lo <= hi ? hi : -1);
+    if (threadIdx.x == 0) {
+      seg_range[0] = S;
+      seg_range[1] = -1;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&seg_range[0], wlo);
+      atomicMax(&seg_range[1], whi);
+    }
+    __syncthreads();
+    sg_lo = seg_range[0] / kSegK;
+    sg_hi = seg_range[1] < 0 ? -1 : seg_range[1] / kSegK;
+  } else {
+    __syncthreads();
+  }
+  for (int sgi = sg_hi; sgi >= sg_lo; --sgi) {
     const int ia = sgi * kSegK, ib = min(ia + kSegK, S) - 1;
     const Brick br = segment_brick(cn, cs, ia, ib, rd.in);
     const bool use_lds = br.size <= kCap;
     if (walk) {
       float val[kSegK];
       bool inr[kSegK];
-      read_border_batch<kSegK, B_CLAMP>(quad, rd.in, q, nearp, step, ib, -1, ia, val, inr);
+      read_border_batch<kSegK, BM>(quad, rd.in, q, nearp, step, ib, -1, ia, val, inr);
 #pragma unroll
       for (int u = 0; u < kSegK; ++u) {
         const int i = ib - u;
         if (i < ia) break;
         const float fi = (float)i;
-        const Taps tp = make_taps<B_CLAMP>(fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
-                                           fmaf(fi, step.z, nearp.z) - 0.5f, rd.in);
+        const Taps tp = make_taps<BM>(fmaf(fi, step.x, nearp.x) - 0.5f, fmaf(fi, step.y, nearp.y) - 0.5f,
+                                      fmaf(fi, step.z, nearp.z) - 0.5f, rd.in);
         const float sigma = rp.step * val[u];
         const float e = __expf(-D);
         D = fmaxf(D - sigma, 0.f);
         const float esg = e * sigma * G;
         const float gds = rp.step * (e * G + (gD - esg));
         gD -= esg;
+        if (BM == B_BORDER && !inr[u]) continue;  // outside the grid: no gradient
         const float wx[2] = {1.f - tp.wx, tp.wx}, wy[2] = {1.f - tp.wy, tp.wy}, wz[2] = {1.f - tp.wz, tp.wz};
 #pragma unroll
         for (int k = 0; k < 2; ++k)
@@ -641,6 +807,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float4* 
             for (int ii = 0; ii < 2; ++ii) {
               const float w = wx[ii] * wy[j] * wz[k];
               if (w == 0.f) continue;
+              if (BM == B_BORDER && !(tp.vx[ii] & tp.vy[j] & tp.vz[k])) continue;
               const int bx = tp.ix[ii] - br.lo[0], by = tp.iy[j] - br.lo[1], bz = tp.iz[k] - br.lo[2];
               const bool in = use_lds & ((unsigned)bx < (unsigned)br.n[0]) & ((unsigned)by < (unsigned)br.n[1]) &
                               ((unsigned)bz < (unsigned)br.n[2]);
@@ -740,16 +907,29 @@ static dim3 tile_grid(const RayDims& rd, int views) {
 size_t quad_copy_floats(Dims3 d) { return quad_entries(make_quad_dims(d)) * 4; }
 
 // bm: B_BORDER or B_CLAMP
-hipError_t quad_copy_launch(hipStream_t s, int bm, const float* lin, float* out, Dims3 d) {
+// fac: null, or a per-voxel factor the copy multiplies in (lin * fac)
+hipError_t quad_copy_launch(hipStream_t s, int bm, const float* lin, float* out, Dims3 d, const float* fac) {
   const QuadDims q = make_quad_dims(d);
   const size_t n = quad_entries(q);
   KernelTimer t(s, "quad_copy");
   if (bm == B_CLAMP)
-    hipLaunchKernelGGL(k_quad_copy<B_CLAMP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin,
+    hipLaunchKernelGGL(k_quad_copy<B_CLAMP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin, fac,
                        reinterpret_cast<float4*>(out), d, q);
   else
-    hipLaunchKernelGGL(k_quad_copy<B_BORDER>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin,
+    hipLaunchKernelGGL(k_quad_copy<B_BORDER>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin, fac,
                        reinterpret_cast<float4*>(out), d, q);
+  return hipGetLastError();
+}
+
+// spot-lit forward (BORDER): dq, eq the corner-quad copies of d and d * lambda
+hipError_t render_density_lit_fwd_launch(hipStream_t s, const float* dq, const float* eq, float* img,
+                                         const RayDims& rd, const RayLaunch& L, const RenderParams& rp,
+                                         const RenderSaved& sv) {
+  const dim3 grid = tile_grid(rd, L.num_views);
+  const QuadDims q = make_quad_dims(rd.in);
+  KernelTimer t(s, "render_lit_fwd");
+  hipLaunchKernelGGL(k_render_density_lit_fwd, grid, dim3(256), 0, s, reinterpret_cast<const float4*>(dq),
+                     reinterpret_cast<const float4*>(eq), img, rd, L, rp, q, sv);
   return hipGetLastError();
 }
 
@@ -800,12 +980,24 @@ hipError_t render_density_gather_launch(hipStream_t s, const RenderSaved& sv, fl
                  ((rd.in.z + kGatherWZ - 1) / kGatherWZ);
   KernelTimer t(s, "render_bwd_gather");
   auto kern = k_render_density_bwd_gather<kGatherWX, kGatherWY, kGatherWZ>;
-  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL);
+  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
-// CLAMP / WRAP scatter backward (grad must be zero on entry)
-// CLAMP: dens is the CLAMP corner-quad copy; WRAP: the x-fastest grid.
+// spot-lit gather: grad and lam_grad from the saved c_i and u_i planes
+hipError_t render_density_lit_gather_launch(hipStream_t s, const RenderSaved& sv, const float* dens,
+                                            const float* lam, float* grad, float* lam_grad, const RayDims& rd,
+                                            const ProjLaunch& PL) {
+  const int nb = ((rd.in.x + kGatherWX - 1) / kGatherWX) * ((rd.in.y + kGatherWY - 1) / kGatherWY) *
+                 ((rd.in.z + kGatherWZ - 1) / kGatherWZ);
+  KernelTimer t(s, "render_lit_bwd_gather");
+  auto kern = k_render_density_bwd_gather<kGatherWX, kGatherWY, kGatherWZ, true>;
+  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL, dens, lam, lam_grad);
+  return hipGetLastError();
+}
+
+// scatter backward (grad must be zero on entry): BORDER and CLAMP walk the
+// corner-quad copy of that boundary in dens; WRAP reads the x-fastest grid.
 hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
                                          const float* second, float* grad, float* loss, const RayDims& rd,
                                          const RayLaunch& L, const RenderParams& rp) {
@@ -816,6 +1008,11 @@ hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const f
     const QuadDims q = make_quad_dims(rd.in);
     if (l2) hipLaunchKernelGGL(k_render_density_bwd_clamp<true>, grid, dim3(256), 0, s, quad, q, img, second, grad, loss, rd, L, rp);
     else hipLaunchKernelGGL(k_render_density_bwd_clamp<false>, grid, dim3(256), 0, s, quad, q, img, second, grad, loss, rd, L, rp);
+  } else if (bm == B_BORDER) {
+    const float4* quad = reinterpret_cast<const float4*>(dens);
+    const QuadDims q = make_quad_dims(rd.in);
+    if (l2) hipLaunchKernelGGL((k_render_density_bwd_clamp<true, B_BORDER>), grid, dim3(256), 0, s, quad, q, img, second, grad, loss, rd, L, rp);
+    else hipLaunchKernelGGL((k_render_density_bwd_clamp<false, B_BORDER>), grid, dim3(256), 0, s, quad, q, img, second, grad, loss, rd, L, rp);
   } else {
     if (l2) hipLaunchKernelGGL(k_render_density_bwd_wrap<true>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);
     else hipLaunchKernelGGL(k_render_density_bwd_wrap<false>, grid, dim3(256), 0, s, dens, img, second, grad, loss, rd, L, rp);

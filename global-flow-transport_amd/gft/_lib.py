@@ -68,6 +68,15 @@ _SIGNATURES = {
                                                    _vp]),
     "gft_render_density_bwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
                                               _i32, _vp, _vp, _vp, _sz, _vp, _sz, _vp]),
+    "gft_render_density_lit_fwd_workspace": (_sz, [_i64p, _i32]),
+    "gft_render_density_lit_saved_bytes": (_sz, [_i64p, _i32, _i32p, _i32]),
+    "gft_render_density_lit_fwd": (ctypes.c_int, [_vp, _vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p,
+                                                  _f32, _f32, _i32, _vp, _sz, _vp, _sz, _vp]),
+    "gft_render_density_lit_bwd_loss": (ctypes.c_int, [_vp, _vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p,
+                                                       _f32p, _f32, _f32, _i32, _vp, _vp, _vp, _i32, _f32, _vp, _sz,
+                                                       _vp, _vp, _vp]),
+    "gft_render_density_lit_bwd": (ctypes.c_int, [_vp, _vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p,
+                                                  _f32, _f32, _i32, _vp, _vp, _vp, _sz, _vp, _vp]),
     "gft_mip_atlas_bytes": (_sz, [_i64p, _i32]),
     "gft_build_mips": (ctypes.c_int, [_vp, _vp, _i64p, _i32, _vp, _sz]),
     "gft_sample_grid_transform_workspace": (_sz, [_i64p, _i32, _i32]),

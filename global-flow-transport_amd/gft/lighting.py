@@ -42,7 +42,15 @@ class PointLight(Light):
         self.r = range_limit
         self.s = range_scale
 
+    def falloff_factor(self, density_transform):
+        """[1,D,H,W,1] float32 falloff 1 / (1 + (|p - l| s)^2) without the colour
+        (lighting.py:54-73)."""
+        return self._falloff_impl(density_transform, None)
+
     def _falloff(self, density_transform):
+        return self._falloff_impl(density_transform, self.color_intensity())
+
+    def _falloff_impl(self, density_transform, color):
         data = density_transform.data
         D, H, W = data.shape[1:4]
         cell = density_transform.cell_size_world()
@@ -55,11 +63,18 @@ class PointLight(Light):
         z = (torch.arange(D, dtype=torch.float64, device=dev) + 0.5) * cell[2] - lpos[2]
         dist = torch.sqrt(z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2)
         fall = 1.0 / (1.0 + (dist * self.s) ** 2)
-        c = torch.tensor(self.color_intensity(), dtype=torch.float64, device=dev)
+        if color is None:
+            return fall[..., None].to(torch.float32)[None]
+        c = torch.tensor(color, dtype=torch.float64, device=dev)
         return (fall[..., None] * c).to(torch.float32)[None]
 
     def grid_lighting(self, density_transform, renderer=None):
         return density_transform.data * self._falloff(density_transform)
+
+    def light_factor(self, density_transform, renderer):
+        """Per-cell factor of this light without colour and intensity (the
+        falloff), [1,D,H,W,1]."""
+        return self.falloff_factor(density_transform)
 
 
 class SpotLight(PointLight):
@@ -127,3 +142,9 @@ class SpotLight(PointLight):
     def grid_lighting(self, density_transform, renderer=None):
         light = self.render_shadow_map(density_transform, renderer)
         return light * super().grid_lighting(density_transform, renderer)
+
+    def light_factor(self, density_transform, renderer):
+        """Per-cell factor of this light without colour and intensity: the
+        shadow transmittance on the grid times the falloff, [1,D,H,W,1]."""
+        t = self.render_shadow_map(density_transform, renderer)
+        return t.reshape(density_transform.data.shape) * self.falloff_factor(density_transform)

@@ -193,6 +193,70 @@ def render_density(d, matrix_m, matrix_v, matrix_p, frustum_params, output_shape
     return _RenderFn.apply(d, cams, float(depth_step), tuple(float(x) for x in light), boundary)
 
 
+# ----------------------------------------------------------- lit render
+def render_lit_saved_bytes(shape, cams):
+    return int(_lib.lib().gft_render_density_lit_saved_bytes(_lib.i64_array(shape), cams.nCam, cams.c_out,
+                                                             BOUNDARY["BORDER"]))
+
+
+def render_lit_forward(d, lam, cams, depth_step, color=(1.0, 1.0, 1.0), ambient=0.0, out=None, ws=None, saved=None):
+    """Images of a density under a light with per-cell factor ``lam`` plus an
+    ambient term (gft_render_density_lit_fwd, BORDER)."""
+    L = _lib.lib()
+    if out is None:
+        out = torch.empty((1, cams.nCam, cams.out_dhw[1], cams.out_dhw[2], 4), device=d.device, dtype=torch.float32)
+    lt = (ctypes.c_float * 3)(*[float(x) for x in color])
+    shape = _lib.i64_array(d.shape)
+    nws = int(L.gft_render_density_lit_fwd_workspace(shape, BOUNDARY["BORDER"]))
+    if ws is None:
+        ws = torch.empty((max(nws, 1),), dtype=torch.uint8, device=d.device)
+    sp, sn = (_lib.ptr(saved), saved.numel()) if saved is not None else (None, 0)
+    _lib.check(L.gft_render_density_lit_fwd(_lib.stream_of(d), _lib.ptr(d), _lib.ptr(lam), shape, _np_ptr(cams.M),
+                                            cams.nM, _np_ptr(cams.V), _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam,
+                                            cams.c_out, lt, float(ambient), float(depth_step), BOUNDARY["BORDER"],
+                                            _lib.ptr(ws), nws, sp, sn, _lib.ptr(out)))
+    return out
+
+
+class _LitRenderFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, d, lam, cams, depth_step, color, ambient):
+        saved = torch.empty((render_lit_saved_bytes(d.shape, cams),), dtype=torch.uint8, device=d.device)
+        img = render_lit_forward(d, lam, cams, depth_step, color, ambient, saved=saved)
+        ctx.save_for_backward(d, lam, img, saved)
+        ctx.cfg = (cams, depth_step, color, ambient)
+        return img
+
+    @staticmethod
+    def backward(ctx, g):
+        d, lam, img, saved = ctx.saved_tensors
+        cams, depth_step, color, ambient = ctx.cfg
+        gd, gl = torch.empty_like(d), torch.empty_like(lam)
+        lt = (ctypes.c_float * 3)(*[float(x) for x in color])
+        saved = saved.clone()  # the backward overwrites it; a retained graph may run backward again
+        _lib.check(_lib.lib().gft_render_density_lit_bwd(
+            _lib.stream_of(d), _lib.ptr(d), _lib.ptr(lam), _lib.i64_array(d.shape), _np_ptr(cams.M), cams.nM,
+            _np_ptr(cams.V), _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt, float(ambient),
+            float(depth_step), BOUNDARY["BORDER"], _lib.ptr(img), _lib.ptr(g.contiguous()), _lib.ptr(saved),
+            saved.numel(), _lib.ptr(gd), _lib.ptr(gl)))
+        return gd, gl, None, None, None, None
+
+
+def render_density_lit(d, light_factor, matrix_m, matrix_v, matrix_p, frustum_params, output_shape, depth_step,
+                       color=(1.0, 1.0, 1.0), ambient=0.0):
+    """Images [1,V,H,W,4] = (color * E, optical depth) of a density [1,D,H,W,1]
+    whose light grid is color * d * (light_factor + ambient) (a spot or point
+    light with its per-cell factor plus an ambient light of the same colour);
+    BORDER; differentiable (exact) w.r.t. the density and the light factor."""
+    d = _scalar_grid(d, "density")
+    lam = _scalar_grid(light_factor, "light_factor")
+    if tuple(lam.shape) != tuple(d.shape):
+        raise _lib.InvalidArgumentError("light_factor shape %s differs from the density shape %s" %
+                                        (tuple(lam.shape), tuple(d.shape)))
+    cams = _Cameras(matrix_m, matrix_v, matrix_p, frustum_params, output_shape)
+    return _LitRenderFn.apply(d, lam, cams, float(depth_step), tuple(float(x) for x in color), float(ambient))
+
+
 # ------------------------------------------------------------- fused step
 LOSS = {"SE": 0, "AE": 1}
 
@@ -209,12 +273,23 @@ class TransportRenderStep(object):
     ``loss_scale`` (default 1 / (views * pixels * 3), the mean). With a
     ``background`` ([1,V,H,W,3] tensor or an rgb triple) the loss compares
     rgb + background * exp(-depth) with the targets (loss_dens_target_raw,
-    optimization.py:414-430)."""
+    optimization.py:414-430).
+
+    ``reduction`` picks how the BORDER render backward reduces the sample
+    gradients into the density gradient: "gather" (the forward saves two
+    things per sample and pixel, the backward sums per voxel block in a
+    fixed order: deterministic) or "scatter" (nothing saved: the backward
+    walks every ray from far to near recomputing the transmittance and
+    scatters with an LDS pre-reduction and float atomics, whose order varies
+    between runs). CLAMP and WRAP always scatter."""
 
     def __init__(self, grid_dhw, matrix_m, matrix_v, matrix_p, frustum_params, output_shape, depth_step,
                  light=(1.0, 1.0, 1.0), order=2, clamp="MC_SMOOTH", dt=1.0, boundary="BORDER", device="cuda",
-                 loss="SE", background=None, loss_scale=None, storage="float32"):
+                 loss="SE", background=None, loss_scale=None, storage="float32", reduction="gather"):
         D, H, W = grid_dhw
+        if reduction not in ("gather", "scatter"):
+            raise _lib.InvalidArgumentError("reduction must be 'gather' or 'scatter'")
+        self.reduction = reduction
         if storage not in STORAGE:
             raise _lib.InvalidArgumentError("storage must be one of %s" % sorted(STORAGE))
         self.storage = STORAGE[storage]
@@ -252,7 +327,10 @@ class TransportRenderStep(object):
         self.saved = torch.empty((self._nsaved,), dtype=torch.uint8, device=dev)
         self.ws = torch.empty((max(self._nws, 1),), dtype=torch.uint8, device=dev)
         self._nrsaved = L.gft_render_density_saved_bytes(self._shape, self.cams.nCam, self.cams.c_out, self.boundary)
+        if reduction == "scatter":
+            self._nrsaved = 0
         self.render_saved = torch.empty((max(self._nrsaved, 1),), dtype=torch.uint8, device=dev)
+        self._rsp = _lib.ptr(self.render_saved) if self._nrsaved else None
         # one workspace for the render forward and backward (stream-ordered): the
         # corner-quad copy each of them writes
         self._nfws = max(L.gft_render_density_fwd_workspace(self._shape, self.boundary),
@@ -280,7 +358,7 @@ class TransportRenderStep(object):
         _lib.check(L.gft_render_density_fwd(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
                                             _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light, self.depth_step,
                                             self.boundary, _lib.ptr(self.render_fwd_ws), self._nfws,
-                                            _lib.ptr(self.render_saved), self._nrsaved, _lib.ptr(self.images)))
+                                            self._rsp, self._nrsaved, _lib.ptr(self.images)))
         self._mark(timers, "render_fwd", False)
         return self.images
 
@@ -294,7 +372,7 @@ class TransportRenderStep(object):
                                                  _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light,
                                                  self.depth_step, self.boundary, _lib.ptr(self.images),
                                                  _lib.ptr(target), bg, self.loss_kind, self.loss_scale,
-                                                 _lib.ptr(self.render_saved), self._nrsaved,
+                                                 self._rsp, self._nrsaved,
                                                  _lib.ptr(self.render_fwd_ws), self._nfws,
                                                  _lib.ptr(self.grad_d1), _lib.ptr(self.loss)))
         self._mark(timers, "render_bwd", False)

@@ -25,6 +25,7 @@ from .camera import camera_params_batch
 from .lighting import Light
 from .ops import blending_ops, lighting_ops, raymarching_ops, sampling_ops
 from .pipeline import render_density as fused_render_density
+from .pipeline import render_density_lit as fused_render_density_lit
 
 
 def _as_list(x):
@@ -48,6 +49,7 @@ class Renderer(object):
         self.luma = kwargs.get("luma", [0.2126, 0.7152, 0.0722])
         self.allow_fused = kwargs.get("fused", False)
         self.fused_shadow = kwargs.get("fused_shadow", True)
+        self.fused_lit = kwargs.get("fused_lit", True)
         self.scattering_ratio = kwargs.get("scattering_ratio", 1.0)
 
     @property
@@ -330,15 +332,61 @@ class Renderer(object):
         return (not monochrome and len(light_list) > 0
                 and all(type(l) is Light and not l.monochrome for l in light_list))
 
+    def _lit_split(self, light_list, monochrome):
+        """For the fused lit render: (color, lights with a per-cell factor and
+        their scales, ambient scale) when every light is a Light whose
+        colour * intensity is a multiple of one colour and at least one is a
+        point or spot light; None otherwise."""
+        if monochrome or not light_list or not all(isinstance(l, Light) for l in light_list):
+            return None
+        vecs = [np.asarray(l.color_intensity(), dtype=np.float64).reshape(-1) for l in light_list]
+        if any(v.shape != (3,) for v in vecs):
+            return None
+        ref = max(vecs, key=lambda v: float(np.abs(v).max()))
+        if not np.abs(ref).max() > 0:
+            return None
+        color = ref / np.abs(ref).max()
+        scales = []
+        for v in vecs:
+            k = float(v @ color / (color @ color))
+            if np.abs(v - k * color).max() > 1e-6 * max(np.abs(v).max(), 1e-30):
+                return None
+            scales.append(k)
+        factored = [(l, k) for l, k in zip(light_list, scales) if type(l) is not Light]
+        if not factored:
+            return None
+        ambient = float(sum(k for l, k in zip(light_list, scales) if type(l) is Light))
+        return tuple(float(c) for c in color), factored, ambient
+
     def render_density(self, density_transform, light_list, camera_list, cut_alpha=True, background=None,
                        monochrome=False, split_cameras=False, tonemapping="NONE"):
         """Images of the density under the lights, one [1,H,W,C] tensor per
-        camera in camera order; C = 3 (+1 optical depth unless cut_alpha)."""
+        camera in camera order; C = 3 (+1 optical depth unless cut_alpha).
+
+        With ``fused`` and BEER_LAMBERT, an ambient-only light list runs the
+        fused density kernel, and a list of point/spot lights plus ambient
+        lights with proportional colours (the production [SpotLight, Light])
+        runs the fused lit kernel on BORDER: the light grid d * (c lambda +
+        c a) is never materialised; lambda (shadow transmittance times falloff,
+        with the intensities) is built with the differentiable ops as the
+        reference's _build_light_grid does (renderer.py:573-601)."""
         density = density_transform.data
+        single = density.shape[0] == 1 and density.shape[-1] == 1
         fast = (self.allow_fused and self.blend_mode == "BEER_LAMBERT" and self._ambient_only(light_list, monochrome)
-                and density.shape[0] == 1 and density.shape[-1] == 1)
+                and single)
+        lit = None
+        if (self.allow_fused and self.fused_lit and self.blend_mode == "BEER_LAMBERT" and not fast and single
+                and self.boundary_mode == "BORDER"):
+            lit = self._lit_split(light_list, monochrome)
+        lam = None
+        if lit is not None:
+            color, factored, ambient = lit
+            for l, k in factored:
+                f = l.light_factor(density_transform, self) * k
+                lam = f if lam is None else lam + f
+            lam = lam.contiguous()
         data = None
-        if not fast:
+        if not fast and lit is None:
             light_data = self._build_light_grid(density_transform, light_list, monochrome)
             data = torch.cat([light_data, density], dim=-1)
         cam_images = [None] * len(camera_list)
@@ -352,6 +400,11 @@ class Renderer(object):
                     light = np.sum([l.color_intensity() for l in light_list], axis=0)
                     images = fused_render_density(density.contiguous(), M, V, P, F, cam_size, steps[0],
                                                   tuple(float(v) for v in light), self.boundary_mode)[0]
+                elif lit is not None and all(s == steps[0] for s in steps):
+                    M = np.asarray(density_transform.get_transform_matrix(), dtype=np.float32).T[None]
+                    V, P, F = self._get_camera_params_batch(group)
+                    images = fused_render_density_lit(density.contiguous(), lam, M, V, P, F, cam_size, steps[0],
+                                                      color=lit[0], ambient=lit[2])[0]
                 else:
                     if data is None:
                         data = torch.cat([self._build_light_grid(density_transform, light_list, monochrome), density],

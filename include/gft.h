@@ -162,12 +162,15 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
  * loss = loss_scale * sum (rgb - target)^2 (targets [1,V,H,W,3]); writes the
  * scalar loss (device float) and the exact density gradient (overwritten).
  * Replaces loss_dens_target (optimization.py:396-412) + the raymarch gradient
- * (raymarch_grid.cu.cc:343-372). BORDER needs the `saved` buffer that the
- * forward filled from the same density and cameras; the backward overwrites it
- * with per-sample gradients and is deterministic. CLAMP/WRAP need none.
- * CLAMP needs a workspace of gft_render_density_bwd_workspace bytes (the
- * call writes the density's CLAMP corner-quad copy there); it is 0 bytes for
- * BORDER and WRAP.
+ * (raymarch_grid.cu.cc:343-372). BORDER with the `saved` buffer that the
+ * forward filled from the same density and cameras reduces the gradient with
+ * a voxel-block gather (it overwrites `saved`) and is deterministic. BORDER
+ * with saved = NULL (the forward then needs none either), CLAMP and WRAP walk
+ * every ray from far to near, recomputing the transmittance, and scatter with
+ * an LDS pre-reduction and float atomics (summation order varies between
+ * runs). The walk of BORDER and CLAMP needs a workspace of
+ * gft_render_density_bwd_workspace bytes (the call writes the density's
+ * corner-quad copy there); WRAP needs none.
  */
 size_t gft_render_density_bwd_workspace(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                         int32_t boundary);
@@ -200,6 +203,49 @@ int gft_render_density_bwd(void* stream, const float* density, const int64_t sha
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
                            const float* images, const float* images_grad, void* saved, size_t saved_bytes,
                            void* workspace, size_t workspace_bytes, float* density_grad);
+
+/*
+ * Fused density render under a spot or point light plus ambient light:
+ * Renderer.render_density with [SpotLight, Light] (renderer.py:573-651,
+ * lighting.py:23-147) whose light grid is d * (c_spot lambda + c_ambient),
+ * with the two colours proportional: light_color = c, c_spot = c * (the
+ * intensity folded into lambda), c_ambient = c * ambient. light_factor
+ * [1,D,H,W,1] (device) is lambda per cell: the spot intensity times the shadow
+ * transmittance resampled to the grid (SpotLight._render_shadow_map) times the
+ * PointLight falloff. The raymarch is the BEER_LAMBERT one of
+ * gft_render_density_fwd without the materialised 4-channel grid: images
+ * [1,V,H,W,4] = (c * E, optical depth). BORDER only (other boundaries return
+ * GFT_ERR_UNIMPLEMENTED). The workspace (gft_render_density_lit_fwd_workspace)
+ * receives the corner-quad copies of d and d * lambda; a non-null `saved`
+ * (gft_render_density_lit_saved_bytes) receives two values per sample for the
+ * backward, which is deterministic (no atomics) and writes the exact
+ * gradients w.r.t. the density and the light factor (both overwritten); the
+ * caller chains the latter through the shadow map and the falloff.
+ */
+size_t gft_render_density_lit_fwd_workspace(const int64_t shape[5], int32_t boundary);
+size_t gft_render_density_lit_saved_bytes(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
+                                          int32_t boundary);
+int gft_render_density_lit_fwd(void* stream, const float* density, const float* light_factor,
+                               const int64_t shape[5], const float* M, int32_t num_M, const float* V,
+                               const float* P, const float* frustum, int32_t num_cameras,
+                               const int32_t output_shape[3], const float light_color[3], float ambient,
+                               float depth_step, int32_t boundary, void* workspace, size_t workspace_bytes,
+                               void* saved, size_t saved_bytes, float* images);
+/* backward with the image loss folded in, as gft_render_density_bwd_loss */
+int gft_render_density_lit_bwd_loss(void* stream, const float* density, const float* light_factor,
+                                    const int64_t shape[5], const float* M, int32_t num_M, const float* V,
+                                    const float* P, const float* frustum, int32_t num_cameras,
+                                    const int32_t output_shape[3], const float light_color[3], float ambient,
+                                    float depth_step, int32_t boundary, const float* images, const float* targets,
+                                    const float* backgrounds, int32_t loss_kind, float loss_scale, void* saved,
+                                    size_t saved_bytes, float* density_grad, float* light_factor_grad, float* loss);
+/* backward from an image gradient [1,V,H,W,4] */
+int gft_render_density_lit_bwd(void* stream, const float* density, const float* light_factor,
+                               const int64_t shape[5], const float* M, int32_t num_M, const float* V,
+                               const float* P, const float* frustum, int32_t num_cameras,
+                               const int32_t output_shape[3], const float light_color[3], float ambient,
+                               float depth_step, int32_t boundary, const float* images, const float* images_grad,
+                               void* saved, size_t saved_bytes, float* density_grad, float* light_factor_grad);
 
 /*
  * Mip pyramid of a grid batch [N,D,H,W,C]: levels 1..L (L = num_mipmaps,
@@ -413,6 +459,7 @@ int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
  * render_fwd, render_bwd_pixel, render_bwd_gather, render_bwd_scatter, quad_copy, fill,
+ * render_lit_fwd, render_lit_bwd_gather,
  * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam, transform_grad_gather, lut_grad.
  */
 int gft_timer_start(const char* kernels);

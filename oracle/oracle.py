@@ -340,6 +340,32 @@ def render_density_bwd(d, out, out_grad, M, V, P, F, out_dhw, depth_step, light_
     return depth_step * (g4[..., :3] @ cvec + g4[..., 3])[..., None]
 
 
+def render_lit_fwd(d, lam, M, V, P, F, out_dhw, depth_step, color=(1.0, 1.0, 1.0), ambient=0.0,
+                   boundary="BORDER", prec=1):
+    """Density render under a spot/point light and an ambient light whose
+    colours are proportional (Renderer.render_density over _build_light_grid,
+    renderer.py:573-651; SpotLight.grid_lighting lighting.py:143-147): light
+    grid = color * d * (lam + ambient) with lam the light's per-cell factor,
+    grid = [light, d] * depth_step, BEER_LAMBERT raymarch."""
+    d, lam = _d(d), _d(lam)
+    c = np.asarray(color, dtype=np.float64)
+    grid = np.concatenate([d * (lam + ambient) * c, d], axis=-1) * depth_step
+    return raymarch_fwd(grid, M, V, P, F, out_dhw, "LINEAR", boundary, "BEER_LAMBERT", True, prec)
+
+
+def render_lit_bwd(d, lam, out, out_grad, M, V, P, F, out_dhw, depth_step, color=(1.0, 1.0, 1.0), ambient=0.0,
+                   boundary="BORDER", grad_mode="EXACT", prec=1):
+    """Gradients of render_lit_fwd w.r.t. the density and the light factor:
+    the raymarch gradient g4 of the 4-channel grid chained through
+    light = color * d * (lam + ambient) * step and sigma = d * step."""
+    d, lam = _d(d), _d(lam)
+    c = np.asarray(color, dtype=np.float64)
+    grid = np.concatenate([d * (lam + ambient) * c, d], axis=-1) * depth_step
+    g4 = raymarch_bwd(grid, out, out_grad, M, V, P, F, out_dhw, boundary, "BEER_LAMBERT", True, grad_mode, prec)
+    gl = (g4[..., :3] @ c)[..., None] * depth_step
+    return gl * (lam + ambient) + depth_step * g4[..., 3:4], gl * d
+
+
 def l2_loss_and_grad(img, target):
     """loss = mean((rgb - target)^2); returns (loss, d loss / d img) with the
     optical-depth channel receiving zero gradient."""

@@ -179,12 +179,16 @@ def test_validation_returns_before_device_work(lib):
                                          ctypes.c_float(1.0), 1, p16, p16, null, 7, ctypes.c_float(1.0), null, 0, null,
                                          0, p16, p16)
     assert rc == _lib.GFT_ERR_INVALID_ARGUMENT and b"loss_kind" in lib.gft_last_error()
-    # the CLAMP render needs its corner-quad copy in both directions; BORDER and WRAP backwards need none
+    # the CLAMP render needs its corner-quad copy in both directions, BORDER for the walk without a
+    # saved buffer; WRAP needs none; the lit render holds two copies and a second per-sample plane
     sh = _i64(1, 4, 5, 6, 1)
     nq = lib.gft_render_density_fwd_workspace(sh, 1)
     assert nq > 0 and lib.gft_render_density_bwd_workspace(sh, 1, _i32(4, 4, 4), 1) == nq
-    assert lib.gft_render_density_bwd_workspace(sh, 1, _i32(4, 4, 4), 0) == 0
+    assert lib.gft_render_density_bwd_workspace(sh, 1, _i32(4, 4, 4), 0) == nq
     assert lib.gft_render_density_bwd_workspace(sh, 1, _i32(4, 4, 4), 2) == 0
+    assert lib.gft_render_density_lit_fwd_workspace(sh, 0) == 2 * nq
+    ns = lib.gft_render_density_saved_bytes(sh, 2, _i32(4, 4, 4), 0)
+    assert lib.gft_render_density_lit_saved_bytes(sh, 2, _i32(4, 4, 4), 0) == ns + 512  # 2 views x 4x4x4 floats
     assert lib.gft_render_density_fwd_workspace(sh, 2) == 0
     rc = lib.gft_render_density_bwd_loss(null, p16, sh, vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), light,
                                          ctypes.c_float(1.0), 1, p16, p16, null, 0, ctypes.c_float(1.0), null, 0, null,

@@ -279,3 +279,33 @@ def test_uneven_reconstructions_in_one_graph(cuda, branches):
             assert abs(st.loss.item() - lo.item()) <= 1e-6 * abs(lo.item())
             assert norm_err(st.grad_density.cpu().numpy(), gd.cpu().numpy()) < 1e-5
             assert norm_err(st.grad_velocity.cpu().numpy(), gv.cpu().numpy()) < 1e-5
+
+
+@pytest.mark.parametrize("loss", ["SE", "AE"])
+def test_border_scatter_reduction_matches_gather_and_oracle(cuda, loss):
+    """The BORDER render backward reduced two ways: the deterministic
+    voxel-block gather over the forward's saved per-sample values, and the
+    reverse walk that recomputes the transmittance and scatters with float
+    atomics (nothing saved). Both against the FP64 oracle (loss 1e-5,
+    gradients 1e-3) and against each other (1e-5); the gather repeats bit for
+    bit."""
+    from gft.pipeline import TransportRenderStep
+    grid = (18, 26, 22)
+    sc = scene(grid=grid, image=(24, 18), views=3, seed=90)
+    d, v = _fields(grid, seed=91)
+    tgt = orc.render_density_fwd(syn.gaussian_blobs(grid, 5, seed=92), sc["M"], sc["V"], sc["P"], sc["F"],
+                                 sc["out_dhw"], sc["step"])[..., :3]
+    ins = (torch.from_numpy(d).cuda(), torch.from_numpy(v).cuda(), torch.from_numpy(tgt.astype(np.float32)).cuda())
+    out = {}
+    for red in ("gather", "scatter"):
+        st = TransportRenderStep(grid, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], loss=loss,
+                                 reduction=red)
+        st.run(*ins)
+        torch.cuda.synchronize()
+        out[red] = (st.loss.item(), st.grad_d1.cpu().numpy(), st.grad_density.cpu().numpy())
+    r = orc.step_fwd_bwd(d, v, tgt, sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"], loss_kind=loss)
+    for red in out:
+        assert abs(out[red][0] - r["loss"]) <= 1e-5 * abs(r["loss"]), red
+        assert norm_err(out[red][1], r["gd1"]) < 1e-3, red
+        assert norm_err(out[red][2], r["gd"]) < 1e-3, red
+    assert norm_err(out["scatter"][1], out["gather"][1]) < 1e-5

@@ -812,6 +812,36 @@ def test_chain_reference_recursion_closed_form(decay):
     assert np.array_equal(r1["gd0"], g[0])
 
 
+# ----------------------------------------------------- lit density render
+def test_lit_render_reduces_to_the_ambient_render_and_fd():
+    """A constant light factor k with ambient a is the ambient render of
+    intensity k + a (closed form of the light grid); the gradients w.r.t. the
+    density and the light factor are the central differences of <img, w>."""
+    from helpers import scene
+    sc = scene(grid=(6, 7, 5), image=(6, 5), views=2, seed=23)
+    d = sc["data"].astype(np.float64)
+    args = (sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"])
+    color, a = (0.9, 0.6, 0.3), 0.4
+    lam = np.full(d.shape, 0.7)
+    img = orc.render_lit_fwd(d, lam, *args, color=color, ambient=a)
+    ref = orc.render_density_fwd(d, *args, light_color=color, light_intensity=0.7 + a)
+    assert np.abs(img - ref).max() <= 1e-13 * np.abs(ref).max()
+    lam = RNG.uniform(0, 1.5, size=d.shape)
+    img = orc.render_lit_fwd(d, lam, *args, color=color, ambient=a)
+    w = RNG.uniform(-1, 1, size=img.shape)
+    gd, gl = orc.render_lit_bwd(d, lam, img, w, *args, color=color, ambient=a)
+
+    def loss_d(x):
+        return float((orc.render_lit_fwd(x, lam, *args, color=color, ambient=a) * w).sum())
+
+    def loss_l(x):
+        return float((orc.render_lit_fwd(d, x, *args, color=color, ambient=a) * w).sum())
+    for g, fn, x in ((gd, loss_d, d), (gl, loss_l, lam)):
+        assert np.abs(g).max() > 0
+        for idx in np.argsort(-np.abs(g).ravel())[:5]:
+            assert abs(_fd(fn, x, int(idx), 1e-6) - g.flat[idx]) <= 1e-6 * np.abs(g).max()
+
+
 # ------------------------------------------------- loss / optimizer epilogue
 @pytest.mark.parametrize("kind", ["SE", "AE"])
 @pytest.mark.parametrize("with_bkg", [False, True])
