@@ -90,15 +90,58 @@ def build_inputs(name):
                 step=float(cams[0].depth_step), out_dhw=(S,) + tuple(image))
 
 
-def cpu_baseline(inp, target_np, boundary="BORDER"):
-    """The FP32 oracle (C++/OpenMP restatement of the reference ops) timed on
-    one full fwd+bwd step of the same workload."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(inp, target_np, nvs, ray_samples, boundary="BORDER", warmups=2, runs=5):
+    """The FP32 oracle (C++/OpenMP restatement of the reference ops) on the
+    same workload: all host threads on one full fwd+bwd step (warm-ups, then
+    the median of `runs`), and one thread on a bounded sample of it (the first
+    view: the whole advection and one view's render, median of 3 after one
+    warm-up). Returns the cpu_baseline record (voxel-samples/s)."""
+    import ctypes
+
+    import numpy as np
     from oracle import oracle as orc
-    threads = int(orc.lib().orc_num_threads())
-    t0 = time.perf_counter()
-    orc.step_fwd_bwd(inp["d0"], inp["vel"], target_np, inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"],
-                     inp["step"], prec=0, boundary_render=boundary)
-    return time.perf_counter() - t0, threads
+    L = orc.lib()
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    threads = int(L.orc_num_threads())
+    V = int(np.prod(inp["d0"].shape))
+
+    def timed(views, n_warm, n_runs):
+        sl = slice(0, views)
+        args = (inp["d0"], inp["vel"], target_np[:, sl], inp["M"], inp["V"][sl], inp["P"][sl], inp["F"][sl],
+                inp["out_dhw"], inp["step"])
+        ts = []
+        for k in range(n_warm + n_runs):
+            t0 = time.perf_counter()
+            orc.step_fwd_bwd(*args, prec=0, boundary_render=boundary)
+            if k >= n_warm:
+                ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    nviews = int(inp["V"].shape[0])
+    t_all = timed(nviews, warmups, runs)
+    L.orc_set_threads(1)
+    try:
+        t_one = timed(1, 1, 3)
+    finally:
+        L.orc_set_threads(threads)
+    nvs_one = 6 * V + 2 * ray_samples // nviews
+    return {"value": nvs / t_all, "unit": "voxel-samples/s", "cores": threads, "kind": "port",
+            "sample": "one full fwd+bwd step of the same workload with the FP32 oracle, median of %d runs after %d "
+                      "warm-ups (%.2f s per step)" % (runs, warmups, t_all),
+            "single_thread": {"value": nvs_one / t_one, "cores": 1,
+                              "sample": "the advection and the first view's render and backward, median of 3 after "
+                                        "one warm-up (%.2f s)" % t_one},
+            "cpu_model": _cpu_model()}
 
 
 def measured_traffic(kernel, workload):
@@ -772,6 +815,9 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--boundary", default="BORDER", choices=["BORDER", "CLAMP", "WRAP"],
                     help="render boundary of the fused step (the reference renderer's default is BORDER)")
+    ap.add_argument("--reduction", default="gather", choices=["gather", "scatter"],
+                    help="BORDER render backward: deterministic gather over saved per-sample values, or the "
+                         "recompute walk with float atomics")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -823,7 +869,7 @@ def main():
     d0 = torch.from_numpy(inp["d0"]).to(dev)
     vel = torch.from_numpy(inp["vel"]).to(dev)
     step = TransportRenderStep(inp["grid"], inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"],
-                               boundary=args.boundary)
+                               boundary=args.boundary, reduction=args.reduction)
     # targets: render of the second density through the same kernels
     cams = _Cameras(inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"])
     tgt_img = render_density_forward(torch.from_numpy(inp["d_target"]).to(dev), cams, inp["step"])
@@ -888,6 +934,7 @@ def main():
             "voxel_samples_per_step": nvs,
             "grad_mode": "EXACT",
             "render_boundary": args.boundary,
+            "render_reduction": args.reduction,
         },
         "roofline": {
             "bound": "hbm",
@@ -929,10 +976,8 @@ def main():
 
     if not args.no_cpu_baseline:
         tgt_np = target.cpu().numpy().astype(np.float64)
-        secs, threads = cpu_baseline(inp, tgt_np, args.boundary)
-        result["cpu_baseline"] = {"value": nvs / secs, "unit": "voxel-samples/s", "cores": threads, "kind": "port",
-                                  "sample": "one full fwd+bwd step of the same workload with the FP32 oracle "
-                                            "(%.1f s)" % secs}
+        ray = step.n_pixels * step.cams.out_dhw[0]
+        result["cpu_baseline"] = cpu_baseline(inp, tgt_np, nvs, ray, args.boundary)
     print(json.dumps(result))
 
 

@@ -1216,6 +1216,15 @@ void lod_transform(const int64_t* in_dhw, const float* MV, const float* P, const
 extern "C" {
 
 int orc_num_threads() { return orc::nthreads(); }
+// thread count of the later OpenMP regions of the calling thread (the CPU
+// baseline's single-thread figure)
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
 
 #define ORC_DISPATCH(prec, call) \
   do {                           \
