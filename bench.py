@@ -35,8 +35,10 @@ libgft's render kernels.
 with the density and velocity stored in bfloat16: ms per fused step against
 float32 storage, and the errors of images, loss and gradients against the
 FP64 oracle on the bfloat16 values (the float32 arithmetic) and on the
-unrounded inputs (arithmetic and storage), for the BORDER render's
-deterministic segmented gather and the CLAMP render's float32-atomic scatter.
+unrounded inputs (arithmetic and storage). Variants: the render sampling the
+advected density stored in bfloat16 (gather reduction), and the same BORDER
+render gradient on a float32 render reduced two ways, the deterministic
+segmented gather and the reverse walk with float32 atomics.
 
 ``--batched`` instead runs workload c4: 32 independent reconstructions of
 uneven grid sizes (32^3 to 128x224x128) and view counts (1-8), each a fused
@@ -666,36 +668,52 @@ def precision_bench(args):
         return float(np.mean(np.abs(a - b) > tol * np.abs(b).max()))
 
     res = {}
-    for boundary, reduction in (("BORDER", "deterministic segmented gather"), ("CLAMP", "float32-atomic scatter")):
-        step16 = TransportRenderStep(*cam_args, boundary=boundary, storage="bfloat16")
-        step32 = TransportRenderStep(*cam_args, boundary=boundary)
+    # the BORDER render gradient reduced two ways on the same render (the density sampled at float32):
+    # the deterministic voxel-block gather and the reverse walk with float32 atomics; and the headline
+    # variant whose render samples the advected density stored in bfloat16
+    variants = (("bf16_render_gather", "BORDER", "gather", "bfloat16", "deterministic segmented gather"),
+                ("fp32_render_gather", "BORDER", "gather", "float32", "deterministic segmented gather"),
+                ("fp32_render_scatter", "BORDER", "scatter", "float32", "float32-atomic scatter (reverse walk)"))
+    oracle_cache = {}
+    for key, boundary, reduction, rstore, desc in variants:
+        step16 = TransportRenderStep(*cam_args, boundary=boundary, storage="bfloat16", reduction=reduction,
+                                     render_storage=rstore)
+        step32 = TransportRenderStep(*cam_args, boundary=boundary, reduction=reduction)
         ms16, ms32 = timed(step16, d16, v16), timed(step32, d32, v32)
         loss, gd, gv = step16.run(d16, v16, target)
         torch.cuda.synchronize()
         got = dict(loss=float(loss.item()), img=step16.images.cpu().numpy(), gd=gd.cpu().numpy(),
                    gv=gv.cpu().numpy())
-        say("%s timed: %.3f ms (bf16), %.3f ms (fp32)" % (boundary, ms16, ms32))
+        say("%s timed: %.3f ms (bf16 storage), %.3f ms (fp32)" % (key, ms16, ms32))
         errs = {}
         for name, (d64, v64) in (("vs_fp64_on_bf16_values", rounded), ("vs_fp64_on_unrounded_inputs", exact)):
-            r = orc.step_fwd_bwd(d64, v64, tgt64, inp["M"], inp["V"], inp["P"], inp["F"], inp["out_dhw"], inp["step"],
-                                 boundary_render=boundary)
+            # on the bf16 values the oracle stores the advected density as the step does; on the
+            # unrounded inputs it is the FP64 step itself
+            rs = rstore if name == "vs_fp64_on_bf16_values" else "float32"
+            ck = (name, rs)
+            if ck not in oracle_cache:
+                oracle_cache[ck] = orc.step_fwd_bwd(d64, v64, tgt64, inp["M"], inp["V"], inp["P"], inp["F"],
+                                                    inp["out_dhw"], inp["step"], boundary_render=boundary,
+                                                    render_storage=rs)
+                say("oracle %s %s done" % ck)
+            r = oracle_cache[ck]
             errs[name] = {"images": nerr(got["img"], r["img"]), "loss": abs(got["loss"] - r["loss"]) / abs(r["loss"]),
                           "grad_density": nerr(got["gd"], r["gd"]), "grad_velocity": nerr(got["gv"], r["gv"]),
                           "grad_density_frac_beyond_1e-3": frac(got["gd"], r["gd"]),
                           "grad_velocity_frac_beyond_1e-3": frac(got["gv"], r["gv"])}
-            say("%s oracle %s done" % (boundary, name))
-        res[boundary] = {"reduction": reduction, "ms_bf16": ms16, "ms_fp32": ms32, "errors": errs}
+        res[key] = {"boundary": boundary, "reduction": desc, "render_storage": rstore, "ms_bf16_storage": ms16,
+                    "ms_fp32_storage": ms32, "errors": errs}
         del step16, step32
         torch.cuda.empty_cache()
     nvs = TransportRenderStep(*cam_args, storage="bfloat16").voxel_samples()
-    ms = res["BORDER"]["ms_bf16"]
+    ms = res["bf16_render_gather"]["ms_bf16_storage"]
     print(json.dumps({
         "metric": "voxel-samples/s fwd+bwd, density and velocity stored in bfloat16", "value": nvs / (ms * 1e-3),
         "unit": "voxel-samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "dtype": "bf16 storage, f32 arithmetic", "data": "synthetic",
         "config": {"workload": "c5: grid 200x356x200 (x,y,z), 8 views 540x960 px, %d samples/ray, MacCormack+"
                                "MC_SMOOTH, L2 loss" % inp["S"], "voxel_samples_per_step": nvs},
-        "boundaries": res}))
+        "variants": res}))
 
 
 def lit_bench(args):

@@ -29,6 +29,10 @@ hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const f
                                          const RayLaunch& L, const RenderParams& rp);
 hipError_t fill_launch(hipStream_t s, float* p, size_t n, float v);
 hipError_t quad_copy_launch(hipStream_t s, int bm, const float* lin, float* out, Dims3 d, const float* fac = nullptr);
+size_t quad_copy_bf16_bytes(Dims3 d);
+hipError_t quad_copy_bf16_launch(hipStream_t s, const float* lin, void* out, Dims3 d);
+hipError_t render_density_fwd_bf16_launch(hipStream_t s, const void* quad, float* img, const RayDims& rd,
+                                          const RayLaunch& L, const RenderParams& rp, const RenderSaved& sv);
 hipError_t render_density_lit_fwd_launch(hipStream_t s, const float* dq, const float* eq, float* img,
                                          const RayDims& rd, const RayLaunch& L, const RenderParams& rp,
                                          const RenderSaved& sv);
@@ -321,6 +325,43 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
     if ((rc = ray_views(pairs, first, cnt, M, V, P, L))) return rc;
     if ((rc = check_hip(render_density_fwd_launch(boundary, (hipStream_t)stream, density, images, rd, L, rp, sv),
                         "render forward")))
+      return rc;
+  }
+  return GFT_OK;
+}
+
+int gft_render_density_fwd_bf16(void* stream, const float* density, const int64_t shape[5], const float* M,
+                                int32_t num_M, const float* V, const float* P, const float* frustum,
+                                int32_t num_cameras, const int32_t output_shape[3], const float light[3],
+                                float depth_step, int32_t boundary, void* workspace, size_t workspace_bytes,
+                                void* saved, size_t saved_bytes, float* images) {
+  (void)frustum;
+  int rc;
+  if ((rc = check_render_args(shape, num_M, num_cameras, output_shape, boundary))) return rc;
+  if (boundary != GFT_BORDER)
+    return fail(GFT_ERR_UNIMPLEMENTED, "the bfloat16 density render supports the BORDER boundary only");
+  if (!density || !images || !M || !V || !P || !light) return fail(GFT_ERR_INVALID_ARGUMENT, "null pointer argument");
+  const RayDims rd = render_dims(shape, output_shape);
+  const size_t need = align256(quad_copy_bf16_bytes(rd.in));
+  if (!workspace || workspace_bytes < need)
+    return fail(GFT_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required", workspace_bytes,
+                need);
+  const size_t need_saved = gft_render_density_saved_bytes(shape, num_cameras, output_shape, boundary);
+  if (saved && saved_bytes < need_saved)
+    return fail(GFT_ERR_WORKSPACE, "saved buffer of %zu bytes is smaller than the %zu bytes required", saved_bytes,
+                need_saved);
+  RenderSaved sv{};
+  if (saved) sv = saved_parts(saved, shape, output_shape, num_cameras);
+  const RenderParams rp{{light[0], light[1], light[2]}, depth_step, 0.f};
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = check_hip(quad_copy_bf16_launch(s, density, workspace, rd.in), "bfloat16 quad copy"))) return rc;
+  auto pairs = view_pairs(1, num_cameras, true);
+  for (size_t first = 0; first < pairs.size(); first += kMaxViewsPerLaunch) {
+    size_t cnt = std::min(pairs.size() - first, (size_t)kMaxViewsPerLaunch);
+    RayLaunch L;
+    if ((rc = ray_views(pairs, first, cnt, M, V, P, L))) return rc;
+    if ((rc = check_hip(render_density_fwd_bf16_launch(s, workspace, images, rd, L, rp, sv),
+                        "bfloat16 render forward")))
       return rc;
   }
   return GFT_OK;

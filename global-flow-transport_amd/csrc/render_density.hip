@@ -63,22 +63,55 @@ __device__ __forceinline__ void pixel_ray(const RayView& v, const RayDims& rd, i
 // the forward is bound by lines per load instruction rather than bytes. At c2
 // the render forward takes 0.209 ms with rows (8 x 1 x 1 or 8 x 1 x 2/4 bricks)
 // and 0.169-0.175 ms with 4 x 1 x 2, 2 x 1 x 4, 4 x 1 x 4, 4 x 1 x 8 or 2 x 2 x 2.
-constexpr unsigned kQBXl = 2, kQBYl = 0, kQBZl = 1;  // log2 of the brick dims
-constexpr unsigned kQBX = 1u << kQBXl, kQBY = 1u << kQBYl, kQBZ = 1u << kQBZl, kQB = kQBX * kQBY * kQBZ;
+//
+// With the density stored in bfloat16 the copy holds bfloat16 corners (8-byte
+// entries, uint2: corners 0/1 in x, 2/3 in y, low half first) in 4 x 2 x 2
+// bricks, again one 128-byte line per brick: half the bytes per tap.
+template <class Q> struct QuadLayout;
+template <> struct QuadLayout<float4> {
+  static constexpr unsigned Xl = 2, Yl = 0, Zl = 1;  // log2 of the brick dims
+};
+template <> struct QuadLayout<uint2> {
+  static constexpr unsigned Xl = 2, Yl = 1, Zl = 1;
+};
+template <class Q> struct QuadBrick {
+  static constexpr unsigned Xl = QuadLayout<Q>::Xl, Yl = QuadLayout<Q>::Yl, Zl = QuadLayout<Q>::Zl;
+  static constexpr unsigned X = 1u << Xl, Y = 1u << Yl, Z = 1u << Zl, N = X * Y * Z;
+};
 struct QuadDims {
   unsigned qx, qy, qz;     // entries: W+1, H+1, D+2
   unsigned nbx, nby, nbz;  // bricks along each axis
 };
-__host__ __device__ inline QuadDims make_quad_dims(Dims3 d) {
+template <class Q = float4> __host__ __device__ inline QuadDims make_quad_dims(Dims3 d) {
+  using B = QuadBrick<Q>;
   const unsigned qx = (unsigned)d.x + 1u, qy = (unsigned)d.y + 1u, qz = (unsigned)d.z + 2u;
-  return QuadDims{qx, qy, qz, (qx + kQBX - 1) / kQBX, (qy + kQBY - 1) / kQBY, (qz + kQBZ - 1) / kQBZ};
+  return QuadDims{qx, qy, qz, (qx + B::X - 1) / B::X, (qy + B::Y - 1) / B::Y, (qz + B::Z - 1) / B::Z};
 }
-__host__ __device__ inline size_t quad_entries(const QuadDims& q) { return (size_t)q.nbx * q.nby * q.nbz * kQB; }
+template <class Q = float4> __host__ __device__ inline size_t quad_entries(const QuadDims& q) {
+  return (size_t)q.nbx * q.nby * q.nbz * QuadBrick<Q>::N;
+}
 // index of entry (ix, iy, iz) (0-based: ix = x + 1, iy = y + 1, iz = z + 1)
+template <class Q = float4>
 __device__ __forceinline__ unsigned quad_index(const QuadDims& q, unsigned ix, unsigned iy, unsigned iz) {
-  const unsigned b = ((iz >> kQBZl) * q.nby + (iy >> kQBYl)) * q.nbx + (ix >> kQBXl);
-  const unsigned o = ((((iz & (kQBZ - 1)) << kQBYl) | (iy & (kQBY - 1))) << kQBXl) | (ix & (kQBX - 1));
-  return b * kQB + o;
+  using B = QuadBrick<Q>;
+  const unsigned b = ((iz >> B::Zl) * q.nby + (iy >> B::Yl)) * q.nbx + (ix >> B::Xl);
+  const unsigned o = ((((iz & (B::Z - 1)) << B::Yl) | (iy & (B::Y - 1))) << B::Xl) | (ix & (B::X - 1));
+  return b * B::N + o;
+}
+
+// float -> bfloat16 bits, rounded to nearest even (finite values)
+__device__ __forceinline__ unsigned bf16_bits(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float4 quad_value(const float4& e) { return e; }
+__device__ __forceinline__ float4 quad_value(const uint2& e) {
+  return make_float4(__uint_as_float(e.x << 16), __uint_as_float(e.x & 0xffff0000u), __uint_as_float(e.y << 16),
+                     __uint_as_float(e.y & 0xffff0000u));
+}
+__device__ __forceinline__ void quad_store(float4* p, float a, float b, float c, float d) { *p = make_float4(a, b, c, d); }
+__device__ __forceinline__ void quad_store(uint2* p, float a, float b, float c, float d) {
+  *p = make_uint2(bf16_bits(a) | (bf16_bits(b) << 16), bf16_bits(c) | (bf16_bits(d) << 16));
 }
 
 // one thread per stored entry (coalesced writes); padding entries of partial
@@ -88,15 +121,16 @@ __device__ __forceinline__ unsigned quad_index(const QuadDims& q, unsigned ix, u
 // clamped into the copy reads the CLAMP value whatever its weights.
 // With `fac` non-null the copy holds the product lin * fac per voxel (the
 // spot-lit emission grid d * lambda of the lit render).
-template <int BM>
+template <int BM, class Q = float4>
 __global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin, const float* __restrict__ fac,
-                                                   float4* __restrict__ quad, Dims3 d, QuadDims q) {
+                                                   Q* __restrict__ quad, Dims3 d, QuadDims q) {
+  using B = QuadBrick<Q>;
   const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= quad_entries(q)) return;
-  const unsigned b = (unsigned)(j / kQB), o = (unsigned)(j % kQB);
-  const unsigned ix = (b % q.nbx) * kQBX + (o & (kQBX - 1));
-  const unsigned iy = ((b / q.nbx) % q.nby) * kQBY + ((o >> kQBXl) & (kQBY - 1));
-  const unsigned iz = (b / (q.nbx * q.nby)) * kQBZ + (o >> (kQBXl + kQBYl));
+  if (j >= quad_entries<Q>(q)) return;
+  const unsigned b = (unsigned)(j / B::N), o = (unsigned)(j % B::N);
+  const unsigned ix = (b % q.nbx) * B::X + (o & (B::X - 1));
+  const unsigned iy = ((b / q.nbx) % q.nby) * B::Y + ((o >> B::Xl) & (B::Y - 1));
+  const unsigned iz = (b / (q.nbx * q.nby)) * B::Z + (o >> (B::Xl + B::Yl));
   const int x = (int)ix - 1, y = (int)iy - 1, z = (int)iz - 1;
   auto at = [&](size_t f) -> float { return fac ? lin[f] * fac[f] : lin[f]; };
   auto v = [&](int xx, int yy) -> float {
@@ -105,7 +139,7 @@ __global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin
                ? at(flat3(xx, yy, z, d))
                : 0.f;
   };
-  quad[j] = make_float4(v(x, y), v(x + 1, y), v(x, y + 1), v(x + 1, y + 1));
+  quad_store(quad + j, v(x, y), v(x + 1, y), v(x, y + 1), v(x + 1, y + 1));
 }
 
 // Branch-free read of U consecutive samples i0, i0+dir, ... of one ray (dir =
@@ -114,8 +148,8 @@ __global__ __launch_bounds__(256) void k_quad_copy(const float* __restrict__ lin
 // `last` (in walk direction) read 0 and report inside = false; so do, with
 // BORDER, samples outside [-0.5, dims-0.5). With CLAMP every other sample is
 // inside and reads the clamped value from the CLAMP copy.
-template <int U, int BM = B_BORDER>
-__device__ __forceinline__ void read_border_batch(const float4* __restrict__ quad, Dims3 d, QuadDims q, float3 n,
+template <int U, int BM = B_BORDER, class Q = float4>
+__device__ __forceinline__ void read_border_batch(const Q* __restrict__ quad, Dims3 d, QuadDims q, float3 n,
                                                   float3 st, int i0, int dir, int last, float val[U],
                                                   bool inside[U]) {
   unsigned a[U], a1[U];
@@ -136,19 +170,20 @@ __device__ __forceinline__ void read_border_batch(const float4* __restrict__ qua
     w[u][2] = pz - fz;
     const unsigned ix = (unsigned)(clampi((int)fx, -1, d.x - 1) + 1), iy = (unsigned)(clampi((int)fy, -1, d.y - 1) + 1),
                    iz = (unsigned)(clampi((int)fz, -1, d.z - 1) + 1);
-    a[u] = quad_index(q, ix, iy, iz);
-    a1[u] = quad_index(q, ix, iy, iz + 1);
+    a[u] = quad_index<Q>(q, ix, iy, iz);
+    a1[u] = quad_index<Q>(q, ix, iy, iz + 1);
   }
-  float4 c0[U], c1[U];
+  Q r0[U], r1[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    c0[u] = quad[a[u]];
-    c1[u] = quad[a1[u]];
+    r0[u] = quad[a[u]];
+    r1[u] = quad[a1[u]];
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const float v0 = lerpf(lerpf(c0[u].x, c0[u].y, w[u][0]), lerpf(c0[u].z, c0[u].w, w[u][0]), w[u][1]);
-    const float v1 = lerpf(lerpf(c1[u].x, c1[u].y, w[u][0]), lerpf(c1[u].z, c1[u].w, w[u][0]), w[u][1]);
+    const float4 c0 = quad_value(r0[u]), c1 = quad_value(r1[u]);
+    const float v0 = lerpf(lerpf(c0.x, c0.y, w[u][0]), lerpf(c0.z, c0.w, w[u][0]), w[u][1]);
+    const float v1 = lerpf(lerpf(c1.x, c1.y, w[u][0]), lerpf(c1.z, c1.w, w[u][0]), w[u][1]);
     val[u] = inside[u] ? lerpf(v0, v1, w[u][2]) : 0.f;
   }
 }
@@ -228,7 +263,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // sample range and E, per sample of the wave's clipped range a value c_i at
 // samples[((slot*S + i)*H + y)*W + x] (a NaN mark for a sample outside the
 // grid).
-template <int BM>
+template <int BM, class Q = float4>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
                                                             RayDims rd, RayLaunch L, RenderParams rp, QuadDims q,
                                                             RenderSaved sv) {
@@ -259,7 +294,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     for (int i = wlo; i <= whi; i += kBatch) {
       float val[kBatch];
       bool in[kBatch];
-      read_border_batch<kBatch, BM>(reinterpret_cast<const float4*>(dens), rd.in, q, nearp, step, i, 1, hi, val, in);
+      read_border_batch<kBatch, BM, Q>(reinterpret_cast<const Q*>(dens), rd.in, q, nearp, step, i, 1, hi, val, in);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         // samples outside the grid or the lane's range read 0 and leave D and E unchanged
@@ -905,6 +940,28 @@ static dim3 tile_grid(const RayDims& rd, int views) {
 }
 
 size_t quad_copy_floats(Dims3 d) { return quad_entries(make_quad_dims(d)) * 4; }
+size_t quad_copy_bf16_bytes(Dims3 d) { return quad_entries<uint2>(make_quad_dims<uint2>(d)) * sizeof(uint2); }
+
+// bfloat16 corner-quad copy (BORDER) of a float grid
+hipError_t quad_copy_bf16_launch(hipStream_t s, const float* lin, void* out, Dims3 d) {
+  const QuadDims q = make_quad_dims<uint2>(d);
+  const size_t n = quad_entries<uint2>(q);
+  KernelTimer t(s, "quad_copy");
+  hipLaunchKernelGGL((k_quad_copy<B_BORDER, uint2>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lin,
+                     nullptr, reinterpret_cast<uint2*>(out), d, q);
+  return hipGetLastError();
+}
+
+// BORDER forward from the bfloat16 corner-quad copy
+hipError_t render_density_fwd_bf16_launch(hipStream_t s, const void* quad, float* img, const RayDims& rd,
+                                          const RayLaunch& L, const RenderParams& rp, const RenderSaved& sv) {
+  const dim3 grid = tile_grid(rd, L.num_views);
+  const QuadDims q = make_quad_dims<uint2>(rd.in);
+  KernelTimer t(s, "render_fwd");
+  hipLaunchKernelGGL((k_render_density_fwd<B_BORDER, uint2>), grid, dim3(256), 0, s,
+                     reinterpret_cast<const float*>(quad), img, rd, L, rp, q, sv);
+  return hipGetLastError();
+}
 
 // bm: B_BORDER or B_CLAMP
 // fac: null, or a per-voxel factor the copy multiplies in (lin * fac)

@@ -60,6 +60,8 @@ _SIGNATURES = {
     "gft_render_density_saved_bytes": (_sz, [_i64p, _i32, _i32p, _i32]),
     "gft_render_density_fwd": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
                                               _i32, _vp, _sz, _vp, _sz, _vp]),
+    "gft_render_density_fwd_bf16": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p,
+                                                   _f32, _i32, _vp, _sz, _vp, _sz, _vp]),
     "gft_render_density_bwd_workspace": (_sz, [_i64p, _i32, _i32p, _i32]),
     "gft_render_density_bwd_l2": (ctypes.c_int, [_vp, _vp, _i64p, _vp, _i32, _vp, _vp, _vp, _i32, _i32p, _f32p, _f32,
                                                  _i32, _vp, _vp, _f32, _vp, _sz, _vp, _sz, _vp, _vp]),

@@ -285,7 +285,8 @@ class TransportRenderStep(object):
 
     def __init__(self, grid_dhw, matrix_m, matrix_v, matrix_p, frustum_params, output_shape, depth_step,
                  light=(1.0, 1.0, 1.0), order=2, clamp="MC_SMOOTH", dt=1.0, boundary="BORDER", device="cuda",
-                 loss="SE", background=None, loss_scale=None, storage="float32", reduction="gather"):
+                 loss="SE", background=None, loss_scale=None, storage="float32", reduction="gather",
+                 render_storage=None):
         D, H, W = grid_dhw
         if reduction not in ("gather", "scatter"):
             raise _lib.InvalidArgumentError("reduction must be 'gather' or 'scatter'")
@@ -337,6 +338,18 @@ class TransportRenderStep(object):
                          L.gft_render_density_bwd_workspace(self._shape, self.cams.nCam, self.cams.c_out,
                                                             self.boundary))
         self.render_fwd_ws = torch.empty((max(self._nfws, 1),), dtype=torch.uint8, device=dev)
+        # bfloat16 storage: the BORDER render with the gather reduction samples a bfloat16 copy of the
+        # advected density (the stored precision) unless render_storage is "float32"; CLAMP, WRAP and the
+        # scatter walk sample float32
+        bf16_ok = self.boundary == BOUNDARY["BORDER"] and reduction == "gather"
+        if render_storage is None:
+            render_storage = "bfloat16" if (self.storage == torch.bfloat16 and bf16_ok) else "float32"
+        if render_storage not in ("float32", "bfloat16") or (render_storage == "bfloat16" and not bf16_ok):
+            raise _lib.InvalidArgumentError("render_storage '%s' needs the BORDER boundary and the gather reduction "
+                                            "for bfloat16" % render_storage)
+        self.render_storage = render_storage
+        self._render_fwd = L.gft_render_density_fwd_bf16 if render_storage == "bfloat16" else \
+            L.gft_render_density_fwd
 
     PHASES = ("advect_fwd", "render_fwd", "render_bwd", "advect_bwd")
 
@@ -355,10 +368,10 @@ class TransportRenderStep(object):
                                     _lib.ptr(self.d1), _lib.ptr(self.saved), self._nsaved))
         self._mark(timers, "advect_fwd", False)
         self._mark(timers, "render_fwd", True)
-        _lib.check(L.gft_render_density_fwd(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
-                                            _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light, self.depth_step,
-                                            self.boundary, _lib.ptr(self.render_fwd_ws), self._nfws,
-                                            self._rsp, self._nrsaved, _lib.ptr(self.images)))
+        _lib.check(self._render_fwd(s, _lib.ptr(self.d1), self._shape, _np_ptr(c.M), c.nM, _np_ptr(c.V),
+                                    _np_ptr(c.P), _np_ptr(c.F), c.nCam, c.c_out, self.light, self.depth_step,
+                                    self.boundary, _lib.ptr(self.render_fwd_ws), self._nfws,
+                                    self._rsp, self._nrsaved, _lib.ptr(self.images)))
         self._mark(timers, "render_fwd", False)
         return self.images
 
@@ -391,12 +404,13 @@ class TransportRenderStep(object):
 
     def algorithmic_bytes(self):
         """Compulsory HBM bytes per phase: each persistent tensor read once
-        and written once (intermediates excluded); d0 and v at their storage
-        size s (4 or 2 bytes), d', the images and every gradient float32."""
+        and written once (intermediates excluded); d0, v and the render's read
+        of d' at the storage size s (4 or 2 bytes), the advection's write of
+        d', the images and every gradient float32."""
         V = int(np.prod(self.shape))
         N = self.n_pixels
         st = self.storage.itemsize
-        return {"advect_fwd": (4 * st + 4) * V, "render_fwd": 4 * V + 16 * N, "render_bwd": 8 * V + 28 * N,
+        return {"advect_fwd": (4 * st + 4) * V, "render_fwd": st * V + 16 * N, "render_bwd": 8 * V + 28 * N,
                 "advect_bwd": (20 + 4 * st) * V}
 
     def kernel_algorithmic_bytes(self):
@@ -410,7 +424,7 @@ class TransportRenderStep(object):
         V = int(np.prod(self.shape))
         N = self.n_pixels
         st = self.storage.itemsize
-        return {"advect_sl": 4 * st * V, "advect_mc": 4 * V, "advect_o1_out": 4 * V, "render_fwd": 4 * V + 16 * N,
+        return {"advect_sl": 4 * st * V, "advect_mc": 4 * V, "advect_o1_out": 4 * V, "render_fwd": st * V + 16 * N,
                 "render_bwd_pixel": 28 * N, "render_bwd_gather": 8 * V, "render_bwd_scatter": 8 * V + 28 * N,
                 "advect_bwd_a": (4 + 3 * st) * V, "advect_bwd_b": (16 + st) * V, "quad_copy": 0, "fill": 0}
 

@@ -158,6 +158,21 @@ int gft_render_density_fwd(void* stream, const float* density, const int64_t sha
                            const int32_t output_shape[3], const float light[3], float depth_step, int32_t boundary,
                            void* workspace, size_t workspace_bytes, void* saved, size_t saved_bytes, float* images);
 /*
+ * gft_render_density_fwd with the density's corner-quad copy stored in
+ * bfloat16 (rounded to nearest even; the workspace of
+ * gft_render_density_fwd_workspace suffices): the render samples the density
+ * at bfloat16 precision, as stored densities are in the bfloat16 precision
+ * variant, with half the bytes per trilinear tap; interpolation, the
+ * transmittance and E stay float32. BORDER only. The backward is
+ * gft_render_density_bwd_l2 / _bwd_loss / _bwd with the `saved` buffer this
+ * forward filled (the rounding is passed through as the identity).
+ */
+int gft_render_density_fwd_bf16(void* stream, const float* density, const int64_t shape[5], const float* M,
+                                int32_t num_M, const float* V, const float* P, const float* frustum,
+                                int32_t num_cameras, const int32_t output_shape[3], const float light[3],
+                                float depth_step, int32_t boundary, void* workspace, size_t workspace_bytes,
+                                void* saved, size_t saved_bytes, float* images);
+/*
  * Backward of gft_render_density_fwd with the L2 image loss folded in:
  * loss = loss_scale * sum (rgb - target)^2 (targets [1,V,H,W,3]); writes the
  * scalar loss (device float) and the exact density gradient (overwritten).

@@ -434,23 +434,36 @@ def adam_tf(p, g, m, v, lr, beta1, beta2, epsilon, t):
     return p - lr_t * m / (np.sqrt(v) + epsilon), m, v
 
 
+def bf16_round(a):
+    """Values rounded to bfloat16 (nearest even), as float64: the float32
+    value's upper 16 bits after adding half an ulp of the bfloat16 grid."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def step_fwd_bwd(d0, vel_c, target, M, V, P, F, out_dhw, depth_step, order=2, clamp="MC_SMOOTH", prec=1,
-                 grad_mode="EXACT", boundary_render="BORDER", loss_kind=None, background=None, loss_scale=None):
+                 grad_mode="EXACT", boundary_render="BORDER", loss_kind=None, background=None, loss_scale=None,
+                 render_storage="float32"):
     """One advect -> render -> image loss step and its gradients w.r.t. d0
     and the centred velocity. Default: the mean squared error
     (l2_loss_and_grad); with loss_kind / background / loss_scale the image
-    loss of image_loss_and_grad. Returns dict(loss, img, d1, gd, gv)."""
+    loss of image_loss_and_grad. render_storage "bfloat16" renders the
+    advected density rounded to bfloat16 (the stored precision), its gradient
+    passed through the rounding. Returns dict(loss, img, d1, d1r, gd, gv)."""
     d1 = advect_fwd(d0, vel_c, order, clamp, 1.0, "CLAMP", prec)
-    img = render_density_fwd(d1, M, V, P, F, out_dhw, depth_step, boundary=boundary_render, prec=prec)
+    d1r = bf16_round(d1) if render_storage == "bfloat16" else d1
+    img = render_density_fwd(d1r, M, V, P, F, out_dhw, depth_step, boundary=boundary_render, prec=prec)
     if loss_kind is None and background is None and loss_scale is None:
         loss, gimg = l2_loss_and_grad(img, target)
     else:
         scale = loss_scale if loss_scale is not None else 1.0 / np.asarray(target).size
         loss, gimg = image_loss_and_grad(img, target, loss_kind or "SE", background, scale)
-    gd1 = render_density_bwd(d1, img, gimg, M, V, P, F, out_dhw, depth_step, boundary=boundary_render,
+    gd1 = render_density_bwd(d1r, img, gimg, M, V, P, F, out_dhw, depth_step, boundary=boundary_render,
                              grad_mode=grad_mode, prec=prec)
     gd, gv = advect_bwd(d0, vel_c, gd1, order, clamp, 1.0, "CLAMP", grad_mode, prec)
-    return dict(loss=loss, img=img, d1=d1, gd=gd, gv=gv, gd1=gd1)
+    return dict(loss=loss, img=img, d1=d1, d1r=d1r, gd=gd, gv=gv, gd1=gd1)
 
 
 # ------------------------------------------- renderer gather gradient (A4)

@@ -3,11 +3,14 @@ float32 (the precision variant of the transport step).
 
 The kernels widen every bfloat16 value to float32 on load, so a float32
 tensor holding the same (bfloat16-representable) values must give the same
-advected grid and images bit for bit, with BORDER also the same loss and
-render gradient, and the rest to rounding (the advection adjoint and the
-CLAMP render backward accumulate with atomics). Against the
-FP64 oracle on the bfloat16 values: advected grids 1e-5 outside clamp-switch
-voxels and images 1e-5, gradients 1e-3 (normalised max error).
+advected grid bit for bit. The BORDER render of the step samples the advected
+density stored in bfloat16 too: its images equal the float32 render of the
+rounded density bit for bit. CLAMP renders float32, so there the images and
+the loss equal the float32-storage step, the gradients to rounding (the
+advection adjoint and the CLAMP render backward accumulate with atomics).
+Against the FP64 oracle on the bfloat16 values (with the advected density
+rounded to bfloat16 for the BORDER render): advected grids 1e-5 outside
+clamp-switch voxels, images 1e-5, gradients 1e-3 (normalised max error).
 """
 import numpy as np
 import pytest
@@ -92,17 +95,31 @@ def test_fused_step_bf16_storage(cuda, boundary):
     lo, gd, gv = step.run(td, tv, ttgt)
     lo32, gd32, gv32 = step32.run(td.float(), tv.float(), ttgt)
     torch.cuda.synchronize()
-    assert torch.equal(step.d1, step32.d1) and torch.equal(step.images, step32.images)
-    if boundary == "BORDER":  # fixed-order loss sum and gather
-        assert torch.equal(lo, lo32) and torch.equal(step.grad_d1, step32.grad_d1)
-    else:  # the CLAMP backward adds tile losses and scatters with float atomics
+    assert torch.equal(step.d1, step32.d1)
+    render_storage = "float32"
+    if boundary == "BORDER":  # the render samples the advected density at bfloat16
+        from gft.pipeline import _Cameras, render_density_forward
+        cams = _Cameras(sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"])
+        d1r = step.d1.to(torch.bfloat16).float()
+        assert not torch.equal(d1r, step.d1)
+        assert torch.equal(step.images, render_density_forward(d1r, cams, sc["step"]))
+        render_storage = "bfloat16"
+        r = orc.step_fwd_bwd(d64, v64, tgt.astype(np.float64), sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"],
+                             sc["step"], boundary_render=boundary, render_storage="bfloat16")
+        # the GPU's and the oracle's bfloat16 roundings of the advected density agree but at ties
+        assert np.mean(d1r.cpu().numpy() != r["d1r"]) <= 1e-3
+        ref_img = orc.render_density_fwd(d1r.cpu().numpy(), sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"],
+                                         sc["step"])
+    else:  # CLAMP renders float32: the float32-storage step's images and loss, the backward to rounding
+        assert torch.equal(step.images, step32.images)
         assert abs(lo.item() - lo32.item()) <= 1e-6 * abs(lo32.item())
-    assert norm_err(gd.cpu().numpy(), gd32.cpu().numpy()) < 1e-6
-    assert norm_err(gv.cpu().numpy(), gv32.cpu().numpy()) < 1e-6
-    r = orc.step_fwd_bwd(d64, v64, tgt.astype(np.float64), sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"],
-                         sc["step"], boundary_render=boundary)
+        assert norm_err(gd.cpu().numpy(), gd32.cpu().numpy()) < 1e-6
+        assert norm_err(gv.cpu().numpy(), gv32.cpu().numpy()) < 1e-6
+        r = orc.step_fwd_bwd(d64, v64, tgt.astype(np.float64), sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"],
+                             sc["step"], boundary_render=boundary)
+        ref_img = r["img"]
     assert abs(lo.item() - r["loss"]) <= 1e-5 * abs(r["loss"])
-    assert norm_err(step.images.cpu().numpy(), r["img"]) < 1e-5
+    assert norm_err(step.images.cpu().numpy(), ref_img) < 1e-5
     assert norm_err(gd.cpu().numpy(), r["gd"]) < 1e-3
     assert norm_err(gv.cpu().numpy(), r["gv"]) < 1e-3
     with pytest.raises(_lib.InvalidArgumentError):

@@ -330,7 +330,7 @@ def test_mc_smooth_branches_agree_with_the_oracle(cuda, name, storage):
         name, dis32.sum(), dis64.sum(), m_gpu.size, tie.sum()), flush=True)
     assert not (dis32 & ~tie).any() and not (dis64 & ~tie).any()
     r = orc.step_fwd_bwd(d0, vel, tgt.cpu().numpy().astype(np.float64), w["M"], w["V"], w["P"], w["F"],
-                         w["out_dhw"], w["step"])
+                         w["out_dhw"], w["step"], render_storage=storage)
     rng = np.random.Generator(np.random.PCG64(3))
     ind = (dis64 * rng.uniform(0.5, 1.5, size=dis64.shape))[None, ..., None]
     dep_d, dep_v = orc.advect_bwd(d0, vel, ind, 2, "NONE", prec=1)

@@ -170,9 +170,15 @@ def test_c5_bf16_storage_step_matches_fp64_oracle(cuda, c5, clamp):
     torch.cuda.synchronize()
     r = orc.step_fwd_bwd(d0.float().cpu().numpy().astype(np.float64), vel.float().cpu().numpy().astype(np.float64),
                          tgt.cpu().numpy().astype(np.float64), w["M"], w["V"], w["P"], w["F"], w["out_dhw"], w["step"],
-                         clamp=clamp)
+                         clamp=clamp, render_storage="bfloat16")
     assert abs(loss.item() - r["loss"]) <= 1e-5 * abs(r["loss"])
-    assert norm_err(step.images.cpu().numpy(), r["img"]) < 1e-5
+    # the render samples the advected density at bfloat16: the GPU's rounding and the oracle's agree
+    # but where FP32 and FP64 values straddle a rounding tie; the images against the oracle render of
+    # the GPU's own rounded density
+    d1r = step.d1.to(torch.bfloat16).float().cpu().numpy()
+    assert np.mean(d1r != r["d1r"]) <= 1e-3
+    ref_img = orc.render_density_fwd(d1r, w["M"], w["V"], w["P"], w["F"], w["out_dhw"], w["step"])
+    assert norm_err(step.images.cpu().numpy(), ref_img) < 1e-5
     assert norm_err(step.grad_d1.cpu().numpy(), r["gd1"]) < 1e-3
     gd, gv = gd.cpu().numpy(), gv.cpu().numpy()
     if clamp == "NONE":

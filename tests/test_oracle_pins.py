@@ -911,3 +911,15 @@ def test_adam_tf_closed_forms():
         want = prev - lr * g / (np.abs(g) + eps / np.sqrt(1.0 - b2 ** t))
         assert np.allclose(p, want, rtol=0, atol=1e-13), t
     assert p[3] == p0[3]
+
+
+def test_bf16_round_matches_torch():
+    """bfloat16 rounding (nearest even) equals torch's float32 -> bfloat16
+    conversion, ties and subnormals included."""
+    import torch
+    rng = np.random.Generator(np.random.PCG64(31))
+    x = np.concatenate([rng.uniform(-3, 3, 5000), rng.uniform(0, 1e-38, 200),
+                        (np.arange(1, 2000) * 2.0 ** -8 + 1.0) - 2.0 ** -9, [0.0, -0.0, 1.0, 65504.0]])
+    x = x.astype(np.float32)
+    want = torch.from_numpy(x).to(torch.bfloat16).float().numpy().astype(np.float64)
+    assert np.array_equal(orc.bf16_round(x), want)
