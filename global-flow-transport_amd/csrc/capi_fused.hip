@@ -270,8 +270,8 @@ size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_camera
   if (boundary != GFT_BORDER) return 0;
   const RayDims rd = render_dims(shape, output_shape);
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
-  return align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float)) +
-         align256(plane * num_cameras * sizeof(float4)) +
+  const size_t nsamples = render_waves_per_view(output_shape[2], output_shape[1]) * kSampleRun * output_shape[0];
+  return align256(nsamples * num_cameras * sizeof(float)) + align256(plane * num_cameras * sizeof(float4)) +
          align256(render_tiles_per_view(rd) * num_cameras * sizeof(float));
 }
 
@@ -280,10 +280,11 @@ size_t gft_render_density_saved_bytes(const int64_t shape[5], int32_t num_camera
 static RenderSaved saved_parts(void* saved, const int64_t* shape, const int32_t* output_shape, int32_t num_cameras) {
   const RayDims rd = render_dims(shape, output_shape);
   const size_t plane = (size_t)output_shape[1] * output_shape[2];
+  const size_t nsamples = render_waves_per_view(output_shape[2], output_shape[1]) * kSampleRun * output_shape[0];
   char* p = (char*)saved;
   RenderSaved sv{};
   sv.samples = (float*)p;
-  p += align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float));
+  p += align256(nsamples * num_cameras * sizeof(float));
   sv.records = (float4*)p;
   p += align256(plane * num_cameras * sizeof(float4));
   sv.partials = (float*)p;
@@ -482,9 +483,9 @@ size_t gft_render_density_lit_fwd_workspace(const int64_t shape[5], int32_t boun
 size_t gft_render_density_lit_saved_bytes(const int64_t shape[5], int32_t num_cameras, const int32_t output_shape[3],
                                           int32_t boundary) {
   if (boundary != GFT_BORDER) return 0;
-  const size_t plane = (size_t)output_shape[1] * output_shape[2];
+  const size_t nsamples = render_waves_per_view(output_shape[2], output_shape[1]) * kSampleRun * output_shape[0];
   return gft_render_density_saved_bytes(shape, num_cameras, output_shape, boundary) +
-         align256(plane * (size_t)output_shape[0] * num_cameras * sizeof(float));
+         align256(nsamples * num_cameras * sizeof(float));
 }
 
 // the lit saved buffer: the ambient render's layout followed by the u_i planes

@@ -403,8 +403,24 @@ struct ProjLaunch {
   int accumulate;  // 0: write the sum, 1: add it to the existing gradient
 };
 
+// Per-sample values of the BORDER density render are stored wave-major: each
+// 16 x 4-pixel wave of a 16 x 16-pixel tile owns S consecutive runs of 64
+// floats, one run per sample (lane = 16 (y % 4) + x % 16), so that the
+// forward's one-sample store of a wave fills two whole 128-byte lines.
+constexpr int kSampleRun = 64;
+__host__ __device__ inline size_t render_waves_per_view(int out_w, int out_h) {
+  return (size_t)((out_w + 15) / 16) * (size_t)((out_h + 15) / 16) * 4;
+}
+// offset of sample 0 of pixel (x, y) of output slot `slot`; sample i is at + i * kSampleRun
+__host__ __device__ inline size_t sample_base(int out_w, int out_h, int S, int slot, int x, int y) {
+  const size_t tiles_x = (size_t)((out_w + 15) / 16);
+  const size_t wave = ((size_t)(y >> 4) * tiles_x + (size_t)(x >> 4)) * 4 + (size_t)((y & 15) >> 2);
+  return ((size_t)slot * render_waves_per_view(out_w, out_h) + wave) * (size_t)S * kSampleRun +
+         (size_t)(((y & 3) << 4) | (x & 15));
+}
+
 // What the BORDER density render forward saves for its backward (device
-// pointers, indexed by output view slot): per-sample values [views][S][H][W],
+// pointers, indexed by output view slot): per-sample values (wave-major, above),
 // per-pixel records [views][H][W] (sample range lo, hi as int bits in x, y;
 // coefficients in z, w) and per-tile partial sums of the L2 loss
 // [views][tiles].

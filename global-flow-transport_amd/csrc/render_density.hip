@@ -261,8 +261,8 @@ __device__ __forceinline__ int wave_max(int v) {
 // clipped range of each ray, CLAMP every sample. With gs != nullptr (BORDER, training) the
 // forward also saves what the backward needs (RenderSaved): per pixel the
 // sample range and E, per sample of the wave's clipped range a value c_i at
-// samples[((slot*S + i)*H + y)*W + x] (a NaN mark for a sample outside the
-// grid).
+// samples[sample_base(pixel) + i * kSampleRun] (wave-major: one instruction's
+// stores fill two whole lines; a NaN mark for a sample outside the grid).
 template <int BM, class Q = float4>
 __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restrict__ dens, float* __restrict__ img,
                                                             RayDims rd, RayLaunch L, RenderParams rp, QuadDims q,
@@ -287,9 +287,10 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
     }
     const size_t plane = size_t(rd.out_w) * rd.out_h;
     const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
-    float* gview = (sv.samples && valid) ? sv.samples + (size_t)v.out_slot * rd.out_d * plane + pix : nullptr;
+    float* gview = (sv.samples && valid) ? sv.samples + sample_base(rd.out_w, rd.out_h, rd.out_d, v.out_slot, x, y)
+                                         : nullptr;
     // march the wave's union of ranges with one sample index for all lanes, so
-    // that the saved-sigma stores of one instruction fill one sample plane
+    // that the saved-value stores of one instruction fill one wave-major run
     const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
     for (int i = wlo; i <= whi; i += kBatch) {
       float val[kBatch];
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(256) void k_render_density_fwd(const float* __restr
         const float e = __expf(-D);
         // saved for the backward: c_i = exp(-D_i) + E_{i-1}; a NaN mark for a
         // sample outside the grid (its gradient is 0)
-        if (gview && i + u <= whi) gview[(size_t)(i + u) * plane] = act ? e + E : kOutsideMark;
+        if (gview && i + u <= whi) gview[(size_t)(i + u) * kSampleRun] = act ? e + E : kOutsideMark;
         E = fmaf(e, sigma, E);
       }
     }
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(256) void k_render_density_lit_fwd(const float4* __
   }
   const size_t plane = size_t(rd.out_w) * rd.out_h;
   const size_t pix = valid ? size_t(y) * rd.out_w + x : 0;
-  const size_t vbase = (size_t)v.out_slot * rd.out_d * plane + pix;
+  const size_t vbase = valid ? sample_base(rd.out_w, rd.out_h, rd.out_d, v.out_slot, x, y) : 0;
   float* cview = (sv.samples && valid) ? sv.samples + vbase : nullptr;
   float* uview = (sv.samples2 && valid) ? sv.samples2 + vbase : nullptr;
   const int wlo = wave_min(lo <= hi ? lo : rd.out_d), whi = wave_max(lo <= hi ? hi : -1);
@@ -376,8 +377,8 @@ __global__ __launch_bounds__(256) void k_render_density_lit_fwd(const float4* __
       D = act ? fmaxf(D + sigma, 0.f) : D;
       const float e = __expf(-D);
       if (cview && i + u <= whi) {
-        cview[(size_t)(i + u) * plane] = act ? fmaf(a, e, E) : kOutsideMark;
-        uview[(size_t)(i + u) * plane] = act ? e : 0.f;
+        cview[(size_t)(i + u) * kSampleRun] = act ? fmaf(a, e, E) : kOutsideMark;
+        uview[(size_t)(i + u) * kSampleRun] = act ? e : 0.f;
       }
       E = fmaf(e, fmaf(a, sigma, eps), E);
     }
@@ -576,8 +577,7 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
       ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
     }
     if (xs > xe || ys > ye) continue;
-    const float* gview = sv.samples + (size_t)pv.out_slot * rd.out_d * plane;
-    const float* uview = LIT ? sv.samples2 + (size_t)pv.out_slot * rd.out_d * plane : nullptr;
+
     const float4* rview = sv.records + (size_t)pv.out_slot * plane;
     {
       const int nx = xe - xs + 1, cnt = nx * (ye - ys + 1);
@@ -628,19 +628,21 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
         const float fi0 = (float)i0;
         float qx = fmaf(fi0, sx, nx_) - (fx0 + 0.5f), qy = fmaf(fi0, sy, ny_) - (fy0 + 0.5f),
               qz = fmaf(fi0, sz, nz_) - (fz0 + 0.5f);
-        const float* gp = gview + (size_t)i0 * plane + pix;
-        const float* up = LIT ? uview + (size_t)i0 * plane + pix : nullptr;
+        const size_t sb = sample_base(rd.out_w, rd.out_h, rd.out_d, pv.out_slot, (int)fpx, (int)fpy) +
+                          (size_t)i0 * kSampleRun;
+        const float* gp = sv.samples + sb;
+        const float* up = LIT ? sv.samples2 + sb : nullptr;
         for (int ib = i0; ib <= i1; ib += kGatherBatch) {
           // the batch's saved values are all loaded before the first is used
           float cs[kGatherBatch], us[kGatherBatch];
 #pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) cs[u] = ib + u <= i1 ? gp[(size_t)u * plane] : kOutsideMark;
+          for (int u = 0; u < kGatherBatch; ++u) cs[u] = ib + u <= i1 ? gp[u * kSampleRun] : kOutsideMark;
           if (LIT) {
 #pragma unroll
-            for (int u = 0; u < kGatherBatch; ++u) us[u] = ib + u <= i1 ? up[(size_t)u * plane] : 0.f;
-            up += (size_t)kGatherBatch * plane;
+            for (int u = 0; u < kGatherBatch; ++u) us[u] = ib + u <= i1 ? up[u * kSampleRun] : 0.f;
+            up += kGatherBatch * kSampleRun;
           }
-          gp += (size_t)kGatherBatch * plane;
+          gp += kGatherBatch * kSampleRun;
 #pragma unroll
           for (int u = 0; u < kGatherBatch; ++u) {
           const float c = cs[u];

@@ -188,7 +188,8 @@ def test_validation_returns_before_device_work(lib):
     assert lib.gft_render_density_bwd_workspace(sh, 1, _i32(4, 4, 4), 2) == 0
     assert lib.gft_render_density_lit_fwd_workspace(sh, 0) == 2 * nq
     ns = lib.gft_render_density_saved_bytes(sh, 2, _i32(4, 4, 4), 0)
-    assert lib.gft_render_density_lit_saved_bytes(sh, 2, _i32(4, 4, 4), 0) == ns + 512  # 2 views x 4x4x4 floats
+    # a second per-sample array: 2 views x 4 waves (one 16x16 tile) x 64 lanes x 4 samples x 4 B
+    assert lib.gft_render_density_lit_saved_bytes(sh, 2, _i32(4, 4, 4), 0) == ns + 2 * 4 * 64 * 4 * 4
     assert lib.gft_render_density_fwd_workspace(sh, 2) == 0
     rc = lib.gft_render_density_bwd_loss(null, p16, sh, vp, 1, vp, vp, fp, 1, _i32(4, 4, 4), light,
                                          ctypes.c_float(1.0), 1, p16, p16, null, 0, ctypes.c_float(1.0), null, 0, null,
