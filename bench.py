@@ -508,7 +508,7 @@ def epilogue_bench(args):
         return float(np.median(ts))
 
     ms_f, ms_c = timeit(fused), timeit(composed)
-    with KernelTimers("render_fwd,render_bwd_pixel,render_bwd_gather,loss_sum,divergence,divergence_grad,adam,"
+    with KernelTimers("render_fwd,render_bwd_pixel,render_bwd_gather,divergence,divergence_grad,adam,"
                       "quad_copy") as kt:
         for _ in range(args.steps):
             fused()

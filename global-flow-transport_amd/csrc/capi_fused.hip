@@ -21,9 +21,8 @@ size_t render_tiles_per_view(const RayDims& rd);
 hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img, const float* second,
                                        const RenderSaved& sv, const RayDims& rd, const RayLaunch& L,
                                        const RenderParams& rp);
-hipError_t loss_sum_launch(hipStream_t s, const float* partials, int n, float scale, float* loss);
 hipError_t render_density_gather_launch(hipStream_t s, const RenderSaved& sv, float* grad, const RayDims& rd,
-                                        const ProjLaunch& PL);
+                                        const ProjLaunch& PL, float* loss, float loss_scale);
 hipError_t render_density_scatter_launch(int bm, bool l2, hipStream_t s, const float* dens, const float* img,
                                          const float* second, float* grad, float* loss, const RayDims& rd,
                                          const RayLaunch& L, const RenderParams& rp);
@@ -38,7 +37,7 @@ hipError_t render_density_lit_fwd_launch(hipStream_t s, const float* dq, const f
                                          const RenderSaved& sv);
 hipError_t render_density_lit_gather_launch(hipStream_t s, const RenderSaved& sv, const float* dens,
                                             const float* lam, float* grad, float* lam_grad, const RayDims& rd,
-                                            const ProjLaunch& PL);
+                                            const ProjLaunch& PL, float* loss, float loss_scale);
 size_t quad_copy_floats(Dims3 d);
 
 // Bytes of the corner-quad density copy the BORDER render kernels read.
@@ -288,7 +287,7 @@ static RenderSaved saved_parts(void* saved, const int64_t* shape, const int32_t*
   sv.records = (float4*)p;
   p += align256(plane * num_cameras * sizeof(float4));
   sv.partials = (float*)p;
-  (void)rd;
+  sv.n_partials = (int)(render_tiles_per_view(rd) * num_cameras);
   return sv;
 }
 
@@ -406,14 +405,13 @@ static int render_bwd_common(void* stream, const float* density, const int64_t s
       if ((rc = check_hip(render_density_pixel_launch(l2, s, images, second, sv, rd, L, rp),
                           "render backward pixel pass")))
         return rc;
-      if ((rc = check_hip(render_density_gather_launch(s, sv, density_grad, rd, PL), "render backward gather")))
+      // the last chunk's gather also sums the per-tile loss partials of every view in a fixed order
+      const bool lastc = first + cnt == pairs.size();
+      if ((rc = check_hip(render_density_gather_launch(s, sv, density_grad, rd, PL, (l2 && lastc) ? loss : nullptr,
+                                                       loss_scale),
+                          "render backward gather")))
         return rc;
     }
-    // the loss from the per-tile partials of every view, in a fixed order
-    if (l2 && (rc = check_hip(loss_sum_launch(s, sv.partials, (int)(render_tiles_per_view(rd) * num_cameras),
-                                              loss_scale, loss),
-                              "loss sum")))
-      return rc;
     return GFT_OK;
   }
   const size_t need_ws = gft_render_density_bwd_workspace(shape, num_cameras, output_shape, boundary);
@@ -579,15 +577,13 @@ static int render_lit_bwd_common(void* stream, const float* density, const float
     if ((rc = check_hip(render_density_pixel_launch(l2, s, images, second, sv, rd, L, rp),
                         "lit render backward pixel pass")))
       return rc;
+    const bool lastc = first + cnt == pairs.size();
     if ((rc = check_hip(render_density_lit_gather_launch(s, sv, density, light_factor, density_grad,
-                                                         light_factor_grad, rd, PL),
+                                                         light_factor_grad, rd, PL, (l2 && lastc) ? loss : nullptr,
+                                                         loss_scale),
                         "lit render backward gather")))
       return rc;
   }
-  if (l2 && (rc = check_hip(loss_sum_launch(s, sv.partials, (int)(render_tiles_per_view(rd) * num_cameras),
-                                            loss_scale, loss),
-                            "loss sum")))
-    return rc;
   return GFT_OK;
 }
 

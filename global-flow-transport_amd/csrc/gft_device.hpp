@@ -428,7 +428,8 @@ struct RenderSaved {
   float* samples;
   float4* records;
   float* partials;
-  float* samples2;  // the lit render's second per-sample plane (u_i), else null
+  float* samples2;   // the lit render's second per-sample array (u_i), else null
+  int n_partials;    // loss partials: tiles of every view
 };
 
 // Fused advection launch parameters.

@@ -461,6 +461,20 @@ __device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, f
 // walk along the ray: this pass turns each pixel's loss gradient into (alpha,
 // beta) (and the L2 loss), the gather evaluates alpha * c_k + beta per sample.
 // A pixel with no gradient gets lo > hi so the gather skips it.
+// loss = scale * sum of the n tile partials, in a fixed order (deterministic):
+// 256 strided sums, then a tree over them
+__device__ __forceinline__ void loss_sum_block(const float* partials, int n, float scale, float* loss, float* red) {
+  float t = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) t += partials[i];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[0] = red[0] * scale;
+}
+
 template <bool L2>
 __global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* __restrict__ img,
                                                                   const float* __restrict__ second, RenderSaved sv,
@@ -481,7 +495,8 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* _
                                                : make_record(lo, hi, rp.step * G, rp.step * fmaf(-G, rec.z, gD));
   }
   if (L2) {
-    // per-tile partial sum (fixed order); a single-block pass adds the tiles
+    // per-tile partial sum (fixed order); the first workgroup of the gather
+    // that follows adds the tiles of every view
     __shared__ float red[256 / 64];
     for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
@@ -492,21 +507,6 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* _
       sv.partials[(size_t)v.out_slot * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = t;
     }
   }
-}
-
-// loss = scale * sum of the n tile partials, in a fixed order (deterministic)
-__global__ __launch_bounds__(256) void k_loss_sum(const float* __restrict__ partials, int n, float scale,
-                                                  float* __restrict__ loss) {
-  __shared__ float red[256];
-  float t = 0.f;
-  for (int i = threadIdx.x; i < n; i += 256) t += partials[i];
-  red[threadIdx.x] = t;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) loss[0] = red[0] * scale;
 }
 
 // Gather: one wave per block of WX x WY x WZ voxels, accumulated in a
@@ -523,12 +523,22 @@ constexpr int kGatherBatch = 4;  // saved values loaded per round trip (2: 0.187
 // LIT (the spot-lit render): a second region sums the emission gradients
 // alpha u_i * w of the d * lambda grid; the store then writes
 // grad = sum_sigma + lambda * sum_eps and lambda_grad = d * sum_eps.
+//
+// With `loss` non-null (the last launch of a backward with the image loss
+// folded in), workgroup 0 first sums the pixel pass's per-tile loss partials
+// of every view in a fixed order; the kernel boundary after the pixel pass
+// makes them visible, so no separate launch is needed.
 template <int WX, int WY, int WZ, bool LIT = false>
 __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved sv, float* __restrict__ grad,
                                                                    RayDims rd, ProjLaunch PL,
                                                                    const float* __restrict__ dens,
                                                                    const float* __restrict__ lam,
-                                                                   float* __restrict__ lam_grad) {
+                                                                   float* __restrict__ lam_grad,
+                                                                   float* __restrict__ loss, float loss_scale) {
+  if (loss && blockIdx.x == 0) {
+    __shared__ float red[256];
+    loss_sum_block(sv.partials, sv.n_partials, loss_scale, loss, red);
+  }
   constexpr int kVox = WX * WY * WZ;
   __shared__ double region[LIT ? 8 : 4][kVox];
   typedef __attribute__((address_space(3))) double lds_double;
@@ -1013,7 +1023,8 @@ size_t render_tiles_per_view(const RayDims& rd) {
 }
 
 // BORDER pixel pass (per-tile loss partials and per-pixel gradient coefficients)
-// for one chunk of views
+// for one chunk of views; with the loss folded in, the last block of the last
+// chunk writes the loss
 hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img, const float* second,
                                        const RenderSaved& sv, const RayDims& rd, const RayLaunch& L,
                                        const RenderParams& rp) {
@@ -1024,34 +1035,30 @@ hipError_t render_density_pixel_launch(bool l2, hipStream_t s, const float* img,
   return hipGetLastError();
 }
 
-// loss = scale * (sum of the n partials)
-hipError_t loss_sum_launch(hipStream_t s, const float* partials, int n, float scale, float* loss) {
-  KernelTimer t(s, "loss_sum");
-  hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, partials, n, scale, loss);
-  return hipGetLastError();
-}
-
 constexpr int kGatherWX = 16, kGatherWY = 16, kGatherWZ = 2;
 
+// loss: null, or where workgroup 0 writes loss_scale * the sum of the loss partials
 hipError_t render_density_gather_launch(hipStream_t s, const RenderSaved& sv, float* grad, const RayDims& rd,
-                                        const ProjLaunch& PL) {
+                                        const ProjLaunch& PL, float* loss, float loss_scale) {
   const int nb = ((rd.in.x + kGatherWX - 1) / kGatherWX) * ((rd.in.y + kGatherWY - 1) / kGatherWY) *
                  ((rd.in.z + kGatherWZ - 1) / kGatherWZ);
   KernelTimer t(s, "render_bwd_gather");
   auto kern = k_render_density_bwd_gather<kGatherWX, kGatherWY, kGatherWZ>;
-  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL, nullptr, nullptr, nullptr, loss,
+                     loss_scale);
   return hipGetLastError();
 }
 
-// spot-lit gather: grad and lam_grad from the saved c_i and u_i planes
+// spot-lit gather: grad and lam_grad from the saved c_i and u_i arrays
 hipError_t render_density_lit_gather_launch(hipStream_t s, const RenderSaved& sv, const float* dens,
                                             const float* lam, float* grad, float* lam_grad, const RayDims& rd,
-                                            const ProjLaunch& PL) {
+                                            const ProjLaunch& PL, float* loss, float loss_scale) {
   const int nb = ((rd.in.x + kGatherWX - 1) / kGatherWX) * ((rd.in.y + kGatherWY - 1) / kGatherWY) *
                  ((rd.in.z + kGatherWZ - 1) / kGatherWZ);
   KernelTimer t(s, "render_lit_bwd_gather");
   auto kern = k_render_density_bwd_gather<kGatherWX, kGatherWY, kGatherWZ, true>;
-  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL, dens, lam, lam_grad);
+  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL, dens, lam, lam_grad, loss,
+                     loss_scale);
   return hipGetLastError();
 }
 

@@ -473,7 +473,7 @@ int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params
  * the launch count and summed duration of one name; gft_timer_names lists
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
- * render_fwd, render_bwd_pixel, render_bwd_gather, render_bwd_scatter, quad_copy, fill,
+ * render_fwd, render_bwd_pixel (with the folded loss sum), render_bwd_gather, render_bwd_scatter, quad_copy, fill,
  * render_lit_fwd, render_lit_bwd_gather,
  * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam, transform_grad_gather, lut_grad.
  */
