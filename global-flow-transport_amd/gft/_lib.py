@@ -113,6 +113,9 @@ _SIGNATURES = {
     "gft_adam_step_device": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                             ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int64), _f32, _f32, _f32, _f32,
                                             _vp]),
+    "gft_batch_render_workspace": (_sz, [_i32, _vp, _i32]),
+    "gft_batch_render_prepare": (ctypes.c_int, [_vp, _i32, _vp, _i32, _vp, _sz]),
+    "gft_batch_render_run": (ctypes.c_int, [_vp, _i32, _vp, _i32, _vp, _sz]),
     "gft_timer_start": (ctypes.c_int, [ctypes.c_char_p]),
     "gft_timer_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
     "gft_timer_names": (ctypes.c_int, [ctypes.c_char_p, _sz]),

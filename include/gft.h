@@ -466,6 +466,54 @@ int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params
                          float beta1, float beta2, float epsilon, int64_t* step_counter);
 
 /*
+ * Batched density-reconstruction step of many independent reconstructions of
+ * uneven grid sizes and view counts: per reconstruction the reference's
+ * non-fused render of a density under an ambient light (Renderer.render_density
+ * with fused=False, renderer.py:555-568,603-651: SampleGridTransform
+ * TRANSFORM_LINDEPTH with LINEAR mipmapping, the LoD of every sample from its
+ * frustum footprint, levels 0..num_mipmaps <= 3, resample_grid.cc:268-447 and
+ * sampling.hpp:114-123; then _blend_grid BEER_LAMBERT, renderer.py:488-499),
+ * the mean squared rgb error times loss_scale against the targets, and the
+ * exact gradient w.r.t. the density through blend, resample and mips. Replaces
+ * the per-(batch, camera) launch loops of SampleGridKernelLauncher
+ * (resample_grid.cu.cc:372-436) and their gradients, for all reconstructions
+ * at once: a level pass per mip level, one launch that runs the forward and
+ * the backward of every ray of every view of every reconstruction, and one
+ * that applies the mip adjoint and sums the losses. The backward recomputes
+ * each ray's transmittance walking back from far to near (nothing is saved
+ * per sample) and scatters through an FP64 LDS brick per tile and segment;
+ * `brick` picks the tile and segment per reconstruction: 0 = 16x16-pixel tiles
+ * with 8-sample segments, 1 = 8x8 with 4, 2 = 8x8 with 8. Float atomics: the
+ * gradient's last bits vary between runs.
+ * gft_batch_render_prepare writes the reconstruction table into the workspace
+ * (a set-up call that waits for its copy); gft_batch_render_run only launches
+ * kernels (capturable in a HIP graph) and reads the host array for launch sizes.
+ */
+typedef struct {
+  int32_t grid_dhw[3];       /* D, H, W of the density */
+  const float* density;      /* [1,D,H,W,1] device */
+  const float* targets;      /* [V,H,W,3] device */
+  float* images;             /* [V,H,W,4] device: (rgb, optical depth) */
+  float* density_grad;       /* [1,D,H,W,1] device, overwritten */
+  float* loss;               /* one device float */
+  const float* M;            /* [4,4] host, column-major */
+  const float* V;            /* [num_views,4,4] host */
+  const float* P;            /* [num_views,4,4] host */
+  const float* frustum;      /* [num_views,6] host */
+  int32_t num_views;         /* 1..8 */
+  int32_t output_shape[3];   /* S, H, W of every view */
+  float depth_step;
+  float light[3];            /* ambient colour * intensity */
+  float loss_scale;
+  int32_t brick;             /* 0, 1 or 2 (above) */
+} gft_batch_recon;
+size_t gft_batch_render_workspace(int32_t n, const gft_batch_recon* recons, int32_t num_mipmaps);
+int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* recons, int32_t num_mipmaps,
+                             void* workspace, size_t workspace_bytes);
+int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons, int32_t num_mipmaps,
+                         void* workspace, size_t workspace_bytes);
+
+/*
  * Kernel timers (instrumentation; no counterpart in the reference, which
  * has none). gft_timer_start("a,b" or "*") makes every later launch of the
  * named kernels on the calling thread record a HIP event before and after
@@ -474,7 +522,7 @@ int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
  * render_fwd, render_bwd_pixel (with the folded loss sum), render_bwd_gather, render_bwd_scatter, quad_copy, fill,
- * render_lit_fwd, render_lit_bwd_gather,
+ * render_lit_fwd, render_lit_bwd_gather, batch_levels, batch_render, batch_finish,
  * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam, transform_grad_gather, lut_grad.
  */
 int gft_timer_start(const char* kernels);

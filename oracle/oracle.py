@@ -466,6 +466,52 @@ def step_fwd_bwd(d0, vel_c, target, M, V, P, F, out_dhw, depth_step, order=2, cl
     return dict(loss=loss, img=img, d1=d1, d1r=d1r, gd=gd, gv=gv, gd1=gd1)
 
 
+# --------------------------------------- mipmapped non-fused render (c4)
+def blend_beer_lambert_fwd_bwd(sigma, light, img_grad_rgb=None):
+    """_blend_grid BEER_LAMBERT on a density frustum [V,S,H,W] whose light
+    channels are light * sigma (renderer.py:488-499): D_i = sum_{j<=i}
+    sigma_j, rgb = light * sum_i sigma_i exp(-D_i), depth = D_{S-1}. With an
+    rgb gradient [V,H,W,3], also d loss / d sigma: with G = light . g_rgb,
+    G (u_k - sum_{i>=k} u_i sigma_i), u = exp(-D)."""
+    sig = _d(sigma)
+    D = np.cumsum(sig, axis=1)
+    u = np.exp(-D)
+    E = (u * sig).sum(1)
+    light = np.asarray(light, dtype=np.float64)
+    img = np.concatenate([E[..., None] * light, D[:, -1][..., None]], axis=-1)
+    if img_grad_rgb is None:
+        return img, None
+    G = _d(img_grad_rgb) @ light
+    R = np.flip(np.cumsum(np.flip(u * sig, 1), 1), 1)
+    return img, G[:, None] * (u - R)
+
+
+def mip_render_fwd_bwd(d, target, M, V, P, F, out_dhw, depth_step, light=(1.0, 1.0, 1.0), num_mipmaps=3,
+                       loss_scale=None, prec=1):
+    """The density-reconstruction step of the batched render for one
+    reconstruction: the non-fused ambient render (Renderer.render_density with
+    fused=False, renderer.py:555-568,603-651), i.e. SampleGridTransform
+    TRANSFORM_LINDEPTH of the step-scaled density with LINEAR mips
+    (num_mipmaps levels, LoD per sample), the BEER_LAMBERT blend, loss =
+    loss_scale * sum (rgb - target)^2 (default: the mean), and the exact
+    density gradient through blend, resample and mips. target [V,H,W,3].
+    Returns dict(img [V,H,W,4], loss, gd [1,D,H,W,1])."""
+    dd = _d(d) * depth_step
+    mm = "LINEAR" if num_mipmaps > 0 else "NONE"
+    fr = sample_transform_fwd(dd, M, V, P, F, out_dhw, "LINEAR", "BORDER", mm, num_mipmaps, 0.0,
+                              "TRANSFORM_LINDEPTH", prec=prec)
+    sig = fr[0, ..., 0]
+    img, _ = blend_beer_lambert_fwd_bwd(sig, light)
+    tg = _d(target)
+    scale = 1.0 / tg.size if loss_scale is None else loss_scale
+    e = img[..., :3] - tg
+    loss = float(scale * (e * e).sum())
+    _, gsig = blend_beer_lambert_fwd_bwd(sig, light, 2.0 * scale * e)
+    gd = sample_transform_bwd(dd, gsig[None, ..., None], M, V, P, F, "LINEAR", "BORDER", mm, num_mipmaps, 0.0,
+                              "TRANSFORM_LINDEPTH", grad_mode="EXACT", prec=prec) * depth_step
+    return dict(img=img, loss=loss, gd=gd)
+
+
 # ------------------------------------------- renderer gather gradient (A4)
 def sample_transform_gather_grad(output_grad, M, V, P, F, in_dhw, interpolation="LINEAR", boundary="BORDER",
                                  mipmapping="NONE", num_mipmaps=0, mip_bias=0.0,

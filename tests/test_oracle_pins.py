@@ -923,3 +923,27 @@ def test_bf16_round_matches_torch():
     x = x.astype(np.float32)
     want = torch.from_numpy(x).to(torch.bfloat16).float().numpy().astype(np.float64)
     assert np.array_equal(orc.bf16_round(x), want)
+
+
+def test_blend_beer_lambert_closed_form_and_fd():
+    """The non-fused BEER_LAMBERT blend: constant sigma c over S cells gives
+    rgb = light c sum_{i=1..S} e^{-ic} and depth S c; its sigma gradient is the
+    central difference of <rgb, w>."""
+    S = 7
+    sig = np.full((1, S, 2, 3), 0.3)
+    img, _ = orc.blend_beer_lambert_fwd_bwd(sig, (0.5, 1.0, 2.0))
+    want = 0.3 * np.exp(-0.3 * np.arange(1, S + 1)).sum()
+    assert np.allclose(img[..., 0], 0.5 * want, atol=1e-14) and np.allclose(img[..., 3], S * 0.3, atol=1e-14)
+    rng = np.random.Generator(np.random.PCG64(8))
+    sig = rng.uniform(0, 0.5, size=(2, 5, 3, 4))
+    w = rng.uniform(-1, 1, size=(2, 3, 4, 3))
+    light = (0.7, 0.2, 1.1)
+    _, g = orc.blend_beer_lambert_fwd_bwd(sig, light, w)
+
+    def loss(x):
+        return float((orc.blend_beer_lambert_fwd_bwd(x, light)[0][..., :3] * w).sum())
+    for idx in [(0, 0, 0, 0), (1, 4, 2, 3), (0, 2, 1, 2), (1, 1, 0, 1)]:
+        a, b = sig.copy(), sig.copy()
+        a[idx] += 1e-6
+        b[idx] -= 1e-6
+        assert abs((loss(a) - loss(b)) / 2e-6 - g[idx]) <= 1e-8
