@@ -41,11 +41,14 @@ render gradient on a float32 render reduced two ways, the deterministic
 segmented gather and the reverse walk with float32 atomics.
 
 ``--batched`` instead runs workload c4: 32 independent reconstructions of
-uneven grid sizes (32^3 to 128x224x128) and view counts (1-8), each a fused
-fwd+bwd step, issued eagerly one after another, replayed from one captured
-graph on one stream, and replayed from one graph in which every
-reconstruction is a branch on its own stream (forked from and joined to the
-capture stream), so that the small reconstructions run side by side.
+uneven grid sizes (32^3 to 128x224x128) and view counts (1-8). The headline
+is the batched mipmapped density-reconstruction step (every reconstruction's
+non-fused render with LINEAR mips and per-sample LoD, L2 loss, exact
+gradient, in a few launches over one table; bricks autotuned per grid class),
+also replayed from a graph and against running each reconstruction alone.
+Beside it: the fused transport step of every reconstruction issued eagerly,
+replayed from one graph on one stream, and from one graph with a branch per
+reconstruction on its own stream.
 
 ``--lit`` instead runs the c2 step under the production light (a spot light
 with a 256x196x196 shadow map, LINEAR mips with 4 levels and the gather
@@ -609,17 +612,76 @@ def batched_bench(args):
     for r in recs:
         key = "x".join(str(x) for x in r["grid"][::-1])
         sizes[key] = sizes.get(key, 0) + 1
+    fused = {"voxel_samples_per_batch": nvs, "ms_per_batch": best, "value": nvs / (best * 1e-3),
+             "ms": {"eager, one stream": ms_eager, "one graph, one stream": ms_graph,
+                    "one graph, a branch per reconstruction": ms_branches},
+             "replay_vs_eager": {"max_rel_loss": worst_loss, "images_equal": images_equal,
+                                 "max_norm_grad_density": worst_gd}}
+    del steps, inputs, eager, g1, gb
+    torch.cuda.empty_cache()
+
+    # the batched mipmapped density-reconstruction step: every reconstruction in one table
+    from gft._lib import KernelTimers
+    from gft.batch import BatchedRenderStep
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_batch_gpu import c4_batch
+    brecs, dens, tdens = c4_batch(32, seed=7)
+    print("batched: mipmapped step built", file=sys.stderr, flush=True)
+    tb = BatchedRenderStep(brecs, num_mipmaps=3)
+    tb.bind([torch.from_numpy(d).to(dev) for d in tdens],
+            [torch.zeros((r["V"].shape[0],) + tuple(r["out_dhw"][1:]) + (3,), device=dev) for r in brecs])
+    tb.run()
+    targets = [i[..., :3].contiguous() for i in tb.images]
+    b = BatchedRenderStep(brecs, num_mipmaps=3)
+    b.bind([torch.from_numpy(d).to(dev) for d in dens], targets)
+    ms_default = b.time_ms(args.steps)
+    tuned = b.autotune(reps=max(3, args.steps // 2))
+    ms_tuned = b.time_ms(args.steps)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        b.run()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    gm = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gm):
+        b.run()
+    ms_mgraph = timed(gm.replay)
+    # each reconstruction as a batch of its own, one after another (same kernels and bricks)
+    singles = []
+    for k, r in enumerate(brecs):
+        r1 = dict(r, brick=b.bricks[k])
+        s1 = BatchedRenderStep([r1], num_mipmaps=3)
+        s1.bind([torch.from_numpy(dens[k]).to(dev)], [targets[k]])
+        singles.append(s1)
+
+    def run_singles():
+        for s1 in singles:
+            s1.run()
+    ms_singles = timed(run_singles)
+    with KernelTimers("batch_levels,batch_render,batch_finish") as kt:
+        for _ in range(args.steps):
+            b.run()
+        torch.cuda.synchronize()
+    kern = {k: v[1] / max(v[0], 1) for k, v in kt.result.items()}
+    mvs = b.voxel_samples()
+    mbest = min(ms_tuned, ms_mgraph)
     print(json.dumps({
-        "metric": "voxel-samples/s fwd+bwd of 32 uneven reconstructions", "value": nvs / (best * 1e-3),
-        "unit": "voxel-samples/s", "n_gpus": 1, "steps": args.steps, "ms_per_batch": best,
-        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "c4: 32 reconstructions, grids (x,y,z) %s, 1-8 views each, 120x214 to 256x456 px, "
-                               "MacCormack+MC_SMOOTH, L2 loss" % sizes,
-                   "views": [r["views"] for r in recs], "voxel_samples_per_batch": nvs},
-        "ms": {"eager, one stream": ms_eager, "one graph, one stream": ms_graph,
-               "one graph, a branch per reconstruction": ms_branches},
-        "replay_vs_eager": {"max_rel_loss": worst_loss, "images_equal": images_equal,
-                            "max_norm_grad_density": worst_gd}}))
+        "metric": "voxel-samples/s fwd+bwd of 32 uneven reconstructions (batched mipmapped step)",
+        "value": mvs / (mbest * 1e-3), "unit": "voxel-samples/s", "n_gpus": 1, "steps": args.steps,
+        "ms_per_batch": mbest, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c4: 32 reconstructions, grids (x,y,z) %s, 1-8 views each, 120x214 to 256x456 px; "
+                               "per reconstruction the non-fused ambient render (TRANSFORM_LINDEPTH resample with "
+                               "LINEAR mips, LoD 0-3 per sample, BEER_LAMBERT blend), L2 loss, exact density "
+                               "gradient" % sizes,
+                   "views": [r["views"] for r in recs], "voxel_samples_per_batch": mvs},
+        "ms": {"batched, default bricks": ms_default, "batched, autotuned": ms_tuned,
+               "batched, autotuned, graph replay": ms_mgraph,
+               "each reconstruction alone, one after another": ms_singles},
+        "kernel_ms": kern,
+        "autotune": {"x".join(str(x) for x in g[::-1]): {"brick": v[0], "ms_per_candidate": v[1]}
+                     for g, v in tuned.items()},
+        "fused_transport_step_graph": fused}))
 
 
 def precision_bench(args):

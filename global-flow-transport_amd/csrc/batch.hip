@@ -16,26 +16,29 @@
 // mip pyramid.
 //
 // Launches (each one over every reconstruction, grid-stride over its items):
-//   k_batch_level(m)   m = 0: clear the density gradients; m >= 1: build mip
+//   k_batch_level(m)   m = 0: copy the density into level 0 of the pyramid
+//                      and clear the density gradient; m >= 1: build mip
 //                      level m from level m-1 (kGenerateMip3D's pairwise
 //                      means) and clear its gradient level;
 //   k_batch_render     forward and backward of every ray in one pass: items
-//                      are (reconstruction, view, 16x16-pixel tile); a ray
-//                      marches its samples, writes its pixel and loss partial,
-//                      forms its loss gradient and walks back from far to
-//                      near recomputing the transmittance, scattering the
-//                      sample gradients through the level taps; level-0 taps
-//                      go to an FP64 LDS brick per tile (or 8x8 sub-tile) and
-//                      K-sample segment, flushed with float atomics; coarser
-//                      levels take float atomics directly;
+//                      are (reconstruction, view, 16x16-pixel tile); a ray,
+//                      clipped to the grid box, marches its samples keeping
+//                      each sample's c_k, writes its pixel and loss partial,
+//                      forms its loss gradient and scatters the sample
+//                      gradients alpha c_k + beta through the level taps;
+//                      level-0 and level-1 taps go to FP64 LDS bricks per tile
+//                      (or 8x8 sub-tile) and segment of at most K samples,
+//                      flushed with float atomics; coarser levels (rare) take
+//                      float atomics directly;
 //   k_batch_finish     the mip adjoint (every level-0 voxel adds 1/8^m of its
 //                      level-m ancestor's gradient) and, in the first item of
 //                      each reconstruction, its loss: the per-tile partials
 //                      summed in a fixed order.
-// The brick of the backward (tile size x segment length) is chosen per
+// The brick of the backward (tile size x longest segment) is chosen per
 // reconstruction: 16x16-pixel tiles with 8-sample segments (about 8^3 cells
 // at two pixels per cell), 8x8 tiles with 4-sample segments (about 4^3), or
-// 8x8 tiles with 8 (about 4x4x8).
+// 8x8 tiles with 8 (about 4x4x8). A segment whose bricks would not fit the
+// LDS is halved until they do.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -48,13 +51,14 @@ namespace gft {
 constexpr int kBatchMaxViews = 8;
 constexpr int kBatchMaxLevels = 3;
 constexpr int kBatchCap = 6144;  // FP64 brick cells per workgroup (48 KiB)
+constexpr int kRenderGrid = 768;  // render workgroups: as many as are resident (three per CU, by LDS)
 
 struct BatchRec {
-  MipTable mt;            // density pyramid: level 0 = dens, levels 1..L in atlas
+  MipTable mt;            // dims of levels 0..L; offset[m]: float offset of level m in pyr (offset[0] = 0)
   const float* dens;
-  float* atlas;
+  float* pyr;             // density pyramid, levels 0..L (level 0 a copy of dens)
   float* grad;            // level-0 gradient, overwritten
-  float* gatlas;          // gradients of levels 1..L (same layout as atlas)
+  float* gatlas;          // gradients of levels 1..L, level m at offset[m] - offset[1]
   const float* target;    // [V,H,W,3]
   float* images;          // [V,H,W,4]
   float* partials;        // per render item
@@ -83,9 +87,14 @@ __device__ __forceinline__ int rec_of(const BatchRec* __restrict__ recs, int nre
   return lo;
 }
 
+__device__ __forceinline__ float* level_grad(const BatchRec& R, int m) {
+  return m == 0 ? R.grad : R.gatlas + (R.mt.offset[m] - R.mt.offset[1]);
+}
+
 // ------------------------------------------------------------ level passes
-// m = 0: zero the level-0 gradient; m >= 1: level m of the pyramid (pairwise
-// means in z, then y, then x, as kGenerateMip3D) and zero its gradient.
+// m = 0: copy the density into the pyramid and zero the level-0 gradient;
+// m >= 1: level m of the pyramid (pairwise means in z, then y, then x, as
+// kGenerateMip3D) and zero its gradient.
 __global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict__ recs, int nrec, int m,
                                                      int total_items) {
 #pragma clang fp contract(off)
@@ -97,11 +106,12 @@ __global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict_
     const size_t f = (size_t)(item - R.lvl_item0[m]) * 256 + threadIdx.x;
     if (f >= n) continue;
     if (m == 0) {
+      R.pyr[f] = R.dens[f];
       R.grad[f] = 0.f;
       continue;
     }
     const Dims3 s = R.mt.dims[m - 1];
-    const float* in = m == 1 ? R.dens : R.atlas + R.mt.offset[m - 1];
+    const float* in = m == 1 ? R.dens : R.pyr + R.mt.offset[m - 1];
     const int x = (int)(f % (size_t)o.x), y = (int)((f / (size_t)o.x) % (size_t)o.y), z = (int)(f / ((size_t)o.x * o.y));
     float ya[2];
 #pragma unroll
@@ -114,69 +124,226 @@ __global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict_
       }
       ya[xi] = (za[0] + za[1]) * 0.5f;
     }
-    R.atlas[R.mt.offset[m] + f] = (ya[0] + ya[1]) * 0.5f;
-    R.gatlas[R.mt.offset[m] + f] = 0.f;
+    R.pyr[R.mt.offset[m] + f] = (ya[0] + ya[1]) * 0.5f;
+    level_grad(R, m)[f] = 0.f;
   }
 }
 
 // ---------------------------------------------------------------- render
-__device__ __forceinline__ float batch_sample(const BatchRec& R, const SampleCfg& cfg, float3 p, float lod) {
-  int lv[2];
-  float w[2];
-  const int n = level_taps(cfg, lod, lv, w);
-  float v = 0.f;
-  for (int k = 0; k < n; ++k) {
-    const float sc = 1.f / (float)(1 << lv[k]);
-    const float* buf = lv[k] == 0 ? R.dens : R.atlas + R.mt.offset[lv[k]];
-    const float a = read_linear<1, B_BORDER>(buf, R.mt.dims[lv[k]], p.x * sc - 0.5f, p.y * sc - 0.5f,
-                                             p.z * sc - 0.5f).v[0];
-    v = k == 0 ? a : lerpf(v, a, w[1]);
-  }
-  return v;
-}
-
-struct BatchBrick {
-  int lo[3], n[3];
-  int size;
+// One pixel column of the TRANSFORM_LINDEPTH resample. LINDEPTH spaces the
+// samples uniformly in view depth and a pixel's unprojection is affine in
+// view depth, so the positions are affine in the depth index: p(i) = p0 +
+// i dp, from the two end samples of the column. The LoD footprint's
+// differences to the +x and +y pixels are affine in i as well (e0 + i e1)
+// and the difference to sample i + 1 is dp.
+struct Column {
+  float3 p0, dp, ex0, ex1, ey0, ey1;
 };
 
-// bounding brick (level-0 corner indices) of the samples ia..ib of the rays
-// of pixels [x0, x1] x [y0, y1]: the index -> object map is affine along a
-// ray (uniform view depth) and across pixels at one depth, so the four corner
-// rays at the two segment ends bound every sample of the tile
-__device__ __forceinline__ BatchBrick batch_brick(const SampleView& v, float3 oinv, int x0, int x1, int y0, int y1,
-                                                  int ia, int ib, Dims3 d) {
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 scale3(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 fma3(float s, float3 a, float3 b) {
+  return make_float3(fmaf(s, a.x, b.x), fmaf(s, a.y, b.y), fmaf(s, a.z, b.z));
+}
+
+__device__ __forceinline__ Column lindepth_column(const SampleView& v, float qx, float qy, int S, float3 oinv,
+                                                  bool lod) {
+  const float3 z3 = make_float3(0.f, 0.f, 0.f);
+  const float qa = 0.5f, qb = (float)S - 0.5f, inv = S > 1 ? 1.f / (float)(S - 1) : 0.f;
+  Column c;
+  const float3 pa = map_pos(v, CM_LINDEPTH, qx, qy, qa, oinv, z3), pb = map_pos(v, CM_LINDEPTH, qx, qy, qb, oinv, z3);
+  c.p0 = pa;
+  c.dp = scale3(sub3(pb, pa), inv);
+  c.ex0 = c.ex1 = c.ey0 = c.ey1 = z3;
+  if (lod) {
+    const float3 xa = sub3(pa, map_pos(v, CM_LINDEPTH, qx + 1.f, qy, qa, oinv, z3));
+    const float3 xb = sub3(pb, map_pos(v, CM_LINDEPTH, qx + 1.f, qy, qb, oinv, z3));
+    const float3 ya = sub3(pa, map_pos(v, CM_LINDEPTH, qx, qy + 1.f, qa, oinv, z3));
+    const float3 yb = sub3(pb, map_pos(v, CM_LINDEPTH, qx, qy + 1.f, qb, oinv, z3));
+    c.ex0 = xa;
+    c.ex1 = scale3(sub3(xb, xa), inv);
+    c.ey0 = ya;
+    c.ey1 = scale3(sub3(yb, ya), inv);
+  }
+  return c;
+}
+
+// samplePos3D's LoD (sampling.hpp:114-123): log2 of the largest per-axis
+// footprint, the norms of the Jacobian rows of the forward differences
+__device__ __forceinline__ float column_lod(const Column& c, float fi) {
+  const float3 ex = fma3(fi, c.ex1, c.ex0), ey = fma3(fi, c.ey1, c.ey0);
+  const float fx = fmaf(ex.x, ex.x, fmaf(ey.x, ey.x, c.dp.x * c.dp.x));
+  const float fy = fmaf(ex.y, ex.y, fmaf(ey.y, ey.y, c.dp.y * c.dp.y));
+  const float fz = fmaf(ex.z, ex.z, fmaf(ey.z, ey.z, c.dp.z * c.dp.z));
+  return fmaxf(0.f, 0.5f * log2f(fmaxf(fx, fmaxf(fy, fz))));
+}
+
+// weight of level m among a sample's mip taps (0 if the sample does not read it)
+__device__ __forceinline__ float level_weight(int m, int nl, const int lv[2], const float lw[2]) {
+  return lv[0] == m ? lw[0] : (((nl == 2) & (lv[1] == m)) ? lw[1] : 0.f);
+}
+
+// BORDER trilinear value at level base `buf` (make_taps' corners, zeroing and
+// lerp order, as read_linear): flat indices of one level fit 32 bits
+__device__ __forceinline__ float level_read(const float* __restrict__ buf, Dims3 d, float px, float py, float pz) {
+  const Taps t = make_taps<B_BORDER>(px, py, pz, d);
+  if (!t.inside) return 0.f;
+  float c[2][2][2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const unsigned row = ((unsigned)t.iz[k] * (unsigned)d.y + (unsigned)t.iy[j]) * (unsigned)d.x;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const float a = buf[row + (unsigned)t.ix[i]];
+        c[k][j][i] = (t.vx[i] & t.vy[j] & t.vz[k]) ? a : 0.f;
+      }
+    }
+  const float v0 = lerpf(lerpf(c[0][0][0], c[0][0][1], t.wx), lerpf(c[0][1][0], c[0][1][1], t.wx), t.wy);
+  const float v1 = lerpf(lerpf(c[1][0][0], c[1][0][1], t.wx), lerpf(c[1][1][0], c[1][1][1], t.wx), t.wy);
+  return lerpf(v0, v1, t.wz);
+}
+
+// the mipmapped sample: level taps of the LoD, each level read at a
+// wave-uniform base (the level loop is uniform, the taps per lane)
+__device__ __forceinline__ float batch_sample(const BatchRec& R, const SampleCfg& cfg, float3 p, float lod) {
+  int lv[2];
+  float lw[2];
+  const int nl = level_taps(cfg, lod, lv, lw);
+  float vf = 0.f, vc = 0.f;
+#pragma unroll
+  for (int m = 0; m <= kBatchMaxLevels; ++m) {
+    if (m > R.mt.levels) break;
+    const bool f = lv[0] == m, c = (nl == 2) & (lv[1] == m);
+    if (f | c) {
+      const float sc = 1.f / (float)(1 << m);
+      const float a = level_read(R.pyr + R.mt.offset[m], R.mt.dims[m], p.x * sc - 0.5f, p.y * sc - 0.5f,
+                                 p.z * sc - 0.5f);
+      if (f) vf = a;
+      else vc = a;
+    }
+  }
+  return nl == 2 ? lerpf(vf, vc, lw[1]) : vf;
+}
+
+// Bounding bricks of levels 0 and 1 (corner indices) of the samples ia..ib
+// of a tile's pixel columns: positions are affine along a column and, at one
+// depth, across pixels, so the four corner columns at the two segment ends
+// bound every sample of the tile. A tap outside its brick takes a float
+// atomic, so the bound only decides where a tap is summed, never whether.
+struct BatchBrick {
+  int lo0[3], n0[3], lo1[3], n1[3];
+  int size0, size;
+};
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// corner columns of a tile in LDS: [c][0..2] = p0, [c][3..5] = dp; a 16-pixel
+// brick uses corners 0..3 of a 2x2 table, the 8-pixel brick s of a 4x4 table
+// (pixel offsets 0, 7, 8, 15) the corners of its quadrant
+__device__ __forceinline__ BatchBrick batch_brick(const float (*corner)[6], int tile, int sub, int ia, int ib,
+                                                  const MipTable& mt) {
   float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-  const int px[2] = {x0, x1}, py[2] = {y0, y1}, pz[2] = {ia, ib};
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const float3 p = map_pos(v, CM_LINDEPTH, px[c & 1] + 0.5f, py[(c >> 1) & 1] + 0.5f, pz[c >> 2] + 0.5f, oinv,
-                             make_float3(0.f, 0.f, 0.f));
-    const float q[3] = {p.x - 0.5f, p.y - 0.5f, p.z - 0.5f};
+    const int cx = c & 1, cy = (c >> 1) & 1;
+    const int k = tile == 16 ? cy * 2 + cx : ((sub >> 1) * 2 + cy) * 4 + (sub & 1) * 2 + cx;
+    const float fi = (float)(c < 4 ? ia : ib);
+    const float q[3] = {fmaf(fi, corner[k][3], corner[k][0]), fmaf(fi, corner[k][4], corner[k][1]),
+                        fmaf(fi, corner[k][5], corner[k][2])};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       mn[a] = fminf(mn[a], q[a]);
       mx[a] = fmaxf(mx[a], q[a]);
     }
   }
-  const int dim[3] = {d.x, d.y, d.z};
+  const int d0[3] = {mt.dims[0].x, mt.dims[0].y, mt.dims[0].z};
+  const int d1[3] = {mt.dims[1].x, mt.dims[1].y, mt.dims[1].z};
   BatchBrick b;
-  b.size = 1;
+  int s0 = 1, s1 = 1;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int lo = clampi((int)floorf(mn[a] - 1e-3f), 0, dim[a] - 1);
-    const int hi = clampi((int)floorf(mx[a] + 1e-3f) + 1, 0, dim[a] - 1);
-    b.lo[a] = lo;
-    b.n[a] = hi - lo + 1;
-    b.size *= b.n[a];
+    int lo = clampi((int)floorf(mn[a] - 0.5f - 1e-3f), 0, d0[a] - 1);
+    int hi = clampi((int)floorf(mx[a] - 0.5f + 1e-3f) + 1, 0, d0[a] - 1);
+    b.lo0[a] = uniform_i(lo);
+    b.n0[a] = uniform_i(hi - lo + 1);
+    s0 *= b.n0[a];
+    if (mt.levels > 0) {
+      lo = clampi((int)floorf(mn[a] * 0.5f - 0.5f - 1e-3f), 0, d1[a] - 1);
+      hi = clampi((int)floorf(mx[a] * 0.5f - 0.5f + 1e-3f) + 1, 0, d1[a] - 1);
+    } else {
+      lo = 0;
+      hi = -1;
+    }
+    b.lo1[a] = uniform_i(lo);
+    b.n1[a] = uniform_i(hi - lo + 1);
+    s1 *= b.n1[a];
   }
+  b.size0 = s0;
+  b.size = s0 + s1;
   return b;
 }
 
-__global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict__ recs, int nrec, int total_items) {
+// gradient of one level tap (BORDER corners, weight products in the order
+// ((g w_x) w_y) w_z): into the level's LDS brick when the whole corner cube
+// lies in it, float atomics otherwise
+__device__ __forceinline__ void level_scatter(double* __restrict__ lds, bool staged, const int blo[3],
+                                              const int bn[3], float* __restrict__ gbuf, Dims3 d, float px, float py,
+                                              float pz, float gw) {
+  const Taps t = make_taps<B_BORDER>(px, py, pz, d);
+  if (!t.inside) return;
+  const float wx[2] = {1.f - t.wx, t.wx}, wy[2] = {1.f - t.wy, t.wy}, wz[2] = {1.f - t.wz, t.wz};
+  float w[2][2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const float gx = gw * wx[i];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const float gxy = gx * wy[j];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) w[k][j][i] = (t.vx[i] & t.vy[j] & t.vz[k]) ? gxy * wz[k] : 0.f;
+    }
+  }
+  const int lx = t.ix[0] - blo[0], ly = t.iy[0] - blo[1], lz = t.iz[0] - blo[2];
+  const int hx = t.ix[1] - blo[0], hy = t.iy[1] - blo[1], hz = t.iz[1] - blo[2];
+  if (staged & (lx >= 0) & (ly >= 0) & (lz >= 0) & (hx < bn[0]) & (hy < bn[1]) & (hz < bn[2])) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int row = ((k ? hz : lz) * bn[1] + (j ? hy : ly)) * bn[0];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) atomicAdd(&lds[row + (i ? hx : lx)], (double)w[k][j][i]);
+      }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const unsigned row = ((unsigned)t.iz[k] * (unsigned)d.y + (unsigned)t.iy[j]) * (unsigned)d.x;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          if (w[k][j][i] != 0.f) atomic_addf(gbuf + row + (unsigned)t.ix[i], w[k][j][i]);
+      }
+  }
+}
+
+// The forward keeps c_k = exp(-D_k) + E_{k-1} of every sample of its column
+// (D_k the optical depth through sample k, E_{k-1} the emission before it):
+// with E the pixel's total, dE/dsigma_k = exp(-D_k) - sum_{i>=k} exp(-D_i)
+// sigma_i = c_k - E, so the backward needs no second march. The values live
+// in a per-workgroup scratch of smax samples x 256 columns (sample-major, so
+// a wave's 64 columns are one coalesced run), reused by every item of the
+// workgroup and read back soon after they are written.
+__global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict__ recs, int nrec, int total_items,
+                                                      float* __restrict__ csave, int smax) {
   __shared__ double lds[kBatchCap];
   __shared__ float red[256];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ int span[2];
+  __shared__ float corner[16][6];
+  const int wave = uniform_i((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
+  float* __restrict__ cs = csave + (size_t)blockIdx.x * smax * 256 + threadIdx.x;
   for (int i = threadIdx.x; i < kBatchCap; i += 256) lds[i] = 0.0;
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
     const BatchRec& R = recs[rec_of(recs, nrec, item, -1)];
@@ -185,40 +352,40 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
     const int vi = local / per_view, t = local % per_view;
     const int tx0 = (t % R.tiles_x) * 16, ty0 = (t / R.tiles_x) * 16;
     const SampleView& V = R.view[vi];
-    // this thread's pixel and the pixels of its brick
-    int x, y, bx0, by0, bw, cap, cap0;
+    // this thread's pixel: 16x16 bricks take the tile in rows, 8x8 bricks give each wave a quadrant
+    int x, y, cap, cap0;
     if (R.tile == 16) {
       x = tx0 + (threadIdx.x & 15);
       y = ty0 + (threadIdx.x >> 4);
-      bx0 = tx0;
-      by0 = ty0;
-      bw = 16;
       cap = kBatchCap;
       cap0 = 0;
     } else {
-      bx0 = tx0 + (wave & 1) * 8;
-      by0 = ty0 + (wave >> 1) * 8;
-      x = bx0 + (lane & 7);
-      y = by0 + (lane >> 3);
-      bw = 8;
+      x = tx0 + (wave & 1) * 8 + (lane & 7);
+      y = ty0 + (wave >> 1) * 8 + (lane >> 3);
       cap = kBatchCap / 4;
       cap0 = wave * cap;
     }
     const bool valid = x < R.out_w && y < R.out_h;
     const int S = R.out_d;
     const float3 oinv = make_float3(1.f / (float)R.out_w, 1.f / (float)R.out_h, 1.f / (float)S);
-    const SampleCfg cfg{F_LINEAR, R.mt.levels > 0 ? MIP_LINEAR : MIP_NONE, R.mt.levels, 0.f, 0.5f};
+    const bool mips = R.mt.levels > 0;
+    const SampleCfg cfg{F_LINEAR, mips ? MIP_LINEAR : MIP_NONE, R.mt.levels, 0.f, 0.5f};
+    const Column col = lindepth_column(V, x + 0.5f, y + 0.5f, S, oinv, mips);
+    // samples outside the box [0, dims) read zero at every mip level (a level-m
+    // sample is inside only where 0 <= p < dims_m 2^m <= dims), so they add
+    // nothing to the blend or the gradient: the column is clipped to the box
+    int lo = 1, hi = 0;
+    if (valid) clip_ray(col.p0, col.dp, R.mt.dims[0], S, 0.f, lo, hi);
     // ---- forward: BEER_LAMBERT blend of the sampled frustum column
     float D = 0.f, E = 0.f;
-    if (valid) {
-      for (int i = 0; i < S; ++i) {
-        float lod;
-        const float3 p = sample_pos(V, CM_LINDEPTH, R.mt.levels > 0, x + 0.5f, y + 0.5f, i + 0.5f, oinv,
-                                    make_float3(0.f, 0.f, 0.f), lod);
-        const float sigma = R.step * batch_sample(R, cfg, p, lod);
-        D += sigma;
-        E = fmaf(__expf(-D), sigma, E);
-      }
+    for (int i = lo; i <= hi; ++i) {
+      const float fi = (float)i;
+      const float lod = mips ? column_lod(col, fi) : 0.f;
+      const float sigma = R.step * batch_sample(R, cfg, fma3(fi, col.dp, col.p0), lod);
+      D += sigma;
+      const float u = __expf(-D);
+      cs[(size_t)i * 256] = u + E;
+      E = fmaf(u, sigma, E);
     }
     float G = 0.f, sq = 0.f;
     if (valid) {
@@ -230,82 +397,109 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
       sq = e0 * e0 + e1 * e1 + e2 * e2;
       G = R.loss_scale * 2.f * (R.light[0] * e0 + R.light[1] * e1 + R.light[2] * e2);
     }
+    const bool walk = valid && G != 0.f && lo <= hi;
+    const float alpha = R.step * G, beta = -(R.step * G) * E;
     // per-item loss partial: a fixed-order tree over the tile's 256 pixel
-    // slots, the same sum whatever the brick's thread-to-pixel mapping
+    // slots, the same sum whatever the brick's thread-to-pixel mapping; the
+    // union of the walking columns' sample ranges; the bricks' corner columns
     red[(y - ty0) * 16 + (x - tx0)] = sq;
+    if (threadIdx.x == 0) {
+      span[0] = S;
+      span[1] = -1;
+    }
+    if (threadIdx.x < (R.tile == 16 ? 4 : 16)) {
+      const int ncx = R.tile == 16 ? 2 : 4;
+      const int ox = (int)threadIdx.x % ncx, oy = (int)threadIdx.x / ncx;
+      const int offs[4] = {0, 7, 8, 15};
+      const int dx = R.tile == 16 ? 15 * ox : offs[ox], dy = R.tile == 16 ? 15 * oy : offs[oy];
+      const Column cc = lindepth_column(V, min(tx0 + dx, R.out_w - 1) + 0.5f, min(ty0 + dy, R.out_h - 1) + 0.5f, S,
+                                        oinv, false);
+      float* cr = corner[threadIdx.x];
+      cr[0] = cc.p0.x;
+      cr[1] = cc.p0.y;
+      cr[2] = cc.p0.z;
+      cr[3] = cc.dp.x;
+      cr[4] = cc.dp.y;
+      cr[5] = cc.dp.z;
+    }
     __syncthreads();
+    if (walk) {
+      atomicMin(&span[0], lo);
+      atomicMax(&span[1], hi);
+    }
     for (int w = 128; w > 0; w >>= 1) {
       if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
       __syncthreads();
     }
     if (threadIdx.x == 0) R.partials[local] = red[0];
-    // ---- backward: walk far to near, recomputing sigma and the transmittance
-    const bool walk = valid && G != 0.f;
-    float Racc = 0.f;
-    const int bx1 = min(bx0 + bw - 1, R.out_w - 1), by1 = min(by0 + bw - 1, R.out_h - 1);
-    const int K = R.seg;
-    for (int sgi = (S + K - 1) / K - 1; sgi >= 0; --sgi) {
-      const int ia = sgi * K, ib = min(ia + K, S) - 1;
-      const BatchBrick br = batch_brick(V, oinv, bx0, bx1, by0, by1, ia, ib, R.mt.dims[0]);
+    const int s_lo = span[0], s_hi = span[1];
+    __syncthreads();  // span and red are rewritten by the next item
+    // ---- backward: segments of at most R.seg samples, each as long as its
+    // bricks fit the LDS (halved until they do, down to one sample; in the
+    // 8-pixel mode until every wave's brick fits, so all waves take the same
+    // segments), the sample gradients c_k -> alpha c_k + beta scattered
+    // through the level taps
+    const int sub = R.tile == 16 ? 0 : wave;
+    for (int ia = s_lo; ia <= s_hi;) {
+      int k = R.seg, ib;
+      BatchBrick br;
+      for (;;) {
+        ib = min(ia + k - 1, s_hi);
+        br = batch_brick(corner, R.tile, sub, ia, ib, R.mt);
+        int need = br.size;
+        if (R.tile == 8)
+          for (int s2 = 0; s2 < 4; ++s2)
+            if (s2 != sub) need = max(need, batch_brick(corner, 8, s2, ia, ib, R.mt).size);
+        if (need <= cap || k == 1) break;
+        k >>= 1;
+      }
       const bool use_lds = br.size <= cap;
       if (walk) {
-        for (int i = ib; i >= ia; --i) {
-          float lod;
-          const float3 p = sample_pos(V, CM_LINDEPTH, R.mt.levels > 0, x + 0.5f, y + 0.5f, i + 0.5f, oinv,
-                                      make_float3(0.f, 0.f, 0.f), lod);
-          const float sigma = R.step * batch_sample(R, cfg, p, lod);
-          const float u = __expf(-D);
-          Racc = fmaf(u, sigma, Racc);
-          const float g = R.step * G * (u - Racc);
-          D -= sigma;
+        for (int i = max(ia, lo); i <= min(ib, hi); ++i) {
+          const float fi = (float)i;
+          const float3 p = fma3(fi, col.dp, col.p0);
+          const float lod = mips ? column_lod(col, fi) : 0.f;
+          const float g = fmaf(alpha, cs[(size_t)i * 256], beta);
           int lv[2];
           float lw[2];
           const int nl = level_taps(cfg, lod, lv, lw);
-          for (int k = 0; k < nl; ++k) {
-            const int m = lv[k];
-            const float sc = 1.f / (float)(1 << m);
-            const Dims3 dm = R.mt.dims[m];
-            const Taps tp = make_taps<B_BORDER>(p.x * sc - 0.5f, p.y * sc - 0.5f, p.z * sc - 0.5f, dm);
-            if (!tp.inside) continue;
-            const float gw = g * lw[k];
-            const float wx[2] = {1.f - tp.wx, tp.wx}, wy[2] = {1.f - tp.wy, tp.wy}, wz[2] = {1.f - tp.wz, tp.wz};
-            for (int kz = 0; kz < 2; ++kz)
-              for (int ky = 0; ky < 2; ++ky)
-                for (int kx = 0; kx < 2; ++kx) {
-                  if (!(tp.vx[kx] & tp.vy[ky] & tp.vz[kz])) continue;
-                  const float w = gw * wx[kx] * wy[ky] * wz[kz];
-                  if (w == 0.f) continue;
-                  const int cx = tp.ix[kx], cy = tp.iy[ky], cz = tp.iz[kz];
-                  if (m == 0) {
-                    const int lx = cx - br.lo[0], ly = cy - br.lo[1], lz = cz - br.lo[2];
-                    if (use_lds & ((unsigned)lx < (unsigned)br.n[0]) & ((unsigned)ly < (unsigned)br.n[1]) &
-                        ((unsigned)lz < (unsigned)br.n[2]))
-                      atomicAdd(&lds[cap0 + (lz * br.n[1] + ly) * br.n[0] + lx], (double)w);
-                    else
-                      atomic_addf(R.grad + flat3(cx, cy, cz, dm), w);
-                  } else {
-                    atomic_addf(R.gatlas + R.mt.offset[m] + flat3(cx, cy, cz, dm), w);
-                  }
-                }
+#pragma unroll
+          for (int m = 0; m <= kBatchMaxLevels; ++m) {
+            if (m > R.mt.levels) break;
+            const float wm = level_weight(m, nl, lv, lw);
+            if (wm != 0.f) {
+              const float sc = 1.f / (float)(1 << m);
+              level_scatter(lds + cap0 + (m == 1 ? br.size0 : 0), use_lds & (m <= 1), m == 0 ? br.lo0 : br.lo1,
+                            m == 0 ? br.n0 : br.n1, level_grad(R, m), R.mt.dims[m], p.x * sc - 0.5f,
+                            p.y * sc - 0.5f, p.z * sc - 0.5f, g * wm);
+            }
           }
         }
       }
-      // (8-pixel tiles: each wave owns its brick, so use_lds may differ between waves and the
+      // (8-pixel bricks: each wave owns its brick, so use_lds may differ between waves and the
       // barriers stay outside the branch)
       __syncthreads();
       if (use_lds) {
-        // flush the brick (all 256 threads for a 16-pixel tile, the wave's 64 lanes for an 8-pixel one)
+        // flush the bricks (all 256 threads for a 16-pixel tile, the wave's 64 lanes for an 8-pixel one)
         const int nthr = R.tile == 16 ? 256 : 64, tid = R.tile == 16 ? (int)threadIdx.x : lane;
         for (int idx = tid; idx < br.size; idx += nthr) {
           const double val = lds[cap0 + idx];
           if (val != 0.0) {
-            const int lx = idx % br.n[0], ly = (idx / br.n[0]) % br.n[1], lz = idx / (br.n[0] * br.n[1]);
-            atomic_addf(R.grad + flat3(br.lo[0] + lx, br.lo[1] + ly, br.lo[2] + lz, R.mt.dims[0]), (float)val);
+            const bool l0 = idx < br.size0;
+            const int j = l0 ? idx : idx - br.size0;
+            const int nx = l0 ? br.n0[0] : br.n1[0], ny = l0 ? br.n0[1] : br.n1[1];
+            const int lx = j % nx, ly = (j / nx) % ny, lz = j / (nx * ny);
+            if (l0)
+              atomic_addf(R.grad + flat3(br.lo0[0] + lx, br.lo0[1] + ly, br.lo0[2] + lz, R.mt.dims[0]), (float)val);
+            else
+              atomic_addf(level_grad(R, 1) + flat3(br.lo1[0] + lx, br.lo1[1] + ly, br.lo1[2] + lz, R.mt.dims[1]),
+                          (float)val);
             lds[cap0 + idx] = 0.0;
           }
         }
       }
       __syncthreads();
+      ia = ib + 1;
     }
   }
 }
@@ -341,7 +535,7 @@ __global__ __launch_bounds__(256) void k_batch_finish(const BatchRec* __restrict
       s *= 0.125f;
       const Dims3 o = R.mt.dims[m];
       const int px = x >> m, py = y >> m, pz = z >> m;
-      if (px < o.x && py < o.y && pz < o.z) g = fmaf(s, R.gatlas[R.mt.offset[m] + flat3(px, py, pz, o)], g);
+      if (px < o.x && py < o.y && pz < o.z) g = fmaf(s, level_grad(R, m)[flat3(px, py, pz, o)], g);
     }
     R.grad[f] += g;
   }
@@ -362,24 +556,28 @@ static MipTable batch_mip_table(Dims3 d, int levels) {
   mt.C = 1;
   mt.dims[0] = d;
   mt.base_stride = (size_t)d.x * d.y * d.z;
-  size_t off = 0;
+  // the pyramid: level 0 at offset 0, each level on a 256-byte boundary
+  size_t off = (mt.base_stride + 63) & ~size_t(63);
   for (int m = 1; m <= levels; ++m) {
     mt.dims[m] = Dims3{d.x >> m, d.y >> m, d.z >> m};
     mt.offset[m] = off;
-    off += (size_t)mt.dims[m].x * mt.dims[m].y * mt.dims[m].z;
+    off += ((size_t)mt.dims[m].x * mt.dims[m].y * mt.dims[m].z + 63) & ~size_t(63);
   }
-  mt.atlas_stride = (off + 63) & ~size_t(63);
+  mt.atlas_stride = off;
   return mt;
 }
 
 static int brick_tile(int brick) { return brick == 0 ? 16 : 8; }
 static int brick_seg(int brick) { return brick == 1 ? 4 : 8; }
 
-// workspace layout: the record table, then per reconstruction its atlas,
-// gradient atlas and render partials
+// workspace layout: the record table, then per reconstruction its density
+// pyramid (levels 0..L), the gradients of levels 1..L and the render
+// partials, then the render's c_k scratch
 struct BatchLayout {
   size_t table, total;
   std::vector<size_t> atlas, gatlas, partials;
+  size_t csave;  // the render's per-workgroup c_k scratch (kRenderGrid x 256 x smax floats)
+  int smax;      // the most depth samples of any reconstruction
 };
 
 static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmaps, BatchLayout& L) {
@@ -407,11 +605,15 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
     L.atlas[r] = off;
     off += align256b(mt.atlas_stride * sizeof(float));
     L.gatlas[r] = off;
-    off += align256b(mt.atlas_stride * sizeof(float));
+    off += align256b((mt.levels > 0 ? mt.atlas_stride - mt.offset[1] : 0) * sizeof(float));
     const size_t items = (size_t)c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
     L.partials[r] = off;
     off += align256b(items * sizeof(float));
   }
+  L.smax = 1;
+  for (int r = 0; r < n; ++r) L.smax = std::max(L.smax, (int)rs[r].output_shape[0]);
+  L.csave = off;
+  off += align256b((size_t)kRenderGrid * 256 * (size_t)L.smax * sizeof(float));
   L.total = off;
   return GFT_OK;
 }
@@ -463,7 +665,7 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
     const Dims3 d{c.grid_dhw[2], c.grid_dhw[1], c.grid_dhw[0]};
     R.mt = batch_mip_table(d, num_mipmaps);
     R.dens = c.density;
-    R.atlas = (float*)(ws + L.atlas[r]);
+    R.pyr = (float*)(ws + L.atlas[r]);
     R.grad = c.density_grad;
     R.gatlas = (float*)(ws + L.gatlas[r]);
     R.target = c.targets;
@@ -533,7 +735,8 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
   if ((rc = check_hip(hipGetLastError(), "batched mip levels"))) return rc;
   {
     KernelTimer t(s, "batch_render");
-    hipLaunchKernelGGL(k_batch_render, dim3(persistent_grid(render)), dim3(256), 0, s, recs, n, render);
+    hipLaunchKernelGGL(k_batch_render, dim3((unsigned)std::max(1, std::min(render, kRenderGrid))), dim3(256), 0, s,
+                       recs, n, render, (float*)((char*)workspace + L.csave), L.smax);
   }
   if ((rc = check_hip(hipGetLastError(), "batched render"))) return rc;
   {

@@ -9,8 +9,10 @@
 //
 // Backward (raymarch_grid.cu.cc:343-372): the ray is walked in reverse from
 // the forward output; transmittance is recovered by subtracting each cell's
-// optical depth (no per-sample state), and the per-sample gradient is
-// scattered to the 8 trilinear corners with float atomics.
+// optical depth. CLAMP scatters each sample's gradient to its 8 trilinear
+// corners with float atomics (k_raymarch_bwd); BORDER stores every sample's
+// gradient and sums them per voxel block without atomics in global memory
+// (k_raymarch_bwd_walk + k_raymarch_bwd_gather, below).
 //   EXACT     : true adjoint, all views summed into one gradient, sample
 //               positions identical to the forward.
 //   REFERENCE : per-camera normalisation by the accumulated trilinear weight
