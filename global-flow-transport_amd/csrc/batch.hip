@@ -16,10 +16,9 @@
 // mip pyramid.
 //
 // Launches (each one over every reconstruction, grid-stride over its items):
-//   k_batch_level(m)   m = 0: copy the density into level 0 of the pyramid
-//                      and clear the density gradient; m >= 1: build mip
-//                      level m from level m-1 (kGenerateMip3D's pairwise
-//                      means) and clear its gradient level;
+//   k_batch_level(m)   level m of the zero-bordered pyramid (m = 0: the
+//                      density; m >= 1: kGenerateMip3D's pairwise means of
+//                      level m-1) and a cleared gradient level;
 //   k_batch_render     forward and backward of every ray in one pass: items
 //                      are (reconstruction, view, 16x16-pixel tile); a ray,
 //                      clipped to the grid box, marches its samples keeping
@@ -30,10 +29,11 @@
 //                      (or 8x8 sub-tile) and segment of at most K samples,
 //                      flushed with float atomics; coarser levels (rare) take
 //                      float atomics directly;
-//   k_batch_finish     the mip adjoint (every level-0 voxel adds 1/8^m of its
-//                      level-m ancestor's gradient) and, in the first item of
-//                      each reconstruction, its loss: the per-tile partials
-//                      summed in a fixed order.
+//   k_batch_finish     the density gradient: every voxel's level-0 gradient
+//                      plus 1/8^m of its level-m ancestor's (the mip
+//                      adjoint); in the first item of each reconstruction,
+//                      its loss: the per-tile partials summed in a fixed
+//                      order.
 // The brick of the backward (tile size x longest segment) is chosen per
 // reconstruction: 16x16-pixel tiles with 8-sample segments (about 8^3 cells
 // at two pixels per cell), 8x8 tiles with 4-sample segments (about 4^3), or
@@ -50,15 +50,24 @@ namespace gft {
 
 constexpr int kBatchMaxViews = 8;
 constexpr int kBatchMaxLevels = 3;
+constexpr int kBatchPasses = kBatchMaxLevels + 2;  // level passes 0..3, then the finishing pass
 constexpr int kBatchCap = 6144;  // FP64 brick cells per workgroup (48 KiB)
 constexpr int kRenderGrid = 768;  // render workgroups: as many as are resident (three per CU, by LDS)
 
+// global-memory float (loads through it are global, not flat, instructions)
+typedef __attribute__((address_space(1))) const float gcfloat;
+
+// Every pyramid level is stored with a one-cell zero border, (W+2)(H+2)(D+2)
+// cells: a BORDER trilinear read of an inside sample is then eight unmasked
+// loads (a corner outside the grid reads the border's zero, as BORDER
+// defines), and the backward's taps of such corners land in border cells
+// that nothing reads.
 struct BatchRec {
-  MipTable mt;            // dims of levels 0..L; offset[m]: float offset of level m in pyr (offset[0] = 0)
+  MipTable mt;            // dims: levels 0..L without the border; offset[m]: level m in pyr and gpyr
   const float* dens;
-  float* pyr;             // density pyramid, levels 0..L (level 0 a copy of dens)
-  float* grad;            // level-0 gradient, overwritten
-  float* gatlas;          // gradients of levels 1..L, level m at offset[m] - offset[1]
+  float* pyr;             // bordered density pyramid, levels 0..L
+  float* gpyr;            // bordered gradients of the same cells
+  float* grad;            // level-0 gradient output
   const float* target;    // [V,H,W,3]
   float* images;          // [V,H,W,4]
   float* partials;        // per render item
@@ -69,32 +78,35 @@ struct BatchRec {
   int num_views, out_w, out_h, out_d;
   int tiles_x, tiles_y;   // 16x16-pixel tiles per view
   int tile;               // brick tile: 16 (one brick per workgroup) or 8 (one per wave)
-  int seg;                // samples per backward segment
+  int seg;                // longest backward segment in samples
   int item0;              // first render item
-  int lvl_item0[kBatchMaxLevels + 1];  // first item of the level passes
+  int pass_item0[kBatchPasses];  // first item of the level passes and of the finishing pass
   SampleView view[kBatchMaxViews];
 };
 
-// index of the reconstruction owning global item g (records sorted by first item)
-__device__ __forceinline__ int rec_of(const BatchRec* __restrict__ recs, int nrec, int g, int level) {
+// index of the reconstruction owning global item g of a pass (records sorted by first item)
+__device__ __forceinline__ int rec_of(const BatchRec* __restrict__ recs, int nrec, int g, int pass) {
   int lo = 0, hi = nrec - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    const int first = level < 0 ? recs[mid].item0 : recs[mid].lvl_item0[level];
+    const int first = pass < 0 ? recs[mid].item0 : recs[mid].pass_item0[pass];
     if (first <= g) lo = mid;
     else hi = mid - 1;
   }
   return lo;
 }
 
-__device__ __forceinline__ float* level_grad(const BatchRec& R, int m) {
-  return m == 0 ? R.grad : R.gatlas + (R.mt.offset[m] - R.mt.offset[1]);
+// flat index of bordered cell (x, y, z) (border coordinates: 0 and dims + 1
+// are the border) of a level of dims d; the host keeps (H+2)(D+2) and W+2
+// below 2^24
+__device__ __forceinline__ unsigned pad_index(int x, int y, int z, Dims3 d) {
+  return __umul24(__umul24((unsigned)z, (unsigned)(d.y + 2)) + (unsigned)y, (unsigned)(d.x + 2)) + (unsigned)x;
 }
 
 // ------------------------------------------------------------ level passes
-// m = 0: copy the density into the pyramid and zero the level-0 gradient;
-// m >= 1: level m of the pyramid (pairwise means in z, then y, then x, as
-// kGenerateMip3D) and zero its gradient.
+// Over the bordered cells of level m (border cells get zero): m = 0 copies
+// the density; m >= 1 takes pairwise means of level m-1 in z, then y, then x
+// (kGenerateMip3D's order). Both clear the level's gradient.
 __global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict__ recs, int nrec, int m,
                                                      int total_items) {
 #pragma clang fp contract(off)
@@ -102,30 +114,36 @@ __global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict_
     const BatchRec& R = recs[rec_of(recs, nrec, item, m)];
     if (m > R.mt.levels) continue;
     const Dims3 o = R.mt.dims[m];
-    const size_t n = (size_t)o.x * o.y * o.z;
-    const size_t f = (size_t)(item - R.lvl_item0[m]) * 256 + threadIdx.x;
+    const int px = o.x + 2, py = o.y + 2;
+    const size_t n = (size_t)px * py * (o.z + 2);
+    const size_t f = (size_t)(item - R.pass_item0[m]) * 256 + threadIdx.x;
     if (f >= n) continue;
-    if (m == 0) {
-      R.pyr[f] = R.dens[f];
-      R.grad[f] = 0.f;
-      continue;
-    }
-    const Dims3 s = R.mt.dims[m - 1];
-    const float* in = m == 1 ? R.dens : R.pyr + R.mt.offset[m - 1];
-    const int x = (int)(f % (size_t)o.x), y = (int)((f / (size_t)o.x) % (size_t)o.y), z = (int)(f / ((size_t)o.x * o.y));
-    float ya[2];
+    const int x = (int)(f % (size_t)px) - 1, y = (int)((f / (size_t)px) % (size_t)py) - 1,
+              z = (int)(f / ((size_t)px * py)) - 1;
+    float v = 0.f;
+    if (x >= 0 && x < o.x && y >= 0 && y < o.y && z >= 0 && z < o.z) {
+      if (m == 0) {
+        v = R.dens[flat3(x, y, z, o)];
+      } else {
+        const Dims3 s = R.mt.dims[m - 1];
+        const float* in = R.pyr + R.mt.offset[m - 1];
+        float ya[2];
 #pragma unroll
-    for (int xi = 0; xi < 2; ++xi) {
-      float za[2];
+        for (int xi = 0; xi < 2; ++xi) {
+          float za[2];
 #pragma unroll
-      for (int yi = 0; yi < 2; ++yi) {
-        const float a = in[flat3(2 * x + xi, 2 * y + yi, 2 * z, s)], b = in[flat3(2 * x + xi, 2 * y + yi, 2 * z + 1, s)];
-        za[yi] = (a + b) * 0.5f;
+          for (int yi = 0; yi < 2; ++yi) {
+            const float a = in[pad_index(2 * x + xi + 1, 2 * y + yi + 1, 2 * z + 1, s)];
+            const float b = in[pad_index(2 * x + xi + 1, 2 * y + yi + 1, 2 * z + 2, s)];
+            za[yi] = (a + b) * 0.5f;
+          }
+          ya[xi] = (za[0] + za[1]) * 0.5f;
+        }
+        v = (ya[0] + ya[1]) * 0.5f;
       }
-      ya[xi] = (za[0] + za[1]) * 0.5f;
     }
-    R.pyr[R.mt.offset[m] + f] = (ya[0] + ya[1]) * 0.5f;
-    level_grad(R, m)[f] = 0.f;
+    R.pyr[R.mt.offset[m] + f] = v;
+    R.gpyr[R.mt.offset[m] + f] = 0.f;
   }
 }
 
@@ -183,26 +201,23 @@ __device__ __forceinline__ float level_weight(int m, int nl, const int lv[2], co
   return lv[0] == m ? lw[0] : (((nl == 2) & (lv[1] == m)) ? lw[1] : 0.f);
 }
 
-// BORDER trilinear value at level base `buf` (make_taps' corners, zeroing and
-// lerp order, as read_linear): flat indices of one level fit 32 bits
-__device__ __forceinline__ float level_read(const float* __restrict__ buf, Dims3 d, float px, float py, float pz) {
-  const Taps t = make_taps<B_BORDER>(px, py, pz, d);
-  if (!t.inside) return 0.f;
-  float c[2][2][2];
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const unsigned row = ((unsigned)t.iz[k] * (unsigned)d.y + (unsigned)t.iy[j]) * (unsigned)d.x;
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const float a = buf[row + (unsigned)t.ix[i]];
-        c[k][j][i] = (t.vx[i] & t.vy[j] & t.vz[k]) ? a : 0.f;
-      }
-    }
-  const float v0 = lerpf(lerpf(c[0][0][0], c[0][0][1], t.wx), lerpf(c[0][1][0], c[0][1][1], t.wx), t.wy);
-  const float v1 = lerpf(lerpf(c[1][0][0], c[1][0][1], t.wx), lerpf(c[1][1][0], c[1][1][1], t.wx), t.wy);
-  return lerpf(v0, v1, t.wz);
+// BORDER trilinear value at bordered level `buf` of dims d, (px, py, pz) in
+// cell coordinates (p / 2^m - 0.5): make_taps' inside test, corners and lerp
+// order, as read_linear (a corner outside the grid reads the border's zero,
+// the value read_linear puts there)
+__device__ __forceinline__ float level_read(const gcfloat* __restrict__ buf, Dims3 d, float px, float py, float pz) {
+  const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+  if (!((0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) & (qz < (float)d.z)))
+    return 0.f;
+  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+  const float tx = px - fx, ty = py - fy, tz = pz - fz;
+  const unsigned sy = (unsigned)(d.x + 2), sz = __umul24(sy, (unsigned)(d.y + 2));
+  const gcfloat* c = buf + pad_index((int)fx + 1, (int)fy + 1, (int)fz + 1, d);
+  const float c000 = c[0], c001 = c[1], c010 = c[sy], c011 = c[sy + 1];
+  const float c100 = c[sz], c101 = c[sz + 1], c110 = c[sz + sy], c111 = c[sz + sy + 1];
+  const float v0 = lerpf(lerpf(c000, c001, tx), lerpf(c010, c011, tx), ty);
+  const float v1 = lerpf(lerpf(c100, c101, tx), lerpf(c110, c111, tx), ty);
+  return lerpf(v0, v1, tz);
 }
 
 // the mipmapped sample: level taps of the LoD, each level read at a
@@ -218,8 +233,8 @@ __device__ __forceinline__ float batch_sample(const BatchRec& R, const SampleCfg
     const bool f = lv[0] == m, c = (nl == 2) & (lv[1] == m);
     if (f | c) {
       const float sc = 1.f / (float)(1 << m);
-      const float a = level_read(R.pyr + R.mt.offset[m], R.mt.dims[m], p.x * sc - 0.5f, p.y * sc - 0.5f,
-                                 p.z * sc - 0.5f);
+      const float a = level_read((const gcfloat*)R.pyr + R.mt.offset[m], R.mt.dims[m], p.x * sc - 0.5f,
+                                 p.y * sc - 0.5f, p.z * sc - 0.5f);
       if (f) vf = a;
       else vc = a;
     }
@@ -227,11 +242,12 @@ __device__ __forceinline__ float batch_sample(const BatchRec& R, const SampleCfg
   return nl == 2 ? lerpf(vf, vc, lw[1]) : vf;
 }
 
-// Bounding bricks of levels 0 and 1 (corner indices) of the samples ia..ib
-// of a tile's pixel columns: positions are affine along a column and, at one
-// depth, across pixels, so the four corner columns at the two segment ends
-// bound every sample of the tile. A tap outside its brick takes a float
-// atomic, so the bound only decides where a tap is summed, never whether.
+// Bounding bricks of levels 0 and 1 (bordered cell coordinates) of the
+// samples ia..ib of a tile's pixel columns: positions are affine along a
+// column and, at one depth, across pixels, so the four corner columns at the
+// two segment ends bound every sample of the tile. A tap outside its brick
+// takes a float atomic, so the bound only decides where a tap is summed,
+// never whether.
 struct BatchBrick {
   int lo0[3], n0[3], lo1[3], n1[3];
   int size0, size;
@@ -264,14 +280,14 @@ __device__ __forceinline__ BatchBrick batch_brick(const float (*corner)[6], int 
   int s0 = 1, s1 = 1;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    int lo = clampi((int)floorf(mn[a] - 0.5f - 1e-3f), 0, d0[a] - 1);
-    int hi = clampi((int)floorf(mx[a] - 0.5f + 1e-3f) + 1, 0, d0[a] - 1);
+    int lo = clampi((int)floorf(mn[a] - 0.5f - 1e-3f) + 1, 0, d0[a] + 1);
+    int hi = clampi((int)floorf(mx[a] - 0.5f + 1e-3f) + 2, 0, d0[a] + 1);
     b.lo0[a] = uniform_i(lo);
     b.n0[a] = uniform_i(hi - lo + 1);
     s0 *= b.n0[a];
     if (mt.levels > 0) {
-      lo = clampi((int)floorf(mn[a] * 0.5f - 0.5f - 1e-3f), 0, d1[a] - 1);
-      hi = clampi((int)floorf(mx[a] * 0.5f - 0.5f + 1e-3f) + 1, 0, d1[a] - 1);
+      lo = clampi((int)floorf(mn[a] * 0.5f - 0.5f - 1e-3f) + 1, 0, d1[a] + 1);
+      hi = clampi((int)floorf(mx[a] * 0.5f - 0.5f + 1e-3f) + 2, 0, d1[a] + 1);
     } else {
       lo = 0;
       hi = -1;
@@ -285,15 +301,19 @@ __device__ __forceinline__ BatchBrick batch_brick(const float (*corner)[6], int 
   return b;
 }
 
-// gradient of one level tap (BORDER corners, weight products in the order
-// ((g w_x) w_y) w_z): into the level's LDS brick when the whole corner cube
-// lies in it, float atomics otherwise
+// gradient of one level tap (an inside sample's eight corners, weight
+// products in the order ((g w_x) w_y) w_z, corners outside the grid landing
+// in the border): into the level's LDS brick when the whole corner cube lies
+// in it, float atomics into the bordered gradient level otherwise
 __device__ __forceinline__ void level_scatter(double* __restrict__ lds, bool staged, const int blo[3],
                                               const int bn[3], float* __restrict__ gbuf, Dims3 d, float px, float py,
                                               float pz, float gw) {
-  const Taps t = make_taps<B_BORDER>(px, py, pz, d);
-  if (!t.inside) return;
-  const float wx[2] = {1.f - t.wx, t.wx}, wy[2] = {1.f - t.wy, t.wy}, wz[2] = {1.f - t.wz, t.wz};
+  const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+  if (!((0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) & (qz < (float)d.z)))
+    return;
+  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+  const float tx = px - fx, ty = py - fy, tz = pz - fz;
+  const float wx[2] = {1.f - tx, tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
   float w[2][2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -302,30 +322,30 @@ __device__ __forceinline__ void level_scatter(double* __restrict__ lds, bool sta
     for (int j = 0; j < 2; ++j) {
       const float gxy = gx * wy[j];
 #pragma unroll
-      for (int k = 0; k < 2; ++k) w[k][j][i] = (t.vx[i] & t.vy[j] & t.vz[k]) ? gxy * wz[k] : 0.f;
+      for (int k = 0; k < 2; ++k) w[k][j][i] = gxy * wz[k];
     }
   }
-  const int lx = t.ix[0] - blo[0], ly = t.iy[0] - blo[1], lz = t.iz[0] - blo[2];
-  const int hx = t.ix[1] - blo[0], hy = t.iy[1] - blo[1], hz = t.iz[1] - blo[2];
-  if (staged & (lx >= 0) & (ly >= 0) & (lz >= 0) & (hx < bn[0]) & (hy < bn[1]) & (hz < bn[2])) {
+  const int ix = (int)fx + 1, iy = (int)fy + 1, iz = (int)fz + 1;
+  const int lx = ix - blo[0], ly = iy - blo[1], lz = iz - blo[2];
+  if (staged & (lx >= 0) & (ly >= 0) & (lz >= 0) & (lx + 1 < bn[0]) & (ly + 1 < bn[1]) & (lz + 1 < bn[2])) {
+    const int sy = bn[0], sz = bn[0] * bn[1];
+    double* c = lds + (lz * bn[1] + ly) * bn[0] + lx;
 #pragma unroll
     for (int k = 0; k < 2; ++k)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int row = ((k ? hz : lz) * bn[1] + (j ? hy : ly)) * bn[0];
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int i = 0; i < 2; ++i) atomicAdd(&lds[row + (i ? hx : lx)], (double)w[k][j][i]);
-      }
+        for (int i = 0; i < 2; ++i) atomicAdd(c + k * sz + j * sy + i, (double)w[k][j][i]);
   } else {
+    const unsigned sy = (unsigned)(d.x + 2), sz = __umul24(sy, (unsigned)(d.y + 2));
+    float* c = gbuf + pad_index(ix, iy, iz, d);
 #pragma unroll
     for (int k = 0; k < 2; ++k)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const unsigned row = ((unsigned)t.iz[k] * (unsigned)d.y + (unsigned)t.iy[j]) * (unsigned)d.x;
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-          if (w[k][j][i] != 0.f) atomic_addf(gbuf + row + (unsigned)t.ix[i], w[k][j][i]);
-      }
+          if (w[k][j][i] != 0.f) atomic_addf(c + k * sz + j * sy + i, w[k][j][i]);
   }
 }
 
@@ -470,7 +490,7 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
             if (wm != 0.f) {
               const float sc = 1.f / (float)(1 << m);
               level_scatter(lds + cap0 + (m == 1 ? br.size0 : 0), use_lds & (m <= 1), m == 0 ? br.lo0 : br.lo1,
-                            m == 0 ? br.n0 : br.n1, level_grad(R, m), R.mt.dims[m], p.x * sc - 0.5f,
+                            m == 0 ? br.n0 : br.n1, R.gpyr + R.mt.offset[m], R.mt.dims[m], p.x * sc - 0.5f,
                             p.y * sc - 0.5f, p.z * sc - 0.5f, g * wm);
             }
           }
@@ -490,9 +510,11 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
             const int nx = l0 ? br.n0[0] : br.n1[0], ny = l0 ? br.n0[1] : br.n1[1];
             const int lx = j % nx, ly = (j / nx) % ny, lz = j / (nx * ny);
             if (l0)
-              atomic_addf(R.grad + flat3(br.lo0[0] + lx, br.lo0[1] + ly, br.lo0[2] + lz, R.mt.dims[0]), (float)val);
+              atomic_addf(R.gpyr + pad_index(br.lo0[0] + lx, br.lo0[1] + ly, br.lo0[2] + lz, R.mt.dims[0]),
+                          (float)val);
             else
-              atomic_addf(level_grad(R, 1) + flat3(br.lo1[0] + lx, br.lo1[1] + ly, br.lo1[2] + lz, R.mt.dims[1]),
+              atomic_addf(R.gpyr + R.mt.offset[1] +
+                              pad_index(br.lo1[0] + lx, br.lo1[1] + ly, br.lo1[2] + lz, R.mt.dims[1]),
                           (float)val);
             lds[cap0 + idx] = 0.0;
           }
@@ -505,13 +527,14 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
 }
 
 // ------------------------------------------------------------- finishing
-// Mip adjoint into the level-0 gradient (g0 += sum_m 8^-m g_m(ancestor)) and,
-// in each reconstruction's first item, its loss from the render partials.
+// The level-0 gradient with the mip adjoint (g0 + sum_m 8^-m g_m(ancestor))
+// and, in each reconstruction's first item, its loss from the render
+// partials.
 __global__ __launch_bounds__(256) void k_batch_finish(const BatchRec* __restrict__ recs, int nrec, int total_items) {
   __shared__ float red[256];
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, 0)];
-    const int local = item - R.lvl_item0[0];
+    const BatchRec& R = recs[rec_of(recs, nrec, item, kBatchPasses - 1)];
+    const int local = item - R.pass_item0[kBatchPasses - 1];
     if (local == 0) {
       const int n = R.num_views * R.tiles_x * R.tiles_y;
       float t = 0.f;
@@ -525,7 +548,6 @@ __global__ __launch_bounds__(256) void k_batch_finish(const BatchRec* __restrict
       if (threadIdx.x == 0) R.loss[0] = red[0] * R.loss_scale;
       __syncthreads();
     }
-    if (R.mt.levels == 0) continue;
     const Dims3 d = R.mt.dims[0];
     const size_t f = (size_t)local * 256 + threadIdx.x;
     if (f >= (size_t)d.x * d.y * d.z) continue;
@@ -535,15 +557,20 @@ __global__ __launch_bounds__(256) void k_batch_finish(const BatchRec* __restrict
       s *= 0.125f;
       const Dims3 o = R.mt.dims[m];
       const int px = x >> m, py = y >> m, pz = z >> m;
-      if (px < o.x && py < o.y && pz < o.z) g = fmaf(s, level_grad(R, m)[flat3(px, py, pz, o)], g);
+      if (px < o.x && py < o.y && pz < o.z)
+        g = fmaf(s, R.gpyr[R.mt.offset[m] + pad_index(px + 1, py + 1, pz + 1, o)], g);
     }
-    R.grad[f] += g;
+    R.grad[f] = R.gpyr[pad_index(x + 1, y + 1, z + 1, d)] + g;
   }
 }
 
 // ------------------------------------------------------------------ host
 static size_t align256b(size_t b) { return (b + 255) & ~size_t(255); }
 
+static size_t padded_cells(Dims3 d) { return (size_t)(d.x + 2) * (d.y + 2) * (d.z + 2); }
+
+// levels 0..L (L <= num_mipmaps, while every axis halves to at least one
+// cell), each bordered and starting on a 256-byte boundary
 static MipTable batch_mip_table(Dims3 d, int levels) {
   MipTable mt{};
   levels = std::min(levels, kBatchMaxLevels);
@@ -554,14 +581,12 @@ static MipTable batch_mip_table(Dims3 d, int levels) {
     }
   mt.levels = levels;
   mt.C = 1;
-  mt.dims[0] = d;
   mt.base_stride = (size_t)d.x * d.y * d.z;
-  // the pyramid: level 0 at offset 0, each level on a 256-byte boundary
-  size_t off = (mt.base_stride + 63) & ~size_t(63);
-  for (int m = 1; m <= levels; ++m) {
+  size_t off = 0;
+  for (int m = 0; m <= levels; ++m) {
     mt.dims[m] = Dims3{d.x >> m, d.y >> m, d.z >> m};
     mt.offset[m] = off;
-    off += ((size_t)mt.dims[m].x * mt.dims[m].y * mt.dims[m].z + 63) & ~size_t(63);
+    off += (padded_cells(mt.dims[m]) + 63) & ~size_t(63);
   }
   mt.atlas_stride = off;
   return mt;
@@ -570,12 +595,12 @@ static MipTable batch_mip_table(Dims3 d, int levels) {
 static int brick_tile(int brick) { return brick == 0 ? 16 : 8; }
 static int brick_seg(int brick) { return brick == 1 ? 4 : 8; }
 
-// workspace layout: the record table, then per reconstruction its density
-// pyramid (levels 0..L), the gradients of levels 1..L and the render
-// partials, then the render's c_k scratch
+// workspace layout: the record table, then per reconstruction its bordered
+// density pyramid, the gradients of the same cells and the render partials,
+// then the render's c_k scratch
 struct BatchLayout {
   size_t table, total;
-  std::vector<size_t> atlas, gatlas, partials;
+  std::vector<size_t> pyr, gpyr, partials;
   size_t csave;  // the render's per-workgroup c_k scratch (kRenderGrid x 256 x smax floats)
   int smax;      // the most depth samples of any reconstruction
 };
@@ -587,8 +612,8 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
     return fail(GFT_ERR_INVALID_ARGUMENT, "num_mipmaps must be in [0, %d]", kBatchMaxLevels);
   size_t off = align256b(sizeof(BatchRec) * (size_t)n);
   L.table = off;
-  L.atlas.resize(n);
-  L.gatlas.resize(n);
+  L.pyr.resize(n);
+  L.gpyr.resize(n);
   L.partials.resize(n);
   for (int r = 0; r < n; ++r) {
     const gft_batch_recon& c = rs[r];
@@ -600,12 +625,16 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
                   c.num_views);
     if (c.brick < 0 || c.brick > 2) return fail(GFT_ERR_INVALID_ARGUMENT, "reconstruction %d: brick must be 0, 1 or 2", r);
     const Dims3 d{c.grid_dhw[2], c.grid_dhw[1], c.grid_dhw[0]};
-    if ((int64_t)d.x * d.y * d.z >= ((int64_t)1 << 31)) return fail(GFT_ERR_INVALID_ARGUMENT, "grid too large");
+    // bordered cell indices are formed with 24-bit multiplies and stay 32-bit
+    if ((int64_t)(d.y + 2) * (d.z + 2) >= ((int64_t)1 << 24) || d.x + 2 >= (1 << 24) ||
+        (int64_t)(d.x + 2) * (d.y + 2) * (d.z + 2) >= ((int64_t)1 << 31))
+      return fail(GFT_ERR_INVALID_ARGUMENT,
+                  "reconstruction %d: grid too large for the batched step ((H+2)(D+2) < 2^24, cells < 2^31)", r);
     const MipTable mt = batch_mip_table(d, num_mipmaps);
-    L.atlas[r] = off;
+    L.pyr[r] = off;
     off += align256b(mt.atlas_stride * sizeof(float));
-    L.gatlas[r] = off;
-    off += align256b((mt.levels > 0 ? mt.atlas_stride - mt.offset[1] : 0) * sizeof(float));
+    L.gpyr[r] = off;
+    off += align256b(mt.atlas_stride * sizeof(float));
     const size_t items = (size_t)c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
     L.partials[r] = off;
     off += align256b(items * sizeof(float));
@@ -618,16 +647,17 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
   return GFT_OK;
 }
 
-// item counts of the launches: render items, then per level
-static void batch_items(int32_t n, const gft_batch_recon* rs, int32_t num_mipmaps, int& render, int level[4]) {
+// item counts of the launches: render items; the level passes (bordered
+// cells of each level) and the finishing pass (level-0 voxels)
+static void batch_items(int32_t n, const gft_batch_recon* rs, int32_t num_mipmaps, int& render, int pass[kBatchPasses]) {
   render = 0;
-  for (int m = 0; m <= kBatchMaxLevels; ++m) level[m] = 0;
+  for (int m = 0; m < kBatchPasses; ++m) pass[m] = 0;
   for (int r = 0; r < n; ++r) {
     const gft_batch_recon& c = rs[r];
     render += c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
     const MipTable mt = batch_mip_table(Dims3{c.grid_dhw[2], c.grid_dhw[1], c.grid_dhw[0]}, num_mipmaps);
-    for (int m = 0; m <= mt.levels; ++m)
-      level[m] += (int)(((size_t)mt.dims[m].x * mt.dims[m].y * mt.dims[m].z + 255) / 256);
+    for (int m = 0; m <= mt.levels; ++m) pass[m] += (int)((padded_cells(mt.dims[m]) + 255) / 256);
+    pass[kBatchPasses - 1] += (int)((mt.base_stride + 255) / 256);
   }
 }
 
@@ -655,7 +685,7 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
                 L.total);
   std::vector<BatchRec> recs(n);
   char* ws = (char*)workspace;
-  int item = 0, lvl_item[kBatchMaxLevels + 1] = {0, 0, 0, 0};
+  int item = 0, pass_item[kBatchPasses] = {};
   for (int r = 0; r < n; ++r) {
     const gft_batch_recon& c = recons[r];
     if (!c.density || !c.targets || !c.images || !c.density_grad || !c.loss || !c.M || !c.V || !c.P || !c.frustum)
@@ -665,9 +695,9 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
     const Dims3 d{c.grid_dhw[2], c.grid_dhw[1], c.grid_dhw[0]};
     R.mt = batch_mip_table(d, num_mipmaps);
     R.dens = c.density;
-    R.pyr = (float*)(ws + L.atlas[r]);
+    R.pyr = (float*)(ws + L.pyr[r]);
+    R.gpyr = (float*)(ws + L.gpyr[r]);
     R.grad = c.density_grad;
-    R.gatlas = (float*)(ws + L.gatlas[r]);
     R.target = c.targets;
     R.images = c.images;
     R.partials = (float*)(ws + L.partials[r]);
@@ -685,11 +715,12 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
     R.seg = brick_seg(c.brick);
     R.item0 = item;
     item += R.num_views * R.tiles_x * R.tiles_y;
-    for (int m = 0; m <= kBatchMaxLevels; ++m) {
-      R.lvl_item0[m] = lvl_item[m];
-      if (m <= R.mt.levels)
-        lvl_item[m] += (int)(((size_t)R.mt.dims[m].x * R.mt.dims[m].y * R.mt.dims[m].z + 255) / 256);
+    for (int m = 0; m < kBatchPasses; ++m) {
+      R.pass_item0[m] = pass_item[m];
+      if (m <= R.mt.levels) pass_item[m] += (int)((padded_cells(R.mt.dims[m]) + 255) / 256);
     }
+    R.pass_item0[kBatchPasses - 1] = pass_item[kBatchPasses - 1];
+    pass_item[kBatchPasses - 1] += (int)((R.mt.base_stride + 255) / 256);
     const Mat4d m = mat_from_colmajor(c.M);
     for (int v = 0; v < c.num_views; ++v) {
       const Mat4d mv = mat_mul(mat_from_colmajor(c.V + 16 * v), m), p = mat_from_colmajor(c.P + 16 * v);
@@ -724,13 +755,13 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
                 L.total);
   hipStream_t s = (hipStream_t)stream;
   const BatchRec* recs = (const BatchRec*)workspace;
-  int render = 0, level[kBatchMaxLevels + 1];
-  batch_items(n, recons, num_mipmaps, render, level);
+  int render = 0, pass[kBatchPasses];
+  batch_items(n, recons, num_mipmaps, render, pass);
   {
     KernelTimer t(s, "batch_levels");
     for (int m = 0; m <= kBatchMaxLevels; ++m)
-      if (level[m])
-        hipLaunchKernelGGL(k_batch_level, dim3(persistent_grid(level[m])), dim3(256), 0, s, recs, n, m, level[m]);
+      if (pass[m])
+        hipLaunchKernelGGL(k_batch_level, dim3(persistent_grid(pass[m])), dim3(256), 0, s, recs, n, m, pass[m]);
   }
   if ((rc = check_hip(hipGetLastError(), "batched mip levels"))) return rc;
   {
@@ -741,7 +772,8 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
   if ((rc = check_hip(hipGetLastError(), "batched render"))) return rc;
   {
     KernelTimer t(s, "batch_finish");
-    hipLaunchKernelGGL(k_batch_finish, dim3(persistent_grid(level[0])), dim3(256), 0, s, recs, n, level[0]);
+    hipLaunchKernelGGL(k_batch_finish, dim3(persistent_grid(pass[kBatchPasses - 1])), dim3(256), 0, s, recs, n,
+                       pass[kBatchPasses - 1]);
   }
   return check_hip(hipGetLastError(), "batched finish");
 }
