@@ -19,16 +19,16 @@
 //   k_batch_level(m)   level m of the zero-bordered pyramid (m = 0: the
 //                      density; m >= 1: kGenerateMip3D's pairwise means of
 //                      level m-1) and a cleared gradient level;
-//   k_batch_render     forward and backward of every ray in one pass: items
-//                      are (reconstruction, view, 16x16-pixel tile); a ray,
-//                      clipped to the grid box, marches its samples keeping
-//                      each sample's c_k, writes its pixel and loss partial,
-//                      forms its loss gradient and scatters the sample
-//                      gradients alpha c_k + beta through the level taps;
-//                      level-0 and level-1 taps go to FP64 LDS bricks per tile
-//                      (or 8x8 sub-tile) and segment of at most K samples,
-//                      flushed with float atomics; coarser levels (rare) take
-//                      float atomics directly;
+//   k_batch_fwd        items (reconstruction, view, 16x16-pixel tile): a
+//                      ray, clipped to the grid box, marches its samples
+//                      keeping each sample's c_k, writes its pixel, its loss
+//                      term and its gradient record;
+//   k_batch_bwd        the same items: every sample's gradient alpha c_k +
+//                      beta scattered through its level taps; level-0 and
+//                      level-1 taps go to FP64 LDS bricks per tile (or 8x8
+//                      sub-tile) and segment of at most K samples, flushed
+//                      with float atomics; coarser levels (rare) take float
+//                      atomics directly;
 //   k_batch_finish     the density gradient: every voxel's level-0 gradient
 //                      plus 1/8^m of its level-m ancestor's (the mip
 //                      adjoint); in the first item of each reconstruction,
@@ -40,37 +40,47 @@
 // 8x8 tiles with 8 (about 4x4x8). A segment whose bricks would not fit the
 // LDS is halved until they do.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <vector>
 
 #include "gft_host.hpp"
+#include "quad.hpp"
 #include "sampler_v1.hpp"
 
 namespace gft {
 
 constexpr int kBatchMaxViews = 8;
 constexpr int kBatchMaxLevels = 3;
-constexpr int kBatchPasses = kBatchMaxLevels + 2;  // level passes 0..3, then the finishing pass
+constexpr int kBatchLevels = kBatchMaxLevels + 1;
+// item tables: level passes 0..3, corner-quad passes 0..3, the finishing pass
+constexpr int kBatchPasses = 2 * kBatchLevels + 1;
+constexpr int kFinishPass = kBatchPasses - 1;
 constexpr int kBatchCap = 6144;  // FP64 brick cells per workgroup (48 KiB)
-constexpr int kRenderGrid = 768;  // render workgroups: as many as are resident (three per CU, by LDS)
 
-// global-memory float (loads through it are global, not flat, instructions)
-typedef __attribute__((address_space(1))) const float gcfloat;
+// four floats in global memory (loads through it are global, not flat,
+// 16-byte instructions)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const f32x4 gcf32x4;
 
 // Every pyramid level is stored with a one-cell zero border, (W+2)(H+2)(D+2)
-// cells: a BORDER trilinear read of an inside sample is then eight unmasked
-// loads (a corner outside the grid reads the border's zero, as BORDER
-// defines), and the backward's taps of such corners land in border cells
-// that nothing reads.
+// cells, from which the next level is averaged and the level's corner-quad
+// copy (the forward's taps: two 16-byte loads per level) is taken. The
+// gradient levels have the same border: the backward's taps of corners
+// outside the grid land there and nothing reads them.
 struct BatchRec {
   MipTable mt;            // dims: levels 0..L without the border; offset[m]: level m in pyr and gpyr
   const float* dens;
   float* pyr;             // bordered density pyramid, levels 0..L
   float* gpyr;            // bordered gradients of the same cells
+  float4* quad[kBatchLevels];   // corner-quad copy of each level
+  QuadDims qd[kBatchLevels];
   float* grad;            // level-0 gradient output
   const float* target;    // [V,H,W,3]
   float* images;          // [V,H,W,4]
   float* partials;        // per render item
+  float* records;         // per render item, 256 float4 pixel records (lo, hi, alpha, beta)
+  float* csave;           // per render item, out_d x 256 sample values c_k
   float* loss;            // [1]
   float light[3];
   float step;
@@ -80,7 +90,7 @@ struct BatchRec {
   int tile;               // brick tile: 16 (one brick per workgroup) or 8 (one per wave)
   int seg;                // longest backward segment in samples
   int item0;              // first render item
-  int pass_item0[kBatchPasses];  // first item of the level passes and of the finishing pass
+  int pass_item0[kBatchPasses];  // first item of the level, quad and finishing passes
   SampleView view[kBatchMaxViews];
 };
 
@@ -147,6 +157,33 @@ __global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict_
   }
 }
 
+// Corner-quad copy of level m (the layout of the fused render's copy), taken
+// from the bordered level: entry (x, y, z) holds v(x..x+1, y..y+1, z), the
+// border supplying the zeros of corners outside the grid.
+__global__ __launch_bounds__(256) void k_batch_quad(const BatchRec* __restrict__ recs, int nrec, int m,
+                                                    int total_items) {
+  using B = QuadBrick<float4>;
+  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
+    const BatchRec& R = recs[rec_of(recs, nrec, item, kBatchLevels + m)];
+    if (m > R.mt.levels) continue;
+    const QuadDims q = R.qd[m];
+    const size_t j = (size_t)(item - R.pass_item0[kBatchLevels + m]) * 256 + threadIdx.x;
+    if (j >= quad_entries(q)) continue;
+    const unsigned b = (unsigned)(j / B::N), o = (unsigned)(j % B::N);
+    const unsigned ix = (b % q.nbx) * B::X + (o & (B::X - 1));
+    const unsigned iy = ((b / q.nbx) % q.nby) * B::Y + ((o >> B::Xl) & (B::Y - 1));
+    const unsigned iz = (b / (q.nbx * q.nby)) * B::Z + (o >> (B::Xl + B::Yl));
+    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);  // padding entries of partial bricks: never read
+    if (ix < q.qx && iy < q.qy && iz < q.qz) {
+      const Dims3 d = R.mt.dims[m];
+      const float* c = R.pyr + R.mt.offset[m] + pad_index((int)ix, (int)iy, (int)iz, d);
+      const unsigned sy = (unsigned)(d.x + 2);
+      e = make_float4(c[0], c[1], c[sy], c[sy + 1]);
+    }
+    R.quad[m][j] = e;
+  }
+}
+
 // ---------------------------------------------------------------- render
 // One pixel column of the TRANSFORM_LINDEPTH resample. LINDEPTH spaces the
 // samples uniformly in view depth and a pixel's unprojection is affine in
@@ -201,22 +238,21 @@ __device__ __forceinline__ float level_weight(int m, int nl, const int lv[2], co
   return lv[0] == m ? lw[0] : (((nl == 2) & (lv[1] == m)) ? lw[1] : 0.f);
 }
 
-// BORDER trilinear value at bordered level `buf` of dims d, (px, py, pz) in
+// BORDER trilinear value from a level's corner-quad copy, (px, py, pz) in
 // cell coordinates (p / 2^m - 0.5): make_taps' inside test, corners and lerp
-// order, as read_linear (a corner outside the grid reads the border's zero,
-// the value read_linear puts there)
-__device__ __forceinline__ float level_read(const gcfloat* __restrict__ buf, Dims3 d, float px, float py, float pz) {
+// order, as read_linear (an inside sample's corners are entries x, y in
+// [-1, dims-1] and slices z, z+1 in [-1, dims])
+__device__ __forceinline__ float quad_read(const gcf32x4* __restrict__ quad, const QuadDims& qd, Dims3 d, float px,
+                                           float py, float pz) {
   const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
   if (!((0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) & (qz < (float)d.z)))
     return 0.f;
   const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
   const float tx = px - fx, ty = py - fy, tz = pz - fz;
-  const unsigned sy = (unsigned)(d.x + 2), sz = __umul24(sy, (unsigned)(d.y + 2));
-  const gcfloat* c = buf + pad_index((int)fx + 1, (int)fy + 1, (int)fz + 1, d);
-  const float c000 = c[0], c001 = c[1], c010 = c[sy], c011 = c[sy + 1];
-  const float c100 = c[sz], c101 = c[sz + 1], c110 = c[sz + sy], c111 = c[sz + sy + 1];
-  const float v0 = lerpf(lerpf(c000, c001, tx), lerpf(c010, c011, tx), ty);
-  const float v1 = lerpf(lerpf(c100, c101, tx), lerpf(c110, c111, tx), ty);
+  const unsigned ix = (unsigned)((int)fx + 1), iy = (unsigned)((int)fy + 1), iz = (unsigned)((int)fz + 1);
+  const f32x4 c0 = quad[quad_index(qd, ix, iy, iz)], c1 = quad[quad_index(qd, ix, iy, iz + 1)];
+  const float v0 = lerpf(lerpf(c0.x, c0.y, tx), lerpf(c0.z, c0.w, tx), ty);
+  const float v1 = lerpf(lerpf(c1.x, c1.y, tx), lerpf(c1.z, c1.w, tx), ty);
   return lerpf(v0, v1, tz);
 }
 
@@ -233,8 +269,8 @@ __device__ __forceinline__ float batch_sample(const BatchRec& R, const SampleCfg
     const bool f = lv[0] == m, c = (nl == 2) & (lv[1] == m);
     if (f | c) {
       const float sc = 1.f / (float)(1 << m);
-      const float a = level_read((const gcfloat*)R.pyr + R.mt.offset[m], R.mt.dims[m], p.x * sc - 0.5f,
-                                 p.y * sc - 0.5f, p.z * sc - 0.5f);
+      const float a = quad_read((const gcf32x4*)R.quad[m], R.qd[m], R.mt.dims[m], p.x * sc - 0.5f,
+                                p.y * sc - 0.5f, p.z * sc - 0.5f);
       if (f) vf = a;
       else vc = a;
     }
@@ -254,6 +290,15 @@ struct BatchBrick {
 };
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// j / n for 0 <= j, n < 2^20 with rn = 1 / n in float: the float quotient
+// is within 2^20 * 2^-23 = 0.125 of j / n, so truncation is the quotient or
+// one off, and one correction step makes it exact
+__device__ __forceinline__ int div_small(int j, int n, float rn) {
+  const int q = (int)((float)j * rn);
+  const int r = j - q * n;
+  return q + (r >= n) - (r < 0);
+}
 
 // corner columns of a tile in LDS: [c][0..2] = p0, [c][3..5] = dp; a 16-pixel
 // brick uses corners 0..3 of a 2x2 table, the 8-pixel brick s of a 4x4 table
@@ -344,26 +389,87 @@ __device__ __forceinline__ void level_scatter(double* __restrict__ lds, bool sta
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-          if (w[k][j][i] != 0.f) atomic_addf(c + k * sz + j * sy + i, w[k][j][i]);
+        for (int i = 0; i < 2; ++i) atomic_addf(c + k * sz + j * sy + i, w[k][j][i]);
   }
 }
 
-// The forward keeps c_k = exp(-D_k) + E_{k-1} of every sample of its column
-// (D_k the optical depth through sample k, E_{k-1} the emission before it):
-// with E the pixel's total, dE/dsigma_k = exp(-D_k) - sum_{i>=k} exp(-D_i)
-// sigma_i = c_k - E, so the backward needs no second march. The values live
-// in a per-workgroup scratch of smax samples x 256 columns (sample-major, so
-// a wave's 64 columns are one coalesced run), reused by every item of the
-// workgroup and read back soon after they are written.
-__global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict__ recs, int nrec, int total_items,
-                                                      float* __restrict__ csave, int smax) {
-  __shared__ double lds[kBatchCap];
+// ---- forward
+// One thread per pixel of a 16x16 tile: the clipped column's BEER_LAMBERT
+// march, the image, the pixel's loss term and gradient, and for the backward
+// c_k = exp(-D_k) + E_{k-1} of every sample it marched (D_k the optical depth
+// through sample k, E_{k-1} the emission before it): with E the pixel's
+// total, dE/dsigma_k = exp(-D_k) - sum_{i>=k} exp(-D_i) sigma_i = c_k - E,
+// so the gradient of sample k is alpha c_k + beta with alpha = step G and
+// beta = -step G E, kept in the pixel's record (lo, hi, alpha, beta). c_k is
+// stored item-major, [item][k][256 pixels] (a wave's 64 pixels in one run).
+__global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ recs, int nrec, int total_items) {
   __shared__ float red[256];
+  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
+    const BatchRec& R = recs[rec_of(recs, nrec, item, -1)];
+    const int local = item - R.item0;
+    const int per_view = R.tiles_x * R.tiles_y;
+    const int vi = local / per_view, t = local % per_view;
+    const int x = (t % R.tiles_x) * 16 + (threadIdx.x & 15), y = (t / R.tiles_x) * 16 + (threadIdx.x >> 4);
+    const SampleView& V = R.view[vi];
+    const bool valid = x < R.out_w && y < R.out_h;
+    const int S = R.out_d;
+    const float3 oinv = make_float3(1.f / (float)R.out_w, 1.f / (float)R.out_h, 1.f / (float)S);
+    const bool mips = R.mt.levels > 0;
+    const SampleCfg cfg{F_LINEAR, mips ? MIP_LINEAR : MIP_NONE, R.mt.levels, 0.f, 0.5f};
+    float D = 0.f, E = 0.f, sq = 0.f;
+    float4 rec = make_float4(__int_as_float(1), __int_as_float(0), 0.f, 0.f);
+    if (valid) {
+      const Column col = lindepth_column(V, x + 0.5f, y + 0.5f, S, oinv, mips);
+      // samples outside the box [0, dims) read zero at every mip level (a
+      // level-m sample is inside only where 0 <= p < dims_m 2^m <= dims), so
+      // they add nothing to the blend or the gradient: the column is clipped
+      int lo, hi;
+      clip_ray(col.p0, col.dp, R.mt.dims[0], S, 0.f, lo, hi);
+      float* __restrict__ cs = R.csave + (size_t)local * S * 256 + threadIdx.x;
+      for (int i = lo; i <= hi; ++i) {
+        const float fi = (float)i;
+        const float lod = mips ? column_lod(col, fi) : 0.f;
+        const float sigma = R.step * batch_sample(R, cfg, fma3(fi, col.dp, col.p0), lod);
+        D += sigma;
+        const float u = __expf(-D);
+        cs[(size_t)i * 256] = u + E;
+        E = fmaf(u, sigma, E);
+      }
+      const size_t pix = ((size_t)vi * R.out_h + y) * R.out_w + x;
+      reinterpret_cast<float4*>(R.images)[pix] = make_float4(R.light[0] * E, R.light[1] * E, R.light[2] * E, D);
+      const float* tg = R.target + pix * 3;
+      const float e0 = R.light[0] * E - tg[0], e1 = R.light[1] * E - tg[1], e2 = R.light[2] * E - tg[2];
+      sq = e0 * e0 + e1 * e1 + e2 * e2;
+      const float G = R.loss_scale * 2.f * (R.light[0] * e0 + R.light[1] * e1 + R.light[2] * e2);
+      if (G != 0.f && lo <= hi)
+        rec = make_float4(__int_as_float(lo), __int_as_float(hi), R.step * G, -(R.step * G) * E);
+    }
+    reinterpret_cast<float4*>(R.records)[(size_t)local * 256 + threadIdx.x] = rec;
+    // per-item loss partial: a fixed-order tree over the tile's 256 pixels
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) R.partials[local] = red[0];
+    __syncthreads();  // red is rewritten by the next item
+  }
+}
+
+// ---- backward
+// Per item the tile's pixels (16x16 bricks: the tile in rows; 8x8 bricks:
+// each wave a quadrant), in segments of at most R.seg samples, each as long
+// as its bricks fit the LDS (halved until they do, down to one sample; in the
+// 8-pixel mode until every wave's brick fits, so all waves take the same
+// segments). Each sample's gradient alpha c_k + beta is scattered through
+// its level taps.
+
+__global__ __launch_bounds__(256) void k_batch_bwd(const BatchRec* __restrict__ recs, int nrec, int total_items) {
+  __shared__ double lds[kBatchCap];
   __shared__ int span[2];
   __shared__ float corner[16][6];
   const int wave = uniform_i((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
-  float* __restrict__ cs = csave + (size_t)blockIdx.x * smax * 256 + threadIdx.x;
   for (int i = threadIdx.x; i < kBatchCap; i += 256) lds[i] = 0.0;
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
     const BatchRec& R = recs[rec_of(recs, nrec, item, -1)];
@@ -372,7 +478,6 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
     const int vi = local / per_view, t = local % per_view;
     const int tx0 = (t % R.tiles_x) * 16, ty0 = (t / R.tiles_x) * 16;
     const SampleView& V = R.view[vi];
-    // this thread's pixel: 16x16 bricks take the tile in rows, 8x8 bricks give each wave a quadrant
     int x, y, cap, cap0;
     if (R.tile == 16) {
       x = tx0 + (threadIdx.x & 15);
@@ -385,44 +490,16 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
       cap = kBatchCap / 4;
       cap0 = wave * cap;
     }
-    const bool valid = x < R.out_w && y < R.out_h;
+    const int slot = (y - ty0) * 16 + (x - tx0);  // the forward's thread of this pixel
+    const float4 rec = reinterpret_cast<const float4*>(R.records)[(size_t)local * 256 + slot];
+    const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
+    const float alpha = rec.z, beta = rec.w;
     const int S = R.out_d;
     const float3 oinv = make_float3(1.f / (float)R.out_w, 1.f / (float)R.out_h, 1.f / (float)S);
     const bool mips = R.mt.levels > 0;
     const SampleCfg cfg{F_LINEAR, mips ? MIP_LINEAR : MIP_NONE, R.mt.levels, 0.f, 0.5f};
-    const Column col = lindepth_column(V, x + 0.5f, y + 0.5f, S, oinv, mips);
-    // samples outside the box [0, dims) read zero at every mip level (a level-m
-    // sample is inside only where 0 <= p < dims_m 2^m <= dims), so they add
-    // nothing to the blend or the gradient: the column is clipped to the box
-    int lo = 1, hi = 0;
-    if (valid) clip_ray(col.p0, col.dp, R.mt.dims[0], S, 0.f, lo, hi);
-    // ---- forward: BEER_LAMBERT blend of the sampled frustum column
-    float D = 0.f, E = 0.f;
-    for (int i = lo; i <= hi; ++i) {
-      const float fi = (float)i;
-      const float lod = mips ? column_lod(col, fi) : 0.f;
-      const float sigma = R.step * batch_sample(R, cfg, fma3(fi, col.dp, col.p0), lod);
-      D += sigma;
-      const float u = __expf(-D);
-      cs[(size_t)i * 256] = u + E;
-      E = fmaf(u, sigma, E);
-    }
-    float G = 0.f, sq = 0.f;
-    if (valid) {
-      const size_t pix = ((size_t)vi * R.out_h + y) * R.out_w + x;
-      reinterpret_cast<float4*>(R.images)[pix] =
-          make_float4(R.light[0] * E, R.light[1] * E, R.light[2] * E, D);
-      const float* tg = R.target + pix * 3;
-      const float e0 = R.light[0] * E - tg[0], e1 = R.light[1] * E - tg[1], e2 = R.light[2] * E - tg[2];
-      sq = e0 * e0 + e1 * e1 + e2 * e2;
-      G = R.loss_scale * 2.f * (R.light[0] * e0 + R.light[1] * e1 + R.light[2] * e2);
-    }
-    const bool walk = valid && G != 0.f && lo <= hi;
-    const float alpha = R.step * G, beta = -(R.step * G) * E;
-    // per-item loss partial: a fixed-order tree over the tile's 256 pixel
-    // slots, the same sum whatever the brick's thread-to-pixel mapping; the
-    // union of the walking columns' sample ranges; the bricks' corner columns
-    red[(y - ty0) * 16 + (x - tx0)] = sq;
+    Column col{};
+    if (lo <= hi) col = lindepth_column(V, x + 0.5f, y + 0.5f, S, oinv, mips);
     if (threadIdx.x == 0) {
       span[0] = S;
       span[1] = -1;
@@ -443,22 +520,13 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
       cr[5] = cc.dp.z;
     }
     __syncthreads();
-    if (walk) {
+    if (lo <= hi) {
       atomicMin(&span[0], lo);
       atomicMax(&span[1], hi);
     }
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) R.partials[local] = red[0];
+    __syncthreads();
     const int s_lo = span[0], s_hi = span[1];
-    __syncthreads();  // span and red are rewritten by the next item
-    // ---- backward: segments of at most R.seg samples, each as long as its
-    // bricks fit the LDS (halved until they do, down to one sample; in the
-    // 8-pixel mode until every wave's brick fits, so all waves take the same
-    // segments), the sample gradients c_k -> alpha c_k + beta scattered
-    // through the level taps
+    const float* __restrict__ cs = R.csave + (size_t)local * S * 256 + slot;
     const int sub = R.tile == 16 ? 0 : wave;
     for (int ia = s_lo; ia <= s_hi;) {
       int k = R.seg, ib;
@@ -474,12 +542,17 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
         k >>= 1;
       }
       const bool use_lds = br.size <= cap;
-      if (walk) {
-        for (int i = max(ia, lo); i <= min(ib, hi); ++i) {
+      const int i0 = max(ia, lo), i1 = min(ib, hi);
+      if (i0 <= i1) {
+        // c_k of the next sample is loaded while this one is scattered
+        float cnext = cs[(size_t)i0 * 256];
+        for (int i = i0; i <= i1; ++i) {
+          const float c = cnext;
+          if (i < i1) cnext = cs[(size_t)(i + 1) * 256];
           const float fi = (float)i;
           const float3 p = fma3(fi, col.dp, col.p0);
           const float lod = mips ? column_lod(col, fi) : 0.f;
-          const float g = fmaf(alpha, cs[(size_t)i * 256], beta);
+          const float g = fmaf(alpha, c, beta);
           int lv[2];
           float lw[2];
           const int nl = level_taps(cfg, lod, lv, lw);
@@ -502,13 +575,16 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
       if (use_lds) {
         // flush the bricks (all 256 threads for a 16-pixel tile, the wave's 64 lanes for an 8-pixel one)
         const int nthr = R.tile == 16 ? 256 : 64, tid = R.tile == 16 ? (int)threadIdx.x : lane;
+        const float rx0 = 1.f / (float)br.n0[0], rxy0 = 1.f / (float)(br.n0[0] * br.n0[1]);
+        const float rx1 = 1.f / (float)br.n1[0], rxy1 = 1.f / (float)(br.n1[0] * br.n1[1]);
         for (int idx = tid; idx < br.size; idx += nthr) {
           const double val = lds[cap0 + idx];
           if (val != 0.0) {
             const bool l0 = idx < br.size0;
             const int j = l0 ? idx : idx - br.size0;
-            const int nx = l0 ? br.n0[0] : br.n1[0], ny = l0 ? br.n0[1] : br.n1[1];
-            const int lx = j % nx, ly = (j / nx) % ny, lz = j / (nx * ny);
+            const int nx = l0 ? br.n0[0] : br.n1[0], nxy = nx * (l0 ? br.n0[1] : br.n1[1]);
+            const int lz = div_small(j, nxy, l0 ? rxy0 : rxy1), rz = j - lz * nxy;
+            const int ly = div_small(rz, nx, l0 ? rx0 : rx1), lx = rz - ly * nx;
             if (l0)
               atomic_addf(R.gpyr + pad_index(br.lo0[0] + lx, br.lo0[1] + ly, br.lo0[2] + lz, R.mt.dims[0]),
                           (float)val);
@@ -533,8 +609,8 @@ __global__ __launch_bounds__(256) void k_batch_render(const BatchRec* __restrict
 __global__ __launch_bounds__(256) void k_batch_finish(const BatchRec* __restrict__ recs, int nrec, int total_items) {
   __shared__ float red[256];
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, kBatchPasses - 1)];
-    const int local = item - R.pass_item0[kBatchPasses - 1];
+    const BatchRec& R = recs[rec_of(recs, nrec, item, kFinishPass)];
+    const int local = item - R.pass_item0[kFinishPass];
     if (local == 0) {
       const int n = R.num_views * R.tiles_x * R.tiles_y;
       float t = 0.f;
@@ -596,14 +672,25 @@ static int brick_tile(int brick) { return brick == 0 ? 16 : 8; }
 static int brick_seg(int brick) { return brick == 1 ? 4 : 8; }
 
 // workspace layout: the record table, then per reconstruction its bordered
-// density pyramid, the gradients of the same cells and the render partials,
-// then the render's c_k scratch
+// density pyramid, the gradients of the same cells, the corner-quad copy of
+// each level, the render partials, the
+// pixel records and the sample values c_k (items x out_d x 256 floats: the
+// largest part, 4 bytes per frustum sample)
 struct BatchLayout {
   size_t table, total;
-  std::vector<size_t> pyr, gpyr, partials;
-  size_t csave;  // the render's per-workgroup c_k scratch (kRenderGrid x 256 x smax floats)
-  int smax;      // the most depth samples of any reconstruction
+  std::vector<size_t> pyr, gpyr, partials, records, csave;
+  std::vector<std::array<size_t, kBatchLevels>> quad;
 };
+
+// items (of 256 threads) of pass p for a reconstruction with pyramid mt
+static int pass_items(const MipTable& mt, int p) {
+  if (p < kBatchLevels) return p <= mt.levels ? (int)((padded_cells(mt.dims[p]) + 255) / 256) : 0;
+  if (p < 2 * kBatchLevels) {
+    const int m = p - kBatchLevels;
+    return m <= mt.levels ? (int)((quad_entries(make_quad_dims(mt.dims[m])) + 255) / 256) : 0;
+  }
+  return (int)((mt.base_stride + 255) / 256);
+}
 
 static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmaps, BatchLayout& L) {
   if (n < 1) return fail(GFT_ERR_INVALID_ARGUMENT, "the batch needs at least one reconstruction");
@@ -614,7 +701,10 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
   L.table = off;
   L.pyr.resize(n);
   L.gpyr.resize(n);
+  L.records.resize(n);
+  L.csave.resize(n);
   L.partials.resize(n);
+  L.quad.resize(n);
   for (int r = 0; r < n; ++r) {
     const gft_batch_recon& c = rs[r];
     for (int a = 0; a < 3; ++a)
@@ -627,7 +717,7 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
     const Dims3 d{c.grid_dhw[2], c.grid_dhw[1], c.grid_dhw[0]};
     // bordered cell indices are formed with 24-bit multiplies and stay 32-bit
     if ((int64_t)(d.y + 2) * (d.z + 2) >= ((int64_t)1 << 24) || d.x + 2 >= (1 << 24) ||
-        (int64_t)(d.x + 2) * (d.y + 2) * (d.z + 2) >= ((int64_t)1 << 31))
+        (int64_t)(d.x + 4) * (d.y + 2) * (d.z + 3) >= ((int64_t)1 << 31))
       return fail(GFT_ERR_INVALID_ARGUMENT,
                   "reconstruction %d: grid too large for the batched step ((H+2)(D+2) < 2^24, cells < 2^31)", r);
     const MipTable mt = batch_mip_table(d, num_mipmaps);
@@ -635,29 +725,33 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
     off += align256b(mt.atlas_stride * sizeof(float));
     L.gpyr[r] = off;
     off += align256b(mt.atlas_stride * sizeof(float));
+    for (int m = 0; m <= mt.levels; ++m) {
+      L.quad[r][m] = off;
+      off += align256b(quad_entries(make_quad_dims(mt.dims[m])) * sizeof(float4));
+    }
     const size_t items = (size_t)c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
     L.partials[r] = off;
     off += align256b(items * sizeof(float));
+    L.records[r] = off;
+    off += align256b(items * 256 * 4 * sizeof(float));
+    L.csave[r] = off;
+    off += align256b(items * (size_t)c.output_shape[0] * 256 * sizeof(float));
   }
-  L.smax = 1;
-  for (int r = 0; r < n; ++r) L.smax = std::max(L.smax, (int)rs[r].output_shape[0]);
-  L.csave = off;
-  off += align256b((size_t)kRenderGrid * 256 * (size_t)L.smax * sizeof(float));
   L.total = off;
   return GFT_OK;
 }
 
 // item counts of the launches: render items; the level passes (bordered
-// cells of each level) and the finishing pass (level-0 voxels)
+// cells of each level), the quad passes and the finishing pass (level-0
+// voxels)
 static void batch_items(int32_t n, const gft_batch_recon* rs, int32_t num_mipmaps, int& render, int pass[kBatchPasses]) {
   render = 0;
-  for (int m = 0; m < kBatchPasses; ++m) pass[m] = 0;
+  for (int p = 0; p < kBatchPasses; ++p) pass[p] = 0;
   for (int r = 0; r < n; ++r) {
     const gft_batch_recon& c = rs[r];
     render += c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
     const MipTable mt = batch_mip_table(Dims3{c.grid_dhw[2], c.grid_dhw[1], c.grid_dhw[0]}, num_mipmaps);
-    for (int m = 0; m <= mt.levels; ++m) pass[m] += (int)((padded_cells(mt.dims[m]) + 255) / 256);
-    pass[kBatchPasses - 1] += (int)((mt.base_stride + 255) / 256);
+    for (int p = 0; p < kBatchPasses; ++p) pass[p] += pass_items(mt, p);
   }
 }
 
@@ -701,6 +795,8 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
     R.target = c.targets;
     R.images = c.images;
     R.partials = (float*)(ws + L.partials[r]);
+    R.records = (float*)(ws + L.records[r]);
+    R.csave = (float*)(ws + L.csave[r]);
     R.loss = c.loss;
     for (int k = 0; k < 3; ++k) R.light[k] = c.light[k];
     R.step = c.depth_step;
@@ -715,12 +811,14 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
     R.seg = brick_seg(c.brick);
     R.item0 = item;
     item += R.num_views * R.tiles_x * R.tiles_y;
-    for (int m = 0; m < kBatchPasses; ++m) {
-      R.pass_item0[m] = pass_item[m];
-      if (m <= R.mt.levels) pass_item[m] += (int)((padded_cells(R.mt.dims[m]) + 255) / 256);
+    for (int p = 0; p < kBatchPasses; ++p) {
+      R.pass_item0[p] = pass_item[p];
+      pass_item[p] += pass_items(R.mt, p);
     }
-    R.pass_item0[kBatchPasses - 1] = pass_item[kBatchPasses - 1];
-    pass_item[kBatchPasses - 1] += (int)((R.mt.base_stride + 255) / 256);
+    for (int m = 0; m <= R.mt.levels; ++m) {
+      R.quad[m] = (float4*)(ws + L.quad[r][m]);
+      R.qd[m] = make_quad_dims(R.mt.dims[m]);
+    }
     const Mat4d m = mat_from_colmajor(c.M);
     for (int v = 0; v < c.num_views; ++v) {
       const Mat4d mv = mat_mul(mat_from_colmajor(c.V + 16 * v), m), p = mat_from_colmajor(c.P + 16 * v);
@@ -762,18 +860,26 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
     for (int m = 0; m <= kBatchMaxLevels; ++m)
       if (pass[m])
         hipLaunchKernelGGL(k_batch_level, dim3(persistent_grid(pass[m])), dim3(256), 0, s, recs, n, m, pass[m]);
+    for (int m = 0; m <= kBatchMaxLevels; ++m)
+      if (pass[kBatchLevels + m])
+        hipLaunchKernelGGL(k_batch_quad, dim3(persistent_grid(pass[kBatchLevels + m])), dim3(256), 0, s, recs, n, m,
+                           pass[kBatchLevels + m]);
   }
   if ((rc = check_hip(hipGetLastError(), "batched mip levels"))) return rc;
   {
-    KernelTimer t(s, "batch_render");
-    hipLaunchKernelGGL(k_batch_render, dim3((unsigned)std::max(1, std::min(render, kRenderGrid))), dim3(256), 0, s,
-                       recs, n, render, (float*)((char*)workspace + L.csave), L.smax);
+    KernelTimer t(s, "batch_fwd");
+    hipLaunchKernelGGL(k_batch_fwd, dim3(persistent_grid(render)), dim3(256), 0, s, recs, n, render);
   }
-  if ((rc = check_hip(hipGetLastError(), "batched render"))) return rc;
+  if ((rc = check_hip(hipGetLastError(), "batched render forward"))) return rc;
+  {
+    KernelTimer t(s, "batch_bwd");
+    hipLaunchKernelGGL(k_batch_bwd, dim3(persistent_grid(render)), dim3(256), 0, s, recs, n, render);
+  }
+  if ((rc = check_hip(hipGetLastError(), "batched render backward"))) return rc;
   {
     KernelTimer t(s, "batch_finish");
-    hipLaunchKernelGGL(k_batch_finish, dim3(persistent_grid(pass[kBatchPasses - 1])), dim3(256), 0, s, recs, n,
-                       pass[kBatchPasses - 1]);
+    hipLaunchKernelGGL(k_batch_finish, dim3(persistent_grid(pass[kFinishPass])), dim3(256), 0, s, recs, n,
+                       pass[kFinishPass]);
   }
   return check_hip(hipGetLastError(), "batched finish");
 }
