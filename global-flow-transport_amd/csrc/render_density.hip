@@ -781,7 +781,17 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_clamp(const float4* 
         const float gds = rp.step * (e * G + (gD - esg));
         gD -= esg;
         if (BM == B_BORDER && !inr[u]) continue;  // outside the grid: no gradient
-        const float wx[2] = {1.f - tp.wx, tp.wx}, wy[2] = {1.f - tp.wy, tp.wy}, wz[2] = {1.f - tp.wz, tp.wz};
+        float wx[2] = {1.f - tp.wx, tp.wx}, wy[2] = {1.f - tp.wy, tp.wy}, wz[2] = {1.f - tp.wz, tp.wz};
+        if (BM == B_CLAMP) {
+          // an axis whose two corners clamp to one cell (the sample lies outside
+          // the grid, or in the last half cell, along it) gives that cell both
+          // weights: one tap of weight 1 instead of two. Most of a CLAMP ray's
+          // samples lie outside the grid and would otherwise spend 8 atomics on
+          // 4, 2 or 1 distinct cells.
+          if (tp.ix[0] == tp.ix[1]) { wx[0] = 1.f; wx[1] = 0.f; }
+          if (tp.iy[0] == tp.iy[1]) { wy[0] = 1.f; wy[1] = 0.f; }
+          if (tp.iz[0] == tp.iz[1]) { wz[0] = 1.f; wz[1] = 0.f; }
+        }
 #pragma unroll
         for (int k = 0; k < 2; ++k)
 #pragma unroll
