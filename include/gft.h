@@ -477,14 +477,18 @@ int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params
  * exact gradient w.r.t. the density through blend, resample and mips. Replaces
  * the per-(batch, camera) launch loops of SampleGridKernelLauncher
  * (resample_grid.cu.cc:372-436) and their gradients, for all reconstructions
- * at once: a level pass per mip level, one launch that runs the forward and
- * the backward of every ray of every view of every reconstruction, and one
- * that applies the mip adjoint and sums the losses. The backward recomputes
- * each ray's transmittance walking back from far to near (nothing is saved
- * per sample) and scatters through an FP64 LDS brick per tile and segment;
- * `brick` picks the tile and segment per reconstruction: 0 = 16x16-pixel tiles
- * with 8-sample segments, 1 = 8x8 with 4, 2 = 8x8 with 8. Float atomics: the
- * gradient's last bits vary between runs.
+ * at once: per mip level a pass building the zero-bordered level and one
+ * taking its corner-quad copy, one forward launch over every pixel of every
+ * view of every reconstruction (rays clipped to the grid box; it keeps each
+ * sample's c_k = exp(-D_k) + E_{k-1} and each pixel's gradient record in the
+ * workspace), one backward launch scattering every sample's gradient through
+ * FP64 LDS bricks per tile and segment, and one that applies the mip adjoint
+ * and sums the losses. `brick` picks the tile and the longest segment per
+ * reconstruction: 0 = 16x16-pixel tiles with up to 8 samples, 1 = 8x8 with 4,
+ * 2 = 8x8 with 8 (a segment whose bricks exceed the LDS is halved). The
+ * workspace holds 4 bytes per frustum sample besides the pyramids. Float
+ * atomics: the gradient's last bits vary between runs; images and losses
+ * repeat bit for bit. Grids with (H+2)(D+2) >= 2^24 are rejected.
  * gft_batch_render_prepare writes the reconstruction table into the workspace
  * (a set-up call that waits for its copy); gft_batch_render_run only launches
  * kernels (capturable in a HIP graph) and reads the host array for launch sizes.
