@@ -18,27 +18,26 @@
 // Launches (each one over every reconstruction, grid-stride over its items):
 //   k_batch_level(m)   level m of the zero-bordered pyramid (m = 0: the
 //                      density; m >= 1: kGenerateMip3D's pairwise means of
-//                      level m-1) and a cleared gradient level;
+//                      level m-1);
+//   k_batch_quad(m)    the corner-quad copy of level m;
 //   k_batch_fwd        items (reconstruction, view, 16x16-pixel tile): a
 //                      ray, clipped to the grid box, marches its samples
 //                      keeping each sample's c_k, writes its pixel, its loss
-//                      term and its gradient record;
-//   k_batch_bwd        the same items: every sample's gradient alpha c_k +
-//                      beta scattered through its level taps; level-0 and
-//                      level-1 taps go to FP64 LDS bricks per tile (or 8x8
-//                      sub-tile) and segment of at most K samples, flushed
-//                      with float atomics; coarser levels (rare) take float
-//                      atomics directly;
+//                      term, its gradient record and the view's largest LoD;
+//   k_batch_gather(m)  one wave per block of level-m voxels: for every view
+//                      whose samples read level m, the candidate pixels of the
+//                      block's support box, each column cut by the box, every
+//                      sample's gradient alpha c_k + beta times its level-m
+//                      weight and its trilinear weights summed in FP64 LDS,
+//                      then one store per voxel (no atomics in global memory:
+//                      the gradient is the same bit for bit on every run);
 //   k_batch_finish     the density gradient: every voxel's level-0 gradient
 //                      plus 1/8^m of its level-m ancestor's (the mip
 //                      adjoint); in the first item of each reconstruction,
 //                      its loss: the per-tile partials summed in a fixed
 //                      order.
-// The brick of the backward (tile size x longest segment) is chosen per
-// reconstruction: 16x16-pixel tiles with 8-sample segments (about 8^3 cells
-// at two pixels per cell), 8x8 tiles with 4-sample segments (about 4^3), or
-// 8x8 tiles with 8 (about 4x4x8). A segment whose bricks would not fit the
-// LDS is halved until they do.
+// The gather's block shape (16x16x2, 8x8x8 or 16x8x4 voxels) is chosen per
+// reconstruction.
 #include <algorithm>
 #include <array>
 #include <cstring>
@@ -53,26 +52,39 @@ namespace gft {
 constexpr int kBatchMaxViews = 8;
 constexpr int kBatchMaxLevels = 3;
 constexpr int kBatchLevels = kBatchMaxLevels + 1;
-// item tables: level passes 0..3, corner-quad passes 0..3, the finishing pass
-constexpr int kBatchPasses = 2 * kBatchLevels + 1;
-constexpr int kFinishPass = kBatchPasses - 1;
-constexpr int kBatchCap = 6144;  // FP64 brick cells per workgroup (48 KiB)
+// item tables: level passes, corner-quad passes, gather passes (one per
+// level each), the finishing pass
+constexpr int kQuadPass = kBatchLevels;
+constexpr int kGatherPass = 2 * kBatchLevels;
+constexpr int kFinishPass = 3 * kBatchLevels;
+constexpr int kBatchPasses = kFinishPass + 1;
+constexpr int kBlockVox = 512;  // voxels of a gather block (every shape)
 
 // four floats in global memory (loads through it are global, not flat,
 // 16-byte instructions)
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const f32x4 gcf32x4;
 
+// One view of a reconstruction. TRANSFORM_LINDEPTH samples pixel (x, y) at
+// depth index q in object space at H.xyz / H.w, H = C (nx, ny, nz(q), 1) with
+// C the inverse model-view times the inverse projection (the model-view is
+// affine, so its product with the un-divided point commutes with the
+// perspective divide) and nx = (x + 0.5) 2 / W - 1: at the first and last
+// depth index H is n0 + x gx + y gy and f0 + x gx + y gy. pmv is the
+// object -> clip projection used to find a voxel block's pixels.
+struct ColumnView {
+  float n0[4], f0[4], gx[4], gy[4];
+  float pmv[16];
+};
+
 // Every pyramid level is stored with a one-cell zero border, (W+2)(H+2)(D+2)
 // cells, from which the next level is averaged and the level's corner-quad
-// copy (the forward's taps: two 16-byte loads per level) is taken. The
-// gradient levels have the same border: the backward's taps of corners
-// outside the grid land there and nothing reads them.
+// copy (the forward's taps: two 16-byte loads per level) is taken.
 struct BatchRec {
   MipTable mt;            // dims: levels 0..L without the border; offset[m]: level m in pyr and gpyr
   const float* dens;
   float* pyr;             // bordered density pyramid, levels 0..L
-  float* gpyr;            // bordered gradients of the same cells
+  float* gpyr;            // bordered gradients of the same cells (the gather writes the interior)
   float4* quad[kBatchLevels];   // corner-quad copy of each level
   QuadDims qd[kBatchLevels];
   float* grad;            // level-0 gradient output
@@ -81,20 +93,22 @@ struct BatchRec {
   float* partials;        // per render item
   float* records;         // per render item, 256 float4 pixel records (lo, hi, alpha, beta)
   float* csave;           // per render item, out_d x 256 sample values c_k
+  int* lodmax;            // per view, the largest LoD of its samples (float bits)
   float* loss;            // [1]
   float light[3];
   float step;
   float loss_scale;
   int num_views, out_w, out_h, out_d;
   int tiles_x, tiles_y;   // 16x16-pixel tiles per view
-  int tile;               // brick tile: 16 (one brick per workgroup) or 8 (one per wave)
-  int seg;                // longest backward segment in samples
+  int bw, bh, bd;         // gather block (voxels)
   int item0;              // first render item
-  int pass_item0[kBatchPasses];  // first item of the level, quad and finishing passes
-  SampleView view[kBatchMaxViews];
+  int pass_item0[kBatchPasses];  // first item of the level, quad, gather and finishing passes
+  ColumnView view[kBatchMaxViews];
 };
 
-// index of the reconstruction owning global item g of a pass (records sorted by first item)
+// index of the reconstruction owning global item g of a pass (records sorted
+// by first item; one without items in the pass shares its successor's first
+// item and is never chosen)
 __device__ __forceinline__ int rec_of(const BatchRec* __restrict__ recs, int nrec, int g, int pass) {
   int lo = 0, hi = nrec - 1;
   while (lo < hi) {
@@ -115,8 +129,9 @@ __device__ __forceinline__ unsigned pad_index(int x, int y, int z, Dims3 d) {
 
 // ------------------------------------------------------------ level passes
 // Over the bordered cells of level m (border cells get zero): m = 0 copies
-// the density; m >= 1 takes pairwise means of level m-1 in z, then y, then x
-// (kGenerateMip3D's order). Both clear the level's gradient.
+// the density (and clears the views' LoD maxima); m >= 1 takes pairwise means
+// of level m-1 in z, then y, then x (kGenerateMip3D's order). Both clear the
+// level's gradient.
 __global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict__ recs, int nrec, int m,
                                                      int total_items) {
 #pragma clang fp contract(off)
@@ -127,6 +142,7 @@ __global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict_
     const int px = o.x + 2, py = o.y + 2;
     const size_t n = (size_t)px * py * (o.z + 2);
     const size_t f = (size_t)(item - R.pass_item0[m]) * 256 + threadIdx.x;
+    if (m == 0 && f < (size_t)R.num_views) R.lodmax[f] = 0;
     if (f >= n) continue;
     const int x = (int)(f % (size_t)px) - 1, y = (int)((f / (size_t)px) % (size_t)py) - 1,
               z = (int)(f / ((size_t)px * py)) - 1;
@@ -164,10 +180,10 @@ __global__ __launch_bounds__(256) void k_batch_quad(const BatchRec* __restrict__
                                                     int total_items) {
   using B = QuadBrick<float4>;
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, kBatchLevels + m)];
+    const BatchRec& R = recs[rec_of(recs, nrec, item, kQuadPass + m)];
     if (m > R.mt.levels) continue;
     const QuadDims q = R.qd[m];
-    const size_t j = (size_t)(item - R.pass_item0[kBatchLevels + m]) * 256 + threadIdx.x;
+    const size_t j = (size_t)(item - R.pass_item0[kQuadPass + m]) * 256 + threadIdx.x;
     if (j >= quad_entries(q)) continue;
     const unsigned b = (unsigned)(j / B::N), o = (unsigned)(j % B::N);
     const unsigned ix = (b % q.nbx) * B::X + (o & (B::X - 1));
@@ -184,13 +200,11 @@ __global__ __launch_bounds__(256) void k_batch_quad(const BatchRec* __restrict__
   }
 }
 
-// ---------------------------------------------------------------- render
-// One pixel column of the TRANSFORM_LINDEPTH resample. LINDEPTH spaces the
-// samples uniformly in view depth and a pixel's unprojection is affine in
-// view depth, so the positions are affine in the depth index: p(i) = p0 +
-// i dp, from the two end samples of the column. The LoD footprint's
-// differences to the +x and +y pixels are affine in i as well (e0 + i e1)
-// and the difference to sample i + 1 is dp.
+// ----------------------------------------------------------------- columns
+// A pixel's column: positions p(i) = p0 + i dp from its first and last
+// sample, and the LoD footprint's differences to the +x and +y pixels,
+// e0 + i e1 (the +z difference is dp). The forward and the gather evaluate a
+// column with this one function, so they see the same positions and LoDs.
 struct Column {
   float3 p0, dp, ex0, ex1, ey0, ey1;
 };
@@ -201,24 +215,27 @@ __device__ __forceinline__ float3 fma3(float s, float3 a, float3 b) {
   return make_float3(fmaf(s, a.x, b.x), fmaf(s, a.y, b.y), fmaf(s, a.z, b.z));
 }
 
-__device__ __forceinline__ Column lindepth_column(const SampleView& v, float qx, float qy, int S, float3 oinv,
-                                                  bool lod) {
-  const float3 z3 = make_float3(0.f, 0.f, 0.f);
-  const float qa = 0.5f, qb = (float)S - 0.5f, inv = S > 1 ? 1.f / (float)(S - 1) : 0.f;
+__device__ __forceinline__ float3 column_point(const float* b, const ColumnView& v, float x, float y) {
+  const float h0 = fmaf(y, v.gy[0], fmaf(x, v.gx[0], b[0])), h1 = fmaf(y, v.gy[1], fmaf(x, v.gx[1], b[1]));
+  const float h2 = fmaf(y, v.gy[2], fmaf(x, v.gx[2], b[2])), h3 = fmaf(y, v.gy[3], fmaf(x, v.gx[3], b[3]));
+  const float iw = __builtin_amdgcn_rcpf(h3);
+  return make_float3(h0 * iw, h1 * iw, h2 * iw);
+}
+
+__device__ __forceinline__ Column column_of(const ColumnView& v, int x, int y, float inv_s1, bool lod) {
+  const float fx = (float)x, fy = (float)y;
   Column c;
-  const float3 pa = map_pos(v, CM_LINDEPTH, qx, qy, qa, oinv, z3), pb = map_pos(v, CM_LINDEPTH, qx, qy, qb, oinv, z3);
+  const float3 pa = column_point(v.n0, v, fx, fy), pb = column_point(v.f0, v, fx, fy);
   c.p0 = pa;
-  c.dp = scale3(sub3(pb, pa), inv);
-  c.ex0 = c.ex1 = c.ey0 = c.ey1 = z3;
+  c.dp = scale3(sub3(pb, pa), inv_s1);
+  c.ex0 = c.ex1 = c.ey0 = c.ey1 = make_float3(0.f, 0.f, 0.f);
   if (lod) {
-    const float3 xa = sub3(pa, map_pos(v, CM_LINDEPTH, qx + 1.f, qy, qa, oinv, z3));
-    const float3 xb = sub3(pb, map_pos(v, CM_LINDEPTH, qx + 1.f, qy, qb, oinv, z3));
-    const float3 ya = sub3(pa, map_pos(v, CM_LINDEPTH, qx, qy + 1.f, qa, oinv, z3));
-    const float3 yb = sub3(pb, map_pos(v, CM_LINDEPTH, qx, qy + 1.f, qb, oinv, z3));
+    const float3 xa = sub3(pa, column_point(v.n0, v, fx + 1.f, fy)), xb = sub3(pb, column_point(v.f0, v, fx + 1.f, fy));
+    const float3 ya = sub3(pa, column_point(v.n0, v, fx, fy + 1.f)), yb = sub3(pb, column_point(v.f0, v, fx, fy + 1.f));
     c.ex0 = xa;
-    c.ex1 = scale3(sub3(xb, xa), inv);
+    c.ex1 = scale3(sub3(xb, xa), inv_s1);
     c.ey0 = ya;
-    c.ey1 = scale3(sub3(yb, ya), inv);
+    c.ey1 = scale3(sub3(yb, ya), inv_s1);
   }
   return c;
 }
@@ -278,122 +295,7 @@ __device__ __forceinline__ float batch_sample(const BatchRec& R, const SampleCfg
   return nl == 2 ? lerpf(vf, vc, lw[1]) : vf;
 }
 
-// Bounding bricks of levels 0 and 1 (bordered cell coordinates) of the
-// samples ia..ib of a tile's pixel columns: positions are affine along a
-// column and, at one depth, across pixels, so the four corner columns at the
-// two segment ends bound every sample of the tile. A tap outside its brick
-// takes a float atomic, so the bound only decides where a tap is summed,
-// never whether.
-struct BatchBrick {
-  int lo0[3], n0[3], lo1[3], n1[3];
-  int size0, size;
-};
-
-__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// j / n for 0 <= j, n < 2^20 with rn = 1 / n in float: the float quotient
-// is within 2^20 * 2^-23 = 0.125 of j / n, so truncation is the quotient or
-// one off, and one correction step makes it exact
-__device__ __forceinline__ int div_small(int j, int n, float rn) {
-  const int q = (int)((float)j * rn);
-  const int r = j - q * n;
-  return q + (r >= n) - (r < 0);
-}
-
-// corner columns of a tile in LDS: [c][0..2] = p0, [c][3..5] = dp; a 16-pixel
-// brick uses corners 0..3 of a 2x2 table, the 8-pixel brick s of a 4x4 table
-// (pixel offsets 0, 7, 8, 15) the corners of its quadrant
-__device__ __forceinline__ BatchBrick batch_brick(const float (*corner)[6], int tile, int sub, int ia, int ib,
-                                                  const MipTable& mt) {
-  float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int cx = c & 1, cy = (c >> 1) & 1;
-    const int k = tile == 16 ? cy * 2 + cx : ((sub >> 1) * 2 + cy) * 4 + (sub & 1) * 2 + cx;
-    const float fi = (float)(c < 4 ? ia : ib);
-    const float q[3] = {fmaf(fi, corner[k][3], corner[k][0]), fmaf(fi, corner[k][4], corner[k][1]),
-                        fmaf(fi, corner[k][5], corner[k][2])};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      mn[a] = fminf(mn[a], q[a]);
-      mx[a] = fmaxf(mx[a], q[a]);
-    }
-  }
-  const int d0[3] = {mt.dims[0].x, mt.dims[0].y, mt.dims[0].z};
-  const int d1[3] = {mt.dims[1].x, mt.dims[1].y, mt.dims[1].z};
-  BatchBrick b;
-  int s0 = 1, s1 = 1;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int lo = clampi((int)floorf(mn[a] - 0.5f - 1e-3f) + 1, 0, d0[a] + 1);
-    int hi = clampi((int)floorf(mx[a] - 0.5f + 1e-3f) + 2, 0, d0[a] + 1);
-    b.lo0[a] = uniform_i(lo);
-    b.n0[a] = uniform_i(hi - lo + 1);
-    s0 *= b.n0[a];
-    if (mt.levels > 0) {
-      lo = clampi((int)floorf(mn[a] * 0.5f - 0.5f - 1e-3f) + 1, 0, d1[a] + 1);
-      hi = clampi((int)floorf(mx[a] * 0.5f - 0.5f + 1e-3f) + 2, 0, d1[a] + 1);
-    } else {
-      lo = 0;
-      hi = -1;
-    }
-    b.lo1[a] = uniform_i(lo);
-    b.n1[a] = uniform_i(hi - lo + 1);
-    s1 *= b.n1[a];
-  }
-  b.size0 = s0;
-  b.size = s0 + s1;
-  return b;
-}
-
-// gradient of one level tap (an inside sample's eight corners, weight
-// products in the order ((g w_x) w_y) w_z, corners outside the grid landing
-// in the border): into the level's LDS brick when the whole corner cube lies
-// in it, float atomics into the bordered gradient level otherwise
-__device__ __forceinline__ void level_scatter(double* __restrict__ lds, bool staged, const int blo[3],
-                                              const int bn[3], float* __restrict__ gbuf, Dims3 d, float px, float py,
-                                              float pz, float gw) {
-  const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
-  if (!((0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) & (qz < (float)d.z)))
-    return;
-  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
-  const float tx = px - fx, ty = py - fy, tz = pz - fz;
-  const float wx[2] = {1.f - tx, tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
-  float w[2][2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const float gx = gw * wx[i];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const float gxy = gx * wy[j];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) w[k][j][i] = gxy * wz[k];
-    }
-  }
-  const int ix = (int)fx + 1, iy = (int)fy + 1, iz = (int)fz + 1;
-  const int lx = ix - blo[0], ly = iy - blo[1], lz = iz - blo[2];
-  if (staged & (lx >= 0) & (ly >= 0) & (lz >= 0) & (lx + 1 < bn[0]) & (ly + 1 < bn[1]) & (lz + 1 < bn[2])) {
-    const int sy = bn[0], sz = bn[0] * bn[1];
-    double* c = lds + (lz * bn[1] + ly) * bn[0] + lx;
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) atomicAdd(c + k * sz + j * sy + i, (double)w[k][j][i]);
-  } else {
-    const unsigned sy = (unsigned)(d.x + 2), sz = __umul24(sy, (unsigned)(d.y + 2));
-    float* c = gbuf + pad_index(ix, iy, iz, d);
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) atomic_addf(c + k * sz + j * sy + i, w[k][j][i]);
-  }
-}
-
-// ---- forward
+// ------------------------------------------------------------------ forward
 // One thread per pixel of a 16x16 tile: the clipped column's BEER_LAMBERT
 // march, the image, the pixel's loss term and gradient, and for the backward
 // c_k = exp(-D_k) + E_{k-1} of every sample it marched (D_k the optical depth
@@ -402,6 +304,9 @@ __device__ __forceinline__ void level_scatter(double* __restrict__ lds, bool sta
 // so the gradient of sample k is alpha c_k + beta with alpha = step G and
 // beta = -step G E, kept in the pixel's record (lo, hi, alpha, beta). c_k is
 // stored item-major, [item][k][256 pixels] (a wave's 64 pixels in one run).
+// The view's largest LoD (the LoD is convex along a column, so a column's
+// largest is at one of its ends) lets the gather skip the levels no sample
+// of a view reads.
 __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ recs, int nrec, int total_items) {
   __shared__ float red[256];
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
@@ -410,16 +315,14 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
     const int per_view = R.tiles_x * R.tiles_y;
     const int vi = local / per_view, t = local % per_view;
     const int x = (t % R.tiles_x) * 16 + (threadIdx.x & 15), y = (t / R.tiles_x) * 16 + (threadIdx.x >> 4);
-    const SampleView& V = R.view[vi];
     const bool valid = x < R.out_w && y < R.out_h;
     const int S = R.out_d;
-    const float3 oinv = make_float3(1.f / (float)R.out_w, 1.f / (float)R.out_h, 1.f / (float)S);
     const bool mips = R.mt.levels > 0;
     const SampleCfg cfg{F_LINEAR, mips ? MIP_LINEAR : MIP_NONE, R.mt.levels, 0.f, 0.5f};
-    float D = 0.f, E = 0.f, sq = 0.f;
+    float D = 0.f, E = 0.f, sq = 0.f, lmax = 0.f;
     float4 rec = make_float4(__int_as_float(1), __int_as_float(0), 0.f, 0.f);
     if (valid) {
-      const Column col = lindepth_column(V, x + 0.5f, y + 0.5f, S, oinv, mips);
+      const Column col = column_of(R.view[vi], x, y, S > 1 ? 1.f / (float)(S - 1) : 0.f, mips);
       // samples outside the box [0, dims) read zero at every mip level (a
       // level-m sample is inside only where 0 <= p < dims_m 2^m <= dims), so
       // they add nothing to the blend or the gradient: the column is clipped
@@ -435,6 +338,7 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
         cs[(size_t)i * 256] = u + E;
         E = fmaf(u, sigma, E);
       }
+      if (mips && lo <= hi) lmax = fmaxf(column_lod(col, (float)lo), column_lod(col, (float)hi));
       const size_t pix = ((size_t)vi * R.out_h + y) * R.out_w + x;
       reinterpret_cast<float4*>(R.images)[pix] = make_float4(R.light[0] * E, R.light[1] * E, R.light[2] * E, D);
       const float* tg = R.target + pix * 3;
@@ -445,6 +349,8 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
         rec = make_float4(__int_as_float(lo), __int_as_float(hi), R.step * G, -(R.step * G) * E);
     }
     reinterpret_cast<float4*>(R.records)[(size_t)local * 256 + threadIdx.x] = rec;
+    for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off, 64));
+    if ((threadIdx.x & 63) == 0 && lmax > 0.f) atomicMax(R.lodmax + vi, __float_as_int(lmax));
     // per-item loss partial: a fixed-order tree over the tile's 256 pixels
     red[threadIdx.x] = sq;
     __syncthreads();
@@ -457,148 +363,212 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
   }
 }
 
-// ---- backward
-// Per item the tile's pixels (16x16 bricks: the tile in rows; 8x8 bricks:
-// each wave a quadrant), in segments of at most R.seg samples, each as long
-// as its bricks fit the LDS (halved until they do, down to one sample; in the
-// 8-pixel mode until every wave's brick fits, so all waves take the same
-// segments). Each sample's gradient alpha c_k + beta is scattered through
-// its level taps.
+// ------------------------------------------------------------------- gather
+// The transpose of the resample's trilinear scatter: one wave per voxel block,
+// accumulated in wave-private FP64 LDS regions. PAIR: levels 0 and 1 at once,
+// a block of bw x bh x bd level-0 voxels and the (bw/2) x (bh/2) x (bd/2)
+// level-1 cells over it (the level-1 cells' support box contains the level-0
+// voxels', so one walk over the candidate samples serves both); otherwise
+// level m alone (m >= 2, for views whose samples reach it). For each view the
+// wave projects the support box (level-m cell coordinates (x0-1, x0+bw) ...,
+// i.e. object positions ((x0 - 0.5) 2^m, (x0 + bw + 0.5) 2^m) ...), takes the
+// pixels whose columns can cross it (lanes: consecutive pixels in row order,
+// the next pixel's record loaded while one is walked), cuts each column by
+// the box, and adds every sample's gradient (alpha c_k + beta) times its
+// level weight to the block's corners (weight prod_a max(0, 1 - |q_a -
+// c_a|)); samples outside a level (BORDER) add nothing to it. The sums run in
+// a fixed order and one store per voxel follows.
+constexpr int kGatherBatch = 4;  // c_k loaded per round trip
 
-__global__ __launch_bounds__(256) void k_batch_bwd(const BatchRec* __restrict__ recs, int nrec, int total_items) {
-  __shared__ double lds[kBatchCap];
-  __shared__ int span[2];
-  __shared__ float corner[16][6];
-  const int wave = uniform_i((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
-  for (int i = threadIdx.x; i < kBatchCap; i += 256) lds[i] = 0.0;
-  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, -1)];
-    const int local = item - R.item0;
-    const int per_view = R.tiles_x * R.tiles_y;
-    const int vi = local / per_view, t = local % per_view;
-    const int tx0 = (t % R.tiles_x) * 16, ty0 = (t / R.tiles_x) * 16;
-    const SampleView& V = R.view[vi];
-    int x, y, cap, cap0;
-    if (R.tile == 16) {
-      x = tx0 + (threadIdx.x & 15);
-      y = ty0 + (threadIdx.x >> 4);
-      cap = kBatchCap;
-      cap0 = 0;
-    } else {
-      x = tx0 + (wave & 1) * 8 + (lane & 7);
-      y = ty0 + (wave >> 1) * 8 + (lane >> 3);
-      cap = kBatchCap / 4;
-      cap0 = wave * cap;
+// add a sample's level tap to the block region acc (block dims W, origin o,
+// (qx, qy, qz) in the level's cell coordinates, g the weighted gradient)
+__device__ __forceinline__ void gather_tap(__attribute__((address_space(3))) double* acc, int WX, int WY, int WZ,
+                                           int ox, int oy, int oz, float qx, float qy, float qz, float g) {
+  const float rx = qx - (float)ox, ry = qy - (float)oy, rz = qz - (float)oz;
+  const float flx = floorf(rx), fly = floorf(ry), flz = floorf(rz);
+  const float tx = rx - flx, ty = ry - fly, tz = rz - flz;
+  const int lx = (int)flx, ly = (int)fly, lz = (int)flz;
+  const float wx[2] = {g * (1.f - tx), g * tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
+#pragma unroll
+  for (int kz = 0; kz < 2; ++kz) {
+    const int cz = lz + kz;
+    if ((unsigned)cz >= (unsigned)WZ) continue;
+#pragma unroll
+    for (int ky = 0; ky < 2; ++ky) {
+      const int cy = ly + ky;
+      if ((unsigned)cy >= (unsigned)WY) continue;
+      const float wyz = wy[ky] * wz[kz];
+#pragma unroll
+      for (int kx = 0; kx < 2; ++kx) {
+        const int cx = lx + kx;
+        if ((unsigned)cx >= (unsigned)WX) continue;
+        __hip_atomic_fetch_add(acc + (cz * WY + cy) * WX + cx, (double)(wx[kx] * wyz), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
     }
-    const int slot = (y - ty0) * 16 + (x - tx0);  // the forward's thread of this pixel
-    const float4 rec = reinterpret_cast<const float4*>(R.records)[(size_t)local * 256 + slot];
-    const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
-    const float alpha = rec.z, beta = rec.w;
+  }
+}
+
+__device__ __forceinline__ bool inside_level(float qx, float qy, float qz, Dims3 d) {
+  return (0.f <= qx + 0.5f) & (qx + 0.5f < (float)d.x) & (0.f <= qy + 0.5f) & (qy + 0.5f < (float)d.y) &
+         (0.f <= qz + 0.5f) & (qz + 0.5f < (float)d.z);
+}
+
+template <bool PAIR>
+__global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict__ recs, int nrec, int m,
+                                                      int total_items) {
+  constexpr int kVox = PAIR ? kBlockVox + kBlockVox / 8 : kBlockVox;
+  __shared__ double region[4][kVox];
+  typedef __attribute__((address_space(3))) double lds_double;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  lds_double* acc = (lds_double*)region[wave];
+  lds_double* acc1 = acc + kBlockVox;  // PAIR: the level-1 cells
+  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
+    const BatchRec& R = recs[rec_of(recs, nrec, item, kGatherPass + m)];
+    const Dims3 dm = R.mt.dims[m];
+    const bool two = PAIR && R.mt.levels > 0;
+    const Dims3 d1 = R.mt.dims[two ? 1 : 0];
+    const int WX = R.bw, WY = R.bh, WZ = R.bd;
+    const int nbx = (dm.x + WX - 1) / WX, nby = (dm.y + WY - 1) / WY, nbz = (dm.z + WZ - 1) / WZ;
+    const int blk = (item - R.pass_item0[kGatherPass + m]) * 4 + wave;
+    const bool live = m <= R.mt.levels && blk < nbx * nby * nbz;  // other waves only join the barriers
+    const int x0 = (blk % nbx) * WX, y0 = ((blk / nbx) % nby) * WY, z0 = (blk / (nbx * nby)) * WZ;
+    for (int k = lane; k < kVox; k += 64) acc[k] = 0.0;
+    // support box (object space) of the block's cells; PAIR: of its level-1 cells
+    const int mb = two ? 1 : m;
+    const float s2 = (float)(1 << mb), sc = 1.f / (float)(1 << m);
+    const int bx = two ? x0 / 2 : x0, by = two ? y0 / 2 : y0, bz = two ? z0 / 2 : z0;
+    const int bwx = two ? WX / 2 : WX, bwy = two ? WY / 2 : WY, bwz = two ? WZ / 2 : WZ;
+    const float b0[3] = {((float)bx - 0.5f) * s2, ((float)by - 0.5f) * s2, ((float)bz - 0.5f) * s2};
+    const float b1[3] = {((float)(bx + bwx) + 0.5f) * s2, ((float)(by + bwy) + 0.5f) * s2,
+                         ((float)(bz + bwz) + 0.5f) * s2};
     const int S = R.out_d;
-    const float3 oinv = make_float3(1.f / (float)R.out_w, 1.f / (float)R.out_h, 1.f / (float)S);
+    const float inv_s1 = S > 1 ? 1.f / (float)(S - 1) : 0.f;
     const bool mips = R.mt.levels > 0;
     const SampleCfg cfg{F_LINEAR, mips ? MIP_LINEAR : MIP_NONE, R.mt.levels, 0.f, 0.5f};
-    Column col{};
-    if (lo <= hi) col = lindepth_column(V, x + 0.5f, y + 0.5f, S, oinv, mips);
-    if (threadIdx.x == 0) {
-      span[0] = S;
-      span[1] = -1;
-    }
-    if (threadIdx.x < (R.tile == 16 ? 4 : 16)) {
-      const int ncx = R.tile == 16 ? 2 : 4;
-      const int ox = (int)threadIdx.x % ncx, oy = (int)threadIdx.x / ncx;
-      const int offs[4] = {0, 7, 8, 15};
-      const int dx = R.tile == 16 ? 15 * ox : offs[ox], dy = R.tile == 16 ? 15 * oy : offs[oy];
-      const Column cc = lindepth_column(V, min(tx0 + dx, R.out_w - 1) + 0.5f, min(ty0 + dy, R.out_h - 1) + 0.5f, S,
-                                        oinv, false);
-      float* cr = corner[threadIdx.x];
-      cr[0] = cc.p0.x;
-      cr[1] = cc.p0.y;
-      cr[2] = cc.p0.z;
-      cr[3] = cc.dp.x;
-      cr[4] = cc.dp.y;
-      cr[5] = cc.dp.z;
-    }
-    __syncthreads();
-    if (lo <= hi) {
-      atomicMin(&span[0], lo);
-      atomicMax(&span[1], hi);
-    }
-    __syncthreads();
-    const int s_lo = span[0], s_hi = span[1];
-    const float* __restrict__ cs = R.csave + (size_t)local * S * 256 + slot;
-    const int sub = R.tile == 16 ? 0 : wave;
-    for (int ia = s_lo; ia <= s_hi;) {
-      int k = R.seg, ib;
-      BatchBrick br;
-      for (;;) {
-        ib = min(ia + k - 1, s_hi);
-        br = batch_brick(corner, R.tile, sub, ia, ib, R.mt);
-        int need = br.size;
-        if (R.tile == 8)
-          for (int s2 = 0; s2 < 4; ++s2)
-            if (s2 != sub) need = max(need, batch_brick(corner, 8, s2, ia, ib, R.mt).size);
-        if (need <= cap || k == 1) break;
-        k >>= 1;
-      }
-      const bool use_lds = br.size <= cap;
-      const int i0 = max(ia, lo), i1 = min(ib, hi);
-      if (i0 <= i1) {
-        // c_k of the next sample is loaded while this one is scattered
-        float cnext = cs[(size_t)i0 * 256];
-        for (int i = i0; i <= i1; ++i) {
-          const float c = cnext;
-          if (i < i1) cnext = cs[(size_t)(i + 1) * 256];
-          const float fi = (float)i;
-          const float3 p = fma3(fi, col.dp, col.p0);
-          const float lod = mips ? column_lod(col, fi) : 0.f;
-          const float g = fmaf(alpha, c, beta);
-          int lv[2];
-          float lw[2];
-          const int nl = level_taps(cfg, lod, lv, lw);
+    const int per_view = R.tiles_x * R.tiles_y;
+    const float4* __restrict__ recv = reinterpret_cast<const float4*>(R.records);
+    for (int vi = 0; vi < R.num_views && live; ++vi) {
+      if (!PAIR && __int_as_float(R.lodmax[vi]) <= (float)(m - 1)) continue;  // no sample reads level m
+      const ColumnView& cv = R.view[vi];
+      const float* A = cv.pmv;
+      float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
+      bool behind = false;
 #pragma unroll
-          for (int m = 0; m <= kBatchMaxLevels; ++m) {
-            if (m > R.mt.levels) break;
+      for (int k = 0; k < 8; ++k) {
+        const float ox = (k & 1) ? b1[0] : b0[0], oy = (k & 2) ? b1[1] : b0[1], oz = (k & 4) ? b1[2] : b0[2];
+        const float4 c = mat_vec(A, ox, oy, oz, 1.f);
+        behind |= !(c.w > 0.f);
+        const float iw = __builtin_amdgcn_rcpf(c.w);
+        const float px = fmaf(c.x * iw, 0.5f, 0.5f) * (float)R.out_w, py = fmaf(c.y * iw, 0.5f, 0.5f) * (float)R.out_h;
+        mnx = fminf(mnx, px);
+        mxx = fmaxf(mxx, px);
+        mny = fminf(mny, py);
+        mxy = fmaxf(mxy, py);
+      }
+      int xs = 0, xe = R.out_w - 1, ys = 0, ye = R.out_h - 1;
+      if (!behind) {
+        xs = max(xs, (int)ceilf(mnx - 0.5f - 1e-2f));
+        xe = min(xe, (int)floorf(mxx - 0.5f + 1e-2f));
+        ys = max(ys, (int)ceilf(mny - 0.5f - 1e-2f));
+        ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
+      }
+      if (xs > xe || ys > ye) continue;
+      const int nxr = xe - xs + 1, cnt = nxr * (ye - ys + 1);
+      // lane k of the box's row-major pixel list; the next record (k + 64) is
+      // loaded while the current one is walked
+      const int dy64 = 64 / nxr, dx64 = 64 - dy64 * nxr;
+      int px = xs + lane % nxr, py = ys + lane / nxr;
+      auto rec_at = [&](int qx, int qy, int& it, int& slot) {
+        it = vi * per_view + (qy >> 4) * R.tiles_x + (qx >> 4);
+        slot = ((qy & 15) << 4) | (qx & 15);
+      };
+      int it_n = 0, slot_n = 0;
+      float4 rec_n = make_float4(__int_as_float(1), __int_as_float(0), 0.f, 0.f);
+      if (lane < cnt) {
+        rec_at(px, py, it_n, slot_n);
+        rec_n = recv[(size_t)it_n * 256 + slot_n];
+      }
+      for (int k = lane; k < cnt; k += 64) {
+        const float4 rec = rec_n;
+        const int it = it_n, slot = slot_n, cpx = px, cpy = py;
+        px += dx64;
+        py += dy64;
+        if (px > xe) {
+          px -= nxr;
+          ++py;
+        }
+        if (k + 64 < cnt) {
+          rec_at(px, py, it_n, slot_n);
+          rec_n = recv[(size_t)it_n * 256 + slot_n];
+        }
+        const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
+        if (lo > hi) continue;
+        const Column col = column_of(cv, cpx, cpy, inv_s1, mips);
+        // the column's samples in the support box (one hundredth of a sample of
+        // slack); a zero step component (reciprocal inf) gives (-inf, inf) for
+        // a column inside that slab and an empty range outside it
+        const float p0[3] = {col.p0.x, col.p0.y, col.p0.z}, dp[3] = {col.dp.x, col.dp.y, col.dp.z};
+        float tlo = (float)lo, thi = (float)hi;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float ir = __builtin_amdgcn_rcpf(dp[a]);
+          const float t0 = (b0[a] - p0[a]) * ir, t1 = (b1[a] - p0[a]) * ir;
+          tlo = fmaxf(tlo, fminf(t0, t1));
+          thi = fminf(thi, fmaxf(t0, t1));
+        }
+        if (!(tlo <= thi + 1.f)) continue;  // (also a NaN slab: 0 * inf)
+        // tlo, thi lie within [lo, hi] here, so the conversions stay in range
+        const int i0 = max(lo, (int)ceilf(tlo - 1e-2f)), i1 = min(hi, (int)floorf(thi + 1e-2f));
+        const float* __restrict__ cs = R.csave + ((size_t)it * S) * 256 + slot;
+        for (int ib = i0; ib <= i1; ib += kGatherBatch) {
+          float cv4[kGatherBatch];
+#pragma unroll
+          for (int u = 0; u < kGatherBatch; ++u) cv4[u] = ib + u <= i1 ? cs[(size_t)(ib + u) * 256] : 0.f;
+#pragma unroll
+          for (int u = 0; u < kGatherBatch; ++u) {
+            const int i = ib + u;
+            if (i > i1) break;
+            const float fi = (float)i;
+            int lv[2];
+            float lw[2];
+            const int nl = level_taps(cfg, mips ? column_lod(col, fi) : 0.f, lv, lw);
+            const float g = fmaf(rec.z, cv4[u], rec.w);
+            const float3 p = fma3(fi, col.dp, col.p0);
             const float wm = level_weight(m, nl, lv, lw);
             if (wm != 0.f) {
-              const float sc = 1.f / (float)(1 << m);
-              level_scatter(lds + cap0 + (m == 1 ? br.size0 : 0), use_lds & (m <= 1), m == 0 ? br.lo0 : br.lo1,
-                            m == 0 ? br.n0 : br.n1, R.gpyr + R.mt.offset[m], R.mt.dims[m], p.x * sc - 0.5f,
-                            p.y * sc - 0.5f, p.z * sc - 0.5f, g * wm);
+              const float qx = p.x * sc - 0.5f, qy = p.y * sc - 0.5f, qz = p.z * sc - 0.5f;
+              if (inside_level(qx, qy, qz, dm)) gather_tap(acc, WX, WY, WZ, x0, y0, z0, qx, qy, qz, g * wm);
+            }
+            if (two) {
+              const float w1 = level_weight(1, nl, lv, lw);
+              if (w1 != 0.f) {
+                const float qx = p.x * 0.5f - 0.5f, qy = p.y * 0.5f - 0.5f, qz = p.z * 0.5f - 0.5f;
+                if (inside_level(qx, qy, qz, d1))
+                  gather_tap(acc1, WX / 2, WY / 2, WZ / 2, x0 / 2, y0 / 2, z0 / 2, qx, qy, qz, g * w1);
+              }
             }
           }
         }
       }
-      // (8-pixel bricks: each wave owns its brick, so use_lds may differ between waves and the
-      // barriers stay outside the branch)
-      __syncthreads();
-      if (use_lds) {
-        // flush the bricks (all 256 threads for a 16-pixel tile, the wave's 64 lanes for an 8-pixel one)
-        const int nthr = R.tile == 16 ? 256 : 64, tid = R.tile == 16 ? (int)threadIdx.x : lane;
-        const float rx0 = 1.f / (float)br.n0[0], rxy0 = 1.f / (float)(br.n0[0] * br.n0[1]);
-        const float rx1 = 1.f / (float)br.n1[0], rxy1 = 1.f / (float)(br.n1[0] * br.n1[1]);
-        for (int idx = tid; idx < br.size; idx += nthr) {
-          const double val = lds[cap0 + idx];
-          if (val != 0.0) {
-            const bool l0 = idx < br.size0;
-            const int j = l0 ? idx : idx - br.size0;
-            const int nx = l0 ? br.n0[0] : br.n1[0], nxy = nx * (l0 ? br.n0[1] : br.n1[1]);
-            const int lz = div_small(j, nxy, l0 ? rxy0 : rxy1), rz = j - lz * nxy;
-            const int ly = div_small(rz, nx, l0 ? rx0 : rx1), lx = rz - ly * nx;
-            if (l0)
-              atomic_addf(R.gpyr + pad_index(br.lo0[0] + lx, br.lo0[1] + ly, br.lo0[2] + lz, R.mt.dims[0]),
-                          (float)val);
-            else
-              atomic_addf(R.gpyr + R.mt.offset[1] +
-                              pad_index(br.lo1[0] + lx, br.lo1[1] + ly, br.lo1[2] + lz, R.mt.dims[1]),
-                          (float)val);
-            lds[cap0 + idx] = 0.0;
-          }
+    }
+    __syncthreads();
+    if (live) {
+      float* __restrict__ gout = R.gpyr + R.mt.offset[m];
+      for (int k = lane; k < kBlockVox; k += 64) {
+        const int x = x0 + k % WX, y = y0 + (k / WX) % WY, z = z0 + k / (WX * WY);
+        if (x < dm.x && y < dm.y && z < dm.z) gout[pad_index(x + 1, y + 1, z + 1, dm)] = (float)acc[k];
+      }
+      if (two) {
+        float* __restrict__ gout1 = R.gpyr + R.mt.offset[1];
+        const int W1 = WX / 2, H1 = WY / 2, Z1 = WZ / 2;
+        for (int k = lane; k < kBlockVox / 8; k += 64) {
+          const int x = x0 / 2 + k % W1, y = y0 / 2 + (k / W1) % H1, z = z0 / 2 + k / (W1 * H1);
+          if (x < d1.x && y < d1.y && z < d1.z) gout1[pad_index(x + 1, y + 1, z + 1, d1)] = (float)acc1[k];
         }
       }
-      __syncthreads();
-      ia = ib + 1;
     }
+    __syncthreads();  // the regions are cleared by the next item
   }
 }
 
@@ -668,26 +638,38 @@ static MipTable batch_mip_table(Dims3 d, int levels) {
   return mt;
 }
 
-static int brick_tile(int brick) { return brick == 0 ? 16 : 8; }
-static int brick_seg(int brick) { return brick == 1 ? 4 : 8; }
+// gather block (voxels) of each brick choice: 16x16x2, 8x8x8, 16x8x4
+static void brick_block(int brick, int& bw, int& bh, int& bd) {
+  bw = brick == 1 ? 8 : 16;
+  bh = brick == 0 ? 16 : 8;
+  bd = brick == 0 ? 2 : (brick == 1 ? 8 : 4);
+}
 
 // workspace layout: the record table, then per reconstruction its bordered
 // density pyramid, the gradients of the same cells, the corner-quad copy of
-// each level, the render partials, the
-// pixel records and the sample values c_k (items x out_d x 256 floats: the
-// largest part, 4 bytes per frustum sample)
+// each level, the render partials, the views' LoD maxima, the pixel records
+// and the sample values c_k (items x out_d x 256 floats: the largest part,
+// 4 bytes per frustum sample)
 struct BatchLayout {
   size_t table, total;
-  std::vector<size_t> pyr, gpyr, partials, records, csave;
+  std::vector<size_t> pyr, gpyr, partials, lodmax, records, csave;
   std::vector<std::array<size_t, kBatchLevels>> quad;
 };
 
-// items (of 256 threads) of pass p for a reconstruction with pyramid mt
-static int pass_items(const MipTable& mt, int p) {
-  if (p < kBatchLevels) return p <= mt.levels ? (int)((padded_cells(mt.dims[p]) + 255) / 256) : 0;
-  if (p < 2 * kBatchLevels) {
-    const int m = p - kBatchLevels;
+// items (of 256 threads) of pass p for a reconstruction with pyramid mt and
+// gather block bw x bh x bd
+static int pass_items(const MipTable& mt, int bw, int bh, int bd, int p) {
+  if (p < kQuadPass) return p <= mt.levels ? (int)((padded_cells(mt.dims[p]) + 255) / 256) : 0;
+  if (p < kGatherPass) {
+    const int m = p - kQuadPass;
     return m <= mt.levels ? (int)((quad_entries(make_quad_dims(mt.dims[m])) + 255) / 256) : 0;
+  }
+  if (p < kFinishPass) {
+    const int m = p - kGatherPass;
+    if (m > mt.levels || m == 1) return 0;  // level 1 is gathered with level 0
+    const Dims3 d = mt.dims[m];
+    const int64_t blocks = (int64_t)((d.x + bw - 1) / bw) * ((d.y + bh - 1) / bh) * ((d.z + bd - 1) / bd);
+    return (int)((blocks + 3) / 4);  // four waves (blocks) per item
   }
   return (int)((mt.base_stride + 255) / 256);
 }
@@ -703,6 +685,7 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
   L.gpyr.resize(n);
   L.records.resize(n);
   L.csave.resize(n);
+  L.lodmax.resize(n);
   L.partials.resize(n);
   L.quad.resize(n);
   for (int r = 0; r < n; ++r) {
@@ -732,6 +715,8 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
     const size_t items = (size_t)c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
     L.partials[r] = off;
     off += align256b(items * sizeof(float));
+    L.lodmax[r] = off;
+    off += align256b(kBatchMaxViews * sizeof(int));
     L.records[r] = off;
     off += align256b(items * 256 * 4 * sizeof(float));
     L.csave[r] = off;
@@ -751,7 +736,9 @@ static void batch_items(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
     const gft_batch_recon& c = rs[r];
     render += c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
     const MipTable mt = batch_mip_table(Dims3{c.grid_dhw[2], c.grid_dhw[1], c.grid_dhw[0]}, num_mipmaps);
-    for (int p = 0; p < kBatchPasses; ++p) pass[p] += pass_items(mt, p);
+    int bw, bh, bd;
+    brick_block(c.brick, bw, bh, bd);
+    for (int p = 0; p < kBatchPasses; ++p) pass[p] += pass_items(mt, bw, bh, bd, p);
   }
 }
 
@@ -795,6 +782,7 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
     R.target = c.targets;
     R.images = c.images;
     R.partials = (float*)(ws + L.partials[r]);
+    R.lodmax = (int*)(ws + L.lodmax[r]);
     R.records = (float*)(ws + L.records[r]);
     R.csave = (float*)(ws + L.csave[r]);
     R.loss = c.loss;
@@ -807,32 +795,41 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
     R.out_w = c.output_shape[2];
     R.tiles_x = (R.out_w + 15) / 16;
     R.tiles_y = (R.out_h + 15) / 16;
-    R.tile = brick_tile(c.brick);
-    R.seg = brick_seg(c.brick);
+    brick_block(c.brick, R.bw, R.bh, R.bd);
     R.item0 = item;
     item += R.num_views * R.tiles_x * R.tiles_y;
     for (int p = 0; p < kBatchPasses; ++p) {
       R.pass_item0[p] = pass_item[p];
-      pass_item[p] += pass_items(R.mt, p);
+      pass_item[p] += pass_items(R.mt, R.bw, R.bh, R.bd, p);
     }
     for (int m = 0; m <= R.mt.levels; ++m) {
       R.quad[m] = (float4*)(ws + L.quad[r][m]);
       R.qd[m] = make_quad_dims(R.mt.dims[m]);
     }
     const Mat4d m = mat_from_colmajor(c.M);
+    const double W = R.out_w, Hh = R.out_h, S = R.out_d;
     for (int v = 0; v < c.num_views; ++v) {
       const Mat4d mv = mat_mul(mat_from_colmajor(c.V + 16 * v), m), p = mat_from_colmajor(c.P + 16 * v);
       Mat4d mvinv, pinv;
       if (!mat_inverse(mv, mvinv)) return fail(GFT_ERR_INVALID_ARGUMENT, "model-view matrix is singular");
       if (!mat_inverse(p, pinv)) return fail(GFT_ERR_INVALID_ARGUMENT, "projection matrix is singular");
-      SampleView& sv = R.view[v];
-      mat_to_float(pinv, sv.A);
-      mat_to_float(mvinv, sv.B);
-      sv.p22 = c.P[16 * v + 4 * 2 + 2];
-      sv.p32 = c.P[16 * v + 4 * 3 + 2];
-      sv.nearp = c.frustum[6 * v + 0];
-      sv.farp = c.frustum[6 * v + 1];
-      sv.out_slot = v;
+      // object position of pixel (x, y) at depth index q: (C (nx, ny, nz(q), 1)).xyz / .w with C = mv^-1 p^-1,
+      // nx = (x + 0.5) 2 / W - 1 and the TRANSFORM_LINDEPTH depth nz(q) = -p22 - p32 / z, z = -(near + (far -
+      // near) q / S) (transformations.hpp:86-105)
+      const Mat4d C = mat_mul(mvinv, pinv);
+      const double p22 = c.P[16 * v + 4 * 2 + 2], p32 = c.P[16 * v + 4 * 3 + 2];
+      const double nearp = c.frustum[6 * v + 0], farp = c.frustum[6 * v + 1];
+      auto nz_of = [&](double q) { return -p22 - p32 / (-(nearp + (farp - nearp) * (q / S))); };
+      ColumnView& cv = R.view[v];
+      const double ax = 1.0 / W - 1.0, ay = 1.0 / Hh - 1.0;
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const double c0 = C.m[0 * 4 + r4], c1 = C.m[1 * 4 + r4], c2 = C.m[2 * 4 + r4], c3 = C.m[3 * 4 + r4];
+        cv.n0[r4] = (float)(c0 * ax + c1 * ay + c2 * nz_of(0.5) + c3);
+        cv.f0[r4] = (float)(c0 * ax + c1 * ay + c2 * nz_of(S - 0.5) + c3);
+        cv.gx[r4] = (float)(c0 * 2.0 / W);
+        cv.gy[r4] = (float)(c1 * 2.0 / Hh);
+      }
+      mat_to_float(mat_mul(p, mv), cv.pmv);
     }
   }
   // a set-up call: the table is copied before it returns
@@ -861,9 +858,9 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
       if (pass[m])
         hipLaunchKernelGGL(k_batch_level, dim3(persistent_grid(pass[m])), dim3(256), 0, s, recs, n, m, pass[m]);
     for (int m = 0; m <= kBatchMaxLevels; ++m)
-      if (pass[kBatchLevels + m])
-        hipLaunchKernelGGL(k_batch_quad, dim3(persistent_grid(pass[kBatchLevels + m])), dim3(256), 0, s, recs, n, m,
-                           pass[kBatchLevels + m]);
+      if (pass[kQuadPass + m])
+        hipLaunchKernelGGL(k_batch_quad, dim3(persistent_grid(pass[kQuadPass + m])), dim3(256), 0, s, recs, n, m,
+                           pass[kQuadPass + m]);
   }
   if ((rc = check_hip(hipGetLastError(), "batched mip levels"))) return rc;
   {
@@ -872,10 +869,16 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
   }
   if ((rc = check_hip(hipGetLastError(), "batched render forward"))) return rc;
   {
-    KernelTimer t(s, "batch_bwd");
-    hipLaunchKernelGGL(k_batch_bwd, dim3(persistent_grid(render)), dim3(256), 0, s, recs, n, render);
+    KernelTimer t(s, "batch_gather");
+    if (pass[kGatherPass])
+      hipLaunchKernelGGL(k_batch_gather<true>, dim3(persistent_grid(pass[kGatherPass])), dim3(256), 0, s, recs, n, 0,
+                         pass[kGatherPass]);
+    for (int m = 2; m <= kBatchMaxLevels; ++m)
+      if (pass[kGatherPass + m])
+        hipLaunchKernelGGL(k_batch_gather<false>, dim3(persistent_grid(pass[kGatherPass + m])), dim3(256), 0, s, recs,
+                           n, m, pass[kGatherPass + m]);
   }
-  if ((rc = check_hip(hipGetLastError(), "batched render backward"))) return rc;
+  if ((rc = check_hip(hipGetLastError(), "batched render gather"))) return rc;
   {
     KernelTimer t(s, "batch_finish");
     hipLaunchKernelGGL(k_batch_finish, dim3(persistent_grid(pass[kFinishPass])), dim3(256), 0, s, recs, n,

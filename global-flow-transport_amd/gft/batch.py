@@ -6,14 +6,16 @@ ambient light (Renderer.render_density with fused=False,
 phitest/render/renderer.py:555-568,603-651: SampleGridTransform
 TRANSFORM_LINDEPTH with LINEAR mipmapping and a per-sample LoD, then the
 BEER_LAMBERT depth blend), the mean squared rgb error against its targets and
-the exact density gradient. All reconstructions run together: one level pass
-per mip level, one launch for the forward and backward of every ray, one for
-the mip adjoint and the losses. ``run()`` only launches kernels, so a
-``torch.cuda.graph`` can capture it.
+the exact density gradient. All reconstructions run together: per mip level
+a pass building the level and one taking its corner-quad copy, one forward
+launch over every pixel, one gather launch per level (the gradient summed per
+voxel block, no atomics in global memory: images, losses and gradients repeat
+bit for bit), one for the mip adjoint and the losses. ``run()`` only launches
+kernels, so a ``torch.cuda.graph`` can capture it.
 
-``autotune()`` picks the backward's brick (tile x segment) per grid-size
-class by timing the whole batch: for each class in turn, every candidate with
-the other classes at their current choice.
+``autotune()`` picks the backward's brick (the voxel block of its gather)
+per grid-size class by timing the whole batch: for each class in turn, every
+candidate with the other classes at their current choice.
 """
 import ctypes
 
@@ -22,9 +24,7 @@ import torch
 
 from . import _lib
 
-BRICKS = {0: "16x16-pixel tiles, 8-sample segments (about 8^3 cells)",
-          1: "8x8-pixel tiles, 4-sample segments (about 4^3 cells)",
-          2: "8x8-pixel tiles, 8-sample segments (about 4x4x8 cells)"}
+BRICKS = {0: "16x16x2-voxel gather blocks", 1: "8x8x8-voxel gather blocks", 2: "16x8x4-voxel gather blocks"}
 
 
 class BatchRecon(ctypes.Structure):

@@ -1,19 +1,21 @@
 """The batched density-reconstruction step (gft_batch_render_*): many
 reconstructions of uneven grid sizes and view counts, each rendered with the
 non-fused mipmapped resample (TRANSFORM_LINDEPTH, LINEAR mips, per-sample
-LoD) and the BEER_LAMBERT blend, L2 loss and exact density gradient, all in
-one level pass per mip level, one forward+backward launch and one finishing
-launch.
+LoD) and the BEER_LAMBERT blend, L2 loss and exact density gradient: level
+and corner-quad passes per mip level, one forward launch, one gather launch
+per level and one finishing launch.
 
 Against the FP64 oracle composition mip_render_fwd_bwd (the pinned resample
-with mips, the pinned blend and the resample's exact adjoint): images and
-losses within 1e-5, gradients within 1e-4 (normalised max error). Images do
-not depend on the brick; gradients agree across bricks and between a graph
-replay and the eager run to 1e-6 (float atomics). The seeded c4 draw of 32
-reconstructions: replay equals eager, the reconstructions up to 64x112x64 vs
-the oracle, and every reconstruction at full size vs an independent GPU path
-(the per-op resample kernel with its exact adjoint, a float64 torch blend and
-loss).
+with mips, the pinned blend and the resample's exact adjoint): images within
+1e-5, gradients within 1e-4 (normalised max error), losses equal to the FP64
+loss of the GPU's own images and within what the image tolerance allows of
+the oracle's. Images and losses do not depend on the brick (the gather's
+voxel block); gradients agree across bricks to 1e-6 (summation order), and a
+graph replay repeats the eager run bit for bit (no atomics in global
+memory). The seeded c4 draw of 32 reconstructions: replay equals eager, the
+reconstructions up to 64x112x64 vs the oracle, and every reconstruction at
+full size vs an independent GPU path (the per-op resample kernel with its
+exact adjoint, a float64 torch blend and loss).
 """
 import numpy as np
 import pytest
@@ -119,7 +121,7 @@ def test_batched_render_bricks_and_graph_replay(cuda):
         torch.cuda.synchronize()
         for k in range(len(recons)):
             assert torch.equal(b.images[k], results[2][0][k]) and torch.equal(b.losses[k], results[2][2][k])
-            assert norm_err(b.grads[k].cpu().numpy(), results[2][1][k].cpu().numpy()) < 1e-6
+            assert torch.equal(b.grads[k], results[2][1][k])
 
 
 def test_batched_render_validation(cuda):
@@ -184,7 +186,7 @@ def test_c4_batched_draw(cuda):
         torch.cuda.synchronize()
         for k in range(len(recons)):
             assert torch.equal(b.images[k], eager[0][k]) and torch.equal(b.losses[k], eager[2][k])
-            assert norm_err(b.grads[k].cpu().numpy(), eager[1][k].cpu().numpy()) < 1e-6
+            assert torch.equal(b.grads[k], eager[1][k])
     checked = 0
     for k, (r, d) in enumerate(zip(recons, dens)):
         if np.prod(r["grid"]) > 64 * 112 * 64:
