@@ -59,6 +59,7 @@ constexpr int kGatherPass = 2 * kBatchLevels;
 constexpr int kFinishPass = 3 * kBatchLevels;
 constexpr int kBatchPasses = kFinishPass + 1;
 constexpr int kBlockVox = 512;  // voxels of a gather block (every shape)
+constexpr int kFwdBatch = 4;    // samples per forward round trip
 
 // four floats in global memory (loads through it are global, not flat,
 // 16-byte instructions)
@@ -273,6 +274,45 @@ __device__ __forceinline__ float quad_read(const gcf32x4* __restrict__ quad, con
   return lerpf(v0, v1, tz);
 }
 
+// U samples' BORDER trilinear values at one level, every load in flight
+// before any value is used: the values of quad_read for the samples with
+// use[u] (others read a clamped entry and give 0)
+template <int U>
+__device__ __forceinline__ void quad_read_batch(const gcf32x4* __restrict__ quad, const QuadDims& qd, Dims3 d,
+                                                const float3* p, float sc, const bool* use, float* val) {
+  unsigned a0[U], a1[U];
+  float w[U][3];
+  bool in[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const float px = p[u].x * sc - 0.5f, py = p[u].y * sc - 0.5f, pz = p[u].z * sc - 0.5f;
+    const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+    in[u] = use[u] & (0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) &
+            (qz < (float)d.z);
+    const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    w[u][0] = px - fx;
+    w[u][1] = py - fy;
+    w[u][2] = pz - fz;
+    const unsigned ix = (unsigned)clampi((int)fx + 1, 0, (int)qd.qx - 1);
+    const unsigned iy = (unsigned)clampi((int)fy + 1, 0, (int)qd.qy - 1);
+    const unsigned iz = (unsigned)clampi((int)fz + 1, 0, (int)qd.qz - 2);
+    a0[u] = quad_index(qd, ix, iy, iz);
+    a1[u] = quad_index(qd, ix, iy, iz + 1);
+  }
+  f32x4 c0[U], c1[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    c0[u] = quad[a0[u]];
+    c1[u] = quad[a1[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const float v0 = lerpf(lerpf(c0[u].x, c0[u].y, w[u][0]), lerpf(c0[u].z, c0[u].w, w[u][0]), w[u][1]);
+    const float v1 = lerpf(lerpf(c1[u].x, c1[u].y, w[u][0]), lerpf(c1[u].z, c1[u].w, w[u][0]), w[u][1]);
+    val[u] = in[u] ? lerpf(v0, v1, w[u][2]) : 0.f;
+  }
+}
+
 // the mipmapped sample: level taps of the LoD, each level read at a
 // wave-uniform base (the level loop is uniform, the taps per lane)
 __device__ __forceinline__ float batch_sample(const BatchRec& R, const SampleCfg& cfg, float3 p, float lod) {
@@ -329,14 +369,48 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
       int lo, hi;
       clip_ray(col.p0, col.dp, R.mt.dims[0], S, 0.f, lo, hi);
       float* __restrict__ cs = R.csave + (size_t)local * S * 256 + threadIdx.x;
-      for (int i = lo; i <= hi; ++i) {
-        const float fi = (float)i;
-        const float lod = mips ? column_lod(col, fi) : 0.f;
-        const float sigma = R.step * batch_sample(R, cfg, fma3(fi, col.dp, col.p0), lod);
-        D += sigma;
-        const float u = __expf(-D);
-        cs[(size_t)i * 256] = u + E;
-        E = fmaf(u, sigma, E);
+      for (int ib = lo; ib <= hi; ib += kFwdBatch) {
+        // kFwdBatch samples at a time: per mip level any of them reads, all
+        // their loads in flight (the values are quad_read's)
+        float3 p[kFwdBatch];
+        int lv[kFwdBatch][2], nl[kFwdBatch];
+        float lw[kFwdBatch][2], vf[kFwdBatch], vc[kFwdBatch];
+#pragma unroll
+        for (int u = 0; u < kFwdBatch; ++u) {
+          const float fi = (float)(ib + u);
+          p[u] = fma3(fi, col.dp, col.p0);
+          nl[u] = level_taps(cfg, mips ? column_lod(col, fi) : 0.f, lv[u], lw[u]);
+          vf[u] = vc[u] = 0.f;
+        }
+#pragma unroll
+        for (int m = 0; m <= kBatchMaxLevels; ++m) {
+          if (m > R.mt.levels) break;
+          bool use[kFwdBatch], any = false;
+#pragma unroll
+          for (int u = 0; u < kFwdBatch; ++u) {
+            use[u] = (ib + u <= hi) & ((lv[u][0] == m) | ((nl[u] == 2) & (lv[u][1] == m)));
+            any |= use[u];
+          }
+          if (!any) continue;
+          float val[kFwdBatch];
+          quad_read_batch<kFwdBatch>((const gcf32x4*)R.quad[m], R.qd[m], R.mt.dims[m], p, 1.f / (float)(1 << m),
+                                     use, val);
+#pragma unroll
+          for (int u = 0; u < kFwdBatch; ++u)
+            if (use[u]) {
+              if (lv[u][0] == m) vf[u] = val[u];
+              else vc[u] = val[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFwdBatch; ++u) {
+          if (ib + u > hi) break;
+          const float sigma = R.step * (nl[u] == 2 ? lerpf(vf[u], vc[u], lw[u][1]) : vf[u]);
+          D += sigma;
+          const float e = __expf(-D);
+          cs[(size_t)(ib + u) * 256] = e + E;
+          E = fmaf(e, sigma, E);
+        }
       }
       if (mips && lo <= hi) lmax = fmaxf(column_lod(col, (float)lo), column_lod(col, (float)hi));
       const size_t pix = ((size_t)vi * R.out_h + y) * R.out_w + x;
