@@ -65,6 +65,7 @@ constexpr int kFwdBatch = 4;    // samples per forward round trip
 // 16-byte instructions)
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const f32x4 gcf32x4;
+typedef __attribute__((address_space(1))) const float gcfloat;
 
 // One view of a reconstruction. TRANSFORM_LINDEPTH samples pixel (x, y) at
 // depth index q in object space at H.xyz / H.w, H = C (nx, ny, nz(q), 1) with
@@ -249,11 +250,6 @@ __device__ __forceinline__ float column_lod(const Column& c, float fi) {
   const float fy = fmaf(ex.y, ex.y, fmaf(ey.y, ey.y, c.dp.y * c.dp.y));
   const float fz = fmaf(ex.z, ex.z, fmaf(ey.z, ey.z, c.dp.z * c.dp.z));
   return fmaxf(0.f, 0.5f * log2f(fmaxf(fx, fmaxf(fy, fz))));
-}
-
-// weight of level m among a sample's mip taps (0 if the sample does not read it)
-__device__ __forceinline__ float level_weight(int m, int nl, const int lv[2], const float lw[2]) {
-  return lv[0] == m ? lw[0] : (((nl == 2) & (lv[1] == m)) ? lw[1] : 0.f);
 }
 
 // BORDER trilinear value from a level's corner-quad copy, (px, py, pz) in
@@ -454,33 +450,42 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
 // a fixed order and one store per voxel follows.
 constexpr int kGatherBatch = 4;  // c_k loaded per round trip
 
-// add a sample's level tap to the block region acc (block dims W, origin o,
-// (qx, qy, qz) in the level's cell coordinates, g the weighted gradient)
+// add a sample's level tap to the block region acc (block W x H x Z with
+// WXY = W H, origin o, (qx, qy, qz) in the level's cell coordinates, g the
+// weighted gradient); corners outside the block are skipped
 __device__ __forceinline__ void gather_tap(__attribute__((address_space(3))) double* acc, int WX, int WY, int WZ,
-                                           int ox, int oy, int oz, float qx, float qy, float qz, float g) {
+                                           int WXY, int ox, int oy, int oz, float qx, float qy, float qz, float g) {
   const float rx = qx - (float)ox, ry = qy - (float)oy, rz = qz - (float)oz;
   const float flx = floorf(rx), fly = floorf(ry), flz = floorf(rz);
   const float tx = rx - flx, ty = ry - fly, tz = rz - flz;
   const int lx = (int)flx, ly = (int)fly, lz = (int)flz;
   const float wx[2] = {g * (1.f - tx), g * tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
+  const bool vx[2] = {(unsigned)lx < (unsigned)WX, (unsigned)(lx + 1) < (unsigned)WX};
+  const bool vy[2] = {(unsigned)ly < (unsigned)WY, (unsigned)(ly + 1) < (unsigned)WY};
+  const bool vz[2] = {(unsigned)lz < (unsigned)WZ, (unsigned)(lz + 1) < (unsigned)WZ};
+  const int base = __mul24(__mul24(lz, WY) + ly, WX) + lx;
 #pragma unroll
   for (int kz = 0; kz < 2; ++kz) {
-    const int cz = lz + kz;
-    if ((unsigned)cz >= (unsigned)WZ) continue;
+    if (!vz[kz]) continue;
 #pragma unroll
     for (int ky = 0; ky < 2; ++ky) {
-      const int cy = ly + ky;
-      if ((unsigned)cy >= (unsigned)WY) continue;
+      if (!vy[ky]) continue;
       const float wyz = wy[ky] * wz[kz];
+      const int row = base + (kz ? WXY : 0) + (ky ? WX : 0);
 #pragma unroll
       for (int kx = 0; kx < 2; ++kx) {
-        const int cx = lx + kx;
-        if ((unsigned)cx >= (unsigned)WX) continue;
-        __hip_atomic_fetch_add(acc + (cz * WY + cy) * WX + cx, (double)(wx[kx] * wyz), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (!vx[kx]) continue;
+        __hip_atomic_fetch_add(acc + row + kx, (double)(wx[kx] * wyz), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
   }
+}
+
+// LINEAR-mip level weights (level_taps for lod >= 0 and bias 0): with
+// l = min(lod, L), level floor(l) has weight 1 - frac(l) and level
+// floor(l) + 1 has frac(l)
+__device__ __forceinline__ float mip_weight(int m, int fl, float fr) {
+  return m == fl ? 1.f - fr : (m == fl + 1 ? fr : 0.f);
 }
 
 __device__ __forceinline__ bool inside_level(float qx, float qy, float qz, Dims3 d) {
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
     const int S = R.out_d;
     const float inv_s1 = S > 1 ? 1.f / (float)(S - 1) : 0.f;
     const bool mips = R.mt.levels > 0;
-    const SampleCfg cfg{F_LINEAR, mips ? MIP_LINEAR : MIP_NONE, R.mt.levels, 0.f, 0.5f};
+    const float lmaxf = (float)R.mt.levels;
     const int per_view = R.tiles_x * R.tiles_y;
     const float4* __restrict__ recv = reinterpret_cast<const float4*>(R.records);
     for (int vi = 0; vi < R.num_views && live; ++vi) {
@@ -594,32 +599,33 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
         if (!(tlo <= thi + 1.f)) continue;  // (also a NaN slab: 0 * inf)
         // tlo, thi lie within [lo, hi] here, so the conversions stay in range
         const int i0 = max(lo, (int)ceilf(tlo - 1e-2f)), i1 = min(hi, (int)floorf(thi + 1e-2f));
-        const float* __restrict__ cs = R.csave + ((size_t)it * S) * 256 + slot;
+        const gcfloat* __restrict__ cs = (const gcfloat*)R.csave + ((size_t)it * S) * 256 + slot;
         for (int ib = i0; ib <= i1; ib += kGatherBatch) {
           float cv4[kGatherBatch];
 #pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) cv4[u] = ib + u <= i1 ? cs[(size_t)(ib + u) * 256] : 0.f;
+          for (int u = 0; u < kGatherBatch; ++u) cv4[u] = cs[(unsigned)min(ib + u, i1) * 256u];
 #pragma unroll
           for (int u = 0; u < kGatherBatch; ++u) {
             const int i = ib + u;
             if (i > i1) break;
             const float fi = (float)i;
-            int lv[2];
-            float lw[2];
-            const int nl = level_taps(cfg, mips ? column_lod(col, fi) : 0.f, lv, lw);
+            const float l = mips ? fminf(column_lod(col, fi), lmaxf) : 0.f;
+            const int fl = (int)l;
+            const float fr = l - (float)fl;
             const float g = fmaf(rec.z, cv4[u], rec.w);
             const float3 p = fma3(fi, col.dp, col.p0);
-            const float wm = level_weight(m, nl, lv, lw);
+            const float wm = mip_weight(m, fl, fr);
             if (wm != 0.f) {
               const float qx = p.x * sc - 0.5f, qy = p.y * sc - 0.5f, qz = p.z * sc - 0.5f;
-              if (inside_level(qx, qy, qz, dm)) gather_tap(acc, WX, WY, WZ, x0, y0, z0, qx, qy, qz, g * wm);
+              if (inside_level(qx, qy, qz, dm)) gather_tap(acc, WX, WY, WZ, WX * WY, x0, y0, z0, qx, qy, qz, g * wm);
             }
             if (two) {
-              const float w1 = level_weight(1, nl, lv, lw);
+              const float w1 = mip_weight(1, fl, fr);
               if (w1 != 0.f) {
                 const float qx = p.x * 0.5f - 0.5f, qy = p.y * 0.5f - 0.5f, qz = p.z * 0.5f - 0.5f;
                 if (inside_level(qx, qy, qz, d1))
-                  gather_tap(acc1, WX / 2, WY / 2, WZ / 2, x0 / 2, y0 / 2, z0 / 2, qx, qy, qz, g * w1);
+                  gather_tap(acc1, WX / 2, WY / 2, WZ / 2, (WX / 2) * (WY / 2), x0 / 2, y0 / 2, z0 / 2, qx, qy, qz,
+                             g * w1);
               }
             }
           }
