@@ -481,13 +481,12 @@ int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params
  * taking its corner-quad copy, one forward launch over every pixel of every
  * view of every reconstruction (rays clipped to the grid box; it keeps each
  * sample's c_k = exp(-D_k) + E_{k-1} and each pixel's gradient record in the
- * workspace), one backward launch scattering every sample's gradient through
- * FP64 LDS bricks per tile and segment, and one that applies the mip adjoint
- * and sums the losses. `brick` picks the tile and the longest segment per
- * reconstruction: 0 = 16x16-pixel tiles with up to 8 samples, 1 = 8x8 with 4,
- * 2 = 8x8 with 8 (a segment whose bricks exceed the LDS is halved). The
- * workspace holds 4 bytes per frustum sample besides the pyramids. Float
- * atomics: the gradient's last bits vary between runs; images and losses
+ * workspace), a voxel-block gather of every sample's gradient for levels 0
+ * and 1 together (and per coarser level a view's samples reach), and one
+ * launch that applies the mip adjoint and sums the losses. `brick` picks the
+ * gather's voxel block per reconstruction: 0 = 16x16x2, 1 = 8x8x8,
+ * 2 = 16x8x4. The workspace holds 4 bytes per frustum sample besides the
+ * pyramids. No atomics in global memory: images, losses and gradients
  * repeat bit for bit. Grids with (H+2)(D+2) >= 2^24 are rejected.
  * gft_batch_render_prepare writes the reconstruction table into the workspace
  * (a set-up call that waits for its copy); gft_batch_render_run only launches
