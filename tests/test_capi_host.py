@@ -273,3 +273,64 @@ def test_graft_entry_points_exist():
     sys.path.insert(0, ROOT)
     import __graft_entry__ as ge
     assert callable(ge.build) and callable(ge.smoke)
+
+
+def _batch_records(specs, brick=0):
+    """gft_batch_recon records of small synthetic reconstructions (host
+    matrices real, device pointers placeholders: nothing here touches a GPU)."""
+    from gft.batch import BatchRecon
+    from gft.camera import camera_params_batch, model_matrices
+    from gft import synthetic as syn
+    arr = (BatchRecon * len(specs))()
+    keep = []
+    for k, (grid, image, views) in enumerate(specs):
+        cams, S = syn.arc_cameras(grid, image, views)
+        M = np.ascontiguousarray(model_matrices([syn.grid_transform(grid)]), dtype=np.float32)
+        V, P, F = (np.ascontiguousarray(x, dtype=np.float32) for x in camera_params_batch(cams))
+        keep += [M, V, P, F]
+        c = arr[k]
+        c.grid_dhw[:] = list(grid)
+        c.output_shape[:] = [S, image[0], image[1]]
+        c.num_views = views
+        c.depth_step = float(cams[0].depth_step)
+        c.light[:] = [1.0, 1.0, 1.0]
+        c.loss_scale = 1.0
+        c.brick = brick
+        c.M, c.V, c.P, c.frustum = (x.ctypes.data for x in (M, V, P, F))
+        c.density = c.targets = c.images = c.density_grad = c.loss = 0x1000  # never dereferenced here
+    return arr, keep
+
+
+def test_batch_workspace_and_validation(lib):
+    from gft import _lib
+    specs = [((12, 20, 14), (24, 14), 3), ((8, 9, 10), (16, 16), 1)]
+    arr, keep = _batch_records(specs)
+    p = ctypes.cast(arr, ctypes.c_void_p)
+    ws = lib.gft_batch_render_workspace(2, p, 3)
+    # at least 4 bytes of c_k per frustum sample and one 16-byte record per pixel of every tile
+    samples = sum(v * ((h + 15) // 16) * ((w + 15) // 16) * 256 * arr[k].output_shape[0]
+                  for k, (_, (h, w), v) in enumerate(specs))
+    assert ws >= 4 * samples + 16 * samples // arr[0].output_shape[0]
+    # the brick (the gather's voxel block) changes launch sizes, not the workspace
+    for b in (1, 2):
+        arr_b, keep_b = _batch_records(specs, brick=b)
+        assert lib.gft_batch_render_workspace(2, ctypes.cast(arr_b, ctypes.c_void_p), 3) == ws
+    # one more reconstruction needs more; fewer mip levels need less
+    arr3, keep3 = _batch_records(specs + [((16, 16, 16), (20, 20), 2)])
+    assert lib.gft_batch_render_workspace(3, ctypes.cast(arr3, ctypes.c_void_p), 3) > ws
+    assert 0 < lib.gft_batch_render_workspace(2, p, 0) < ws
+    # a set-up call with a too small workspace fails before any device work
+    assert lib.gft_batch_render_prepare(None, 2, p, 3, None, 0) == _lib.GFT_ERR_WORKSPACE
+    # invalid records: the query returns 0 and the set-up call names the problem
+    bad = [("num_views", 0), ("num_views", 9), ("brick", 3)]
+    for field, value in bad:
+        arr_x, keep_x = _batch_records(specs)
+        setattr(arr_x[1], field, value)
+        px = ctypes.cast(arr_x, ctypes.c_void_p)
+        assert lib.gft_batch_render_workspace(2, px, 3) == 0, field
+        assert lib.gft_batch_render_prepare(None, 2, px, 3, None, 0) == _lib.GFT_ERR_INVALID_ARGUMENT, field
+        assert b"reconstruction 1" in lib.gft_last_error(), field
+    assert lib.gft_batch_render_workspace(2, p, 4) == 0  # at most 3 mip levels
+    assert lib.gft_batch_render_workspace(0, p, 3) == 0
+    arr_h, keep_h = _batch_records([((5000, 4000, 8), (16, 16), 1)])  # (H+2)(D+2) >= 2^24
+    assert lib.gft_batch_render_workspace(1, ctypes.cast(arr_h, ctypes.c_void_p), 3) == 0
