@@ -252,27 +252,11 @@ __device__ __forceinline__ float column_lod(const Column& c, float fi) {
   return fmaxf(0.f, 0.5f * log2f(fmaxf(fx, fmaxf(fy, fz))));
 }
 
-// BORDER trilinear value from a level's corner-quad copy, (px, py, pz) in
-// cell coordinates (p / 2^m - 0.5): make_taps' inside test, corners and lerp
-// order, as read_linear (an inside sample's corners are entries x, y in
-// [-1, dims-1] and slices z, z+1 in [-1, dims])
-__device__ __forceinline__ float quad_read(const gcf32x4* __restrict__ quad, const QuadDims& qd, Dims3 d, float px,
-                                           float py, float pz) {
-  const float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
-  if (!((0.f <= qx) & (qx < (float)d.x) & (0.f <= qy) & (qy < (float)d.y) & (0.f <= qz) & (qz < (float)d.z)))
-    return 0.f;
-  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
-  const float tx = px - fx, ty = py - fy, tz = pz - fz;
-  const unsigned ix = (unsigned)((int)fx + 1), iy = (unsigned)((int)fy + 1), iz = (unsigned)((int)fz + 1);
-  const f32x4 c0 = quad[quad_index(qd, ix, iy, iz)], c1 = quad[quad_index(qd, ix, iy, iz + 1)];
-  const float v0 = lerpf(lerpf(c0.x, c0.y, tx), lerpf(c0.z, c0.w, tx), ty);
-  const float v1 = lerpf(lerpf(c1.x, c1.y, tx), lerpf(c1.z, c1.w, tx), ty);
-  return lerpf(v0, v1, tz);
-}
-
-// U samples' BORDER trilinear values at one level, every load in flight
-// before any value is used: the values of quad_read for the samples with
-// use[u] (others read a clamped entry and give 0)
+// U samples' BORDER trilinear values at one level from its corner-quad copy,
+// (p / 2^m - 0.5) in cell coordinates, every load in flight before any value
+// is used: make_taps' inside test, corners and lerp order, as read_linear (an
+// inside sample's corners are entries x, y in [-1, dims-1] and slices z, z+1
+// in [-1, dims]); samples without use[u] read a clamped entry and give 0
 template <int U>
 __device__ __forceinline__ void quad_read_batch(const gcf32x4* __restrict__ quad, const QuadDims& qd, Dims3 d,
                                                 const float3* p, float sc, const bool* use, float* val) {
@@ -307,28 +291,6 @@ __device__ __forceinline__ void quad_read_batch(const gcf32x4* __restrict__ quad
     const float v1 = lerpf(lerpf(c1[u].x, c1[u].y, w[u][0]), lerpf(c1[u].z, c1[u].w, w[u][0]), w[u][1]);
     val[u] = in[u] ? lerpf(v0, v1, w[u][2]) : 0.f;
   }
-}
-
-// the mipmapped sample: level taps of the LoD, each level read at a
-// wave-uniform base (the level loop is uniform, the taps per lane)
-__device__ __forceinline__ float batch_sample(const BatchRec& R, const SampleCfg& cfg, float3 p, float lod) {
-  int lv[2];
-  float lw[2];
-  const int nl = level_taps(cfg, lod, lv, lw);
-  float vf = 0.f, vc = 0.f;
-#pragma unroll
-  for (int m = 0; m <= kBatchMaxLevels; ++m) {
-    if (m > R.mt.levels) break;
-    const bool f = lv[0] == m, c = (nl == 2) & (lv[1] == m);
-    if (f | c) {
-      const float sc = 1.f / (float)(1 << m);
-      const float a = quad_read((const gcf32x4*)R.quad[m], R.qd[m], R.mt.dims[m], p.x * sc - 0.5f,
-                                p.y * sc - 0.5f, p.z * sc - 0.5f);
-      if (f) vf = a;
-      else vc = a;
-    }
-  }
-  return nl == 2 ? lerpf(vf, vc, lw[1]) : vf;
 }
 
 // ------------------------------------------------------------------ forward
@@ -367,7 +329,7 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
       float* __restrict__ cs = R.csave + (size_t)local * S * 256 + threadIdx.x;
       for (int ib = lo; ib <= hi; ib += kFwdBatch) {
         // kFwdBatch samples at a time: per mip level any of them reads, all
-        // their loads in flight (the values are quad_read's)
+        // their loads in flight
         float3 p[kFwdBatch];
         int lv[kFwdBatch][2], nl[kFwdBatch];
         float lw[kFwdBatch][2], vf[kFwdBatch], vc[kFwdBatch];
