@@ -525,7 +525,7 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
  * render_fwd, render_bwd_pixel (with the folded loss sum), render_bwd_gather, render_bwd_scatter, quad_copy, fill,
- * render_lit_fwd, render_lit_bwd_gather, batch_levels, batch_fwd, batch_bwd, batch_finish,
+ * render_lit_fwd, render_lit_bwd_gather, batch_levels, batch_fwd, batch_gather, batch_finish,
  * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam, transform_grad_gather, lut_grad.
  */
 int gft_timer_start(const char* kernels);
