@@ -15,12 +15,14 @@
 // exact gradient w.r.t. the density through the blend, the resample and the
 // mip pyramid.
 //
-// Launches (each one over every reconstruction, grid-stride over its items):
-//   k_batch_level(m)   level m of the zero-bordered pyramid (m = 0: the
-//                      density; m >= 1: kGenerateMip3D's pairwise means of
-//                      level m-1);
-//   k_batch_quad(m)    the corner-quad copy of level m;
-//   k_batch_fwd        items (reconstruction, view, 16x16-pixel tile): a
+// Launches (each one over the items of every reconstruction; the forward is
+// persistent over a device work queue, the others run a workgroup per item):
+//   k_batch_prep(m)    level m+1 of the zero-bordered pyramid
+//                      (kGenerateMip3D's pairwise means of level m, level 0
+//                      being the density) and the corner-quad copy of level
+//                      m, in one launch per m;
+//   k_batch_fwd        items (reconstruction, view, 16x16-pixel tile),
+//                      taken from the queue by resident workgroups: a
 //                      ray, clipped to the grid box, marches its samples
 //                      keeping each sample's c_k, writes its pixel, its loss
 //                      term, its gradient record and the view's largest LoD;
@@ -41,6 +43,7 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "gft_host.hpp"
@@ -52,8 +55,8 @@ namespace gft {
 constexpr int kBatchMaxViews = 8;
 constexpr int kBatchMaxLevels = 3;
 constexpr int kBatchLevels = kBatchMaxLevels + 1;
-// item tables: level passes, corner-quad passes, gather passes (one per
-// level each), the finishing pass
+// item tables: level passes (level 0 has none: it is the density),
+// corner-quad passes, gather passes (one per level each), the finishing pass
 constexpr int kQuadPass = kBatchLevels;
 constexpr int kGatherPass = 2 * kBatchLevels;
 constexpr int kFinishPass = 3 * kBatchLevels;
@@ -79,13 +82,16 @@ struct ColumnView {
   float pmv[16];
 };
 
-// Every pyramid level is stored with a one-cell zero border, (W+2)(H+2)(D+2)
-// cells, from which the next level is averaged and the level's corner-quad
-// copy (the forward's taps: two 16-byte loads per level) is taken.
+// Every pyramid level above 0 is stored with a one-cell zero border,
+// (W+2)(H+2)(D+2) cells, from which the next level is averaged and the
+// level's corner-quad copy (the forward's taps: two 16-byte loads per level)
+// is taken; level 0 is read from the density. The gradients of every level
+// use the same bordered cells (the gather writes every interior cell, the
+// border is never read).
 struct BatchRec {
-  MipTable mt;            // dims: levels 0..L without the border; offset[m]: level m in pyr and gpyr
+  MipTable mt;            // dims: levels 0..L without the border; offset[m]: level m in gpyr
   const float* dens;
-  float* pyr;             // bordered density pyramid, levels 0..L
+  float* pyr;             // bordered density pyramid, levels 1..L (level m at offset[m] - offset[1])
   float* gpyr;            // bordered gradients of the same cells (the gather writes the interior)
   float4* quad[kBatchLevels];   // corner-quad copy of each level
   QuadDims qd[kBatchLevels];
@@ -122,6 +128,39 @@ __device__ __forceinline__ int rec_of(const BatchRec* __restrict__ recs, int nre
   return lo;
 }
 
+// rec_of in a grid-stride loop: items only grow, so the owner of the next
+// item is found by stepping on from the previous one (r < 0: none yet)
+__device__ __forceinline__ int rec_from(const BatchRec* __restrict__ recs, int nrec, int g, int pass, int r) {
+  if (r < 0) return rec_of(recs, nrec, g, pass);
+  while (r + 1 < nrec && (pass < 0 ? recs[r + 1].item0 : recs[r + 1].pass_item0[pass]) <= g) ++r;
+  return r;
+}
+
+// The device work queue of a persistent launch (the forward's (reconstruction,
+// view, tile) items): each workgroup takes its next item from the pass's
+// counter (one atomic per item), so items of uneven cost (reconstructions of
+// different sizes and depths) spread over the workgroups as they finish, and
+// each workgroup sees its items in increasing order. The first prep launch of
+// a run zeroes the counters; stream order makes that visible to the later
+// launches of the run. The other passes are launched one workgroup per item:
+// their items are small or even, and the hardware's dispatch measured faster
+// than a queue's atomic and two barriers per item.
+enum BatchQueue { Q_FWD, kBatchQueues };
+
+__device__ __forceinline__ int next_item(int* __restrict__ ctr, int* slot) {
+  if (threadIdx.x == 0) *slot = atomicAdd(ctr, 1);
+  __syncthreads();
+  const int item = *slot;
+  __syncthreads();  // the slot is rewritten by the next call
+  return item;
+}
+
+// runs of 256 cells per item of the level and quad passes (kPrepRuns) and
+// of the finishing pass (kCellRuns): a thread's runs are independent, so their
+// loads are in flight together
+constexpr int kPrepRuns = 2;
+constexpr int kCellRuns = 4;
+
 // flat index of bordered cell (x, y, z) (border coordinates: 0 and dims + 1
 // are the border) of a level of dims d; the host keeps (H+2)(D+2) and W+2
 // below 2^24
@@ -129,76 +168,117 @@ __device__ __forceinline__ unsigned pad_index(int x, int y, int z, Dims3 d) {
   return __umul24(__umul24((unsigned)z, (unsigned)(d.y + 2)) + (unsigned)y, (unsigned)(d.x + 2)) + (unsigned)x;
 }
 
-// ------------------------------------------------------------ level passes
-// Over the bordered cells of level m (border cells get zero): m = 0 copies
-// the density (and clears the views' LoD maxima); m >= 1 takes pairwise means
-// of level m-1 in z, then y, then x (kGenerateMip3D's order). Both clear the
-// level's gradient.
-__global__ __launch_bounds__(256) void k_batch_level(const BatchRec* __restrict__ recs, int nrec, int m,
-                                                     int total_items) {
+// bordered cells of pyramid level m >= 1 (pyr starts at level 1)
+__device__ __forceinline__ float* level_cells(const BatchRec& R, int m) {
+  return R.pyr + (R.mt.offset[m] - R.mt.offset[1]);
+}
+
+// ------------------------------------------------------------- prep passes
+// Level m >= 1 of the zero-bordered pyramid (bordered cells, border cells get
+// zero): kGenerateMip3D's pairwise means of level m-1 in z, then y, then x.
+// Level 0 is the density itself: the means of level 1 read cells 2x+1 <=
+// 2 (W>>1) - 1 < W only, so they never reach the border, and the level-0
+// corner quads take their zeros from bounds checks.
+__device__ __forceinline__ void batch_level_item(const BatchRec& R, int m, int local) {
 #pragma clang fp contract(off)
-  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, m)];
-    if (m > R.mt.levels) continue;
-    const Dims3 o = R.mt.dims[m];
-    const int px = o.x + 2, py = o.y + 2;
-    const size_t n = (size_t)px * py * (o.z + 2);
-    const size_t f = (size_t)(item - R.pass_item0[m]) * 256 + threadIdx.x;
-    if (m == 0 && f < (size_t)R.num_views) R.lodmax[f] = 0;
-    if (f >= n) continue;
-    const int x = (int)(f % (size_t)px) - 1, y = (int)((f / (size_t)px) % (size_t)py) - 1,
-              z = (int)(f / ((size_t)px * py)) - 1;
-    float v = 0.f;
-    if (x >= 0 && x < o.x && y >= 0 && y < o.y && z >= 0 && z < o.z) {
-      if (m == 0) {
-        v = R.dens[flat3(x, y, z, o)];
-      } else {
-        const Dims3 s = R.mt.dims[m - 1];
-        const float* in = R.pyr + R.mt.offset[m - 1];
-        float ya[2];
+  const Dims3 o = R.mt.dims[m], s = R.mt.dims[m - 1];
+  // 32-bit cell indices: the host keeps every level's bordered cells below 2^31
+  const unsigned px = (unsigned)o.x + 2u, py = (unsigned)o.y + 2u;
+  const unsigned n = px * py * ((unsigned)o.z + 2u);
+  const float* __restrict__ in = m == 1 ? R.dens : level_cells(R, m - 1);
+  float* __restrict__ out = level_cells(R, m);
+  auto at = [&](int cx, int cy, int cz) -> float {
+    return m == 1 ? in[flat3(cx, cy, cz, s)] : in[pad_index(cx + 1, cy + 1, cz + 1, s)];
+  };
+  float v[kPrepRuns];
 #pragma unroll
-        for (int xi = 0; xi < 2; ++xi) {
-          float za[2];
+  for (int k = 0; k < kPrepRuns; ++k) {
+    const unsigned f = ((unsigned)local * kPrepRuns + k) * 256u + threadIdx.x, fy = f / px;
+    const int x = (int)(f - fy * px) - 1, y = (int)(fy % py) - 1, z = (int)(fy / py) - 1;
+    v[k] = 0.f;
+    if (f < n && x >= 0 && x < o.x && y >= 0 && y < o.y && z >= 0 && z < o.z) {
+      float ya[2];
 #pragma unroll
-          for (int yi = 0; yi < 2; ++yi) {
-            const float a = in[pad_index(2 * x + xi + 1, 2 * y + yi + 1, 2 * z + 1, s)];
-            const float b = in[pad_index(2 * x + xi + 1, 2 * y + yi + 1, 2 * z + 2, s)];
-            za[yi] = (a + b) * 0.5f;
-          }
-          ya[xi] = (za[0] + za[1]) * 0.5f;
+      for (int xi = 0; xi < 2; ++xi) {
+        float za[2];
+#pragma unroll
+        for (int yi = 0; yi < 2; ++yi) {
+          const float a = at(2 * x + xi, 2 * y + yi, 2 * z);
+          const float b = at(2 * x + xi, 2 * y + yi, 2 * z + 1);
+          za[yi] = (a + b) * 0.5f;
         }
-        v = (ya[0] + ya[1]) * 0.5f;
+        ya[xi] = (za[0] + za[1]) * 0.5f;
       }
+      v[k] = (ya[0] + ya[1]) * 0.5f;
     }
-    R.pyr[R.mt.offset[m] + f] = v;
-    R.gpyr[R.mt.offset[m] + f] = 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < kPrepRuns; ++k) {
+    const unsigned f = ((unsigned)local * kPrepRuns + k) * 256u + threadIdx.x;
+    if (f < n) out[f] = v[k];
   }
 }
 
-// Corner-quad copy of level m (the layout of the fused render's copy), taken
-// from the bordered level: entry (x, y, z) holds v(x..x+1, y..y+1, z), the
-// border supplying the zeros of corners outside the grid.
-__global__ __launch_bounds__(256) void k_batch_quad(const BatchRec* __restrict__ recs, int nrec, int m,
-                                                    int total_items) {
+// Corner-quad copy of level m (the layout of the fused render's copy): entry
+// (x, y, z) holds v(x..x+1, y..y+1, z) in bordered coordinates, zero outside
+// the grid. The first item of level 0 also clears the views' LoD maxima.
+__device__ __forceinline__ void batch_quad_item(const BatchRec& R, int m, int local) {
   using B = QuadBrick<float4>;
-  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, kQuadPass + m)];
-    if (m > R.mt.levels) continue;
-    const QuadDims q = R.qd[m];
-    const size_t j = (size_t)(item - R.pass_item0[kQuadPass + m]) * 256 + threadIdx.x;
-    if (j >= quad_entries(q)) continue;
+  if (m == 0 && local == 0 && threadIdx.x < (unsigned)R.num_views) R.lodmax[threadIdx.x] = 0;
+  const QuadDims q = R.qd[m];
+  const Dims3 d = R.mt.dims[m];
+  const unsigned n = (unsigned)quad_entries(q);  // below 2^31 (host check)
+  const float* __restrict__ lv = m == 0 ? R.dens : level_cells(R, m);
+  float4* __restrict__ out = R.quad[m];
+  float4 e[kPrepRuns];
+#pragma unroll
+  for (int k = 0; k < kPrepRuns; ++k) {
+    const unsigned j = ((unsigned)local * kPrepRuns + k) * 256u + threadIdx.x;
     const unsigned b = (unsigned)(j / B::N), o = (unsigned)(j % B::N);
     const unsigned ix = (b % q.nbx) * B::X + (o & (B::X - 1));
     const unsigned iy = ((b / q.nbx) % q.nby) * B::Y + ((o >> B::Xl) & (B::Y - 1));
     const unsigned iz = (b / (q.nbx * q.nby)) * B::Z + (o >> (B::Xl + B::Yl));
-    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);  // padding entries of partial bricks: never read
-    if (ix < q.qx && iy < q.qy && iz < q.qz) {
-      const Dims3 d = R.mt.dims[m];
-      const float* c = R.pyr + R.mt.offset[m] + pad_index((int)ix, (int)iy, (int)iz, d);
-      const unsigned sy = (unsigned)(d.x + 2);
-      e = make_float4(c[0], c[1], c[sy], c[sy + 1]);
+    e[k] = make_float4(0.f, 0.f, 0.f, 0.f);  // padding entries of partial bricks: never read
+    if (j < n && ix < q.qx && iy < q.qy && iz < q.qz) {
+      if (m == 0) {
+        const int x = (int)ix - 1, y = (int)iy - 1, z = (int)iz - 1;
+        auto v = [&](int xx, int yy) -> float {
+          return (((unsigned)xx < (unsigned)d.x) & ((unsigned)yy < (unsigned)d.y) & ((unsigned)z < (unsigned)d.z))
+                     ? lv[flat3(xx, yy, z, d)]
+                     : 0.f;
+        };
+        e[k] = make_float4(v(x, y), v(x + 1, y), v(x, y + 1), v(x + 1, y + 1));
+      } else {
+        const float* c = lv + pad_index((int)ix, (int)iy, (int)iz, d);
+        const unsigned sy = (unsigned)(d.x + 2);
+        e[k] = make_float4(c[0], c[1], c[sy], c[sy + 1]);
+      }
     }
-    R.quad[m][j] = e;
+  }
+#pragma unroll
+  for (int k = 0; k < kPrepRuns; ++k) {
+    const unsigned j = ((unsigned)local * kPrepRuns + k) * 256u + threadIdx.x;
+    if (j < n) out[j] = e[k];
+  }
+}
+
+// One launch per m: the items of level m+1 (they read level m), then those of
+// the corner quads of level m.
+__global__ __launch_bounds__(256) void k_batch_prep(const BatchRec* __restrict__ recs, int nrec, int m,
+                                                    int level_items, int total_items, int* __restrict__ queues) {
+  if (m == 0 && blockIdx.x == 0 && threadIdx.x < kBatchQueues) queues[threadIdx.x] = 0;
+  int rl = -1, rq = -1;
+  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
+    if (item < level_items) {
+      rl = rec_from(recs, nrec, item, m + 1, rl);
+      const BatchRec& R = recs[rl];
+      if (m + 1 <= R.mt.levels) batch_level_item(R, m + 1, item - R.pass_item0[m + 1]);
+    } else {
+      const int g = item - level_items;
+      rq = rec_from(recs, nrec, g, kQuadPass + m, rq);
+      const BatchRec& R = recs[rq];
+      if (m <= R.mt.levels) batch_quad_item(R, m, g - R.pass_item0[kQuadPass + m]);
+    }
   }
 }
 
@@ -305,10 +385,14 @@ __device__ __forceinline__ void quad_read_batch(const gcf32x4* __restrict__ quad
 // The view's largest LoD (the LoD is convex along a column, so a column's
 // largest is at one of its ends) lets the gather skip the levels no sample
 // of a view reads.
-__global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ recs, int nrec, int total_items) {
+__global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ recs, int nrec, int total_items,
+                                                   int* __restrict__ queues) {
   __shared__ float red[256];
-  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, -1)];
+  __shared__ int slot;
+  int r = -1;
+  for (int item = next_item(queues + Q_FWD, &slot); item < total_items; item = next_item(queues + Q_FWD, &slot)) {
+    r = rec_from(recs, nrec, item, -1, r);
+    const BatchRec& R = recs[r];
     const int local = item - R.item0;
     const int per_view = R.tiles_x * R.tiles_y;
     const int vi = local / per_view, t = local % per_view;
@@ -464,8 +548,10 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   lds_double* acc = (lds_double*)region[wave];
   lds_double* acc1 = acc + kBlockVox;  // PAIR: the level-1 cells
+  int r = -1;
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, kGatherPass + m)];
+    r = rec_from(recs, nrec, item, kGatherPass + m, r);
+    const BatchRec& R = recs[r];
     const Dims3 dm = R.mt.dims[m];
     const bool two = PAIR && R.mt.levels > 0;
     const Dims3 d1 = R.mt.dims[two ? 1 : 0];
@@ -620,8 +706,10 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
 // partials.
 __global__ __launch_bounds__(256) void k_batch_finish(const BatchRec* __restrict__ recs, int nrec, int total_items) {
   __shared__ float red[256];
+  int r = -1;
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    const BatchRec& R = recs[rec_of(recs, nrec, item, kFinishPass)];
+    r = rec_from(recs, nrec, item, kFinishPass, r);
+    const BatchRec& R = recs[r];
     const int local = item - R.pass_item0[kFinishPass];
     if (local == 0) {
       const int n = R.num_views * R.tiles_x * R.tiles_y;
@@ -637,18 +725,31 @@ __global__ __launch_bounds__(256) void k_batch_finish(const BatchRec* __restrict
       __syncthreads();
     }
     const Dims3 d = R.mt.dims[0];
-    const size_t f = (size_t)local * 256 + threadIdx.x;
-    if (f >= (size_t)d.x * d.y * d.z) continue;
-    const int x = (int)(f % (size_t)d.x), y = (int)((f / (size_t)d.x) % (size_t)d.y), z = (int)(f / ((size_t)d.x * d.y));
-    float g = 0.f, s = 1.f;
-    for (int m = 1; m <= R.mt.levels; ++m) {
-      s *= 0.125f;
-      const Dims3 o = R.mt.dims[m];
-      const int px = x >> m, py = y >> m, pz = z >> m;
-      if (px < o.x && py < o.y && pz < o.z)
-        g = fmaf(s, R.gpyr[R.mt.offset[m] + pad_index(px + 1, py + 1, pz + 1, o)], g);
+    // 32-bit voxel indices: the host keeps the cells below 2^31
+    const unsigned n = (unsigned)d.x * (unsigned)d.y * (unsigned)d.z;
+    const float* __restrict__ gp = R.gpyr;
+    float gv[kCellRuns];
+#pragma unroll
+    for (int k = 0; k < kCellRuns; ++k) {
+      const unsigned f = ((unsigned)local * kCellRuns + k) * 256u + threadIdx.x, fy = f / (unsigned)d.x;
+      gv[k] = 0.f;
+      if (f >= n) continue;
+      const int x = (int)(f - fy * (unsigned)d.x), y = (int)(fy % (unsigned)d.y), z = (int)(fy / (unsigned)d.y);
+      float g = 0.f, s = 1.f;
+      for (int m = 1; m <= R.mt.levels; ++m) {
+        s *= 0.125f;
+        const Dims3 o = R.mt.dims[m];
+        const int px = x >> m, py = y >> m, pz = z >> m;
+        if (px < o.x && py < o.y && pz < o.z) g = fmaf(s, gp[R.mt.offset[m] + pad_index(px + 1, py + 1, pz + 1, o)], g);
+      }
+      gv[k] = gp[pad_index(x + 1, y + 1, z + 1, d)] + g;
     }
-    R.grad[f] = R.gpyr[pad_index(x + 1, y + 1, z + 1, d)] + g;
+    float* __restrict__ out = R.grad;
+#pragma unroll
+    for (int k = 0; k < kCellRuns; ++k) {
+      const unsigned f = ((unsigned)local * kCellRuns + k) * 256u + threadIdx.x;
+      if (f < n) out[f] = gv[k];
+    }
   }
 }
 
@@ -688,12 +789,12 @@ static void brick_block(int brick, int& bw, int& bh, int& bd) {
 }
 
 // workspace layout: the record table, then per reconstruction its bordered
-// density pyramid, the gradients of the same cells, the corner-quad copy of
-// each level, the render partials, the views' LoD maxima, the pixel records
-// and the sample values c_k (items x out_d x 256 floats: the largest part,
-// 4 bytes per frustum sample)
+// density pyramid above level 0, the bordered gradients of every level, the
+// corner-quad copy of each level, the render partials, the views' LoD maxima,
+// the pixel records and the sample values c_k (items x out_d x 256 floats:
+// the largest part, 4 bytes per frustum sample)
 struct BatchLayout {
-  size_t table, total;
+  size_t table, queues, total;
   std::vector<size_t> pyr, gpyr, partials, lodmax, records, csave;
   std::vector<std::array<size_t, kBatchLevels>> quad;
 };
@@ -701,10 +802,11 @@ struct BatchLayout {
 // items (of 256 threads) of pass p for a reconstruction with pyramid mt and
 // gather block bw x bh x bd
 static int pass_items(const MipTable& mt, int bw, int bh, int bd, int p) {
-  if (p < kQuadPass) return p <= mt.levels ? (int)((padded_cells(mt.dims[p]) + 255) / 256) : 0;
+  constexpr size_t kPrep = 256 * kPrepRuns, kCells = 256 * kCellRuns;
+  if (p < kQuadPass) return p >= 1 && p <= mt.levels ? (int)((padded_cells(mt.dims[p]) + kPrep - 1) / kPrep) : 0;
   if (p < kGatherPass) {
     const int m = p - kQuadPass;
-    return m <= mt.levels ? (int)((quad_entries(make_quad_dims(mt.dims[m])) + 255) / 256) : 0;
+    return m <= mt.levels ? (int)((quad_entries(make_quad_dims(mt.dims[m])) + kPrep - 1) / kPrep) : 0;
   }
   if (p < kFinishPass) {
     const int m = p - kGatherPass;
@@ -713,7 +815,7 @@ static int pass_items(const MipTable& mt, int bw, int bh, int bd, int p) {
     const int64_t blocks = (int64_t)((d.x + bw - 1) / bw) * ((d.y + bh - 1) / bh) * ((d.z + bd - 1) / bd);
     return (int)((blocks + 3) / 4);  // four waves (blocks) per item
   }
-  return (int)((mt.base_stride + 255) / 256);
+  return (int)((mt.base_stride + kCells - 1) / kCells);
 }
 
 static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmaps, BatchLayout& L) {
@@ -722,6 +824,8 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
   if (num_mipmaps < 0 || num_mipmaps > kBatchMaxLevels)
     return fail(GFT_ERR_INVALID_ARGUMENT, "num_mipmaps must be in [0, %d]", kBatchMaxLevels);
   size_t off = align256b(sizeof(BatchRec) * (size_t)n);
+  L.queues = off;
+  off += align256b(kBatchQueues * sizeof(int));
   L.table = off;
   L.pyr.resize(n);
   L.gpyr.resize(n);
@@ -747,7 +851,7 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
                   "reconstruction %d: grid too large for the batched step ((H+2)(D+2) < 2^24, cells < 2^31)", r);
     const MipTable mt = batch_mip_table(d, num_mipmaps);
     L.pyr[r] = off;
-    off += align256b(mt.atlas_stride * sizeof(float));
+    if (mt.levels >= 1) off += align256b((mt.atlas_stride - mt.offset[1]) * sizeof(float));
     L.gpyr[r] = off;
     off += align256b(mt.atlas_stride * sizeof(float));
     for (int m = 0; m <= mt.levels; ++m) {
@@ -784,7 +888,36 @@ static void batch_items(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
   }
 }
 
-static unsigned persistent_grid(int items) { return (unsigned)std::max(1, std::min(items, 256 * 8)); }
+// Workgroups of the persistent forward: as many as the device holds at once
+// (the kernel's occupancy per CU times the CUs); a workgroup that starts late
+// finds fewer items left, so the count only needs to fill the chip. Queried
+// once per device (by prepare, outside any graph capture).
+struct ResidentCap {
+  int dev = -1, cap = 0;
+};
+static std::mutex g_caps_mu;
+static std::vector<ResidentCap> g_caps;
+
+static int forward_cap(int& cap) {
+  int dev = 0, rc;
+  if ((rc = check_hip(hipGetDevice(&dev), "current device"))) return rc;
+  std::lock_guard<std::mutex> lock(g_caps_mu);
+  for (const ResidentCap& c : g_caps)
+    if (c.dev == dev) {
+      cap = c.cap;
+      return GFT_OK;
+    }
+  int cus = 0, per_cu = 0;
+  if ((rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count")))
+    return rc;
+  if ((rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &per_cu, reinterpret_cast<const void*>(&k_batch_fwd), 256, 0),
+                      "occupancy query")))
+    return rc;
+  cap = std::max(1, per_cu) * std::max(1, cus);
+  g_caps.push_back(ResidentCap{dev, cap});
+  return GFT_OK;
+}
 
 }  // namespace gft
 
@@ -874,6 +1007,8 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
       mat_to_float(mat_mul(p, mv), cv.pmv);
     }
   }
+  int cap;  // the forward's launch size, queried here (never under capture)
+  if ((rc = forward_cap(cap))) return rc;
   // a set-up call: the table is copied before it returns
   if ((rc = check_hip(hipMemcpyAsync(ws, recs.data(), sizeof(BatchRec) * (size_t)n, hipMemcpyHostToDevice,
                                      (hipStream_t)stream),
@@ -892,39 +1027,39 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
                 L.total);
   hipStream_t s = (hipStream_t)stream;
   const BatchRec* recs = (const BatchRec*)workspace;
+  int* queues = (int*)((char*)workspace + L.queues);
+  int cap;
+  if ((rc = forward_cap(cap))) return rc;
   int render = 0, pass[kBatchPasses];
   batch_items(n, recons, num_mipmaps, render, pass);
   {
     KernelTimer t(s, "batch_levels");
-    for (int m = 0; m <= kBatchMaxLevels; ++m)
-      if (pass[m])
-        hipLaunchKernelGGL(k_batch_level, dim3(persistent_grid(pass[m])), dim3(256), 0, s, recs, n, m, pass[m]);
-    for (int m = 0; m <= kBatchMaxLevels; ++m)
-      if (pass[kQuadPass + m])
-        hipLaunchKernelGGL(k_batch_quad, dim3(persistent_grid(pass[kQuadPass + m])), dim3(256), 0, s, recs, n, m,
-                           pass[kQuadPass + m]);
+    for (int m = 0; m <= kBatchMaxLevels; ++m) {
+      const int nl = m < kBatchMaxLevels ? pass[m + 1] : 0, total = nl + pass[kQuadPass + m];
+      if (total)
+        hipLaunchKernelGGL(k_batch_prep, dim3(total), dim3(256), 0, s, recs, n, m, nl, total, queues);
+    }
   }
   if ((rc = check_hip(hipGetLastError(), "batched mip levels"))) return rc;
   {
     KernelTimer t(s, "batch_fwd");
-    hipLaunchKernelGGL(k_batch_fwd, dim3(persistent_grid(render)), dim3(256), 0, s, recs, n, render);
+    hipLaunchKernelGGL(k_batch_fwd, dim3(std::max(1, std::min(render, cap))), dim3(256), 0, s, recs, n, render,
+                       queues);
   }
   if ((rc = check_hip(hipGetLastError(), "batched render forward"))) return rc;
   {
     KernelTimer t(s, "batch_gather");
     if (pass[kGatherPass])
-      hipLaunchKernelGGL(k_batch_gather<true>, dim3(persistent_grid(pass[kGatherPass])), dim3(256), 0, s, recs, n, 0,
-                         pass[kGatherPass]);
+      hipLaunchKernelGGL(k_batch_gather<true>, dim3(pass[kGatherPass]), dim3(256), 0, s, recs, n, 0, pass[kGatherPass]);
     for (int m = 2; m <= kBatchMaxLevels; ++m)
       if (pass[kGatherPass + m])
-        hipLaunchKernelGGL(k_batch_gather<false>, dim3(persistent_grid(pass[kGatherPass + m])), dim3(256), 0, s, recs,
-                           n, m, pass[kGatherPass + m]);
+        hipLaunchKernelGGL(k_batch_gather<false>, dim3(pass[kGatherPass + m]), dim3(256), 0, s, recs, n, m,
+                           pass[kGatherPass + m]);
   }
   if ((rc = check_hip(hipGetLastError(), "batched render gather"))) return rc;
   {
     KernelTimer t(s, "batch_finish");
-    hipLaunchKernelGGL(k_batch_finish, dim3(persistent_grid(pass[kFinishPass])), dim3(256), 0, s, recs, n,
-                       pass[kFinishPass]);
+    hipLaunchKernelGGL(k_batch_finish, dim3(pass[kFinishPass]), dim3(256), 0, s, recs, n, pass[kFinishPass]);
   }
   return check_hip(hipGetLastError(), "batched finish");
 }
