@@ -477,17 +477,20 @@ int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params
  * exact gradient w.r.t. the density through blend, resample and mips. Replaces
  * the per-(batch, camera) launch loops of SampleGridKernelLauncher
  * (resample_grid.cu.cc:372-436) and their gradients, for all reconstructions
- * at once: per mip level a pass building the zero-bordered level and one
- * taking its corner-quad copy, one forward launch over every pixel of every
- * view of every reconstruction (rays clipped to the grid box; it keeps each
- * sample's c_k = exp(-D_k) + E_{k-1} and each pixel's gradient record in the
- * workspace), a voxel-block gather of every sample's gradient for levels 0
+ * at once: per mip level m one pass building the zero-bordered level m+1 and
+ * level m's corner-quad copy (level 0 is the density itself), one persistent
+ * forward launch whose resident workgroups take the (reconstruction, view,
+ * 16x16 tile) items from a device work queue (rays clipped to the grid box;
+ * it keeps each sample's c_k = exp(-D_k) + E_{k-1} and each pixel's gradient
+ * record in the workspace), a voxel-block gather of every sample's gradient for levels 0
  * and 1 together (and per coarser level a view's samples reach), and one
  * launch that applies the mip adjoint and sums the losses. `brick` picks the
  * gather's voxel block per reconstruction: 0 = 16x16x2, 1 = 8x8x8,
  * 2 = 16x8x4. The workspace holds 4 bytes per frustum sample besides the
- * pyramids. No atomics in global memory: images, losses and gradients
- * repeat bit for bit. Grids with (H+2)(D+2) >= 2^24 are rejected.
+ * pyramids. No atomics in global memory besides the queue counter (zeroed
+ * by the run's first launch): images, losses and gradients repeat bit for
+ * bit. Grids with (H+2)(D+2) >= 2^24 are rejected. Runs that overlap on one
+ * workspace race.
  * gft_batch_render_prepare writes the reconstruction table into the workspace
  * (a set-up call that waits for its copy); gft_batch_render_run only launches
  * kernels (capturable in a HIP graph) and reads the host array for launch sizes.
