@@ -153,6 +153,33 @@ def render_density_forward(d, cams, depth_step, light=(1.0, 1.0, 1.0), boundary=
     return out
 
 
+def _align256(n):
+    return (int(n) + 255) // 256 * 256
+
+
+def _rewritten_tail(saved, n_ambient, cams):
+    """The part of a saved buffer that a render backward rewrites in place: the
+    per-pixel records and the per-tile loss partials, which end the ambient
+    render's layout at byte ``n_ambient`` (the per-sample values before them
+    are only read). Bounded from above by 16 + 4 bytes per pixel, each block
+    256-byte aligned, so the slice may also cover the last per-sample values.
+    A view through ``.data``: restoring it must not count as an in-place change
+    of a tensor autograd saved."""
+    plane = int(cams.out_dhw[1]) * int(cams.out_dhw[2]) * int(cams.nCam)
+    tail = min(int(n_ambient), _align256(plane * 16) + _align256(plane * 4))
+    return saved.data[int(n_ambient) - tail:int(n_ambient)]
+
+
+def _keep_or_restore(ctx, tail):
+    """The first backward keeps a copy of what it will overwrite (a few bytes
+    per pixel, not the per-sample values); a later backward of a retained
+    graph puts it back first."""
+    if getattr(ctx, "pristine", None) is None:
+        ctx.pristine = tail.clone()
+    else:
+        tail.copy_(ctx.pristine)
+
+
 class _RenderFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, d, cams, depth_step, light, boundary):
@@ -172,9 +199,8 @@ class _RenderFn(torch.autograd.Function):
         L = _lib.lib()
         shape = _lib.i64_array(d.shape)
         bm = _enum(BOUNDARY, boundary, "boundary")
-        # the backward overwrites the saved buffer: work on a copy so that a
-        # retained graph can run backward again
-        saved = saved.clone()
+        if saved.numel() > 1:
+            _keep_or_restore(ctx, _rewritten_tail(saved, saved.numel(), cams))
         nws = int(L.gft_render_density_bwd_workspace(shape, cams.nCam, cams.c_out, bm))
         ws = torch.empty((max(nws, 1),), dtype=torch.uint8, device=d.device)
         _lib.check(L.gft_render_density_bwd(
@@ -233,7 +259,7 @@ class _LitRenderFn(torch.autograd.Function):
         cams, depth_step, color, ambient = ctx.cfg
         gd, gl = torch.empty_like(d), torch.empty_like(lam)
         lt = (ctypes.c_float * 3)(*[float(x) for x in color])
-        saved = saved.clone()  # the backward overwrites it; a retained graph may run backward again
+        _keep_or_restore(ctx, _rewritten_tail(saved, render_saved_bytes(d.shape, cams, "BORDER"), cams))
         _lib.check(_lib.lib().gft_render_density_lit_bwd(
             _lib.stream_of(d), _lib.ptr(d), _lib.ptr(lam), _lib.i64_array(d.shape), _np_ptr(cams.M), cams.nM,
             _np_ptr(cams.V), _np_ptr(cams.P), _np_ptr(cams.F), cams.nCam, cams.c_out, lt, float(ambient),

@@ -160,3 +160,31 @@ def test_lit_render_rejects_clamp_and_shape_mismatch(cuda):
                                                _np_ptr(cams.F), cams.nCam, cams.c_out, lc, 0.5, 1.0, 1, None, 0,
                                                None, 0, _lib.ptr(out))
     assert rc == _lib.GFT_ERR_UNIMPLEMENTED
+
+
+@pytest.mark.parametrize("lit", [False, True])
+def test_render_backward_twice_on_a_retained_graph(cuda, lit):
+    """The render backward rewrites the per-pixel records of its saved buffer
+    in place (the per-sample values are only read); the autograd function
+    keeps a copy of that part only, so a retained graph's second backward
+    restores it and gives the same gradients bit for bit, for the ambient
+    render (gft_render_density_bwd) and the lit one."""
+    from gft.pipeline import render_density, render_density_lit
+    sc = scene(grid=(14, 20, 16), image=(16, 12), views=3, seed=91, yaw_range=(-170.0, 170.0))
+    args = (sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"], sc["step"])
+    d = _t(sc["data"]).requires_grad_(True)
+    lt = _t(np.random.Generator(np.random.PCG64(91)).uniform(0.0, 1.5, size=sc["data"].shape)).requires_grad_(True)
+    img = render_density_lit(d, lt, *args, (1.0, 0.8, 0.6), 0.3) if lit else render_density(d, *args)
+    w = torch.linspace(-1, 1, img.numel(), device="cuda").reshape(img.shape)
+    loss = (img * w).sum()
+    loss.backward(retain_graph=True)
+    g1 = d.grad.clone()
+    l1 = lt.grad.clone() if lit else None
+    d.grad = None
+    lt.grad = None
+    loss.backward()
+    torch.cuda.synchronize()
+    assert torch.equal(g1, d.grad)
+    assert float(g1.abs().max()) > 0.0
+    if lit:
+        assert torch.equal(l1, lt.grad)
