@@ -75,7 +75,8 @@ typedef __attribute__((address_space(1))) const float gcfloat;
 // C the inverse model-view times the inverse projection (the model-view is
 // affine, so its product with the un-divided point commutes with the
 // perspective divide) and nx = (x + 0.5) 2 / W - 1: at the first and last
-// depth index H is n0 + x gx + y gy and f0 + x gx + y gy. pmv is the
+// depth index (indices 0 and 1 for a single sample) H is n0 + x gx + y gy
+// and f0 + x gx + y gy. pmv is the
 // object -> clip projection used to find a voxel block's pixels.
 struct ColumnView {
   float n0[4], f0[4], gx[4], gy[4];
@@ -304,6 +305,11 @@ __device__ __forceinline__ float3 column_point(const float* b, const ColumnView&
   return make_float3(h0 * iw, h1 * iw, h2 * iw);
 }
 
+// 1 / (index of the column's second defining sample): S - 1, or 1 for a
+// single sample (whose column is defined by indices 0 and 1, so that its step
+// is the difference to the next depth index, which the LoD uses)
+__device__ __forceinline__ float column_rcp(int S) { return S > 1 ? 1.f / (float)(S - 1) : 1.f; }
+
 __device__ __forceinline__ Column column_of(const ColumnView& v, int x, int y, float inv_s1, bool lod) {
   const float fx = (float)x, fy = (float)y;
   Column c;
@@ -404,7 +410,7 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
     float D = 0.f, E = 0.f, sq = 0.f, lmax = 0.f;
     float4 rec = make_float4(__int_as_float(1), __int_as_float(0), 0.f, 0.f);
     if (valid) {
-      const Column col = column_of(R.view[vi], x, y, S > 1 ? 1.f / (float)(S - 1) : 0.f, mips);
+      const Column col = column_of(R.view[vi], x, y, column_rcp(S), mips);
       // samples outside the box [0, dims) read zero at every mip level (a
       // level-m sample is inside only where 0 <= p < dims_m 2^m <= dims), so
       // they add nothing to the blend or the gradient: the column is clipped
@@ -570,7 +576,7 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
     const float b1[3] = {((float)(bx + bwx) + 0.5f) * s2, ((float)(by + bwy) + 0.5f) * s2,
                          ((float)(bz + bwz) + 0.5f) * s2};
     const int S = R.out_d;
-    const float inv_s1 = S > 1 ? 1.f / (float)(S - 1) : 0.f;
+    const float inv_s1 = column_rcp(S);
     const bool mips = R.mt.levels > 0;
     const float lmaxf = (float)R.mt.levels;
     const int per_view = R.tiles_x * R.tiles_y;
@@ -1000,7 +1006,7 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
       for (int r4 = 0; r4 < 4; ++r4) {
         const double c0 = C.m[0 * 4 + r4], c1 = C.m[1 * 4 + r4], c2 = C.m[2 * 4 + r4], c3 = C.m[3 * 4 + r4];
         cv.n0[r4] = (float)(c0 * ax + c1 * ay + c2 * nz_of(0.5) + c3);
-        cv.f0[r4] = (float)(c0 * ax + c1 * ay + c2 * nz_of(S - 0.5) + c3);
+        cv.f0[r4] = (float)(c0 * ax + c1 * ay + c2 * nz_of(S > 1 ? S - 0.5 : 1.5) + c3);
         cv.gx[r4] = (float)(c0 * 2.0 / W);
         cv.gy[r4] = (float)(c1 * 2.0 / Hh);
       }

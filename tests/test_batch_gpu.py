@@ -124,6 +124,38 @@ def test_batched_render_bricks_and_graph_replay(cuda):
             assert torch.equal(b.grads[k], results[2][1][k])
 
 
+def test_batched_render_edge_shapes(cuda):
+    """Shapes at the edges of the batched passes, against the FP64 oracle with
+    3 mip levels requested: a 1x1x1 grid (no level above 0), a grid whose
+    chain stops at level 1 (an axis of 2 cells) seen by the largest view count
+    (8) in one-pixel images, a single depth sample per ray (the column's LoD
+    then steps to the next depth index, as the oracle's forward difference
+    does), and an odd grid in wide images."""
+    from gft.batch import BatchedRenderStep
+    specs = [((1, 1, 1), 2, (5, 7), None, 0), ((5, 3, 2), 8, (1, 1), None, 1), ((7, 9, 6), 3, (4, 6), 1, 2),
+             ((9, 17, 11), 1, (13, 40), None, 0)]
+    recons, dens, tgts = [], [], []
+    rng = np.random.Generator(np.random.PCG64(17))
+    for k, (grid, views, image, S, brick) in enumerate(specs):
+        sc = scene(grid=grid, image=image, views=views, seed=90 + k, yaw_range=(-170.0, 170.0))
+        out = (S if S is not None else sc["out_dhw"][0],) + tuple(image)
+        recons.append(dict(grid=grid, M=sc["M"], V=sc["V"], P=sc["P"], F=sc["F"], out_dhw=out,
+                           depth_step=sc["step"], light=(1.0, 0.9, 0.7), brick=brick))
+        dens.append(sc["data"] * 3.0)
+        tgts.append(rng.uniform(0, 0.5, size=(views,) + tuple(image) + (3,)).astype(np.float32))
+    print("edge shapes", [r["grid"] for r in recons], [r["out_dhw"] for r in recons], flush=True)
+    b = BatchedRenderStep(recons, num_mipmaps=3)
+    b.bind([_t(d) for d in dens], [_t(t) for t in tgts])
+    b.run()
+    torch.cuda.synchronize()
+    for k, (r, d, t) in enumerate(zip(recons, dens, tgts)):
+        ref = orc.mip_render_fwd_bwd(d, t, r["M"], r["V"], r["P"], r["F"], r["out_dhw"], r["depth_step"],
+                                     r["light"], num_mipmaps=3)
+        assert norm_err(b.images[k].cpu().numpy(), ref["img"]) < 1e-5, k
+        _check_loss(b.losses[k].item(), b.images[k].cpu().numpy(), t, ref["img"], ref["loss"])
+        assert norm_err(b.grads[k].cpu().numpy(), ref["gd"]) < 1e-4, k
+
+
 def test_batched_render_validation(cuda):
     from gft import _lib
     from gft.batch import BatchedRenderStep
