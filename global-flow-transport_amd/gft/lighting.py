@@ -58,13 +58,23 @@ class PointLight(Light):
         light_os = (np.linalg.inv(np.asarray(density_transform.get_transform_matrix(), dtype=np.float64)) @ light_ws)[:3]
         lpos = light_os * cell
         dev = data.device
+        if color is None:
+            # the factor depends on nothing but these (not on the density): a
+            # step that renders the same grid again reuses it
+            key = (int(D), int(H), int(W), tuple(float(c) for c in cell), tuple(float(c) for c in lpos), float(self.s),
+                   str(dev))
+            cached = getattr(self, "_falloff_cache", None)
+            if cached is not None and cached[0] == key:
+                return cached[1]
         x = (torch.arange(W, dtype=torch.float64, device=dev) + 0.5) * cell[0] - lpos[0]
         y = (torch.arange(H, dtype=torch.float64, device=dev) + 0.5) * cell[1] - lpos[1]
         z = (torch.arange(D, dtype=torch.float64, device=dev) + 0.5) * cell[2] - lpos[2]
         dist = torch.sqrt(z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2)
         fall = 1.0 / (1.0 + (dist * self.s) ** 2)
         if color is None:
-            return fall[..., None].to(torch.float32)[None]
+            out = fall[..., None].to(torch.float32)[None]
+            self._falloff_cache = (key, out)
+            return out
         c = torch.tensor(color, dtype=torch.float64, device=dev)
         return (fall[..., None] * c).to(torch.float32)[None]
 
