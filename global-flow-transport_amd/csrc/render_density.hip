@@ -404,10 +404,10 @@ __device__ __forceinline__ void block_loss(float sq, float* __restrict__ loss, f
 // 256 strided sums, then a tree over them
 __device__ __forceinline__ void loss_sum_block(const float* partials, int n, float scale, float* loss, float* red) {
   float t = 0.f;
-  for (int i = threadIdx.x; i < n; i += 256) t += partials[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t += partials[i];
   red[threadIdx.x] = t;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
@@ -458,6 +458,10 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_pixel(const float* _
 // is amortised over a whole block. Only this wave touches its region, so the
 // sums run in a fixed order (deterministic) and one store per voxel follows.
 constexpr int kGatherBatch = 4;  // saved values loaded per round trip (2: 0.187 ms, 8: 0.211 ms at c2)
+// voxel blocks (one per wave) per workgroup: every c2 block is resident at
+// once, so 4 per workgroup costs no waiting there (c2: 0.175 ms; 1: 0.178);
+// c5 runs several rounds (1: 1.648 ms against 1.676)
+constexpr int kRGatherWaves = 4;
 //
 // LIT (the spot-lit render): a second region sums the emission gradients
 // alpha u_i * w of the d * lambda grid; the store then writes
@@ -479,14 +483,14 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
     loss_sum_block(sv.partials, sv.n_partials, loss_scale, loss, red);
   }
   constexpr int kVox = WX * WY * WZ;
-  __shared__ double region[LIT ? 8 : 4][kVox];
+  __shared__ double region[LIT ? 2 * kRGatherWaves : kRGatherWaves][kVox];
   typedef __attribute__((address_space(3))) double lds_double;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   lds_double* acc = (lds_double*)region[wave];
-  lds_double* acc2 = (lds_double*)region[LIT ? 4 + wave : wave];
+  lds_double* acc2 = (lds_double*)region[LIT ? kRGatherWaves + wave : wave];
   const Dims3 d = rd.in;
   const int nbx = (d.x + WX - 1) / WX, nby = (d.y + WY - 1) / WY, nbz = (d.z + WZ - 1) / WZ;
-  const int blk = blockIdx.x * 4 + wave;
+  const int blk = blockIdx.x * kRGatherWaves + wave;
   const bool live = blk < nbx * nby * nbz;  // a wave past the last block only joins the barriers
   const int x0 = (blk % nbx) * WX, y0 = ((blk / nbx) % nby) * WY, z0 = (blk / (nbx * nby)) * WZ;
   for (int k = lane; k < kVox; k += 64) acc[k] = 0.0;
@@ -993,8 +997,8 @@ hipError_t render_density_gather_launch(hipStream_t s, const RenderSaved& sv, fl
                  ((rd.in.z + kGatherWZ - 1) / kGatherWZ);
   KernelTimer t(s, "render_bwd_gather");
   auto kern = k_render_density_bwd_gather<kGatherWX, kGatherWY, kGatherWZ>;
-  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL, nullptr, nullptr, nullptr, loss,
-                     loss_scale);
+  hipLaunchKernelGGL(kern, dim3((nb + kRGatherWaves - 1) / kRGatherWaves), dim3(64 * kRGatherWaves), 0, s, sv, grad, rd,
+                     PL, nullptr, nullptr, nullptr, loss, loss_scale);
   return hipGetLastError();
 }
 
@@ -1006,8 +1010,8 @@ hipError_t render_density_lit_gather_launch(hipStream_t s, const RenderSaved& sv
                  ((rd.in.z + kGatherWZ - 1) / kGatherWZ);
   KernelTimer t(s, "render_lit_bwd_gather");
   auto kern = k_render_density_bwd_gather<kGatherWX, kGatherWY, kGatherWZ, true>;
-  hipLaunchKernelGGL(kern, dim3((nb + 3) / 4), dim3(256), 0, s, sv, grad, rd, PL, dens, lam, lam_grad, loss,
-                     loss_scale);
+  hipLaunchKernelGGL(kern, dim3((nb + kRGatherWaves - 1) / kRGatherWaves), dim3(64 * kRGatherWaves), 0, s, sv, grad, rd,
+                     PL, dens, lam, lam_grad, loss, loss_scale);
   return hipGetLastError();
 }
 
