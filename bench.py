@@ -659,7 +659,7 @@ def batched_bench(args):
         for s1 in singles:
             s1.run()
     ms_singles = timed(run_singles)
-    with KernelTimers("batch_levels,batch_fwd,batch_gather,batch_finish") as kt:
+    with KernelTimers("batch_levels,batch_fwd,batch_gather") as kt:
         for _ in range(args.steps):
             b.run()
         torch.cuda.synchronize()

@@ -32,12 +32,14 @@
 //                      sample's gradient alpha c_k + beta times its level-m
 //                      weight and its trilinear weights summed in FP64 LDS,
 //                      then one store per voxel (no atomics in global memory:
-//                      the gradient is the same bit for bit on every run);
-//   k_batch_finish     the density gradient: every voxel's level-0 gradient
-//                      plus 1/8^m of its level-m ancestor's (the mip
-//                      adjoint); in the first item of each reconstruction,
-//                      its loss: the per-tile partials summed in a fixed
-//                      order.
+//                      the gradient is the same bit for bit on every run).
+//                      Levels 3 and 2 first; then levels 0 and 1 in one walk,
+//                      whose store is the density gradient itself: every
+//                      voxel's level-0 gradient plus 1/8^m of its level-m
+//                      ancestor's (the mip adjoint; level 1 from the same
+//                      wave's region); the first block of each reconstruction
+//                      also sums its loss from the per-tile partials in a
+//                      fixed order.
 // The gather's block shape (16x16x2, 8x8x8 or 16x8x4 voxels) is chosen per
 // reconstruction.
 #include <algorithm>
@@ -56,11 +58,11 @@ constexpr int kBatchMaxViews = 8;
 constexpr int kBatchMaxLevels = 3;
 constexpr int kBatchLevels = kBatchMaxLevels + 1;
 // item tables: level passes (level 0 has none: it is the density),
-// corner-quad passes, gather passes (one per level each), the finishing pass
+// corner-quad passes, gather passes (one per level each; level 1 has none:
+// it is gathered with level 0)
 constexpr int kQuadPass = kBatchLevels;
 constexpr int kGatherPass = 2 * kBatchLevels;
-constexpr int kFinishPass = 3 * kBatchLevels;
-constexpr int kBatchPasses = kFinishPass + 1;
+constexpr int kBatchPasses = 3 * kBatchLevels;
 constexpr int kBlockVox = 512;  // voxels of a gather block (every shape)
 constexpr int kFwdBatch = 4;    // samples per forward round trip
 
@@ -90,10 +92,11 @@ struct ColumnView {
 // use the same bordered cells (the gather writes every interior cell, the
 // border is never read).
 struct BatchRec {
-  MipTable mt;            // dims: levels 0..L without the border; offset[m]: level m in gpyr
+  MipTable mt;            // dims: levels 0..L without the border
   const float* dens;
   float* pyr;             // bordered density pyramid, levels 1..L (level m at offset[m] - offset[1])
-  float* gpyr;            // bordered gradients of the same cells (the gather writes the interior)
+  float* gpyr;            // bordered gradients of levels 2..L (level m at offset[m] - offset[2]; the
+                          // gather writes the interior)
   float4* quad[kBatchLevels];   // corner-quad copy of each level
   QuadDims qd[kBatchLevels];
   float* grad;            // level-0 gradient output
@@ -156,11 +159,9 @@ __device__ __forceinline__ int next_item(int* __restrict__ ctr, int* slot) {
   return item;
 }
 
-// runs of 256 cells per item of the level and quad passes (kPrepRuns) and
-// of the finishing pass (kCellRuns): a thread's runs are independent, so their
-// loads are in flight together
+// runs of 256 cells per item of the level and quad passes: a thread's runs
+// are independent, so their loads are in flight together
 constexpr int kPrepRuns = 2;
-constexpr int kCellRuns = 4;
 
 // flat index of bordered cell (x, y, z) (border coordinates: 0 and dims + 1
 // are the border) of a level of dims d; the host keeps (H+2)(D+2) and W+2
@@ -172,6 +173,11 @@ __device__ __forceinline__ unsigned pad_index(int x, int y, int z, Dims3 d) {
 // bordered cells of pyramid level m >= 1 (pyr starts at level 1)
 __device__ __forceinline__ float* level_cells(const BatchRec& R, int m) {
   return R.pyr + (R.mt.offset[m] - R.mt.offset[1]);
+}
+
+// bordered gradient cells of level m >= 2 (gpyr starts at level 2)
+__device__ __forceinline__ float* grad_cells(const BatchRec& R, int m) {
+  return R.gpyr + (R.mt.offset[m] - R.mt.offset[2]);
 }
 
 // ------------------------------------------------------------- prep passes
@@ -692,75 +698,46 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
       }
     }
     __syncthreads();
-    if (live) {
-      float* __restrict__ gout = R.gpyr + R.mt.offset[m];
+    if (live && PAIR) {
+      // the density gradient: g_0 + sum_l 8^-l g_l(ancestor), level 1 from
+      // this wave's region (the level-1 cells over the block), levels 2 and 3
+      // from their gathers, which ran before; each level's value rounded to
+      // float and accumulated by fma from level 1 up
+      const int W1 = WX / 2, H1 = WY / 2;
+      for (int k = lane; k < kBlockVox; k += 64) {
+        const int lx = k % WX, ly = (k / WX) % WY, lz = k / (WX * WY);
+        const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+        if (x >= dm.x || y >= dm.y || z >= dm.z) continue;
+        float g = 0.f, s = 1.f;
+        for (int l = 1; l <= R.mt.levels; ++l) {
+          s *= 0.125f;
+          const Dims3 o = R.mt.dims[l];
+          const int px = x >> l, py = y >> l, pz = z >> l;
+          if (px < o.x && py < o.y && pz < o.z) {
+            const float gl = l == 1 ? (float)acc1[(lx >> 1) + W1 * ((ly >> 1) + H1 * (lz >> 1))]
+                                    : grad_cells(R, l)[pad_index(px + 1, py + 1, pz + 1, o)];
+            g = fmaf(s, gl, g);
+          }
+        }
+        R.grad[flat3(x, y, z, dm)] = (float)acc[k] + g;
+      }
+      if (blk == 0) {
+        // the reconstruction's loss: lane-strided sums of its per-tile
+        // partials, then a butterfly over the wave (a fixed order)
+        const int np = R.num_views * R.tiles_x * R.tiles_y;
+        float t = 0.f;
+        for (int i = lane; i < np; i += 64) t += R.partials[i];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane == 0) R.loss[0] = t * R.loss_scale;
+      }
+    } else if (live) {
+      float* __restrict__ gout = grad_cells(R, m);
       for (int k = lane; k < kBlockVox; k += 64) {
         const int x = x0 + k % WX, y = y0 + (k / WX) % WY, z = z0 + k / (WX * WY);
         if (x < dm.x && y < dm.y && z < dm.z) gout[pad_index(x + 1, y + 1, z + 1, dm)] = (float)acc[k];
       }
-      if (two) {
-        float* __restrict__ gout1 = R.gpyr + R.mt.offset[1];
-        const int W1 = WX / 2, H1 = WY / 2, Z1 = WZ / 2;
-        for (int k = lane; k < kBlockVox / 8; k += 64) {
-          const int x = x0 / 2 + k % W1, y = y0 / 2 + (k / W1) % H1, z = z0 / 2 + k / (W1 * H1);
-          if (x < d1.x && y < d1.y && z < d1.z) gout1[pad_index(x + 1, y + 1, z + 1, d1)] = (float)acc1[k];
-        }
-      }
     }
     __syncthreads();  // the regions are cleared by the next item
-  }
-}
-
-// ------------------------------------------------------------- finishing
-// The level-0 gradient with the mip adjoint (g0 + sum_m 8^-m g_m(ancestor))
-// and, in each reconstruction's first item, its loss from the render
-// partials.
-__global__ __launch_bounds__(256) void k_batch_finish(const BatchRec* __restrict__ recs, int nrec, int total_items) {
-  __shared__ float red[256];
-  int r = -1;
-  for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
-    r = rec_from(recs, nrec, item, kFinishPass, r);
-    const BatchRec& R = recs[r];
-    const int local = item - R.pass_item0[kFinishPass];
-    if (local == 0) {
-      const int n = R.num_views * R.tiles_x * R.tiles_y;
-      float t = 0.f;
-      for (int i = threadIdx.x; i < n; i += 256) t += R.partials[i];
-      red[threadIdx.x] = t;
-      __syncthreads();
-      for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) R.loss[0] = red[0] * R.loss_scale;
-      __syncthreads();
-    }
-    const Dims3 d = R.mt.dims[0];
-    // 32-bit voxel indices: the host keeps the cells below 2^31
-    const unsigned n = (unsigned)d.x * (unsigned)d.y * (unsigned)d.z;
-    const float* __restrict__ gp = R.gpyr;
-    float gv[kCellRuns];
-#pragma unroll
-    for (int k = 0; k < kCellRuns; ++k) {
-      const unsigned f = ((unsigned)local * kCellRuns + k) * 256u + threadIdx.x, fy = f / (unsigned)d.x;
-      gv[k] = 0.f;
-      if (f >= n) continue;
-      const int x = (int)(f - fy * (unsigned)d.x), y = (int)(fy % (unsigned)d.y), z = (int)(fy / (unsigned)d.y);
-      float g = 0.f, s = 1.f;
-      for (int m = 1; m <= R.mt.levels; ++m) {
-        s *= 0.125f;
-        const Dims3 o = R.mt.dims[m];
-        const int px = x >> m, py = y >> m, pz = z >> m;
-        if (px < o.x && py < o.y && pz < o.z) g = fmaf(s, gp[R.mt.offset[m] + pad_index(px + 1, py + 1, pz + 1, o)], g);
-      }
-      gv[k] = gp[pad_index(x + 1, y + 1, z + 1, d)] + g;
-    }
-    float* __restrict__ out = R.grad;
-#pragma unroll
-    for (int k = 0; k < kCellRuns; ++k) {
-      const unsigned f = ((unsigned)local * kCellRuns + k) * 256u + threadIdx.x;
-      if (f < n) out[f] = gv[k];
-    }
   }
 }
 
@@ -813,20 +790,17 @@ struct BatchLayout {
 // items (of 256 threads) of pass p for a reconstruction with pyramid mt and
 // gather block bw x bh x bd
 static int pass_items(const MipTable& mt, int bw, int bh, int bd, int p) {
-  constexpr size_t kPrep = 256 * kPrepRuns, kCells = 256 * kCellRuns;
+  constexpr size_t kPrep = 256 * kPrepRuns;
   if (p < kQuadPass) return p >= 1 && p <= mt.levels ? (int)((padded_cells(mt.dims[p]) + kPrep - 1) / kPrep) : 0;
   if (p < kGatherPass) {
     const int m = p - kQuadPass;
     return m <= mt.levels ? (int)((quad_entries(make_quad_dims(mt.dims[m])) + kPrep - 1) / kPrep) : 0;
   }
-  if (p < kFinishPass) {
-    const int m = p - kGatherPass;
-    if (m > mt.levels || m == 1) return 0;  // level 1 is gathered with level 0
-    const Dims3 d = mt.dims[m];
-    const int64_t blocks = (int64_t)((d.x + bw - 1) / bw) * ((d.y + bh - 1) / bh) * ((d.z + bd - 1) / bd);
-    return (int)((blocks + kGatherWaves - 1) / kGatherWaves);  // a block per wave
-  }
-  return (int)((mt.base_stride + kCells - 1) / kCells);
+  const int m = p - kGatherPass;
+  if (m > mt.levels || m == 1) return 0;  // level 1 is gathered with level 0
+  const Dims3 d = mt.dims[m];
+  const int64_t blocks = (int64_t)((d.x + bw - 1) / bw) * ((d.y + bh - 1) / bh) * ((d.z + bd - 1) / bd);
+  return (int)((blocks + kGatherWaves - 1) / kGatherWaves);  // a block per wave
 }
 
 static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmaps, BatchLayout& L) {
@@ -864,7 +838,7 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
     L.pyr[r] = off;
     if (mt.levels >= 1) off += align256b((mt.atlas_stride - mt.offset[1]) * sizeof(float));
     L.gpyr[r] = off;
-    off += align256b(mt.atlas_stride * sizeof(float));
+    if (mt.levels >= 2) off += align256b((mt.atlas_stride - mt.offset[2]) * sizeof(float));
     for (int m = 0; m <= mt.levels; ++m) {
       L.quad[r][m] = off;
       off += align256b(quad_entries(make_quad_dims(mt.dims[m])) * sizeof(float4));
@@ -1060,20 +1034,15 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
   if ((rc = check_hip(hipGetLastError(), "batched render forward"))) return rc;
   {
     KernelTimer t(s, "batch_gather");
-    if (pass[kGatherPass])
-      hipLaunchKernelGGL(k_batch_gather<true>, dim3(pass[kGatherPass]), dim3(64 * kGatherWaves), 0, s, recs, n, 0,
-                         pass[kGatherPass]);
-    for (int m = 2; m <= kBatchMaxLevels; ++m)
+    // the coarse levels first: the level-0/1 gather adds them into the density gradient
+    for (int m = kBatchMaxLevels; m >= 2; --m)
       if (pass[kGatherPass + m])
         hipLaunchKernelGGL(k_batch_gather<false>, dim3(pass[kGatherPass + m]), dim3(64 * kGatherWaves), 0, s, recs, n, m,
                            pass[kGatherPass + m]);
+    hipLaunchKernelGGL(k_batch_gather<true>, dim3(pass[kGatherPass]), dim3(64 * kGatherWaves), 0, s, recs, n, 0,
+                       pass[kGatherPass]);
   }
-  if ((rc = check_hip(hipGetLastError(), "batched render gather"))) return rc;
-  {
-    KernelTimer t(s, "batch_finish");
-    hipLaunchKernelGGL(k_batch_finish, dim3(pass[kFinishPass]), dim3(256), 0, s, recs, n, pass[kFinishPass]);
-  }
-  return check_hip(hipGetLastError(), "batched finish");
+  return check_hip(hipGetLastError(), "batched render gather");
 }
 
 }  // extern "C"

@@ -482,9 +482,10 @@ int gft_adam_step_device(void* stream, int32_t num_tensors, float* const* params
  * forward launch whose resident workgroups take the (reconstruction, view,
  * 16x16 tile) items from a device work queue (rays clipped to the grid box;
  * it keeps each sample's c_k = exp(-D_k) + E_{k-1} and each pixel's gradient
- * record in the workspace), a voxel-block gather of every sample's gradient for levels 0
- * and 1 together (and per coarser level a view's samples reach), and one
- * launch that applies the mip adjoint and sums the losses. `brick` picks the
+ * record in the workspace), voxel-block gathers of every sample's gradient:
+ * one per coarser level (3, then 2) for the views whose samples reach it,
+ * then levels 0 and 1 together, whose store applies the mip adjoint (the
+ * density gradient) and sums each reconstruction's loss. `brick` picks the
  * gather's voxel block per reconstruction: 0 = 16x16x2, 1 = 8x8x8,
  * 2 = 16x8x4. The workspace holds 4 bytes per frustum sample besides the
  * pyramids. No atomics in global memory besides the queue counter (zeroed
@@ -528,7 +529,7 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
  * the names seen (returns the full length); gft_timer_stop frees the events.
  * Names: advect_sl, advect_mc, advect_o1_out, advect_bwd_a, advect_bwd_b,
  * render_fwd, render_bwd_pixel (with the folded loss sum), render_bwd_gather, render_bwd_scatter, quad_copy, fill,
- * render_lit_fwd, render_lit_bwd_gather, batch_levels, batch_fwd, batch_gather, batch_finish,
+ * render_lit_fwd, render_lit_bwd_gather, batch_levels, batch_fwd, batch_gather,
  * shadow_fwd, shadow_bwd, divergence, divergence_grad, adam, transform_grad_gather, lut_grad.
  */
 int gft_timer_start(const char* kernels);
