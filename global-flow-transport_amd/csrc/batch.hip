@@ -88,9 +88,10 @@ struct ColumnView {
 // Every pyramid level above 0 is stored with a one-cell zero border,
 // (W+2)(H+2)(D+2) cells, from which the next level is averaged and the
 // level's corner-quad copy (the forward's taps: two 16-byte loads per level)
-// is taken; level 0 is read from the density. The gradients of every level
-// use the same bordered cells (the gather writes every interior cell, the
-// border is never read).
+// is taken; level 0 is read from the density. The gradients of levels 2 and
+// up use the same bordered cells (their gathers write every interior cell,
+// the border is never read); those of levels 0 and 1 stay in the level-0/1
+// gather's LDS regions.
 struct BatchRec {
   MipTable mt;            // dims: levels 0..L without the border
   const float* dens;
@@ -777,7 +778,7 @@ static void brick_block(int brick, int& bw, int& bh, int& bd) {
 }
 
 // workspace layout: the record table, then per reconstruction its bordered
-// density pyramid above level 0, the bordered gradients of every level, the
+// density pyramid above level 0, the bordered gradients of levels 2 and up, the
 // corner-quad copy of each level, the render partials, the views' LoD maxima,
 // the pixel records and the sample values c_k (items x out_d x 256 floats:
 // the largest part, 4 bytes per frustum sample)
