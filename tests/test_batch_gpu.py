@@ -1,9 +1,10 @@
 """The batched density-reconstruction step (gft_batch_render_*): many
 reconstructions of uneven grid sizes and view counts, each rendered with the
 non-fused mipmapped resample (TRANSFORM_LINDEPTH, LINEAR mips, per-sample
-LoD) and the BEER_LAMBERT blend, L2 loss and exact density gradient: level
-and corner-quad passes per mip level, one forward launch, one gather launch
-per level and one finishing launch.
+LoD) and the BEER_LAMBERT blend, L2 loss and exact density gradient: prep
+passes (pyramid levels and corner quads), one persistent forward launch and
+the gathers (coarse levels, then levels 0 and 1 with the mip adjoint and the
+losses).
 
 Against the FP64 oracle composition mip_render_fwd_bwd (the pinned resample
 with mips, the pinned blend and the resample's exact adjoint): images within
