@@ -65,6 +65,11 @@ constexpr int kGatherPass = 2 * kBatchLevels;
 constexpr int kBatchPasses = 3 * kBatchLevels;
 constexpr int kBlockVox = 512;  // voxels of a gather block (every shape)
 constexpr int kFwdBatch = 4;    // samples per forward round trip
+// the forward's item: a kTileW x kTileH-pixel tile, one thread per pixel.
+// Smaller tiles wait less at an item's barriers for its slowest ray but pay
+// the queue, record lookup and column set-up more often (c4 forward: 16x16
+// 2.55 ms, 16x8 2.53, 8x8 2.59, 16x4 2.81)
+constexpr int kTileW = 16, kTileH = 16, kTilePix = kTileW * kTileH;
 
 // four floats in global memory (loads through it are global, not flat,
 // 16-byte instructions)
@@ -104,15 +109,15 @@ struct BatchRec {
   const float* target;    // [V,H,W,3]
   float* images;          // [V,H,W,4]
   float* partials;        // per render item
-  float* records;         // per render item, 256 float4 pixel records (lo, hi, alpha, beta)
-  float* csave;           // per render item, out_d x 256 sample values c_k
+  float* records;         // per render item, kTilePix float4 pixel records (lo, hi, alpha, beta)
+  float* csave;           // per render item, out_d x kTilePix sample values c_k
   int* lodmax;            // per view, the largest LoD of its samples (float bits)
   float* loss;            // [1]
   float light[3];
   float step;
   float loss_scale;
   int num_views, out_w, out_h, out_d;
-  int tiles_x, tiles_y;   // 16x16-pixel tiles per view
+  int tiles_x, tiles_y;   // kTileW x kTileH-pixel tiles per view
   int bw, bh, bd;         // gather block (voxels)
   int item0;              // first render item
   int pass_item0[kBatchPasses];  // first item of the level, quad, gather and finishing passes
@@ -387,20 +392,20 @@ __device__ __forceinline__ void quad_read_batch(const gcf32x4* __restrict__ quad
 }
 
 // ------------------------------------------------------------------ forward
-// One thread per pixel of a 16x16 tile: the clipped column's BEER_LAMBERT
+// One thread per pixel of a kTileW x kTileH tile: the clipped column's BEER_LAMBERT
 // march, the image, the pixel's loss term and gradient, and for the backward
 // c_k = exp(-D_k) + E_{k-1} of every sample it marched (D_k the optical depth
 // through sample k, E_{k-1} the emission before it): with E the pixel's
 // total, dE/dsigma_k = exp(-D_k) - sum_{i>=k} exp(-D_i) sigma_i = c_k - E,
 // so the gradient of sample k is alpha c_k + beta with alpha = step G and
 // beta = -step G E, kept in the pixel's record (lo, hi, alpha, beta). c_k is
-// stored item-major, [item][k][256 pixels] (a wave's 64 pixels in one run).
+// stored item-major, [item][k][kTilePix pixels] (a wave's 64 pixels in one run).
 // The view's largest LoD (the LoD is convex along a column, so a column's
 // largest is at one of its ends) lets the gather skip the levels no sample
 // of a view reads.
-__global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ recs, int nrec, int total_items,
-                                                   int* __restrict__ queues) {
-  __shared__ float red[256];
+__global__ __launch_bounds__(kTilePix) void k_batch_fwd(const BatchRec* __restrict__ recs, int nrec,
+                                                        int total_items, int* __restrict__ queues) {
+  __shared__ float red[kTilePix];
   __shared__ int slot;
   int r = -1;
   for (int item = next_item(queues + Q_FWD, &slot); item < total_items; item = next_item(queues + Q_FWD, &slot)) {
@@ -409,7 +414,8 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
     const int local = item - R.item0;
     const int per_view = R.tiles_x * R.tiles_y;
     const int vi = local / per_view, t = local % per_view;
-    const int x = (t % R.tiles_x) * 16 + (threadIdx.x & 15), y = (t / R.tiles_x) * 16 + (threadIdx.x >> 4);
+    const int x = (t % R.tiles_x) * kTileW + (int)(threadIdx.x % kTileW);
+    const int y = (t / R.tiles_x) * kTileH + (int)(threadIdx.x / kTileW);
     const bool valid = x < R.out_w && y < R.out_h;
     const int S = R.out_d;
     const bool mips = R.mt.levels > 0;
@@ -423,7 +429,7 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
       // they add nothing to the blend or the gradient: the column is clipped
       int lo, hi;
       clip_ray(col.p0, col.dp, R.mt.dims[0], S, 0.f, lo, hi);
-      float* __restrict__ cs = R.csave + (size_t)local * S * 256 + threadIdx.x;
+      float* __restrict__ cs = R.csave + (size_t)local * S * kTilePix + threadIdx.x;
       for (int ib = lo; ib <= hi; ib += kFwdBatch) {
         // kFwdBatch samples at a time: per mip level any of them reads, all
         // their loads in flight
@@ -463,7 +469,7 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
           const float sigma = R.step * (nl[u] == 2 ? lerpf(vf[u], vc[u], lw[u][1]) : vf[u]);
           D += sigma;
           const float e = __expf(-D);
-          cs[(size_t)(ib + u) * 256] = e + E;
+          cs[(size_t)(ib + u) * kTilePix] = e + E;
           E = fmaf(e, sigma, E);
         }
       }
@@ -477,13 +483,13 @@ __global__ __launch_bounds__(256) void k_batch_fwd(const BatchRec* __restrict__ 
       if (G != 0.f && lo <= hi)
         rec = make_float4(__int_as_float(lo), __int_as_float(hi), R.step * G, -(R.step * G) * E);
     }
-    reinterpret_cast<float4*>(R.records)[(size_t)local * 256 + threadIdx.x] = rec;
+    reinterpret_cast<float4*>(R.records)[(size_t)local * kTilePix + threadIdx.x] = rec;
     for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off, 64));
     if ((threadIdx.x & 63) == 0 && lmax > 0.f) atomicMax(R.lodmax + vi, __float_as_int(lmax));
-    // per-item loss partial: a fixed-order tree over the tile's 256 pixels
+    // per-item loss partial: a fixed-order tree over the tile's pixels
     red[threadIdx.x] = sq;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = kTilePix / 2; w > 0; w >>= 1) {
       if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
       __syncthreads();
     }
@@ -625,14 +631,14 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
       const int dy64 = 64 / nxr, dx64 = 64 - dy64 * nxr;
       int px = xs + lane % nxr, py = ys + lane / nxr;
       auto rec_at = [&](int qx, int qy, int& it, int& slot) {
-        it = vi * per_view + (qy >> 4) * R.tiles_x + (qx >> 4);
-        slot = ((qy & 15) << 4) | (qx & 15);
+        it = vi * per_view + (qy / kTileH) * R.tiles_x + qx / kTileW;
+        slot = (qy % kTileH) * kTileW + qx % kTileW;
       };
       int it_n = 0, slot_n = 0;
       float4 rec_n = make_float4(__int_as_float(1), __int_as_float(0), 0.f, 0.f);
       if (lane < cnt) {
         rec_at(px, py, it_n, slot_n);
-        rec_n = recv[(size_t)it_n * 256 + slot_n];
+        rec_n = recv[(size_t)it_n * kTilePix + slot_n];
       }
       for (int k = lane; k < cnt; k += 64) {
         const float4 rec = rec_n;
@@ -645,7 +651,7 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
         }
         if (k + 64 < cnt) {
           rec_at(px, py, it_n, slot_n);
-          rec_n = recv[(size_t)it_n * 256 + slot_n];
+          rec_n = recv[(size_t)it_n * kTilePix + slot_n];
         }
         const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
         if (lo > hi) continue;
@@ -665,11 +671,11 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
         if (!(tlo <= thi + 1.f)) continue;  // (also a NaN slab: 0 * inf)
         // tlo, thi lie within [lo, hi] here, so the conversions stay in range
         const int i0 = max(lo, (int)ceilf(tlo - 1e-2f)), i1 = min(hi, (int)floorf(thi + 1e-2f));
-        const gcfloat* __restrict__ cs = (const gcfloat*)R.csave + ((size_t)it * S) * 256 + slot;
+        const gcfloat* __restrict__ cs = (const gcfloat*)R.csave + ((size_t)it * S) * kTilePix + slot;
         for (int ib = i0; ib <= i1; ib += kGatherBatch) {
           float cv4[kGatherBatch];
 #pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) cv4[u] = cs[(unsigned)min(ib + u, i1) * 256u];
+          for (int u = 0; u < kGatherBatch; ++u) cv4[u] = cs[(unsigned)min(ib + u, i1) * (unsigned)kTilePix];
 #pragma unroll
           for (int u = 0; u < kGatherBatch; ++u) {
             const int i = ib + u;
@@ -780,7 +786,7 @@ static void brick_block(int brick, int& bw, int& bh, int& bd) {
 // workspace layout: the record table, then per reconstruction its bordered
 // density pyramid above level 0, the bordered gradients of levels 2 and up, the
 // corner-quad copy of each level, the render partials, the views' LoD maxima,
-// the pixel records and the sample values c_k (items x out_d x 256 floats:
+// the pixel records and the sample values c_k (items x out_d x kTilePix floats:
 // the largest part, 4 bytes per frustum sample)
 struct BatchLayout {
   size_t table, queues, total;
@@ -788,7 +794,7 @@ struct BatchLayout {
   std::vector<std::array<size_t, kBatchLevels>> quad;
 };
 
-// items (of 256 threads) of pass p for a reconstruction with pyramid mt and
+// items of pass p (prep passes: 256 threads each) for a reconstruction with pyramid mt and
 // gather block bw x bh x bd
 static int pass_items(const MipTable& mt, int bw, int bh, int bd, int p) {
   constexpr size_t kPrep = 256 * kPrepRuns;
@@ -844,15 +850,16 @@ static int batch_layout(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
       L.quad[r][m] = off;
       off += align256b(quad_entries(make_quad_dims(mt.dims[m])) * sizeof(float4));
     }
-    const size_t items = (size_t)c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
+    const size_t items = (size_t)c.num_views * ((c.output_shape[2] + kTileW - 1) / kTileW) *
+                         ((c.output_shape[1] + kTileH - 1) / kTileH);
     L.partials[r] = off;
     off += align256b(items * sizeof(float));
     L.lodmax[r] = off;
     off += align256b(kBatchMaxViews * sizeof(int));
     L.records[r] = off;
-    off += align256b(items * 256 * 4 * sizeof(float));
+    off += align256b(items * kTilePix * 4 * sizeof(float));
     L.csave[r] = off;
-    off += align256b(items * (size_t)c.output_shape[0] * 256 * sizeof(float));
+    off += align256b(items * (size_t)c.output_shape[0] * kTilePix * sizeof(float));
   }
   L.total = off;
   return GFT_OK;
@@ -866,7 +873,7 @@ static void batch_items(int32_t n, const gft_batch_recon* rs, int32_t num_mipmap
   for (int p = 0; p < kBatchPasses; ++p) pass[p] = 0;
   for (int r = 0; r < n; ++r) {
     const gft_batch_recon& c = rs[r];
-    render += c.num_views * ((c.output_shape[2] + 15) / 16) * ((c.output_shape[1] + 15) / 16);
+    render += c.num_views * ((c.output_shape[2] + kTileW - 1) / kTileW) * ((c.output_shape[1] + kTileH - 1) / kTileH);
     const MipTable mt = batch_mip_table(Dims3{c.grid_dhw[2], c.grid_dhw[1], c.grid_dhw[0]}, num_mipmaps);
     int bw, bh, bd;
     brick_block(c.brick, bw, bh, bd);
@@ -897,7 +904,7 @@ static int forward_cap(int& cap) {
   if ((rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count")))
     return rc;
   if ((rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                          &per_cu, reinterpret_cast<const void*>(&k_batch_fwd), 256, 0),
+                          &per_cu, reinterpret_cast<const void*>(&k_batch_fwd), kTilePix, 0),
                       "occupancy query")))
     return rc;
   cap = std::max(1, per_cu) * std::max(1, cus);
@@ -954,8 +961,8 @@ int gft_batch_render_prepare(void* stream, int32_t n, const gft_batch_recon* rec
     R.out_d = c.output_shape[0];
     R.out_h = c.output_shape[1];
     R.out_w = c.output_shape[2];
-    R.tiles_x = (R.out_w + 15) / 16;
-    R.tiles_y = (R.out_h + 15) / 16;
+    R.tiles_x = (R.out_w + kTileW - 1) / kTileW;
+    R.tiles_y = (R.out_h + kTileH - 1) / kTileH;
     brick_block(c.brick, R.bw, R.bh, R.bd);
     R.item0 = item;
     item += R.num_views * R.tiles_x * R.tiles_y;
@@ -1029,7 +1036,7 @@ int gft_batch_render_run(void* stream, int32_t n, const gft_batch_recon* recons,
   if ((rc = check_hip(hipGetLastError(), "batched mip levels"))) return rc;
   {
     KernelTimer t(s, "batch_fwd");
-    hipLaunchKernelGGL(k_batch_fwd, dim3(std::max(1, std::min(render, cap))), dim3(256), 0, s, recs, n, render,
+    hipLaunchKernelGGL(k_batch_fwd, dim3(std::max(1, std::min(render, cap))), dim3(kTilePix), 0, s, recs, n, render,
                        queues);
   }
   if ((rc = check_hip(hipGetLastError(), "batched render forward"))) return rc;
