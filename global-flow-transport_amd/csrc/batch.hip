@@ -519,6 +519,12 @@ constexpr int kGatherBatch = 4;  // c_k loaded per round trip
 // wave per workgroup measured best (c4: 4.84 ms against 4.95 with 2 and 5.15
 // with 4)
 constexpr int kGatherWaves = 1;
+// copies of the level-1 region, lane l adding into copy l % kL1Copies: the
+// level-1 cells are 8 times the volume of level 0's, so neighbouring pixels'
+// samples often hit the same cell, and same-address LDS atomics of one
+// instruction serialize; the copies are summed in a fixed order at the store
+constexpr int kL1Copies = 4;
+constexpr int kL1Stride = kBlockVox / 8 + 1;  // odd: the copies start on different banks
 
 // add a sample's level tap to the block region acc (block W x H x Z with
 // WXY = W H, origin o, (qx, qy, qz) in the level's cell coordinates, g the
@@ -566,12 +572,13 @@ __device__ __forceinline__ bool inside_level(float qx, float qy, float qz, Dims3
 template <bool PAIR>
 __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict__ recs, int nrec, int m,
                                                       int total_items) {
-  constexpr int kVox = PAIR ? kBlockVox + kBlockVox / 8 : kBlockVox;
+  constexpr int kVox = PAIR ? kBlockVox + kL1Copies * kL1Stride : kBlockVox;
   __shared__ double region[kGatherWaves][kVox];
   typedef __attribute__((address_space(3))) double lds_double;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   lds_double* acc = (lds_double*)region[wave];
-  lds_double* acc1 = acc + kBlockVox;  // PAIR: the level-1 cells
+  lds_double* acc1s = acc + kBlockVox;                               // PAIR: the level-1 cells' copies
+  lds_double* acc1 = acc1s + (threadIdx.x % kL1Copies) * kL1Stride;  // this lane's copy
   int r = -1;
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
     r = rec_from(recs, nrec, item, kGatherPass + m, r);
@@ -721,8 +728,16 @@ __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict
           const Dims3 o = R.mt.dims[l];
           const int px = x >> l, py = y >> l, pz = z >> l;
           if (px < o.x && py < o.y && pz < o.z) {
-            const float gl = l == 1 ? (float)acc1[(lx >> 1) + W1 * ((ly >> 1) + H1 * (lz >> 1))]
-                                    : grad_cells(R, l)[pad_index(px + 1, py + 1, pz + 1, o)];
+            float gl;
+            if (l == 1) {
+              const int c1 = (lx >> 1) + W1 * ((ly >> 1) + H1 * (lz >> 1));
+              double t1 = acc1s[c1];
+#pragma unroll
+              for (int c = 1; c < kL1Copies; ++c) t1 += acc1s[c * kL1Stride + c1];
+              gl = (float)t1;
+            } else {
+              gl = grad_cells(R, l)[pad_index(px + 1, py + 1, pz + 1, o)];
+            }
             g = fmaf(s, gl, g);
           }
         }
