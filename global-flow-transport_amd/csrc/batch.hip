@@ -513,6 +513,7 @@ __global__ __launch_bounds__(kTilePix) void k_batch_fwd(const BatchRec* __restri
 // level weight to the block's corners (weight prod_a max(0, 1 - |q_a -
 // c_a|)); samples outside a level (BORDER) add nothing to it. The sums run in
 // a fixed order and one store per voxel follows.
+typedef __attribute__((address_space(3))) double lds_f64;  // an FP64 LDS region
 constexpr int kGatherBatch = 4;  // c_k loaded per round trip
 // voxel blocks (one per wave) per gather item: the waves of an item meet at
 // its barriers, so with several the fast blocks wait for the slow one; one
@@ -573,205 +574,216 @@ __device__ __forceinline__ bool inside_level(float qx, float qy, float qz, Dims3
          (0.f <= qz + 0.5f) & (qz + 0.5f < (float)d.z);
 }
 
+// One item of the gather: the block (blk = item's block) of reconstruction
+// R at level m (PAIR: levels 0 and 1), with the block shape WX x WY x WZ as
+// compile-time constants (index arithmetic and bounds on immediates).
+template <bool PAIR, int WX, int WY, int WZ>
+__device__ __forceinline__ void gather_item(const BatchRec& R, int m, int item, int wave, int lane, lds_f64* accs,
+                                            lds_f64* acc, lds_f64* acc1s, lds_f64* acc1) {
+  constexpr int kVox = kL0Copies * kL0Stride + (PAIR ? kL1Copies * kL1Stride : 0);
+  const Dims3 dm = R.mt.dims[m];
+  const bool two = PAIR && R.mt.levels > 0;
+  const Dims3 d1 = R.mt.dims[two ? 1 : 0];
+  const int nbx = (dm.x + WX - 1) / WX, nby = (dm.y + WY - 1) / WY, nbz = (dm.z + WZ - 1) / WZ;
+  const int blk = (item - R.pass_item0[kGatherPass + m]) * kGatherWaves + wave;
+  const bool live = m <= R.mt.levels && blk < nbx * nby * nbz;  // other waves only join the barriers
+  const int x0 = (blk % nbx) * WX, y0 = ((blk / nbx) % nby) * WY, z0 = (blk / (nbx * nby)) * WZ;
+  for (int k = lane; k < kVox; k += 64) accs[k] = 0.0;
+  // support box (object space) of the block's cells; PAIR: of its level-1 cells
+  const int mb = two ? 1 : m;
+  const float s2 = (float)(1 << mb), sc = 1.f / (float)(1 << m);
+  const int bx = two ? x0 / 2 : x0, by = two ? y0 / 2 : y0, bz = two ? z0 / 2 : z0;
+  const int bwx = two ? WX / 2 : WX, bwy = two ? WY / 2 : WY, bwz = two ? WZ / 2 : WZ;
+  const float b0[3] = {((float)bx - 0.5f) * s2, ((float)by - 0.5f) * s2, ((float)bz - 0.5f) * s2};
+  const float b1[3] = {((float)(bx + bwx) + 0.5f) * s2, ((float)(by + bwy) + 0.5f) * s2,
+                       ((float)(bz + bwz) + 0.5f) * s2};
+  const int S = R.out_d;
+  const float inv_s1 = column_rcp(S);
+  const bool mips = R.mt.levels > 0;
+  const float lmaxf = (float)R.mt.levels;
+  const int per_view = R.tiles_x * R.tiles_y;
+  const float4* __restrict__ recv = reinterpret_cast<const float4*>(R.records);
+  for (int vi = 0; vi < R.num_views && live; ++vi) {
+    if (!PAIR && __int_as_float(R.lodmax[vi]) <= (float)(m - 1)) continue;  // no sample reads level m
+    const ColumnView& cv = R.view[vi];
+    const float* A = cv.pmv;
+    float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
+    bool behind = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float ox = (k & 1) ? b1[0] : b0[0], oy = (k & 2) ? b1[1] : b0[1], oz = (k & 4) ? b1[2] : b0[2];
+      const float4 c = mat_vec(A, ox, oy, oz, 1.f);
+      behind |= !(c.w > 0.f);
+      const float iw = __builtin_amdgcn_rcpf(c.w);
+      const float px = fmaf(c.x * iw, 0.5f, 0.5f) * (float)R.out_w, py = fmaf(c.y * iw, 0.5f, 0.5f) * (float)R.out_h;
+      mnx = fminf(mnx, px);
+      mxx = fmaxf(mxx, px);
+      mny = fminf(mny, py);
+      mxy = fmaxf(mxy, py);
+    }
+    int xs = 0, xe = R.out_w - 1, ys = 0, ye = R.out_h - 1;
+    if (!behind) {
+      xs = max(xs, (int)ceilf(mnx - 0.5f - 1e-2f));
+      xe = min(xe, (int)floorf(mxx - 0.5f + 1e-2f));
+      ys = max(ys, (int)ceilf(mny - 0.5f - 1e-2f));
+      ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
+    }
+    if (xs > xe || ys > ye) continue;
+    const int nxr = xe - xs + 1, cnt = nxr * (ye - ys + 1);
+    // lane k of the box's row-major pixel list; the next record (k + 64) is
+    // loaded while the current one is walked
+    const int dy64 = 64 / nxr, dx64 = 64 - dy64 * nxr;
+    int px = xs + lane % nxr, py = ys + lane / nxr;
+    auto rec_at = [&](int qx, int qy, int& it, int& slot) {
+      it = vi * per_view + (qy / kTileH) * R.tiles_x + qx / kTileW;
+      slot = (qy % kTileH) * kTileW + qx % kTileW;
+    };
+    int it_n = 0, slot_n = 0;
+    float4 rec_n = make_float4(__int_as_float(1), __int_as_float(0), 0.f, 0.f);
+    if (lane < cnt) {
+      rec_at(px, py, it_n, slot_n);
+      rec_n = recv[(size_t)it_n * kTilePix + slot_n];
+    }
+    for (int k = lane; k < cnt; k += 64) {
+      const float4 rec = rec_n;
+      const int it = it_n, slot = slot_n, cpx = px, cpy = py;
+      px += dx64;
+      py += dy64;
+      if (px > xe) {
+        px -= nxr;
+        ++py;
+      }
+      if (k + 64 < cnt) {
+        rec_at(px, py, it_n, slot_n);
+        rec_n = recv[(size_t)it_n * kTilePix + slot_n];
+      }
+      const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
+      if (lo > hi) continue;
+      const Column col = column_of(cv, cpx, cpy, inv_s1, mips);
+      // the column's samples in the support box (one hundredth of a sample of
+      // slack); a zero step component (reciprocal inf) gives (-inf, inf) for
+      // a column inside that slab and an empty range outside it
+      const float p0[3] = {col.p0.x, col.p0.y, col.p0.z}, dp[3] = {col.dp.x, col.dp.y, col.dp.z};
+      float tlo = (float)lo, thi = (float)hi;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float ir = __builtin_amdgcn_rcpf(dp[a]);
+        const float t0 = (b0[a] - p0[a]) * ir, t1 = (b1[a] - p0[a]) * ir;
+        tlo = fmaxf(tlo, fminf(t0, t1));
+        thi = fminf(thi, fmaxf(t0, t1));
+      }
+      if (!(tlo <= thi + 1.f)) continue;  // (also a NaN slab: 0 * inf)
+      // tlo, thi lie within [lo, hi] here, so the conversions stay in range
+      const int i0 = max(lo, (int)ceilf(tlo - 1e-2f)), i1 = min(hi, (int)floorf(thi + 1e-2f));
+      const gcfloat* __restrict__ cs = (const gcfloat*)R.csave + ((size_t)it * S) * kTilePix + slot;
+      for (int ib = i0; ib <= i1; ib += kGatherBatch) {
+        float cv4[kGatherBatch];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) cv4[u] = cs[(unsigned)min(ib + u, i1) * (unsigned)kTilePix];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) {
+          const int i = ib + u;
+          if (i > i1) break;
+          const float fi = (float)i;
+          const float l = mips ? fminf(column_lod(col, fi), lmaxf) : 0.f;
+          const int fl = (int)l;
+          const float fr = l - (float)fl;
+          const float g = fmaf(rec.z, cv4[u], rec.w);
+          const float3 p = fma3(fi, col.dp, col.p0);
+          const float wm = mip_weight(m, fl, fr);
+          if (wm != 0.f) {
+            const float qx = p.x * sc - 0.5f, qy = p.y * sc - 0.5f, qz = p.z * sc - 0.5f;
+            if (inside_level(qx, qy, qz, dm)) gather_tap(acc, WX, WY, WZ, WX * WY, x0, y0, z0, qx, qy, qz, g * wm);
+          }
+          if (two) {
+            const float w1 = mip_weight(1, fl, fr);
+            if (w1 != 0.f) {
+              const float qx = p.x * 0.5f - 0.5f, qy = p.y * 0.5f - 0.5f, qz = p.z * 0.5f - 0.5f;
+              if (inside_level(qx, qy, qz, d1))
+                gather_tap(acc1, WX / 2, WY / 2, WZ / 2, (WX / 2) * (WY / 2), x0 / 2, y0 / 2, z0 / 2, qx, qy, qz,
+                           g * w1);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (live && PAIR) {
+    // the density gradient: g_0 + sum_l 8^-l g_l(ancestor), level 1 from
+    // this wave's region (the level-1 cells over the block), levels 2 and 3
+    // from their gathers, which ran before; each level's value rounded to
+    // float and accumulated by fma from level 1 up
+    const int W1 = WX / 2, H1 = WY / 2;
+    for (int k = lane; k < kBlockVox; k += 64) {
+      const int lx = k % WX, ly = (k / WX) % WY, lz = k / (WX * WY);
+      const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+      if (x >= dm.x || y >= dm.y || z >= dm.z) continue;
+      float g = 0.f, s = 1.f;
+      for (int l = 1; l <= R.mt.levels; ++l) {
+        s *= 0.125f;
+        const Dims3 o = R.mt.dims[l];
+        const int px = x >> l, py = y >> l, pz = z >> l;
+        if (px < o.x && py < o.y && pz < o.z) {
+          float gl;
+          if (l == 1) {
+            const int c1 = (lx >> 1) + W1 * ((ly >> 1) + H1 * (lz >> 1));
+            double t1 = acc1s[c1];
+#pragma unroll
+            for (int c = 1; c < kL1Copies; ++c) t1 += acc1s[c * kL1Stride + c1];
+            gl = (float)t1;
+          } else {
+            gl = grad_cells(R, l)[pad_index(px + 1, py + 1, pz + 1, o)];
+          }
+          g = fmaf(s, gl, g);
+        }
+      }
+      double t0 = accs[k];
+#pragma unroll
+      for (int c = 1; c < kL0Copies; ++c) t0 += accs[c * kL0Stride + k];
+      R.grad[flat3(x, y, z, dm)] = (float)t0 + g;
+    }
+    if (blk == 0) {
+      // the reconstruction's loss: lane-strided sums of its per-tile
+      // partials, then a butterfly over the wave (a fixed order)
+      const int np = R.num_views * R.tiles_x * R.tiles_y;
+      float t = 0.f;
+      for (int i = lane; i < np; i += 64) t += R.partials[i];
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+      if (lane == 0) R.loss[0] = t * R.loss_scale;
+    }
+  } else if (live) {
+    float* __restrict__ gout = grad_cells(R, m);
+    for (int k = lane; k < kBlockVox; k += 64) {
+      const int x = x0 + k % WX, y = y0 + (k / WX) % WY, z = z0 + k / (WX * WY);
+      if (x >= dm.x || y >= dm.y || z >= dm.z) continue;
+      double t0 = accs[k];
+#pragma unroll
+      for (int c = 1; c < kL0Copies; ++c) t0 += accs[c * kL0Stride + k];
+      gout[pad_index(x + 1, y + 1, z + 1, dm)] = (float)t0;
+    }
+  }
+  __syncthreads();  // the regions are cleared by the next item
+}
+
 template <bool PAIR>
 __global__ __launch_bounds__(256) void k_batch_gather(const BatchRec* __restrict__ recs, int nrec, int m,
                                                       int total_items) {
   constexpr int kVox = kL0Copies * kL0Stride + (PAIR ? kL1Copies * kL1Stride : 0);
   __shared__ double region[kGatherWaves][kVox];
-  typedef __attribute__((address_space(3))) double lds_double;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  lds_double* accs = (lds_double*)region[wave];                      // the block's cells' copies
-  lds_double* acc = accs + (threadIdx.x % kL0Copies) * kL0Stride;    // this lane's copy
-  lds_double* acc1s = accs + kL0Copies * kL0Stride;                  // PAIR: the level-1 cells' copies
-  lds_double* acc1 = acc1s + (threadIdx.x % kL1Copies) * kL1Stride;  // this lane's copy
+  lds_f64* accs = (lds_f64*)region[wave];                        // the block's cells' copies
+  lds_f64* acc = accs + (threadIdx.x % kL0Copies) * kL0Stride;    // this lane's copy
+  lds_f64* acc1s = accs + kL0Copies * kL0Stride;                  // PAIR: the level-1 cells' copies
+  lds_f64* acc1 = acc1s + (threadIdx.x % kL1Copies) * kL1Stride;  // this lane's copy
   int r = -1;
   for (int item = blockIdx.x; item < total_items; item += gridDim.x) {
     r = rec_from(recs, nrec, item, kGatherPass + m, r);
     const BatchRec& R = recs[r];
-    const Dims3 dm = R.mt.dims[m];
-    const bool two = PAIR && R.mt.levels > 0;
-    const Dims3 d1 = R.mt.dims[two ? 1 : 0];
-    const int WX = R.bw, WY = R.bh, WZ = R.bd;
-    const int nbx = (dm.x + WX - 1) / WX, nby = (dm.y + WY - 1) / WY, nbz = (dm.z + WZ - 1) / WZ;
-    const int blk = (item - R.pass_item0[kGatherPass + m]) * kGatherWaves + wave;
-    const bool live = m <= R.mt.levels && blk < nbx * nby * nbz;  // other waves only join the barriers
-    const int x0 = (blk % nbx) * WX, y0 = ((blk / nbx) % nby) * WY, z0 = (blk / (nbx * nby)) * WZ;
-    for (int k = lane; k < kVox; k += 64) accs[k] = 0.0;
-    // support box (object space) of the block's cells; PAIR: of its level-1 cells
-    const int mb = two ? 1 : m;
-    const float s2 = (float)(1 << mb), sc = 1.f / (float)(1 << m);
-    const int bx = two ? x0 / 2 : x0, by = two ? y0 / 2 : y0, bz = two ? z0 / 2 : z0;
-    const int bwx = two ? WX / 2 : WX, bwy = two ? WY / 2 : WY, bwz = two ? WZ / 2 : WZ;
-    const float b0[3] = {((float)bx - 0.5f) * s2, ((float)by - 0.5f) * s2, ((float)bz - 0.5f) * s2};
-    const float b1[3] = {((float)(bx + bwx) + 0.5f) * s2, ((float)(by + bwy) + 0.5f) * s2,
-                         ((float)(bz + bwz) + 0.5f) * s2};
-    const int S = R.out_d;
-    const float inv_s1 = column_rcp(S);
-    const bool mips = R.mt.levels > 0;
-    const float lmaxf = (float)R.mt.levels;
-    const int per_view = R.tiles_x * R.tiles_y;
-    const float4* __restrict__ recv = reinterpret_cast<const float4*>(R.records);
-    for (int vi = 0; vi < R.num_views && live; ++vi) {
-      if (!PAIR && __int_as_float(R.lodmax[vi]) <= (float)(m - 1)) continue;  // no sample reads level m
-      const ColumnView& cv = R.view[vi];
-      const float* A = cv.pmv;
-      float mnx = 3.4e38f, mxx = -3.4e38f, mny = 3.4e38f, mxy = -3.4e38f;
-      bool behind = false;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const float ox = (k & 1) ? b1[0] : b0[0], oy = (k & 2) ? b1[1] : b0[1], oz = (k & 4) ? b1[2] : b0[2];
-        const float4 c = mat_vec(A, ox, oy, oz, 1.f);
-        behind |= !(c.w > 0.f);
-        const float iw = __builtin_amdgcn_rcpf(c.w);
-        const float px = fmaf(c.x * iw, 0.5f, 0.5f) * (float)R.out_w, py = fmaf(c.y * iw, 0.5f, 0.5f) * (float)R.out_h;
-        mnx = fminf(mnx, px);
-        mxx = fmaxf(mxx, px);
-        mny = fminf(mny, py);
-        mxy = fmaxf(mxy, py);
-      }
-      int xs = 0, xe = R.out_w - 1, ys = 0, ye = R.out_h - 1;
-      if (!behind) {
-        xs = max(xs, (int)ceilf(mnx - 0.5f - 1e-2f));
-        xe = min(xe, (int)floorf(mxx - 0.5f + 1e-2f));
-        ys = max(ys, (int)ceilf(mny - 0.5f - 1e-2f));
-        ye = min(ye, (int)floorf(mxy - 0.5f + 1e-2f));
-      }
-      if (xs > xe || ys > ye) continue;
-      const int nxr = xe - xs + 1, cnt = nxr * (ye - ys + 1);
-      // lane k of the box's row-major pixel list; the next record (k + 64) is
-      // loaded while the current one is walked
-      const int dy64 = 64 / nxr, dx64 = 64 - dy64 * nxr;
-      int px = xs + lane % nxr, py = ys + lane / nxr;
-      auto rec_at = [&](int qx, int qy, int& it, int& slot) {
-        it = vi * per_view + (qy / kTileH) * R.tiles_x + qx / kTileW;
-        slot = (qy % kTileH) * kTileW + qx % kTileW;
-      };
-      int it_n = 0, slot_n = 0;
-      float4 rec_n = make_float4(__int_as_float(1), __int_as_float(0), 0.f, 0.f);
-      if (lane < cnt) {
-        rec_at(px, py, it_n, slot_n);
-        rec_n = recv[(size_t)it_n * kTilePix + slot_n];
-      }
-      for (int k = lane; k < cnt; k += 64) {
-        const float4 rec = rec_n;
-        const int it = it_n, slot = slot_n, cpx = px, cpy = py;
-        px += dx64;
-        py += dy64;
-        if (px > xe) {
-          px -= nxr;
-          ++py;
-        }
-        if (k + 64 < cnt) {
-          rec_at(px, py, it_n, slot_n);
-          rec_n = recv[(size_t)it_n * kTilePix + slot_n];
-        }
-        const int lo = __float_as_int(rec.x), hi = __float_as_int(rec.y);
-        if (lo > hi) continue;
-        const Column col = column_of(cv, cpx, cpy, inv_s1, mips);
-        // the column's samples in the support box (one hundredth of a sample of
-        // slack); a zero step component (reciprocal inf) gives (-inf, inf) for
-        // a column inside that slab and an empty range outside it
-        const float p0[3] = {col.p0.x, col.p0.y, col.p0.z}, dp[3] = {col.dp.x, col.dp.y, col.dp.z};
-        float tlo = (float)lo, thi = (float)hi;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const float ir = __builtin_amdgcn_rcpf(dp[a]);
-          const float t0 = (b0[a] - p0[a]) * ir, t1 = (b1[a] - p0[a]) * ir;
-          tlo = fmaxf(tlo, fminf(t0, t1));
-          thi = fminf(thi, fmaxf(t0, t1));
-        }
-        if (!(tlo <= thi + 1.f)) continue;  // (also a NaN slab: 0 * inf)
-        // tlo, thi lie within [lo, hi] here, so the conversions stay in range
-        const int i0 = max(lo, (int)ceilf(tlo - 1e-2f)), i1 = min(hi, (int)floorf(thi + 1e-2f));
-        const gcfloat* __restrict__ cs = (const gcfloat*)R.csave + ((size_t)it * S) * kTilePix + slot;
-        for (int ib = i0; ib <= i1; ib += kGatherBatch) {
-          float cv4[kGatherBatch];
-#pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) cv4[u] = cs[(unsigned)min(ib + u, i1) * (unsigned)kTilePix];
-#pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) {
-            const int i = ib + u;
-            if (i > i1) break;
-            const float fi = (float)i;
-            const float l = mips ? fminf(column_lod(col, fi), lmaxf) : 0.f;
-            const int fl = (int)l;
-            const float fr = l - (float)fl;
-            const float g = fmaf(rec.z, cv4[u], rec.w);
-            const float3 p = fma3(fi, col.dp, col.p0);
-            const float wm = mip_weight(m, fl, fr);
-            if (wm != 0.f) {
-              const float qx = p.x * sc - 0.5f, qy = p.y * sc - 0.5f, qz = p.z * sc - 0.5f;
-              if (inside_level(qx, qy, qz, dm)) gather_tap(acc, WX, WY, WZ, WX * WY, x0, y0, z0, qx, qy, qz, g * wm);
-            }
-            if (two) {
-              const float w1 = mip_weight(1, fl, fr);
-              if (w1 != 0.f) {
-                const float qx = p.x * 0.5f - 0.5f, qy = p.y * 0.5f - 0.5f, qz = p.z * 0.5f - 0.5f;
-                if (inside_level(qx, qy, qz, d1))
-                  gather_tap(acc1, WX / 2, WY / 2, WZ / 2, (WX / 2) * (WY / 2), x0 / 2, y0 / 2, z0 / 2, qx, qy, qz,
-                             g * w1);
-              }
-            }
-          }
-        }
-      }
-    }
-    __syncthreads();
-    if (live && PAIR) {
-      // the density gradient: g_0 + sum_l 8^-l g_l(ancestor), level 1 from
-      // this wave's region (the level-1 cells over the block), levels 2 and 3
-      // from their gathers, which ran before; each level's value rounded to
-      // float and accumulated by fma from level 1 up
-      const int W1 = WX / 2, H1 = WY / 2;
-      for (int k = lane; k < kBlockVox; k += 64) {
-        const int lx = k % WX, ly = (k / WX) % WY, lz = k / (WX * WY);
-        const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
-        if (x >= dm.x || y >= dm.y || z >= dm.z) continue;
-        float g = 0.f, s = 1.f;
-        for (int l = 1; l <= R.mt.levels; ++l) {
-          s *= 0.125f;
-          const Dims3 o = R.mt.dims[l];
-          const int px = x >> l, py = y >> l, pz = z >> l;
-          if (px < o.x && py < o.y && pz < o.z) {
-            float gl;
-            if (l == 1) {
-              const int c1 = (lx >> 1) + W1 * ((ly >> 1) + H1 * (lz >> 1));
-              double t1 = acc1s[c1];
-#pragma unroll
-              for (int c = 1; c < kL1Copies; ++c) t1 += acc1s[c * kL1Stride + c1];
-              gl = (float)t1;
-            } else {
-              gl = grad_cells(R, l)[pad_index(px + 1, py + 1, pz + 1, o)];
-            }
-            g = fmaf(s, gl, g);
-          }
-        }
-        double t0 = accs[k];
-#pragma unroll
-        for (int c = 1; c < kL0Copies; ++c) t0 += accs[c * kL0Stride + k];
-        R.grad[flat3(x, y, z, dm)] = (float)t0 + g;
-      }
-      if (blk == 0) {
-        // the reconstruction's loss: lane-strided sums of its per-tile
-        // partials, then a butterfly over the wave (a fixed order)
-        const int np = R.num_views * R.tiles_x * R.tiles_y;
-        float t = 0.f;
-        for (int i = lane; i < np; i += 64) t += R.partials[i];
-        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-        if (lane == 0) R.loss[0] = t * R.loss_scale;
-      }
-    } else if (live) {
-      float* __restrict__ gout = grad_cells(R, m);
-      for (int k = lane; k < kBlockVox; k += 64) {
-        const int x = x0 + k % WX, y = y0 + (k / WX) % WY, z = z0 + k / (WX * WY);
-        if (x >= dm.x || y >= dm.y || z >= dm.z) continue;
-        double t0 = accs[k];
-#pragma unroll
-        for (int c = 1; c < kL0Copies; ++c) t0 += accs[c * kL0Stride + k];
-        gout[pad_index(x + 1, y + 1, z + 1, dm)] = (float)t0;
-      }
-    }
-    __syncthreads();  // the regions are cleared by the next item
+    // the reconstruction's block shape (bricks 0, 1, 2: 16x16x2, 8x8x8, 16x8x4)
+    if (R.bd == 2) gather_item<PAIR, 16, 16, 2>(R, m, item, wave, lane, accs, acc, acc1s, acc1);
+    else if (R.bd == 8) gather_item<PAIR, 8, 8, 8>(R, m, item, wave, lane, accs, acc, acc1s, acc1);
+    else gather_item<PAIR, 16, 8, 4>(R, m, item, wave, lane, accs, acc, acc1s, acc1);
   }
 }
 
