@@ -171,7 +171,7 @@ __device__ __forceinline__ void read_border_batch2(const float4* __restrict__ qa
 
 // samples per forward round trip (every corner-quad load of the batch in
 // flight): 4 keeps the forward at 87 VGPRs, 5 waves per SIMD, on the bricked
-// copy (c2: 0.157 ms; 6: 0.164, 8 (147 VGPRs): 0.163)
+// copy (c2: 0.157 ms; 2: 0.181, 3: 0.185, 6: 0.164, 8 (147 VGPRs): 0.163)
 constexpr int kBatch = 4;
 // Per-pixel record (float4): the clipped sample range lo, hi (int bits in x, y;
 // lo > hi: no sample, or after the backward's pixel pass, no gradient) and
