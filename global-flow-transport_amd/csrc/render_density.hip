@@ -482,7 +482,10 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
     __shared__ float red[256];
     loss_sum_block(sv.partials, sv.n_partials, loss_scale, loss, red);
   }
-  constexpr int kVox = WX * WY * WZ;
+  // rows padded to WX + 1 doubles: lanes at one x in neighbouring rows fall on
+  // different LDS banks (16 doubles are 128 bytes, the banks' width)
+  constexpr int kRX = WX + 1;
+  constexpr int kVox = kRX * WY * WZ;
   __shared__ double region[LIT ? 2 * kRGatherWaves : kRGatherWaves][kVox];
   typedef __attribute__((address_space(3))) double lds_double;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -622,10 +625,10 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
                 for (int kx = 0; kx < 2; ++kx) {
                   const int cx = lx + kx;
                   if ((unsigned)cx >= (unsigned)WX) continue;
-                  __hip_atomic_fetch_add(acc + (cz * WY + cy) * WX + cx, (double)(wx[kx] * wyz), __ATOMIC_RELAXED,
+                  __hip_atomic_fetch_add(acc + (cz * WY + cy) * kRX + cx, (double)(wx[kx] * wyz), __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_WORKGROUP);
                   if (LIT)
-                    __hip_atomic_fetch_add(acc2 + (cz * WY + cy) * WX + cx, (double)(ex[kx] * wyz),
+                    __hip_atomic_fetch_add(acc2 + (cz * WY + cy) * kRX + cx, (double)(ex[kx] * wyz),
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
               }
@@ -641,8 +644,9 @@ __global__ __launch_bounds__(256) void k_render_density_bwd_gather(RenderSaved s
   }
   __syncthreads();
   if (!live) return;
-  for (int k = lane; k < kVox; k += 64) {
-    const int x = x0 + k % WX, y = y0 + (k / WX) % WY, z = z0 + k / (WX * WY);
+  for (int k0 = lane; k0 < WX * WY * WZ; k0 += 64) {
+    const int x = x0 + k0 % WX, y = y0 + (k0 / WX) % WY, z = z0 + k0 / (WX * WY);
+    const int k = (k0 / WX) * kRX + k0 % WX;
     if (x < d.x && y < d.y && z < d.z) {
       const size_t f = flat3(x, y, z, d);
       float* g = grad + f;
