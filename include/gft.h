@@ -179,7 +179,8 @@ int gft_render_density_fwd_bf16(void* stream, const float* density, const int64_
  * Replaces loss_dens_target (optimization.py:396-412) + the raymarch gradient
  * (raymarch_grid.cu.cc:343-372). BORDER with the `saved` buffer that the
  * forward filled from the same density and cameras reduces the gradient with
- * a voxel-block gather (it overwrites `saved`) and is deterministic. BORDER
+ * a voxel-block gather (it rewrites the per-pixel records and loss partials
+ * that follow the per-sample values in `saved`) and is deterministic. BORDER
  * with saved = NULL (the forward then needs none either), CLAMP and WRAP walk
  * every ray from far to near, recomputing the transmittance, and scatter with
  * an LDS pre-reduction and float atomics (summation order varies between
