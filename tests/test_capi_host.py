@@ -334,3 +334,27 @@ def test_batch_workspace_and_validation(lib):
     assert lib.gft_batch_render_workspace(0, p, 3) == 0
     arr_h, keep_h = _batch_records([((5000, 4000, 8), (16, 16), 1)])  # (H+2)(D+2) >= 2^24
     assert lib.gft_batch_render_workspace(1, ctypes.cast(arr_h, ctypes.c_void_p), 3) == 0
+
+
+def test_render_saved_tail_covers_the_rewritten_records(lib):
+    """The render backward rewrites the per-pixel records (16 B a pixel) and
+    the per-tile loss partials at the end of the ambient saved buffer; the
+    slice the autograd functions keep for a retained graph's second backward
+    must reach back over both, whatever the shape (the per-sample values
+    before them are 4 B per sample and stay untouched)."""
+    import torch
+    from gft import pipeline as pl
+    from helpers import scene
+    for grid, image, views in [((12, 20, 14), (24, 14), 3), ((8, 9, 10), (7, 33), 1), ((30, 36, 40), (96, 80), 18)]:
+        sc = scene(grid=grid, image=image, views=views, seed=3)
+        cams = pl._Cameras(sc["M"], sc["V"], sc["P"], sc["F"], sc["out_dhw"])
+        n = pl.render_saved_bytes((1,) + tuple(grid) + (1,), cams, "BORDER")
+        h, w = image
+        plane = h * w * views
+        tiles = ((w + 15) // 16) * ((h + 15) // 16) * views
+        a256 = lambda b: (b + 255) // 256 * 256
+        rewritten = a256(plane * 16) + a256(tiles * 4)
+        samples = n - rewritten
+        assert samples >= sc["out_dhw"][0] * plane * 4  # the per-sample values come first
+        tail = pl._rewritten_tail(torch.zeros(n, dtype=torch.uint8), n, cams)
+        assert tail.numel() >= rewritten and tail.numel() <= n
